@@ -1,0 +1,409 @@
+"""Python driver of the CPU restatement (liboracle_f32.so / liboracle_f64.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker. It runs the correlator-node data
+path for one slice the way sfxc/src/correlator_node.cc:263-296 drives it:
+Bit2float_worker -> Delay_correction per stream, then Correlation_core (or
+Correlation_core_pulsar), normalisation and integration_write.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+CLOCK_RATE = 512_000_000
+
+
+class B2fParams(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "bits_per_sample", "sample_rate", "base_sample_rate", "tsys_freq", "fft_size",
+        "nfft_max", "n_ffts_per_integration", "delay_in_samples", "lut", "pad")]
+
+
+class DelayModelC(C.Structure):
+    _fields_ = [("scan_begin_ticks", C.c_int64), ("interval", C.c_double), ("n", C.c_int32),
+                ("pad", C.c_int32), ("delay", C.POINTER(C.c_double)),
+                ("phase", C.POINTER(C.c_double)), ("amp", C.POINTER(C.c_double))]
+
+
+class DcParams(C.Structure):
+    _fields_ = [("fft_size_delaycor", C.c_int32), ("fft_size_correlation", C.c_int32),
+                ("sample_rate", C.c_int32), ("station_sample_rate", C.c_int32),
+                ("bandwidth", C.c_int32), ("sideband", C.c_int32), ("job_sideband", C.c_int32),
+                ("window", C.c_int32), ("channel_freq", C.c_int64), ("job_channel_freq", C.c_int64),
+                ("LO_offset", C.c_double), ("start_ticks", C.c_int64),
+                ("experiment_start_ticks", C.c_int64), ("n_ffts_per_integration", C.c_int32),
+                ("stride", C.c_int32)]
+
+
+class CoreParams(C.Structure):
+    _fields_ = [("n_streams", C.c_int32), ("fft_size", C.c_int32), ("stride", C.c_int32),
+                ("n_baselines", C.c_int32), ("number_ffts_in_integration", C.c_int32),
+                ("number_ffts_in_sub_integration", C.c_int32),
+                ("bl_first", C.POINTER(C.c_int32)), ("bl_second", C.POINTER(C.c_int32))]
+
+
+class PulsarParamsC(C.Structure):
+    _fields_ = [("nbins", C.c_int32), ("pad", C.c_int32), ("start_phase", C.c_double),
+                ("phase_coef", C.c_double), ("fft_duration", C.c_double),
+                ("gate_begin", C.c_double), ("gate_end", C.c_double),
+                ("offsets", C.POINTER(C.c_double))]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_libs: Dict[str, C.CDLL] = {}
+
+
+def lib(precision: str = "f32") -> C.CDLL:
+    if precision not in _libs:
+        path = os.path.join(BUILD, f"liboracle_{precision}.so")
+        if not os.path.exists(path):
+            build()
+        _libs[precision] = C.CDLL(path)
+    return _libs[precision]
+
+
+def _dtype(precision):
+    return np.float32 if precision == "f32" else np.float64
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _rptr(a, precision):
+    return _ptr(a, C.c_float if precision == "f32" else C.c_double)
+
+
+def sb_sign(c: str) -> int:
+    return -1 if c == "L" else 1
+
+
+# ----------------------------------------------------------------------------
+def bit2float(hdr: bytes, p: dict, precision="f32"):
+    """Returns dict(samples, chunks, invalid_start, invalid_n, n_invalid, counts_on, counts_off)."""
+    L = lib(precision)
+    fn = getattr(L, f"or_bit2float_{precision}")
+    prm = B2fParams(**p)
+    n_out = p["n_ffts_per_integration"] * p["fft_size"]
+    out = np.zeros(n_out, dtype=_dtype(precision))
+    chunks = np.zeros(n_out // max(1, p["fft_size"]) + 8, dtype=np.int32)
+    max_inv = 1 << 16
+    inv_s = np.zeros(max_inv, dtype=np.int32)
+    inv_n = np.zeros(max_inv, dtype=np.int32)
+    on = np.zeros(256, dtype=np.int32)
+    off = np.zeros(256, dtype=np.int32)
+    res = np.zeros(4, dtype=np.int32)
+    buf = np.frombuffer(hdr, dtype=np.uint8)
+    rc = fn(C.byref(prm), _ptr(buf, C.c_uint8), C.c_int64(len(buf)), _rptr(out, precision),
+            _ptr(chunks, C.c_int32), C.c_int32(len(chunks)), _ptr(inv_s, C.c_int32),
+            _ptr(inv_n, C.c_int32), C.c_int32(max_inv), _ptr(on, C.c_int32), _ptr(off, C.c_int32),
+            _ptr(res, C.c_int32))
+    if rc != 0:
+        raise RuntimeError(f"or_bit2float failed: {rc}")
+    return dict(samples=out, chunks=chunks[:res[0]].copy(), invalid_start=inv_s[:res[1]].copy(),
+                invalid_n=inv_n[:res[1]].copy(), n_invalid=int(res[2]), bytes_read=int(res[3]),
+                counts_on=on, counts_off=off)
+
+
+def bit_statistics(bits, rate, base_rate, on, off, n_invalid, precision="f32"):
+    L = lib(precision)
+    levels = np.zeros(5, dtype=np.int32)
+    tsys = np.zeros(4, dtype=np.int32)
+    getattr(L, f"or_bit_statistics_{precision}")(
+        C.c_int(bits), C.c_int64(rate), C.c_int64(base_rate), _ptr(on, C.c_int32),
+        _ptr(off, C.c_int32), C.c_int32(n_invalid), _ptr(levels, C.c_int32), _ptr(tsys, C.c_int32))
+    return levels, tsys
+
+
+def fft_c2c(x: np.ndarray, sign: int, precision="f32") -> np.ndarray:
+    d = _dtype(precision)
+    a = np.empty(2 * len(x), dtype=d)
+    a[0::2] = x.real
+    a[1::2] = x.imag
+    getattr(lib(precision), f"or_fft_c2c_{precision}")(_rptr(a, precision), C.c_int(len(x)), C.c_int(sign))
+    return a[0::2] + 1j * a[1::2]
+
+
+def rfft(x: np.ndarray, precision="f32") -> np.ndarray:
+    d = _dtype(precision)
+    a = np.ascontiguousarray(x, dtype=d)
+    out = np.empty(2 * (len(x) // 2 + 1), dtype=d)
+    getattr(lib(precision), f"or_rfft_{precision}")(_rptr(a, precision), _rptr(out, precision),
+                                                    C.c_int(len(x)))
+    return out[0::2] + 1j * out[1::2]
+
+
+def window(kind: int, n: int, precision="f32") -> np.ndarray:
+    w = np.empty(n, dtype=_dtype(precision))
+    getattr(lib(precision), f"or_window_{precision}")(C.c_int(kind), C.c_int(n), _rptr(w, precision))
+    return w
+
+
+def _model_c(model, keep):
+    def arr(a):
+        if a is None:
+            return C.POINTER(C.c_double)()
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        keep.append(a)
+        return _ptr(a, C.c_double)
+    return DelayModelC(model.scan_begin_ticks, model.interval, model.delay.shape[0], 0,
+                       arr(model.delay), arr(model.phase), arr(model.amp))
+
+
+def eval_delay(model, ticks: int, precision="f64") -> float:
+    keep = []
+    mc = _model_c(model, keep)
+    f = getattr(lib(precision), f"or_eval_delay_{precision}")
+    f.restype = C.c_double
+    return f(C.byref(mc), C.c_int64(ticks))
+
+
+def b2f_params(job, i, lut) -> dict:
+    st = job.station_streams[i]
+    delay0 = job.delay_models[i].eval(job.start_time)
+    return dict(bits_per_sample=st.bits_per_sample, sample_rate=st.sample_rate,
+                base_sample_rate=job.sample_rate, tsys_freq=st.tsys_freq,
+                fft_size=job.fft_size_delaycor, nfft_max=job.n_ffts_per_buffer(i),
+                n_ffts_per_integration=job.n_ffts_per_integration(i),
+                delay_in_samples=int(float(delay0) * st.sample_rate), lut=lut, pad=0)
+
+
+def delay_correction(job, i, samples, chunks, precision="f32", want_time_series=False):
+    st = job.station_streams[i]
+    stride = job.fft_size_correlation + 4
+    prm = DcParams(job.fft_size_delaycor, job.fft_size_correlation, job.sample_rate, st.sample_rate,
+                   st.bandwidth, sb_sign(st.sideband), sb_sign(job.sideband), job.window,
+                   st.channel_freq, job.channel_freq, st.LO_offset, job.start_time,
+                   job.experiment_start, job.n_ffts_per_integration(i), stride)
+    keep = []
+    mc = _model_c(job.delay_models[i], keep)
+    nseg_max = int(len(samples) // job.fft_size_correlation) + 2
+    spec = np.zeros(2 * nseg_max * stride, dtype=_dtype(precision))
+    ts = np.zeros(len(samples), dtype=_dtype(precision)) if want_time_series else None
+    spc = np.zeros(len(chunks), dtype=np.int32)
+    s = np.ascontiguousarray(samples, dtype=_dtype(precision))
+    ch = np.ascontiguousarray(chunks, dtype=np.int32)
+    f = getattr(lib(precision), f"or_delay_correction_{precision}")
+    f.restype = C.c_int64
+    nseg = f(C.byref(prm), C.byref(mc), _rptr(s, precision), _ptr(ch, C.c_int32), C.c_int32(len(ch)),
+             _rptr(spec, precision), _rptr(ts, precision) if ts is not None else None,
+             _ptr(spc, C.c_int32))
+    spec = spec[:2 * nseg * stride].reshape(nseg, stride, 2)
+    return dict(spectra=spec, seg_per_chunk=spc, time_series=ts, nseg=int(nseg))
+
+
+def _core_params(job, keep):
+    bl = job.baselines()
+    f = np.array([b[0] for b in bl], dtype=np.int32)
+    s = np.array([b[1] for b in bl], dtype=np.int32)
+    keep += [f, s]
+    return CoreParams(len(job.station_streams), job.fft_size_correlation, job.fft_size_correlation + 4,
+                      len(bl), job.number_ffts_in_integration(), job.number_ffts_in_sub_integration(),
+                      _ptr(f, C.c_int32), _ptr(s, C.c_int32)), bl
+
+
+def correlate(job, spectra: List[np.ndarray], seg_per_chunk, precision="f32"):
+    keep = []
+    cp, bl = _core_params(job, keep)
+    nf = job.fft_size_correlation + 1
+    arrs = [np.ascontiguousarray(s, dtype=_dtype(precision)) for s in spectra]
+    ptrs = (C.POINTER(C.c_float if precision == "f32" else C.c_double) * len(arrs))(
+        *[_rptr(a, precision) for a in arrs])
+    vis = np.zeros(2 * len(bl) * nf, dtype=_dtype(precision))
+    spc = np.ascontiguousarray(seg_per_chunk, dtype=np.int32)
+    getattr(lib(precision), f"or_correlate_{precision}")(C.byref(cp), ptrs, _ptr(spc, C.c_int32),
+                                                         C.c_int32(len(spc)), _rptr(vis, precision))
+    return vis.reshape(len(bl), nf, 2)
+
+
+def find_invalid(job, invalid: List[tuple]):
+    """n_flagged per baseline (correlation_core.cc:599-665)."""
+    bl = job.baselines()
+    M = len(job.station_streams)
+    out = np.zeros((len(bl), 2), dtype=np.int64)
+    f = getattr(lib("f32"), "or_find_invalid_f32")
+    for b in range(M, len(bl)):
+        i, j = bl[b]
+        s0, n0 = invalid[i]
+        s1, n1 = invalid[j]
+        s0 = np.ascontiguousarray(s0, dtype=np.int32)
+        n0 = np.ascontiguousarray(n0, dtype=np.int32)
+        s1 = np.ascontiguousarray(s1, dtype=np.int32)
+        n1 = np.ascontiguousarray(n1, dtype=np.int32)
+        acc = np.zeros(2, dtype=np.int64)
+        sc0 = job.station_streams[i].sample_rate / job.sample_rate
+        sc1 = job.station_streams[j].sample_rate / job.sample_rate
+        f(_ptr(s0, C.c_int32), _ptr(n0, C.c_int32), C.c_int32(len(s0)), _ptr(s1, C.c_int32),
+          _ptr(n1, C.c_int32), C.c_int32(len(s1)), C.c_double(sc0), C.c_double(sc1),
+          _ptr(acc, C.c_int64))
+        out[b] = acc
+    return out
+
+
+def normalize(job, vis, levels4, n_flagged, precision="f32"):
+    keep = []
+    cp, bl = _core_params(job, keep)
+    nf = job.fft_size_correlation + 1
+    v = np.ascontiguousarray(vis, dtype=_dtype(precision)).ravel()
+    out = np.zeros(2 * len(bl) * nf, dtype=np.float32)
+    l4 = np.ascontiguousarray(levels4, dtype=np.int32)
+    nf1 = np.ascontiguousarray(n_flagged[:, 0], dtype=np.int64)
+    nf2 = np.ascontiguousarray(n_flagged[:, 1], dtype=np.int64)
+    getattr(lib(precision), f"or_normalize_{precision}")(
+        C.byref(cp), _rptr(v, precision), _ptr(l4, C.c_int32), _ptr(nf1, C.c_int64),
+        _ptr(nf2, C.c_int64), _ptr(out, C.c_float))
+    return out.reshape(len(bl), nf, 2)
+
+
+def pulsar_setup(job, psr, precision="f64"):
+    nf = job.fft_size_correlation + 1
+    offsets = np.zeros(nf, dtype=np.float64)
+    out3 = np.zeros(3, dtype=np.float64)
+    coef = np.ascontiguousarray(psr.coef, dtype=np.float64)
+    from_ticks = job.start_time
+    tpd = 86400 * CLOCK_RATE
+    start_mjd = 51544 + from_ticks // tpd + (from_ticks % tpd) / CLOCK_RATE / 86400
+    getattr(lib(precision), f"or_pulsar_setup_{precision}")(
+        C.c_double(start_mjd), C.c_int(job.window), C.c_int(job.fft_size_correlation),
+        C.c_int(job.sample_rate), C.c_int(sb_sign(job.sideband)), C.c_int64(job.channel_freq),
+        C.c_double(psr.tmid), C.c_double(psr.DM), C.c_double(psr.ref_phase), C.c_double(psr.ref_freq),
+        C.c_double(psr.obs_freq), _ptr(coef, C.c_double), C.c_int(len(coef)), _ptr(offsets, C.c_double),
+        _ptr(out3, C.c_double))
+    return offsets, out3
+
+
+def pulsar_params_c(psr, offsets, out3, keep):
+    keep.append(offsets)
+    return PulsarParamsC(psr.nbins + 1, 0, out3[0], out3[1], out3[2], psr.gate_start, psr.gate_stop,
+                         _ptr(offsets, C.c_double))
+
+
+def pulsar_bins(job, psr, current_fft, precision="f64"):
+    offsets, out3 = pulsar_setup(job, psr, precision)
+    keep = []
+    pp = pulsar_params_c(psr, offsets, out3, keep)
+    nf = job.fft_size_correlation + 1
+    bins = np.zeros(nf, dtype=np.int32)
+    getattr(lib(precision), f"or_pulsar_bins_{precision}")(C.byref(pp), C.c_int(nf),
+                                                          C.c_int64(current_fft), _ptr(bins, C.c_int32))
+    return bins
+
+
+def pulsar_correlate(job, psr, spectra, nseg, precision="f32"):
+    keep = []
+    cp, bl = _core_params(job, keep)
+    offsets, out3 = pulsar_setup(job, psr, "f64")
+    pp = pulsar_params_c(psr, offsets, out3, keep)
+    nf = job.fft_size_correlation + 1
+    arrs = [np.ascontiguousarray(s, dtype=_dtype(precision)) for s in spectra]
+    ptrs = (C.POINTER(C.c_float if precision == "f32" else C.c_double) * len(arrs))(
+        *[_rptr(a, precision) for a in arrs])
+    acc = np.zeros(2 * (psr.nbins + 1) * len(bl) * nf, dtype=_dtype(precision))
+    getattr(lib(precision), f"or_pulsar_correlate_{precision}")(
+        C.byref(cp), C.byref(pp), ptrs, C.c_int64(nseg), _rptr(acc, precision))
+    return acc.reshape(psr.nbins + 1, len(bl), nf, 2)
+
+
+def stats_header_levels(bits, levels):
+    """Output_header_bitstatistics levels as integration_write stores them
+    (correlation_core.cc:378-390)."""
+    if bits == 2:
+        return [levels[0], levels[1], levels[2], levels[3], levels[4]]
+    return [0, levels[0], levels[1], 0, levels[4]]
+
+
+def write_output(job, sourcenr, integration_slice, vis_f32, stats5, weights, uvw=None):
+    M = len(job.station_streams)
+    bl = job.baselines()
+    station = np.array([s.station_number for s in job.station_streams], dtype=np.int32)
+    pol = np.array([0 if s.polarisation == "R" else 1 for s in job.station_streams], dtype=np.int32)
+    if uvw is None:
+        uvw = np.zeros(3 * M, dtype=np.float64)
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    st5 = np.ascontiguousarray(np.asarray(stats5, dtype=np.int32).ravel())
+    f = np.array([b[0] for b in bl], dtype=np.int32)
+    s = np.array([b[1] for b in bl], dtype=np.int32)
+    w = np.ascontiguousarray(weights, dtype=np.int32)
+    v = np.ascontiguousarray(vis_f32, dtype=np.float32).ravel()
+    nchan = job.number_channels
+    size = 4 + 16 + 32 * M + 24 * M + len(bl) * (8 + 8 * (nchan + 1))
+    dst = np.zeros(size, dtype=np.uint8)
+    fn = getattr(lib("f32"), "or_write_output_f32")
+    fn.restype = C.c_int64
+    n = fn(C.c_int32(sourcenr), C.c_int32(integration_slice), C.c_int(M), _ptr(station, C.c_int32),
+           _ptr(pol, C.c_int32), C.c_int(0 if job.sideband == "L" else 1), C.c_int(job.frequency_nr),
+           _ptr(uvw, C.c_double), _ptr(st5, C.c_int32), C.c_int(len(bl)), _ptr(f, C.c_int32),
+           _ptr(s, C.c_int32), _ptr(w, C.c_int32), C.c_int(nchan), _ptr(v, C.c_float),
+           _ptr(dst, C.c_uint8))
+    return dst[:n].tobytes()
+
+
+# ----------------------------------------------------------------------------
+def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32",
+              keep_intermediate: bool = False, pulsar=None, uvw=None):
+    """Full correlator-node path for one slice; returns dict with the output
+    bytes and intermediates. One integration per slice (the reference asserts
+    (stop - start) / integration_time == 1, correlator_node.cc:327-328)."""
+    M = len(job.station_streams)
+    b2f = []
+    dc = []
+    for i in range(M):
+        r = bit2float(headers[i], b2f_params(job, i, lut), precision)
+        b2f.append(r)
+        d = delay_correction(job, i, r["samples"], r["chunks"], precision, keep_intermediate)
+        dc.append(d)
+        if not keep_intermediate:
+            r["samples"] = None
+    stats = []
+    for i in range(M):
+        st = job.station_streams[i]
+        lv, ts = bit_statistics(st.bits_per_sample, st.sample_rate, job.sample_rate,
+                                b2f[i]["counts_on"], b2f[i]["counts_off"], b2f[i]["n_invalid"])
+        stats.append((lv, ts))
+    invalid = [(b["invalid_start"], b["invalid_n"]) for b in b2f]
+    nflag = find_invalid(job, invalid)
+    levels4 = np.array([s[0][4] for s in stats], dtype=np.int32)
+    spectra = [d["spectra"] for d in dc]
+    bl = job.baselines()
+    total_samples = job.number_ffts_in_integration() * job.fft_size_correlation
+    weights = []
+    for b, (i, j) in enumerate(bl):
+        lv = stats[i][0]
+        if i == j:
+            weights.append(max(total_samples - int(lv[4]), 0))
+        else:
+            weights.append(max(total_samples - int(lv[4]) - int(nflag[b, 0]), 0))
+    stats5 = [stats_header_levels(job.station_streams[i].bits_per_sample, stats[i][0]) for i in range(M)]
+    outputs = []
+    if pulsar is None:
+        vis = correlate(job, spectra, dc[0]["seg_per_chunk"], precision)
+        norm = normalize(job, vis, levels4, nflag, precision)
+        outputs.append(write_output(job, 0, job.integration_nr, norm, stats5, weights, uvw))
+        vis_bins = [vis]
+        norm_bins = [norm]
+    else:
+        acc = pulsar_correlate(job, pulsar, spectra, dc[0]["nseg"], precision)
+        vis_bins, norm_bins = [], []
+        for b in range(pulsar.nbins + 1):
+            norm = normalize(job, acc[b], levels4, nflag, precision)
+            outputs.append(write_output(job, b, job.integration_nr, norm, stats5, weights, uvw))
+            vis_bins.append(acc[b])
+            norm_bins.append(norm)
+    res = dict(output=b"".join(outputs), stats=stats, nflag=nflag, weights=weights,
+               vis=vis_bins, norm=norm_bins, seg_per_chunk=dc[0]["seg_per_chunk"],
+               chunks=b2f[0]["chunks"], invalid=invalid)
+    if keep_intermediate:
+        res["b2f"] = b2f
+        res["dc"] = dc
+    return res

@@ -1,0 +1,125 @@
+"""Builders for the seeded synthetic correlation jobs.
+
+Each builder returns a list of slices; a slice is (CorrelationParameters,
+[HEADER byte stream per station stream]) for one subband and one integration,
+the unit the manager node hands to one correlator node
+(sfxc/src/manager_node.cc:288-391).
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import synth
+from .job import (CLOCK_RATE, WINDOW_HANN, CorrelationParameters, DelayModel, StationStream,
+                  mjd_to_ticks)
+
+START_MJD = 60000
+
+
+def build_slices(*, seed: int, n_stations: int, pols: Sequence[str], fs: int, bw: int, nchan: int,
+                 seconds: float, t_int_usec: int, rho: float, bits: int, sky_freq: int,
+                 offsets: Sequence[float], rates: Sequence[float], subband: int = 0,
+                 window: int = WINDOW_HANN, sub_int_usec: Optional[int] = None,
+                 extra: Optional[Tuple[float, Sequence[float], Sequence[float]]] = None,
+                 fft_size_delaycor: Optional[int] = None, sideband: str = "U",
+                 invalid: Optional[dict] = None, n_integrations: Optional[int] = None
+                 ) -> List[Tuple[CorrelationParameters, List[bytes]]]:
+    """Synthesise the raw streams of one subband and cut them into slices.
+
+    offsets/rates: per-station delay polynomial tau = offset + rate * t.
+    extra: (rho2, offsets2, rates2) adds a second common component.
+    invalid: {stream_index: [(byte_start_in_slice_raw, nbytes), ...]} marks
+    input bytes invalid (relative to the first byte sent for each slice).
+    """
+    start = mjd_to_ticks(START_MJD, 0.0)
+    tps = int(round(CLOCK_RATE / fs))
+    t_int_ticks = int(round(t_int_usec * (CLOCK_RATE / 1e6)))
+    n_int = n_integrations or int(round(seconds * 1e6 / t_int_usec))
+    total_sec = n_int * t_int_usec / 1e6
+    # margin before the first slice: covers the most negative delay
+    max_neg = max(0.0, -min(min(offsets), min(o + r * total_sec for o, r in zip(offsets, rates))))
+    max_pos = max(0.0, max(max(offsets), max(o + r * total_sec for o, r in zip(offsets, rates))))
+    spb = 8 // bits
+    margin = int(math.ceil(max_neg * fs)) + 4096
+    margin -= margin % (4 * spb)
+    raw_start = start - margin * tps
+    n_raw = margin + int(total_sec * fs) + int(math.ceil(max_pos * fs)) + 8192
+    n_raw += (-n_raw) % (4096 * spb)
+    seconds_model = int(math.ceil(total_sec + (margin / fs) + 2))
+    st_models = [DelayModel.linear(raw_start, o + r * (raw_start - start) / CLOCK_RATE, r, seconds_model)
+                 for o, r in zip(offsets, rates)]
+    # shift the models so that t is seconds since `start` in the polynomial:
+    # tau(t) = o + r * (t - start); pieces start at raw_start
+    streams = []
+    models = []
+    for s in range(n_stations):
+        for p in pols:
+            streams.append(StationStream(station_number=s, station_stream=len(streams), sample_rate=fs,
+                                         channel_freq=sky_freq, bandwidth=bw, sideband=sideband,
+                                         polarisation=p, bits_per_sample=bits))
+            models.append(st_models[s])
+    groups = []
+    # one common signal per polarisation: streams are ordered station-major
+    stream_models_by_pol = {p: [] for p in pols}
+    for i, st in enumerate(streams):
+        stream_models_by_pol[st.polarisation].append(i)
+    order = [i for p in pols for i in stream_models_by_pol[p]]
+    extra_arg = None
+    if extra is not None:
+        rho2, o2, r2 = extra
+        m2 = [DelayModel.linear(raw_start, o + r * (raw_start - start) / CLOCK_RATE, r, seconds_model)
+              for o, r in zip(o2, r2)]
+        extra_arg = [(rho2, [m2[streams[i].station_number] for i in order])]
+    raw_ordered = synth.synth_station_streams(seed, [len(stream_models_by_pol[p]) for p in pols], fs, n_raw,
+                                              [models[i] for i in order], raw_start, rho, bits, sky_freq, extra_arg)
+    raw = [None] * len(streams)
+    for k, i in enumerate(order):
+        raw[i] = raw_ordered[k]
+    slices = []
+    nc = max(256, nchan)
+    nd = fft_size_delaycor or min(256, nc)
+    for k in range(n_int):
+        t0 = start + k * t_int_ticks
+        job = CorrelationParameters(
+            experiment_start=start, start_time=t0, stop_time=t0 + t_int_ticks,
+            integration_time=t_int_ticks,
+            sub_integration_time=int(round(min(t_int_usec, sub_int_usec or 125000) * (CLOCK_RATE / 1e6))),
+            number_channels=nchan, fft_size_delaycor=nd, fft_size_correlation=nc, sample_rate=fs,
+            channel_freq=sky_freq, bandwidth=bw, sideband=sideband, frequency_nr=subband,
+            station_streams=streams, integration_nr=k, slice_nr=k, window=window,
+            delay_models=models)
+        nsamp = job.n_ffts_per_integration(0) * nd
+        hdrs = []
+        for i in range(len(streams)):
+            inv = None
+            if invalid and i in invalid:
+                dsamples = (t0 - raw_start) // tps
+                base = dsamples * bits // 8
+                inv = [(base + a, n) for a, n in invalid[i]]
+            hdrs.append(synth.header_stream(raw[i], raw_start, models[i], t0, nsamp, fs, bits, inv))
+        slices.append((job, hdrs))
+    return slices
+
+
+def config0(seconds: float = 2.0, t_int_usec: int = 1_000_000, nchan: int = 1024):
+    """2 stations, 1 subband, 1 pol, 16 MHz USB at 32 Ms/s, 4.974 GHz, 2-bit,
+    rho 0.05, tau_2 = 1.234567 us + 1e-7 s/s * t, 1024 channels, 1 s."""
+    return build_slices(seed=0, n_stations=2, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
+                        seconds=seconds, t_int_usec=t_int_usec, rho=0.05, bits=2, sky_freq=4_974_000_000,
+                        offsets=[0.0, 1.234567e-6], rates=[0.0, 1e-7])
+
+
+def config1_subband(subband: int, seconds: float = 10.0, t_int_usec: int = 1_000_000, nchan: int = 4096,
+                    n_stations: int = 8):
+    """8 stations x 2 pols, subband `subband` of 8 (32 MHz at 64 Ms/s, 6.0 GHz +
+    32 MHz * subband), 2-bit, rho 0.02, offsets U[-5, 5] ms, rates U[-3e-6, 3e-6]."""
+    rng = np.random.default_rng(1)
+    offsets = rng.uniform(-5e-3, 5e-3, n_stations)
+    rates = rng.uniform(-3e-6, 3e-6, n_stations)
+    return build_slices(seed=1000 + subband, n_stations=n_stations, pols=["R", "L"], fs=64_000_000,
+                        bw=32_000_000, nchan=nchan, seconds=seconds, t_int_usec=t_int_usec, rho=0.02,
+                        bits=2, sky_freq=6_000_000_000 + 32_000_000 * subband, offsets=list(offsets),
+                        rates=list(rates), subband=subband)

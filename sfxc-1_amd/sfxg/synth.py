@@ -1,0 +1,290 @@
+"""Seeded synthetic station data and the input-node HEADER byte stream.
+
+Signal model: x_i(t) = sqrt(rho) * s(t - tau_i(t)) + sqrt(1 - rho) * n_i(t),
+s and n_i i.i.d. N(0, 1), quantised to 2 bits (thresholds +/- v*sigma, codes
+offset binary {0:-hi, 1:-lo, 2:+lo, 3:+hi}) or 1 bit (threshold 0), packed
+first-sample-in-LSB (the order sfxc/src/bit2float_worker.cc:25-31 decodes).
+
+The HEADER stream the correlator node receives for one slice is produced by
+restating the input node's integer-delay rule:
+  * Input_node_tasklet::get_delay / get_delays (sfxc/src/input_node_tasklet.cc:235-282):
+    n = floor(tau * fs + 0.5), split into whole bytes and remaining samples,
+    change instants at sample resolution;
+  * Input_node_data_writer::do_task / write_* (sfxc/src/input_node_data_writer.cc:105-228,
+    388-462): DELAY(remaining) at slice start, DATA runs broken at delay changes
+    (byte positions from get_next_delay_pos, :421-433) and at SHRT_MAX bytes,
+    DELAY at each change, ENDSTREAM at the end.
+The input node's read buffer is taken as one block covering the slice, so
+DATA records are not additionally split at input-block boundaries (the
+correlator node's result does not depend on where DATA records are split).
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Tuple
+
+import numpy as np
+
+from .job import CLOCK_RATE, DelayModel, usec_to_ticks
+
+HEADER_DATA, HEADER_INVALID, HEADER_DELAY, HEADER_ENDSTREAM = 0, 1, 2, 3
+SHRT_MAX = 32767
+
+
+# --------------------------------------------------------------------------
+# integer delay rule
+# --------------------------------------------------------------------------
+def delay_split(n_samples: np.ndarray, bits: int):
+    """Input_node_tasklet::get_delay (input_node_tasklet.cc:45-63): split an
+    integer sample delay into (bytes, remaining samples)."""
+    spb = 8 // bits
+    n = np.asarray(n_samples, dtype=np.int64)
+    nbytes = np.floor((n - 1) / spb).astype(np.int64)
+    rem = n - nbytes * spb
+    full = rem * bits == 8
+    nbytes = np.where(full, nbytes + 1, nbytes)
+    rem = np.where(full, 0, rem)
+    return nbytes, rem
+
+
+def integer_delay(model: DelayModel, ticks, fs: int) -> np.ndarray:
+    return np.floor(model.eval(ticks) * fs + 0.5).astype(np.int64)
+
+
+def delay_changes(model: DelayModel, start_ticks: int, nsamples: int, fs: int) -> List[Tuple[int, int]]:
+    """Sample instants (tick, n) where floor(tau*fs + 0.5) changes inside the
+    slice: what the bisection of get_delays (input_node_tasklet.cc:235-262)
+    finds for a monotonic delay model (one change per sample step at most).
+    Returns the list including the start delay as its first element."""
+    tps = int(round(CLOCK_RATE / fs))
+    out = [(start_ticks, int(integer_delay(model, start_ticks, fs)))]
+    n0 = out[0][1]
+    n1 = int(integer_delay(model, start_ticks + nsamples * tps, fs))
+    if n0 == n1:
+        return out
+    step = 1 if n1 > n0 else -1
+    # one bisection per integer level, all levels at once: first sample index
+    # k at which n(k) has reached level m
+    m = np.arange(n0 + step, n1 + step, step, dtype=np.int64)
+    lo = np.zeros(m.shape, dtype=np.int64)
+    hi = np.full(m.shape, nsamples, dtype=np.int64)
+    while np.any(hi - lo > 1):
+        mid = (lo + hi) // 2
+        nm = integer_delay(model, start_ticks + mid * tps, fs)
+        reached = (nm - m) * step >= 0
+        hi = np.where(reached, mid, hi)
+        lo = np.where(reached, lo, mid)
+    for k, level in zip(hi.tolist(), m.tolist()):
+        out.append((start_ticks + k * tps, level))
+    return out
+
+
+def header_stream(raw: np.ndarray, raw_start_ticks: int, model: DelayModel, start_ticks: int,
+                  nsamples: int, fs: int, bits: int, invalid=None) -> bytes:
+    """Input_node_data_writer::do_task for one slice (input_node_data_writer.cc:105-228).
+
+    raw: the station's packed byte stream whose first byte holds the sample
+    at raw_start_ticks. invalid: optional list of (byte_start, nbytes) ranges
+    of raw flagged invalid (the Input_buffer_element invalid list).
+    """
+    spb = 8 // bits
+    tps = int(round(CLOCK_RATE / fs))
+    byte_length = usec_to_ticks(8 * 1_000_000.0 / (fs * bits))
+    changes = delay_changes(model, start_ticks, nsamples, fs)
+    dl = []
+    for t, n in changes:
+        b, r = delay_split(np.array([n]), bits)
+        dl.append((t, int(b[0]), int(r[0])))
+    out = bytearray()
+
+    def w_delay(r):
+        out.extend([HEADER_DELAY, r & 0xFF])
+
+    def w_data(n, start):
+        written = 0
+        while written < n:
+            k = min(n - written, SHRT_MAX)
+            out.append(HEADER_DATA)
+            out.extend(int(k).to_bytes(2, "little"))
+            out.extend(raw[start + written:start + written + k].tobytes())
+            written += k
+
+    def w_invalid(nsamp):
+        written = 0
+        while written < nsamp:
+            k = min(nsamp - written, SHRT_MAX)
+            out.append(HEADER_INVALID)
+            out.extend(int(k).to_bytes(2, "little"))
+            written += k
+
+    current = start_ticks
+    di = 0
+    while di < len(dl) - 1 and dl[di + 1][0] <= current + byte_length:
+        di += 1
+    dsamples = (current - raw_start_ticks) // tps
+    byte_offset = dsamples * bits // 8 + dl[di][1]
+    if byte_offset < 0:
+        raise ValueError("slice starts before the raw data")
+    slice_size = nsamples + dl[di][2]
+    w_delay(dl[di][2])
+    current -= dl[di][2] * tps
+
+    def next_delay_pos():
+        if di < len(dl) - 1:
+            pos = int((dl[di + 1][0] - current) / byte_length)
+        else:
+            pos = (2**31 - 1) // 2
+        return max(pos, 0)
+
+    inv = sorted(invalid or [])
+    inv_i = 0
+    block_size = len(raw)
+    index = byte_offset
+    ndp = next_delay_pos() + byte_offset
+    total = min(block_size - byte_offset, (slice_size + spb - 1) // spb)
+    end_index = total + byte_offset
+    nip = inv[0][0] if inv else block_size + 1
+    while index < end_index:
+        if index >= ndp:
+            di += 1
+            w_delay(dl[di][2])
+            d = dl[di][2] - dl[di - 1][2]
+            slice_size += -1 if (d > 1 or d == -1) else 1
+            ndp = next_delay_pos() + byte_offset
+            total = min(block_size - byte_offset, (slice_size + spb - 1) // spb)
+            end_index = total + byte_offset
+        elif index >= nip:
+            n = inv[inv_i][1]
+            end_pos = min(nip + n, ndp)
+            nr = max(0, end_pos - index)
+            nr = min(nr, end_index - index)
+            if nr > 0:
+                w_invalid(nr * spb)
+                index += nr
+            if end_pos == nip + n:
+                inv_i += 1
+                nip = inv[inv_i][0] if inv_i < len(inv) else block_size + 1
+        else:
+            k = min(ndp - index, end_index - index, nip - index)
+            w_data(k, index)
+            index += k
+    if index > block_size:
+        raise ValueError("raw data too short for the slice")
+    slice_size -= total * spb
+    if slice_size > 0:
+        raise ValueError("raw data too short for the slice")
+    out.append(HEADER_ENDSTREAM)
+    return bytes(out)
+
+
+# --------------------------------------------------------------------------
+# signal synthesis
+# --------------------------------------------------------------------------
+def _frac_delay_fir(x: np.ndarray, frac: float, taps: int = 32) -> np.ndarray:
+    """Delay x by frac samples (0 <= frac < 1) with a Blackman-windowed sinc."""
+    k = np.arange(-taps // 2 + 1, taps // 2 + 1)
+    h = np.sinc(k - frac) * np.blackman(taps + 2)[1:-1]
+    h /= h.sum()
+    return np.convolve(x, h, mode="same")
+
+
+def analytic(x: np.ndarray) -> np.ndarray:
+    """x + i H(x) by FFT."""
+    n = len(x)
+    X = np.fft.rfft(x)
+    X[1:(n + 1) // 2] *= 2
+    full = np.zeros(n, dtype=np.complex128)
+    full[:len(X)] = X
+    if n % 2 == 0:
+        full[n // 2] = X[n // 2]
+    return np.fft.ifft(full)
+
+
+def delayed_signal(common_a: np.ndarray, common_start: int, n: int, delays_samples: np.ndarray,
+                   fringe_cycles: np.ndarray, block: int = 4096) -> np.ndarray:
+    """Real baseband sample k (k = 0 .. n-1) of a signal delayed at RF:
+    Re{ s_a(k - d(k)) exp(-2 pi i nu_LO tau(k)) } (upper sideband), with
+    common_a[j] the analytic baseband signal s_a at sample common_start + j,
+    d(k) given per block of `block` samples (a linear delay moves by far less
+    than a sample per block) and nu_LO tau(k) given per sample in cycles."""
+    out = np.empty(n, dtype=np.float64)
+    for b0 in range(0, n, block):
+        b1 = min(n, b0 + block)
+        d = delays_samples[b0 // block]
+        di = int(math.floor(d))
+        frac = d - di
+        lo = b0 - di - common_start - 32
+        seg = common_a[lo:lo + (b1 - b0) + 64]
+        y = _frac_delay_fir(seg.real, frac) + 1j * _frac_delay_fir(seg.imag, frac)
+        ph = np.exp(-2j * np.pi * fringe_cycles[b0:b1])
+        out[b0:b1] = (y[32:32 + (b1 - b0)] * ph).real
+    return out
+
+
+def quantise(x: np.ndarray, bits: int, threshold: float = 0.9816) -> np.ndarray:
+    if bits == 2:
+        return (np.digitize(x, [-threshold, 0.0, threshold])).astype(np.uint8)
+    return (x >= 0).astype(np.uint8)
+
+
+def pack(codes: np.ndarray, bits: int) -> np.ndarray:
+    spb = 8 // bits
+    c = codes.reshape(-1, spb).astype(np.uint8)
+    out = np.zeros(c.shape[0], dtype=np.uint8)
+    for k in range(spb):
+        out |= (c[:, k] << (bits * k)).astype(np.uint8)
+    return out
+
+
+def synth_station_streams(seed: int, n_streams_per_signal: List[int], fs: int, n_samples: int,
+                          delays: List[DelayModel], raw_start_ticks: int, rho: float, bits: int,
+                          lo_freq: float, extra_sources=None) -> List[np.ndarray]:
+    """Raw packed byte streams, one per entry of `delays` (upper sideband,
+    LO at lo_freq Hz).
+
+    n_streams_per_signal groups consecutive streams sharing one common signal
+    (e.g. one group per polarisation). extra_sources: list of (rho2, [DelayModel
+    per stream]) adding further common components (one per group).
+    """
+    rng = np.random.default_rng(seed)
+    tps = int(round(CLOCK_RATE / fs))
+    pad = 1 << 12
+    block = 4096
+    tt_blk = raw_start_ticks + np.arange(0, n_samples + block, block) * tps
+
+    def per_stream(m: DelayModel):
+        d_blk = m.eval(tt_blk) * fs
+        # nu_LO * tau per sample, linear interpolation of tau inside a block
+        tau_blk = m.eval(tt_blk)
+        k = np.arange(n_samples)
+        tau = np.interp(k, np.arange(len(tau_blk)) * block, tau_blk)
+        fr = lo_freq * tau
+        return d_blk, fr - np.floor(fr)
+
+    per = [per_stream(m) for m in delays]
+    dmin = min(int(math.floor(d.min())) for d, _ in per)
+    dmax = max(int(math.ceil(d.max())) for d, _ in per)
+    extra_per = [[per_stream(m) for m in models] for _, models in (extra_sources or [])]
+    for ep in extra_per:
+        dmin = min(dmin, min(int(math.floor(d.min())) for d, _ in ep))
+        dmax = max(dmax, max(int(math.ceil(d.max())) for d, _ in ep))
+    out = []
+    idx = 0
+    for group in n_streams_per_signal:
+        cstart = -dmax - pad
+        clen = n_samples + (dmax - dmin) + 2 * pad
+        common = analytic(rng.standard_normal(clen))
+        extras = [(rho2, analytic(rng.standard_normal(clen)), k)
+                  for k, (rho2, _) in enumerate(extra_sources or [])]
+        for _ in range(group):
+            d, fr = per[idx]
+            x = np.sqrt(rho) * delayed_signal(common, cstart, n_samples, d, fr)
+            rest = 1.0 - rho
+            for rho2, sig2, k in extras:
+                d2, fr2 = extra_per[k][idx]
+                x += np.sqrt(rho2) * delayed_signal(sig2, cstart, n_samples, d2, fr2)
+                rest -= rho2
+            x += np.sqrt(rest) * rng.standard_normal(n_samples)
+            out.append(pack(quantise(x, bits), bits))
+            idx += 1
+    return out
