@@ -1,0 +1,228 @@
+/*
+ * sfxg.h -- C ABI of the MI355X correlator-node data path.
+ *
+ * One context replaces the DSP objects one SFXC correlator node owns for a
+ * time slice: a Bit2float_worker per stream, a Delay_correction per stream and
+ * the Correlation_core (or Correlation_core_pulsar). The reference
+ * interface each entry point replaces is cited next to it (paths relative
+ * to the SFXC source tree, sfxc/).
+ *
+ * Conventions: every function returns SFXG_OK (0) or a negative SFXG_ERR_*
+ * code; sfxg_last_error() then describes the failure. The caller owns all host
+ * buffers. The context owns its device memory. Calls on one context are not
+ * thread-safe and must come from one thread at a time, as the reference's
+ * correlator-node main thread drives Delay_correction and Correlation_core
+ * (src/correlator_node.cc:150-188).
+ *
+ * Times are int64 ticks of the reference Time class: 512 MHz clock ticks
+ * since MJD 51544 (include/correlator_time.h:9-64, include/utils.h:36).
+ */
+#ifndef SFXG_H
+#define SFXG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFXG_OK 0
+#define SFXG_ERR_ARG (-1)      /* invalid argument or unsupported parameter */
+#define SFXG_ERR_STREAM (-2)   /* malformed or incomplete HEADER byte stream */
+#define SFXG_ERR_DEVICE (-3)   /* HIP runtime failure */
+#define SFXG_ERR_STATE (-4)    /* call out of order (e.g. correlate before all streams) */
+#define SFXG_ERR_NOMEM (-5)    /* device or host allocation failed */
+
+/* Window functions: include/utils.h:209-213 */
+#define SFXG_WINDOW_NONE 0
+#define SFXG_WINDOW_RECT 1
+#define SFXG_WINDOW_COS 2
+#define SFXG_WINDOW_HAMMING 3
+#define SFXG_WINDOW_HANN 4
+
+/* Sample decode tables. REFERENCE: {-7,-2,+2,+7} and {-5,+5}
+ * (src/bit2float_worker.cc:4-10). CONFIG: {-3.3359,-1,+1,+3.3359} and {-1,+1}. */
+#define SFXG_LUT_REFERENCE 0
+#define SFXG_LUT_CONFIG 1
+
+/* X-step arithmetic. FP32 is the reference's arithmetic
+ * (src/correlation_core.cc:241-273). */
+#define SFXG_PREC_FP32 0
+
+/* Stages readable with sfxg_fetch_stage (verification of each stage). */
+#define SFXG_STAGE_SAMPLES 0      /* Bit2float_worker output, float[n_samples] */
+#define SFXG_STAGE_TIME_SERIES 1  /* fringe-stopped real series, float[n_samples] */
+#define SFXG_STAGE_SPECTRA 2      /* float2[n_segments][N_c + 1] */
+#define SFXG_STAGE_ACCUMULATED 3  /* phase-centre / bin accumulators, float2[n_bins][B][N_c + 1] */
+#define SFXG_STAGE_NORMALISED 4   /* normalised visibilities, float2[n_bins][B][nchan + 1] */
+
+typedef struct sfxg_ctx sfxg_ctx;
+
+/* Correlation_parameters::Station_parameters (include/control_parameters.h:132-151). */
+typedef struct {
+  int32_t station_number;
+  int32_t station_stream;
+  int32_t sample_rate;
+  int32_t bandwidth;
+  int64_t channel_freq;
+  char sideband;      /* 'U' or 'L' */
+  char polarisation;  /* 'R' or 'L' */
+  int16_t reserved0;
+  int32_t bits_per_sample;
+  double LO_offset;
+  int32_t tsys_freq;
+  int32_t reserved1;
+} sfxg_station_parameters;
+
+/* Correlation_parameters (include/control_parameters.h:117-182), one
+ * integration slice of one subband. */
+typedef struct {
+  int64_t experiment_start;
+  int64_t start_time;
+  int64_t stop_time;
+  int64_t integration_time;
+  int64_t sub_integration_time;
+  int32_t number_channels;
+  int32_t fft_size_delaycor;
+  int32_t fft_size_correlation;
+  int32_t integration_nr;
+  int32_t slice_nr;
+  int32_t slice_offset;
+  int32_t sample_rate;
+  int32_t bandwidth;
+  int64_t channel_freq;
+  char sideband;      /* 'U' or 'L' */
+  char polarisation;
+  int16_t reserved0;
+  int32_t frequency_nr;
+  int32_t cross_polarize;
+  int32_t reference_station;
+  int32_t window;
+  int32_t n_phase_centers;
+  int32_t pulsar_binning;
+  int32_t n_streams;
+  const sfxg_station_parameters *station_streams;
+} sfxg_correlation_parameters;
+
+/* The delay model of one stream as Delay_table_akima evaluates it
+ * (src/delay_table_akima.cc:114-166): a piecewise cubic on a uniform grid
+ * starting at scan_begin, coefficients c0..c3 of each piece in seconds since
+ * the piece start. Clock offset and rate are folded into the delay. */
+typedef struct {
+  int64_t scan_begin;
+  double interval;
+  int32_t n_pieces;
+  int32_t reserved0;
+  const double *delay;      /* 4 * n_pieces, seconds */
+  const double *phase;      /* 4 * n_pieces, radians; NULL means 0 */
+  const double *amplitude;  /* 4 * n_pieces; NULL means 1 */
+} sfxg_delay_model;
+
+/* Pulsar_parameters::Pulsar plus the polyco in use
+ * (include/control_parameters.h:73-98). */
+typedef struct {
+  int32_t nbins;      /* on-pulse bins; an off-pulse bin 0 is added */
+  int32_t n_coef;
+  double gate_start;
+  double gate_stop;
+  double tmid;
+  double DM;
+  double ref_phase;
+  double ref_freq;
+  double obs_freq;
+  const double *coef;
+} sfxg_pulsar;
+
+typedef struct {
+  int32_t lut;        /* SFXG_LUT_* */
+  int32_t precision;  /* SFXG_PREC_* */
+} sfxg_options;
+
+/* Creates a context on HIP device `device`. Replaces the construction of
+ * Correlator_node's DSP objects (src/correlator_node.cc:37-47,213-249). */
+int sfxg_create(int device, sfxg_ctx **out);
+void sfxg_destroy(sfxg_ctx *ctx);
+const char *sfxg_last_error(const sfxg_ctx *ctx);
+int sfxg_set_options(sfxg_ctx *ctx, const sfxg_options *opt);
+
+/* Replaces Correlator_node::set_parameters fan-out
+ * (src/correlator_node.cc:310-405): Correlation_core::set_parameters
+ * (src/correlation_core.cc:100-131) or Correlation_core_pulsar::set_parameters
+ * (src/correlation_core_pulsar.cc:12-99) when pulsar != NULL,
+ * Delay_correction::set_parameters (src/delay_correction.cc:186-259) and
+ * Bit2float_worker::set_new_parameters (src/bit2float_worker.cc:235-276).
+ * models: one per stream. uvw: 3 doubles per stream (or NULL for zeros). */
+int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params,
+                        const sfxg_delay_model *models, const double *uvw,
+                        const sfxg_pulsar *pulsar);
+
+/* Hands the complete HEADER byte stream of stream `stream_idx` (index into
+ * station_streams) for this slice to the context. Replaces the reader thread
+ * filling Channel_circular_input_buffer and Bit2float_worker::do_task
+ * (src/correlator_node_data_reader_tasklet.cc:28-109,
+ * src/bit2float_worker.cc:35-197). The bytes are parsed on the host and the
+ * sample payload is copied to the device. */
+int sfxg_push_stream(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *bytes, size_t n);
+
+/* Same as sfxg_push_stream, for a stream whose sample payload already lives
+ * in device memory: `control` holds the HEADER records with the payload bytes
+ * of every DATA record left out (the 3-byte DATA header stays), and
+ * device_payload holds those payload bytes back to back. device_payload must
+ * stay valid until sfxg_synchronize returns after sfxg_correlate. */
+int sfxg_push_stream_elided(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *control, size_t n,
+                            const void *device_payload, size_t payload_bytes);
+
+/* Enqueues the slice on the context's HIP stream: unpack and integer shift,
+ * Delay_correction (src/delay_correction.cc:21-184) and Correlation_core
+ * (src/correlation_core.cc:20-73, :241-314; pulsar: src/correlation_core_pulsar.cc:101-238).
+ * Returns once the work is enqueued. */
+int sfxg_correlate(sfxg_ctx *ctx);
+int sfxg_synchronize(sfxg_ctx *ctx);
+
+/* Size in bytes of the slice's output: the bytes Correlation_core::integration_write
+ * (src/correlation_core.cc:316-477) passes to Data_writer::put_bytes, one
+ * record per bin (pulsar) or phase centre. Equals the slice_size negotiated by
+ * src/correlator_node.cc:386-398. */
+int sfxg_output_size(sfxg_ctx *ctx, size_t *bytes);
+/* Copies the output bytes (waits for the slice to finish). */
+int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written);
+
+/* bit_statistics::get_statistics / get_tsys of a stream
+ * (src/bit_statistics.cc:33-91). Waits for the slice to finish. */
+int sfxg_get_statistics(sfxg_ctx *ctx, int32_t stream_idx, int32_t levels[5], int32_t tsys[4]);
+
+/* Correlation_core::number_of_baselines (include/correlation_core.h:50-52). */
+int sfxg_number_of_baselines(sfxg_ctx *ctx, int32_t *n);
+
+/* Copies one stage's device output (SFXG_STAGE_*) for verification; waits
+ * for the slice. stream_idx selects the stream for per-stream stages. */
+int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst, size_t cap,
+                     size_t *written);
+
+/* Kernel durations of the last sfxg_correlate, in milliseconds, measured with
+ * HIP events on the context's stream: ms[SFXG_KERNEL_*]. The reference's
+ * counterpart is its per-component timing instrumentation
+ * (src/correlation_core.cc:11-17, src/delay_correction.cc:12-18,
+ * lib/common/src/monitor.h:80-173). n: capacity of ms. */
+#define SFXG_KERNEL_STATS 0
+#define SFXG_KERNEL_DELAY 1
+#define SFXG_KERNEL_CHANNELISE 2
+#define SFXG_KERNEL_XSTEP 3
+#define SFXG_KERNEL_NORMALISE 4
+#define SFXG_N_KERNELS 5
+int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n);
+
+/* Synthetic 2-bit/1-bit raw station data written straight into device
+ * memory (for throughput measurements on device-resident input): n_bytes
+ * bytes of stream `stream_seed` starting at raw sample `first_sample`,
+ * x = sqrt(rho) * s[k - delay_samples] + sqrt(1 - rho) * n_stream[k], s shared
+ * by all streams with the same common_seed. No reference counterpart. */
+int sfxg_synthesize_raw(int device, void *device_dst, size_t n_bytes, int32_t bits_per_sample,
+                        int64_t first_sample, int64_t delay_samples, uint64_t common_seed,
+                        uint64_t stream_seed, float rho, float threshold);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,826 @@
+// C ABI of the correlator-node data path (include/sfxg.h): parameter
+// handling, host-side parsing, device memory and kernel scheduling for one
+// time slice at a time.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/sfxg.h"
+#include "sfxg_internal.h"
+
+using namespace sfxg;
+
+namespace {
+
+struct DBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+struct ModelCopy {
+  int64_t scan_begin = 0;
+  double interval = 1.0;
+  int n = 0;
+  std::vector<double> delay, phase, amp;
+};
+
+const double kLut2[2][4] = {{-7, -2, 2, 7}, {-3.3359, -1, 1, 3.3359}};  // bit2float_worker.cc:4-6
+const double kLut1[2][2] = {{-5, 5}, {-1, 1}};                          // bit2float_worker.cc:8-10
+
+int nr_ffts_per_integration_slice(int64_t integration_ticks, int data_rate, int fft_size) {
+  // Control_parameters::nr_ffts_per_integration_slice (control_parameters.h:301-307),
+  // called with (int)integration_time.get_time_usec()
+  const int usec = (int)((double)integration_ticks / (kClockRate / 1000000));
+  const int64_t t_int = (int64_t)llround(usec * (kClockRate / 1000000));
+  const int64_t t_fft = (int64_t)llround(fft_size / (data_rate / 1000000.) * (kClockRate / 1000000));
+  return (int)floor((double)t_int / (double)t_fft);
+}
+
+}  // namespace
+
+struct sfxg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sfxg_options opt{SFXG_LUT_REFERENCE, SFXG_PREC_FP32};
+  std::string err;
+
+  // slice parameters
+  bool have_params = false;
+  sfxg_correlation_parameters p{};
+  std::vector<sfxg_station_parameters> st;
+  std::vector<ModelCopy> models;
+  std::vector<double> uvw;
+  bool pulsar = false;
+  sfxg_pulsar psr{};
+  std::vector<double> psr_coef;
+
+  // derived
+  int M = 0, nd = 0, nc = 0, nchan = 0, B = 0, nbins = 1;
+  int64_t n_ffts_per_integration = 0, n_blocks = 0, n_samples = 0, nseg = 0, S = 0;
+  int n_integration = 0, n_sub = 0, nfft_max = 0;
+  std::vector<int32_t> bl_first, bl_second;
+  std::vector<StreamPlan> plans;
+  std::vector<int32_t> seg_per_chunk, folds;
+  std::vector<double> flag_factor;
+  std::vector<int64_t> n_flagged_first;
+  int64_t total_samples = 0;
+
+  // device buffers
+  DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
+      flags, bl, fold_buf, model_buf, tw_nd, tw_n, tw_2n, window, offsets, samples;
+  int tables_nc = -1;
+  int tables_window = -1;
+  bool enqueued = false;
+  hipEvent_t done = nullptr;
+  hipEvent_t ev[SFXG_N_KERNELS + 1] = {};
+
+  int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  int hip(hipError_t e, const char *what) {
+    if (e == hipSuccess) return SFXG_OK;
+    return fail(SFXG_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+  }
+};
+
+#define HIPCHK(ctx, call)                         \
+  do {                                            \
+    int _rc = (ctx)->hip((call), #call);          \
+    if (_rc != SFXG_OK) return _rc;               \
+  } while (0)
+
+extern "C" {
+
+int sfxg_create(int device, sfxg_ctx **out) {
+  if (!out) return SFXG_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return SFXG_ERR_DEVICE;
+  std::unique_ptr<sfxg_ctx> ctx(new sfxg_ctx());
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) return SFXG_ERR_DEVICE;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SFXG_ERR_DEVICE;
+  if (hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming) != hipSuccess) return SFXG_ERR_DEVICE;
+  for (hipEvent_t &e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) return SFXG_ERR_DEVICE;
+  *out = ctx.release();
+  return SFXG_OK;
+}
+
+void sfxg_destroy(sfxg_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DBuf *bufs[] = {&ctx->payload, &ctx->runs, &ctx->block_run, &ctx->counted, &ctx->dstreams,
+                  &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
+                  &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->model_buf, &ctx->tw_nd,
+                  &ctx->tw_n, &ctx->tw_2n, &ctx->window, &ctx->offsets, &ctx->samples};
+  for (DBuf *b : bufs) b->release();
+  if (ctx->done) (void)hipEventDestroy(ctx->done);
+  for (hipEvent_t e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *sfxg_last_error(const sfxg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sfxg_set_options(sfxg_ctx *ctx, const sfxg_options *opt) {
+  if (!ctx || !opt) return SFXG_ERR_ARG;
+  if (opt->lut != SFXG_LUT_REFERENCE && opt->lut != SFXG_LUT_CONFIG)
+    return ctx->fail(SFXG_ERR_ARG, "unknown lut %d", opt->lut);
+  if (opt->precision != SFXG_PREC_FP32)
+    return ctx->fail(SFXG_ERR_ARG, "unsupported precision %d", opt->precision);
+  ctx->opt = *opt;
+  return SFXG_OK;
+}
+
+static int sb_sign(char c) { return c == 'L' ? -1 : 1; }
+
+int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params,
+                        const sfxg_delay_model *models, const double *uvw, const sfxg_pulsar *pulsar) {
+  if (!ctx || !params || !models) return SFXG_ERR_ARG;
+  if (ctx->enqueued) (void)hipStreamSynchronize(ctx->stream);
+  ctx->enqueued = false;
+  ctx->have_params = false;
+  const sfxg_correlation_parameters &p = *params;
+  if (p.n_streams <= 0 || !p.station_streams) return ctx->fail(SFXG_ERR_ARG, "no station streams");
+  if (p.fft_size_correlation != p.number_channels)
+    return ctx->fail(SFXG_ERR_ARG, "fft_size_correlation != number_channels (lag re-channelisation) "
+                                   "is not supported");
+  if (p.n_phase_centers != 1) return ctx->fail(SFXG_ERR_ARG, "only one phase centre is supported");
+  if (p.fft_size_delaycor != 256) return ctx->fail(SFXG_ERR_ARG, "fft_size_delaycor must be 256");
+  const int nc = p.fft_size_correlation;
+  if (nc < 256 || nc > 16384 || (nc & (nc - 1)))
+    return ctx->fail(SFXG_ERR_ARG, "fft_size_correlation must be a power of two in [256, 16384]");
+  if (p.window < SFXG_WINDOW_NONE || p.window > SFXG_WINDOW_HANN)
+    return ctx->fail(SFXG_ERR_ARG, "unknown window %d", p.window);
+  for (int i = 0; i < p.n_streams; i++) {
+    const sfxg_station_parameters &s = p.station_streams[i];
+    if (s.sample_rate != p.sample_rate)
+      return ctx->fail(SFXG_ERR_ARG, "stream %d: station sample rate differs from the job's", i);
+    if (s.bits_per_sample != 1 && s.bits_per_sample != 2)
+      return ctx->fail(SFXG_ERR_ARG, "stream %d: bits_per_sample must be 1 or 2", i);
+    if (s.bandwidth <= 0) return ctx->fail(SFXG_ERR_ARG, "stream %d: bandwidth", i);
+    // output offset must be zero (delay_correction.cc:226-241)
+    int64_t freq = s.channel_freq;
+    if (p.sideband != s.sideband) freq += sb_sign(s.sideband) * (int64_t)s.bandwidth;
+    const int64_t delta = p.sideband == 'L' ? freq - p.channel_freq : p.channel_freq - freq;
+    if (delta != 0) return ctx->fail(SFXG_ERR_ARG, "stream %d: frequency offset to the job band", i);
+    if (models[i].n_pieces <= 0 || !models[i].delay)
+      return ctx->fail(SFXG_ERR_ARG, "stream %d: empty delay model", i);
+  }
+  ctx->p = p;
+  ctx->st.assign(p.station_streams, p.station_streams + p.n_streams);
+  ctx->p.station_streams = ctx->st.data();
+  ctx->models.assign(p.n_streams, ModelCopy());
+  for (int i = 0; i < p.n_streams; i++) {
+    ModelCopy &m = ctx->models[i];
+    m.scan_begin = models[i].scan_begin;
+    m.interval = models[i].interval;
+    m.n = models[i].n_pieces;
+    m.delay.assign(models[i].delay, models[i].delay + 4 * m.n);
+    if (models[i].phase) m.phase.assign(models[i].phase, models[i].phase + 4 * m.n);
+    if (models[i].amplitude) m.amp.assign(models[i].amplitude, models[i].amplitude + 4 * m.n);
+  }
+  ctx->uvw.assign(3 * p.n_streams, 0.0);
+  if (uvw) std::copy(uvw, uvw + 3 * p.n_streams, ctx->uvw.begin());
+  ctx->pulsar = pulsar != nullptr;
+  if (pulsar) {
+    if (pulsar->nbins <= 0 || pulsar->n_coef < 2 || !pulsar->coef)
+      return ctx->fail(SFXG_ERR_ARG, "invalid pulsar parameters");
+    ctx->psr = *pulsar;
+    ctx->psr_coef.assign(pulsar->coef, pulsar->coef + pulsar->n_coef);
+    ctx->psr.coef = ctx->psr_coef.data();
+  }
+
+  // derived sizes
+  ctx->M = p.n_streams;
+  ctx->nd = p.fft_size_delaycor;
+  ctx->nc = nc;
+  ctx->nchan = p.number_channels;
+  ctx->nbins = pulsar ? pulsar->nbins + 1 : 1;
+  const int nr = nr_ffts_per_integration_slice(p.integration_time, p.sample_rate, nc);
+  ctx->n_ffts_per_integration = (int64_t)(nc / ctx->nd) * nr;  // delay_correction.cc:244-249
+  ctx->n_blocks = ctx->n_ffts_per_integration;
+  ctx->n_samples = ctx->n_blocks * ctx->nd;
+  ctx->n_integration = p.window != SFXG_WINDOW_NONE ? nr - 1 : nr;  // correlation_core.cc:135-142
+  ctx->n_sub = nr_ffts_per_integration_slice(p.sub_integration_time, p.sample_rate, nc);
+  ctx->nfft_max = std::max(kCorrelatorBufferSize / ctx->nd, std::max(nc / ctx->nd, 1));
+  ctx->nseg = ctx->n_integration;
+  ctx->S = ((int64_t)nc + 1 + 63) / 64 * 64;
+  ctx->total_samples = (int64_t)ctx->n_integration * nc;
+  if (ctx->n_integration <= 0) return ctx->fail(SFXG_ERR_ARG, "integration shorter than two FFTs");
+
+  // baselines (correlation_core.cc:149-170)
+  ctx->bl_first.clear();
+  ctx->bl_second.clear();
+  for (int i = 0; i < ctx->M; i++) {
+    ctx->bl_first.push_back(i);
+    ctx->bl_second.push_back(i);
+  }
+  for (int i = 0; i < ctx->M; i++) {
+    for (int j = i + 1; j < ctx->M; j++) {
+      const int s1 = ctx->st[i].station_number, s2 = ctx->st[j].station_number;
+      if (s1 == s2) continue;
+      if (p.reference_station >= 0 && p.reference_station != s1 && p.reference_station != s2) continue;
+      if (s1 < s2) {
+        ctx->bl_first.push_back(i);
+        ctx->bl_second.push_back(j);
+      } else {
+        ctx->bl_first.push_back(j);
+        ctx->bl_second.push_back(i);
+      }
+    }
+  }
+  ctx->B = (int)ctx->bl_first.size();
+  ctx->plans.assign(ctx->M, StreamPlan());
+  ctx->have_params = true;
+  return SFXG_OK;
+}
+
+static int push_common(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *bytes, size_t n, bool elided,
+                       const void *dev_payload, size_t payload_bytes) {
+  if (!ctx) return SFXG_ERR_ARG;
+  if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
+  if (stream_idx < 0 || stream_idx >= ctx->M) return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
+  if (!bytes && n) return ctx->fail(SFXG_ERR_ARG, "null stream");
+  if (ctx->enqueued) {
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->enqueued = false;
+  }
+  const sfxg_station_parameters &s = ctx->st[stream_idx];
+  B2fParams bp;
+  bp.bits_per_sample = s.bits_per_sample;
+  bp.sample_rate = s.sample_rate;
+  bp.base_sample_rate = ctx->p.sample_rate;
+  bp.tsys_freq = s.tsys_freq;
+  bp.fft_size = ctx->nd;
+  bp.nfft_max = (int)(((int64_t)ctx->nfft_max * s.sample_rate) / ctx->p.sample_rate);
+  bp.n_ffts_per_integration = (int)ctx->n_ffts_per_integration;
+  {
+    // Bit2float_worker::set_new_parameters: delay(start_time) * sample_rate into an int
+    const ModelCopy &m = ctx->models[stream_idx];
+    const double sec = (double)(ctx->p.start_time - m.scan_begin) / kClockRate;
+    int64_t i = (int64_t)floor(sec / m.interval);
+    if (i < 0) i = 0;
+    if (i > m.n - 1) i = m.n - 1;
+    const double u = sec - (double)i * m.interval;
+    const double *c = &m.delay[4 * i];
+    const double d = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+    bp.delay_in_samples = (int)(d * s.sample_rate);
+  }
+  StreamPlan &plan = ctx->plans[stream_idx];
+  std::string e;
+  int rc = parse_header_stream(bytes, n, elided, bp, plan, e);
+  if (rc != SFXG_OK) return ctx->fail(rc, "stream %d: %s", stream_idx, e.c_str());
+  if (elided) {
+    if (!dev_payload && plan.payload_bytes > 0) return ctx->fail(SFXG_ERR_ARG, "null device payload");
+    if ((int64_t)payload_bytes < plan.payload_bytes)
+      return ctx->fail(SFXG_ERR_ARG, "stream %d: device payload holds %zu bytes, the records announce %lld",
+                       stream_idx, payload_bytes, (long long)plan.payload_bytes);
+    plan.ext_payload = dev_payload;
+  }
+  return SFXG_OK;
+}
+
+int sfxg_push_stream(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *bytes, size_t n) {
+  return push_common(ctx, stream_idx, bytes, n, false, nullptr, 0);
+}
+
+int sfxg_push_stream_elided(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *control, size_t n,
+                            const void *device_payload, size_t payload_bytes) {
+  return push_common(ctx, stream_idx, control, n, true, device_payload, payload_bytes);
+}
+
+static void make_tables(sfxg_ctx *ctx, std::vector<float2> &tnd, std::vector<float2> &tn,
+                        std::vector<float2> &t2n, std::vector<float> &win) {
+  const int nd = ctx->nd, nc = ctx->nc;
+  tnd.resize(nd);
+  for (int k = 0; k < nd; k++) {
+    const double a = -2.0 * M_PI * k / nd;
+    tnd[k] = make_float2((float)cos(a), (float)sin(a));
+  }
+  tn.resize(nc);
+  for (int k = 0; k < nc; k++) {
+    const double a = -2.0 * M_PI * k / nc;
+    tn[k] = make_float2((float)cos(a), (float)sin(a));
+  }
+  t2n.resize(nc + 1);
+  for (int k = 0; k <= nc; k++) {
+    const double a = -M_PI * k / nc;
+    t2n[k] = make_float2((float)cos(a), (float)sin(a));
+  }
+  // create_window (delay_correction.cc:299-341), n = fft_rot_size = 2 N_c
+  const int n = 2 * nc;
+  win.resize(n);
+  for (int i = 0; i < n; i++) {
+    double v;
+    switch (ctx->p.window) {
+      case SFXG_WINDOW_NONE: v = i < n / 2 ? 1 : 0; break;
+      case SFXG_WINDOW_RECT: v = (i >= n / 4 && i < 3 * n / 4) ? 1 : 0; break;
+      case SFXG_WINDOW_COS: v = sin(M_PI * i / (n - 1)); break;
+      case SFXG_WINDOW_HAMMING: v = 0.54 - 0.46 * cos(2 * M_PI * i / (n - 1)); break;
+      default: v = 0.5 * (1 - cos(2 * M_PI * i / (n - 1))); break;
+    }
+    win[i] = (float)v;
+  }
+}
+
+int sfxg_correlate(sfxg_ctx *ctx) {
+  if (!ctx) return SFXG_ERR_ARG;
+  if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
+  for (int i = 0; i < ctx->M; i++)
+    if (!ctx->plans[i].ready) return ctx->fail(SFXG_ERR_STATE, "stream %d has not been pushed", i);
+  for (int i = 1; i < ctx->M; i++)
+    if (ctx->plans[i].chunks != ctx->plans[0].chunks)
+      return ctx->fail(SFXG_ERR_STATE, "streams deliver different chunkings");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (ctx->enqueued) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const int M = ctx->M, nd = ctx->nd, nc = ctx->nc, B = ctx->B;
+  hipStream_t s = ctx->stream;
+
+  // chunk bookkeeping (delay_correction.cc:24-36, correlation_core.cc:42-72)
+  const bool windowed = ctx->p.window != SFXG_WINDOW_NONE;
+  ctx->seg_per_chunk = segments_per_chunk(ctx->plans[0].chunks, nd, 2LL * nc, windowed);
+  int64_t total_seg = 0;
+  for (int32_t v : ctx->seg_per_chunk) total_seg += v;
+  if (total_seg != ctx->n_integration)
+    return ctx->fail(SFXG_ERR_STATE, "segment count %lld != %d", (long long)total_seg, ctx->n_integration);
+  ctx->folds = fold_points(ctx->seg_per_chunk, ctx->n_integration, ctx->n_sub);
+
+  // invalid data bookkeeping (find_invalid :599-665, integration_normalize :295-313)
+  std::vector<int32_t> levels4(M);
+  for (int i = 0; i < M; i++)
+    levels4[i] = (int32_t)(((int64_t)ctx->p.sample_rate * ctx->plans[i].n_invalid) / ctx->st[i].sample_rate);
+  ctx->flag_factor.assign(B, 1.0);
+  ctx->n_flagged_first.assign(B, 0);
+  for (int b = M; b < B; b++) {
+    const int a = ctx->bl_first[b], c = ctx->bl_second[b];
+    int64_t nf[2] = {0, 0};
+    find_invalid(ctx->plans[a].invalid, ctx->plans[c].invalid,
+                 ctx->st[a].sample_rate * 1. / ctx->p.sample_rate, ctx->st[c].sample_rate * 1. / ctx->p.sample_rate,
+                 nf);
+    const int64_t nv1 = ctx->total_samples - levels4[a], nv2 = ctx->total_samples - levels4[c];
+    const double N1 = nv1 > 0 ? 1 - nf[0] * 1. / nv1 : 1;
+    const double N2 = nv2 > 0 ? 1 - nf[1] * 1. / nv2 : 1;
+    double N = N1 * N2;
+    if (N < 0.01) N = 1;
+    ctx->flag_factor[b] = N;
+    ctx->n_flagged_first[b] = nf[0];
+  }
+
+  // upload tables
+  if (ctx->tables_nc != nc || ctx->tables_window != ctx->p.window) {
+    std::vector<float2> tnd, tn, t2n;
+    std::vector<float> win;
+    make_tables(ctx, tnd, tn, t2n, win);
+    HIPCHK(ctx, ctx->tw_nd.ensure(tnd.size() * sizeof(float2)));
+    HIPCHK(ctx, ctx->tw_n.ensure(tn.size() * sizeof(float2)));
+    HIPCHK(ctx, ctx->tw_2n.ensure(t2n.size() * sizeof(float2)));
+    HIPCHK(ctx, ctx->window.ensure(win.size() * sizeof(float)));
+    HIPCHK(ctx, hipMemcpy(ctx->tw_nd.p, tnd.data(), tnd.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->tw_n.p, tn.data(), tn.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->tw_2n.p, t2n.data(), t2n.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->window.p, win.data(), win.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->tables_nc = nc;
+    ctx->tables_window = ctx->p.window;
+  }
+
+  // per-stream tables, concatenated
+  std::vector<int64_t> pay_off(M), run_off(M), cnt_off(M), blk_off(M), mod_off(M);
+  int64_t pay_tot = 0, run_tot = 0, cnt_tot = 0, mod_tot = 0, max_counted = 0;
+  for (int i = 0; i < M; i++) {
+    const StreamPlan &pl = ctx->plans[i];
+    pay_off[i] = pay_tot;
+    if (!pl.ext_payload) pay_tot += ((int64_t)pl.payload.size() + 255) / 256 * 256;
+    run_off[i] = run_tot;
+    run_tot += (int64_t)pl.runs.size();
+    cnt_off[i] = cnt_tot;
+    cnt_tot += (int64_t)pl.counted.size();
+    if (!pl.counted.empty())
+      max_counted = std::max<int64_t>(max_counted, pl.counted.back().ordinal + pl.counted.back().n_bytes);
+    blk_off[i] = (int64_t)i * ctx->n_blocks;
+    mod_off[i] = mod_tot;
+    const ModelCopy &m = ctx->models[i];
+    mod_tot += (int64_t)(m.delay.size() + m.phase.size() + m.amp.size());
+  }
+  HIPCHK(ctx, ctx->payload.ensure((size_t)pay_tot));
+  HIPCHK(ctx, ctx->runs.ensure((size_t)run_tot * sizeof(Run)));
+  HIPCHK(ctx, ctx->counted.ensure((size_t)std::max<int64_t>(cnt_tot, 1) * sizeof(CountRange)));
+  HIPCHK(ctx, ctx->block_run.ensure((size_t)M * ctx->n_blocks * sizeof(int32_t)));
+  HIPCHK(ctx, ctx->model_buf.ensure((size_t)mod_tot * sizeof(double)));
+  HIPCHK(ctx, ctx->dstreams.ensure((size_t)M * sizeof(DevStream)));
+  HIPCHK(ctx, ctx->time_series.ensure((size_t)M * ctx->n_samples * sizeof(float)));
+  HIPCHK(ctx, ctx->spectra.ensure((size_t)M * ctx->nseg * ctx->S * sizeof(float2)));
+  const size_t vis_elems = (size_t)ctx->nbins * B * (nc + 1);
+  HIPCHK(ctx, ctx->vis.ensure(vis_elems * sizeof(float2)));
+  HIPCHK(ctx, ctx->norm_out.ensure(vis_elems * sizeof(float2)));
+  HIPCHK(ctx, ctx->hist.ensure((size_t)M * 512 * sizeof(int32_t)));
+  HIPCHK(ctx, ctx->norms.ensure((size_t)ctx->nbins * M * sizeof(double)));
+  HIPCHK(ctx, ctx->flags.ensure((size_t)B * sizeof(double)));
+  HIPCHK(ctx, ctx->bl.ensure((size_t)2 * B * sizeof(int32_t)));
+  HIPCHK(ctx, ctx->fold_buf.ensure((size_t)std::max<size_t>(ctx->folds.size(), 1) * sizeof(int32_t)));
+
+  std::vector<int32_t> block_run((size_t)M * ctx->n_blocks);
+  std::vector<double> model_host((size_t)mod_tot);
+  std::vector<DevStream> ds(M);
+  for (int i = 0; i < M; i++) {
+    const StreamPlan &pl = ctx->plans[i];
+    if (!pl.ext_payload && !pl.payload.empty())
+      HIPCHK(ctx, hipMemcpyAsync(ctx->payload.as<uint8_t>() + pay_off[i], pl.payload.data(), pl.payload.size(),
+                                 hipMemcpyHostToDevice, s));
+    if (!pl.runs.empty())
+      HIPCHK(ctx, hipMemcpyAsync(ctx->runs.as<Run>() + run_off[i], pl.runs.data(), pl.runs.size() * sizeof(Run),
+                                 hipMemcpyHostToDevice, s));
+    if (!pl.counted.empty())
+      HIPCHK(ctx, hipMemcpyAsync(ctx->counted.as<CountRange>() + cnt_off[i], pl.counted.data(),
+                                 pl.counted.size() * sizeof(CountRange), hipMemcpyHostToDevice, s));
+    // index of the run holding the first sample of each N_d block
+    int64_t r = 0;
+    for (int64_t k = 0; k < ctx->n_blocks; k++) {
+      const int64_t o = k * nd;
+      while (r + 1 < (int64_t)pl.runs.size() && pl.runs[r + 1].out_start <= o) r++;
+      block_run[blk_off[i] + k] = (int32_t)r;
+    }
+    const ModelCopy &m = ctx->models[i];
+    double *mh = model_host.data() + mod_off[i];
+    std::copy(m.delay.begin(), m.delay.end(), mh);
+    std::copy(m.phase.begin(), m.phase.end(), mh + m.delay.size());
+    std::copy(m.amp.begin(), m.amp.end(), mh + m.delay.size() + m.phase.size());
+    const double *md = ctx->model_buf.as<double>() + mod_off[i];
+
+    const sfxg_station_parameters &st = ctx->st[i];
+    DevStream &d = ds[i];
+    memset(&d, 0, sizeof(d));
+    d.payload = pl.ext_payload ? (const uint8_t *)pl.ext_payload : ctx->payload.as<uint8_t>() + pay_off[i];
+    d.runs = ctx->runs.as<Run>() + run_off[i];
+    d.block_run = ctx->block_run.as<int32_t>() + blk_off[i];
+    d.counted = ctx->counted.as<CountRange>() + cnt_off[i];
+    d.n_runs = (int64_t)pl.runs.size();
+    d.n_counted = (int64_t)pl.counted.size();
+    d.bits = st.bits_per_sample;
+    d.tsys_count0 = pl.tsys_count0;
+    d.tsys_on0 = pl.tsys_on0;
+    d.tsys_half = pl.tsys_half;
+    for (int k = 0; k < 8; k++) d.lut[k] = 0.0f;
+    if (st.bits_per_sample == 2)
+      for (int k = 0; k < 4; k++) d.lut[k] = (float)kLut2[ctx->opt.lut][k];
+    else
+      for (int k = 0; k < 2; k++) d.lut[k] = (float)kLut1[ctx->opt.lut][k];
+    d.model.scan_begin = m.scan_begin;
+    d.model.interval = m.interval;
+    d.model.n = m.n;
+    d.model.has_phase = !m.phase.empty();
+    d.model.has_amp = !m.amp.empty();
+    d.model.delay = md;
+    d.model.phase = m.phase.empty() ? nullptr : md + m.delay.size();
+    d.model.amp = m.amp.empty() ? nullptr : md + m.delay.size() + m.phase.size();
+    const int sb = sb_sign(st.sideband);
+    const int fs = st.sample_rate;
+    d.fs = fs;
+    d.bandwidth = st.bandwidth;
+    d.center_freq = (double)st.channel_freq + sb * st.bandwidth * 0.5 + st.LO_offset;
+    d.mult_factor_phi = -sb * 2.0 * M_PI;
+    d.LO_offset = st.LO_offset;
+    {
+      // start phase of the LO offset (delay_correction.cc:251-255)
+      const double dt = (double)(ctx->p.start_time - ctx->p.experiment_start) / kClockRate;
+      const double LO = st.LO_offset;
+      double sp = LO * (dt - trunc(dt)) + (LO - trunc(LO)) * trunc(dt);
+      d.start_phase = sp - floor(sp);
+    }
+    d.start_ticks = ctx->p.start_time;
+    d.tps = (int64_t)llround(kClockRate / fs);
+    d.fl = (int64_t)llround((double)nd / (fs / 1000000) * (kClockRate / 1000000));
+    d.fl_half = (int64_t)llround(d.fl / 2.0);
+    d.fl_sec = d.fl / kClockRate;
+    d.oversamp = (int)round(fs / (2.0 * st.bandwidth));
+    d.flip = ctx->p.sideband != st.sideband;
+    d.time_series = ctx->time_series.as<float>() + (int64_t)i * ctx->n_samples;
+    d.spectra = ctx->spectra.as<float2>() + (int64_t)i * ctx->nseg * ctx->S;
+    d.samples = nullptr;
+    d.hist = ctx->hist.as<int32_t>() + 512 * i;
+  }
+  HIPCHK(ctx, hipMemcpyAsync(ctx->block_run.p, block_run.data(), block_run.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice, s));
+  if (mod_tot)
+    HIPCHK(ctx, hipMemcpyAsync(ctx->model_buf.p, model_host.data(), model_host.size() * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->dstreams.p, ds.data(), ds.size() * sizeof(DevStream), hipMemcpyHostToDevice, s));
+  std::vector<int32_t> blv(2 * B);
+  std::copy(ctx->bl_first.begin(), ctx->bl_first.end(), blv.begin());
+  std::copy(ctx->bl_second.begin(), ctx->bl_second.end(), blv.begin() + B);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->bl.p, blv.data(), blv.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->flags.p, ctx->flag_factor.data(), B * sizeof(double), hipMemcpyHostToDevice, s));
+  if (!ctx->folds.empty())
+    HIPCHK(ctx, hipMemcpyAsync(ctx->fold_buf.p, ctx->folds.data(), ctx->folds.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemsetAsync(ctx->hist.p, 0, (size_t)M * 512 * sizeof(int32_t), s));
+  HIPCHK(ctx, hipMemsetAsync(ctx->vis.p, 0, vis_elems * sizeof(float2), s));
+
+  DevJob job;
+  job.n_streams = M;
+  job.nd = nd;
+  job.nc = nc;
+  job.window = ctx->p.window;
+  job.n_blocks = ctx->n_blocks;
+  job.n_samples = ctx->n_samples;
+  job.n_segments = ctx->nseg;
+  job.spectra_stride = ctx->S;
+  job.seg_step = nc;  // fft_rot_size / 2
+
+  const DevStream *dstr = ctx->dstreams.as<DevStream>();
+  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_STATS], s));
+  HIPCHK(ctx, launch_stats(dstr, M, max_counted, s));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_DELAY], s));
+  HIPCHK(ctx, launch_delay(dstr, job, ctx->tw_nd.as<float2>(), s));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_CHANNELISE], s));
+  HIPCHK(ctx, launch_channelise(dstr, job, ctx->window.as<float>(), ctx->tw_n.as<float2>(), ctx->tw_2n.as<float2>(), s));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_XSTEP], s));
+  const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
+  if (!ctx->pulsar) {
+    HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S, nc + 1, ctx->nseg, blf, bls, B,
+                             ctx->fold_buf.as<int32_t>(), (int)ctx->folds.size(), ctx->vis.as<float2>(), s));
+  } else {
+    // Correlation_core_pulsar::set_parameters (correlation_core_pulsar.cc:46-93)
+    const int64_t tpd = (int64_t)86400 * (int64_t)kClockRate;
+    const int64_t t = ctx->p.start_time;
+    const double start_mjd = 51544 + (double)(t / tpd) + (t % tpd) / kClockRate / 86400;
+    const double us_per_day = (double)((int64_t)24 * 60 * 60 * 1000000);
+    const double fft_duration = ((double)nc * 1000000) / ctx->p.sample_rate;
+    const sfxg_pulsar &ps = ctx->psr;
+    double DT;
+    if (ctx->p.window == SFXG_WINDOW_NONE) DT = ((start_mjd - ps.tmid) + fft_duration / (2 * us_per_day)) * 1440;
+    else DT = ((start_mjd - ps.tmid) + fft_duration / us_per_day) * 1440;
+    const int N = ps.n_coef;
+    double start_phase = 0;
+    for (int i = N - 1; i > 0; i--) start_phase = (start_phase + ps.coef[i]) * DT;
+    start_phase += (ps.ref_phase - floor(ps.ref_phase)) + DT * 60 * ps.ref_freq + ps.coef[0];
+    const int sb = sb_sign(ctx->p.sideband);
+    const double base_freq = ctx->p.channel_freq * 1e-6;
+    const double dfreq = sb * ctx->p.sample_rate * 1e-6 / (2 * nc);
+    const double inv_freq_obs2 = 1 / (ps.obs_freq * ps.obs_freq);
+    double freq = 0;
+    for (int i = N - 1; i > 1; i--) freq = (freq + i * ps.coef[i]) * DT;
+    freq = (freq + ps.coef[1]) / 60 + ps.ref_freq;
+    std::vector<double> offs(nc + 1);
+    for (int i = 0; i < nc + 1; i++)
+      offs[i] = 4148.808 * ps.DM * (1 / pow(base_freq + i * dfreq, 2) - inv_freq_obs2) * freq;
+    HIPCHK(ctx, ctx->offsets.ensure(offs.size() * sizeof(double)));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->offsets.p, offs.data(), offs.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S, nc + 1, ctx->nseg, blf,
+                                    bls, B, ctx->offsets.as<double>(), start_phase, ps.coef[1] + 60 * ps.ref_freq,
+                                    fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins, ctx->vis.as<float2>(), s));
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_NORMALISE], s));
+  HIPCHK(ctx, launch_auto_norms(ctx->vis.as<float2>(), M, nc + 1, nc, ctx->nbins, B, ctx->norms.as<double>(), s));
+  HIPCHK(ctx, launch_normalise(ctx->vis.as<float2>(), ctx->norms.as<double>(), blf, bls, ctx->flags.as<double>(), M,
+                               B, nc + 1, ctx->nbins, ctx->norm_out.as<float2>(), s));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_N_KERNELS], s));
+  HIPCHK(ctx, hipEventRecord(ctx->done, s));
+  // host staging vectors above are released on return: wait for the copies
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  ctx->enqueued = true;
+  return SFXG_OK;
+}
+
+int sfxg_synchronize(sfxg_ctx *ctx) {
+  if (!ctx) return SFXG_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return SFXG_OK;
+}
+
+static size_t output_size(const sfxg_ctx *ctx) {
+  std::set<int> stations;
+  for (const auto &s : ctx->st) stations.insert(s.station_number);
+  const size_t one = 4 + 16 + 32 * stations.size() + 24 * (size_t)ctx->M +
+                     (size_t)ctx->B * (8 + 8 * (size_t)(ctx->nchan + 1));
+  return one * ctx->nbins;
+}
+
+int sfxg_output_size(sfxg_ctx *ctx, size_t *bytes) {
+  if (!ctx || !bytes) return SFXG_ERR_ARG;
+  if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
+  *bytes = output_size(ctx);
+  return SFXG_OK;
+}
+
+static int stats_of(sfxg_ctx *ctx, std::vector<int32_t> &levels, std::vector<int32_t> &tsys) {
+  std::vector<int32_t> hist((size_t)ctx->M * 512);
+  HIPCHK(ctx, hipMemcpy(hist.data(), ctx->hist.p, hist.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  levels.assign(5 * ctx->M, 0);
+  tsys.assign(4 * ctx->M, 0);
+  for (int i = 0; i < ctx->M; i++)
+    bit_statistics(ctx->st[i].bits_per_sample, ctx->st[i].sample_rate, ctx->p.sample_rate, &hist[512 * i],
+                   &hist[512 * i + 256], ctx->plans[i].n_invalid, &levels[5 * i], &tsys[4 * i]);
+  return SFXG_OK;
+}
+
+int sfxg_get_statistics(sfxg_ctx *ctx, int32_t stream_idx, int32_t levels[5], int32_t tsys[4]) {
+  if (!ctx || !levels || !tsys) return SFXG_ERR_ARG;
+  if (!ctx->enqueued) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
+  if (stream_idx < 0 || stream_idx >= ctx->M) return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int32_t> lv, ts;
+  int rc = stats_of(ctx, lv, ts);
+  if (rc) return rc;
+  std::copy(&lv[5 * stream_idx], &lv[5 * stream_idx + 5], levels);
+  std::copy(&ts[4 * stream_idx], &ts[4 * stream_idx + 4], tsys);
+  return SFXG_OK;
+}
+
+int sfxg_number_of_baselines(sfxg_ctx *ctx, int32_t *n) {
+  if (!ctx || !n) return SFXG_ERR_ARG;
+  if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
+  *n = ctx->B;
+  return SFXG_OK;
+}
+
+// integration_write (correlation_core.cc:316-477) with fft_size == number_channels
+int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) {
+  if (!ctx || !dst) return SFXG_ERR_ARG;
+  if (!ctx->enqueued) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
+  const size_t need = output_size(ctx);
+  if (cap < need) return ctx->fail(SFXG_ERR_ARG, "output buffer holds %zu bytes, %zu needed", cap, need);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const int M = ctx->M, B = ctx->B, nch = ctx->nchan + 1;
+  std::vector<float2> vis((size_t)ctx->nbins * B * nch);
+  HIPCHK(ctx, hipMemcpy(vis.data(), ctx->norm_out.p, vis.size() * sizeof(float2), hipMemcpyDeviceToHost));
+  std::vector<int32_t> lv, ts;
+  int rc = stats_of(ctx, lv, ts);
+  if (rc) return rc;
+  // unique stations in stream order and the first stream of each
+  std::vector<int> stations, first_stream;
+  for (int i = 0; i < M; i++) {
+    if (std::find(stations.begin(), stations.end(), ctx->st[i].station_number) == stations.end()) {
+      stations.push_back(ctx->st[i].station_number);
+      first_stream.push_back(i);
+    }
+  }
+  uint8_t *o = dst;
+  auto put = [&o](const void *src, size_t n) {
+    memcpy(o, src, n);
+    o += n;
+  };
+  for (int bin = 0; bin < ctx->nbins; bin++) {
+    int32_t sourcenr = ctx->pulsar ? bin : (ctx->p.pulsar_binning ? 1 : 0);
+    put(&sourcenr, 4);
+    const int32_t ts_hdr[4] = {ctx->p.integration_nr, B, (int32_t)stations.size(), M};
+    put(ts_hdr, 16);
+    for (size_t k = 0; k < stations.size(); k++) {
+      const int32_t h[2] = {stations[k], 0};
+      put(h, 8);
+      put(&ctx->uvw[3 * first_stream[k]], 24);
+    }
+    for (int i = 0; i < M; i++) {
+      const sfxg_station_parameters &st = ctx->st[i];
+      const uint8_t h[4] = {(uint8_t)st.station_number, (uint8_t)ctx->p.frequency_nr,
+                            (uint8_t)(ctx->p.sideband == 'L' ? 0 : 1), (uint8_t)(st.polarisation == 'R' ? 0 : 1)};
+      put(h, 4);
+      int32_t l5[5];
+      const int32_t *L = &lv[5 * i];
+      if (st.bits_per_sample == 2) {
+        l5[0] = L[0]; l5[1] = L[1]; l5[2] = L[2]; l5[3] = L[3];
+      } else {
+        l5[0] = 0; l5[1] = L[0]; l5[2] = L[1]; l5[3] = 0;
+      }
+      l5[4] = L[4];
+      put(l5, 20);
+    }
+    for (int b = 0; b < B; b++) {
+      const int a = ctx->bl_first[b], c = ctx->bl_second[b];
+      int64_t w;
+      if (a == c) w = std::max<int64_t>(ctx->total_samples - lv[5 * a + 4], 0);
+      else w = std::max<int64_t>(ctx->total_samples - lv[5 * a + 4] - ctx->n_flagged_first[b], 0);
+      const int32_t w32 = (int32_t)w;
+      put(&w32, 4);
+      uint8_t h[4];
+      h[0] = (uint8_t)ctx->st[a].station_number;
+      h[1] = (uint8_t)ctx->st[c].station_number;
+      const int p1 = ctx->st[a].polarisation == 'R' ? 0 : 1, p2 = ctx->st[c].polarisation == 'R' ? 0 : 1;
+      const int sbu = ctx->p.sideband == 'U' ? 1 : 0;
+      h[2] = (uint8_t)((p1 & 1) | ((p2 & 1) << 1) | (sbu << 2) | ((ctx->p.frequency_nr & 31) << 3));
+      h[3] = ' ';
+      put(h, 4);
+      put(&vis[((size_t)bin * B + b) * nch], sizeof(float2) * nch);
+    }
+  }
+  if (written) *written = (size_t)(o - dst);
+  return SFXG_OK;
+}
+
+int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst, size_t cap, size_t *written) {
+  if (!ctx || !dst) return SFXG_ERR_ARG;
+  if (!ctx->enqueued) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const bool per_stream = stage == SFXG_STAGE_SAMPLES || stage == SFXG_STAGE_TIME_SERIES || stage == SFXG_STAGE_SPECTRA;
+  if (per_stream && (stream_idx < 0 || stream_idx >= ctx->M))
+    return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
+  size_t need = 0;
+  switch (stage) {
+    case SFXG_STAGE_SAMPLES: {
+      need = (size_t)ctx->n_samples * sizeof(float);
+      if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
+      HIPCHK(ctx, ctx->samples.ensure(need));
+      DevStream d;
+      HIPCHK(ctx, hipMemcpy(&d, ctx->dstreams.as<DevStream>() + stream_idx, sizeof(d), hipMemcpyDeviceToHost));
+      d.samples = ctx->samples.as<float>();
+      DBuf one;
+      HIPCHK(ctx, one.ensure(sizeof(d)));
+      HIPCHK(ctx, hipMemcpy(one.p, &d, sizeof(d), hipMemcpyHostToDevice));
+      hipError_t e = launch_unpack(one.as<DevStream>(), 1, ctx->n_samples, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e == hipSuccess) e = hipMemcpy(dst, ctx->samples.p, need, hipMemcpyDeviceToHost);
+      one.release();
+      HIPCHK(ctx, e);
+      break;
+    }
+    case SFXG_STAGE_TIME_SERIES:
+      need = (size_t)ctx->n_samples * sizeof(float);
+      if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
+      HIPCHK(ctx, hipMemcpy(dst, ctx->time_series.as<float>() + (int64_t)stream_idx * ctx->n_samples, need,
+                            hipMemcpyDeviceToHost));
+      break;
+    case SFXG_STAGE_SPECTRA:
+      need = (size_t)ctx->nseg * (ctx->nc + 1) * sizeof(float2);
+      if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
+      HIPCHK(ctx, hipMemcpy2D(dst, (ctx->nc + 1) * sizeof(float2),
+                              ctx->spectra.as<float2>() + (int64_t)stream_idx * ctx->nseg * ctx->S,
+                              ctx->S * sizeof(float2), (ctx->nc + 1) * sizeof(float2), ctx->nseg,
+                              hipMemcpyDeviceToHost));
+      break;
+    case SFXG_STAGE_ACCUMULATED:
+    case SFXG_STAGE_NORMALISED:
+      need = (size_t)ctx->nbins * ctx->B * (ctx->nc + 1) * sizeof(float2);
+      if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
+      HIPCHK(ctx, hipMemcpy(dst, stage == SFXG_STAGE_ACCUMULATED ? ctx->vis.p : ctx->norm_out.p, need,
+                            hipMemcpyDeviceToHost));
+      break;
+    default:
+      return ctx->fail(SFXG_ERR_ARG, "unknown stage %d", stage);
+  }
+  if (written) *written = need;
+  return SFXG_OK;
+}
+
+int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n) {
+  if (!ctx || !ms) return SFXG_ERR_ARG;
+  if (!ctx->enqueued) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < SFXG_N_KERNELS && k < n; k++) {
+    float t = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[k], ctx->ev[k + 1]));
+    ms[k] = t;
+  }
+  return SFXG_OK;
+}
+
+int sfxg_synthesize_raw(int device, void *device_dst, size_t n_bytes, int32_t bits_per_sample, int64_t first_sample,
+                        int64_t delay_samples, uint64_t common_seed, uint64_t stream_seed, float rho,
+                        float threshold) {
+  if (!device_dst || (bits_per_sample != 1 && bits_per_sample != 2) || rho < 0 || rho > 1) return SFXG_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return SFXG_ERR_DEVICE;
+  if (launch_synthesize(device_dst, n_bytes, bits_per_sample, first_sample, delay_samples, common_seed, stream_seed,
+                        rho, threshold, nullptr) != hipSuccess)
+    return SFXG_ERR_DEVICE;
+  if (hipDeviceSynchronize() != hipSuccess) return SFXG_ERR_DEVICE;
+  return SFXG_OK;
+}
+
+}  // extern "C"

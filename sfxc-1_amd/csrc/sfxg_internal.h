@@ -1,0 +1,170 @@
+// Internal types shared by the host side (sfxg_host.cpp, sfxg_capi.cpp) and
+// the device kernels (sfxg_kernels.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+namespace sfxg {
+
+constexpr double kClockRate = 512000000.0;  // MAX_SAMPLE_RATE, sfxc/include/utils.h:36
+constexpr int kHeaderData = 0, kHeaderInvalid = 1, kHeaderDelay = 2, kHeaderEndstream = 3;
+constexpr int kCorrelatorBufferSize = 8192;  // sfxc/include/utils.h:50
+
+// One run of consecutive output samples of a stream. src >= 0: output sample
+// out_start + k is payload sample src + k; src < 0: the run is zero-filled
+// (an inserted sample or invalid data).
+struct Run {
+  int64_t out_start;
+  int64_t src;
+  int64_t count;
+};
+
+// Payload bytes the reference's statistics count, in counting order:
+// bytes [byte_start, byte_start + n_bytes) carry ordinals ordinal .. ordinal + n_bytes - 1.
+struct CountRange {
+  int64_t byte_start;
+  int64_t n_bytes;
+  int64_t ordinal;
+};
+
+// Bit2float_worker parameters for one slice (bit2float_worker.cc:235-313).
+struct B2fParams {
+  int bits_per_sample = 2;
+  int sample_rate = 0;
+  int base_sample_rate = 0;
+  int tsys_freq = 80;
+  int fft_size = 256;  // N_d
+  int nfft_max = 32;
+  int n_ffts_per_integration = 0;
+  int delay_in_samples = 0;
+};
+
+// Result of parsing one stream's HEADER byte stream for one slice.
+struct StreamPlan {
+  std::vector<uint8_t> payload;       // payload bytes (empty when elided)
+  const void *ext_payload = nullptr;  // caller-owned device payload (elided form)
+  int64_t payload_bytes = 0;
+  std::vector<Run> runs;
+  std::vector<CountRange> counted;
+  std::vector<std::pair<int32_t, int32_t>> invalid;  // Invalid{start, n_invalid}
+  int64_t n_invalid = 0;                             // bit_statistics nInvalid
+  std::vector<int32_t> chunks;                       // nfft of every output buffer
+  int tsys_count0 = 0, tsys_on0 = 1, tsys_half = 0;  // tsys toggling in counted bytes
+  bool ready = false;
+};
+
+// Parses a complete HEADER stream (Bit2float_worker::do_task restated at
+// record granularity). Returns 0 or a negative SFXG_ERR_* code.
+int parse_header_stream(const uint8_t *bytes, size_t n, bool elided, const B2fParams &p,
+                        StreamPlan &out, std::string &err);
+
+// Number of correlation segments Delay_correction pushes per bit2float
+// chunk (delay_correction.cc:24-36,58-69).
+std::vector<int32_t> segments_per_chunk(const std::vector<int32_t> &chunks, int fft_size_delaycor,
+                                        int64_t fft_rot_size, bool windowed);
+
+// Segment counts after which Correlation_core folds its accumulators into the
+// phase centre (correlation_core.cc:42-72).
+std::vector<int32_t> fold_points(const std::vector<int32_t> &seg_per_chunk, int n_integration,
+                                 int n_sub_integration);
+
+// find_invalid for one baseline (correlation_core.cc:599-665).
+void find_invalid(const std::vector<std::pair<int32_t, int32_t>> &inv0,
+                  const std::vector<std::pair<int32_t, int32_t>> &inv1, double scale0,
+                  double scale1, int64_t nflagged[2]);
+
+// bit_statistics::get_statistics / get_tsys (bit_statistics.cc:33-91).
+void bit_statistics(int bits, int64_t sample_rate, int64_t base_sample_rate, const int32_t *on,
+                    const int32_t *off, int64_t n_invalid, int32_t levels[5], int32_t tsys[4]);
+
+// ---------------------------------------------------------------------------
+// device-side parameter blocks
+// ---------------------------------------------------------------------------
+struct DevModel {
+  int64_t scan_begin;
+  double interval;
+  int32_t n;
+  int32_t has_phase;
+  int32_t has_amp;
+  int32_t pad;
+  const double *delay;
+  const double *phase;
+  const double *amp;
+};
+
+struct DevStream {
+  const uint8_t *payload;
+  const Run *runs;
+  const int32_t *block_run;  // index of the run holding the first sample of each N_d block
+  const CountRange *counted;
+  int64_t n_runs;
+  int64_t n_counted;
+  int32_t bits;
+  int32_t tsys_count0, tsys_on0, tsys_half;
+  float lut[8];  // decode table (4 entries for 2 bit, 2 for 1 bit)
+  DevModel model;
+  // Delay_correction constants (delay_correction.cc:139-205)
+  double center_freq;
+  double mult_factor_phi;
+  double start_phase;
+  double LO_offset;
+  double fs;         // station sample rate
+  double bandwidth;
+  double fl_sec;
+  int64_t start_ticks;
+  int64_t tps;       // ticks per sample
+  int64_t fl;        // fft_length in ticks
+  int64_t fl_half;   // fft_length / 2 in ticks
+  int32_t oversamp;
+  int32_t flip;      // sideband flip for the channeliser
+  float *time_series;
+  float2 *spectra;   // [n_segments][spectra_stride]
+  float *samples;    // optional unpack output (verification)
+  int32_t *hist;     // [2][256] byte histograms (tsys on, off)
+};
+
+struct DevJob {
+  int32_t n_streams;
+  int32_t nd;               // fft_size_delaycor
+  int32_t nc;               // fft_size_correlation
+  int32_t window;
+  int64_t n_blocks;         // N_d blocks per stream
+  int64_t n_samples;        // n_blocks * nd
+  int64_t n_segments;
+  int64_t spectra_stride;   // complex elements per segment row
+  int64_t seg_step;         // fft_rot_size / 2 real samples between segments
+};
+
+// kernel launchers (sfxg_kernels.hip)
+hipError_t launch_unpack(const DevStream *d_streams, int n_streams, int64_t n_samples, hipStream_t s);
+hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_counted_bytes,
+                        hipStream_t s);
+hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const float2 *tw_nd,
+                        hipStream_t s);
+hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
+                             const float2 *tw_n, const float2 *tw_2n, hipStream_t s);
+hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
+                        int64_t n_segments, const int32_t *bl_first, const int32_t *bl_second,
+                        int n_baselines, const int32_t *folds, int n_folds, float2 *vis, hipStream_t s);
+hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+                               int n_ch, int64_t n_segments, const int32_t *bl_first,
+                               const int32_t *bl_second, int n_baselines, const double *offsets,
+                               double start_phase, double phase_coef, double fft_duration,
+                               double gate_begin, double gate_end, int nbins, float2 *acc,
+                               hipStream_t s);
+hipError_t launch_auto_norms(const float2 *vis, int n_streams, int n_ch, int fft_size, int n_bins,
+                             int n_baselines, double *norms, hipStream_t s);
+hipError_t launch_normalise(const float2 *vis, const double *norms, const int32_t *bl_first,
+                            const int32_t *bl_second, const double *flag_factor, int n_streams,
+                            int n_baselines, int n_ch, int n_bins, float2 *out, hipStream_t s);
+hipError_t launch_synthesize(void *dst, size_t n_bytes, int bits, int64_t first_sample,
+                             int64_t delay_samples, uint64_t common_seed, uint64_t stream_seed,
+                             float rho, float threshold, hipStream_t s);
+
+}  // namespace sfxg

@@ -1,0 +1,677 @@
+// Device kernels of the correlator-node data path for gfx950 (CDNA4).
+//
+//   k_unpack      Bit2float_worker output (bit2float_worker.cc:326-424) for verification
+//   k_stats       byte histograms of bit_statistics (bit_statistics.h:36-41)
+//   k_delay       unpack + integer shift fused with Delay_correction's
+//                 fractional_bit_shift and fringe_stopping (delay_correction.cc:82-184)
+//   k_channelise  window + r2c FFT of 2 N_c points (delay_correction.cc:57-75)
+//   k_xstep       Correlation_core::integration_step + sub_integration folds
+//                 (correlation_core.cc:241-273, 515-559)
+//   k_xstep_pulsar Correlation_core_pulsar::integration_step + dedisperse_buffer
+//                 (correlation_core_pulsar.cc:188-238)
+//   k_auto_norms / k_normalise  integration_normalize (correlation_core.cc:275-314)
+//   k_synthesize  device-side synthetic raw station data
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/sfxg.h"
+#include "sfxg_internal.h"
+
+namespace sfxg {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i (sign < 0) or +i (sign > 0)
+template <int SIGN>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+  return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+template <int SIGN>
+__device__ __forceinline__ float2 twiddle(const float2 *tw, int idx) {
+  float2 w = tw[idx];
+  return SIGN < 0 ? w : make_float2(w.x, -w.y);
+}
+
+// 4-point DFT in place, exp(SIGN 2 pi i jk/4)
+template <int SIGN>
+__device__ __forceinline__ void radix4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
+  const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
+  const float2 s13 = cadd(a1, a3), d13 = mul_i<SIGN>(csub(a1, a3));
+  a0 = cadd(s02, s13);
+  a2 = csub(s02, s13);
+  a1 = cadd(d02, d13);
+  a3 = csub(d02, d13);
+}
+
+// Piecewise-cubic delay model in FP64, same operation order as the host and
+// the reference's spline evaluation; no contraction so the integer delay
+// decisions agree bit for bit with the host.
+__device__ double eval_model(const DevModel &m, const double *coef, int64_t ticks, double dflt) {
+#pragma clang fp contract(off)
+  if (coef == nullptr) return dflt;
+  const double sec = (double)(ticks - m.scan_begin) / kClockRate;
+  int64_t i = (int64_t)floor(sec / m.interval);
+  if (i < 0) i = 0;
+  if (i > m.n - 1) i = m.n - 1;
+  const double u = sec - (double)i * m.interval;
+  const double *c = coef + 4 * i;
+  return c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+}
+
+__device__ __forceinline__ float decode_sample(const DevStream &s, int64_t psamp) {
+  if (s.bits == 2) {
+    const uint32_t b = s.payload[psamp >> 2];
+    return s.lut[(b >> ((psamp & 3) * 2)) & 3];
+  }
+  const uint32_t b = s.payload[psamp >> 3];
+  return s.lut[(b >> (psamp & 7)) & 1];
+}
+
+// Output sample o of a stream; r is a run index hint that only moves forward.
+__device__ __forceinline__ float sample_at(const DevStream &s, int64_t o, int64_t &r) {
+  while (r + 1 < s.n_runs && s.runs[r + 1].out_start <= o) r++;
+  const Run run = s.runs[r];
+  if (run.src < 0) return 0.0f;
+  return decode_sample(s, run.src + (o - run.out_start));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// k_unpack: one thread per output sample (verification path)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_unpack(const DevStream *__restrict__ streams, int64_t n_samples) {
+  const DevStream &s = streams[blockIdx.y];
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_samples || s.samples == nullptr) return;
+  int64_t lo = 0, hi = s.n_runs - 1;
+  while (lo < hi) {  // last run with out_start <= o
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (s.runs[mid].out_start <= o) lo = mid; else hi = mid - 1;
+  }
+  int64_t r = lo;
+  s.samples[o] = sample_at(s, o, r);
+}
+
+// ---------------------------------------------------------------------------
+// k_stats: histograms of the counted payload bytes, split by tsys phase
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ streams) {
+  __shared__ int hist[2][256];
+  const DevStream &s = streams[blockIdx.y];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) (&hist[0][0])[i] = 0;
+  __syncthreads();
+  int64_t total = 0;
+  if (s.n_counted > 0) {
+    const CountRange last = s.counted[s.n_counted - 1];
+    total = last.ordinal + last.n_bytes;
+  }
+  const int64_t per_block = (int64_t)blockDim.x * 16;
+  const int64_t g0 = (int64_t)blockIdx.x * per_block;
+  if (g0 < total) {
+    // range holding the first byte of this block
+    int64_t lo = 0, hi = s.n_counted - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (s.counted[mid].ordinal <= g0) lo = mid; else hi = mid - 1;
+    }
+    for (int k = 0; k < 16; k++) {
+      const int64_t g = g0 + (int64_t)k * blockDim.x + threadIdx.x;
+      if (g >= total) break;
+      int64_t r = lo;
+      while (r + 1 < s.n_counted && s.counted[r + 1].ordinal <= g) r++;
+      const CountRange cr = s.counted[r];
+      const uint8_t b = s.payload[cr.byte_start + (g - cr.ordinal)];
+      int on = 1;
+      if (s.bits == 2) {
+        on = s.tsys_on0;
+        if (s.tsys_count0 > 0 && g >= s.tsys_count0) {
+          if (s.tsys_half > 0) {
+            const int64_t k2 = (g - s.tsys_count0) / s.tsys_half;
+            on = s.tsys_on0 ^ (int)((k2 + 1) & 1);
+          } else {
+            on = !s.tsys_on0;
+          }
+        }
+      }
+      atomicAdd(&hist[on ? 0 : 1][b], 1);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    const int v = (&hist[0][0])[i];
+    if (v) atomicAdd(&s.hist[i], v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_delay: one wavefront per N_d = 256 block. Samples are decoded straight
+// from the packed payload, transformed with a radix-4 Stockham FFT held in
+// registers and LDS, phase-ramped, transformed back and fringe-rotated.
+// ---------------------------------------------------------------------------
+namespace {
+
+// One Stockham radix-4 stage over 256 points for one wavefront: v[r] holds
+// data[lane + 64 r] on entry; on exit buf holds the stage output.
+template <int SIGN>
+__device__ __forceinline__ void stage256(float2 (&v)[4], float2 *buf, const float2 *tw, int lane, int ns) {
+  const int k = lane % ns;
+  const int step = 256 / (ns * 4);
+  v[1] = cmul(v[1], twiddle<SIGN>(tw, k * 1 * step));
+  v[2] = cmul(v[2], twiddle<SIGN>(tw, k * 2 * step));
+  v[3] = cmul(v[3], twiddle<SIGN>(tw, k * 3 * step));
+  radix4<SIGN>(v[0], v[1], v[2], v[3]);
+  const int idx = (lane / ns) * ns * 4 + k;
+  __syncthreads();
+  buf[idx] = v[0];
+  buf[idx + ns] = v[1];
+  buf[idx + 2 * ns] = v[2];
+  buf[idx + 3 * ns] = v[3];
+  __syncthreads();
+}
+
+template <int SIGN>
+__device__ __forceinline__ void fft256(float2 (&v)[4], float2 *buf, const float2 *tw, int lane) {
+  for (int ns = 1; ns < 256; ns *= 4) {
+    stage256<SIGN>(v, buf, tw, lane, ns);
+    for (int r = 0; r < 4; r++) v[r] = buf[lane + 64 * r];
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ streams, DevJob job,
+                                               const float2 *__restrict__ tw) {
+  __shared__ float2 lds[4][256];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const DevStream &s = streams[blockIdx.y];
+  const int64_t blk = (int64_t)blockIdx.x * 4 + wave;
+  const bool active = blk < job.n_blocks;
+  float2 *buf = lds[wave];
+  const int nd = 256;
+
+  float2 v[4];
+  if (active) {
+    const int64_t o0 = blk * nd;
+    int64_t r = s.block_run[blk];
+    for (int q = 0; q < 4; q++) v[q] = make_float2(sample_at(s, o0 + lane + 64 * q, r), 0.0f);
+  } else {
+    for (int q = 0; q < 4; q++) v[q] = make_float2(0.0f, 0.0f);
+  }
+  fft256<-1>(v, buf, tw, lane);
+
+  // block constants in FP64 (delay_correction.cc:40-47, 97-103, 139-160)
+  double c_term = 0.0, l_term = 0.0, phi = 0.0, dphi = 0.0, amp = 1.0;
+  {
+#pragma clang fp contract(off)
+    const int64_t current = s.start_ticks + blk * nd * s.tps;
+    const double delay = eval_model(s.model, s.model.delay, current + s.fl_half, 0.0);
+    const double dis = delay * s.fs;
+    const int integer_delay = (int)floor(dis + .5);
+    const double fractional_delay = dis - integer_delay;
+    const double dfr = s.fs / nd;
+    const double tmp1 = -2.0 * kPi * fractional_delay / s.fs;
+    const double tmp2 = kPi * (integer_delay & (4 * s.oversamp - 1)) / (2 * s.oversamp);
+    c_term = tmp2 - tmp1 * 0.5 * s.bandwidth;
+    l_term = tmp1 * dfr;
+    const double lo_phase = s.start_phase + s.LO_offset * ((double)(current - s.start_ticks) / kClockRate);
+    const double *ph = s.model.has_phase ? s.model.phase : nullptr;
+    double p0 = s.center_freq * eval_model(s.model, s.model.delay, current, 0.0) + lo_phase +
+                eval_model(s.model, ph, current, 0.0) / (2 * kPi);
+    const double floor_phi = floor(p0);
+    phi = s.mult_factor_phi * (p0 - floor_phi);
+    double p1 = s.center_freq * eval_model(s.model, s.model.delay, current + s.fl, 0.0) + lo_phase +
+                s.fl_sec * s.LO_offset + eval_model(s.model, ph, current + s.fl, 0.0) / (2 * kPi);
+    p1 = s.mult_factor_phi * (p1 - floor_phi);
+    dphi = (p1 - phi) / nd;
+    if (s.model.has_amp) amp = eval_model(s.model, s.model.amp, current + s.fl_half, 1.0);
+  }
+
+  // halve DC and Nyquist, drop negative frequencies, apply the ramp
+  for (int q = 0; q < 4; q++) {
+    const int k = lane + 64 * q;
+    if (k > nd / 2) {
+      v[q] = make_float2(0.0f, 0.0f);
+    } else {
+      if (k == 0 || k == nd / 2) v[q] = make_float2(v[q].x * 0.5f, v[q].y * 0.5f);
+      const float a = (float)(c_term + k * l_term);
+      float sn, cs;
+      sincosf(a, &sn, &cs);
+      v[q] = cmul(make_float2(cs, -sn), v[q]);
+    }
+  }
+  fft256<+1>(v, buf, tw, lane);
+
+  if (active) {
+    float *out = s.time_series + blk * nd;
+    const float famp = (float)amp;
+    for (int q = 0; q < 4; q++) {
+      const int i = lane + 64 * q;
+      const float a = (float)(phi + i * dphi);
+      float sn, cs;
+      sincosf(a, &sn, &cs);
+      out[i] = famp * (v[q].x * cs + v[q].y * sn);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_channelise: one workgroup per (segment, stream). Real 2N input is packed
+// as N complex points, transformed by an LDS-resident radix-4 (+ radix-2)
+// Stockham FFT and split into the N + 1 bins of the real transform.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int N, int T, int SIGN>
+__device__ __forceinline__ void lds_fft(float2 *lds, const float2 *tw) {
+  constexpr int NB4 = (N / 4 + T - 1) / T;
+  int ns = 1;
+  for (; ns * 4 <= N; ns *= 4) {
+    float2 v[NB4][4];
+#pragma unroll
+    for (int b = 0; b < NB4; b++) {
+      const int j = threadIdx.x + b * T;
+      if (j < N / 4) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[b][r] = lds[j + r * (N / 4)];
+      }
+    }
+    __syncthreads();
+    const int step = N / (ns * 4);
+#pragma unroll
+    for (int b = 0; b < NB4; b++) {
+      const int j = threadIdx.x + b * T;
+      if (j < N / 4) {
+        const int k = j % ns;
+        v[b][1] = cmul(v[b][1], twiddle<SIGN>(tw, k * step));
+        v[b][2] = cmul(v[b][2], twiddle<SIGN>(tw, 2 * k * step));
+        v[b][3] = cmul(v[b][3], twiddle<SIGN>(tw, 3 * k * step));
+        radix4<SIGN>(v[b][0], v[b][1], v[b][2], v[b][3]);
+        const int idx = (j / ns) * ns * 4 + k;
+        lds[idx] = v[b][0];
+        lds[idx + ns] = v[b][1];
+        lds[idx + 2 * ns] = v[b][2];
+        lds[idx + 3 * ns] = v[b][3];
+      }
+    }
+    __syncthreads();
+  }
+  if (ns < N) {  // final radix-2 stage
+    constexpr int NB2 = (N / 2 + T - 1) / T;
+    float2 v[NB2][2];
+#pragma unroll
+    for (int b = 0; b < NB2; b++) {
+      const int j = threadIdx.x + b * T;
+      if (j < N / 2) {
+        v[b][0] = lds[j];
+        v[b][1] = lds[j + N / 2];
+      }
+    }
+    __syncthreads();
+    const int step = N / (ns * 2);
+#pragma unroll
+    for (int b = 0; b < NB2; b++) {
+      const int j = threadIdx.x + b * T;
+      if (j < N / 2) {
+        const int k = j % ns;
+        const float2 w = twiddle<SIGN>(tw, k * step);
+        const float2 t = cmul(v[b][1], w);
+        const int idx = (j / ns) * ns * 2 + k;
+        lds[idx] = cadd(v[b][0], t);
+        lds[idx + ns] = csub(v[b][0], t);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+template <int N, int T>
+__global__ __launch_bounds__(T) void k_channelise(const DevStream *__restrict__ streams, DevJob job,
+                                                  const float *__restrict__ window,
+                                                  const float2 *__restrict__ tw_n,
+                                                  const float2 *__restrict__ tw_2n) {
+  __shared__ float2 lds[N];
+  const DevStream &s = streams[blockIdx.y];
+  const int64_t seg = blockIdx.x;
+  const float2 *x = reinterpret_cast<const float2 *>(s.time_series + seg * job.seg_step);
+  const float2 *w = reinterpret_cast<const float2 *>(window);
+  const bool none = job.window == SFXG_WINDOW_NONE;
+  for (int m = threadIdx.x; m < N; m += T) {
+    float2 z;
+    if (none && m >= N / 2) {
+      z = make_float2(0.0f, 0.0f);
+    } else {
+      const float2 xv = x[m], wv = w[m];
+      z = make_float2(xv.x * wv.x, xv.y * wv.y);
+    }
+    if (s.flip) z.y = -z.y;
+    lds[m] = z;
+  }
+  __syncthreads();
+  lds_fft<N, T, -1>(lds, tw_n);
+  float2 *out = s.spectra + seg * job.spectra_stride;
+  for (int k = threadIdx.x; k <= N; k += T) {
+    const float2 zk = lds[k & (N - 1)], zn = lds[(N - k) & (N - 1)];
+    const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+    const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+    out[k] = cadd(e, cmul(tw_2n[k], o));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_xstep: plain FP32 X-step. Lane = channel, each thread owns XB baselines.
+// Products and sums follow the reference's operation order without
+// contraction (std::complex<float> multiply, then +=).
+// ---------------------------------------------------------------------------
+constexpr int XB = 8;  // baselines per thread
+constexpr int XWAVES = 4;
+
+__global__ __launch_bounds__(64 * XWAVES) void k_xstep(const float2 *__restrict__ spectra,
+                                                       int64_t stream_stride, int64_t seg_stride, int n_ch,
+                                                       int64_t n_segments, const int32_t *__restrict__ bl_first,
+                                                       const int32_t *__restrict__ bl_second, int n_baselines,
+                                                       const int32_t *__restrict__ folds, int n_folds,
+                                                       float2 *__restrict__ vis) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ch = blockIdx.x * 64 + lane;
+  const int b0 = (blockIdx.y * XWAVES + wave) * XB;
+  if (ch >= n_ch || b0 >= n_baselines) return;
+  const float2 *pa[XB];
+  const float2 *pb[XB];
+  float2 acc[XB];
+  int nb = n_baselines - b0 < XB ? n_baselines - b0 : XB;
+#pragma unroll
+  for (int q = 0; q < XB; q++) {
+    const int b = b0 + (q < nb ? q : 0);
+    pa[q] = spectra + bl_first[b] * stream_stride + ch;
+    pb[q] = spectra + bl_second[b] * stream_stride + ch;
+    acc[q] = make_float2(0.0f, 0.0f);
+  }
+  int fi = 0;
+  for (int64_t sgi = 0; sgi < n_segments; sgi++) {
+    const int64_t off = sgi * seg_stride;
+#pragma unroll
+    for (int q = 0; q < XB; q++) {
+      const float2 x = pa[q][off], y = pb[q][off];
+      const float pr = x.x * y.x + x.y * y.y;
+      const float pi = x.y * y.x - x.x * y.y;
+      acc[q].x = acc[q].x + pr;
+      acc[q].y = acc[q].y + pi;
+    }
+    if (fi < n_folds && sgi + 1 == folds[fi]) {
+#pragma unroll
+      for (int q = 0; q < XB; q++) {
+        if (q < nb) {
+          float2 *o = vis + (int64_t)(b0 + q) * n_ch + ch;
+          float2 t = *o;
+          t.x = t.x + acc[q].x;
+          t.y = t.y + acc[q].y;
+          *o = t;
+        }
+        acc[q] = make_float2(0.0f, 0.0f);
+      }
+      fi++;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_xstep_pulsar: per segment, every channel's product goes to the bin its
+// pulse phase selects. Runs of equal bins accumulate in registers; at a bin
+// change the running sums are written back and the new bin's sums loaded,
+// which keeps the reference's per-segment addition order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
+    int64_t n_segments, const int32_t *__restrict__ bl_first, const int32_t *__restrict__ bl_second,
+    int n_baselines, const double *__restrict__ offsets, double start_phase, double phase_coef,
+    double fft_duration, double gate_begin, double gate_end, int nbins, float2 *__restrict__ acc_out) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ch = blockIdx.x * 64 + lane;
+  const int b0 = (blockIdx.y * XWAVES + wave) * XB;
+  if (ch >= n_ch || b0 >= n_baselines) return;
+  const float2 *pa[XB];
+  const float2 *pb[XB];
+  float2 acc[XB];
+  int nb = n_baselines - b0 < XB ? n_baselines - b0 : XB;
+#pragma unroll
+  for (int q = 0; q < XB; q++) {
+    const int b = b0 + (q < nb ? q : 0);
+    pa[q] = spectra + bl_first[b] * stream_stride + ch;
+    pb[q] = spectra + bl_second[b] * stream_stride + ch;
+    acc[q] = make_float2(0.0f, 0.0f);
+  }
+  const double off_ch = offsets[ch];
+  const double len = gate_end - gate_begin;
+  const int64_t us_per_day = (int64_t)24 * 60 * 60 * 1000000;
+  const int64_t bin_stride = (int64_t)n_baselines * n_ch;
+  int cur = -1;
+  for (int64_t sgi = 0; sgi < n_segments; sgi++) {
+    const double obs = start_phase + phase_coef * fft_duration * (double)(int)sgi * 1440 / (double)us_per_day;
+    double phase = obs - off_ch;
+    phase = phase - floor(phase);
+    int bin;
+    if (phase >= gate_begin) {
+      bin = phase < gate_end ? (int)((phase - gate_begin) * (nbins - 1) / len) + 1 : 0;
+    } else if (phase + 1 < gate_end) {
+      bin = (int)((phase + 1 - gate_begin) * (nbins - 1) / len) + 1;
+    } else {
+      bin = 0;
+    }
+    if (bin != cur) {
+#pragma unroll
+      for (int q = 0; q < XB; q++) {
+        if (q < nb) {
+          if (cur >= 0) acc_out[cur * bin_stride + (int64_t)(b0 + q) * n_ch + ch] = acc[q];
+          acc[q] = acc_out[bin * bin_stride + (int64_t)(b0 + q) * n_ch + ch];
+        }
+      }
+      cur = bin;
+    }
+    const int64_t off = sgi * seg_stride;
+#pragma unroll
+    for (int q = 0; q < XB; q++) {
+      const float2 x = pa[q][off], y = pb[q][off];
+      const float pr = x.x * y.x + x.y * y.y;
+      const float pi = x.y * y.x - x.x * y.y;
+      acc[q].x = acc[q].x + pr;
+      acc[q].y = acc[q].y + pi;
+    }
+  }
+  if (cur >= 0) {
+#pragma unroll
+    for (int q = 0; q < XB; q++)
+      if (q < nb) acc_out[cur * bin_stride + (int64_t)(b0 + q) * n_ch + ch] = acc[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// normalisation (correlation_core.cc:275-314)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_auto_norms(const float2 *__restrict__ vis, int n_ch, int fft_size,
+                                                    int n_baselines, double *__restrict__ norms, int n_streams) {
+  __shared__ double part[256];
+  const int i = blockIdx.x, bin = blockIdx.y;
+  const float2 *row = vis + ((int64_t)bin * n_baselines + i) * n_ch;
+  double sum = 0.0;
+  for (int j = threadIdx.x; j < n_ch; j += blockDim.x) sum += (double)row[j].x;
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double n = part[0] / fft_size;
+    if (n < 1) n = 1;
+    norms[bin * n_streams + i] = n;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_normalise(const float2 *__restrict__ vis, const double *__restrict__ norms,
+                                                   const int32_t *__restrict__ bl_first,
+                                                   const int32_t *__restrict__ bl_second,
+                                                   const double *__restrict__ flag_factor, int n_streams,
+                                                   int n_baselines, int n_ch, float2 *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int bin = blockIdx.y;
+  if (idx >= (int64_t)n_baselines * n_ch) return;
+  const int b = (int)(idx / n_ch);
+  const int64_t k = (int64_t)bin * n_baselines * n_ch + idx;
+  const float2 v = vis[k];
+  const double *nm = norms + (int64_t)bin * n_streams;
+  if (b < n_streams) {
+    out[k] = make_float2((float)((double)v.x / nm[b]), 0.0f);
+  } else {
+    const float norm = (float)sqrt(flag_factor[b] * nm[bl_first[b]] * nm[bl_second[b]]);
+    out[k] = make_float2(v.x / norm, v.y / norm);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// synthetic raw data: counter-based Gaussian samples (SplitMix64 + Box-Muller)
+// ---------------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ float gauss(uint64_t seed, int64_t k) {
+  const uint64_t h = splitmix64(seed ^ splitmix64((uint64_t)k));
+  const float u1 = ((float)(uint32_t)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);
+  const float u2 = (float)(uint32_t)(h & 0xFFFFFFu) * (1.0f / 16777216.0f);
+  return sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_synthesize(uint8_t *__restrict__ dst, int64_t n_bytes, int bits,
+                                                    int64_t first_sample, int64_t delay_samples,
+                                                    uint64_t common_seed, uint64_t stream_seed, float rho,
+                                                    float threshold) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_bytes) return;
+  const int spb = 8 / bits;
+  const float a = sqrtf(rho), b = sqrtf(1.0f - rho);
+  uint32_t byte = 0;
+  for (int t = 0; t < spb; t++) {
+    const int64_t k = first_sample + i * spb + t;
+    const float x = a * gauss(common_seed, k - delay_samples) + b * gauss(stream_seed, k);
+    uint32_t c;
+    if (bits == 2) c = x < -threshold ? 0u : (x < 0.0f ? 1u : (x < threshold ? 2u : 3u));
+    else c = x >= 0.0f ? 1u : 0u;
+    byte |= c << (bits * t);
+  }
+  dst[i] = (uint8_t)byte;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_unpack(const DevStream *d_streams, int n_streams, int64_t n_samples, hipStream_t s) {
+  dim3 grid((unsigned)((n_samples + 255) / 256), n_streams);
+  hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, s, d_streams, n_samples);
+  return hipGetLastError();
+}
+
+hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_counted_bytes, hipStream_t s) {
+  const int64_t per_block = 256 * 16;
+  dim3 grid((unsigned)((max_counted_bytes + per_block - 1) / per_block), n_streams);
+  if (grid.x == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stats, grid, dim3(256), 0, s, d_streams);
+  return hipGetLastError();
+}
+
+hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const float2 *tw_nd, hipStream_t s) {
+  if (job.nd != 256) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((job.n_blocks + 3) / 4), job.n_streams);
+  hipLaunchKernelGGL(k_delay, grid, dim3(256), 0, s, d_streams, job, tw_nd);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, const float *window,
+                                const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+  constexpr int T = (N / 4) < 512 ? (N / 4) : 512;
+  dim3 grid((unsigned)job.n_segments, job.n_streams);
+  hipLaunchKernelGGL((k_channelise<N, T>), grid, dim3(T), 0, s, d_streams, job, window, tw_n, tw_2n);
+  return hipGetLastError();
+}
+
+hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
+                             const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+  if (job.n_segments <= 0) return hipSuccess;
+  switch (job.nc) {
+    case 256: return launch_chan_n<256>(d_streams, job, window, tw_n, tw_2n, s);
+    case 512: return launch_chan_n<512>(d_streams, job, window, tw_n, tw_2n, s);
+    case 1024: return launch_chan_n<1024>(d_streams, job, window, tw_n, tw_2n, s);
+    case 2048: return launch_chan_n<2048>(d_streams, job, window, tw_n, tw_2n, s);
+    case 4096: return launch_chan_n<4096>(d_streams, job, window, tw_n, tw_2n, s);
+    case 8192: return launch_chan_n<8192>(d_streams, job, window, tw_n, tw_2n, s);
+    case 16384: return launch_chan_n<16384>(d_streams, job, window, tw_n, tw_2n, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
+                        int64_t n_segments, const int32_t *bl_first, const int32_t *bl_second, int n_baselines,
+                        const int32_t *folds, int n_folds, float2 *vis, hipStream_t s) {
+  dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)((n_baselines + XB * XWAVES - 1) / (XB * XWAVES)));
+  hipLaunchKernelGGL(k_xstep, grid, dim3(64 * XWAVES), 0, s, spectra, stream_stride, seg_stride, n_ch,
+                     n_segments, bl_first, bl_second, n_baselines, folds, n_folds, vis);
+  return hipGetLastError();
+}
+
+hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
+                               int64_t n_segments, const int32_t *bl_first, const int32_t *bl_second,
+                               int n_baselines, const double *offsets, double start_phase, double phase_coef,
+                               double fft_duration, double gate_begin, double gate_end, int nbins, float2 *acc,
+                               hipStream_t s) {
+  dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)((n_baselines + XB * XWAVES - 1) / (XB * XWAVES)));
+  hipLaunchKernelGGL(k_xstep_pulsar, grid, dim3(64 * XWAVES), 0, s, spectra, stream_stride, seg_stride, n_ch,
+                     n_segments, bl_first, bl_second, n_baselines, offsets, start_phase, phase_coef,
+                     fft_duration, gate_begin, gate_end, nbins, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_auto_norms(const float2 *vis, int n_streams, int n_ch, int fft_size, int n_bins,
+                             int n_baselines, double *norms, hipStream_t s) {
+  dim3 grid(n_streams, n_bins);
+  hipLaunchKernelGGL(k_auto_norms, grid, dim3(256), 0, s, vis, n_ch, fft_size, n_baselines, norms, n_streams);
+  return hipGetLastError();
+}
+
+hipError_t launch_normalise(const float2 *vis, const double *norms, const int32_t *bl_first,
+                            const int32_t *bl_second, const double *flag_factor, int n_streams, int n_baselines,
+                            int n_ch, int n_bins, float2 *out, hipStream_t s) {
+  const int64_t n = (int64_t)n_baselines * n_ch;
+  dim3 grid((unsigned)((n + 255) / 256), n_bins);
+  hipLaunchKernelGGL(k_normalise, grid, dim3(256), 0, s, vis, norms, bl_first, bl_second, flag_factor,
+                     n_streams, n_baselines, n_ch, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_synthesize(void *dst, size_t n_bytes, int bits, int64_t first_sample, int64_t delay_samples,
+                             uint64_t common_seed, uint64_t stream_seed, float rho, float threshold,
+                             hipStream_t s) {
+  dim3 grid((unsigned)((n_bytes + 255) / 256));
+  hipLaunchKernelGGL(k_synthesize, grid, dim3(256), 0, s, (uint8_t *)dst, (int64_t)n_bytes, bits, first_sample,
+                     delay_samples, common_seed, stream_seed, rho, threshold);
+  return hipGetLastError();
+}
+
+}  // namespace sfxg

@@ -112,6 +112,44 @@ def config0(seconds: float = 2.0, t_int_usec: int = 1_000_000, nchan: int = 1024
                         offsets=[0.0, 1.234567e-6], rates=[0.0, 1e-7])
 
 
+def config1_jobs(subband: int, n_integrations: int = 10, t_int_usec: int = 1_000_000, nchan: int = 4096,
+                 n_stations: int = 8):
+    """Job descriptions of configuration 1 (no sample data): 8 stations x 2
+    pols, 32 MHz at 64 Ms/s, sky 6.0 GHz + 32 MHz * subband, 2-bit, 4096
+    channels, 1 s integrations, delay offsets U[-5, 5] ms and rates
+    U[-3e-6, 3e-6] (seed 1). Returns (jobs, raw_start_ticks)."""
+    rng = np.random.default_rng(1)
+    offsets = rng.uniform(-5e-3, 5e-3, n_stations)
+    rates = rng.uniform(-3e-6, 3e-6, n_stations)
+    fs, bw = 64_000_000, 32_000_000
+    start = mjd_to_ticks(START_MJD, 0.0)
+    tps = int(round(CLOCK_RATE / fs))
+    margin = int(math.ceil(5.1e-3 * fs)) + 4096
+    margin -= margin % 16
+    raw_start = start - margin * tps
+    secs = int(math.ceil(n_integrations * t_int_usec / 1e6 + margin / fs + 2))
+    st_models = [DelayModel.linear(raw_start, o + r * (raw_start - start) / CLOCK_RATE, r, secs)
+                 for o, r in zip(offsets, rates)]
+    streams, models = [], []
+    for s in range(n_stations):
+        for p in ("R", "L"):
+            streams.append(StationStream(station_number=s, station_stream=len(streams), sample_rate=fs,
+                                         channel_freq=6_000_000_000 + 32_000_000 * subband, bandwidth=bw,
+                                         sideband="U", polarisation=p, bits_per_sample=2))
+            models.append(st_models[s])
+    t_int_ticks = int(round(t_int_usec * (CLOCK_RATE / 1e6)))
+    jobs = []
+    for k in range(n_integrations):
+        t0 = start + k * t_int_ticks
+        jobs.append(CorrelationParameters(
+            experiment_start=start, start_time=t0, stop_time=t0 + t_int_ticks, integration_time=t_int_ticks,
+            sub_integration_time=int(round(min(t_int_usec, 125000) * (CLOCK_RATE / 1e6))),
+            number_channels=nchan, fft_size_delaycor=256, fft_size_correlation=max(256, nchan), sample_rate=fs,
+            channel_freq=6_000_000_000 + 32_000_000 * subband, bandwidth=bw, sideband="U", frequency_nr=subband,
+            station_streams=streams, integration_nr=k, slice_nr=k, delay_models=models))
+    return jobs, raw_start
+
+
 def config1_subband(subband: int, seconds: float = 10.0, t_int_usec: int = 1_000_000, nchan: int = 4096,
                     n_stations: int = 8):
     """8 stations x 2 pols, subband `subband` of 8 (32 MHz at 64 Ms/s, 6.0 GHz +

@@ -79,13 +79,17 @@ def delay_changes(model: DelayModel, start_ticks: int, nsamples: int, fs: int) -
     return out
 
 
-def header_stream(raw: np.ndarray, raw_start_ticks: int, model: DelayModel, start_ticks: int,
-                  nsamples: int, fs: int, bits: int, invalid=None) -> bytes:
+def header_stream(raw, raw_start_ticks: int, model: DelayModel, start_ticks: int,
+                  nsamples: int, fs: int, bits: int, invalid=None, elided: bool = False):
     """Input_node_data_writer::do_task for one slice (input_node_data_writer.cc:105-228).
 
     raw: the station's packed byte stream whose first byte holds the sample
-    at raw_start_ticks. invalid: optional list of (byte_start, nbytes) ranges
-    of raw flagged invalid (the Input_buffer_element invalid list).
+    at raw_start_ticks (with elided=True only its length is used: an int).
+    invalid: optional list of (byte_start, nbytes) ranges of raw flagged
+    invalid (the Input_buffer_element invalid list).
+    elided=True leaves the payload bytes out of the DATA records and returns
+    (control_bytes, first_payload_byte_in_raw, n_payload_bytes); the payload
+    is then raw[first:first + n] when no invalid ranges are given.
     """
     spb = 8 // bits
     tps = int(round(CLOCK_RATE / fs))
@@ -106,7 +110,8 @@ def header_stream(raw: np.ndarray, raw_start_ticks: int, model: DelayModel, star
             k = min(n - written, SHRT_MAX)
             out.append(HEADER_DATA)
             out.extend(int(k).to_bytes(2, "little"))
-            out.extend(raw[start + written:start + written + k].tobytes())
+            if not elided:
+                out.extend(raw[start + written:start + written + k].tobytes())
             written += k
 
     def w_invalid(nsamp):
@@ -138,7 +143,7 @@ def header_stream(raw: np.ndarray, raw_start_ticks: int, model: DelayModel, star
 
     inv = sorted(invalid or [])
     inv_i = 0
-    block_size = len(raw)
+    block_size = raw if elided else len(raw)
     index = byte_offset
     ndp = next_delay_pos() + byte_offset
     total = min(block_size - byte_offset, (slice_size + spb - 1) // spb)
@@ -174,6 +179,35 @@ def header_stream(raw: np.ndarray, raw_start_ticks: int, model: DelayModel, star
     if slice_size > 0:
         raise ValueError("raw data too short for the slice")
     out.append(HEADER_ENDSTREAM)
+    if elided:
+        return bytes(out), byte_offset, index - byte_offset
+    return bytes(out)
+
+
+def join_elided(control: bytes, payload: np.ndarray) -> bytes:
+    """Re-insert payload bytes into an elided HEADER stream (inverse of
+    header_stream(..., elided=True) without invalid ranges)."""
+    out = bytearray()
+    pos = 0
+    i = 0
+    pl = payload.tobytes() if isinstance(payload, np.ndarray) else bytes(payload)
+    while i < len(control):
+        h = control[i]
+        if h == HEADER_DATA:
+            n = control[i + 1] | (control[i + 2] << 8)
+            out.extend(control[i:i + 3])
+            out.extend(pl[pos:pos + n])
+            pos += n
+            i += 3
+        elif h == HEADER_DELAY:
+            out.extend(control[i:i + 2])
+            i += 2
+        elif h == HEADER_INVALID:
+            out.extend(control[i:i + 3])
+            i += 3
+        else:
+            out.append(h)
+            i += 1
     return bytes(out)
 
 
