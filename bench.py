@@ -111,6 +111,7 @@ def main():
             inputs[(k, sb)] = lst
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
+    print(f"setup done: {len(inputs)} slices in {t_setup:.1f} s", file=sys.stderr, flush=True)
 
     ctx = capi.Context(dev)
     ctx.set_options(lut=1)
@@ -127,6 +128,7 @@ def main():
         for sb in range(NSB):
             run_slice(w % K, sb)
     torch.cuda.synchronize()
+    print("warmup done", file=sys.stderr, flush=True)
     times = []
     t0 = time.perf_counter()
     for k in range(K):
@@ -134,6 +136,7 @@ def main():
             times.append(run_slice(k, sb))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    print(f"timed region done: {wall:.3f} s", file=sys.stderr, flush=True)
 
     job = per_sb[0][0][0]
     M = len(job.station_streams)
