@@ -104,3 +104,127 @@ def test_c0_all_slices(ctx, c0_short):
         heads_r, vis_r = vis_from_bytes(job, ref["output"])
         assert heads_g == heads_r
         compare_vis(vis_g[0], vis_r[0], label=f"slice {job.integration_nr}")
+
+
+def run_both(ctx, job, hdrs, lut=LUT_CONFIG, pulsar=None):
+    ref = O.run_slice(job, hdrs, lut=lut, pulsar=pulsar)
+    out = capi.correlate_slice(ctx, job, hdrs, lut=lut, pulsar=pulsar)
+    nb = pulsar.nbins + 1 if pulsar is not None else 1
+    heads_g, vis_g = vis_from_bytes(job, out, nb)
+    heads_r, vis_r = vis_from_bytes(job, ref["output"], nb)
+    assert heads_g == heads_r
+    for i in range(len(job.station_streams)):
+        lv, ts = ctx.statistics(i)
+        np.testing.assert_array_equal(lv, ref["stats"][i][0])
+        np.testing.assert_array_equal(ts, ref["stats"][i][1])
+    return vis_g, vis_r, ref
+
+
+@pytest.mark.parametrize("nchan", [256, 2048, 16384])
+def test_channel_counts(ctx, nchan):
+    """Power-of-four and odd-log2 FFT sizes (the radix-2 stage), up to the
+    largest LDS-resident transform."""
+    job, hdrs = configs.build_slices(seed=5, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
+                                     seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=[0.0, 2.5e-6, -1.75e-6], rates=[0.0, 2e-7, -1e-7])[0]
+    vis_g, vis_r, _ = run_both(ctx, job, hdrs)
+    compare_vis(vis_g[0], vis_r[0], label=f"nchan={nchan}")
+
+
+@pytest.mark.parametrize("window", [0, 1, 2, 3])
+def test_windows(ctx, window):
+    job, hdrs = configs.build_slices(seed=6, n_stations=2, pols=["R", "L"], fs=32_000_000, bw=16_000_000,
+                                     nchan=1024, seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2,
+                                     sky_freq=5_000_000_000, offsets=[0.0, 1e-6], rates=[0.0, 1e-7],
+                                     window=window)[0]
+    vis_g, vis_r, _ = run_both(ctx, job, hdrs)
+    compare_vis(vis_g[0], vis_r[0], label=f"window={window}")
+
+
+def test_one_bit(ctx):
+    job, hdrs = configs.build_slices(seed=7, n_stations=3, pols=["R"], fs=16_000_000, bw=8_000_000, nchan=512,
+                                     seconds=0.125, t_int_usec=125000, rho=0.05, bits=1, sky_freq=1_600_000_000,
+                                     offsets=[0.0, -3e-6, 4e-6], rates=[0.0, 1e-6, -1e-6])[0]
+    for lut in (LUT_REFERENCE, LUT_CONFIG):
+        vis_g, vis_r, _ = run_both(ctx, job, hdrs, lut=lut)
+        compare_vis(vis_g[0], vis_r[0], label=f"1-bit lut={lut}")
+
+
+def test_invalid_data(ctx):
+    """INVALID records: zero fill, n_invalid in the statistics, weights and
+    the flag factor of the normalisation."""
+    inv = {0: [(1000, 300), (50000, 2000)], 2: [(1100, 100), (90000, 5000)]}
+    job, hdrs = configs.build_slices(seed=8, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=1024,
+                                     seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=[0.0, 2e-6, -1e-6], rates=[0.0, 3e-7, -2e-7], invalid=inv)[0]
+    vis_g, vis_r, ref = run_both(ctx, job, hdrs)
+    assert ref["stats"][0][0][4] > 0 and ref["stats"][2][0][4] > 0
+    compare_vis(vis_g[0], vis_r[0], label="invalid")
+
+
+def test_reference_station(ctx):
+    job, hdrs = configs.build_slices(seed=9, n_stations=4, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=512,
+                                     seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=[0.0, 1e-6, 2e-6, 3e-6], rates=[0.0] * 4)[0]
+    job.reference_station = 2
+    vis_g, vis_r, _ = run_both(ctx, job, hdrs)
+    assert vis_g[0].shape[0] == 4 + 3
+    compare_vis(vis_g[0], vis_r[0], label="reference station")
+
+
+def test_pulsar_bins(ctx):
+    """Correlation_core_pulsar: per-channel gate/bin accumulation."""
+    from sfxg.job import Pulsar, ticks_to_mjd
+    job, hdrs = configs.build_slices(seed=10, n_stations=2, pols=["R", "L"], fs=32_000_000, bw=16_000_000,
+                                     nchan=1024, seconds=0.125, t_int_usec=125000, rho=0.05, bits=2,
+                                     sky_freq=1_400_000_000, offsets=[0.0, 1e-6], rates=[0.0, 1e-7])[0]
+    psr = Pulsar(nbins=8, gate_start=0.1, gate_stop=0.6, tmid=ticks_to_mjd(job.start_time) - 0.001, DM=68.0,
+                 ref_phase=0.3, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+    vis_g, vis_r, _ = run_both(ctx, job, hdrs, pulsar=psr)
+    for b in range(psr.nbins + 1):
+        compare_vis(vis_g[b], vis_r[b], label=f"pulsar bin {b}")
+
+
+def test_elided_push_matches(ctx, c0_short):
+    """Device-resident payload (sfxg_push_stream_elided) gives the same bytes
+    as the host HEADER stream."""
+    import ctypes as C
+    from sfxg import synth
+    job, hdrs = c0_short[0]
+    out_host = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    L = C.CDLL("libamdhip64.so")
+    ctx.set_options(lut=LUT_CONFIG)
+    ctx.set_parameters(job)
+    bufs = []
+    for i, h in enumerate(hdrs):
+        ctrl = bytearray()
+        payload = bytearray()
+        k = 0
+        while k < len(h):
+            t = h[k]
+            if t == 0:
+                n = h[k + 1] | (h[k + 2] << 8)
+                ctrl += h[k:k + 3]
+                payload += h[k + 3:k + 3 + n]
+                k += 3 + n
+            elif t == 2:
+                ctrl += h[k:k + 2]
+                k += 2
+            elif t == 1:
+                ctrl += h[k:k + 3]
+                k += 3
+            else:
+                ctrl.append(t)
+                k += 1
+        p = C.c_void_p()
+        assert L.hipMalloc(C.byref(p), C.c_size_t(len(payload))) == 0
+        src = (C.c_uint8 * len(payload)).from_buffer_copy(bytes(payload))
+        assert L.hipMemcpy(p, src, C.c_size_t(len(payload)), 1) == 0
+        bufs.append(p)
+        ctx.push_stream_elided(i, bytes(ctrl), p.value, len(payload))
+    ctx.correlate()
+    out_dev = ctx.output()
+    for p in bufs:
+        L.hipFree(p)
+    assert out_dev == out_host
+    del synth
