@@ -91,7 +91,7 @@ struct sfxg_ctx {
 
   // device buffers
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
-      flags, bl, fold_buf, model_buf, tw_nd, tw_n, tw_2n, window, offsets, samples;
+      flags, bl, fold_buf, partial, block_consts, model_buf, tw_nd, tw_n, tw_2n, window, offsets, samples;
   int tables_nc = -1;
   int tables_window = -1;
   bool enqueued = false;
@@ -143,7 +143,7 @@ void sfxg_destroy(sfxg_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DBuf *bufs[] = {&ctx->payload, &ctx->runs, &ctx->block_run, &ctx->counted, &ctx->dstreams,
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
-                  &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->model_buf, &ctx->tw_nd,
+                  &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
                   &ctx->tw_n, &ctx->tw_2n, &ctx->window, &ctx->offsets, &ctx->samples};
   for (DBuf *b : bufs) b->release();
   if (ctx->done) (void)hipEventDestroy(ctx->done);
@@ -439,6 +439,7 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->runs.ensure((size_t)run_tot * sizeof(Run)));
   HIPCHK(ctx, ctx->counted.ensure((size_t)std::max<int64_t>(cnt_tot, 1) * sizeof(CountRange)));
   HIPCHK(ctx, ctx->block_run.ensure((size_t)M * ctx->n_blocks * sizeof(int32_t)));
+  HIPCHK(ctx, ctx->block_consts.ensure((size_t)M * ctx->n_blocks * sizeof(BlockConst)));
   HIPCHK(ctx, ctx->model_buf.ensure((size_t)mod_tot * sizeof(double)));
   HIPCHK(ctx, ctx->dstreams.ensure((size_t)M * sizeof(DevStream)));
   HIPCHK(ctx, ctx->time_series.ensure((size_t)M * ctx->n_samples * sizeof(float)));
@@ -450,7 +451,11 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->norms.ensure((size_t)ctx->nbins * M * sizeof(double)));
   HIPCHK(ctx, ctx->flags.ensure((size_t)B * sizeof(double)));
   HIPCHK(ctx, ctx->bl.ensure((size_t)2 * B * sizeof(int32_t)));
-  HIPCHK(ctx, ctx->fold_buf.ensure((size_t)std::max<size_t>(ctx->folds.size(), 1) * sizeof(int32_t)));
+  HIPCHK(ctx, ctx->fold_buf.ensure((ctx->folds.size() + 1) * sizeof(int32_t)));
+  // sub-integration partial sums, at most ~1 GiB at a time
+  const size_t part_one = (size_t)B * (nc + 1) * sizeof(float2);
+  const int k_group = (int)std::max<size_t>(1, std::min<size_t>(ctx->folds.size(), ((size_t)1 << 30) / part_one));
+  if (!ctx->pulsar) HIPCHK(ctx, ctx->partial.ensure(part_one * k_group));
 
   std::vector<int32_t> block_run((size_t)M * ctx->n_blocks);
   std::vector<double> model_host((size_t)mod_tot);
@@ -531,6 +536,7 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     d.spectra = ctx->spectra.as<float2>() + (int64_t)i * ctx->nseg * ctx->S;
     d.samples = nullptr;
     d.hist = ctx->hist.as<int32_t>() + 512 * i;
+    d.block_consts = ctx->block_consts.as<BlockConst>() + blk_off[i];
   }
   HIPCHK(ctx, hipMemcpyAsync(ctx->block_run.p, block_run.data(), block_run.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, s));
@@ -543,9 +549,10 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   std::copy(ctx->bl_second.begin(), ctx->bl_second.end(), blv.begin() + B);
   HIPCHK(ctx, hipMemcpyAsync(ctx->bl.p, blv.data(), blv.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemcpyAsync(ctx->flags.p, ctx->flag_factor.data(), B * sizeof(double), hipMemcpyHostToDevice, s));
-  if (!ctx->folds.empty())
-    HIPCHK(ctx, hipMemcpyAsync(ctx->fold_buf.p, ctx->folds.data(), ctx->folds.size() * sizeof(int32_t),
-                               hipMemcpyHostToDevice, s));
+  std::vector<int32_t> fold_start(1, 0);
+  fold_start.insert(fold_start.end(), ctx->folds.begin(), ctx->folds.end());
+  HIPCHK(ctx, hipMemcpyAsync(ctx->fold_buf.p, fold_start.data(), fold_start.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemsetAsync(ctx->hist.p, 0, (size_t)M * 512 * sizeof(int32_t), s));
   HIPCHK(ctx, hipMemsetAsync(ctx->vis.p, 0, vis_elems * sizeof(float2), s));
 
@@ -570,8 +577,13 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_XSTEP], s));
   const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
   if (!ctx->pulsar) {
-    HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S, nc + 1, ctx->nseg, blf, bls, B,
-                             ctx->fold_buf.as<int32_t>(), (int)ctx->folds.size(), ctx->vis.as<float2>(), s));
+    const int n_folds = (int)ctx->folds.size();
+    for (int g0 = 0; g0 < n_folds; g0 += k_group) {
+      const int ng = std::min(k_group, n_folds - g0);
+      HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S, nc + 1, M,
+                               ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),
+                               ctx->vis.as<float2>(), s));
+    }
   } else {
     // Correlation_core_pulsar::set_parameters (correlation_core_pulsar.cc:46-93)
     const int64_t tpd = (int64_t)86400 * (int64_t)kClockRate;

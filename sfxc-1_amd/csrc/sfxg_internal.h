@@ -98,6 +98,13 @@ struct DevModel {
   const double *amp;
 };
 
+// FP64 constants of one N_d block of Delay_correction.
+struct BlockConst {
+  double c_term, l_term;  // fractional-delay ramp: phase c_term + k * l_term on bin k
+  double phi, dphi;       // fringe phase at the block start and per-sample step
+  double amp;
+};
+
 struct DevStream {
   const uint8_t *payload;
   const Run *runs;
@@ -127,6 +134,7 @@ struct DevStream {
   float2 *spectra;   // [n_segments][spectra_stride]
   float *samples;    // optional unpack output (verification)
   int32_t *hist;     // [2][256] byte histograms (tsys on, off)
+  BlockConst *block_consts;  // [n_blocks]
 };
 
 struct DevJob {
@@ -149,9 +157,11 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
                         hipStream_t s);
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
                              const float2 *tw_n, const float2 *tw_2n, hipStream_t s);
-hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
-                        int64_t n_segments, const int32_t *bl_first, const int32_t *bl_second,
-                        int n_baselines, const int32_t *folds, int n_folds, float2 *vis, hipStream_t s);
+// Sub-integration partial sums (fold_start[0..n_k]: segment ranges) and
+// their in-order fold into vis.
+hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M,
+                        const int32_t *fold_start, int n_k, const int32_t *bl_first, const int32_t *bl_second,
+                        int n_baselines, float2 *partial, float2 *vis, hipStream_t s);
 hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
                                int n_ch, int64_t n_segments, const int32_t *bl_first,
                                const int32_t *bl_second, int n_baselines, const double *offsets,

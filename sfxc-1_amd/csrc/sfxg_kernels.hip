@@ -159,6 +159,16 @@ __global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ str
 // ---------------------------------------------------------------------------
 namespace {
 
+// Orders this wave's LDS writes before its later LDS reads of other lanes'
+// data. A wavefront's LDS operations execute in issue order, so keeping the
+// compiler from reordering them across this point is enough; waves working
+// on their own LDS regions never wait for each other.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One Stockham radix-4 stage over 256 points for one wavefront: v[r] holds
 // data[lane + 64 r] on entry; on exit buf holds the stage output.
 template <int SIGN>
@@ -170,79 +180,111 @@ __device__ __forceinline__ void stage256(float2 (&v)[4], float2 *buf, const floa
   v[3] = cmul(v[3], twiddle<SIGN>(tw, k * 3 * step));
   radix4<SIGN>(v[0], v[1], v[2], v[3]);
   const int idx = (lane / ns) * ns * 4 + k;
-  __syncthreads();
+  wave_lds_sync();
   buf[idx] = v[0];
   buf[idx + ns] = v[1];
   buf[idx + 2 * ns] = v[2];
   buf[idx + 3 * ns] = v[3];
-  __syncthreads();
+  wave_lds_sync();
 }
 
 template <int SIGN>
 __device__ __forceinline__ void fft256(float2 (&v)[4], float2 *buf, const float2 *tw, int lane) {
+#pragma unroll
   for (int ns = 1; ns < 256; ns *= 4) {
     stage256<SIGN>(v, buf, tw, lane, ns);
+#pragma unroll
     for (int r = 0; r < 4; r++) v[r] = buf[lane + 64 * r];
   }
 }
 
 }  // namespace
 
+// Per-block FP64 constants of Delay_correction (delay_correction.cc:40-47,
+// 97-103, 139-160), one thread per N_d block: {constant and linear term of the
+// fractional-delay ramp, fringe phase at the block start and its per-sample
+// step} and the amplitude. Compiled without contraction in the host's
+// operation order, so integer/fractional delay decisions match it bit for bit.
+__global__ __launch_bounds__(256) void k_block_consts(const DevStream *__restrict__ streams, int64_t n_blocks,
+                                                      int nd) {
+#pragma clang fp contract(off)
+  const DevStream &s = streams[blockIdx.y];
+  const int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blk >= n_blocks) return;
+  const int64_t current = s.start_ticks + blk * nd * s.tps;
+  const double delay = eval_model(s.model, s.model.delay, current + s.fl_half, 0.0);
+  const double dis = delay * s.fs;
+  const int integer_delay = (int)floor(dis + .5);
+  const double fractional_delay = dis - integer_delay;
+  const double dfr = s.fs / nd;
+  const double tmp1 = -2.0 * kPi * fractional_delay / s.fs;
+  const double tmp2 = kPi * (integer_delay & (4 * s.oversamp - 1)) / (2 * s.oversamp);
+  const double c_term = tmp2 - tmp1 * 0.5 * s.bandwidth;
+  const double l_term = tmp1 * dfr;
+  const double lo_phase = s.start_phase + s.LO_offset * ((double)(current - s.start_ticks) / kClockRate);
+  const double *ph = s.model.has_phase ? s.model.phase : nullptr;
+  const double p0 = s.center_freq * eval_model(s.model, s.model.delay, current, 0.0) + lo_phase +
+                    eval_model(s.model, ph, current, 0.0) / (2 * kPi);
+  const double floor_phi = floor(p0);
+  const double phi = s.mult_factor_phi * (p0 - floor_phi);
+  double p1 = s.center_freq * eval_model(s.model, s.model.delay, current + s.fl, 0.0) + lo_phase +
+              s.fl_sec * s.LO_offset + eval_model(s.model, ph, current + s.fl, 0.0) / (2 * kPi);
+  p1 = s.mult_factor_phi * (p1 - floor_phi);
+  const double dphi = (p1 - phi) / nd;
+  const double amp = s.model.has_amp ? eval_model(s.model, s.model.amp, current + s.fl_half, 1.0) : 1.0;
+  BlockConst bc;
+  bc.c_term = c_term;
+  bc.l_term = l_term;
+  bc.phi = phi;
+  bc.dphi = dphi;
+  bc.amp = amp;
+  s.block_consts[blk] = bc;
+}
+
+// k_delay: one wavefront per N_d = 256 block. Samples are decoded straight
+// from the packed payload, transformed with a radix-4 Stockham FFT held in
+// registers and the wave's own LDS region, phase-ramped, transformed back and
+// fringe-rotated.
 __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ streams, DevJob job,
-                                               const float2 *__restrict__ tw) {
+                                               const float2 *__restrict__ tw_g) {
   __shared__ float2 lds[4][256];
+  __shared__ float2 tw[256];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  tw[threadIdx.x] = tw_g[threadIdx.x];
+  __syncthreads();
   const DevStream &s = streams[blockIdx.y];
   const int64_t blk = (int64_t)blockIdx.x * 4 + wave;
-  const bool active = blk < job.n_blocks;
+  if (blk >= job.n_blocks) return;
   float2 *buf = lds[wave];
   const int nd = 256;
 
   float2 v[4];
-  if (active) {
+  {
     const int64_t o0 = blk * nd;
     int64_t r = s.block_run[blk];
-    for (int q = 0; q < 4; q++) v[q] = make_float2(sample_at(s, o0 + lane + 64 * q, r), 0.0f);
-  } else {
-    for (int q = 0; q < 4; q++) v[q] = make_float2(0.0f, 0.0f);
+    const Run run = s.runs[r];
+    if (run.src >= 0 && run.out_start + run.count >= o0 + nd) {
+      // the whole block comes from one run: consecutive payload samples
+      const int64_t p0 = run.src + (o0 - run.out_start);
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] = make_float2(decode_sample(s, p0 + lane + 64 * q), 0.0f);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] = make_float2(sample_at(s, o0 + lane + 64 * q, r), 0.0f);
+    }
   }
   fft256<-1>(v, buf, tw, lane);
 
-  // block constants in FP64 (delay_correction.cc:40-47, 97-103, 139-160)
-  double c_term = 0.0, l_term = 0.0, phi = 0.0, dphi = 0.0, amp = 1.0;
-  {
-#pragma clang fp contract(off)
-    const int64_t current = s.start_ticks + blk * nd * s.tps;
-    const double delay = eval_model(s.model, s.model.delay, current + s.fl_half, 0.0);
-    const double dis = delay * s.fs;
-    const int integer_delay = (int)floor(dis + .5);
-    const double fractional_delay = dis - integer_delay;
-    const double dfr = s.fs / nd;
-    const double tmp1 = -2.0 * kPi * fractional_delay / s.fs;
-    const double tmp2 = kPi * (integer_delay & (4 * s.oversamp - 1)) / (2 * s.oversamp);
-    c_term = tmp2 - tmp1 * 0.5 * s.bandwidth;
-    l_term = tmp1 * dfr;
-    const double lo_phase = s.start_phase + s.LO_offset * ((double)(current - s.start_ticks) / kClockRate);
-    const double *ph = s.model.has_phase ? s.model.phase : nullptr;
-    double p0 = s.center_freq * eval_model(s.model, s.model.delay, current, 0.0) + lo_phase +
-                eval_model(s.model, ph, current, 0.0) / (2 * kPi);
-    const double floor_phi = floor(p0);
-    phi = s.mult_factor_phi * (p0 - floor_phi);
-    double p1 = s.center_freq * eval_model(s.model, s.model.delay, current + s.fl, 0.0) + lo_phase +
-                s.fl_sec * s.LO_offset + eval_model(s.model, ph, current + s.fl, 0.0) / (2 * kPi);
-    p1 = s.mult_factor_phi * (p1 - floor_phi);
-    dphi = (p1 - phi) / nd;
-    if (s.model.has_amp) amp = eval_model(s.model, s.model.amp, current + s.fl_half, 1.0);
-  }
-
+  const BlockConst bc = s.block_consts[blk];
   // halve DC and Nyquist, drop negative frequencies, apply the ramp
+#pragma unroll
   for (int q = 0; q < 4; q++) {
     const int k = lane + 64 * q;
     if (k > nd / 2) {
       v[q] = make_float2(0.0f, 0.0f);
     } else {
       if (k == 0 || k == nd / 2) v[q] = make_float2(v[q].x * 0.5f, v[q].y * 0.5f);
-      const float a = (float)(c_term + k * l_term);
+      const float a = (float)(bc.c_term + k * bc.l_term);
       float sn, cs;
       sincosf(a, &sn, &cs);
       v[q] = cmul(make_float2(cs, -sn), v[q]);
@@ -250,16 +292,15 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
   }
   fft256<+1>(v, buf, tw, lane);
 
-  if (active) {
-    float *out = s.time_series + blk * nd;
-    const float famp = (float)amp;
-    for (int q = 0; q < 4; q++) {
-      const int i = lane + 64 * q;
-      const float a = (float)(phi + i * dphi);
-      float sn, cs;
-      sincosf(a, &sn, &cs);
-      out[i] = famp * (v[q].x * cs + v[q].y * sn);
-    }
+  float *out = s.time_series + blk * nd;
+  const float famp = (float)bc.amp;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = lane + 64 * q;
+    const float a = (float)(bc.phi + i * bc.dphi);
+    float sn, cs;
+    sincosf(a, &sn, &cs);
+    out[i] = famp * (v[q].x * cs + v[q].y * sn);
   }
 }
 
@@ -369,61 +410,98 @@ __global__ __launch_bounds__(T) void k_channelise(const DevStream *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// k_xstep: plain FP32 X-step. Lane = channel, each thread owns XB baselines.
-// Products and sums follow the reference's operation order without
+// k_xstep_subint: plain FP32 X-step for one sub-integration per workgroup
+// slice of the grid. Lane = channel (64-channel tile), each wave owns up to
+// XBPW baselines. The segments of the tile are staged through LDS in groups
+// of `sps` segments x M streams x 64 channels (coalesced 512 B rows), and
+// every product reads its two operands from LDS.
+//
+// The reference accumulates every (baseline, channel) sequentially over the
+// segments of a sub-integration, starting from zero, and adds the result to
+// the phase-centre buffer at the fold (correlation_core.cc:241-273,515-559).
+// Sub-integrations are therefore independent partial sums: each z-slice of
+// the grid computes one into `partial`, and k_fold_combine adds them in fold
+// order. Products and sums use the reference's operation order without
 // contraction (std::complex<float> multiply, then +=).
 // ---------------------------------------------------------------------------
-constexpr int XB = 8;  // baselines per thread
-constexpr int XWAVES = 4;
+constexpr int XBPW = 16;  // baselines per wave
 
-__global__ __launch_bounds__(64 * XWAVES) void k_xstep(const float2 *__restrict__ spectra,
-                                                       int64_t stream_stride, int64_t seg_stride, int n_ch,
-                                                       int64_t n_segments, const int32_t *__restrict__ bl_first,
-                                                       const int32_t *__restrict__ bl_second, int n_baselines,
-                                                       const int32_t *__restrict__ folds, int n_folds,
-                                                       float2 *__restrict__ vis) {
+__global__ __launch_bounds__(512) void k_xstep_subint(const float2 *__restrict__ spectra, int64_t stream_stride,
+                                                      int64_t seg_stride, int n_ch, int M,
+                                                      const int32_t *__restrict__ fold_start,
+                                                      const int32_t *__restrict__ bl_first,
+                                                      const int32_t *__restrict__ bl_second, int n_baselines,
+                                                      int sps, float2 *__restrict__ partial) {
 #pragma clang fp contract(off)
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ch = blockIdx.x * 64 + lane;
-  const int b0 = (blockIdx.y * XWAVES + wave) * XB;
-  if (ch >= n_ch || b0 >= n_baselines) return;
-  const float2 *pa[XB];
-  const float2 *pb[XB];
-  float2 acc[XB];
-  int nb = n_baselines - b0 < XB ? n_baselines - b0 : XB;
+  extern __shared__ float2 xlds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nwaves = blockDim.x >> 6;
+  const int ch0 = blockIdx.x * 64;
+  const int ch = ch0 + lane;
+  const int k = blockIdx.z;
+  const int64_t s_begin = fold_start[k], s_end = fold_start[k + 1];
+  const int b0 = __builtin_amdgcn_readfirstlane((blockIdx.y * nwaves + wave) * XBPW);
+  int nb = n_baselines - b0;
+  nb = nb < 0 ? 0 : (nb > XBPW ? XBPW : nb);
+  int oa[XBPW], ob[XBPW];
+  float2 acc[XBPW];
 #pragma unroll
-  for (int q = 0; q < XB; q++) {
-    const int b = b0 + (q < nb ? q : 0);
-    pa[q] = spectra + bl_first[b] * stream_stride + ch;
-    pb[q] = spectra + bl_second[b] * stream_stride + ch;
+  for (int q = 0; q < XBPW; q++) {
+    const int b = q < nb ? b0 + q : 0;
+    oa[q] = bl_first[b] * 64 + lane;
+    ob[q] = bl_second[b] * 64 + lane;
     acc[q] = make_float2(0.0f, 0.0f);
   }
-  int fi = 0;
-  for (int64_t sgi = 0; sgi < n_segments; sgi++) {
-    const int64_t off = sgi * seg_stride;
-#pragma unroll
-    for (int q = 0; q < XB; q++) {
-      const float2 x = pa[q][off], y = pb[q][off];
-      const float pr = x.x * y.x + x.y * y.y;
-      const float pi = x.y * y.x - x.x * y.y;
-      acc[q].x = acc[q].x + pr;
-      acc[q].y = acc[q].y + pi;
+  const int rows_per_seg = M * 64;
+  for (int64_t s0 = s_begin; s0 < s_end; s0 += sps) {
+    const int ns = (int)((s_end - s0) < sps ? (s_end - s0) : sps);
+    const int total = ns * rows_per_seg;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int c = idx & 63;
+      const int r = idx >> 6;
+      const int sl = r / M, st = r - sl * M;
+      float2 v = make_float2(0.0f, 0.0f);
+      if (ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + sl) * seg_stride + ch0 + c];
+      xlds[idx] = v;
     }
-    if (fi < n_folds && sgi + 1 == folds[fi]) {
+    __syncthreads();
+    for (int sl = 0; sl < ns; sl++) {
+      const float2 *row = xlds + sl * rows_per_seg;
 #pragma unroll
-      for (int q = 0; q < XB; q++) {
+      for (int q = 0; q < XBPW; q++) {
         if (q < nb) {
-          float2 *o = vis + (int64_t)(b0 + q) * n_ch + ch;
-          float2 t = *o;
-          t.x = t.x + acc[q].x;
-          t.y = t.y + acc[q].y;
-          *o = t;
+          const float2 x = row[oa[q]], y = row[ob[q]];
+          const float pr = x.x * y.x + x.y * y.y;
+          const float pi = x.y * y.x - x.x * y.y;
+          acc[q].x = acc[q].x + pr;
+          acc[q].y = acc[q].y + pi;
         }
-        acc[q] = make_float2(0.0f, 0.0f);
       }
-      fi++;
     }
+    __syncthreads();
   }
+  if (ch < n_ch) {
+    float2 *o = partial + (int64_t)k * n_baselines * n_ch + ch;
+#pragma unroll
+    for (int q = 0; q < XBPW; q++)
+      if (q < nb) o[(int64_t)(b0 + q) * n_ch] = acc[q];
+  }
+}
+
+// vis += partial[k] for k = 0 .. n_k - 1, in fold order.
+__global__ __launch_bounds__(256) void k_fold_combine(const float2 *__restrict__ partial, int n_k, int64_t n,
+                                                      float2 *__restrict__ vis) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float2 t = vis[i];
+  for (int k = 0; k < n_k; k++) {
+    const float2 p = partial[k * n + i];
+    t.x = t.x + p.x;
+    t.y = t.y + p.y;
+  }
+  vis[i] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -432,6 +510,9 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep(const float2 *__restrict_
 // change the running sums are written back and the new bin's sums loaded,
 // which keeps the reference's per-segment addition order.
 // ---------------------------------------------------------------------------
+constexpr int XB = 8;  // baselines per thread in the pulsar kernel
+constexpr int XWAVES = 4;
+
 __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
     const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
     int64_t n_segments, const int32_t *__restrict__ bl_first, const int32_t *__restrict__ bl_second,
@@ -598,6 +679,10 @@ hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_c
 
 hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const float2 *tw_nd, hipStream_t s) {
   if (job.nd != 256) return hipErrorInvalidValue;
+  dim3 gridc((unsigned)((job.n_blocks + 255) / 256), job.n_streams);
+  hipLaunchKernelGGL(k_block_consts, gridc, dim3(256), 0, s, d_streams, job.n_blocks, job.nd);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   dim3 grid((unsigned)((job.n_blocks + 3) / 4), job.n_streams);
   hipLaunchKernelGGL(k_delay, grid, dim3(256), 0, s, d_streams, job, tw_nd);
   return hipGetLastError();
@@ -627,12 +712,24 @@ hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, cons
   }
 }
 
-hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
-                        int64_t n_segments, const int32_t *bl_first, const int32_t *bl_second, int n_baselines,
-                        const int32_t *folds, int n_folds, float2 *vis, hipStream_t s) {
-  dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)((n_baselines + XB * XWAVES - 1) / (XB * XWAVES)));
-  hipLaunchKernelGGL(k_xstep, grid, dim3(64 * XWAVES), 0, s, spectra, stream_stride, seg_stride, n_ch,
-                     n_segments, bl_first, bl_second, n_baselines, folds, n_folds, vis);
+hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M,
+                        const int32_t *fold_start, int n_k, const int32_t *bl_first, const int32_t *bl_second,
+                        int n_baselines, float2 *partial, float2 *vis, hipStream_t s) {
+  if (n_k <= 0) return hipSuccess;
+  int nw = (n_baselines + XBPW - 1) / XBPW;
+  if (nw > 8) nw = 8;
+  const int bl_tiles = (n_baselines + nw * XBPW - 1) / (nw * XBPW);
+  int sps = 65536 / (M * 64 * (int)sizeof(float2));
+  if (sps < 1) sps = 1;
+  const size_t lds = (size_t)sps * M * 64 * sizeof(float2);
+  if (lds > 65536) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)bl_tiles, (unsigned)n_k);
+  hipLaunchKernelGGL(k_xstep_subint, grid, dim3(64 * nw), lds, s, spectra, stream_stride, seg_stride, n_ch, M,
+                     fold_start, bl_first, bl_second, n_baselines, sps, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t n = (int64_t)n_baselines * n_ch;
+  hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
   return hipGetLastError();
 }
 

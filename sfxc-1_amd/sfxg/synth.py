@@ -223,15 +223,20 @@ def _frac_delay_fir(x: np.ndarray, frac: float, taps: int = 32) -> np.ndarray:
 
 
 def analytic(x: np.ndarray) -> np.ndarray:
-    """x + i H(x) by FFT."""
-    n = len(x)
-    X = np.fft.rfft(x)
+    """x + i H(x) by FFT (length padded to an FFT-friendly size; x is white
+    noise, so the periodic extension is as good as any)."""
+    import scipy.fft as sf
+    n0 = len(x)
+    n = sf.next_fast_len(n0)
+    if n > n0:
+        x = np.concatenate([x, np.zeros(n - n0)])
+    X = sf.rfft(x, workers=-1)
     X[1:(n + 1) // 2] *= 2
     full = np.zeros(n, dtype=np.complex128)
     full[:len(X)] = X
     if n % 2 == 0:
         full[n // 2] = X[n // 2]
-    return np.fft.ifft(full)
+    return sf.ifft(full, workers=-1, overwrite_x=True)[:n0]
 
 
 def delayed_signal(common_a: np.ndarray, common_start: int, n: int, delays_samples: np.ndarray,
