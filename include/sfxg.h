@@ -137,6 +137,12 @@ typedef struct {
 typedef struct {
   int32_t lut;        /* SFXG_LUT_* */
   int32_t precision;  /* SFXG_PREC_* */
+  /* Nonzero: run delay correction and the channeliser as two kernels and
+   * keep the real series in HBM for SFXG_STAGE_TIME_SERIES (verification).
+   * Zero (default): for N_c <= 4096 they run fused and the series stays in
+   * LDS. Both give identical results. */
+  int32_t keep_time_series;
+  int32_t reserved;
 } sfxg_options;
 
 /* Creates a context on HIP device `device`. Replaces the construction of
@@ -200,18 +206,15 @@ int sfxg_number_of_baselines(sfxg_ctx *ctx, int32_t *n);
 int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst, size_t cap,
                      size_t *written);
 
-/* Kernel durations of the last sfxg_correlate, in milliseconds, measured with
- * HIP events on the context's stream: ms[SFXG_KERNEL_*]. The reference's
+/* Average kernel durations per slice, in milliseconds, over the slices
+ * correlated since the last reset, measured with HIP events on the context's
+ * stream: ms[SFXG_KERNEL_*]. With the fused F-engine SFXG_KERNEL_DELAY covers
+ * delay correction and channeliser and SFXG_KERNEL_CHANNELISE is 0. Waits for
+ * the slice in flight. reset != 0 clears the sums afterwards. The reference's
  * counterpart is its per-component timing instrumentation
  * (src/correlation_core.cc:11-17, src/delay_correction.cc:12-18,
  * lib/common/src/monitor.h:80-173). n: capacity of ms. */
-#define SFXG_KERNEL_STATS 0
-#define SFXG_KERNEL_DELAY 1
-#define SFXG_KERNEL_CHANNELISE 2
-#define SFXG_KERNEL_XSTEP 3
-#define SFXG_KERNEL_NORMALISE 4
-#define SFXG_N_KERNELS 5
-int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n);
+int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n, int32_t reset);
 
 /* Synthetic 2-bit/1-bit raw station data written straight into device
  * memory (for throughput measurements on device-resident input): n_bytes

@@ -65,7 +65,8 @@ int nr_ffts_per_integration_slice(int64_t integration_ticks, int data_rate, int 
 struct sfxg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  sfxg_options opt{SFXG_LUT_REFERENCE, SFXG_PREC_FP32};
+  sfxg_options opt{SFXG_LUT_REFERENCE, SFXG_PREC_FP32, 0, 0};
+  bool fused = false;  // the last slice ran the fused F-engine
   std::string err;
 
   // slice parameters
@@ -92,11 +93,18 @@ struct sfxg_ctx {
   // device buffers
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
       flags, bl, fold_buf, partial, block_consts, model_buf, tw_nd, tw_n, tw_2n, window, offsets, samples;
+  // host staging of the enqueued slice (kept until the next slice)
+  std::vector<double> h_model, h_offsets;
+  std::vector<DevStream> h_streams;
+  std::vector<int32_t> h_bl, h_fold_start;
   int tables_nc = -1;
   int tables_window = -1;
   bool enqueued = false;
   hipEvent_t done = nullptr;
   hipEvent_t ev[SFXG_N_KERNELS + 1] = {};
+  bool times_pending = false;  // events of an enqueued slice not yet read
+  double kt_sum[SFXG_N_KERNELS] = {};
+  int64_t kt_count = 0;
 
   int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -118,6 +126,24 @@ struct sfxg_ctx {
     int _rc = (ctx)->hip((call), #call);          \
     if (_rc != SFXG_OK) return _rc;               \
   } while (0)
+
+// Adds the kernel durations of the last enqueued slice to the running sums
+// (the caller has synchronised the stream).
+static void collect_times(sfxg_ctx *ctx) {
+  if (!ctx->times_pending) return;
+  for (int k = 0; k < SFXG_N_KERNELS; k++) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) ctx->kt_sum[k] += t;
+  }
+  ctx->kt_count++;
+  ctx->times_pending = false;
+}
+
+static void sync_slice(sfxg_ctx *ctx) {
+  if (!ctx->enqueued) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  collect_times(ctx);
+}
 
 extern "C" {
 
@@ -170,7 +196,7 @@ static int sb_sign(char c) { return c == 'L' ? -1 : 1; }
 int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params,
                         const sfxg_delay_model *models, const double *uvw, const sfxg_pulsar *pulsar) {
   if (!ctx || !params || !models) return SFXG_ERR_ARG;
-  if (ctx->enqueued) (void)hipStreamSynchronize(ctx->stream);
+  sync_slice(ctx);
   ctx->enqueued = false;
   ctx->have_params = false;
   const sfxg_correlation_parameters &p = *params;
@@ -276,7 +302,7 @@ static int push_common(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *bytes, 
   if (stream_idx < 0 || stream_idx >= ctx->M) return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
   if (!bytes && n) return ctx->fail(SFXG_ERR_ARG, "null stream");
   if (ctx->enqueued) {
-    (void)hipStreamSynchronize(ctx->stream);
+    sync_slice(ctx);
     ctx->enqueued = false;
   }
   const sfxg_station_parameters &s = ctx->st[stream_idx];
@@ -366,7 +392,7 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     if (ctx->plans[i].chunks != ctx->plans[0].chunks)
       return ctx->fail(SFXG_ERR_STATE, "streams deliver different chunkings");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (ctx->enqueued) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->enqueued) sync_slice(ctx);
   const int M = ctx->M, nd = ctx->nd, nc = ctx->nc, B = ctx->B;
   hipStream_t s = ctx->stream;
 
@@ -442,7 +468,8 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->block_consts.ensure((size_t)M * ctx->n_blocks * sizeof(BlockConst)));
   HIPCHK(ctx, ctx->model_buf.ensure((size_t)mod_tot * sizeof(double)));
   HIPCHK(ctx, ctx->dstreams.ensure((size_t)M * sizeof(DevStream)));
-  HIPCHK(ctx, ctx->time_series.ensure((size_t)M * ctx->n_samples * sizeof(float)));
+  const bool fused = !ctx->opt.keep_time_series && fengine_supported(nc);
+  if (!fused) HIPCHK(ctx, ctx->time_series.ensure((size_t)M * ctx->n_samples * sizeof(float)));
   HIPCHK(ctx, ctx->spectra.ensure((size_t)M * ctx->nseg * ctx->S * sizeof(float2)));
   const size_t vis_elems = (size_t)ctx->nbins * B * (nc + 1);
   HIPCHK(ctx, ctx->vis.ensure(vis_elems * sizeof(float2)));
@@ -457,9 +484,12 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   const int k_group = (int)std::max<size_t>(1, std::min<size_t>(ctx->folds.size(), ((size_t)1 << 30) / part_one));
   if (!ctx->pulsar) HIPCHK(ctx, ctx->partial.ensure(part_one * k_group));
 
-  std::vector<int32_t> block_run((size_t)M * ctx->n_blocks);
-  std::vector<double> model_host((size_t)mod_tot);
-  std::vector<DevStream> ds(M);
+  // host staging stays alive in the context until the next slice, so the
+  // copies and kernels below run asynchronously
+  std::vector<double> &model_host = ctx->h_model;
+  model_host.assign((size_t)mod_tot, 0.0);
+  std::vector<DevStream> &ds = ctx->h_streams;
+  ds.assign(M, DevStream());
   for (int i = 0; i < M; i++) {
     const StreamPlan &pl = ctx->plans[i];
     if (!pl.ext_payload && !pl.payload.empty())
@@ -471,13 +501,6 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     if (!pl.counted.empty())
       HIPCHK(ctx, hipMemcpyAsync(ctx->counted.as<CountRange>() + cnt_off[i], pl.counted.data(),
                                  pl.counted.size() * sizeof(CountRange), hipMemcpyHostToDevice, s));
-    // index of the run holding the first sample of each N_d block
-    int64_t r = 0;
-    for (int64_t k = 0; k < ctx->n_blocks; k++) {
-      const int64_t o = k * nd;
-      while (r + 1 < (int64_t)pl.runs.size() && pl.runs[r + 1].out_start <= o) r++;
-      block_run[blk_off[i] + k] = (int32_t)r;
-    }
     const ModelCopy &m = ctx->models[i];
     double *mh = model_host.data() + mod_off[i];
     std::copy(m.delay.begin(), m.delay.end(), mh);
@@ -490,7 +513,7 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     memset(&d, 0, sizeof(d));
     d.payload = pl.ext_payload ? (const uint8_t *)pl.ext_payload : ctx->payload.as<uint8_t>() + pay_off[i];
     d.runs = ctx->runs.as<Run>() + run_off[i];
-    d.block_run = ctx->block_run.as<int32_t>() + blk_off[i];
+    d.block_run = ctx->block_run.as<int32_t>() + blk_off[i];  // filled by k_block_consts
     d.counted = ctx->counted.as<CountRange>() + cnt_off[i];
     d.n_runs = (int64_t)pl.runs.size();
     d.n_counted = (int64_t)pl.counted.size();
@@ -532,24 +555,24 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     d.fl_sec = d.fl / kClockRate;
     d.oversamp = (int)round(fs / (2.0 * st.bandwidth));
     d.flip = ctx->p.sideband != st.sideband;
-    d.time_series = ctx->time_series.as<float>() + (int64_t)i * ctx->n_samples;
+    d.time_series = fused ? nullptr : ctx->time_series.as<float>() + (int64_t)i * ctx->n_samples;
     d.spectra = ctx->spectra.as<float2>() + (int64_t)i * ctx->nseg * ctx->S;
     d.samples = nullptr;
     d.hist = ctx->hist.as<int32_t>() + 512 * i;
     d.block_consts = ctx->block_consts.as<BlockConst>() + blk_off[i];
   }
-  HIPCHK(ctx, hipMemcpyAsync(ctx->block_run.p, block_run.data(), block_run.size() * sizeof(int32_t),
-                             hipMemcpyHostToDevice, s));
   if (mod_tot)
     HIPCHK(ctx, hipMemcpyAsync(ctx->model_buf.p, model_host.data(), model_host.size() * sizeof(double),
                                hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemcpyAsync(ctx->dstreams.p, ds.data(), ds.size() * sizeof(DevStream), hipMemcpyHostToDevice, s));
-  std::vector<int32_t> blv(2 * B);
+  std::vector<int32_t> &blv = ctx->h_bl;
+  blv.assign(2 * B, 0);
   std::copy(ctx->bl_first.begin(), ctx->bl_first.end(), blv.begin());
   std::copy(ctx->bl_second.begin(), ctx->bl_second.end(), blv.begin() + B);
   HIPCHK(ctx, hipMemcpyAsync(ctx->bl.p, blv.data(), blv.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemcpyAsync(ctx->flags.p, ctx->flag_factor.data(), B * sizeof(double), hipMemcpyHostToDevice, s));
-  std::vector<int32_t> fold_start(1, 0);
+  std::vector<int32_t> &fold_start = ctx->h_fold_start;
+  fold_start.assign(1, 0);
   fold_start.insert(fold_start.end(), ctx->folds.begin(), ctx->folds.end());
   HIPCHK(ctx, hipMemcpyAsync(ctx->fold_buf.p, fold_start.data(), fold_start.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, s));
@@ -571,9 +594,17 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_STATS], s));
   HIPCHK(ctx, launch_stats(dstr, M, max_counted, s));
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_DELAY], s));
-  HIPCHK(ctx, launch_delay(dstr, job, ctx->tw_nd.as<float2>(), s));
-  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_CHANNELISE], s));
-  HIPCHK(ctx, launch_channelise(dstr, job, ctx->window.as<float>(), ctx->tw_n.as<float2>(), ctx->tw_2n.as<float2>(), s));
+  if (fused) {
+    HIPCHK(ctx, launch_fengine(dstr, job, ctx->window.as<float>(), ctx->tw_nd.as<float2>(), ctx->tw_n.as<float2>(),
+                               ctx->tw_2n.as<float2>(), s));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_CHANNELISE], s));
+  } else {
+    HIPCHK(ctx, launch_delay(dstr, job, ctx->tw_nd.as<float2>(), s));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_CHANNELISE], s));
+    HIPCHK(ctx, launch_channelise(dstr, job, ctx->window.as<float>(), ctx->tw_n.as<float2>(),
+                                  ctx->tw_2n.as<float2>(), s));
+  }
+  ctx->fused = fused;
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_XSTEP], s));
   const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
   if (!ctx->pulsar) {
@@ -606,7 +637,8 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     double freq = 0;
     for (int i = N - 1; i > 1; i--) freq = (freq + i * ps.coef[i]) * DT;
     freq = (freq + ps.coef[1]) / 60 + ps.ref_freq;
-    std::vector<double> offs(nc + 1);
+    std::vector<double> &offs = ctx->h_offsets;
+    offs.assign(nc + 1, 0.0);
     for (int i = 0; i < nc + 1; i++)
       offs[i] = 4148.808 * ps.DM * (1 / pow(base_freq + i * dfreq, 2) - inv_freq_obs2) * freq;
     HIPCHK(ctx, ctx->offsets.ensure(offs.size() * sizeof(double)));
@@ -621,9 +653,8 @@ int sfxg_correlate(sfxg_ctx *ctx) {
                                B, nc + 1, ctx->nbins, ctx->norm_out.as<float2>(), s));
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_N_KERNELS], s));
   HIPCHK(ctx, hipEventRecord(ctx->done, s));
-  // host staging vectors above are released on return: wait for the copies
-  HIPCHK(ctx, hipStreamSynchronize(s));
   ctx->enqueued = true;
+  ctx->times_pending = true;
   return SFXG_OK;
 }
 
@@ -783,6 +814,8 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
       break;
     }
     case SFXG_STAGE_TIME_SERIES:
+      if (ctx->fused)
+        return ctx->fail(SFXG_ERR_STATE, "the real series stayed in LDS: set keep_time_series in sfxg_options");
       need = (size_t)ctx->n_samples * sizeof(float);
       if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
       HIPCHK(ctx, hipMemcpy(dst, ctx->time_series.as<float>() + (int64_t)stream_idx * ctx->n_samples, need,
@@ -810,15 +843,17 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
   return SFXG_OK;
 }
 
-int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n) {
+int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n, int32_t reset) {
   if (!ctx || !ms) return SFXG_ERR_ARG;
-  if (!ctx->enqueued) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  for (int k = 0; k < SFXG_N_KERNELS && k < n; k++) {
-    float t = 0.0f;
-    HIPCHK(ctx, hipEventElapsedTime(&t, ctx->ev[k], ctx->ev[k + 1]));
-    ms[k] = t;
+  if (ctx->enqueued) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    collect_times(ctx);
+  }
+  for (int k = 0; k < SFXG_N_KERNELS && k < n; k++) ms[k] = ctx->kt_count ? ctx->kt_sum[k] / ctx->kt_count : 0.0;
+  if (reset) {
+    for (double &v : ctx->kt_sum) v = 0.0;
+    ctx->kt_count = 0;
   }
   return SFXG_OK;
 }

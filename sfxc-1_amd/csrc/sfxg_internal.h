@@ -108,7 +108,7 @@ struct BlockConst {
 struct DevStream {
   const uint8_t *payload;
   const Run *runs;
-  const int32_t *block_run;  // index of the run holding the first sample of each N_d block
+  int32_t *block_run;        // index of the run holding the first sample of each N_d block
   const CountRange *counted;
   int64_t n_runs;
   int64_t n_counted;
@@ -155,6 +155,11 @@ hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_c
                         hipStream_t s);
 hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const float2 *tw_nd,
                         hipStream_t s);
+// Fused unpack + delay correction + channeliser (the real series stays in
+// LDS) for N_c <= 4096; launches k_block_consts first.
+bool fengine_supported(int nc);
+hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window,
+                          const float2 *tw_nd, const float2 *tw_n, const float2 *tw_2n, hipStream_t s);
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
                              const float2 *tw_n, const float2 *tw_2n, hipStream_t s);
 // Sub-integration partial sums (fold_start[0..n_k]: segment ranges) and

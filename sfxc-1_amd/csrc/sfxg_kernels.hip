@@ -153,9 +153,7 @@ __global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ str
 }
 
 // ---------------------------------------------------------------------------
-// k_delay: one wavefront per N_d = 256 block. Samples are decoded straight
-// from the packed payload, transformed with a radix-4 Stockham FFT held in
-// registers and LDS, phase-ramped, transformed back and fringe-rotated.
+// Delay correction of one N_d = 256 block by one wavefront
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -201,7 +199,8 @@ __device__ __forceinline__ void fft256(float2 (&v)[4], float2 *buf, const float2
 }  // namespace
 
 // Per-block FP64 constants of Delay_correction (delay_correction.cc:40-47,
-// 97-103, 139-160), one thread per N_d block: {constant and linear term of the
+// 97-103, 139-160), one thread per N_d block, and the run index of the
+// block's first sample: {constant and linear term of the
 // fractional-delay ramp, fringe phase at the block start and its per-sample
 // step} and the amplitude. Compiled without contraction in the host's
 // operation order, so integer/fractional delay decisions match it bit for bit.
@@ -211,6 +210,16 @@ __global__ __launch_bounds__(256) void k_block_consts(const DevStream *__restric
   const DevStream &s = streams[blockIdx.y];
   const int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (blk >= n_blocks) return;
+  {
+    // index of the run holding the block's first sample
+    const int64_t o = blk * nd;
+    int64_t lo = 0, hi = s.n_runs - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (s.runs[mid].out_start <= o) lo = mid; else hi = mid - 1;
+    }
+    s.block_run[blk] = (int32_t)lo;
+  }
   const int64_t current = s.start_ticks + blk * nd * s.tps;
   const double delay = eval_model(s.model, s.model.delay, current + s.fl_half, 0.0);
   const double dis = delay * s.fs;
@@ -241,23 +250,14 @@ __global__ __launch_bounds__(256) void k_block_consts(const DevStream *__restric
   s.block_consts[blk] = bc;
 }
 
-// k_delay: one wavefront per N_d = 256 block. Samples are decoded straight
-// from the packed payload, transformed with a radix-4 Stockham FFT held in
-// registers and the wave's own LDS region, phase-ramped, transformed back and
-// fringe-rotated.
-__global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ streams, DevJob job,
-                                               const float2 *__restrict__ tw_g) {
-  __shared__ float2 lds[4][256];
-  __shared__ float2 tw[256];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  tw[threadIdx.x] = tw_g[threadIdx.x];
-  __syncthreads();
-  const DevStream &s = streams[blockIdx.y];
-  const int64_t blk = (int64_t)blockIdx.x * 4 + wave;
-  if (blk >= job.n_blocks) return;
-  float2 *buf = lds[wave];
-  const int nd = 256;
-
+// One N_d = 256 block of one stream by one wavefront: samples are decoded
+// straight from the packed payload (unpack + integer shift), transformed with
+// a radix-4 Stockham FFT held in registers and the wave's own LDS region
+// `buf`, phase-ramped (fractional_bit_shift), transformed back and
+// fringe-rotated (fringe_stopping); out receives the 256 real samples.
+__device__ __forceinline__ void delay_block(const DevStream &s, int64_t blk, float2 *buf, const float2 *tw,
+                                            int lane, float *out) {
+  constexpr int nd = 256;
   float2 v[4];
   {
     const int64_t o0 = blk * nd;
@@ -292,7 +292,6 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
   }
   fft256<+1>(v, buf, tw, lane);
 
-  float *out = s.time_series + blk * nd;
   const float famp = (float)bc.amp;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -302,6 +301,21 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
     sincosf(a, &sn, &cs);
     out[i] = famp * (v[q].x * cs + v[q].y * sn);
   }
+}
+
+// k_delay: one wavefront per block, the real series to HBM (unfused path,
+// used for N_c above the fused F-engine's LDS budget and for verification).
+__global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ streams, DevJob job,
+                                               const float2 *__restrict__ tw_g) {
+  __shared__ float2 lds[4][256];
+  __shared__ float2 tw[256];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  tw[threadIdx.x] = tw_g[threadIdx.x];
+  __syncthreads();
+  const DevStream &s = streams[blockIdx.y];
+  const int64_t blk = (int64_t)blockIdx.x * 4 + wave;
+  if (blk >= job.n_blocks) return;
+  delay_block(s, blk, lds[wave], tw, lane, s.time_series + blk * 256);
 }
 
 // ---------------------------------------------------------------------------
@@ -406,6 +420,83 @@ __global__ __launch_bounds__(T) void k_channelise(const DevStream *__restrict__ 
     const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
     const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
     out[k] = cadd(e, cmul(tw_2n[k], o));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_fengine: Delay_correction and the channeliser fused. A workgroup walks a
+// contiguous range of segments of one stream. The real series lives in an
+// LDS ring of two N_c halves: each step computes the next N_c samples
+// (N_c / 256 delay blocks, one per wave) into the older half, then windows
+// the 2 N_c samples of the segment into the FFT buffer, transforms them and
+// writes the N_c + 1 bins. The series never goes to HBM: per station-sample
+// the kernel reads b/8 bytes of payload and writes the 8 (N_c+1)/N_c bytes of
+// the spectrum. The FFT buffer doubles as the waves' scratch for the 256-point
+// transforms.
+// ---------------------------------------------------------------------------
+template <int N, int T>
+__global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ streams, DevJob job,
+                                               const float *__restrict__ window, const float2 *__restrict__ tw_nd,
+                                               const float2 *__restrict__ tw_n, const float2 *__restrict__ tw_2n,
+                                               int segs_per_wg) {
+  static_assert(N >= 256 && (T / 64) * 256 <= N, "wave scratch must fit in the FFT buffer");
+  __shared__ float ring[2 * N];
+  __shared__ float2 fbuf[N];
+  __shared__ float2 tw[256];
+  const DevStream &s = streams[blockIdx.y];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int nwaves = T / 64;
+  constexpr int BPH = N / 256;  // delay blocks per half
+  const int64_t sa = (int64_t)blockIdx.x * segs_per_wg;
+  int64_t sb = sa + segs_per_wg;
+  if (sb > job.n_segments) sb = job.n_segments;
+  if (sa >= sb) return;  // uniform over the workgroup
+  for (int i = threadIdx.x; i < 256; i += T) tw[i] = tw_nd[i];
+  __syncthreads();
+  const bool none = job.window == SFXG_WINDOW_NONE;
+  float2 *scratch = fbuf + wave * 256;
+  if (!none) {
+    for (int b = wave; b < BPH; b += nwaves) delay_block(s, sa * BPH + b, scratch, tw, lane, ring + b * 256);
+    __syncthreads();
+  }
+  const float2 *w2 = reinterpret_cast<const float2 *>(window);
+  for (int64_t seg = sa; seg < sb; seg++) {
+    const int h_old = (int)((seg - sa) & 1);
+    const int h_new = none ? 0 : 1 - h_old;
+    const int64_t blk0 = none ? seg * BPH : (seg + 1) * BPH;
+    for (int b = wave; b < BPH; b += nwaves)
+      delay_block(s, blk0 + b, scratch, tw, lane, ring + h_new * N + b * 256);
+    __syncthreads();
+    // window the segment's 2 N real samples into N complex points
+    for (int m = threadIdx.x; m < N; m += T) {
+      float2 z;
+      if (none) {
+        if (m < N / 2) {
+          const float2 x = reinterpret_cast<const float2 *>(ring)[m], wv = w2[m];
+          z = make_float2(x.x * wv.x, x.y * wv.y);
+        } else {
+          z = make_float2(0.0f, 0.0f);
+        }
+      } else {
+        const int j = 2 * m;  // j and j + 1 lie in the same half
+        const int half = j < N ? h_old : 1 - h_old;
+        const float2 x = *reinterpret_cast<const float2 *>(ring + half * N + (j & (N - 1)));
+        const float2 wv = w2[m];
+        z = make_float2(x.x * wv.x, x.y * wv.y);
+      }
+      if (s.flip) z.y = -z.y;
+      fbuf[m] = z;
+    }
+    __syncthreads();
+    lds_fft<N, T, -1>(fbuf, tw_n);
+    float2 *out = s.spectra + seg * job.spectra_stride;
+    for (int k = threadIdx.x; k <= N; k += T) {
+      const float2 zk = fbuf[k & (N - 1)], zn = fbuf[(N - k) & (N - 1)];
+      const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+      const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+      out[k] = cadd(e, cmul(tw_2n[k], o));
+    }
+    __syncthreads();
   }
 }
 
@@ -708,6 +799,39 @@ hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, cons
     case 4096: return launch_chan_n<4096>(d_streams, job, window, tw_n, tw_2n, s);
     case 8192: return launch_chan_n<8192>(d_streams, job, window, tw_n, tw_2n, s);
     case 16384: return launch_chan_n<16384>(d_streams, job, window, tw_n, tw_2n, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N, int T>
+static hipError_t launch_fe_n(const DevStream *d_streams, const DevJob &job, const float *window,
+                              const float2 *tw_nd, const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+  // about 1024 workgroups in flight: two per CU
+  int per_stream = (1024 + job.n_streams - 1) / job.n_streams;
+  int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
+  if (segs < 1) segs = 1;
+  const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
+  hipLaunchKernelGGL((k_fengine<N, T>), dim3(gx, job.n_streams), dim3(T), 0, s, d_streams, job, window, tw_nd,
+                     tw_n, tw_2n, (int)segs);
+  return hipGetLastError();
+}
+
+bool fengine_supported(int nc) { return nc >= 256 && nc <= 4096; }
+
+hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window, const float2 *tw_nd,
+                          const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+  if (job.n_segments <= 0) return hipSuccess;
+  if (job.nd != 256) return hipErrorInvalidValue;
+  dim3 gridc((unsigned)((job.n_blocks + 255) / 256), job.n_streams);
+  hipLaunchKernelGGL(k_block_consts, gridc, dim3(256), 0, s, d_streams, job.n_blocks, job.nd);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  switch (job.nc) {
+    case 256: return launch_fe_n<256, 64>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 512: return launch_fe_n<512, 128>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 1024: return launch_fe_n<1024, 256>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 2048: return launch_fe_n<2048, 512>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 4096: return launch_fe_n<4096, 512>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
     default: return hipErrorInvalidValue;
   }
 }

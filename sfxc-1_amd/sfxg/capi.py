@@ -54,7 +54,8 @@ class PulsarC(C.Structure):
 
 
 class Options(C.Structure):
-    _fields_ = [("lut", C.c_int32), ("precision", C.c_int32)]
+    _fields_ = [("lut", C.c_int32), ("precision", C.c_int32), ("keep_time_series", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -89,7 +90,7 @@ def lib() -> C.CDLL:
         L.sfxg_fetch_stage.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t,
                                        C.POINTER(C.c_size_t)]
         L.sfxg_kernel_times.restype = C.c_int
-        L.sfxg_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int32]
+        L.sfxg_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_int32]
         L.sfxg_synthesize_raw.restype = C.c_int
         L.sfxg_synthesize_raw.argtypes = [C.c_int, C.c_void_p, C.c_size_t, C.c_int32, C.c_int64, C.c_int64,
                                           C.c_uint64, C.c_uint64, C.c_float, C.c_float]
@@ -129,8 +130,8 @@ class Context:
             msg = lib().sfxg_last_error(self._h)
             raise SfxgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def set_options(self, lut: int = 0, precision: int = 0):
-        o = Options(lut, precision)
+    def set_options(self, lut: int = 0, precision: int = 0, keep_time_series: bool = False):
+        o = Options(lut, precision, int(keep_time_series), 0)
         self._check(lib().sfxg_set_options(self._h, C.byref(o)), "sfxg_set_options")
 
     def set_parameters(self, job: CorrelationParameters, pulsar: Optional[Pulsar] = None):
@@ -204,11 +205,11 @@ class Context:
         self._check(lib().sfxg_get_statistics(self._h, i, lv, ts), "sfxg_get_statistics")
         return np.array(lv[:], dtype=np.int32), np.array(ts[:], dtype=np.int32)
 
-    def kernel_times(self) -> np.ndarray:
-        """Per-kernel milliseconds of the last correlate: stats, delay,
-        channelise, xstep, normalise."""
+    def kernel_times(self, reset: bool = True) -> np.ndarray:
+        """Average per-slice milliseconds of stats, delay (fused F-engine),
+        channelise, xstep and normalise since the last reset."""
         ms = (C.c_double * 5)()
-        self._check(lib().sfxg_kernel_times(self._h, ms, 5), "sfxg_kernel_times")
+        self._check(lib().sfxg_kernel_times(self._h, ms, 5, int(reset)), "sfxg_kernel_times")
         return np.array(ms[:])
 
     def number_of_baselines(self) -> int:
@@ -242,8 +243,8 @@ def synthesize_raw(device: int, ptr: int, n_bytes: int, bits: int, first_sample:
 
 
 def correlate_slice(ctx: Context, job: CorrelationParameters, headers: Sequence[bytes], lut: int = 0,
-                    pulsar: Optional[Pulsar] = None) -> bytes:
-    ctx.set_options(lut=lut)
+                    pulsar: Optional[Pulsar] = None, keep_time_series: bool = False) -> bytes:
+    ctx.set_options(lut=lut, keep_time_series=keep_time_series)
     ctx.set_parameters(job, pulsar)
     for i, h in enumerate(headers):
         ctx.push_stream(i, h)

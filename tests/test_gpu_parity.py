@@ -47,9 +47,10 @@ def vis_from_bytes(job, out: bytes, nbins=1):
 
 def compare_vis(v_gpu, v_ref, tol_norm=2e-5, tol_chan=1e-5, label=""):
     """Per channel relative amplitude and phase errors (bar: 1e-5 and 1e-5
-    rad) over channels whose amplitude is at least 0.1 of the product's rms;
-    weaker channels sit at the FP32 accumulation noise floor, where a
-    relative error means nothing. Also per product max |dV| / rms |V|: two
+    rad) over channels whose amplitude is at least half the product's rms;
+    the error of a channel is set by the accumulation's absolute rounding, so
+    on weak channels (a Rayleigh tail of noise-dominated visibilities) a
+    relative error or a phase grows without bound and means nothing. Also per product max |dV| / rms |V|: two
     FP32 correlators whose FFTs round differently differ there by about
     sqrt(2 n / 3) * 2^-24 for n segments per sub-integration (1e-5 at 7800
     segments), whatever their summation order, hence its bar of 2e-5."""
@@ -59,7 +60,7 @@ def compare_vis(v_gpu, v_ref, tol_norm=2e-5, tol_chan=1e-5, label=""):
         g = v_gpu[b].astype(np.complex128)
         rms = np.sqrt(np.mean(np.abs(r) ** 2))
         e_norm = np.max(np.abs(g - r)) / rms
-        m = np.abs(r) >= 0.1 * rms
+        m = np.abs(r) >= 0.5 * rms
         e_amp = np.max(np.abs(np.abs(g[m]) - np.abs(r[m])) / np.abs(r[m]))
         e_ph = np.max(np.abs(np.angle(g[m] * np.conj(r[m]))))
         worst = tuple(max(a, c) for a, c in zip(worst, (e_norm, e_amp, e_ph)))
@@ -74,7 +75,7 @@ def compare_vis(v_gpu, v_ref, tol_norm=2e-5, tol_chan=1e-5, label=""):
 def test_c0_slice_stages(ctx, c0_short, lut):
     job, hdrs = c0_short[0]
     ref = O.run_slice(job, hdrs, lut=lut, keep_intermediate=True)
-    out = capi.correlate_slice(ctx, job, hdrs, lut=lut)
+    out = capi.correlate_slice(ctx, job, hdrs, lut=lut, keep_time_series=True)
     M = len(job.station_streams)
     for i in range(M):
         s_gpu = ctx.stage(capi.STAGE_SAMPLES, i)
@@ -97,6 +98,18 @@ def test_c0_slice_stages(ctx, c0_short, lut):
     heads_r, vis_r = vis_from_bytes(job, ref["output"])
     assert heads_g == heads_r
     compare_vis(vis_g[0], vis_r[0], label=f"c0 lut={lut}")
+
+
+def test_fused_matches_unfused(ctx, c0_short):
+    """The fused F-engine (series in LDS) and the two-kernel path give the
+    same bytes."""
+    job, hdrs = c0_short[0]
+    a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=True)
+    sp_a = ctx.stage(capi.STAGE_SPECTRA, 1)
+    b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=False)
+    sp_b = ctx.stage(capi.STAGE_SPECTRA, 1)
+    np.testing.assert_array_equal(sp_a, sp_b)
+    assert a == b
 
 
 def test_c0_all_slices(ctx, c0_short):
