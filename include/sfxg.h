@@ -214,6 +214,12 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
  * counterpart is its per-component timing instrumentation
  * (src/correlation_core.cc:11-17, src/delay_correction.cc:12-18,
  * lib/common/src/monitor.h:80-173). n: capacity of ms. */
+#define SFXG_KERNEL_STATS 0
+#define SFXG_KERNEL_DELAY 1
+#define SFXG_KERNEL_CHANNELISE 2
+#define SFXG_KERNEL_XSTEP 3
+#define SFXG_KERNEL_NORMALISE 4
+#define SFXG_N_KERNELS 5
 int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n, int32_t reset);
 
 /* Synthetic 2-bit/1-bit raw station data written straight into device

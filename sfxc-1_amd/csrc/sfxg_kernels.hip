@@ -277,29 +277,41 @@ __device__ __forceinline__ void delay_block(const DevStream &s, int64_t blk, flo
 
   const BlockConst bc = s.block_consts[blk];
   // halve DC and Nyquist, drop negative frequencies, apply the ramp
+  // exp(-i (c + k l)) on bin k = lane + 64 q: one sincos per lane for
+  // k = lane, then rotations by the block-uniform exp(-i 64 l)
+  {
+    float sn, cs, sn64, cs64;
+    sincosf((float)(bc.c_term + lane * bc.l_term), &sn, &cs);
+    sincosf((float)(64.0 * bc.l_term), &sn64, &cs64);
+    float2 e = make_float2(cs, -sn);
+    const float2 step = make_float2(cs64, -sn64);
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int k = lane + 64 * q;
-    if (k > nd / 2) {
-      v[q] = make_float2(0.0f, 0.0f);
-    } else {
-      if (k == 0 || k == nd / 2) v[q] = make_float2(v[q].x * 0.5f, v[q].y * 0.5f);
-      const float a = (float)(bc.c_term + k * bc.l_term);
-      float sn, cs;
-      sincosf(a, &sn, &cs);
-      v[q] = cmul(make_float2(cs, -sn), v[q]);
+    for (int q = 0; q < 4; q++) {
+      const int k = lane + 64 * q;
+      if (k > nd / 2) {
+        v[q] = make_float2(0.0f, 0.0f);
+      } else {
+        if (k == 0 || k == nd / 2) v[q] = make_float2(v[q].x * 0.5f, v[q].y * 0.5f);
+        v[q] = cmul(e, v[q]);
+      }
+      e = cmul(e, step);
     }
   }
   fft256<+1>(v, buf, tw, lane);
 
+  // fringe rotation: phase phi + i dphi on sample i = lane + 64 q, the same
+  // sincos-and-rotate scheme
   const float famp = (float)bc.amp;
+  float sn, cs, sn64, cs64;
+  sincosf((float)(bc.phi + lane * bc.dphi), &sn, &cs);
+  sincosf((float)(64.0 * bc.dphi), &sn64, &cs64);
+  float2 r = make_float2(cs, sn);
+  const float2 rstep = make_float2(cs64, sn64);
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int i = lane + 64 * q;
-    const float a = (float)(bc.phi + i * bc.dphi);
-    float sn, cs;
-    sincosf(a, &sn, &cs);
-    out[i] = famp * (v[q].x * cs + v[q].y * sn);
+    out[i] = famp * (v[q].x * r.x + v[q].y * r.y);
+    r = cmul(r, rstep);
   }
 }
 
@@ -545,18 +557,40 @@ __global__ __launch_bounds__(512) void k_xstep_subint(const float2 *__restrict__
     acc[q] = make_float2(0.0f, 0.0f);
   }
   const int rows_per_seg = M * 64;
-  for (int64_t s0 = s_begin; s0 < s_end; s0 += sps) {
-    const int ns = (int)((s_end - s0) < sps ? (s_end - s0) : sps);
+  // stage = up to sps segments x M rows x 64 channels; thread t holds the
+  // elements t, t + T, ... of the next stage in registers while the current
+  // one is consumed from LDS
+  constexpr int PF = 16;  // per-thread prefetch registers (float2)
+  float2 pf[PF];
+  auto fetch = [&](int64_t s0, int ns) {
     const int total = ns * rows_per_seg;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int c = idx & 63;
-      const int r = idx >> 6;
-      const int sl = r / M, st = r - sl * M;
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int idx = threadIdx.x + u * blockDim.x;
       float2 v = make_float2(0.0f, 0.0f);
-      if (ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + sl) * seg_stride + ch0 + c];
-      xlds[idx] = v;
+      if (idx < total) {
+        const int c = idx & 63;
+        const int r = idx >> 6;
+        const int sl = r / M, st = r - sl * M;
+        if (ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + sl) * seg_stride + ch0 + c];
+      }
+      pf[u] = v;
+    }
+  };
+  int64_t s0 = s_begin;
+  int ns = (int)((s_end - s0) < sps ? (s_end - s0) : sps);
+  if (s0 < s_end) fetch(s0, ns);
+  while (s0 < s_end) {
+    const int total = ns * rows_per_seg;
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int idx = threadIdx.x + u * blockDim.x;
+      if (idx < total) xlds[idx] = pf[u];
     }
     __syncthreads();
+    const int64_t s1 = s0 + ns;
+    const int ns1 = (int)((s_end - s1) < sps ? (s_end - s1) : sps);
+    if (s1 < s_end) fetch(s1, ns1);
     for (int sl = 0; sl < ns; sl++) {
       const float2 *row = xlds + sl * rows_per_seg;
 #pragma unroll
@@ -571,6 +605,8 @@ __global__ __launch_bounds__(512) void k_xstep_subint(const float2 *__restrict__
       }
     }
     __syncthreads();
+    s0 = s1;
+    ns = ns1;
   }
   if (ch < n_ch) {
     float2 *o = partial + (int64_t)k * n_baselines * n_ch + ch;
@@ -843,8 +879,10 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
   int nw = (n_baselines + XBPW - 1) / XBPW;
   if (nw > 8) nw = 8;
   const int bl_tiles = (n_baselines + nw * XBPW - 1) / (nw * XBPW);
-  int sps = 65536 / (M * 64 * (int)sizeof(float2));
-  if (sps < 1) sps = 1;
+  // a stage must fit the threads' prefetch registers (16 float2 each)
+  int sps = (16 * 64 * nw) / (M * 64);
+  if (sps > 16) sps = 16;
+  if (sps < 1) return hipErrorInvalidValue;  // more streams than one stage holds
   const size_t lds = (size_t)sps * M * 64 * sizeof(float2);
   if (lds > 65536) return hipErrorInvalidValue;
   dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)bl_tiles, (unsigned)n_k);
