@@ -102,7 +102,10 @@ struct DevModel {
 struct BlockConst {
   double c_term, l_term;  // fractional-delay ramp: phase c_term + k * l_term on bin k
   double phi, dphi;       // fringe phase at the block start and per-sample step
-  double amp;
+  float amp;
+  float ramp_step[2];     // cos, sin of 64 l_term
+  float fringe_step[2];   // cos, sin of 64 dphi
+  float pad;
 };
 
 struct DevStream {

@@ -246,45 +246,56 @@ __global__ __launch_bounds__(256) void k_block_consts(const DevStream *__restric
   bc.l_term = l_term;
   bc.phi = phi;
   bc.dphi = dphi;
-  bc.amp = amp;
+  bc.amp = (float)amp;
+  double sn, cs;
+  sincos(64.0 * l_term, &sn, &cs);
+  bc.ramp_step[0] = (float)cs;
+  bc.ramp_step[1] = (float)sn;
+  sincos(64.0 * dphi, &sn, &cs);
+  bc.fringe_step[0] = (float)cs;
+  bc.fringe_step[1] = (float)sn;
+  bc.pad = 0.0f;
   s.block_consts[blk] = bc;
 }
 
-// One N_d = 256 block of one stream by one wavefront: samples are decoded
-// straight from the packed payload (unpack + integer shift), transformed with
-// a radix-4 Stockham FFT held in registers and the wave's own LDS region
-// `buf`, phase-ramped (fractional_bit_shift), transformed back and
-// fringe-rotated (fringe_stopping); out receives the 256 real samples.
-__device__ __forceinline__ void delay_block(const DevStream &s, int64_t blk, float2 *buf, const float2 *tw,
-                                            int lane, float *out) {
+// The 256 samples of block blk, lane holding samples lane + 64 q (zeros
+// for a block past the end).
+__device__ __forceinline__ void load_block(const DevStream &s, int64_t blk, int64_t n_blocks, int lane,
+                                           float (&x)[4]) {
   constexpr int nd = 256;
-  float2 v[4];
-  {
-    const int64_t o0 = blk * nd;
-    int64_t r = s.block_run[blk];
-    const Run run = s.runs[r];
-    if (run.src >= 0 && run.out_start + run.count >= o0 + nd) {
-      // the whole block comes from one run: consecutive payload samples
-      const int64_t p0 = run.src + (o0 - run.out_start);
+  if (blk >= n_blocks) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) v[q] = make_float2(decode_sample(s, p0 + lane + 64 * q), 0.0f);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; q++) v[q] = make_float2(sample_at(s, o0 + lane + 64 * q, r), 0.0f);
-    }
+    for (int q = 0; q < 4; q++) x[q] = 0.0f;
+    return;
   }
-  fft256<-1>(v, buf, tw, lane);
+  const int64_t o0 = blk * nd;
+  int64_t r = s.block_run[blk];
+  const Run run = s.runs[r];
+  if (run.src >= 0 && run.out_start + run.count >= o0 + nd) {
+    // the whole block comes from one run: consecutive payload samples
+    const int64_t p0 = run.src + (o0 - run.out_start);
+#pragma unroll
+    for (int q = 0; q < 4; q++) x[q] = decode_sample(s, p0 + lane + 64 * q);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) x[q] = sample_at(s, o0 + lane + 64 * q, r);
+  }
+}
 
+// Second half of fractional_bit_shift and fringe_stopping for one block
+// whose spectrum X[k] (k = lane + 64 q) is in v: halve DC and Nyquist, drop
+// negative frequencies, apply the ramp exp(-i (c + k l)) (one sincos per
+// lane, then rotations by exp(-i 64 l)), transform back, fringe-rotate by
+// phi + i dphi (the same sincos-and-rotate scheme) and write 256 samples.
+__device__ __forceinline__ void finish_block(const DevStream &s, int64_t blk, float2 (&v)[4], float2 *buf,
+                                             const float2 *tw, int lane, float *out) {
+  constexpr int nd = 256;
   const BlockConst bc = s.block_consts[blk];
-  // halve DC and Nyquist, drop negative frequencies, apply the ramp
-  // exp(-i (c + k l)) on bin k = lane + 64 q: one sincos per lane for
-  // k = lane, then rotations by the block-uniform exp(-i 64 l)
   {
-    float sn, cs, sn64, cs64;
+    float sn, cs;
     sincosf((float)(bc.c_term + lane * bc.l_term), &sn, &cs);
-    sincosf((float)(64.0 * bc.l_term), &sn64, &cs64);
     float2 e = make_float2(cs, -sn);
-    const float2 step = make_float2(cs64, -sn64);
+    const float2 step = make_float2(bc.ramp_step[0], -bc.ramp_step[1]);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int k = lane + 64 * q;
@@ -298,25 +309,47 @@ __device__ __forceinline__ void delay_block(const DevStream &s, int64_t blk, flo
     }
   }
   fft256<+1>(v, buf, tw, lane);
-
-  // fringe rotation: phase phi + i dphi on sample i = lane + 64 q, the same
-  // sincos-and-rotate scheme
-  const float famp = (float)bc.amp;
-  float sn, cs, sn64, cs64;
+  float sn, cs;
   sincosf((float)(bc.phi + lane * bc.dphi), &sn, &cs);
-  sincosf((float)(64.0 * bc.dphi), &sn64, &cs64);
   float2 r = make_float2(cs, sn);
-  const float2 rstep = make_float2(cs64, sn64);
+  const float2 rstep = make_float2(bc.fringe_step[0], bc.fringe_step[1]);
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int i = lane + 64 * q;
-    out[i] = famp * (v[q].x * r.x + v[q].y * r.y);
+    out[i] = bc.amp * (v[q].x * r.x + v[q].y * r.y);
     r = cmul(r, rstep);
   }
 }
 
-// k_delay: one wavefront per block, the real series to HBM (unfused path,
-// used for N_c above the fused F-engine's LDS budget and for verification).
+// Two N_d = 256 blocks (blk, blk + 1) of one stream by one wavefront
+// (unpack + integer shift, fractional_bit_shift, fringe_stopping). The two
+// real blocks share one complex forward FFT, z = a + i b, and are separated
+// by A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i. `buf`
+// is the wave's own LDS region; outb may be null when blk + 1 does not exist.
+__device__ __forceinline__ void delay_block_pair(const DevStream &s, int64_t blk, int64_t n_blocks, float2 *buf,
+                                                 const float2 *tw, int lane, float *outa, float *outb) {
+  float xa[4], xb[4];
+  load_block(s, blk, n_blocks, lane, xa);
+  load_block(s, blk + 1, outb ? n_blocks : 0, lane, xb);
+  float2 v[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) v[q] = make_float2(xa[q], xb[q]);
+  fft256<-1>(v, buf, tw, lane);  // buf keeps Z in natural order
+  float2 va[4], vb[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int k = lane + 64 * q;
+    const float2 zk = v[q], zn = buf[(256 - k) & 255];
+    va[q] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+    vb[q] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+  }
+  finish_block(s, blk, va, buf, tw, lane, outa);
+  if (outb) finish_block(s, blk + 1, vb, buf, tw, lane, outb);
+}
+
+// k_delay: one wavefront per pair of blocks, the real series to HBM
+// (unfused path, used for N_c above the fused F-engine's LDS budget and for
+// verification).
 __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ streams, DevJob job,
                                                const float2 *__restrict__ tw_g) {
   __shared__ float2 lds[4][256];
@@ -325,9 +358,10 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
   tw[threadIdx.x] = tw_g[threadIdx.x];
   __syncthreads();
   const DevStream &s = streams[blockIdx.y];
-  const int64_t blk = (int64_t)blockIdx.x * 4 + wave;
+  const int64_t blk = ((int64_t)blockIdx.x * 4 + wave) * 2;
   if (blk >= job.n_blocks) return;
-  delay_block(s, blk, lds[wave], tw, lane, s.time_series + blk * 256);
+  float *base = s.time_series + blk * 256;
+  delay_block_pair(s, blk, job.n_blocks, lds[wave], tw, lane, base, blk + 1 < job.n_blocks ? base + 256 : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -467,8 +501,11 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
   __syncthreads();
   const bool none = job.window == SFXG_WINDOW_NONE;
   float2 *scratch = fbuf + wave * 256;
+  // blocks go in pairs (blk, blk + 1) with blk even, as in k_delay
   if (!none) {
-    for (int b = wave; b < BPH; b += nwaves) delay_block(s, sa * BPH + b, scratch, tw, lane, ring + b * 256);
+    for (int b = 2 * wave; b < BPH; b += 2 * nwaves)
+      delay_block_pair(s, sa * BPH + b, job.n_blocks, scratch, tw, lane, ring + b * 256,
+                       b + 1 < BPH ? ring + (b + 1) * 256 : nullptr);
     __syncthreads();
   }
   const float2 *w2 = reinterpret_cast<const float2 *>(window);
@@ -476,8 +513,9 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
     const int h_old = (int)((seg - sa) & 1);
     const int h_new = none ? 0 : 1 - h_old;
     const int64_t blk0 = none ? seg * BPH : (seg + 1) * BPH;
-    for (int b = wave; b < BPH; b += nwaves)
-      delay_block(s, blk0 + b, scratch, tw, lane, ring + h_new * N + b * 256);
+    for (int b = 2 * wave; b < BPH; b += 2 * nwaves)
+      delay_block_pair(s, blk0 + b, job.n_blocks, scratch, tw, lane, ring + h_new * N + b * 256,
+                       b + 1 < BPH ? ring + h_new * N + (b + 1) * 256 : nullptr);
     __syncthreads();
     // window the segment's 2 N real samples into N complex points
     for (int m = threadIdx.x; m < N; m += T) {
@@ -810,7 +848,7 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
   hipLaunchKernelGGL(k_block_consts, gridc, dim3(256), 0, s, d_streams, job.n_blocks, job.nd);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  dim3 grid((unsigned)((job.n_blocks + 3) / 4), job.n_streams);
+  dim3 grid((unsigned)((job.n_blocks + 7) / 8), job.n_streams);
   hipLaunchKernelGGL(k_delay, grid, dim3(256), 0, s, d_streams, job, tw_nd);
   return hipGetLastError();
 }

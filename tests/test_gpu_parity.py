@@ -277,3 +277,27 @@ def test_config0_full(ctx):
     for job, hdrs in slices:
         vis_g, vis_r, _ = run_both(ctx, job, hdrs)
         compare_vis(vis_g[0], vis_r[0], label=f"config0 integration {job.integration_nr}")
+
+
+def test_cpp_host_adapter(ctx, c0_short, tmp_path):
+    """A C++ host (host/correlate_slice.cpp through host/sfxg_adapter.h)
+    produces the same integration_write bytes as the oracle's layout."""
+    import os
+    import subprocess
+    from sfxg import slicefile
+    exe = os.path.join(os.path.dirname(capi.LIB_PATH), "correlate_slice")
+    assert os.path.exists(exe), "build first: make -C sfxc-1_amd"
+    job, hdrs = c0_short[0]
+    sl = tmp_path / "slice.bin"
+    cor = tmp_path / "out.cor"
+    slicefile.write_slice(str(sl), job, hdrs)
+    r = subprocess.run([exe, str(sl), str(cor)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out_cpp = cor.read_bytes()
+    out_py = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    assert out_cpp == out_py
+    ref = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+    heads_g, vis_g = vis_from_bytes(job, out_cpp)
+    heads_r, vis_r = vis_from_bytes(job, ref["output"])
+    assert heads_g == heads_r
+    compare_vis(vis_g[0], vis_r[0], label="C++ host")
