@@ -46,9 +46,15 @@ extern "C" {
 #define SFXG_LUT_REFERENCE 0
 #define SFXG_LUT_CONFIG 1
 
-/* X-step arithmetic. FP32 is the reference's arithmetic
- * (src/correlation_core.cc:241-273). */
+/* X-step arithmetic. FP32 is the reference's arithmetic and order
+ * (src/correlation_core.cc:241-273). The others run the X-step as a
+ * per-channel Hermitian rank-k update on MFMA with FP32 accumulation:
+ * FP16 or BF16 operands, or FP16 operands split into hi + lo parts (three
+ * products). Not available with pulsar binning. */
 #define SFXG_PREC_FP32 0
+#define SFXG_PREC_FP16 1
+#define SFXG_PREC_BF16 2
+#define SFXG_PREC_SPLIT_FP16 3
 
 /* Stages readable with sfxg_fetch_stage (verification of each stage). */
 #define SFXG_STAGE_SAMPLES 0      /* Bit2float_worker output, float[n_samples] */

@@ -67,6 +67,7 @@ struct sfxg_ctx {
   hipStream_t stream = nullptr;
   sfxg_options opt{SFXG_LUT_REFERENCE, SFXG_PREC_FP32, 0, 0};
   bool fused = false;  // the last slice ran the fused F-engine
+  float mixed_scale = 1.0f;
   std::string err;
 
   // slice parameters
@@ -92,7 +93,7 @@ struct sfxg_ctx {
 
   // device buffers
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
-      flags, bl, fold_buf, partial, block_consts, model_buf, tw_nd, tw_n, tw_2n, window, offsets, samples;
+      flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, window, offsets, samples;
   // host staging of the enqueued slice (kept until the next slice)
   std::vector<double> h_model, h_offsets;
   std::vector<DevStream> h_streams;
@@ -169,7 +170,7 @@ void sfxg_destroy(sfxg_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DBuf *bufs[] = {&ctx->payload, &ctx->runs, &ctx->block_run, &ctx->counted, &ctx->dstreams,
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
-                  &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
+                  &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
                   &ctx->tw_n, &ctx->tw_2n, &ctx->window, &ctx->offsets, &ctx->samples};
   for (DBuf *b : bufs) b->release();
   if (ctx->done) (void)hipEventDestroy(ctx->done);
@@ -185,7 +186,7 @@ int sfxg_set_options(sfxg_ctx *ctx, const sfxg_options *opt) {
   if (!ctx || !opt) return SFXG_ERR_ARG;
   if (opt->lut != SFXG_LUT_REFERENCE && opt->lut != SFXG_LUT_CONFIG)
     return ctx->fail(SFXG_ERR_ARG, "unknown lut %d", opt->lut);
-  if (opt->precision != SFXG_PREC_FP32)
+  if (opt->precision < SFXG_PREC_FP32 || opt->precision > SFXG_PREC_SPLIT_FP16)
     return ctx->fail(SFXG_ERR_ARG, "unsupported precision %d", opt->precision);
   ctx->opt = *opt;
   return SFXG_OK;
@@ -482,7 +483,14 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   // sub-integration partial sums, at most ~1 GiB at a time
   const size_t part_one = (size_t)B * (nc + 1) * sizeof(float2);
   const int k_group = (int)std::max<size_t>(1, std::min<size_t>(ctx->folds.size(), ((size_t)1 << 30) / part_one));
-  if (!ctx->pulsar) HIPCHK(ctx, ctx->partial.ensure(part_one * k_group));
+  const bool mixed = ctx->opt.precision != SFXG_PREC_FP32;
+  if (mixed && ctx->pulsar)
+    return ctx->fail(SFXG_ERR_ARG, "the mixed-precision X-step does not do pulsar binning");
+  if (!ctx->pulsar && !mixed) HIPCHK(ctx, ctx->partial.ensure(part_one * k_group));
+  const int RT = (2 * M + 31) / 32;
+  const size_t gram_one = (size_t)(nc + 1) * (32 * RT) * (32 * RT) * sizeof(float);
+  const int k_group_m = (int)std::max<size_t>(1, std::min<size_t>(ctx->folds.size(), ((size_t)1 << 30) / gram_one));
+  if (mixed) HIPCHK(ctx, ctx->gram.ensure(gram_one * k_group_m));
 
   // host staging stays alive in the context until the next slice, so the
   // copies and kernels below run asynchronously
@@ -607,7 +615,33 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   ctx->fused = fused;
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_XSTEP], s));
   const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
-  if (!ctx->pulsar) {
+  if (mixed) {
+    // power-of-two scale that brings the spectra's rms near 16: the delay
+    // correction's unnormalised FFT pair gains N_d / 2, the channeliser
+    // sqrt(sum w^2); the decode table bounds the sample rms
+    double wsum = 0.0;
+    {
+      std::vector<float2> tnd, tn, t2n;
+      std::vector<float> win;
+      make_tables(ctx, tnd, tn, t2n, win);
+      for (float w : win) wsum += (double)w * w;
+    }
+    double lmax = 0.0;
+    for (int i = 0; i < M; i++) {
+      const int lut = ctx->opt.lut;
+      lmax = std::max(lmax, ctx->st[i].bits_per_sample == 2 ? fabs(kLut2[lut][0]) : fabs(kLut1[lut][0]));
+    }
+    const double rms = (nd / 2.0) * lmax * sqrt(std::max(wsum, 1.0));
+    const float scale = (float)ldexp(1.0, (int)lround(log2(16.0 / rms)));
+    ctx->mixed_scale = scale;
+    const int n_folds = (int)ctx->folds.size();
+    for (int g0 = 0; g0 < n_folds; g0 += k_group_m) {
+      const int ng = std::min(k_group_m, n_folds - g0);
+      HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S,
+                                    nc + 1, M, ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, scale,
+                                    ctx->gram.as<float>(), ctx->vis.as<float2>(), s));
+    }
+  } else if (!ctx->pulsar) {
     const int n_folds = (int)ctx->folds.size();
     for (int g0 = 0; g0 < n_folds; g0 += k_group) {
       const int ng = std::min(k_group, n_folds - g0);

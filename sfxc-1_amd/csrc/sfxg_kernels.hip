@@ -670,6 +670,160 @@ __global__ __launch_bounds__(256) void k_fold_combine(const float2 *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// k_xstep_mfma: mixed-precision X-step. For every channel the
+// correlation matrix of a sub-integration, V = sum_s x_s x_s^H over the M
+// streams' spectra, is a Hermitian rank-k update. With Y = [Re X; Im X]
+// (2M x k) it is the real Gram matrix G = Y Y^T:
+//   V[a][b] = G[a][b] + G[M+a][M+b] + i (G[M+a][b] - G[a][M+b]).
+// G is computed in 32 x 32 tiles (the upper triangle of the 2M x 2M matrix,
+// padded to R = 32 RT rows) on v_mfma_f32_32x32x16 with FP32 accumulation.
+// Operands are FP16, BF16, or FP16 split into hi + lo (three products,
+// hi.hi + hi.lo + lo.hi). Spectra are scaled by a power of two before the
+// FP16 conversion (exact; undone in k_gram_combine).
+//
+// Workgroup: 4 waves, CH channels, one sub-integration (grid z). Stages of
+// KS segments are converted into LDS as Y[c][r][k]; each wave owns
+// (channel, tile) pairs and keeps their accumulators in registers for the
+// whole sub-integration, then writes them to gram[z][ch][R][R].
+// ---------------------------------------------------------------------------
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+typedef float f16v_t __attribute__((ext_vector_type(16)));
+
+constexpr int MF_KS = 32;             // segments per LDS stage
+constexpr int MF_RS = MF_KS + 8;      // row stride in halves (16-byte aligned rows)
+constexpr int MF_MAXP = 8;            // (channel, tile) pairs per wave
+
+template <int PREC>  // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16
+__global__ __launch_bounds__(256) void k_xstep_mfma(const float2 *__restrict__ spectra, int64_t stream_stride,
+                                                    int64_t seg_stride, int n_ch, int M, int RT, int CH,
+                                                    const int32_t *__restrict__ fold_start, float scale,
+                                                    float *__restrict__ gram) {
+  extern __shared__ _Float16 ylds[];  // [planes][CH][R][MF_RS]; planes = 2 for split (hi, lo)
+  constexpr int planes = PREC == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
+  const int R = 32 * RT;
+  const int nT = RT * (RT + 1) / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ch0 = blockIdx.x * CH;
+  const int z = blockIdx.z;
+  const int64_t s_begin = fold_start[z], s_end = fold_start[z + 1];
+  const int plane_elems = CH * R * MF_RS;
+  // zero all of LDS once: padding rows and columns stay zero
+  for (int i = threadIdx.x; i < planes * plane_elems; i += blockDim.x) ylds[i] = (_Float16)0.0f;
+  // this wave's (channel, tile) pairs
+  int pc[MF_MAXP], pti[MF_MAXP], ptj[MF_MAXP];
+  int np = 0;
+  for (int p = wave; p < CH * nT && np < MF_MAXP; p += 4) {
+    const int c = p / nT;
+    int t = p - c * nT, ti = 0;
+    while (t >= RT - ti) { t -= RT - ti; ti++; }
+    pc[np] = c;
+    pti[np] = ti;
+    ptj[np] = ti + t;
+    np++;
+  }
+  f16v_t acc[MF_MAXP];
+#pragma unroll
+  for (int q = 0; q < MF_MAXP; q++)
+    for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
+  __syncthreads();
+  for (int64_t s0 = s_begin; s0 < s_end; s0 += MF_KS) {
+    const int ns = (int)((s_end - s0) < MF_KS ? (s_end - s0) : MF_KS);
+    // stage: M streams x MF_KS segments x CH channels -> Y[c][st|M+st][k]
+    const int total = M * MF_KS * CH;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      const int c = idx % CH;
+      const int rest = idx / CH;
+      const int k = rest % MF_KS;
+      const int st = rest / MF_KS;
+      float2 v = make_float2(0.0f, 0.0f);
+      if (k < ns && ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + k) * seg_stride + ch0 + c];
+      v.x *= scale;
+      v.y *= scale;
+      const int o_re = (c * R + st) * MF_RS + k, o_im = (c * R + M + st) * MF_RS + k;
+      if (PREC == SFXG_PREC_BF16) {
+        __bf16 *yb = reinterpret_cast<__bf16 *>(ylds);
+        yb[o_re] = (__bf16)v.x;
+        yb[o_im] = (__bf16)v.y;
+      } else {
+        const _Float16 hr = (_Float16)v.x, hi_ = (_Float16)v.y;
+        ylds[o_re] = hr;
+        ylds[o_im] = hi_;
+        if (PREC == SFXG_PREC_SPLIT_FP16) {
+          ylds[plane_elems + o_re] = (_Float16)(v.x - (float)hr);
+          ylds[plane_elems + o_im] = (_Float16)(v.y - (float)hi_);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MF_MAXP; q++) {
+      if (q < np) {
+        const int rowa = (pc[q] * R + pti[q] * 32 + (lane & 31)) * MF_RS;
+        const int rowb = (pc[q] * R + ptj[q] * 32 + (lane & 31)) * MF_RS;
+#pragma unroll
+        for (int kk = 0; kk < MF_KS; kk += 16) {
+          const int ko = kk + 8 * (lane >> 5);
+          if (PREC == SFXG_PREC_BF16) {
+            const b8_t a = *reinterpret_cast<const b8_t *>(ylds + rowa + ko);
+            const b8_t b = *reinterpret_cast<const b8_t *>(ylds + rowb + ko);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+          } else {
+            const h8_t a = *reinterpret_cast<const h8_t *>(ylds + rowa + ko);
+            const h8_t b = *reinterpret_cast<const h8_t *>(ylds + rowb + ko);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[q], 0, 0, 0);
+            if (PREC == SFXG_PREC_SPLIT_FP16) {
+              const h8_t al = *reinterpret_cast<const h8_t *>(ylds + plane_elems + rowa + ko);
+              const h8_t bl = *reinterpret_cast<const h8_t *>(ylds + plane_elems + rowb + ko);
+              acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl, acc[q], 0, 0, 0);
+              acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b, acc[q], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // D[8 i + 4 (lane / 32) + j][lane % 32] = acc[4 i + j]
+#pragma unroll
+  for (int q = 0; q < MF_MAXP; q++) {
+    if (q < np && ch0 + pc[q] < n_ch) {
+      float *g = gram + ((int64_t)z * n_ch + ch0 + pc[q]) * R * R;
+      const int col = ptj[q] * 32 + (lane & 31);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int row = pti[q] * 32 + 8 * i + 4 * (lane >> 5) + j;
+          g[(int64_t)row * R + col] = acc[q][4 * i + j];
+        }
+    }
+  }
+}
+
+// Baseline visibilities from the Gram matrices of n_k sub-integrations,
+// added to vis in fold order and unscaled by inv_scale2 (a power of two).
+__global__ __launch_bounds__(256) void k_gram_combine(const float *__restrict__ gram, int n_k, int n_ch, int M,
+                                                      int R, const int32_t *__restrict__ bl_first,
+                                                      const int32_t *__restrict__ bl_second, int n_baselines,
+                                                      float inv_scale2, float2 *__restrict__ vis) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_baselines * n_ch) return;
+  const int b = (int)(i / n_ch), ch = (int)(i - (int64_t)b * n_ch);
+  const int a = bl_first[b], c = bl_second[b];
+  auto G = [&](const float *g, int r, int q) { return r <= q ? g[(int64_t)r * R + q] : g[(int64_t)q * R + r]; };
+  float2 t = vis[i];
+  for (int k = 0; k < n_k; k++) {
+    const float *g = gram + ((int64_t)k * n_ch + ch) * R * R;
+    const float re = (G(g, a, c) + G(g, M + a, M + c)) * inv_scale2;
+    const float im = (G(g, M + a, c) - G(g, a, M + c)) * inv_scale2;
+    t.x += re;
+    t.y += im;
+  }
+  vis[i] = t;
+}
+
+// ---------------------------------------------------------------------------
 // k_xstep_pulsar: per segment, every channel's product goes to the bin its
 // pulse phase selects. Runs of equal bins accumulate in registers; at a bin
 // change the running sums are written back and the new bin's sums loaded,
@@ -930,6 +1084,47 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
   if (e != hipSuccess) return e;
   const int64_t n = (int64_t)n_baselines * n_ch;
   hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
+  return hipGetLastError();
+}
+
+hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+                             int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
+                             const int32_t *bl_second, int n_baselines, float scale, float *gram, float2 *vis,
+                             hipStream_t s) {
+  if (n_k <= 0) return hipSuccess;
+  const int RT = (2 * M + 31) / 32;
+  const int R = 32 * RT;
+  const int nT = RT * (RT + 1) / 2;
+  const int planes = precision == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
+  // channels per workgroup: all (channel, tile) pairs fit 4 waves x MF_MAXP
+  // and the stage fits 64 KiB of LDS
+  int CH = (4 * MF_MAXP) / nT;
+  while (CH > 1 && (size_t)planes * CH * R * MF_RS * 2 > 65536) CH--;
+  if (CH < 1 || nT > 4 * MF_MAXP) return hipErrorInvalidValue;
+  const size_t lds = (size_t)planes * CH * R * MF_RS * 2;
+  if (lds > 65536) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((n_ch + CH - 1) / CH), 1, (unsigned)n_k);
+  switch (precision) {
+    case SFXG_PREC_FP16:
+      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_FP16>, grid, dim3(256), lds, s, spectra, stream_stride, seg_stride,
+                         n_ch, M, RT, CH, fold_start, scale, gram);
+      break;
+    case SFXG_PREC_BF16:
+      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_BF16>, grid, dim3(256), lds, s, spectra, stream_stride, seg_stride,
+                         n_ch, M, RT, CH, fold_start, scale, gram);
+      break;
+    case SFXG_PREC_SPLIT_FP16:
+      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_SPLIT_FP16>, grid, dim3(256), lds, s, spectra, stream_stride,
+                         seg_stride, n_ch, M, RT, CH, fold_start, scale, gram);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t n = (int64_t)n_baselines * n_ch;
+  hipLaunchKernelGGL(k_gram_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gram, n_k, n_ch, M, R,
+                     bl_first, bl_second, n_baselines, 1.0f / (scale * scale), vis);
   return hipGetLastError();
 }
 

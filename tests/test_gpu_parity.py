@@ -301,3 +301,30 @@ def test_cpp_host_adapter(ctx, c0_short, tmp_path):
     heads_r, vis_r = vis_from_bytes(job, ref["output"])
     assert heads_g == heads_r
     compare_vis(vis_g[0], vis_r[0], label="C++ host")
+
+
+@pytest.mark.parametrize("prec,bar", [(1, 1e-3), (2, 1e-2), (3, 5e-5)])
+def test_mixed_precision_vs_fp64(ctx, prec, bar):
+    """Mixed-precision X-step (FP16, BF16, split-FP16 on MFMA with FP32
+    accumulation) against the FP64 oracle of the same pipeline: per product
+    max |dV| / rms |V| <= 1e-3 for FP16, 1e-2 for BF16
+    (8-bit mantissa), 5e-5 for split-FP16."""
+    job, hdrs = configs.build_slices(seed=12, n_stations=4, pols=["R", "L"], fs=32_000_000, bw=16_000_000,
+                                     nchan=1024, seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2,
+                                     sky_freq=5_000_000_000, offsets=[0.0, 1e-6, -2e-6, 3e-6],
+                                     rates=[0.0, 1e-7, 2e-7, -1e-7])[0]
+    ctx.set_options(lut=LUT_CONFIG, precision=prec)
+    ctx.set_parameters(job)
+    for i, h in enumerate(hdrs):
+        ctx.push_stream(i, h)
+    ctx.correlate()
+    out = ctx.output()
+    ctx.set_options(lut=LUT_CONFIG, precision=0)
+    r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
+    heads_g, vis_g = vis_from_bytes(job, out)
+    heads_r, vis_r = vis_from_bytes(job, r64["output"])
+    assert heads_g == heads_r
+    g, r = vis_g[0].astype(np.complex128), vis_r[0].astype(np.complex128)
+    err = max(np.max(np.abs(g[k] - r[k])) / np.sqrt(np.mean(np.abs(r[k]) ** 2)) for k in range(r.shape[0]))
+    print(f"precision {prec}: max normwise error vs FP64 oracle {err:.3e}")
+    assert err <= bar
