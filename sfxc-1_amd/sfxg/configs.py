@@ -118,10 +118,27 @@ def config1_jobs(subband: int, n_integrations: int = 10, t_int_usec: int = 1_000
     pols, 32 MHz at 64 Ms/s, sky 6.0 GHz + 32 MHz * subband, 2-bit, 4096
     channels, 1 s integrations, delay offsets U[-5, 5] ms and rates
     U[-3e-6, 3e-6] (seed 1). Returns (jobs, raw_start_ticks)."""
-    rng = np.random.default_rng(1)
+    return survey_jobs(seed=1, subband=subband, n_integrations=n_integrations, t_int_usec=t_int_usec,
+                       nchan=nchan, n_stations=n_stations, fs=64_000_000, bw=32_000_000,
+                       sky=6_000_000_000 + 32_000_000 * subband)
+
+
+def config2_jobs(subband: int, n_integrations: int = 8, t_int_usec: int = 500_000, nchan: int = 16384,
+                 n_stations: int = 32):
+    """Job descriptions of configuration 2 (no sample data): 32 stations x 2
+    pols, 64 MHz at 128 Ms/s, 2-bit, 16384 channels, 0.5 s integrations (4 s),
+    delay offsets U[-5, 5] ms and rates U[-3e-6, 3e-6] (seed 2; the
+    configuration leaves them and the sky frequency open: 6.0 GHz + 64 MHz *
+    subband). Returns (jobs, raw_start_ticks)."""
+    return survey_jobs(seed=2, subband=subband, n_integrations=n_integrations, t_int_usec=t_int_usec,
+                       nchan=nchan, n_stations=n_stations, fs=128_000_000, bw=64_000_000,
+                       sky=6_000_000_000 + 64_000_000 * subband)
+
+
+def survey_jobs(*, seed, subband, n_integrations, t_int_usec, nchan, n_stations, fs, bw, sky):
+    rng = np.random.default_rng(seed)
     offsets = rng.uniform(-5e-3, 5e-3, n_stations)
     rates = rng.uniform(-3e-6, 3e-6, n_stations)
-    fs, bw = 64_000_000, 32_000_000
     start = mjd_to_ticks(START_MJD, 0.0)
     tps = int(round(CLOCK_RATE / fs))
     margin = int(math.ceil(5.1e-3 * fs)) + 4096
@@ -134,7 +151,7 @@ def config1_jobs(subband: int, n_integrations: int = 10, t_int_usec: int = 1_000
     for s in range(n_stations):
         for p in ("R", "L"):
             streams.append(StationStream(station_number=s, station_stream=len(streams), sample_rate=fs,
-                                         channel_freq=6_000_000_000 + 32_000_000 * subband, bandwidth=bw,
+                                         channel_freq=sky, bandwidth=bw,
                                          sideband="U", polarisation=p, bits_per_sample=2))
             models.append(st_models[s])
     t_int_ticks = int(round(t_int_usec * (CLOCK_RATE / 1e6)))
@@ -145,7 +162,7 @@ def config1_jobs(subband: int, n_integrations: int = 10, t_int_usec: int = 1_000
             experiment_start=start, start_time=t0, stop_time=t0 + t_int_ticks, integration_time=t_int_ticks,
             sub_integration_time=int(round(min(t_int_usec, 125000) * (CLOCK_RATE / 1e6))),
             number_channels=nchan, fft_size_delaycor=256, fft_size_correlation=max(256, nchan), sample_rate=fs,
-            channel_freq=6_000_000_000 + 32_000_000 * subband, bandwidth=bw, sideband="U", frequency_nr=subband,
+            channel_freq=sky, bandwidth=bw, sideband="U", frequency_nr=subband,
             station_streams=streams, integration_nr=k, slice_nr=k, delay_models=models))
     return jobs, raw_start
 
