@@ -486,11 +486,7 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   const bool mixed = ctx->opt.precision != SFXG_PREC_FP32;
   if (mixed && ctx->pulsar)
     return ctx->fail(SFXG_ERR_ARG, "the mixed-precision X-step does not do pulsar binning");
-  if (!ctx->pulsar && !mixed) HIPCHK(ctx, ctx->partial.ensure(part_one * k_group));
-  const int RT = (2 * M + 31) / 32;
-  const size_t gram_one = (size_t)(nc + 1) * (32 * RT) * (32 * RT) * sizeof(float);
-  const int k_group_m = (int)std::max<size_t>(1, std::min<size_t>(ctx->folds.size(), ((size_t)1 << 30) / gram_one));
-  if (mixed) HIPCHK(ctx, ctx->gram.ensure(gram_one * k_group_m));
+  if (!ctx->pulsar) HIPCHK(ctx, ctx->partial.ensure(part_one * k_group));
 
   // host staging stays alive in the context until the next slice, so the
   // copies and kernels below run asynchronously
@@ -635,11 +631,11 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     const float scale = (float)ldexp(1.0, (int)lround(log2(16.0 / rms)));
     ctx->mixed_scale = scale;
     const int n_folds = (int)ctx->folds.size();
-    for (int g0 = 0; g0 < n_folds; g0 += k_group_m) {
-      const int ng = std::min(k_group_m, n_folds - g0);
+    for (int g0 = 0; g0 < n_folds; g0 += k_group) {
+      const int ng = std::min(k_group, n_folds - g0);
       HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S,
                                     nc + 1, M, ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, scale,
-                                    ctx->gram.as<float>(), ctx->vis.as<float2>(), s));
+                                    ctx->partial.as<float2>(), ctx->vis.as<float2>(), s));
     }
   } else if (!ctx->pulsar) {
     const int n_folds = (int)ctx->folds.size();

@@ -171,11 +171,11 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
                         const int32_t *fold_start, int n_k, const int32_t *bl_first, const int32_t *bl_second,
                         int n_baselines, float2 *partial, float2 *vis, hipStream_t s);
 // Mixed-precision X-step (SFXG_PREC_FP16 / BF16 / SPLIT_FP16): per-channel
-// Gram matrices on MFMA per sub-integration, combined into vis in fold
-// order. gram: n_k * n_ch * R * R floats, R = 32 ceil(2 M / 32).
+// Gram matrices on MFMA per sub-integration, formed into baselines in
+// partial (n_k * n_baselines * n_ch float2) and combined into vis in fold order.
 hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
-                             const int32_t *bl_second, int n_baselines, float scale, float *gram, float2 *vis,
+                             const int32_t *bl_second, int n_baselines, float scale, float2 *partial, float2 *vis,
                              hipStream_t s);
 hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
                                int n_ch, int64_t n_segments, const int32_t *bl_first,

@@ -371,11 +371,66 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
 // ---------------------------------------------------------------------------
 namespace {
 
+// 8-point DFT in place, exp(SIGN 2 pi i jk/8): a radix-2 layer, the W8^k
+// twiddles, and two 4-point DFTs giving the even and odd outputs.
+template <int SIGN>
+__device__ __forceinline__ void radix8(float2 (&a)[8]) {
+  constexpr float r = 0.70710678118654752f;
+  float2 b[4], c[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    b[k] = cadd(a[k], a[k + 4]);
+    c[k] = csub(a[k], a[k + 4]);
+  }
+  c[1] = cmul(c[1], make_float2(r, SIGN * r));
+  c[2] = mul_i<SIGN>(c[2]);
+  c[3] = cmul(c[3], make_float2(-r, SIGN * r));
+  radix4<SIGN>(b[0], b[1], b[2], b[3]);
+  radix4<SIGN>(c[0], c[1], c[2], c[3]);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    a[2 * m] = b[m];
+    a[2 * m + 1] = c[m];
+  }
+}
+
+// In-place Stockham FFT of N points held in LDS by T threads: radix-8
+// stages, then one radix-4 or radix-2 stage for the remaining factor. Every
+// thread loads its butterflies' inputs into registers before the barrier and
+// writes the outputs after it, so one buffer serves as source and sink.
 template <int N, int T, int SIGN>
 __device__ __forceinline__ void lds_fft(float2 *lds, const float2 *tw) {
-  constexpr int NB4 = (N / 4 + T - 1) / T;
   int ns = 1;
-  for (; ns * 4 <= N; ns *= 4) {
+  for (; ns * 8 <= N; ns *= 8) {
+    constexpr int NB = (N / 8 + T - 1) / T;
+    float2 v[NB][8];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int j = threadIdx.x + b * T;
+      if (j < N / 8) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[b][r] = lds[j + r * (N / 8)];
+      }
+    }
+    __syncthreads();
+    const int step = N / (ns * 8);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int j = threadIdx.x + b * T;
+      if (j < N / 8) {
+        const int k = j % ns;
+#pragma unroll
+        for (int r = 1; r < 8; r++) v[b][r] = cmul(v[b][r], twiddle<SIGN>(tw, r * k * step));
+        radix8<SIGN>(v[b]);
+        const int idx = (j / ns) * ns * 8 + k;
+#pragma unroll
+        for (int r = 0; r < 8; r++) lds[idx + r * ns] = v[b][r];
+      }
+    }
+    __syncthreads();
+  }
+  if (ns * 4 <= N) {  // radix-4 stage
+    constexpr int NB4 = (N / 4 + T - 1) / T;
     float2 v[NB4][4];
 #pragma unroll
     for (int b = 0; b < NB4; b++) {
@@ -404,6 +459,7 @@ __device__ __forceinline__ void lds_fft(float2 *lds, const float2 *tw) {
       }
     }
     __syncthreads();
+    ns *= 4;
   }
   if (ns < N) {  // final radix-2 stage
     constexpr int NB2 = (N / 2 + T - 1) / T;
@@ -679,27 +735,34 @@ __global__ __launch_bounds__(256) void k_fold_combine(const float2 *__restrict__
 // padded to R = 32 RT rows) on v_mfma_f32_32x32x16 with FP32 accumulation.
 // Operands are FP16, BF16, or FP16 split into hi + lo (three products,
 // hi.hi + hi.lo + lo.hi). Spectra are scaled by a power of two before the
-// FP16 conversion (exact; undone in k_gram_combine).
+// conversion (exact; undone when the baselines are formed).
 //
-// Workgroup: 4 waves, CH channels, one sub-integration (grid z). Stages of
-// KS segments are converted into LDS as Y[c][r][k]; each wave owns
-// (channel, tile) pairs and keeps their accumulators in registers for the
-// whole sub-integration, then writes them to gram[z][ch][R][R].
+// Workgroup: 8 waves, CH channels, one sub-integration (grid z). Stages of
+// MF_KS segments are loaded with coalesced reads (consecutive threads take
+// consecutive channels, each thread 8 segments), converted, and stored into
+// LDS as Y[c][r][k] with 16-byte stores (a per-channel skew keeps them
+// conflict-free). Each wave owns up to MF_MAXP (channel, tile) pairs and keeps
+// their accumulators for the whole sub-integration. At its end the tiles of
+// each channel go through LDS and the workgroup forms that channel's
+// baseline visibilities into partial[z][b][ch]; k_fold_combine adds the
+// sub-integrations in fold order.
 // ---------------------------------------------------------------------------
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 typedef float f16v_t __attribute__((ext_vector_type(16)));
 
-constexpr int MF_KS = 32;             // segments per LDS stage
-constexpr int MF_RS = MF_KS + 8;      // row stride in halves (16-byte aligned rows)
-constexpr int MF_MAXP = 8;            // (channel, tile) pairs per wave
+constexpr int MF_KS = 16;          // segments per LDS stage (one K = 16 MFMA step)
+constexpr int MF_RS = MF_KS + 8;   // row stride in halves (rows 16-byte aligned)
+constexpr int MF_MAXP = 10;        // (channel, tile) pairs per wave
+constexpr int MF_THREADS = 512;
+constexpr int MF_WAVES = MF_THREADS / 64;
 
 template <int PREC>  // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16
-__global__ __launch_bounds__(256) void k_xstep_mfma(const float2 *__restrict__ spectra, int64_t stream_stride,
-                                                    int64_t seg_stride, int n_ch, int M, int RT, int CH,
-                                                    const int32_t *__restrict__ fold_start, float scale,
-                                                    float *__restrict__ gram) {
-  extern __shared__ _Float16 ylds[];  // [planes][CH][R][MF_RS]; planes = 2 for split (hi, lo)
+__global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT,
+    int CH, const int32_t *__restrict__ fold_start, float scale, const int32_t *__restrict__ bl_first,
+    const int32_t *__restrict__ bl_second, int n_baselines, float inv_scale2, float2 *__restrict__ partial) {
+  extern __shared__ _Float16 ylds[];
   constexpr int planes = PREC == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
   const int R = 32 * RT;
   const int nT = RT * (RT + 1) / 2;
@@ -707,120 +770,132 @@ __global__ __launch_bounds__(256) void k_xstep_mfma(const float2 *__restrict__ s
   const int ch0 = blockIdx.x * CH;
   const int z = blockIdx.z;
   const int64_t s_begin = fold_start[z], s_end = fold_start[z + 1];
-  const int plane_elems = CH * R * MF_RS;
-  // zero all of LDS once: padding rows and columns stay zero
-  for (int i = threadIdx.x; i < planes * plane_elems; i += blockDim.x) ylds[i] = (_Float16)0.0f;
-  // this wave's (channel, tile) pairs
-  int pc[MF_MAXP], pti[MF_MAXP], ptj[MF_MAXP];
-  int np = 0;
-  for (int p = wave; p < CH * nT && np < MF_MAXP; p += 4) {
-    const int c = p / nT;
-    int t = p - c * nT, ti = 0;
-    while (t >= RT - ti) { t -= RT - ti; ti++; }
-    pc[np] = c;
-    pti[np] = ti;
-    ptj[np] = ti + t;
-    np++;
-  }
+  const int chan_stride = R * MF_RS + 8;  // halves, skewed by 16 bytes per channel
+  const int plane_elems = CH * chan_stride;
+  for (int i = threadIdx.x; i < planes * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
   f16v_t acc[MF_MAXP];
 #pragma unroll
   for (int q = 0; q < MF_MAXP; q++)
+#pragma unroll
     for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
   __syncthreads();
+  const int n_pairs = CH * nT;
+  // loader: thread -> (channel, segment group of 8, stream)
+  const int n_items = CH * (MF_KS / 8) * M;
   for (int64_t s0 = s_begin; s0 < s_end; s0 += MF_KS) {
-    const int ns = (int)((s_end - s0) < MF_KS ? (s_end - s0) : MF_KS);
-    // stage: M streams x MF_KS segments x CH channels -> Y[c][st|M+st][k]
-    const int total = M * MF_KS * CH;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int c = idx % CH;
-      const int rest = idx / CH;
-      const int k = rest % MF_KS;
-      const int st = rest / MF_KS;
-      float2 v = make_float2(0.0f, 0.0f);
-      if (k < ns && ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + k) * seg_stride + ch0 + c];
-      v.x *= scale;
-      v.y *= scale;
-      const int o_re = (c * R + st) * MF_RS + k, o_im = (c * R + M + st) * MF_RS + k;
+    for (int item = threadIdx.x; item < n_items; item += MF_THREADS) {
+      const int c = item % CH;
+      const int rest = item / CH;
+      const int kg = rest % (MF_KS / 8);
+      const int st = rest / (MF_KS / 8);
+      float re[8], im[8];
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) {
+        const int64_t seg = s0 + kg * 8 + kk;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (seg < s_end && ch0 + c < n_ch) v = spectra[st * stream_stride + seg * seg_stride + ch0 + c];
+        re[kk] = v.x * scale;
+        im[kk] = v.y * scale;
+      }
+      const int o_re = c * chan_stride + st * MF_RS + kg * 8;
+      const int o_im = c * chan_stride + (M + st) * MF_RS + kg * 8;
       if (PREC == SFXG_PREC_BF16) {
-        __bf16 *yb = reinterpret_cast<__bf16 *>(ylds);
-        yb[o_re] = (__bf16)v.x;
-        yb[o_im] = (__bf16)v.y;
+        b8_t br, bi;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          br[kk] = (__bf16)re[kk];
+          bi[kk] = (__bf16)im[kk];
+        }
+        *reinterpret_cast<b8_t *>(ylds + o_re) = br;
+        *reinterpret_cast<b8_t *>(ylds + o_im) = bi;
       } else {
-        const _Float16 hr = (_Float16)v.x, hi_ = (_Float16)v.y;
-        ylds[o_re] = hr;
-        ylds[o_im] = hi_;
+        h8_t hr, hi_;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          hr[kk] = (_Float16)re[kk];
+          hi_[kk] = (_Float16)im[kk];
+        }
+        *reinterpret_cast<h8_t *>(ylds + o_re) = hr;
+        *reinterpret_cast<h8_t *>(ylds + o_im) = hi_;
         if (PREC == SFXG_PREC_SPLIT_FP16) {
-          ylds[plane_elems + o_re] = (_Float16)(v.x - (float)hr);
-          ylds[plane_elems + o_im] = (_Float16)(v.y - (float)hi_);
+          h8_t lr, li;
+#pragma unroll
+          for (int kk = 0; kk < 8; kk++) {
+            lr[kk] = (_Float16)(re[kk] - (float)hr[kk]);
+            li[kk] = (_Float16)(im[kk] - (float)hi_[kk]);
+          }
+          *reinterpret_cast<h8_t *>(ylds + plane_elems + o_re) = lr;
+          *reinterpret_cast<h8_t *>(ylds + plane_elems + o_im) = li;
         }
       }
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MF_MAXP; q++) {
-      if (q < np) {
-        const int rowa = (pc[q] * R + pti[q] * 32 + (lane & 31)) * MF_RS;
-        const int rowb = (pc[q] * R + ptj[q] * 32 + (lane & 31)) * MF_RS;
-#pragma unroll
-        for (int kk = 0; kk < MF_KS; kk += 16) {
-          const int ko = kk + 8 * (lane >> 5);
-          if (PREC == SFXG_PREC_BF16) {
-            const b8_t a = *reinterpret_cast<const b8_t *>(ylds + rowa + ko);
-            const b8_t b = *reinterpret_cast<const b8_t *>(ylds + rowb + ko);
-            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
-          } else {
-            const h8_t a = *reinterpret_cast<const h8_t *>(ylds + rowa + ko);
-            const h8_t b = *reinterpret_cast<const h8_t *>(ylds + rowb + ko);
-            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[q], 0, 0, 0);
-            if (PREC == SFXG_PREC_SPLIT_FP16) {
-              const h8_t al = *reinterpret_cast<const h8_t *>(ylds + plane_elems + rowa + ko);
-              const h8_t bl = *reinterpret_cast<const h8_t *>(ylds + plane_elems + rowb + ko);
-              acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl, acc[q], 0, 0, 0);
-              acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b, acc[q], 0, 0, 0);
-            }
+      const int p = wave + MF_WAVES * q;
+      if (p < n_pairs) {
+        const int c = p / nT;
+        int t = p - c * nT, ti = 0;
+        while (t >= RT - ti) {
+          t -= RT - ti;
+          ti++;
+        }
+        const int tj = ti + t;
+        const int ko = 8 * (lane >> 5);
+        const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * MF_RS + ko;
+        const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * MF_RS + ko;
+        if (PREC == SFXG_PREC_BF16) {
+          const b8_t a = *reinterpret_cast<const b8_t *>(ylds + oa);
+          const b8_t b = *reinterpret_cast<const b8_t *>(ylds + ob);
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+        } else {
+          const h8_t a = *reinterpret_cast<const h8_t *>(ylds + oa);
+          const h8_t b = *reinterpret_cast<const h8_t *>(ylds + ob);
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[q], 0, 0, 0);
+          if (PREC == SFXG_PREC_SPLIT_FP16) {
+            const h8_t al = *reinterpret_cast<const h8_t *>(ylds + plane_elems + oa);
+            const h8_t bl = *reinterpret_cast<const h8_t *>(ylds + plane_elems + ob);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl, acc[q], 0, 0, 0);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b, acc[q], 0, 0, 0);
           }
         }
       }
     }
     __syncthreads();
   }
-  // D[8 i + 4 (lane / 32) + j][lane % 32] = acc[4 i + j]
+  // baselines, one channel at a time through an R x R FP32 buffer in LDS
+  float *g = reinterpret_cast<float *>(ylds);
+  for (int c = 0; c < CH; c++) {
 #pragma unroll
-  for (int q = 0; q < MF_MAXP; q++) {
-    if (q < np && ch0 + pc[q] < n_ch) {
-      float *g = gram + ((int64_t)z * n_ch + ch0 + pc[q]) * R * R;
-      const int col = ptj[q] * 32 + (lane & 31);
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int row = pti[q] * 32 + 8 * i + 4 * (lane >> 5) + j;
-          g[(int64_t)row * R + col] = acc[q][4 * i + j];
+    for (int q = 0; q < MF_MAXP; q++) {
+      const int p = wave + MF_WAVES * q;
+      if (p < n_pairs && p / nT == c) {
+        int t = p - c * nT, ti = 0;
+        while (t >= RT - ti) {
+          t -= RT - ti;
+          ti++;
         }
+        const int tj = ti + t;
+        const int col = tj * 32 + (lane & 31);
+        // D[8 i + 4 (lane / 32) + j][lane % 32] = acc[4 i + j]
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) g[(ti * 32 + 8 * i + 4 * (lane >> 5) + j) * R + col] = acc[q][4 * i + j];
+      }
     }
+    __syncthreads();
+    if (ch0 + c < n_ch) {
+      for (int b = threadIdx.x; b < n_baselines; b += MF_THREADS) {
+        const int a = bl_first[b], d = bl_second[b];
+        auto G = [&](int r, int s2) { return r <= s2 ? g[r * R + s2] : g[s2 * R + r]; };
+        const float re = (G(a, d) + G(M + a, M + d)) * inv_scale2;
+        const float im = (G(M + a, d) - G(a, M + d)) * inv_scale2;
+        partial[((int64_t)z * n_baselines + b) * n_ch + ch0 + c] = make_float2(re, im);
+      }
+    }
+    __syncthreads();
   }
-}
-
-// Baseline visibilities from the Gram matrices of n_k sub-integrations,
-// added to vis in fold order and unscaled by inv_scale2 (a power of two).
-__global__ __launch_bounds__(256) void k_gram_combine(const float *__restrict__ gram, int n_k, int n_ch, int M,
-                                                      int R, const int32_t *__restrict__ bl_first,
-                                                      const int32_t *__restrict__ bl_second, int n_baselines,
-                                                      float inv_scale2, float2 *__restrict__ vis) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)n_baselines * n_ch) return;
-  const int b = (int)(i / n_ch), ch = (int)(i - (int64_t)b * n_ch);
-  const int a = bl_first[b], c = bl_second[b];
-  auto G = [&](const float *g, int r, int q) { return r <= q ? g[(int64_t)r * R + q] : g[(int64_t)q * R + r]; };
-  float2 t = vis[i];
-  for (int k = 0; k < n_k; k++) {
-    const float *g = gram + ((int64_t)k * n_ch + ch) * R * R;
-    const float re = (G(g, a, c) + G(g, M + a, M + c)) * inv_scale2;
-    const float im = (G(g, M + a, c) - G(g, a, M + c)) * inv_scale2;
-    t.x += re;
-    t.y += im;
-  }
-  vis[i] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,42 +1164,61 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
 
 hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
-                             const int32_t *bl_second, int n_baselines, float scale, float *gram, float2 *vis,
+                             const int32_t *bl_second, int n_baselines, float scale, float2 *partial, float2 *vis,
                              hipStream_t s) {
   if (n_k <= 0) return hipSuccess;
   const int RT = (2 * M + 31) / 32;
   const int R = 32 * RT;
   const int nT = RT * (RT + 1) / 2;
   const int planes = precision == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
-  // channels per workgroup: all (channel, tile) pairs fit 4 waves x MF_MAXP
-  // and the stage fits 64 KiB of LDS
-  int CH = (4 * MF_MAXP) / nT;
-  while (CH > 1 && (size_t)planes * CH * R * MF_RS * 2 > 65536) CH--;
-  if (CH < 1 || nT > 4 * MF_MAXP) return hipErrorInvalidValue;
-  const size_t lds = (size_t)planes * CH * R * MF_RS * 2;
-  if (lds > 65536) return hipErrorInvalidValue;
+  // channels per workgroup: all (channel, tile) pairs fit MF_WAVES waves x
+  // MF_MAXP and the stage fits 150 KiB of LDS
+  int CH = (MF_WAVES * MF_MAXP) / nT;
+  if (CH > 64) CH = 64;
+  auto lds_of = [&](int ch) {
+    const size_t stage = (size_t)planes * ch * (R * MF_RS + 8) * 2;
+    const size_t gbuf = (size_t)R * R * 4;
+    return stage > gbuf ? stage : gbuf;
+  };
+  while (CH > 1 && lds_of(CH) > 150 * 1024) CH--;
+  if (CH < 1 || nT > MF_WAVES * MF_MAXP) return hipErrorInvalidValue;
+  const size_t lds = lds_of(CH);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
   dim3 grid((unsigned)((n_ch + CH - 1) / CH), 1, (unsigned)n_k);
+  const float inv2 = 1.0f / (scale * scale);
+  hipError_t e;
   switch (precision) {
     case SFXG_PREC_FP16:
-      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_FP16>, grid, dim3(256), lds, s, spectra, stream_stride, seg_stride,
-                         n_ch, M, RT, CH, fold_start, scale, gram);
+      e = hipFuncSetAttribute((const void *)k_xstep_mfma<SFXG_PREC_FP16>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_FP16>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride,
+                         seg_stride, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2,
+                         partial);
       break;
     case SFXG_PREC_BF16:
-      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_BF16>, grid, dim3(256), lds, s, spectra, stream_stride, seg_stride,
-                         n_ch, M, RT, CH, fold_start, scale, gram);
+      e = hipFuncSetAttribute((const void *)k_xstep_mfma<SFXG_PREC_BF16>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_BF16>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride,
+                         seg_stride, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2,
+                         partial);
       break;
     case SFXG_PREC_SPLIT_FP16:
-      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_SPLIT_FP16>, grid, dim3(256), lds, s, spectra, stream_stride,
-                         seg_stride, n_ch, M, RT, CH, fold_start, scale, gram);
+      e = hipFuncSetAttribute((const void *)k_xstep_mfma<SFXG_PREC_SPLIT_FP16>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_SPLIT_FP16>, grid, dim3(MF_THREADS), lds, s, spectra,
+                         stream_stride, seg_stride, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second,
+                         n_baselines, inv2, partial);
       break;
     default:
       return hipErrorInvalidValue;
   }
-  hipError_t e = hipGetLastError();
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int64_t n = (int64_t)n_baselines * n_ch;
-  hipLaunchKernelGGL(k_gram_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gram, n_k, n_ch, M, R,
-                     bl_first, bl_second, n_baselines, 1.0f / (scale * scale), vis);
+  hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
   return hipGetLastError();
 }
 
