@@ -365,145 +365,191 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
 }
 
 // ---------------------------------------------------------------------------
-// k_channelise: one workgroup per (segment, stream). Real 2N input is packed
-// as N complex points, transformed by an LDS-resident radix-4 (+ radix-2)
-// Stockham FFT and split into the N + 1 bins of the real transform.
+// Register-resident Stockham FFT. T threads transform N = P * T complex
+// points; thread t holds points t + i T (i < P) in registers. Each stage of
+// radix R (R divides P) does P / R butterflies per thread straight from the
+// registers, writes its outputs to LDS in Stockham order and reads the next
+// stage's inputs back in the same t + i T layout, so a stage costs one LDS
+// write and one read. The LDS buffer is padded by one element every P
+// (lds_index), which keeps the stride-R writes of the early stages free of
+// bank conflicts. Stages: radix P while it fits, then one radix N / P^k.
 // ---------------------------------------------------------------------------
 namespace {
 
-// 8-point DFT in place, exp(SIGN 2 pi i jk/8): a radix-2 layer, the W8^k
-// twiddles, and two 4-point DFTs giving the even and odd outputs.
-template <int SIGN>
-__device__ __forceinline__ void radix8(float2 (&a)[8]) {
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// Returns x unchanged but hides it from the optimiser, so that table loads
+// indexed by it stay inside a loop instead of being hoisted out of it and
+// holding registers across the loop body.
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// Padded LDS position of FFT point i for a P-point-per-thread transform.
+template <int P>
+__device__ __forceinline__ int lds_index(int i) {
+  return i + (i >> ilog2(P));
+}
+
+// LDS elements (float2) a padded N-point buffer needs.
+template <int N, int P>
+constexpr int fft_lds_size() {
+  return N + N / P;
+}
+
+// R-point DFT in place, natural order in and out, exp(SIGN 2 pi i jk / R).
+template <int R, int SIGN>
+__device__ __forceinline__ void dft(float2 (&u)[R]);
+
+template <>
+__device__ __forceinline__ void dft<2, -1>(float2 (&u)[2]) {
+  const float2 a = u[0];
+  u[0] = cadd(a, u[1]);
+  u[1] = csub(a, u[1]);
+}
+template <>
+__device__ __forceinline__ void dft<4, -1>(float2 (&u)[4]) {
+  radix4<-1>(u[0], u[1], u[2], u[3]);
+}
+// radix-2 layer, the W8^k twiddles, two 4-point DFTs for even and odd outputs
+template <>
+__device__ __forceinline__ void dft<8, -1>(float2 (&u)[8]) {
   constexpr float r = 0.70710678118654752f;
   float2 b[4], c[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    b[k] = cadd(a[k], a[k + 4]);
-    c[k] = csub(a[k], a[k + 4]);
+    b[k] = cadd(u[k], u[k + 4]);
+    c[k] = csub(u[k], u[k + 4]);
   }
-  c[1] = cmul(c[1], make_float2(r, SIGN * r));
-  c[2] = mul_i<SIGN>(c[2]);
-  c[3] = cmul(c[3], make_float2(-r, SIGN * r));
-  radix4<SIGN>(b[0], b[1], b[2], b[3]);
-  radix4<SIGN>(c[0], c[1], c[2], c[3]);
+  c[1] = cmul(c[1], make_float2(r, -r));
+  c[2] = mul_i<-1>(c[2]);
+  c[3] = cmul(c[3], make_float2(-r, -r));
+  radix4<-1>(b[0], b[1], b[2], b[3]);
+  radix4<-1>(c[0], c[1], c[2], c[3]);
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    a[2 * m] = b[m];
-    a[2 * m + 1] = c[m];
+    u[2 * m] = b[m];
+    u[2 * m + 1] = c[m];
+  }
+}
+// 4 x 4 decomposition: n = 4 n1 + n2, m = m1 + 4 m2; 4-point DFTs over n1,
+// the W16^(n2 m1) twiddles, 4-point DFTs over n2.
+template <>
+__device__ __forceinline__ void dft<16, -1>(float2 (&u)[16]) {
+  constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508977f, r = 0.70710678118654752f;
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) radix4<-1>(u[n2], u[n2 + 4], u[n2 + 8], u[n2 + 12]);
+  // u[n2 + 4 m1] *= W16^(n2 m1), W16 = exp(-2 pi i / 16)
+  u[5] = cmul(u[5], make_float2(c1, -s1));
+  u[9] = cmul(u[9], make_float2(r, -r));
+  u[13] = cmul(u[13], make_float2(s1, -c1));
+  u[6] = cmul(u[6], make_float2(r, -r));
+  u[10] = mul_i<-1>(u[10]);
+  u[14] = cmul(u[14], make_float2(-r, -r));
+  u[7] = cmul(u[7], make_float2(s1, -c1));
+  u[11] = cmul(u[11], make_float2(-r, -r));
+  u[15] = cmul(u[15], make_float2(-c1, s1));
+  float2 x[16];
+#pragma unroll
+  for (int m1 = 0; m1 < 4; m1++) {
+    radix4<-1>(u[4 * m1], u[4 * m1 + 1], u[4 * m1 + 2], u[4 * m1 + 3]);
+#pragma unroll
+    for (int m2 = 0; m2 < 4; m2++) x[m1 + 4 * m2] = u[4 * m1 + m2];
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) u[k] = x[k];
+}
+
+// One radix-R stage over the registers of all threads; ns = product of the
+// earlier radices. Outputs go to LDS.
+template <int N, int P, int R>
+__device__ __forceinline__ void fft_stage(float2 (&v)[P], float2 *lds, const float2 *__restrict__ tw, int ns) {
+  constexpr int T = N / P, B = P / R;
+  const int lns = __builtin_ctz(ns);
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const int j = threadIdx.x + b * T;
+    const int k = opaque(j) & (ns - 1);
+    float2 u[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) u[q] = v[b + q * B];
+    if (ns > 1) {
+      const int step = N / (ns * R);
+#pragma unroll
+      for (int q = 1; q < R; q++) u[q] = cmul(u[q], tw[q * k * step]);
+    }
+    dft<R, -1>(u);
+    const int idx = ((j >> lns) << (lns + ilog2(R))) + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) lds[lds_index<P>(idx + r * ns)] = u[r];
   }
 }
 
-// In-place Stockham FFT of N points held in LDS by T threads: radix-8
-// stages, then one radix-4 or radix-2 stage for the remaining factor. Every
-// thread loads its butterflies' inputs into registers before the barrier and
-// writes the outputs after it, so one buffer serves as source and sink.
-template <int N, int T, int SIGN>
-__device__ __forceinline__ void lds_fft(float2 *lds, const float2 *tw) {
+// Forward FFT (exp(-2 pi i jk / N)) of the points in v, result in padded LDS
+// (point k at lds_index<P>(k)). The caller guarantees that no thread still
+// reads `lds` on entry; the function ends with a barrier.
+template <int N, int P>
+__device__ __forceinline__ void reg_fft(float2 (&v)[P], float2 *lds, const float2 *__restrict__ tw) {
+  constexpr int T = N / P;
+  constexpr int NS = ilog2(N) / ilog2(P);               // full radix-P stages
+  constexpr int F = 1 << (ilog2(N) - NS * ilog2(P));    // last radix, 1 if none
+  static_assert(P * T == N && (P & (P - 1)) == 0 && P >= 2 && P <= 16, "bad FFT shape");
+  static_assert(P % F == 0, "the last radix must divide the points per thread");
   int ns = 1;
-  for (; ns * 8 <= N; ns *= 8) {
-    constexpr int NB = (N / 8 + T - 1) / T;
-    float2 v[NB][8];
 #pragma unroll
-    for (int b = 0; b < NB; b++) {
-      const int j = threadIdx.x + b * T;
-      if (j < N / 8) {
-#pragma unroll
-        for (int r = 0; r < 8; r++) v[b][r] = lds[j + r * (N / 8)];
-      }
-    }
+  for (int st = 0; st < NS; st++) {
+    fft_stage<N, P, P>(v, lds, tw, ns);
+    ns *= P;
     __syncthreads();
-    const int step = N / (ns * 8);
+    if (st + 1 < NS || F > 1) {
 #pragma unroll
-    for (int b = 0; b < NB; b++) {
-      const int j = threadIdx.x + b * T;
-      if (j < N / 8) {
-        const int k = j % ns;
-#pragma unroll
-        for (int r = 1; r < 8; r++) v[b][r] = cmul(v[b][r], twiddle<SIGN>(tw, r * k * step));
-        radix8<SIGN>(v[b]);
-        const int idx = (j / ns) * ns * 8 + k;
-#pragma unroll
-        for (int r = 0; r < 8; r++) lds[idx + r * ns] = v[b][r];
-      }
+      for (int i = 0; i < P; i++) v[i] = lds[lds_index<P>(threadIdx.x + i * T)];
+      __syncthreads();
     }
+  }
+  if constexpr (F > 1) {
+    fft_stage<N, P, F>(v, lds, tw, ns);
     __syncthreads();
   }
-  if (ns * 4 <= N) {  // radix-4 stage
-    constexpr int NB4 = (N / 4 + T - 1) / T;
-    float2 v[NB4][4];
-#pragma unroll
-    for (int b = 0; b < NB4; b++) {
-      const int j = threadIdx.x + b * T;
-      if (j < N / 4) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) v[b][r] = lds[j + r * (N / 4)];
-      }
-    }
-    __syncthreads();
-    const int step = N / (ns * 4);
-#pragma unroll
-    for (int b = 0; b < NB4; b++) {
-      const int j = threadIdx.x + b * T;
-      if (j < N / 4) {
-        const int k = j % ns;
-        v[b][1] = cmul(v[b][1], twiddle<SIGN>(tw, k * step));
-        v[b][2] = cmul(v[b][2], twiddle<SIGN>(tw, 2 * k * step));
-        v[b][3] = cmul(v[b][3], twiddle<SIGN>(tw, 3 * k * step));
-        radix4<SIGN>(v[b][0], v[b][1], v[b][2], v[b][3]);
-        const int idx = (j / ns) * ns * 4 + k;
-        lds[idx] = v[b][0];
-        lds[idx + ns] = v[b][1];
-        lds[idx + 2 * ns] = v[b][2];
-        lds[idx + 3 * ns] = v[b][3];
-      }
-    }
-    __syncthreads();
-    ns *= 4;
-  }
-  if (ns < N) {  // final radix-2 stage
-    constexpr int NB2 = (N / 2 + T - 1) / T;
-    float2 v[NB2][2];
-#pragma unroll
-    for (int b = 0; b < NB2; b++) {
-      const int j = threadIdx.x + b * T;
-      if (j < N / 2) {
-        v[b][0] = lds[j];
-        v[b][1] = lds[j + N / 2];
-      }
-    }
-    __syncthreads();
-    const int step = N / (ns * 2);
-#pragma unroll
-    for (int b = 0; b < NB2; b++) {
-      const int j = threadIdx.x + b * T;
-      if (j < N / 2) {
-        const int k = j % ns;
-        const float2 w = twiddle<SIGN>(tw, k * step);
-        const float2 t = cmul(v[b][1], w);
-        const int idx = (j / ns) * ns * 2 + k;
-        lds[idx] = cadd(v[b][0], t);
-        lds[idx + ns] = csub(v[b][0], t);
-      }
-    }
-    __syncthreads();
+}
+
+// Split of the N-point complex transform of 2N packed reals into bins
+// k = 0..N of the real transform: X_k = E_k + W_2N^k O_k.
+template <int N, int P, int T>
+__device__ __forceinline__ void real_split(const float2 *lds, const float2 *__restrict__ tw_2n, float2 *out) {
+  for (int k = threadIdx.x; k <= N; k += T) {
+    const float2 zk = lds[lds_index<P>(k & (N - 1))], zn = lds[lds_index<P>((N - k) & (N - 1))];
+    const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+    const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+    out[k] = cadd(e, cmul(tw_2n[k], o));
   }
 }
 
 }  // namespace
 
-template <int N, int T>
-__global__ __launch_bounds__(T) void k_channelise(const DevStream *__restrict__ streams, DevJob job,
-                                                  const float *__restrict__ window,
-                                                  const float2 *__restrict__ tw_n,
-                                                  const float2 *__restrict__ tw_2n) {
-  __shared__ float2 lds[N];
+// ---------------------------------------------------------------------------
+// k_channelise: one workgroup per (segment, stream), N / P threads. The 2N
+// windowed reals of the segment are packed as N complex points, loaded from
+// HBM straight into the FFT's registers, transformed and split into the
+// N + 1 bins of the real transform.
+// ---------------------------------------------------------------------------
+template <int N, int P>
+__global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restrict__ streams, DevJob job,
+                                                      const float *__restrict__ window,
+                                                      const float2 *__restrict__ tw_n,
+                                                      const float2 *__restrict__ tw_2n) {
+  constexpr int T = N / P;
+  __shared__ float2 lds[fft_lds_size<N, P>()];
   const DevStream &s = streams[blockIdx.y];
   const int64_t seg = blockIdx.x;
   const float2 *x = reinterpret_cast<const float2 *>(s.time_series + seg * job.seg_step);
   const float2 *w = reinterpret_cast<const float2 *>(window);
   const bool none = job.window == SFXG_WINDOW_NONE;
-  for (int m = threadIdx.x; m < N; m += T) {
+  float2 v[P];
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    const int m = threadIdx.x + i * T;
     float2 z;
     if (none && m >= N / 2) {
       z = make_float2(0.0f, 0.0f);
@@ -512,17 +558,10 @@ __global__ __launch_bounds__(T) void k_channelise(const DevStream *__restrict__ 
       z = make_float2(xv.x * wv.x, xv.y * wv.y);
     }
     if (s.flip) z.y = -z.y;
-    lds[m] = z;
+    v[i] = z;
   }
-  __syncthreads();
-  lds_fft<N, T, -1>(lds, tw_n);
-  float2 *out = s.spectra + seg * job.spectra_stride;
-  for (int k = threadIdx.x; k <= N; k += T) {
-    const float2 zk = lds[k & (N - 1)], zn = lds[(N - k) & (N - 1)];
-    const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-    const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
-    out[k] = cadd(e, cmul(tw_2n[k], o));
-  }
+  reg_fft<N, P>(v, lds, tw_n);
+  real_split<N, P, T>(lds, tw_2n, s.spectra + seg * job.spectra_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -542,8 +581,9 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
                                                const float2 *__restrict__ tw_n, const float2 *__restrict__ tw_2n,
                                                int segs_per_wg) {
   static_assert(N >= 256 && (T / 64) * 256 <= N, "wave scratch must fit in the FFT buffer");
+  constexpr int P = N / T;
   __shared__ float ring[2 * N];
-  __shared__ float2 fbuf[N];
+  __shared__ float2 fbuf[fft_lds_size<N, P>()];
   __shared__ float2 tw[256];
   const DevStream &s = streams[blockIdx.y];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -573,8 +613,11 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
       delay_block_pair(s, blk0 + b, job.n_blocks, scratch, tw, lane, ring + h_new * N + b * 256,
                        b + 1 < BPH ? ring + h_new * N + (b + 1) * 256 : nullptr);
     __syncthreads();
-    // window the segment's 2 N real samples into N complex points
-    for (int m = threadIdx.x; m < N; m += T) {
+    // window the segment's 2 N real samples into the FFT's registers
+    float2 v[P];
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+      const int m = opaque(threadIdx.x + i * T);
       float2 z;
       if (none) {
         if (m < N / 2) {
@@ -591,17 +634,10 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
         z = make_float2(x.x * wv.x, x.y * wv.y);
       }
       if (s.flip) z.y = -z.y;
-      fbuf[m] = z;
+      v[i] = z;
     }
-    __syncthreads();
-    lds_fft<N, T, -1>(fbuf, tw_n);
-    float2 *out = s.spectra + seg * job.spectra_stride;
-    for (int k = threadIdx.x; k <= N; k += T) {
-      const float2 zk = fbuf[k & (N - 1)], zn = fbuf[(N - k) & (N - 1)];
-      const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-      const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
-      out[k] = cadd(e, cmul(tw_2n[k], o));
-    }
+    reg_fft<N, P>(v, fbuf, tw_n);
+    real_split<N, P, T>(fbuf, tw_2n, s.spectra + seg * job.spectra_stride);
     __syncthreads();
   }
 }
@@ -1085,9 +1121,11 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
 template <int N>
 static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, const float *window,
                                 const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
-  constexpr int T = (N / 4) < 512 ? (N / 4) : 512;
+  // the fused F-engine's schedule where it exists (N <= 4096: 512 threads at
+  // most), so both paths give the same bits; 16 points per thread beyond
+  constexpr int P = N / 512 < 4 ? 4 : (N / 512 > 16 ? 16 : N / 512);
   dim3 grid((unsigned)job.n_segments, job.n_streams);
-  hipLaunchKernelGGL((k_channelise<N, T>), grid, dim3(T), 0, s, d_streams, job, window, tw_n, tw_2n);
+  hipLaunchKernelGGL((k_channelise<N, P>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_n, tw_2n);
   return hipGetLastError();
 }
 
