@@ -45,6 +45,14 @@ def parse():
     ap.add_argument("--contexts", type=int, default=2, help="contexts (HIP streams) slices alternate over")
     ap.add_argument("--main-only", action="store_true", help="skip the mixed-precision and 32-station runs")
     ap.add_argument("--c2-steps", type=int, default=2, help="0.5 s integrations of the 32-station workload")
+    ap.add_argument("--c2-precisions", default="fp32,split_fp16,fp16")
+    ap.add_argument("--only", choices=("all", "c1", "c2"), default="all",
+                    help="c1: the main 8-station workload only; c2: the 32-station workload only")
+    ap.add_argument("--cpu-cores", type=int, default=0, help="CPU baseline worker processes (default: up to 16)")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the achievable-peak microbenchmarks")
+    ap.add_argument("--traffic-bytes", type=float, default=0.0,
+                    help="HBM bytes per launch of the dominant kernel from the calibrated FETCH_SIZE + WRITE_SIZE "
+                         "counters, reported as roofline.traffic")
     return ap.parse_args()
 
 
@@ -63,15 +71,12 @@ def kernel_bytes(job, M, nseg, n_samples, nfold, B, fused):
     }
 
 
-def cpu_baseline(sample_usec):
-    """The FP32 oracle (C restatement, one core) on one slice of the main
-    workload shortened to sample_usec."""
-    from oracle import oracle as O
+def _cpu_slice_inputs(sb, sample_usec):
     from sfxg import configs, synth
 
-    jobs, raw_start = configs.config1_jobs(0, n_integrations=1, t_int_usec=sample_usec)
+    jobs, raw_start = configs.config1_jobs(sb, n_integrations=1, t_int_usec=sample_usec)
     job = jobs[0]
-    rng = np.random.default_rng(7)
+    rng = np.random.default_rng(7 + sb)
     hdrs = []
     nsamp = job.n_ffts_per_integration(0) * job.fft_size_delaycor
     for i, st in enumerate(job.station_streams):
@@ -79,14 +84,87 @@ def cpu_baseline(sample_usec):
                                            st.sample_rate, st.bits_per_sample, elided=True)
         payload = rng.integers(0, 256, n, dtype=np.uint8)
         hdrs.append(synth.join_elided(ctrl, payload))
+    return job, hdrs, len(job.station_streams) * nsamp
+
+
+def _cpu_worker(sb, sample_usec, barrier, out):
+    """One correlator-node worker: builds its slice, waits for the others,
+    then times the FP32 oracle on it."""
+    from oracle import oracle as O
+
+    job, hdrs, ss = _cpu_slice_inputs(sb, sample_usec)
     O.lib("f32")
+    barrier.wait()
     t0 = time.perf_counter()
     O.run_slice(job, hdrs, lut=0)
-    dt = time.perf_counter() - t0
-    ss = len(job.station_streams) * nsamp
-    return {"value": ss / dt / 1e9, "unit": "station-Gsamples/s", "cores": 1, "kind": "port",
-            "sample": f"one {sample_usec / 1e6:g} s slice of subband 0 ({len(job.station_streams)} streams, "
-                      f"{ss:.3g} station-samples), FP32 oracle, {dt:.1f} s"}
+    out.put((sb, t0, time.perf_counter(), ss))
+
+
+def _cpu_run(n, sample_usec):
+    import multiprocessing as mp
+
+    ctx = mp.get_context("fork")
+    barrier, out = ctx.Barrier(n), ctx.Queue()
+    procs = [ctx.Process(target=_cpu_worker, args=(k % 8, sample_usec, barrier, out)) for k in range(n)]
+    for p in procs:
+        p.start()
+    res = [out.get() for _ in procs]
+    for p in procs:
+        p.join()
+        if p.exitcode != 0:
+            raise RuntimeError(f"CPU baseline worker failed with exit code {p.exitcode}")
+    wall = max(r[2] for r in res) - min(r[1] for r in res)
+    return sum(r[3] for r in res), wall
+
+
+def cpu_cores():
+    """Host cores this process may use, at most 16 (the GPU box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(sample_usec, cores):
+    """The FP32 oracle (C restatement, -O2 -ffast-math) as the reference's
+    correlator nodes run it: one worker process per host core, each on an
+    independent slice (subbands 0..7 of the main workload, shortened to
+    sample_usec), plus the same slice on one core alone. Runs before the GPU
+    is initialised (the workers are forked)."""
+    ss1, wall1 = _cpu_run(1, sample_usec)
+    ssn, walln = _cpu_run(cores, sample_usec)
+    return {"value": ssn / walln / 1e9, "unit": "station-Gsamples/s", "cores": cores, "kind": "port",
+            "single_core_value": ss1 / wall1 / 1e9,
+            "sample": f"{cores} worker processes, each one {sample_usec / 1e6:g} s slice of one subband of the "
+                      f"main workload (16 streams, {ss1:.3g} station-samples per worker), FP32 oracle; "
+                      f"{walln:.1f} s wall for all, {wall1:.1f} s for one slice on one core"}
+
+
+def achievable_peaks(torch, dev):
+    """Copy bandwidth (read + write bytes) and dense FP16 MFMA rate measured
+    on this GPU by the microbenchmarks in libsfxg_peaks.so."""
+    import ctypes as C
+
+    path = os.path.join(ROOT, "sfxc-1_amd", "lib", "libsfxg_peaks.so")
+    lib = C.CDLL(path)  # raises if the library was not built
+    n = 1 << 30
+    a = torch.empty(n, dtype=torch.uint8, device=f"cuda:{dev}")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    torch.cuda.synchronize()
+    ms = C.c_float()
+    rc = lib.sfxg_peak_copy(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.c_int64(n), 10, C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"sfxg_peak_copy failed: {rc}")
+    del a, b
+    torch.cuda.empty_cache()
+    tf = C.c_double()
+    rc = lib.sfxg_peak_mfma_fp16(20000, 8, C.byref(tf))
+    if rc != 0:
+        raise RuntimeError(f"sfxg_peak_mfma_fp16 failed: {rc}")
+    return {"hbm_copy_GBs": round(2 * n / (ms.value * 1e-3) / 1e9, 1), "copy_bytes": n,
+            "mfma_fp16_dense_TFLOPs": round(tf.value, 1)}
 
 
 class Workload:
@@ -149,8 +227,8 @@ class Workload:
             c.synchronize()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        for c in ctxs:
-            c.kernel_times(reset=True)
+        timed = [c.kernel_times(reset=True) for c in ctxs]
+        self.kt_timed = np.mean(timed, axis=0)
         # kernel durations without overlap: one step again on one context,
         # each slice waited for
         for sb in range(self.NSB):
@@ -175,6 +253,7 @@ def summarise(wl, wall, kt):
         "value": round(value, 4),
         "ms_per_step": round(wall / wl.K * 1e3, 3),
         "kernel_ms_isolated": {n: round(float(a), 3) for n, a in zip(KERNELS, kt)},
+        "kernel_ms_timed_region": {n: round(float(a), 3) for n, a in zip(KERNELS, wl.kt_timed)},
         "kernel_GBs": {n: round(kb[n] / (a * 1e-3) / 1e9, 1) for n, a in zip(KERNELS, kt) if a > 0},
         "xstep_TFLOPs": round(xflops / (kt[3] * 1e-3) / 1e12, 2) if kt[3] > 0 else None,
         "_kb": kb,
@@ -187,76 +266,106 @@ def main():
         print("bench.py measures one GPU: every box in this pool has one GPU and the path shards by "
               "(slice x subband) with no collectives; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    cpu = None
+    if not args.no_cpu_baseline and args.only in ("all", "c1"):
+        # before the GPU is touched: the workers are forked
+        cpu = cpu_baseline(args.cpu_sample_usec, args.cpu_cores or cpu_cores())
+        print(f"CPU baseline done: {cpu['value']:.4f} station-Gsamples/s on {cpu['cores']} cores",
+              file=sys.stderr, flush=True)
     import torch
     from sfxg import capi, configs, synth
 
     dev = 0
     torch.cuda.set_device(dev)
+    peaks = None if args.no_peaks else achievable_peaks(torch, dev)
+    if peaks:
+        print(f"peaks: {peaks}", file=sys.stderr, flush=True)
     K, W, NSB = args.steps, args.warmup, args.subbands
-    wl = Workload(torch, capi, synth, lambda sb, k: configs.config1_jobs(sb, n_integrations=k), K, NSB, dev, 1000)
-    print(f"setup done: {len(wl.inputs)} slices in {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
-    wall, kt = wl.run(torch, capi, PREC["fp32"], W, args.contexts, dev)
-    main_r = summarise(wl, wall, kt)
-    print(f"plain FP32 done: {main_r['value']:.2f} station-Gsamples/s", file=sys.stderr, flush=True)
-    dom = int(np.argmax(kt))
-    dom_name = KERNELS[dom]
-    achieved = main_r["_kb"][dom_name] / (kt[dom] * 1e-3) / 1e9
-    line = {
-        "metric": "station-Gsamples/s correlated (plain FP32 GPU path)",
-        "value": main_r["value"],
-        "unit": "station-Gsamples/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": main_r["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic: seeded 2-bit station data generated in device memory, HEADER records from the "
-                "input-node delay rule",
-        "config": {"workload": "8 stations x 2 pol, 8 subbands x 32 MHz at 64 Ms/s, 2-bit, 4096 channels, "
-                               "1 s integrations, 10 s", "stations": 8, "polarisations": 2, "subbands": NSB,
-                   "sample_rate": 64_000_000, "channels": 4096, "integration_s": 1.0,
-                   "station_samples_per_step": wl.station_samples_per_step},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
-        "kernel_ms_isolated": main_r["kernel_ms_isolated"],
-        "kernel_GBs": main_r["kernel_GBs"],
-        "setup_s": round(wl.setup_s, 1),
-    }
-    if not args.main_only:
-        wall, kt = wl.run(torch, capi, PREC["split_fp16"], W, args.contexts, dev)
-        mixed = summarise(wl, wall, kt)
-        mixed.pop("_kb")
-        mixed["xstep_mfma_frac"] = round(mixed["xstep_TFLOPs"] / FP16_DENSE_PEAK_TFLOPS, 5) \
-            if mixed["xstep_TFLOPs"] else None
-        line["mixed_split_fp16"] = mixed
-        print(f"mixed done: {mixed['value']:.2f}", file=sys.stderr, flush=True)
+    line = {}
+    if args.only in ("all", "c1"):
+        wl = Workload(torch, capi, synth, lambda sb, k: configs.config1_jobs(sb, n_integrations=k), K, NSB, dev,
+                      1000)
+        print(f"setup done: {len(wl.inputs)} slices in {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
+        wall, kt = wl.run(torch, capi, PREC["fp32"], W, args.contexts, dev)
+        main_r = summarise(wl, wall, kt)
+        print(f"plain FP32 done: {main_r['value']:.2f} station-Gsamples/s", file=sys.stderr, flush=True)
+        dom = int(np.argmax(kt))
+        dom_name = KERNELS[dom]
+        achieved = main_r["_kb"][dom_name] / (kt[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": main_r["_kb"][dom_name], "launch_ms": round(float(kt[dom]), 3)}
+        if peaks:
+            roof["frac_of_achievable"] = round(achieved / peaks["hbm_copy_GBs"], 4)
+        if args.traffic_bytes:
+            roof["traffic"] = args.traffic_bytes
+        line.update({
+            "metric": "station-Gsamples/s correlated (plain FP32 GPU path)",
+            "value": main_r["value"],
+            "unit": "station-Gsamples/s",
+            "n_gpus": 1,
+            "steps": K,
+            "warmup": W,
+            "ms_per_step": main_r["ms_per_step"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "fp32",
+            "data": "synthetic: seeded 2-bit station data generated in device memory, HEADER records from the "
+                    "input-node delay rule",
+            "config": {"workload": "8 stations x 2 pol, 8 subbands x 32 MHz at 64 Ms/s, 2-bit, 4096 channels, "
+                                   "1 s integrations, 10 s", "stations": 8, "polarisations": 2, "subbands": NSB,
+                       "sample_rate": 64_000_000, "channels": 4096, "integration_s": 1.0,
+                       "station_samples_per_step": wl.station_samples_per_step},
+            "roofline": roof,
+            "kernel_ms_isolated": main_r["kernel_ms_isolated"],
+            "kernel_ms_timed_region": main_r["kernel_ms_timed_region"],
+            "kernel_GBs": main_r["kernel_GBs"],
+            "setup_s": round(wl.setup_s, 1),
+        })
+        if not args.main_only:
+            wall, kt = wl.run(torch, capi, PREC["split_fp16"], W, args.contexts, dev)
+            mixed = summarise(wl, wall, kt)
+            mixed.pop("_kb")
+            add_mfma_frac(mixed, peaks)
+            line["mixed_split_fp16"] = mixed
+            print(f"mixed done: {mixed['value']:.2f}", file=sys.stderr, flush=True)
         del wl
         torch.cuda.empty_cache()
+    if args.only == "c2" or (args.only == "all" and not args.main_only):
         w2 = Workload(torch, capi, synth, lambda sb, k: configs.config2_jobs(sb, n_integrations=k), args.c2_steps, 4,
                       dev, 2000)
         print(f"32-station setup done in {w2.setup_s:.1f} s", file=sys.stderr, flush=True)
         c2 = {"workload": "32 stations x 2 pol, 4 subbands x 64 MHz at 128 Ms/s, 2-bit, 16384 channels, "
                           f"0.5 s integrations, {args.c2_steps * 0.5:g} s",
               "station_samples_per_step": w2.station_samples_per_step}
-        for name in ("fp32", "split_fp16", "fp16"):
+        for name in args.c2_precisions.split(","):
             wall, kt = w2.run(torch, capi, PREC[name], 0, args.contexts, dev)
             r = summarise(w2, wall, kt)
             r.pop("_kb")
-            if name != "fp32" and r["xstep_TFLOPs"]:
-                r["xstep_mfma_frac"] = round(r["xstep_TFLOPs"] / FP16_DENSE_PEAK_TFLOPS, 5)
+            if name != "fp32":
+                add_mfma_frac(r, peaks)
             c2[name] = r
             print(f"32-station {name} done: {r['value']:.2f}", file=sys.stderr, flush=True)
-        c2["mixed_vs_plain"] = round(c2["split_fp16"]["value"] / c2["fp32"]["value"], 3)
-        c2["fp16_vs_plain"] = round(c2["fp16"]["value"] / c2["fp32"]["value"], 3)
+        if "fp32" in c2:
+            for name, key in (("split_fp16", "mixed_vs_plain"), ("fp16", "fp16_vs_plain"), ("bf16", "bf16_vs_plain")):
+                if name in c2:
+                    c2[key] = round(c2[name]["value"] / c2["fp32"]["value"], 3)
         line["config2"] = c2
         del w2
         torch.cuda.empty_cache()
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_usec)
+    if peaks:
+        line["achievable_peaks"] = peaks
+    if cpu:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line))
+
+
+def add_mfma_frac(r, peaks):
+    if r["xstep_TFLOPs"]:
+        r["xstep_mfma_frac"] = round(r["xstep_TFLOPs"] / FP16_DENSE_PEAK_TFLOPS, 5)
+        if peaks:
+            r["xstep_mfma_frac_of_achievable"] = round(r["xstep_TFLOPs"] / peaks["mfma_fp16_dense_TFLOPs"], 5)
 
 
 if __name__ == "__main__":

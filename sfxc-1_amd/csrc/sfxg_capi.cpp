@@ -93,7 +93,7 @@ struct sfxg_ctx {
 
   // device buffers
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
-      flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, window, offsets, samples;
+      flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, tw_split, window, offsets, samples;
   // host staging of the enqueued slice (kept until the next slice)
   std::vector<double> h_model, h_offsets;
   std::vector<DevStream> h_streams;
@@ -133,6 +133,9 @@ struct sfxg_ctx {
 static void collect_times(sfxg_ctx *ctx) {
   if (!ctx->times_pending) return;
   for (int k = 0; k < SFXG_N_KERNELS; k++) {
+    // the fused F-engine runs the channeliser inside the delay stage: no
+    // channeliser launch to time
+    if (k == SFXG_KERNEL_CHANNELISE && ctx->fused) continue;
     float t = 0.0f;
     if (hipEventElapsedTime(&t, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) ctx->kt_sum[k] += t;
   }
@@ -171,7 +174,7 @@ void sfxg_destroy(sfxg_ctx *ctx) {
   DBuf *bufs[] = {&ctx->payload, &ctx->runs, &ctx->block_run, &ctx->counted, &ctx->dstreams,
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
                   &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
-                  &ctx->tw_n, &ctx->tw_2n, &ctx->window, &ctx->offsets, &ctx->samples};
+                  &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples};
   for (DBuf *b : bufs) b->release();
   if (ctx->done) (void)hipEventDestroy(ctx->done);
   for (hipEvent_t e : ctx->ev)
@@ -351,7 +354,7 @@ int sfxg_push_stream_elided(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *co
 }
 
 static void make_tables(sfxg_ctx *ctx, std::vector<float2> &tnd, std::vector<float2> &tn,
-                        std::vector<float2> &t2n, std::vector<float> &win) {
+                        std::vector<float2> &t2n, std::vector<float2> &tsp, std::vector<float> &win) {
   const int nd = ctx->nd, nc = ctx->nc;
   tnd.resize(nd);
   for (int k = 0; k < nd; k++) {
@@ -367,6 +370,17 @@ static void make_tables(sfxg_ctx *ctx, std::vector<float2> &tnd, std::vector<flo
   for (int k = 0; k <= nc; k++) {
     const double a = -M_PI * k / nc;
     t2n[k] = make_float2((float)cos(a), (float)sin(a));
+  }
+  // exp(-2 pi i j / 2N) = coarse[j >> 7] * fine[j & 127]
+  const int ncoarse = 2 * nc / 128;
+  tsp.resize(ncoarse + 128);
+  for (int j = 0; j < ncoarse; j++) {
+    const double a = -M_PI * (128.0 * j) / nc;
+    tsp[j] = make_float2((float)cos(a), (float)sin(a));
+  }
+  for (int j = 0; j < 128; j++) {
+    const double a = -M_PI * j / nc;
+    tsp[ncoarse + j] = make_float2((float)cos(a), (float)sin(a));
   }
   // create_window (delay_correction.cc:299-341), n = fft_rot_size = 2 N_c
   const int n = 2 * nc;
@@ -429,16 +443,18 @@ int sfxg_correlate(sfxg_ctx *ctx) {
 
   // upload tables
   if (ctx->tables_nc != nc || ctx->tables_window != ctx->p.window) {
-    std::vector<float2> tnd, tn, t2n;
+    std::vector<float2> tnd, tn, t2n, tsp;
     std::vector<float> win;
-    make_tables(ctx, tnd, tn, t2n, win);
+    make_tables(ctx, tnd, tn, t2n, tsp, win);
     HIPCHK(ctx, ctx->tw_nd.ensure(tnd.size() * sizeof(float2)));
     HIPCHK(ctx, ctx->tw_n.ensure(tn.size() * sizeof(float2)));
     HIPCHK(ctx, ctx->tw_2n.ensure(t2n.size() * sizeof(float2)));
+    HIPCHK(ctx, ctx->tw_split.ensure(tsp.size() * sizeof(float2)));
     HIPCHK(ctx, ctx->window.ensure(win.size() * sizeof(float)));
     HIPCHK(ctx, hipMemcpy(ctx->tw_nd.p, tnd.data(), tnd.size() * sizeof(float2), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->tw_n.p, tn.data(), tn.size() * sizeof(float2), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->tw_2n.p, t2n.data(), t2n.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->tw_split.p, tsp.data(), tsp.size() * sizeof(float2), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->window.p, win.data(), win.size() * sizeof(float), hipMemcpyHostToDevice));
     ctx->tables_nc = nc;
     ctx->tables_window = ctx->p.window;
@@ -606,7 +622,7 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     HIPCHK(ctx, launch_delay(dstr, job, ctx->tw_nd.as<float2>(), s));
     HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_CHANNELISE], s));
     HIPCHK(ctx, launch_channelise(dstr, job, ctx->window.as<float>(), ctx->tw_n.as<float2>(),
-                                  ctx->tw_2n.as<float2>(), s));
+                                  ctx->tw_2n.as<float2>(), ctx->tw_split.as<float2>(), s));
   }
   ctx->fused = fused;
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_XSTEP], s));
@@ -617,9 +633,9 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     // sqrt(sum w^2); the decode table bounds the sample rms
     double wsum = 0.0;
     {
-      std::vector<float2> tnd, tn, t2n;
+      std::vector<float2> tnd, tn, t2n, tsp;
       std::vector<float> win;
-      make_tables(ctx, tnd, tn, t2n, win);
+      make_tables(ctx, tnd, tn, t2n, tsp, win);
       for (float w : win) wsum += (double)w * w;
     }
     double lmax = 0.0;

@@ -163,8 +163,10 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
 bool fengine_supported(int nc);
 hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window,
                           const float2 *tw_nd, const float2 *tw_n, const float2 *tw_2n, hipStream_t s);
+// tw_split: [2 N_c / 128] coarse then [128] fine twiddles of exp(-2 pi i j / 2 N_c)
+// (the LDS tables of the large-N channeliser).
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
-                             const float2 *tw_n, const float2 *tw_2n, hipStream_t s);
+                             const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, hipStream_t s);
 // Sub-integration partial sums (fold_start[0..n_k]: segment ranges) and
 // their in-order fold into vis.
 hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M,

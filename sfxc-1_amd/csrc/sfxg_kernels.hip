@@ -461,23 +461,50 @@ __device__ __forceinline__ void dft<16, -1>(float2 (&u)[16]) {
   for (int k = 0; k < 16; k++) u[k] = x[k];
 }
 
+// Twiddle sources: fft(m) = exp(-2 pi i m / N), split(k) = exp(-pi i k / N).
+// TableTw reads full tables in global memory (N and N + 1 entries);
+// LdsTw multiplies two short tables held in LDS, hi[j] = exp(-2 pi i 128 j /
+// 2N) and lo[j] = exp(-2 pi i j / 2N) (j < 128), so that the FFT issues no
+// global loads of its own.
+struct TableTw {
+  const float2 *__restrict__ n;
+  const float2 *__restrict__ n2;
+  __device__ __forceinline__ float2 fft(int m) const { return n[m]; }
+  __device__ __forceinline__ float2 split(int k) const { return n2[k]; }
+};
+struct LdsTw {
+  const float2 *hi;
+  const float2 *lo;
+  __device__ __forceinline__ float2 at(int j) const {
+    const float2 h = hi[j >> 7];
+    return (j & 127) ? cmul(h, lo[j & 127]) : h;
+  }
+  __device__ __forceinline__ float2 fft(int m) const { return at(2 * m); }
+  __device__ __forceinline__ float2 split(int k) const { return at(k); }
+};
+
 // One radix-R stage over the registers of all threads; ns = product of the
 // earlier radices. Outputs go to LDS.
-template <int N, int P, int R>
-__device__ __forceinline__ void fft_stage(float2 (&v)[P], float2 *lds, const float2 *__restrict__ tw, int ns) {
+template <int N, int P, int R, class TW>
+__device__ __forceinline__ void fft_stage(float2 (&v)[P], float2 *lds, const TW &tw, int ns, int tid) {
   constexpr int T = N / P, B = P / R;
   const int lns = __builtin_ctz(ns);
 #pragma unroll
   for (int b = 0; b < B; b++) {
-    const int j = threadIdx.x + b * T;
-    const int k = opaque(j) & (ns - 1);
+    const int j = tid + b * T;
+    const int k = j & (ns - 1);
     float2 u[R];
 #pragma unroll
     for (int q = 0; q < R; q++) u[q] = v[b + q * B];
     if (ns > 1) {
       const int step = N / (ns * R);
 #pragma unroll
-      for (int q = 1; q < R; q++) u[q] = cmul(u[q], tw[q * k * step]);
+      for (int q = 1; q < R; q++) {
+        u[q] = cmul(u[q], tw.fft(q * k * step));
+        // at most four twiddles in flight, so their loads do not hold
+        // registers needed by data the caller keeps live across the FFT
+        if (q % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+      }
     }
     dft<R, -1>(u);
     const int idx = ((j >> lns) << (lns + ilog2(R))) + k;
@@ -487,10 +514,13 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[P], float2 *lds, const flo
 }
 
 // Forward FFT (exp(-2 pi i jk / N)) of the points in v, result in padded LDS
-// (point k at lds_index<P>(k)). The caller guarantees that no thread still
-// reads `lds` on entry; the function ends with a barrier.
-template <int N, int P>
-__device__ __forceinline__ void reg_fft(float2 (&v)[P], float2 *lds, const float2 *__restrict__ tw) {
+// (point k at lds_index<P>(k)). tid is threadIdx.x; callers that loop pass
+// it through opaque() so that the per-thread index arithmetic is redone in
+// each iteration instead of being hoisted into registers held across the
+// loop. The caller guarantees that no thread still reads `lds` on entry; the
+// function ends with a barrier.
+template <int N, int P, class TW>
+__device__ __forceinline__ void reg_fft(float2 (&v)[P], float2 *lds, const TW &tw, int tid) {
   constexpr int T = N / P;
   constexpr int NS = ilog2(N) / ilog2(P);               // full radix-P stages
   constexpr int F = 1 << (ilog2(N) - NS * ilog2(P));    // last radix, 1 if none
@@ -499,30 +529,31 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[P], float2 *lds, const float
   int ns = 1;
 #pragma unroll
   for (int st = 0; st < NS; st++) {
-    fft_stage<N, P, P>(v, lds, tw, ns);
+    fft_stage<N, P, P, TW>(v, lds, tw, ns, tid);
     ns *= P;
     __syncthreads();
     if (st + 1 < NS || F > 1) {
 #pragma unroll
-      for (int i = 0; i < P; i++) v[i] = lds[lds_index<P>(threadIdx.x + i * T)];
+      for (int i = 0; i < P; i++) v[i] = lds[lds_index<P>(tid + i * T)];
       __syncthreads();
     }
   }
   if constexpr (F > 1) {
-    fft_stage<N, P, F>(v, lds, tw, ns);
+    fft_stage<N, P, F, TW>(v, lds, tw, ns, tid);
     __syncthreads();
   }
 }
 
 // Split of the N-point complex transform of 2N packed reals into bins
 // k = 0..N of the real transform: X_k = E_k + W_2N^k O_k.
-template <int N, int P, int T>
-__device__ __forceinline__ void real_split(const float2 *lds, const float2 *__restrict__ tw_2n, float2 *out) {
-  for (int k = threadIdx.x; k <= N; k += T) {
+template <int N, int P, int T, class TW>
+__device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, float2 *out, int tid) {
+  for (int k = tid; k <= N; k += T) {
     const float2 zk = lds[lds_index<P>(k & (N - 1))], zn = lds[lds_index<P>((N - k) & (N - 1))];
     const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
     const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
-    out[k] = cadd(e, cmul(tw_2n[k], o));
+    out[k] = cadd(e, cmul(tw.split(k), o));
+    __builtin_amdgcn_sched_barrier(0);  // one bin's loads in flight at a time
   }
 }
 
@@ -560,8 +591,87 @@ __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restric
     if (s.flip) z.y = -z.y;
     v[i] = z;
   }
-  reg_fft<N, P>(v, lds, tw_n);
-  real_split<N, P, T>(lds, tw_2n, s.spectra + seg * job.spectra_stride);
+  const TableTw tw{tw_n, tw_2n};
+  reg_fft<N, P>(v, lds, tw, threadIdx.x);
+  real_split<N, P, T>(lds, tw, s.spectra + seg * job.spectra_stride, threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// k_channelise_run: the channeliser for large N (one workgroup per CU by
+// LDS), as persistent workgroups that each walk a run of consecutive
+// segments of one stream. Overlapping segments share half their samples, so
+// the second half of each segment stays in registers as the next one's first
+// half and only N new reals are read per segment; they are prefetched one
+// segment ahead. The FFT takes its twiddles from two short LDS tables, so
+// the only global loads in flight during it are the prefetch.
+// ---------------------------------------------------------------------------
+template <int N, int P>
+__global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__restrict__ streams, DevJob job,
+                                                          const float *__restrict__ window,
+                                                          const float2 *__restrict__ tw_split, int segs_per_wg) {
+  constexpr int T = N / P, H = P / 2, NHI = 2 * N / 128;
+  __shared__ float2 lds[fft_lds_size<N, P>()];
+  __shared__ float2 thi[NHI], tlo[128];
+  const DevStream &s = streams[blockIdx.y];
+  const int64_t sa = (int64_t)blockIdx.x * segs_per_wg;
+  int64_t sb = sa + segs_per_wg;
+  if (sb > job.n_segments) sb = job.n_segments;
+  if (sa >= sb) return;  // uniform over the workgroup
+  for (int i = threadIdx.x; i < NHI + 128; i += T) {
+    const float2 t = tw_split[i];
+    if (i < NHI) thi[i] = t; else tlo[i - NHI] = t;
+  }
+  const LdsTw tw{thi, tlo};
+  const bool none = job.window == SFXG_WINDOW_NONE;
+  const float2 *w2 = reinterpret_cast<const float2 *>(window);
+  // complex point m of segment seg is the pair of reals 2m, 2m + 1 at
+  // seg * N reals; the segment's second half is the next one's first half
+  const float2 *ts = reinterpret_cast<const float2 *>(s.time_series);
+  float2 lo[H], hi[H];
+#pragma unroll
+  for (int i = 0; i < H; i++) {
+    lo[i] = ts[sa * (N / 2) + threadIdx.x + i * T];
+    hi[i] = none ? make_float2(0.0f, 0.0f) : ts[sa * (N / 2) + N / 2 + threadIdx.x + i * T];
+  }
+  __syncthreads();  // twiddle tables
+  for (int64_t seg = sa; seg < sb; seg++) {
+    const int tid = opaque(threadIdx.x);
+    float2 v[P];
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+      const float2 x = i < H ? lo[i] : hi[i - H];
+      const float2 wv = w2[tid + i * T];
+      float2 z = make_float2(x.x * wv.x, x.y * wv.y);
+      if (s.flip) z.y = -z.y;
+      v[i] = z;
+    }
+    if (none) {
+#pragma unroll
+      for (int i = H; i < P; i++) v[i] = make_float2(0.0f, 0.0f);
+    }
+    // prefetch the reals the next segment adds
+    float2 nx[H];
+    const bool more = seg + 1 < sb;
+    if (more) {
+      const int64_t base = (seg + 1) * (N / 2) + (none ? 0 : N / 2);
+#pragma unroll
+      for (int i = 0; i < H; i++) nx[i] = ts[base + tid + i * T];
+    }
+    reg_fft<N, P>(v, lds, tw, tid);
+    real_split<N, P, T>(lds, tw, s.spectra + seg * job.spectra_stride, tid);
+    __syncthreads();
+    if (more) {
+#pragma unroll
+      for (int i = 0; i < H; i++) {
+        if (none) {
+          lo[i] = nx[i];
+        } else {
+          lo[i] = hi[i];
+          hi[i] = nx[i];
+        }
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -614,10 +724,11 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
                        b + 1 < BPH ? ring + h_new * N + (b + 1) * 256 : nullptr);
     __syncthreads();
     // window the segment's 2 N real samples into the FFT's registers
+    const int tid = opaque(threadIdx.x);
     float2 v[P];
 #pragma unroll
     for (int i = 0; i < P; i++) {
-      const int m = opaque(threadIdx.x + i * T);
+      const int m = tid + i * T;
       float2 z;
       if (none) {
         if (m < N / 2) {
@@ -636,8 +747,9 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
       if (s.flip) z.y = -z.y;
       v[i] = z;
     }
-    reg_fft<N, P>(v, fbuf, tw_n);
-    real_split<N, P, T>(fbuf, tw_2n, s.spectra + seg * job.spectra_stride);
+    const TableTw twt{tw_n, tw_2n};
+    reg_fft<N, P>(v, fbuf, twt, tid);
+    real_split<N, P, T>(fbuf, twt, s.spectra + seg * job.spectra_stride, tid);
     __syncthreads();
   }
 }
@@ -1120,26 +1232,38 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
 
 template <int N>
 static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, const float *window,
-                                const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
-  // the fused F-engine's schedule where it exists (N <= 4096: 512 threads at
-  // most), so both paths give the same bits; 16 points per thread beyond
-  constexpr int P = N / 512 < 4 ? 4 : (N / 512 > 16 ? 16 : N / 512);
-  dim3 grid((unsigned)job.n_segments, job.n_streams);
-  hipLaunchKernelGGL((k_channelise<N, P>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_n, tw_2n);
+                                const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, hipStream_t s) {
+  if constexpr (N >= 8192) {
+    // persistent runs of consecutive segments, about 2048 workgroups in all
+    constexpr int P = 16;
+    if (job.seg_step != N) return hipErrorInvalidValue;
+    const int64_t per_stream = (2048 + job.n_streams - 1) / job.n_streams;
+    int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
+    if (segs < 1) segs = 1;
+    const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
+    hipLaunchKernelGGL((k_channelise_run<N, P>), dim3(gx, job.n_streams), dim3(N / P), 0, s, d_streams, job, window,
+                       tw_split, (int)segs);
+  } else {
+    // the fused F-engine's schedule (512 threads at most), so both paths
+    // give the same bits
+    constexpr int P = N / 512 < 4 ? 4 : N / 512;
+    dim3 grid((unsigned)job.n_segments, job.n_streams);
+    hipLaunchKernelGGL((k_channelise<N, P>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_n, tw_2n);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
-                             const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+                             const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
   switch (job.nc) {
-    case 256: return launch_chan_n<256>(d_streams, job, window, tw_n, tw_2n, s);
-    case 512: return launch_chan_n<512>(d_streams, job, window, tw_n, tw_2n, s);
-    case 1024: return launch_chan_n<1024>(d_streams, job, window, tw_n, tw_2n, s);
-    case 2048: return launch_chan_n<2048>(d_streams, job, window, tw_n, tw_2n, s);
-    case 4096: return launch_chan_n<4096>(d_streams, job, window, tw_n, tw_2n, s);
-    case 8192: return launch_chan_n<8192>(d_streams, job, window, tw_n, tw_2n, s);
-    case 16384: return launch_chan_n<16384>(d_streams, job, window, tw_n, tw_2n, s);
+    case 256: return launch_chan_n<256>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
+    case 512: return launch_chan_n<512>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
+    case 1024: return launch_chan_n<1024>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
+    case 2048: return launch_chan_n<2048>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
+    case 4096: return launch_chan_n<4096>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
+    case 8192: return launch_chan_n<8192>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
+    case 16384: return launch_chan_n<16384>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
     default: return hipErrorInvalidValue;
   }
 }

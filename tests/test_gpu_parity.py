@@ -136,7 +136,7 @@ def run_both(ctx, job, hdrs, lut=LUT_CONFIG, pulsar=None):
     return vis_g, vis_r, ref
 
 
-@pytest.mark.parametrize("nchan", [256, 2048, 16384])
+@pytest.mark.parametrize("nchan", [256, 2048, 8192, 16384])
 def test_channel_counts(ctx, nchan):
     """Power-of-four and odd-log2 FFT sizes (the radix-2 stage), up to the
     largest LDS-resident transform."""
@@ -145,6 +145,19 @@ def test_channel_counts(ctx, nchan):
                                      offsets=[0.0, 2.5e-6, -1.75e-6], rates=[0.0, 2e-7, -1e-7])[0]
     vis_g, vis_r, _ = run_both(ctx, job, hdrs)
     compare_vis(vis_g[0], vis_r[0], label=f"nchan={nchan}")
+
+
+@pytest.mark.parametrize("window,sideband", [(0, "U"), (4, "U"), (4, "L"), (3, "L")])
+def test_large_fft_runs(ctx, window, sideband):
+    """The large-N channeliser walks runs of consecutive segments and carries
+    the shared half of each segment in registers: 0.97 s at 32 Ms/s gives 1893
+    segments, so runs of two segments per workgroup and a partial last run;
+    zero padding (window NONE) and the sideband flip."""
+    job, hdrs = configs.build_slices(seed=12, n_stations=2, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=16384,
+                                     seconds=0.97, t_int_usec=970_000, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=[0.0, 1.5e-6], rates=[0.0, 1e-7], window=window, sideband=sideband)[0]
+    vis_g, vis_r, _ = run_both(ctx, job, hdrs)
+    compare_vis(vis_g[0], vis_r[0], label=f"nchan=16384 window={window} sideband={sideband}")
 
 
 @pytest.mark.parametrize("window", [0, 1, 2, 3])
