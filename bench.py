@@ -46,8 +46,11 @@ def parse():
     ap.add_argument("--main-only", action="store_true", help="skip the mixed-precision and 32-station runs")
     ap.add_argument("--c2-steps", type=int, default=2, help="0.5 s integrations of the 32-station workload")
     ap.add_argument("--c2-precisions", default="fp32,split_fp16,fp16")
-    ap.add_argument("--only", choices=("all", "c1", "c2"), default="all",
-                    help="c1: the main 8-station workload only; c2: the 32-station workload only")
+    ap.add_argument("--only", choices=("all", "c1", "c2", "c4"), default="all",
+                    help="c1: the main 8-station workload only; c2: the 32-station workload only; "
+                         "c4: the pulsar-binning workload only")
+    ap.add_argument("--c4-steps", type=int, default=1, help="1 s integrations of the pulsar workload")
+    ap.add_argument("--c4-precisions", default="fp32,fp16,split_fp16")
     ap.add_argument("--cpu-cores", type=int, default=0, help="CPU baseline worker processes (default: up to 16)")
     ap.add_argument("--no-peaks", action="store_true", help="skip the achievable-peak microbenchmarks")
     ap.add_argument("--traffic-bytes", type=float, default=0.0,
@@ -170,7 +173,7 @@ def achievable_peaks(torch, dev):
 class Workload:
     """Device-resident input of n_subbands x K slices."""
 
-    def __init__(self, torch, capi, synth, jobs_fn, K, NSB, dev, seed_base):
+    def __init__(self, torch, capi, synth, jobs_fn, K, NSB, dev, seed_base, rho=0.02, pulsar_fn=None):
         t0 = time.perf_counter()
         self.per_sb = [jobs_fn(sb, K) for sb in range(NSB)]
         self.K, self.NSB = K, NSB
@@ -187,13 +190,14 @@ class Workload:
                     n_int = int(synth.integer_delay(job.delay_models[i], job.start_time, st.sample_rate))
                     capi.synthesize_raw(dev, buf.data_ptr(), n, st.bits_per_sample, off * 4, n_int,
                                         common_seed=seed_base + 2 * sb + (st.polarisation == "L"),
-                                        stream_seed=(seed_base + sb * 64 + i) * 4096 + k, rho=0.02)
+                                        stream_seed=(seed_base + sb * 64 + i) * 4096 + k, rho=rho)
                     lst.append((ctrl, buf, n))
                 self.inputs[(k, sb)] = lst
         torch.cuda.synchronize()
         self.setup_s = time.perf_counter() - t0
         job = self.per_sb[0][0][0]
         self.job = job
+        self.pulsar = pulsar_fn(job) if pulsar_fn else None
         self.M = len(job.station_streams)
         self.nsamp = job.n_ffts_per_integration(0) * job.fft_size_delaycor
         self.station_samples_per_step = NSB * self.M * self.nsamp
@@ -207,7 +211,7 @@ class Workload:
         def run_slice(k, sb, ctx=None):
             ctx = ctx or ctxs[counter[0] % len(ctxs)]
             counter[0] += 1
-            ctx.set_parameters(self.per_sb[sb][0][k])
+            ctx.set_parameters(self.per_sb[sb][0][k], pulsar=self.pulsar)
             for i, (ctrl, buf, n) in enumerate(self.inputs[(k, sb)]):
                 ctx.push_stream_elided(i, ctrl, buf.data_ptr(), n)
             ctx.correlate()
@@ -353,6 +357,28 @@ def main():
                     c2[key] = round(c2[name]["value"] / c2["fp32"]["value"], 3)
         line["config2"] = c2
         del w2
+        torch.cuda.empty_cache()
+    if args.only == "c4" or (args.only == "all" and not args.main_only):
+        w4 = Workload(torch, capi, synth, lambda sb, k: configs.config4_jobs(sb, n_integrations=k)[:2],
+                      args.c4_steps, 8, dev, 4000, rho=0.005, pulsar_fn=configs.pulsar4)
+        print(f"pulsar setup done in {w4.setup_s:.1f} s", file=sys.stderr, flush=True)
+        c4 = {"workload": "12 stations x 2 pol, 8 subbands x 16 MHz at 32 Ms/s, 2-bit, 1024 channels, 1 s "
+                          f"integrations, {args.c4_steps} s, pulsar binning: 64 bins over the gate [0.25, 0.75) "
+                          "+ off-gate bin, P = 0.0893 s, DM 68",
+              "station_samples_per_step": w4.station_samples_per_step}
+        for name in args.c4_precisions.split(","):
+            wall, kt = w4.run(torch, capi, PREC[name], 0, args.contexts, dev)
+            r = summarise(w4, wall, kt)
+            r.pop("_kb")
+            if name != "fp32":
+                add_mfma_frac(r, peaks)
+            c4[name] = r
+            print(f"pulsar {name} done: {r['value']:.2f}", file=sys.stderr, flush=True)
+        for name in ("fp16", "split_fp16"):
+            if name in c4 and "fp32" in c4:
+                c4[f"{name}_vs_plain"] = round(c4[name]["value"] / c4["fp32"]["value"], 3)
+        line["config4_pulsar"] = c4
+        del w4
         torch.cuda.empty_cache()
     if peaks:
         line["achievable_peaks"] = peaks

@@ -500,9 +500,15 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   const size_t part_one = (size_t)B * (nc + 1) * sizeof(float2);
   const int k_group = (int)std::max<size_t>(1, std::min<size_t>(ctx->folds.size(), ((size_t)1 << 30) / part_one));
   const bool mixed = ctx->opt.precision != SFXG_PREC_FP32;
-  if (mixed && ctx->pulsar)
-    return ctx->fail(SFXG_ERR_ARG, "the mixed-precision X-step does not do pulsar binning");
   if (!ctx->pulsar) HIPCHK(ctx, ctx->partial.ensure(part_one * k_group));
+  // mixed precision with pulsar binning: binned Gram tiles for runs of
+  // channels, at most ~2 GiB at a time
+  int psr_run = 0;
+  if (mixed && ctx->pulsar) {
+    const size_t per_ch = pulsar_gram_floats(M, 1, ctx->nbins) * sizeof(float);
+    psr_run = (int)std::max<size_t>(1, std::min<size_t>((size_t)nc + 1, ((size_t)2 << 30) / per_ch));
+    HIPCHK(ctx, ctx->gram.ensure(per_ch * psr_run));
+  }
 
   // host staging stays alive in the context until the next slice, so the
   // copies and kernels below run asynchronously
@@ -627,6 +633,7 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   ctx->fused = fused;
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_XSTEP], s));
   const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
+  float mixed_scale = 1.0f;
   if (mixed) {
     // power-of-two scale that brings the spectra's rms near 16: the delay
     // correction's unnormalised FFT pair gains N_d / 2, the channeliser
@@ -646,6 +653,10 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     const double rms = (nd / 2.0) * lmax * sqrt(std::max(wsum, 1.0));
     const float scale = (float)ldexp(1.0, (int)lround(log2(16.0 / rms)));
     ctx->mixed_scale = scale;
+    mixed_scale = scale;
+  }
+  if (mixed && !ctx->pulsar) {
+    const float scale = mixed_scale;
     const int n_folds = (int)ctx->folds.size();
     for (int g0 = 0; g0 < n_folds; g0 += k_group) {
       const int ng = std::min(k_group, n_folds - g0);
@@ -689,9 +700,21 @@ int sfxg_correlate(sfxg_ctx *ctx) {
       offs[i] = 4148.808 * ps.DM * (1 / pow(base_freq + i * dfreq, 2) - inv_freq_obs2) * freq;
     HIPCHK(ctx, ctx->offsets.ensure(offs.size() * sizeof(double)));
     HIPCHK(ctx, hipMemcpyAsync(ctx->offsets.p, offs.data(), offs.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S, nc + 1, ctx->nseg, blf,
-                                    bls, B, ctx->offsets.as<double>(), start_phase, ps.coef[1] + 60 * ps.ref_freq,
-                                    fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins, ctx->vis.as<float2>(), s));
+    if (mixed) {
+      for (int c0 = 0; c0 < nc + 1; c0 += psr_run) {
+        const int n_run = std::min(psr_run, nc + 1 - c0);
+        HIPCHK(ctx, launch_xstep_mfma_pulsar(ctx->opt.precision, ctx->spectra.as<float2>(), ctx->nseg * ctx->S,
+                                             ctx->S, nc + 1, c0, n_run, M, ctx->nseg, blf, bls, B, mixed_scale,
+                                             ctx->offsets.as<double>(), start_phase, ps.coef[1] + 60 * ps.ref_freq,
+                                             fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins,
+                                             ctx->gram.as<float>(), ctx->vis.as<float2>(), s));
+      }
+    } else {
+      HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S, nc + 1, ctx->nseg,
+                                      blf, bls, B, ctx->offsets.as<double>(), start_phase,
+                                      ps.coef[1] + 60 * ps.ref_freq, fft_duration, ps.gate_start, ps.gate_stop,
+                                      ctx->nbins, ctx->vis.as<float2>(), s));
+    }
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_NORMALISE], s));
   HIPCHK(ctx, launch_auto_norms(ctx->vis.as<float2>(), M, nc + 1, nc, ctx->nbins, B, ctx->norms.as<double>(), s));

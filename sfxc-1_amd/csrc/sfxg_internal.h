@@ -185,6 +185,17 @@ hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int
                                double start_phase, double phase_coef, double fft_duration,
                                double gate_begin, double gate_end, int nbins, float2 *acc,
                                hipStream_t s);
+// Mixed-precision X-step with pulsar binning for channels ch_base ..
+// ch_base + n_ch_run - 1 of n_ch (offsets indexed by channel): binned Gram
+// tiles in gram (pulsar_gram_floats(M, n_ch_run, nbins) floats, zeroed here),
+// then vis[bin][b][ch] for those channels.
+size_t pulsar_gram_floats(int M, int n_ch, int nbins);
+hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+                                    int n_ch, int ch_base, int n_ch_run, int M, int64_t n_segments,
+                                    const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
+                                    const double *offsets, double start_phase, double phase_coef,
+                                    double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
+                                    float2 *vis, hipStream_t s);
 hipError_t launch_auto_norms(const float2 *vis, int n_streams, int n_ch, int fft_size, int n_bins,
                              int n_baselines, double *norms, hipStream_t s);
 hipError_t launch_normalise(const float2 *vis, const double *norms, const int32_t *bl_first,

@@ -905,6 +905,116 @@ constexpr int MF_MAXP = 10;        // (channel, tile) pairs per wave
 constexpr int MF_THREADS = 512;
 constexpr int MF_WAVES = MF_THREADS / 64;
 
+// (channel, tile) pair p of a workgroup: channel c and the tile (ti, tj),
+// ti <= tj, of the upper triangle; t is the tile's index in row-major order
+// of that triangle.
+__device__ __forceinline__ void mf_pair(int p, int nT, int RT, int &c, int &t, int &ti, int &tj) {
+  c = p / nT;
+  t = p - c * nT;
+  int r = t;
+  ti = 0;
+  while (r >= RT - ti) {
+    r -= RT - ti;
+    ti++;
+  }
+  tj = ti + r;
+}
+
+// Loads segments [s0, s0 + MF_KS) (zeros at and past s_end, and for channels
+// past n_ch) of CH channels from ch0 into the LDS planes as Y[c][r][k]:
+// consecutive threads take consecutive channels, each thread 8 segments of
+// one stream, converted to the MFMA operand type and stored 16 bytes at a time.
+template <int PREC>
+__device__ __forceinline__ void mf_load_stage(const float2 *__restrict__ spectra, int64_t stream_stride,
+                                              int64_t seg_stride, int n_ch, int M, int CH, int ch0, int64_t s0,
+                                              int64_t s_end, float scale, int chan_stride, int plane_elems,
+                                              _Float16 *ylds) {
+  const int n_items = CH * (MF_KS / 8) * M;
+  for (int item = threadIdx.x; item < n_items; item += MF_THREADS) {
+    const int c = item % CH;
+    const int rest = item / CH;
+    const int kg = rest % (MF_KS / 8);
+    const int st = rest / (MF_KS / 8);
+    float re[8], im[8];
+#pragma unroll
+    for (int kk = 0; kk < 8; kk++) {
+      const int64_t seg = s0 + kg * 8 + kk;
+      float2 v = make_float2(0.0f, 0.0f);
+      if (seg < s_end && ch0 + c < n_ch) v = spectra[st * stream_stride + seg * seg_stride + ch0 + c];
+      re[kk] = v.x * scale;
+      im[kk] = v.y * scale;
+    }
+    const int o_re = c * chan_stride + st * MF_RS + kg * 8;
+    const int o_im = c * chan_stride + (M + st) * MF_RS + kg * 8;
+    if (PREC == SFXG_PREC_BF16) {
+      b8_t br, bi;
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) {
+        br[kk] = (__bf16)re[kk];
+        bi[kk] = (__bf16)im[kk];
+      }
+      *reinterpret_cast<b8_t *>(ylds + o_re) = br;
+      *reinterpret_cast<b8_t *>(ylds + o_im) = bi;
+    } else {
+      h8_t hr, hi_;
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) {
+        hr[kk] = (_Float16)re[kk];
+        hi_[kk] = (_Float16)im[kk];
+      }
+      *reinterpret_cast<h8_t *>(ylds + o_re) = hr;
+      *reinterpret_cast<h8_t *>(ylds + o_im) = hi_;
+      if (PREC == SFXG_PREC_SPLIT_FP16) {
+        h8_t lr, li;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          lr[kk] = (_Float16)(re[kk] - (float)hr[kk]);
+          li[kk] = (_Float16)(im[kk] - (float)hi_[kk]);
+        }
+        *reinterpret_cast<h8_t *>(ylds + plane_elems + o_re) = lr;
+        *reinterpret_cast<h8_t *>(ylds + plane_elems + o_im) = li;
+      }
+    }
+  }
+}
+
+// acc += A B^T for one 32 x 32 tile and one stage; A rows at LDS offset oa,
+// B rows at ob (this lane's 8 segments). keep: bit e set keeps segment
+// 8 (lane / 32) + e of the A operand (the others count as zero).
+template <int PREC>
+__device__ __forceinline__ void mf_mma(f16v_t &acc, const _Float16 *ylds, int oa, int ob, int plane_elems,
+                                       unsigned keep) {
+  if (PREC == SFXG_PREC_BF16) {
+    b8_t a = *reinterpret_cast<const b8_t *>(ylds + oa);
+    const b8_t b = *reinterpret_cast<const b8_t *>(ylds + ob);
+    if (keep != 0xffu) {
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (!((keep >> e) & 1u)) a[e] = (__bf16)0.0f;
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  } else {
+    h8_t a = *reinterpret_cast<const h8_t *>(ylds + oa);
+    const h8_t b = *reinterpret_cast<const h8_t *>(ylds + ob);
+    h8_t al;
+    if (PREC == SFXG_PREC_SPLIT_FP16) al = *reinterpret_cast<const h8_t *>(ylds + plane_elems + oa);
+    if (keep != 0xffu) {
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (!((keep >> e) & 1u)) {
+          a[e] = (_Float16)0.0f;
+          if (PREC == SFXG_PREC_SPLIT_FP16) al[e] = (_Float16)0.0f;
+        }
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+    if (PREC == SFXG_PREC_SPLIT_FP16) {
+      const h8_t bl = *reinterpret_cast<const h8_t *>(ylds + plane_elems + ob);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b, acc, 0, 0, 0);
+    }
+  }
+}
+
 template <int PREC>  // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
     const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT,
@@ -928,85 +1038,20 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
     for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
   __syncthreads();
   const int n_pairs = CH * nT;
-  // loader: thread -> (channel, segment group of 8, stream)
-  const int n_items = CH * (MF_KS / 8) * M;
   for (int64_t s0 = s_begin; s0 < s_end; s0 += MF_KS) {
-    for (int item = threadIdx.x; item < n_items; item += MF_THREADS) {
-      const int c = item % CH;
-      const int rest = item / CH;
-      const int kg = rest % (MF_KS / 8);
-      const int st = rest / (MF_KS / 8);
-      float re[8], im[8];
-#pragma unroll
-      for (int kk = 0; kk < 8; kk++) {
-        const int64_t seg = s0 + kg * 8 + kk;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (seg < s_end && ch0 + c < n_ch) v = spectra[st * stream_stride + seg * seg_stride + ch0 + c];
-        re[kk] = v.x * scale;
-        im[kk] = v.y * scale;
-      }
-      const int o_re = c * chan_stride + st * MF_RS + kg * 8;
-      const int o_im = c * chan_stride + (M + st) * MF_RS + kg * 8;
-      if (PREC == SFXG_PREC_BF16) {
-        b8_t br, bi;
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++) {
-          br[kk] = (__bf16)re[kk];
-          bi[kk] = (__bf16)im[kk];
-        }
-        *reinterpret_cast<b8_t *>(ylds + o_re) = br;
-        *reinterpret_cast<b8_t *>(ylds + o_im) = bi;
-      } else {
-        h8_t hr, hi_;
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++) {
-          hr[kk] = (_Float16)re[kk];
-          hi_[kk] = (_Float16)im[kk];
-        }
-        *reinterpret_cast<h8_t *>(ylds + o_re) = hr;
-        *reinterpret_cast<h8_t *>(ylds + o_im) = hi_;
-        if (PREC == SFXG_PREC_SPLIT_FP16) {
-          h8_t lr, li;
-#pragma unroll
-          for (int kk = 0; kk < 8; kk++) {
-            lr[kk] = (_Float16)(re[kk] - (float)hr[kk]);
-            li[kk] = (_Float16)(im[kk] - (float)hi_[kk]);
-          }
-          *reinterpret_cast<h8_t *>(ylds + plane_elems + o_re) = lr;
-          *reinterpret_cast<h8_t *>(ylds + plane_elems + o_im) = li;
-        }
-      }
-    }
+    mf_load_stage<PREC>(spectra, stream_stride, seg_stride, n_ch, M, CH, ch0, s0, s_end, scale, chan_stride,
+                        plane_elems, ylds);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MF_MAXP; q++) {
       const int p = wave + MF_WAVES * q;
       if (p < n_pairs) {
-        const int c = p / nT;
-        int t = p - c * nT, ti = 0;
-        while (t >= RT - ti) {
-          t -= RT - ti;
-          ti++;
-        }
-        const int tj = ti + t;
+        int c, t, ti, tj;
+        mf_pair(p, nT, RT, c, t, ti, tj);
         const int ko = 8 * (lane >> 5);
         const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * MF_RS + ko;
         const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * MF_RS + ko;
-        if (PREC == SFXG_PREC_BF16) {
-          const b8_t a = *reinterpret_cast<const b8_t *>(ylds + oa);
-          const b8_t b = *reinterpret_cast<const b8_t *>(ylds + ob);
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
-        } else {
-          const h8_t a = *reinterpret_cast<const h8_t *>(ylds + oa);
-          const h8_t b = *reinterpret_cast<const h8_t *>(ylds + ob);
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[q], 0, 0, 0);
-          if (PREC == SFXG_PREC_SPLIT_FP16) {
-            const h8_t al = *reinterpret_cast<const h8_t *>(ylds + plane_elems + oa);
-            const h8_t bl = *reinterpret_cast<const h8_t *>(ylds + plane_elems + ob);
-            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bl, acc[q], 0, 0, 0);
-            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b, acc[q], 0, 0, 0);
-          }
-        }
+        mf_mma<PREC>(acc[q], ylds, oa, ob, plane_elems, 0xffu);
       }
     }
     __syncthreads();
@@ -1018,12 +1063,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
     for (int q = 0; q < MF_MAXP; q++) {
       const int p = wave + MF_WAVES * q;
       if (p < n_pairs && p / nT == c) {
-        int t = p - c * nT, ti = 0;
-        while (t >= RT - ti) {
-          t -= RT - ti;
-          ti++;
-        }
-        const int tj = ti + t;
+        int cc, t, ti, tj;
+        mf_pair(p, nT, RT, cc, t, ti, tj);
         const int col = tj * 32 + (lane & 31);
         // D[8 i + 4 (lane / 32) + j][lane % 32] = acc[4 i + j]
 #pragma unroll
@@ -1052,8 +1093,27 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
 // change the running sums are written back and the new bin's sums loaded,
 // which keeps the reference's per-segment addition order.
 // ---------------------------------------------------------------------------
-constexpr int XB = 8;  // baselines per thread in the pulsar kernel
+constexpr int XB = 2;  // baselines per thread in the pulsar kernel
 constexpr int XWAVES = 4;
+
+// Pulse-phase bin of segment sgi in a channel with dispersion offset off_ch
+// (get_phase and the gate test of Correlation_core_pulsar::integration_step,
+// correlation_core_pulsar.h:40-44, correlation_core_pulsar.cc:188-208):
+// 1 + floor((phase - gate_begin) (nbins - 1) / gate length) inside the gate,
+// which may wrap through phase 1, else 0. FP64 without contraction, in the
+// reference's operation order.
+__device__ __forceinline__ int pulsar_bin(int64_t sgi, double off_ch, double start_phase, double phase_coef,
+                                          double fft_duration, double gate_begin, double gate_end, int nbins) {
+#pragma clang fp contract(off)
+  const int64_t us_per_day = (int64_t)24 * 60 * 60 * 1000000;
+  const double len = gate_end - gate_begin;
+  const double obs = start_phase + phase_coef * fft_duration * (double)(int)sgi * 1440 / (double)us_per_day;
+  double phase = obs - off_ch;
+  phase = phase - floor(phase);
+  if (phase >= gate_begin) return phase < gate_end ? (int)((phase - gate_begin) * (nbins - 1) / len) + 1 : 0;
+  if (phase + 1 < gate_end) return (int)((phase + 1 - gate_begin) * (nbins - 1) / len) + 1;
+  return 0;
+}
 
 __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
     const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
@@ -1077,22 +1137,10 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
     acc[q] = make_float2(0.0f, 0.0f);
   }
   const double off_ch = offsets[ch];
-  const double len = gate_end - gate_begin;
-  const int64_t us_per_day = (int64_t)24 * 60 * 60 * 1000000;
   const int64_t bin_stride = (int64_t)n_baselines * n_ch;
   int cur = -1;
   for (int64_t sgi = 0; sgi < n_segments; sgi++) {
-    const double obs = start_phase + phase_coef * fft_duration * (double)(int)sgi * 1440 / (double)us_per_day;
-    double phase = obs - off_ch;
-    phase = phase - floor(phase);
-    int bin;
-    if (phase >= gate_begin) {
-      bin = phase < gate_end ? (int)((phase - gate_begin) * (nbins - 1) / len) + 1 : 0;
-    } else if (phase + 1 < gate_end) {
-      bin = (int)((phase + 1 - gate_begin) * (nbins - 1) / len) + 1;
-    } else {
-      bin = 0;
-    }
+    const int bin = pulsar_bin(sgi, off_ch, start_phase, phase_coef, fft_duration, gate_begin, gate_end, nbins);
     if (bin != cur) {
 #pragma unroll
       for (int q = 0; q < XB; q++) {
@@ -1118,6 +1166,158 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
     for (int q = 0; q < XB; q++)
       if (q < nb) acc_out[cur * bin_stride + (int64_t)(b0 + q) * n_ch + ch] = acc[q];
   }
+}
+
+// ---------------------------------------------------------------------------
+// k_xstep_mfma_pulsar: the mixed-precision X-step with pulsar binning. Same
+// stages and (channel, tile) pairs as k_xstep_mfma; the grid's z slices are
+// ranges of segments. Each stage the workgroup evaluates the bin of its
+// channels' segments (pulsar_bin, FP64) into LDS. A pair keeps the Gram tile
+// of the bin its channel is in; where the bin changes inside a stage the
+// stage is split into runs of equal bin (the other segments masked out of
+// the A operand), and the finished tile is added into
+// gram[bin][ch][tile][32][32] with atomic adds: every (bin, channel) gets
+// contributions from several runs and z slices, in no fixed order.
+// k_pulsar_gram_baselines_run then forms the baselines of every bin.
+// ---------------------------------------------------------------------------
+constexpr int MFP_MAXP = 4;  // (channel, tile) pairs per wave with pulsar binning
+
+template <int PREC>
+__global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT,
+    int CH, int64_t n_segments, int64_t segs_per_z, float scale, const double *__restrict__ offsets,
+    double start_phase, double phase_coef, double fft_duration, double gate_begin, double gate_end, int nbins,
+    float *__restrict__ gram) {
+  extern __shared__ _Float16 ylds[];
+  constexpr int planes = PREC == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
+  const int R = 32 * RT;
+  const int nT = RT * (RT + 1) / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ch0 = blockIdx.x * CH;
+  const int64_t s_begin = (int64_t)blockIdx.z * segs_per_z;
+  int64_t s_end = s_begin + segs_per_z;
+  if (s_end > n_segments) s_end = n_segments;
+  if (s_begin >= s_end) return;  // uniform over the workgroup
+  const int chan_stride = R * MF_RS + 8;
+  const int plane_elems = CH * chan_stride;
+  int16_t *bins = reinterpret_cast<int16_t *>(ylds + planes * plane_elems);  // [CH][MF_KS]
+  for (int i = threadIdx.x; i < planes * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
+  f16v_t acc[MFP_MAXP];
+  int cur[MFP_MAXP];
+#pragma unroll
+  for (int q = 0; q < MFP_MAXP; q++) {
+    cur[q] = -1;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
+  }
+  __syncthreads();
+  const int n_pairs = CH * nT;
+  auto flush = [&](int q, int c, int t) {
+    float *o = gram + (((int64_t)cur[q] * n_ch + ch0 + c) * nT + t) * 1024 + (lane & 31);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        unsafeAtomicAdd(o + (8 * i + 4 * (lane >> 5) + j) * 32, acc[q][4 * i + j]);
+        acc[q][4 * i + j] = 0.0f;
+      }
+  };
+  for (int64_t s0 = s_begin; s0 < s_end; s0 += MF_KS) {
+    mf_load_stage<PREC>(spectra, stream_stride, seg_stride, n_ch, M, CH, ch0, s0, s_end, scale, chan_stride,
+                        plane_elems, ylds);
+    for (int i = threadIdx.x; i < CH * MF_KS; i += MF_THREADS) {
+      const int c = i / MF_KS, k = i % MF_KS;
+      const int64_t seg = s0 + k;
+      bins[i] = (seg < s_end && ch0 + c < n_ch)
+                    ? (int16_t)pulsar_bin(seg, offsets[ch0 + c], start_phase, phase_coef, fft_duration,
+                                          gate_begin, gate_end, nbins)
+                    : (int16_t)-1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MFP_MAXP; q++) {
+      const int p = wave + MF_WAVES * q;
+      if (p < n_pairs) {
+        int c, t, ti, tj;
+        mf_pair(p, nT, RT, c, t, ti, tj);
+        const int ko = 8 * (lane >> 5);
+        const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * MF_RS + ko;
+        const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * MF_RS + ko;
+        const int16_t *bc = bins + c * MF_KS;
+        const int b_first = bc[0];
+        bool one = true;
+        for (int k = 1; k < MF_KS; k++) one = one && bc[k] == b_first;
+        if (one && b_first == cur[q]) {
+          mf_mma<PREC>(acc[q], ylds, oa, ob, plane_elems, 0xffu);
+        } else {
+          // runs of equal bin, in segment order
+          int k = 0;
+          while (k < MF_KS) {
+            const int b = bc[k];
+            int k1 = k + 1;
+            while (k1 < MF_KS && bc[k1] == b) k1++;
+            if (b >= 0) {
+              if (b != cur[q]) {
+                if (cur[q] >= 0 && ch0 + c < n_ch) flush(q, c, t);
+                cur[q] = b;
+              }
+              unsigned keep = 0;
+#pragma unroll
+              for (int e = 0; e < 8; e++) keep |= (unsigned)(ko + e >= k && ko + e < k1) << e;
+              mf_mma<PREC>(acc[q], ylds, oa, ob, plane_elems, keep);
+            }
+            k = k1;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < MFP_MAXP; q++) {
+    const int p = wave + MF_WAVES * q;
+    if (p < n_pairs && cur[q] >= 0) {
+      int c, t, ti, tj;
+      mf_pair(p, nT, RT, c, t, ti, tj);
+      if (ch0 + c < n_ch) flush(q, c, t);
+    }
+  }
+}
+
+// Visibilities of every bin from the binned Gram tiles:
+// vis[bin][b][ch] = (G[a][d] + G[M+a][M+d] + i (G[M+a][d] - G[a][M+d])) / scale^2.
+// gram holds channels ch_base .. ch_base + n_run - 1; vis rows have n_ch channels.
+__global__ __launch_bounds__(256) void k_pulsar_gram_baselines_run(const float *__restrict__ gram, int n_run,
+                                                                   int ch_base, int n_ch, int M, int RT,
+                                                                   const int32_t *__restrict__ bl_first,
+                                                                   const int32_t *__restrict__ bl_second,
+                                                                   int n_baselines, float inv_scale2,
+                                                                   float2 *__restrict__ vis) {
+  const int ch = blockIdx.x, bin = blockIdx.y;
+  const int nT = RT * (RT + 1) / 2;
+  const float *g = gram + ((int64_t)bin * n_run + ch) * nT * 1024;
+  auto G = [&](int r, int c) {
+    if (r > c) {
+      const int tmp = r;
+      r = c;
+      c = tmp;
+    }
+    const int ti = r >> 5, tj = c >> 5;
+    const int t = ti * RT - ti * (ti - 1) / 2 + (tj - ti);
+    return g[t * 1024 + (r & 31) * 32 + (c & 31)];
+  };
+  for (int b = threadIdx.x; b < n_baselines; b += blockDim.x) {
+    const int a = bl_first[b], d = bl_second[b];
+    const float re = (G(a, d) + G(M + a, M + d)) * inv_scale2;
+    const float im = (G(M + a, d) - G(a, M + d)) * inv_scale2;
+    vis[((int64_t)bin * n_baselines + b) * n_ch + ch_base + ch] = make_float2(re, im);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero(float4 *__restrict__ p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1393,6 +1593,93 @@ hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int
   hipLaunchKernelGGL(k_xstep_pulsar, grid, dim3(64 * XWAVES), 0, s, spectra, stream_stride, seg_stride, n_ch,
                      n_segments, bl_first, bl_second, n_baselines, offsets, start_phase, phase_coef,
                      fft_duration, gate_begin, gate_end, nbins, acc);
+  return hipGetLastError();
+}
+
+// channels per workgroup of the MFMA pulsar kernel: every (channel, tile) pair
+// fits MF_WAVES waves x maxp and the stage (plus `extra` bytes) fits 150 KiB of LDS
+static int mf_channels(int precision, int M, int maxp, size_t extra, size_t *lds_out) {
+  const int RT = (2 * M + 31) / 32;
+  const int R = 32 * RT;
+  const int nT = RT * (RT + 1) / 2;
+  const int planes = precision == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
+  if (nT > MF_WAVES * maxp) return 0;
+  int CH = (MF_WAVES * maxp) / nT;
+  if (CH > 64) CH = 64;
+  auto lds_of = [&](int ch) { return (size_t)planes * ch * (R * MF_RS + 8) * 2 + extra; };
+  while (CH > 1 && lds_of(CH) > 150 * 1024) CH--;
+  if (lds_of(CH) > 150 * 1024) return 0;
+  *lds_out = lds_of(CH);
+  return CH;
+}
+
+size_t pulsar_gram_floats(int M, int n_ch, int nbins) {
+  const int RT = (2 * M + 31) / 32;
+  return (size_t)nbins * n_ch * (RT * (RT + 1) / 2) * 1024;
+}
+
+template <int PREC>
+static hipError_t launch_mfma_pulsar_t(dim3 grid, size_t lds, hipStream_t s, const float2 *spectra,
+                                       int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT, int CH,
+                                       int64_t n_segments, int64_t segs_per_z, float scale, const double *offsets,
+                                       double start_phase, double phase_coef, double fft_duration,
+                                       double gate_begin, double gate_end, int nbins, float *gram) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma_pulsar<PREC>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_xstep_mfma_pulsar<PREC>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride, seg_stride,
+                     n_ch, M, RT, CH, n_segments, segs_per_z, scale, offsets, start_phase, phase_coef, fft_duration,
+                     gate_begin, gate_end, nbins, gram);
+  return hipGetLastError();
+}
+
+hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+                                    int n_ch, int ch_base, int n_ch_run, int M, int64_t n_segments,
+                                    const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
+                                    const double *offsets, double start_phase, double phase_coef,
+                                    double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
+                                    float2 *vis, hipStream_t s) {
+  if (n_segments <= 0 || n_ch_run <= 0) return hipSuccess;
+  size_t lds = 0;
+  const int CH = mf_channels(precision, M, MFP_MAXP, (size_t)64 * MF_KS * sizeof(int16_t), &lds);
+  if (CH < 1) return hipErrorInvalidValue;
+  const int RT = (2 * M + 31) / 32;
+  // zero the binned Gram tiles of this channel run
+  const int64_t n4 = (int64_t)(pulsar_gram_floats(M, n_ch_run, nbins) / 4);
+  hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, reinterpret_cast<float4 *>(gram), n4);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // about 1024 workgroups: channel blocks x segment ranges (multiples of a stage)
+  const int64_t nblk = (n_ch_run + CH - 1) / CH;
+  int64_t nz = (1024 + nblk - 1) / nblk;
+  int64_t per = (n_segments + nz - 1) / nz;
+  per = (per + MF_KS - 1) / MF_KS * MF_KS;
+  nz = (n_segments + per - 1) / per;
+  dim3 grid((unsigned)nblk, 1, (unsigned)nz);
+  const float2 *sp = spectra + ch_base;
+  const double *off = offsets + ch_base;
+  switch (precision) {
+    case SFXG_PREC_FP16:
+      e = launch_mfma_pulsar_t<SFXG_PREC_FP16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
+                                               n_segments, per, scale, off, start_phase, phase_coef, fft_duration,
+                                               gate_begin, gate_end, nbins, gram);
+      break;
+    case SFXG_PREC_BF16:
+      e = launch_mfma_pulsar_t<SFXG_PREC_BF16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
+                                               n_segments, per, scale, off, start_phase, phase_coef, fft_duration,
+                                               gate_begin, gate_end, nbins, gram);
+      break;
+    case SFXG_PREC_SPLIT_FP16:
+      e = launch_mfma_pulsar_t<SFXG_PREC_SPLIT_FP16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT,
+                                                     CH, n_segments, per, scale, off, start_phase, phase_coef,
+                                                     fft_duration, gate_begin, gate_end, nbins, gram);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  // vis rows keep the full channel stride; this run writes channels ch_base..
+  hipLaunchKernelGGL(k_pulsar_gram_baselines_run, dim3((unsigned)n_ch_run, (unsigned)nbins), dim3(256), 0, s, gram,
+                     n_ch_run, ch_base, n_ch, M, RT, bl_first, bl_second, n_baselines, 1.0f / (scale * scale), vis);
   return hipGetLastError();
 }
 

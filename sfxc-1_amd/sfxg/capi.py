@@ -243,8 +243,8 @@ def synthesize_raw(device: int, ptr: int, n_bytes: int, bits: int, first_sample:
 
 
 def correlate_slice(ctx: Context, job: CorrelationParameters, headers: Sequence[bytes], lut: int = 0,
-                    pulsar: Optional[Pulsar] = None, keep_time_series: bool = False) -> bytes:
-    ctx.set_options(lut=lut, keep_time_series=keep_time_series)
+                    pulsar: Optional[Pulsar] = None, keep_time_series: bool = False, precision: int = 0) -> bytes:
+    ctx.set_options(lut=lut, precision=precision, keep_time_series=keep_time_series)
     ctx.set_parameters(job, pulsar)
     for i, h in enumerate(headers):
         ctx.push_stream(i, h)

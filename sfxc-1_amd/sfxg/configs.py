@@ -135,6 +135,32 @@ def config2_jobs(subband: int, n_integrations: int = 8, t_int_usec: int = 500_00
                        sky=6_000_000_000 + 64_000_000 * subband)
 
 
+def config4_jobs(subband: int, n_integrations: int = 20, t_int_usec: int = 1_000_000, nchan: int = 1024,
+                 n_stations: int = 12):
+    """Job descriptions of configuration 4 (pulsar binning, no sample data):
+    12 stations x 2 pols, 16 MHz at 32 Ms/s, 2-bit, 1024 channels, 1 s
+    integrations (20 s), delay offsets U[-5, 5] ms and rates U[-3e-6, 3e-6]
+    (seed 4). The configuration leaves the sky frequency open: 1.4 GHz + 16
+    MHz * subband. Returns (jobs, raw_start_ticks, pulsar) with the pulsar of
+    pulsar4()."""
+    jobs, raw_start = survey_jobs(seed=4, subband=subband, n_integrations=n_integrations, t_int_usec=t_int_usec,
+                                  nchan=nchan, n_stations=n_stations, fs=32_000_000, bw=16_000_000,
+                                  sky=1_400_000_000 + 16_000_000 * subband)
+    for j in jobs:
+        j.pulsar_binning = True
+    return jobs, raw_start, pulsar4(jobs[0])
+
+
+def pulsar4(job):
+    """Configuration 4's pulsar: period 0.0893 s, DM 68 pc/cm^3, a linear
+    ephemeris (polyco with constant terms only), 64 bins over the gate
+    [0.25, 0.75) of the pulse phase (bin 0 takes the off-gate segments),
+    reference frequency 1.4 GHz."""
+    from .job import Pulsar, ticks_to_mjd
+    return Pulsar(nbins=64, gate_start=0.25, gate_stop=0.75, tmid=ticks_to_mjd(job.start_time) - 0.001, DM=68.0,
+                  ref_phase=0.0, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+
+
 def survey_jobs(*, seed, subband, n_integrations, t_int_usec, nchan, n_stations, fs, bw, sky):
     rng = np.random.default_rng(seed)
     offsets = rng.uniform(-5e-3, 5e-3, n_stations)
