@@ -148,7 +148,11 @@ typedef struct {
    * Zero (default): for N_c <= 4096 they run fused and the series stays in
    * LDS. Both give identical results. */
   int32_t keep_time_series;
-  int32_t reserved;
+  /* Spectra budget in MiB of one chunk: a slice is processed in chunks of
+   * whole sub-integrations whose spectra fit it (at least one
+   * sub-integration per chunk). 0: 8192. The results do not depend on it
+   * (the FP32 path gives the same bytes). */
+  int32_t chunk_mib;
 } sfxg_options;
 
 /* Creates a context on HIP device `device`. Replaces the construction of
@@ -191,6 +195,32 @@ int sfxg_push_stream_elided(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *co
  * Returns once the work is enqueued. */
 int sfxg_correlate(sfxg_ctx *ctx);
 int sfxg_synchronize(sfxg_ctx *ctx);
+
+/* The host half of sfxg_correlate: HEADER-derived tables, buffers and the
+ * slice's uploads, without the device work, which a batch then runs
+ * (sfxg_batch_launch). Same reference interfaces as sfxg_correlate. */
+int sfxg_prepare(sfxg_ctx *ctx);
+
+/* Batches: the device work of several prepared contexts (slices of any
+ * shape: channel counts, station counts, bit depths, precisions, pulsar or
+ * not) captured into one HIP graph, the contexts' work on up to n_branches
+ * concurrent graph branches, and replayed with one launch. No reference
+ * counterpart: Correlator_node correlates one slice at a time
+ * (src/correlator_node.cc:263-296); a batch runs the slices of several
+ * correlator nodes' jobs as one launch. The graph is captured at the first
+ * launch and again whenever a member context was prepared since the last
+ * capture. Results are read per context (sfxg_fetch_output etc.), which waits
+ * for the batch. A context may not be destroyed while it belongs to a batch,
+ * and sfxg_correlate refuses a batch member. */
+typedef struct sfxg_batch sfxg_batch;
+int sfxg_batch_create(int device, int32_t n_branches, sfxg_batch **out);
+void sfxg_batch_destroy(sfxg_batch *b);
+const char *sfxg_batch_last_error(const sfxg_batch *b);
+int sfxg_batch_add(sfxg_batch *b, sfxg_ctx *ctx);
+int sfxg_batch_clear(sfxg_batch *b);
+/* Asynchronous; waits for the previous launch of the same batch first. */
+int sfxg_batch_launch(sfxg_batch *b);
+int sfxg_batch_synchronize(sfxg_batch *b);
 
 /* Size in bytes of the slice's output: the bytes Correlation_core::integration_write
  * (src/correlation_core.cc:316-477) passes to Data_writer::put_bytes, one

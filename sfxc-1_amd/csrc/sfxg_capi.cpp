@@ -62,6 +62,27 @@ int nr_ffts_per_integration_slice(int64_t integration_ticks, int data_rate, int 
 
 }  // namespace
 
+// A run of whole sub-integrations processed together: segments [seg0, seg1)
+// (folds fold0 .. fold1 - 1); the real series of blocks [blk0, blk1).
+struct Chunk {
+  int64_t seg0, seg1, blk0, blk1;
+  int fold0, fold1;
+};
+
+// Launch parameters of a prepared slice's device work.
+struct LaunchPlan {
+  std::vector<Chunk> chunks;
+  int64_t chunk_segs = 0;   // spectra rows per stream (largest chunk)
+  int64_t ts_stride = 0;    // real-series floats per stream (largest chunk)
+  bool fused = false, mixed = false;
+  int k_group = 1, psr_run = 0;
+  size_t vis_elems = 0;
+  int64_t max_counted = 0;
+  DevJob job{};
+  float mixed_scale = 1.0f;
+  double start_phase = 0.0, phase_coef = 0.0, fft_duration = 0.0;  // pulsar
+};
+
 struct sfxg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -93,7 +114,8 @@ struct sfxg_ctx {
 
   // device buffers
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
-      flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, tw_split, window, offsets, samples;
+      flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, tw_split, window, offsets, samples,
+      chan_scratch;
   // host staging of the enqueued slice (kept until the next slice)
   std::vector<double> h_model, h_offsets;
   std::vector<DevStream> h_streams;
@@ -101,6 +123,16 @@ struct sfxg_ctx {
   int tables_nc = -1;
   int tables_window = -1;
   bool enqueued = false;
+  bool have_results = false;  // the current slice ran (correlate or a batch launch)
+  LaunchPlan lp;
+  bool prepared = false;  // the device work of the current slice can be enqueued
+  int batch_member = 0;   // number of batches holding this context
+  std::vector<struct sfxg_batch *> batches;  // those batches
+  uint64_t prep_gen = 0;  // counts sfxg_prepare calls (graph reuse check)
+  hipEvent_t batch_done = nullptr;  // completion of a batch graph holding this context
+  hipEvent_t prep_done = nullptr;   // the slice's uploads are complete
+  std::vector<hipEvent_t> cev;       // 4 per chunk: F-engine start, delay end, channeliser end, X-step end
+  int timed_chunks = 0;              // chunks of the slice whose events are pending
   hipEvent_t done = nullptr;
   hipEvent_t ev[SFXG_N_KERNELS + 1] = {};
   bool times_pending = false;  // events of an enqueued slice not yet read
@@ -132,18 +164,29 @@ struct sfxg_ctx {
 // (the caller has synchronised the stream).
 static void collect_times(sfxg_ctx *ctx) {
   if (!ctx->times_pending) return;
-  for (int k = 0; k < SFXG_N_KERNELS; k++) {
+  auto span = [](hipEvent_t a, hipEvent_t b) {
+    float t = 0.0f;
+    return hipEventElapsedTime(&t, a, b) == hipSuccess ? (double)t : 0.0;
+  };
+  ctx->kt_sum[SFXG_KERNEL_STATS] += span(ctx->ev[SFXG_KERNEL_STATS], ctx->ev[SFXG_KERNEL_DELAY]);
+  ctx->kt_sum[SFXG_KERNEL_NORMALISE] += span(ctx->ev[SFXG_KERNEL_NORMALISE], ctx->ev[SFXG_N_KERNELS]);
+  for (int c = 0; c < ctx->timed_chunks; c++) {
+    hipEvent_t *e = &ctx->cev[4 * c];
+    ctx->kt_sum[SFXG_KERNEL_DELAY] += span(e[0], e[1]);
     // the fused F-engine runs the channeliser inside the delay stage: no
     // channeliser launch to time
-    if (k == SFXG_KERNEL_CHANNELISE && ctx->fused) continue;
-    float t = 0.0f;
-    if (hipEventElapsedTime(&t, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) ctx->kt_sum[k] += t;
+    if (!ctx->fused) ctx->kt_sum[SFXG_KERNEL_CHANNELISE] += span(e[1], e[2]);
+    ctx->kt_sum[SFXG_KERNEL_XSTEP] += span(e[2], e[3]);
   }
   ctx->kt_count++;
   ctx->times_pending = false;
 }
 
 static void sync_slice(sfxg_ctx *ctx) {
+  if (ctx->batch_done) {
+    (void)hipEventSynchronize(ctx->batch_done);
+    ctx->batch_done = nullptr;
+  }
   if (!ctx->enqueued) return;
   (void)hipStreamSynchronize(ctx->stream);
   collect_times(ctx);
@@ -161,22 +204,29 @@ int sfxg_create(int device, sfxg_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return SFXG_ERR_DEVICE;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return SFXG_ERR_DEVICE;
   if (hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming) != hipSuccess) return SFXG_ERR_DEVICE;
+  if (hipEventCreateWithFlags(&ctx->prep_done, hipEventDisableTiming) != hipSuccess) return SFXG_ERR_DEVICE;
   for (hipEvent_t &e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return SFXG_ERR_DEVICE;
   *out = ctx.release();
   return SFXG_OK;
 }
 
+static void leave_batches(sfxg_ctx *ctx);
+
 void sfxg_destroy(sfxg_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  leave_batches(ctx);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DBuf *bufs[] = {&ctx->payload, &ctx->runs, &ctx->block_run, &ctx->counted, &ctx->dstreams,
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
                   &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
-                  &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples};
+                  &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples, &ctx->chan_scratch};
   for (DBuf *b : bufs) b->release();
+  sync_slice(ctx);
   if (ctx->done) (void)hipEventDestroy(ctx->done);
+  if (ctx->prep_done) (void)hipEventDestroy(ctx->prep_done);
+  for (hipEvent_t e : ctx->cev) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -191,6 +241,7 @@ int sfxg_set_options(sfxg_ctx *ctx, const sfxg_options *opt) {
     return ctx->fail(SFXG_ERR_ARG, "unknown lut %d", opt->lut);
   if (opt->precision < SFXG_PREC_FP32 || opt->precision > SFXG_PREC_SPLIT_FP16)
     return ctx->fail(SFXG_ERR_ARG, "unsupported precision %d", opt->precision);
+  if (opt->chunk_mib < 0) return ctx->fail(SFXG_ERR_ARG, "negative chunk_mib");
   ctx->opt = *opt;
   return SFXG_OK;
 }
@@ -203,6 +254,8 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
   sync_slice(ctx);
   ctx->enqueued = false;
   ctx->have_params = false;
+  ctx->have_results = false;
+  ctx->prepared = false;
   const sfxg_correlation_parameters &p = *params;
   if (p.n_streams <= 0 || !p.station_streams) return ctx->fail(SFXG_ERR_ARG, "no station streams");
   if (p.fft_size_correlation != p.number_channels)
@@ -211,8 +264,8 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
   if (p.n_phase_centers != 1) return ctx->fail(SFXG_ERR_ARG, "only one phase centre is supported");
   if (p.fft_size_delaycor != 256) return ctx->fail(SFXG_ERR_ARG, "fft_size_delaycor must be 256");
   const int nc = p.fft_size_correlation;
-  if (nc < 256 || nc > 16384 || (nc & (nc - 1)))
-    return ctx->fail(SFXG_ERR_ARG, "fft_size_correlation must be a power of two in [256, 16384]");
+  if (nc < 256 || nc > 32768 || (nc & (nc - 1)))
+    return ctx->fail(SFXG_ERR_ARG, "fft_size_correlation must be a power of two in [256, 32768]");
   if (p.window < SFXG_WINDOW_NONE || p.window > SFXG_WINDOW_HANN)
     return ctx->fail(SFXG_ERR_ARG, "unknown window %d", p.window);
   for (int i = 0; i < p.n_streams; i++) {
@@ -305,10 +358,10 @@ static int push_common(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *bytes, 
   if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
   if (stream_idx < 0 || stream_idx >= ctx->M) return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
   if (!bytes && n) return ctx->fail(SFXG_ERR_ARG, "null stream");
-  if (ctx->enqueued) {
-    sync_slice(ctx);
-    ctx->enqueued = false;
-  }
+  sync_slice(ctx);
+  ctx->enqueued = false;
+  ctx->have_results = false;
+  ctx->prepared = false;
   const sfxg_station_parameters &s = ctx->st[stream_idx];
   B2fParams bp;
   bp.bits_per_sample = s.bits_per_sample;
@@ -398,8 +451,12 @@ static void make_tables(sfxg_ctx *ctx, std::vector<float2> &tnd, std::vector<flo
   }
 }
 
-int sfxg_correlate(sfxg_ctx *ctx) {
-  if (!ctx) return SFXG_ERR_ARG;
+}  // extern "C"
+
+// Host part of sfxg_correlate: chunk and invalid-data bookkeeping, device
+// buffers, tables and the slice's uploads (asynchronous, on the context's
+// stream), and the launch parameters of the device work (ctx->lp).
+static int prepare_slice(sfxg_ctx *ctx) {
   if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
   for (int i = 0; i < ctx->M; i++)
     if (!ctx->plans[i].ready) return ctx->fail(SFXG_ERR_STATE, "stream %d has not been pushed", i);
@@ -407,7 +464,7 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     if (ctx->plans[i].chunks != ctx->plans[0].chunks)
       return ctx->fail(SFXG_ERR_STATE, "streams deliver different chunkings");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (ctx->enqueued) sync_slice(ctx);
+  sync_slice(ctx);
   const int M = ctx->M, nd = ctx->nd, nc = ctx->nc, B = ctx->B;
   hipStream_t s = ctx->stream;
 
@@ -486,8 +543,45 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->model_buf.ensure((size_t)mod_tot * sizeof(double)));
   HIPCHK(ctx, ctx->dstreams.ensure((size_t)M * sizeof(DevStream)));
   const bool fused = !ctx->opt.keep_time_series && fengine_supported(nc);
-  if (!fused) HIPCHK(ctx, ctx->time_series.ensure((size_t)M * ctx->n_samples * sizeof(float)));
-  HIPCHK(ctx, ctx->spectra.ensure((size_t)M * ctx->nseg * ctx->S * sizeof(float2)));
+  // chunks of whole sub-integrations whose spectra fit the budget (one
+  // sub-integration at least)
+  LaunchPlan &lp = ctx->lp;
+  {
+    const int64_t bph = nc / nd;  // delay blocks per segment advance
+    const size_t seg_bytes = (size_t)M * ctx->S * sizeof(float2);
+    const size_t chunk_bytes = (size_t)(ctx->opt.chunk_mib > 0 ? ctx->opt.chunk_mib : 8192) << 20;
+    const int64_t budget = std::max<int64_t>(1, (int64_t)(chunk_bytes / seg_bytes));
+    lp.chunks.clear();
+    int f0 = 0;
+    int64_t c0 = 0;
+    const int nf = (int)ctx->folds.size();
+    for (int f = 0; f < nf; f++) {
+      if (ctx->folds[f] - c0 > budget && f > f0) {
+        lp.chunks.push_back({c0, ctx->folds[f - 1], 0, 0, f0, f});
+        c0 = ctx->folds[f - 1];
+        f0 = f;
+      }
+    }
+    lp.chunks.push_back({c0, nf ? (int64_t)ctx->folds[nf - 1] : ctx->nseg, 0, 0, f0, nf});
+    lp.chunk_segs = 0;
+    lp.ts_stride = 0;
+    for (size_t c = 0; c < lp.chunks.size(); c++) {
+      Chunk &ch = lp.chunks[c];
+      ch.blk0 = ch.seg0 * bph;
+      ch.blk1 = c + 1 == lp.chunks.size() ? ctx->n_blocks : std::min<int64_t>(ctx->n_blocks, (ch.seg1 + 1) * bph);
+      lp.chunk_segs = std::max<int64_t>(lp.chunk_segs, ch.seg1 - ch.seg0);
+      lp.ts_stride = std::max<int64_t>(lp.ts_stride, (ch.blk1 - ch.blk0) * nd);
+    }
+    while (ctx->cev.size() < 4 * lp.chunks.size()) {
+      hipEvent_t e;
+      HIPCHK(ctx, hipEventCreate(&e));
+      ctx->cev.push_back(e);
+    }
+  }
+  if (!fused) HIPCHK(ctx, ctx->time_series.ensure((size_t)M * lp.ts_stride * sizeof(float)));
+  if (channelise_scratch_elems(nc) > 0)
+    HIPCHK(ctx, ctx->chan_scratch.ensure(channelise_scratch_elems(nc) * sizeof(float2)));
+  HIPCHK(ctx, ctx->spectra.ensure((size_t)M * lp.chunk_segs * ctx->S * sizeof(float2)));
   const size_t vis_elems = (size_t)ctx->nbins * B * (nc + 1);
   HIPCHK(ctx, ctx->vis.ensure(vis_elems * sizeof(float2)));
   HIPCHK(ctx, ctx->norm_out.ensure(vis_elems * sizeof(float2)));
@@ -505,8 +599,12 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   // channels, at most ~2 GiB at a time
   int psr_run = 0;
   if (mixed && ctx->pulsar) {
+    // all channels in one run up to 8 GiB; longer slices than one chunk need that
     const size_t per_ch = pulsar_gram_floats(M, 1, ctx->nbins) * sizeof(float);
-    psr_run = (int)std::max<size_t>(1, std::min<size_t>((size_t)nc + 1, ((size_t)2 << 30) / per_ch));
+    psr_run = (int)std::max<size_t>(1, std::min<size_t>((size_t)nc + 1, ((size_t)8 << 30) / per_ch));
+    if (psr_run < nc + 1 && lp.chunks.size() > 1)
+      return ctx->fail(SFXG_ERR_ARG, "pulsar binning with mixed precision: the binned Gram tiles of %d channels "
+                                     "exceed 8 GiB for a slice longer than one chunk", nc + 1);
     HIPCHK(ctx, ctx->gram.ensure(per_ch * psr_run));
   }
 
@@ -581,8 +679,8 @@ int sfxg_correlate(sfxg_ctx *ctx) {
     d.fl_sec = d.fl / kClockRate;
     d.oversamp = (int)round(fs / (2.0 * st.bandwidth));
     d.flip = ctx->p.sideband != st.sideband;
-    d.time_series = fused ? nullptr : ctx->time_series.as<float>() + (int64_t)i * ctx->n_samples;
-    d.spectra = ctx->spectra.as<float2>() + (int64_t)i * ctx->nseg * ctx->S;
+    d.time_series = fused ? nullptr : ctx->time_series.as<float>() + (int64_t)i * lp.ts_stride;
+    d.spectra = ctx->spectra.as<float2>() + (int64_t)i * lp.chunk_segs * ctx->S;
     d.samples = nullptr;
     d.hist = ctx->hist.as<int32_t>() + 512 * i;
     d.block_consts = ctx->block_consts.as<BlockConst>() + blk_off[i];
@@ -602,38 +700,23 @@ int sfxg_correlate(sfxg_ctx *ctx) {
   fold_start.insert(fold_start.end(), ctx->folds.begin(), ctx->folds.end());
   HIPCHK(ctx, hipMemcpyAsync(ctx->fold_buf.p, fold_start.data(), fold_start.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, s));
-  HIPCHK(ctx, hipMemsetAsync(ctx->hist.p, 0, (size_t)M * 512 * sizeof(int32_t), s));
-  HIPCHK(ctx, hipMemsetAsync(ctx->vis.p, 0, vis_elems * sizeof(float2), s));
-
-  DevJob job;
-  job.n_streams = M;
-  job.nd = nd;
-  job.nc = nc;
-  job.window = ctx->p.window;
-  job.n_blocks = ctx->n_blocks;
-  job.n_samples = ctx->n_samples;
-  job.n_segments = ctx->nseg;
-  job.spectra_stride = ctx->S;
-  job.seg_step = nc;  // fft_rot_size / 2
-
-  const DevStream *dstr = ctx->dstreams.as<DevStream>();
-  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_STATS], s));
-  HIPCHK(ctx, launch_stats(dstr, M, max_counted, s));
-  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_DELAY], s));
-  if (fused) {
-    HIPCHK(ctx, launch_fengine(dstr, job, ctx->window.as<float>(), ctx->tw_nd.as<float2>(), ctx->tw_n.as<float2>(),
-                               ctx->tw_2n.as<float2>(), s));
-    HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_CHANNELISE], s));
-  } else {
-    HIPCHK(ctx, launch_delay(dstr, job, ctx->tw_nd.as<float2>(), s));
-    HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_CHANNELISE], s));
-    HIPCHK(ctx, launch_channelise(dstr, job, ctx->window.as<float>(), ctx->tw_n.as<float2>(),
-                                  ctx->tw_2n.as<float2>(), ctx->tw_split.as<float2>(), s));
-  }
-  ctx->fused = fused;
-  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_XSTEP], s));
-  const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
-  float mixed_scale = 1.0f;
+  // launch parameters of the device work
+  lp.fused = fused;
+  lp.mixed = mixed;
+  lp.k_group = k_group;
+  lp.psr_run = psr_run;
+  lp.vis_elems = vis_elems;
+  lp.max_counted = max_counted;
+  lp.job.n_streams = M;
+  lp.job.nd = nd;
+  lp.job.nc = nc;
+  lp.job.window = ctx->p.window;
+  lp.job.n_blocks = ctx->n_blocks;
+  lp.job.n_samples = ctx->n_samples;
+  lp.job.n_segments = ctx->nseg;  // per chunk in enqueue_slice
+  lp.job.spectra_stride = ctx->S;
+  lp.job.seg_step = nc;  // fft_rot_size / 2
+  lp.mixed_scale = 1.0f;
   if (mixed) {
     // power-of-two scale that brings the spectra's rms near 16: the delay
     // correction's unnormalised FFT pair gains N_d / 2, the channeliser
@@ -651,28 +734,10 @@ int sfxg_correlate(sfxg_ctx *ctx) {
       lmax = std::max(lmax, ctx->st[i].bits_per_sample == 2 ? fabs(kLut2[lut][0]) : fabs(kLut1[lut][0]));
     }
     const double rms = (nd / 2.0) * lmax * sqrt(std::max(wsum, 1.0));
-    const float scale = (float)ldexp(1.0, (int)lround(log2(16.0 / rms)));
-    ctx->mixed_scale = scale;
-    mixed_scale = scale;
+    lp.mixed_scale = (float)ldexp(1.0, (int)lround(log2(16.0 / rms)));
+    ctx->mixed_scale = lp.mixed_scale;
   }
-  if (mixed && !ctx->pulsar) {
-    const float scale = mixed_scale;
-    const int n_folds = (int)ctx->folds.size();
-    for (int g0 = 0; g0 < n_folds; g0 += k_group) {
-      const int ng = std::min(k_group, n_folds - g0);
-      HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S,
-                                    nc + 1, M, ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, scale,
-                                    ctx->partial.as<float2>(), ctx->vis.as<float2>(), s));
-    }
-  } else if (!ctx->pulsar) {
-    const int n_folds = (int)ctx->folds.size();
-    for (int g0 = 0; g0 < n_folds; g0 += k_group) {
-      const int ng = std::min(k_group, n_folds - g0);
-      HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S, nc + 1, M,
-                               ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),
-                               ctx->vis.as<float2>(), s));
-    }
-  } else {
+  if (ctx->pulsar) {
     // Correlation_core_pulsar::set_parameters (correlation_core_pulsar.cc:46-93)
     const int64_t tpd = (int64_t)86400 * (int64_t)kClockRate;
     const int64_t t = ctx->p.start_time;
@@ -700,29 +765,116 @@ int sfxg_correlate(sfxg_ctx *ctx) {
       offs[i] = 4148.808 * ps.DM * (1 / pow(base_freq + i * dfreq, 2) - inv_freq_obs2) * freq;
     HIPCHK(ctx, ctx->offsets.ensure(offs.size() * sizeof(double)));
     HIPCHK(ctx, hipMemcpyAsync(ctx->offsets.p, offs.data(), offs.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    if (mixed) {
-      for (int c0 = 0; c0 < nc + 1; c0 += psr_run) {
-        const int n_run = std::min(psr_run, nc + 1 - c0);
-        HIPCHK(ctx, launch_xstep_mfma_pulsar(ctx->opt.precision, ctx->spectra.as<float2>(), ctx->nseg * ctx->S,
-                                             ctx->S, nc + 1, c0, n_run, M, ctx->nseg, blf, bls, B, mixed_scale,
-                                             ctx->offsets.as<double>(), start_phase, ps.coef[1] + 60 * ps.ref_freq,
-                                             fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins,
-                                             ctx->gram.as<float>(), ctx->vis.as<float2>(), s));
+    lp.start_phase = start_phase;
+    lp.phase_coef = ps.coef[1] + 60 * ps.ref_freq;
+    lp.fft_duration = fft_duration;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->prep_done, s));
+  ctx->prepared = true;
+  ctx->prep_gen++;
+  return SFXG_OK;
+}
+
+// The device work of a prepared slice on stream s: no allocation, no host
+// synchronisation and no memset nodes, so that it can be captured into a
+// graph. timing: bracket the stages with the context's events.
+static int enqueue_slice(sfxg_ctx *ctx, hipStream_t s, bool timing) {
+  const LaunchPlan &lp = ctx->lp;
+  const int M = ctx->M, nc = ctx->nc, B = ctx->B;
+  const DevJob &job = lp.job;
+  auto mark = [&](int k) { return timing ? hipEventRecord(ctx->ev[k], s) : hipSuccess; };
+  HIPCHK(ctx, launch_fill_zero(ctx->hist.p, (size_t)M * 512 * sizeof(int32_t), s));
+  HIPCHK(ctx, launch_fill_zero(ctx->vis.p, lp.vis_elems * sizeof(float2), s));
+  const DevStream *dstr = ctx->dstreams.as<DevStream>();
+  HIPCHK(ctx, mark(SFXG_KERNEL_STATS));
+  HIPCHK(ctx, launch_stats(dstr, M, lp.max_counted, s));
+  HIPCHK(ctx, mark(SFXG_KERNEL_DELAY));
+  ctx->fused = lp.fused;
+  const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
+  const int64_t sstride = lp.chunk_segs * ctx->S;  // spectra elements per stream
+  auto cmark = [&](size_t c, int k) { return timing ? hipEventRecord(ctx->cev[4 * c + k], s) : hipSuccess; };
+  const sfxg_pulsar &ps = ctx->psr;
+  for (size_t c = 0; c < lp.chunks.size(); c++) {
+    const Chunk &ch = lp.chunks[c];
+    DevJob cj = job;
+    cj.seg_base = ch.seg0;
+    cj.n_segments = ch.seg1 - ch.seg0;
+    cj.blk_base = ch.blk0;
+    cj.blk_end = ch.blk1;
+    HIPCHK(ctx, cmark(c, 0));
+    if (c == 0) HIPCHK(ctx, launch_block_consts(dstr, cj, s));
+    if (lp.fused) {
+      HIPCHK(ctx, launch_fengine(dstr, cj, ctx->window.as<float>(), ctx->tw_nd.as<float2>(), ctx->tw_n.as<float2>(),
+                                 ctx->tw_2n.as<float2>(), s));
+      HIPCHK(ctx, cmark(c, 1));
+    } else {
+      HIPCHK(ctx, launch_delay(dstr, cj, ctx->tw_nd.as<float2>(), s));
+      HIPCHK(ctx, cmark(c, 1));
+      HIPCHK(ctx, launch_channelise(dstr, cj, ctx->window.as<float>(), ctx->tw_n.as<float2>(),
+                                    ctx->tw_2n.as<float2>(), ctx->tw_split.as<float2>(),
+                                    ctx->chan_scratch.as<float2>(), s));
+    }
+    HIPCHK(ctx, cmark(c, 2));
+    if (lp.mixed && !ctx->pulsar) {
+      for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
+        const int ng = std::min(lp.k_group, ch.fold1 - g0);
+        HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0,
+                                      nc + 1, M, ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, lp.mixed_scale,
+                                      ctx->partial.as<float2>(), ctx->vis.as<float2>(), s));
+      }
+    } else if (!ctx->pulsar) {
+      for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
+        const int ng = std::min(lp.k_group, ch.fold1 - g0);
+        HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
+                                 ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),
+                                 ctx->vis.as<float2>(), s));
+      }
+    } else if (lp.mixed) {
+      // one channel run per chunk loop when there are several chunks (checked in prepare)
+      for (int c0 = 0; c0 < nc + 1; c0 += lp.psr_run) {
+        const int n_run = std::min(lp.psr_run, nc + 1 - c0);
+        if (c == 0 || lp.psr_run < nc + 1) HIPCHK(ctx, pulsar_gram_zero(M, n_run, ctx->nbins, ctx->gram.as<float>(), s));
+        HIPCHK(ctx, launch_xstep_mfma_pulsar(ctx->opt.precision, ctx->spectra.as<float2>(), sstride, ctx->S, c0,
+                                             n_run, M, ch.seg0, ch.seg1 - ch.seg0, lp.mixed_scale,
+                                             ctx->offsets.as<double>(), lp.start_phase, lp.phase_coef,
+                                             lp.fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins,
+                                             ctx->gram.as<float>(), s));
+        if (c + 1 == lp.chunks.size() || lp.psr_run < nc + 1)
+          HIPCHK(ctx, pulsar_gram_baselines(ctx->gram.as<float>(), nc + 1, c0, n_run, M, ctx->nbins, blf, bls, B,
+                                            lp.mixed_scale, ctx->vis.as<float2>(), s));
       }
     } else {
-      HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), ctx->nseg * ctx->S, ctx->S, nc + 1, ctx->nseg,
-                                      blf, bls, B, ctx->offsets.as<double>(), start_phase,
-                                      ps.coef[1] + 60 * ps.ref_freq, fft_duration, ps.gate_start, ps.gate_stop,
-                                      ctx->nbins, ctx->vis.as<float2>(), s));
+      HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), sstride, ctx->S, nc + 1, ch.seg0, ch.seg1 - ch.seg0,
+                                      blf, bls, B, ctx->offsets.as<double>(), lp.start_phase, lp.phase_coef,
+                                      lp.fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins,
+                                      ctx->vis.as<float2>(), s));
     }
+    HIPCHK(ctx, cmark(c, 3));
   }
-  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_KERNEL_NORMALISE], s));
+  HIPCHK(ctx, mark(SFXG_KERNEL_NORMALISE));
   HIPCHK(ctx, launch_auto_norms(ctx->vis.as<float2>(), M, nc + 1, nc, ctx->nbins, B, ctx->norms.as<double>(), s));
   HIPCHK(ctx, launch_normalise(ctx->vis.as<float2>(), ctx->norms.as<double>(), blf, bls, ctx->flags.as<double>(), M,
                                B, nc + 1, ctx->nbins, ctx->norm_out.as<float2>(), s));
-  HIPCHK(ctx, hipEventRecord(ctx->ev[SFXG_N_KERNELS], s));
-  HIPCHK(ctx, hipEventRecord(ctx->done, s));
+  HIPCHK(ctx, mark(SFXG_N_KERNELS));
+  return SFXG_OK;
+}
+
+extern "C" {
+
+int sfxg_prepare(sfxg_ctx *ctx) { return ctx ? prepare_slice(ctx) : SFXG_ERR_ARG; }
+
+int sfxg_correlate(sfxg_ctx *ctx) {
+  if (!ctx) return SFXG_ERR_ARG;
+  if (ctx->batch_member)
+    return ctx->fail(SFXG_ERR_STATE, "the context belongs to a batch: sfxg_prepare and sfxg_batch_launch");
+  const int rc = prepare_slice(ctx);
+  if (rc != SFXG_OK) return rc;
+  const int rc2 = enqueue_slice(ctx, ctx->stream, true);
+  if (rc2 != SFXG_OK) return rc2;
+  ctx->timed_chunks = (int)ctx->lp.chunks.size();
+  HIPCHK(ctx, hipEventRecord(ctx->done, ctx->stream));
   ctx->enqueued = true;
+  ctx->have_results = true;
   ctx->times_pending = true;
   return SFXG_OK;
 }
@@ -730,6 +882,10 @@ int sfxg_correlate(sfxg_ctx *ctx) {
 int sfxg_synchronize(sfxg_ctx *ctx) {
   if (!ctx) return SFXG_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (ctx->batch_done) {
+    HIPCHK(ctx, hipEventSynchronize(ctx->batch_done));
+    ctx->batch_done = nullptr;
+  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return SFXG_OK;
 }
@@ -762,10 +918,10 @@ static int stats_of(sfxg_ctx *ctx, std::vector<int32_t> &levels, std::vector<int
 
 int sfxg_get_statistics(sfxg_ctx *ctx, int32_t stream_idx, int32_t levels[5], int32_t tsys[4]) {
   if (!ctx || !levels || !tsys) return SFXG_ERR_ARG;
-  if (!ctx->enqueued) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
+  if (!ctx->have_results) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
   if (stream_idx < 0 || stream_idx >= ctx->M) return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  sync_slice(ctx);
   std::vector<int32_t> lv, ts;
   int rc = stats_of(ctx, lv, ts);
   if (rc) return rc;
@@ -784,11 +940,11 @@ int sfxg_number_of_baselines(sfxg_ctx *ctx, int32_t *n) {
 // integration_write (correlation_core.cc:316-477) with fft_size == number_channels
 int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) {
   if (!ctx || !dst) return SFXG_ERR_ARG;
-  if (!ctx->enqueued) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
+  if (!ctx->have_results) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
   const size_t need = output_size(ctx);
   if (cap < need) return ctx->fail(SFXG_ERR_ARG, "output buffer holds %zu bytes, %zu needed", cap, need);
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  sync_slice(ctx);
   const int M = ctx->M, B = ctx->B, nch = ctx->nchan + 1;
   std::vector<float2> vis((size_t)ctx->nbins * B * nch);
   HIPCHK(ctx, hipMemcpy(vis.data(), ctx->norm_out.p, vis.size() * sizeof(float2), hipMemcpyDeviceToHost));
@@ -857,9 +1013,9 @@ int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) 
 
 int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst, size_t cap, size_t *written) {
   if (!ctx || !dst) return SFXG_ERR_ARG;
-  if (!ctx->enqueued) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
+  if (!ctx->have_results) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  sync_slice(ctx);
   const bool per_stream = stage == SFXG_STAGE_SAMPLES || stage == SFXG_STAGE_TIME_SERIES || stage == SFXG_STAGE_SPECTRA;
   if (per_stream && (stream_idx < 0 || stream_idx >= ctx->M))
     return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
@@ -885,16 +1041,22 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
     case SFXG_STAGE_TIME_SERIES:
       if (ctx->fused)
         return ctx->fail(SFXG_ERR_STATE, "the real series stayed in LDS: set keep_time_series in sfxg_options");
+      if (ctx->lp.chunks.size() != 1)
+        return ctx->fail(SFXG_ERR_STATE, "the slice ran in %d chunks: stages hold the last one only",
+                         (int)ctx->lp.chunks.size());
       need = (size_t)ctx->n_samples * sizeof(float);
       if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
-      HIPCHK(ctx, hipMemcpy(dst, ctx->time_series.as<float>() + (int64_t)stream_idx * ctx->n_samples, need,
+      HIPCHK(ctx, hipMemcpy(dst, ctx->time_series.as<float>() + (int64_t)stream_idx * ctx->lp.ts_stride, need,
                             hipMemcpyDeviceToHost));
       break;
     case SFXG_STAGE_SPECTRA:
+      if (ctx->lp.chunks.size() != 1)
+        return ctx->fail(SFXG_ERR_STATE, "the slice ran in %d chunks: stages hold the last one only",
+                         (int)ctx->lp.chunks.size());
       need = (size_t)ctx->nseg * (ctx->nc + 1) * sizeof(float2);
       if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
       HIPCHK(ctx, hipMemcpy2D(dst, (ctx->nc + 1) * sizeof(float2),
-                              ctx->spectra.as<float2>() + (int64_t)stream_idx * ctx->nseg * ctx->S,
+                              ctx->spectra.as<float2>() + (int64_t)stream_idx * ctx->lp.chunk_segs * ctx->S,
                               ctx->S * sizeof(float2), (ctx->nc + 1) * sizeof(float2), ctx->nseg,
                               hipMemcpyDeviceToHost));
       break;
@@ -936,6 +1098,200 @@ int sfxg_synthesize_raw(int device, void *device_dst, size_t n_bytes, int32_t bi
                         rho, threshold, nullptr) != hipSuccess)
     return SFXG_ERR_DEVICE;
   if (hipDeviceSynchronize() != hipSuccess) return SFXG_ERR_DEVICE;
+  return SFXG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: the device work of several prepared contexts as one graph.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct sfxg_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;           // the graph's origin stream
+  std::vector<hipStream_t> branch;        // fork streams of the capture
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<sfxg_ctx *> ctxs;
+  std::vector<uint64_t> gens;             // prep_gen of each context at capture
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+
+  int fail(int code, const std::string &msg) {
+    err = msg;
+    return code;
+  }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+// A destroyed context leaves every batch that holds it (whose graph is
+// dropped and captured again at the next launch).
+static void leave_batches(sfxg_ctx *ctx) {
+  for (sfxg_batch *b : ctx->batches) {
+    (void)hipStreamSynchronize(b->stream);
+    b->ctxs.erase(std::remove(b->ctxs.begin(), b->ctxs.end(), ctx), b->ctxs.end());
+    b->drop_graph();
+  }
+  ctx->batches.clear();
+  ctx->batch_member = 0;
+  ctx->batch_done = nullptr;
+}
+
+extern "C" {
+
+int sfxg_batch_create(int device, int32_t n_branches, sfxg_batch **out) {
+  if (!out || n_branches < 1 || n_branches > 16) return SFXG_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return SFXG_ERR_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return SFXG_ERR_DEVICE;
+  std::unique_ptr<sfxg_batch> b(new sfxg_batch());
+  b->device = device;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return SFXG_ERR_DEVICE;
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return SFXG_ERR_DEVICE;
+  if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) return SFXG_ERR_DEVICE;
+  for (int i = 0; i < n_branches; i++) {
+    hipStream_t st;
+    hipEvent_t ev;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return SFXG_ERR_DEVICE;
+    b->branch.push_back(st);
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return SFXG_ERR_DEVICE;
+    b->join.push_back(ev);
+  }
+  *out = b.release();
+  return SFXG_OK;
+}
+
+const char *sfxg_batch_last_error(const sfxg_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int sfxg_batch_clear(sfxg_batch *b) {
+  if (!b) return SFXG_ERR_ARG;
+  if (hipSetDevice(b->device) != hipSuccess) return SFXG_ERR_DEVICE;
+  (void)hipStreamSynchronize(b->stream);
+  for (sfxg_ctx *c : b->ctxs) {
+    c->batch_member--;
+    c->batches.erase(std::remove(c->batches.begin(), c->batches.end(), b), c->batches.end());
+    if (c->batch_done == b->done) c->batch_done = nullptr;
+  }
+  b->ctxs.clear();
+  b->gens.clear();
+  b->drop_graph();
+  return SFXG_OK;
+}
+
+void sfxg_batch_destroy(sfxg_batch *b) {
+  if (!b) return;
+  (void)sfxg_batch_clear(b);
+  for (hipStream_t st : b->branch) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : b->join) (void)hipEventDestroy(ev);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int sfxg_batch_add(sfxg_batch *b, sfxg_ctx *ctx) {
+  if (!b || !ctx) return SFXG_ERR_ARG;
+  if (ctx->device != b->device) return b->fail(SFXG_ERR_ARG, "context and batch are on different devices");
+  for (sfxg_ctx *c : b->ctxs)
+    if (c == ctx) return b->fail(SFXG_ERR_ARG, "context already in the batch");
+  if (hipSetDevice(b->device) != hipSuccess) return SFXG_ERR_DEVICE;
+  (void)hipStreamSynchronize(b->stream);
+  b->ctxs.push_back(ctx);
+  ctx->batch_member++;
+  ctx->batches.push_back(b);
+  b->drop_graph();
+  return SFXG_OK;
+}
+
+// Captures the device work of every context, one fork stream per context in
+// turn, into one graph.
+static int batch_capture(sfxg_batch *b) {
+  b->drop_graph();
+  const int nb = (int)b->branch.size();
+  hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return b->fail(SFXG_ERR_DEVICE, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  int rc = SFXG_OK;
+  std::vector<bool> used(nb, false);
+  if (hipEventRecord(b->fork, b->stream) != hipSuccess) rc = SFXG_ERR_DEVICE;
+  for (size_t i = 0; rc == SFXG_OK && i < b->ctxs.size(); i++) {
+    const int j = (int)(i % nb);
+    if (!used[j]) {
+      if (hipStreamWaitEvent(b->branch[j], b->fork, 0) != hipSuccess) {
+        rc = SFXG_ERR_DEVICE;
+        break;
+      }
+      used[j] = true;
+    }
+    rc = enqueue_slice(b->ctxs[i], b->branch[j], false);
+    if (rc != SFXG_OK) b->err = "context " + std::to_string(i) + ": " + b->ctxs[i]->err;
+  }
+  for (int j = 0; j < nb; j++) {
+    if (!used[j]) continue;
+    if (hipEventRecord(b->join[j], b->branch[j]) != hipSuccess ||
+        hipStreamWaitEvent(b->stream, b->join[j], 0) != hipSuccess)
+      if (rc == SFXG_OK) rc = SFXG_ERR_DEVICE;
+  }
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(b->stream, &g);
+  if (rc != SFXG_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return b->err.empty() ? b->fail(rc, "capture failed") : rc;
+  }
+  if (e != hipSuccess) return b->fail(SFXG_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  b->graph = g;
+  e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    b->drop_graph();
+    return b->fail(SFXG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  b->gens.clear();
+  for (sfxg_ctx *c : b->ctxs) b->gens.push_back(c->prep_gen);
+  return SFXG_OK;
+}
+
+int sfxg_batch_launch(sfxg_batch *b) {
+  if (!b) return SFXG_ERR_ARG;
+  if (b->ctxs.empty()) return b->fail(SFXG_ERR_STATE, "empty batch");
+  if (hipSetDevice(b->device) != hipSuccess) return SFXG_ERR_DEVICE;
+  bool same = b->exec != nullptr && b->gens.size() == b->ctxs.size();
+  for (size_t i = 0; i < b->ctxs.size(); i++) {
+    sfxg_ctx *c = b->ctxs[i];
+    if (!c->prepared) return b->fail(SFXG_ERR_STATE, "context " + std::to_string(i) + " is not prepared");
+    if (same && b->gens[i] != c->prep_gen) same = false;
+  }
+  // a replay must not overlap the previous one (same buffers)
+  hipError_t e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return b->fail(SFXG_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  if (!same) {
+    const int rc = batch_capture(b);
+    if (rc != SFXG_OK) return rc;
+  }
+  for (sfxg_ctx *c : b->ctxs)
+    if (hipStreamWaitEvent(b->stream, c->prep_done, 0) != hipSuccess) return SFXG_ERR_DEVICE;
+  e = hipGraphLaunch(b->exec, b->stream);
+  if (e != hipSuccess) return b->fail(SFXG_ERR_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+  if (hipEventRecord(b->done, b->stream) != hipSuccess) return SFXG_ERR_DEVICE;
+  for (sfxg_ctx *c : b->ctxs) {
+    c->batch_done = b->done;
+    c->enqueued = false;
+    c->have_results = true;
+    c->times_pending = false;
+  }
+  return SFXG_OK;
+}
+
+int sfxg_batch_synchronize(sfxg_batch *b) {
+  if (!b) return SFXG_ERR_ARG;
+  if (hipSetDevice(b->device) != hipSuccess) return SFXG_ERR_DEVICE;
+  const hipError_t e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return b->fail(SFXG_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
   return SFXG_OK;
 }
 

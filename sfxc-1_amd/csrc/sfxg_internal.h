@@ -140,47 +140,64 @@ struct DevStream {
   BlockConst *block_consts;  // [n_blocks]
 };
 
+// A slice is processed in chunks of whole sub-integrations; the F-engine
+// kernels see one chunk: segments seg_base .. seg_base + n_segments - 1 of the
+// slice, whose spectra rows (and, unfused, real series from block blk_base)
+// start at the buffers' beginning.
 struct DevJob {
   int32_t n_streams;
   int32_t nd;               // fft_size_delaycor
   int32_t nc;               // fft_size_correlation
   int32_t window;
-  int64_t n_blocks;         // N_d blocks per stream
+  int64_t n_blocks;         // N_d blocks per stream in the slice
   int64_t n_samples;        // n_blocks * nd
-  int64_t n_segments;
+  int64_t n_segments;       // segments of this chunk
   int64_t spectra_stride;   // complex elements per segment row
   int64_t seg_step;         // fft_rot_size / 2 real samples between segments
+  int64_t seg_base;         // first segment of the chunk
+  int64_t blk_base;         // first block of the chunk's real series (seg_base * nc / nd)
+  int64_t blk_end;          // end of the chunk's blocks (k_delay)
 };
 
 // kernel launchers (sfxg_kernels.hip)
 hipError_t launch_unpack(const DevStream *d_streams, int n_streams, int64_t n_samples, hipStream_t s);
 hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_counted_bytes,
                         hipStream_t s);
+// Per-block delay constants and run indices of the whole slice (once, before
+// the first chunk's F-engine).
+hipError_t launch_block_consts(const DevStream *d_streams, const DevJob &job, hipStream_t s);
 hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const float2 *tw_nd,
                         hipStream_t s);
 // Fused unpack + delay correction + channeliser (the real series stays in
-// LDS) for N_c <= 4096; launches k_block_consts first.
+// LDS) for N_c <= 4096.
 bool fengine_supported(int nc);
 hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window,
                           const float2 *tw_nd, const float2 *tw_n, const float2 *tw_2n, hipStream_t s);
 // tw_split: [2 N_c / 128] coarse then [128] fine twiddles of exp(-2 pi i j / 2 N_c)
 // (the LDS tables of the large-N channeliser).
+// N_c = 32768 goes through HBM scratch of channelise_scratch_elems(nc) float2
+// (0 for the LDS-resident sizes).
+size_t channelise_scratch_elems(int nc);
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
-                             const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, hipStream_t s);
+                             const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
+                             hipStream_t s);
 // Sub-integration partial sums (fold_start[0..n_k]: segment ranges) and
 // their in-order fold into vis.
-hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M,
-                        const int32_t *fold_start, int n_k, const int32_t *bl_first, const int32_t *bl_second,
-                        int n_baselines, float2 *partial, float2 *vis, hipStream_t s);
+// spectra rows hold segments seg_base ..; fold_start holds slice segment indices.
+hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
+                        int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
+                        const int32_t *bl_second, int n_baselines, float2 *partial, float2 *vis, hipStream_t s);
 // Mixed-precision X-step (SFXG_PREC_FP16 / BF16 / SPLIT_FP16): per-channel
 // Gram matrices on MFMA per sub-integration, formed into baselines in
 // partial (n_k * n_baselines * n_ch float2) and combined into vis in fold order.
 hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
-                             int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
-                             const int32_t *bl_second, int n_baselines, float scale, float2 *partial, float2 *vis,
-                             hipStream_t s);
+                             int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
+                             const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
+                             float2 *partial, float2 *vis, hipStream_t s);
+// segments seg_base .. seg_base + n_segments - 1 (spectra rows from seg_base),
+// continuing the bins' sums in acc.
 hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
-                               int n_ch, int64_t n_segments, const int32_t *bl_first,
+                               int n_ch, int64_t seg_base, int64_t n_segments, const int32_t *bl_first,
                                const int32_t *bl_second, int n_baselines, const double *offsets,
                                double start_phase, double phase_coef, double fft_duration,
                                double gate_begin, double gate_end, int nbins, float2 *acc,
@@ -190,12 +207,21 @@ hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int
 // tiles in gram (pulsar_gram_floats(M, n_ch_run, nbins) floats, zeroed here),
 // then vis[bin][b][ch] for those channels.
 size_t pulsar_gram_floats(int M, int n_ch, int nbins);
+// In two calls: zero the binned Gram tiles (pulsar_gram_zero, before the
+// slice's first chunk), accumulate segments seg_base .. seg_base + n_segments
+// - 1 of each chunk (launch_xstep_mfma_pulsar), form the baselines after the
+// last chunk (pulsar_gram_baselines).
+hipError_t pulsar_gram_zero(int M, int n_ch_run, int nbins, float *gram, hipStream_t s);
 hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
-                                    int n_ch, int ch_base, int n_ch_run, int M, int64_t n_segments,
-                                    const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
-                                    const double *offsets, double start_phase, double phase_coef,
+                                    int ch_base, int n_ch_run, int M, int64_t seg_base, int64_t n_segments,
+                                    float scale, const double *offsets, double start_phase, double phase_coef,
                                     double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
-                                    float2 *vis, hipStream_t s);
+                                    hipStream_t s);
+hipError_t pulsar_gram_baselines(const float *gram, int n_ch, int ch_base, int n_ch_run, int M, int nbins,
+                                 const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
+                                 float2 *vis, hipStream_t s);
+// Zeroes `bytes` (a multiple of 4) with a kernel (no memset node in graphs).
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_auto_norms(const float2 *vis, int n_streams, int n_ch, int fft_size, int n_bins,
                              int n_baselines, double *norms, hipStream_t s);
 hipError_t launch_normalise(const float2 *vis, const double *norms, const int32_t *bl_first,

@@ -358,10 +358,11 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
   tw[threadIdx.x] = tw_g[threadIdx.x];
   __syncthreads();
   const DevStream &s = streams[blockIdx.y];
-  const int64_t blk = ((int64_t)blockIdx.x * 4 + wave) * 2;
-  if (blk >= job.n_blocks) return;
-  float *base = s.time_series + blk * 256;
-  delay_block_pair(s, blk, job.n_blocks, lds[wave], tw, lane, base, blk + 1 < job.n_blocks ? base + 256 : nullptr);
+  const int64_t blk = job.blk_base + ((int64_t)blockIdx.x * 4 + wave) * 2;
+  const int64_t end = job.blk_end < job.n_blocks ? job.blk_end : job.n_blocks;
+  if (blk >= end) return;
+  float *base = s.time_series + (blk - job.blk_base) * 256;
+  delay_block_pair(s, blk, job.n_blocks, lds[wave], tw, lane, base, blk + 1 < end ? base + 256 : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -675,6 +676,97 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
 }
 
 // ---------------------------------------------------------------------------
+// Channeliser for N = 32768 complex points (2 N_c = 65536 reals), which do
+// not fit in LDS: a four-step FFT through HBM scratch, N = 128 x 256,
+//   X[k1 + 128 k2] = sum_n2 W256^(n2 k2) W_N^(n2 k1) sum_n1 x[256 n1 + n2] W128^(n1 k1).
+// k_chan_cols: 128-point FFTs over n1 for 16 columns n2 per workgroup, times
+// W_N^(n2 k1), into Y[k1][n2]; k_chan_rows: 256-point FFTs over n2 for 16 rows
+// k1 into Z in natural order; k_chan_split: the N + 1 bins of the real
+// transform. Work items are (stream, segment) pairs item0 + blockIdx.y; the
+// scratch holds one N-point row per item of the launch.
+// ---------------------------------------------------------------------------
+constexpr int LG_N1 = 128, LG_N2 = 256, LG_N = LG_N1 * LG_N2;
+
+struct StrideTw {  // exp(-2 pi i m / n) = tw_N[m * stride], stride = N / n
+  const float2 *__restrict__ tw;
+  int stride;
+  __device__ __forceinline__ float2 fft(int m) const { return tw[m * stride]; }
+};
+
+__global__ __launch_bounds__(256) void k_chan_cols(const DevStream *__restrict__ streams, DevJob job,
+                                                   const float *__restrict__ window, const float2 *__restrict__ tw_n,
+                                                   int64_t item0, float2 *__restrict__ Y) {
+  constexpr int C = 16, P = 8, T1 = LG_N1 / P;  // 16 column FFTs of 16 threads
+  constexpr int RS = fft_lds_size<LG_N1, P>() + 1;  // odd region stride
+  __shared__ float2 lds[C * RS];
+  const int64_t item = item0 + blockIdx.y;
+  const int st = (int)(item / job.n_segments);
+  const int64_t seg = item - (int64_t)st * job.n_segments;
+  const DevStream &s = streams[st];
+  const float2 *x = reinterpret_cast<const float2 *>(s.time_series + seg * job.seg_step);
+  const float2 *w = reinterpret_cast<const float2 *>(window);
+  const bool none = job.window == SFXG_WINDOW_NONE;
+  const int c = threadIdx.x % C, tid = threadIdx.x / C;  // consecutive threads: consecutive columns
+  const int n2 = blockIdx.x * C + c;
+  float2 v[P];
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    const int m = LG_N2 * (tid + i * T1) + n2;
+    float2 z = make_float2(0.0f, 0.0f);
+    if (!(none && m >= LG_N / 2)) {
+      const float2 xv = x[m], wv = w[m];
+      z = make_float2(xv.x * wv.x, xv.y * wv.y);
+    }
+    if (s.flip) z.y = -z.y;
+    v[i] = z;
+  }
+  float2 *reg = lds + c * RS;
+  reg_fft<LG_N1, P>(v, reg, StrideTw{tw_n, LG_N / LG_N1}, tid);
+  float2 *y = Y + (int64_t)blockIdx.y * LG_N;
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    const int k1 = tid + i * T1;
+    y[k1 * LG_N2 + n2] = cmul(reg[lds_index<P>(k1)], tw_n[n2 * k1]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_chan_rows(const float2 *__restrict__ Y, const float2 *__restrict__ tw_n,
+                                                   float2 *__restrict__ Z) {
+  constexpr int R = 16, P = 16, T1 = LG_N2 / P;  // 16 row FFTs of 16 threads
+  constexpr int RS = fft_lds_size<LG_N2, P>() + 1;
+  __shared__ float2 lds[R * RS];
+  const int tid = threadIdx.x % T1, r = threadIdx.x / T1;  // consecutive threads: consecutive n2
+  const int k1 = blockIdx.x * R + r;
+  const float2 *y = Y + (int64_t)blockIdx.y * LG_N + k1 * LG_N2;
+  float2 v[P];
+#pragma unroll
+  for (int i = 0; i < P; i++) v[i] = y[tid + i * T1];
+  reg_fft<LG_N2, P>(v, lds + r * RS, StrideTw{tw_n, LG_N / LG_N2}, tid);
+  // Z[k1 + 128 k2]: consecutive threads take consecutive k1 of the block
+  float2 *z = Z + (int64_t)blockIdx.y * LG_N;
+  for (int f = threadIdx.x; f < R * LG_N2; f += 256) {
+    const int rr = f % R, k2 = f / R;
+    z[blockIdx.x * R + rr + LG_N1 * k2] = lds[rr * RS + lds_index<P>(k2)];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_chan_split(const DevStream *__restrict__ streams, DevJob job,
+                                                    const float2 *__restrict__ Z, const float2 *__restrict__ tw_2n,
+                                                    int64_t item0) {
+  const int64_t item = item0 + blockIdx.x;
+  const int st = (int)(item / job.n_segments);
+  const int64_t seg = item - (int64_t)st * job.n_segments;
+  const float2 *z = Z + (int64_t)blockIdx.x * LG_N;
+  float2 *out = streams[st].spectra + seg * job.spectra_stride;
+  for (int k = threadIdx.x; k <= LG_N; k += 256) {
+    const float2 zk = z[k & (LG_N - 1)], zn = z[(LG_N - k) & (LG_N - 1)];
+    const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+    const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+    out[k] = cadd(e, cmul(tw_2n[k], o));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_fengine: Delay_correction and the channeliser fused. A workgroup walks a
 // contiguous range of segments of one stream. The real series lives in an
 // LDS ring of two N_c halves: each step computes the next N_c samples
@@ -710,7 +802,7 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
   // blocks go in pairs (blk, blk + 1) with blk even, as in k_delay
   if (!none) {
     for (int b = 2 * wave; b < BPH; b += 2 * nwaves)
-      delay_block_pair(s, sa * BPH + b, job.n_blocks, scratch, tw, lane, ring + b * 256,
+      delay_block_pair(s, (job.seg_base + sa) * BPH + b, job.n_blocks, scratch, tw, lane, ring + b * 256,
                        b + 1 < BPH ? ring + (b + 1) * 256 : nullptr);
     __syncthreads();
   }
@@ -718,7 +810,7 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
   for (int64_t seg = sa; seg < sb; seg++) {
     const int h_old = (int)((seg - sa) & 1);
     const int h_new = none ? 0 : 1 - h_old;
-    const int64_t blk0 = none ? seg * BPH : (seg + 1) * BPH;
+    const int64_t blk0 = (job.seg_base + seg + (none ? 0 : 1)) * BPH;
     for (int b = 2 * wave; b < BPH; b += 2 * nwaves)
       delay_block_pair(s, blk0 + b, job.n_blocks, scratch, tw, lane, ring + h_new * N + b * 256,
                        b + 1 < BPH ? ring + h_new * N + (b + 1) * 256 : nullptr);
@@ -772,7 +864,7 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
 constexpr int XBPW = 16;  // baselines per wave
 
 __global__ __launch_bounds__(512) void k_xstep_subint(const float2 *__restrict__ spectra, int64_t stream_stride,
-                                                      int64_t seg_stride, int n_ch, int M,
+                                                      int64_t seg_stride, int64_t seg_base, int n_ch, int M,
                                                       const int32_t *__restrict__ fold_start,
                                                       const int32_t *__restrict__ bl_first,
                                                       const int32_t *__restrict__ bl_second, int n_baselines,
@@ -814,7 +906,7 @@ __global__ __launch_bounds__(512) void k_xstep_subint(const float2 *__restrict__
         const int c = idx & 63;
         const int r = idx >> 6;
         const int sl = r / M, st = r - sl * M;
-        if (ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + sl) * seg_stride + ch0 + c];
+        if (ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + sl - seg_base) * seg_stride + ch0 + c];
       }
       pf[u] = v;
     }
@@ -921,12 +1013,14 @@ __device__ __forceinline__ void mf_pair(int p, int nT, int RT, int &c, int &t, i
 }
 
 // Loads segments [s0, s0 + MF_KS) (zeros at and past s_end, and for channels
-// past n_ch) of CH channels from ch0 into the LDS planes as Y[c][r][k]:
+// past n_ch; spectra rows start at segment seg_base) of CH channels from ch0
+// into the LDS planes as Y[c][r][k]:
 // consecutive threads take consecutive channels, each thread 8 segments of
 // one stream, converted to the MFMA operand type and stored 16 bytes at a time.
 template <int PREC>
 __device__ __forceinline__ void mf_load_stage(const float2 *__restrict__ spectra, int64_t stream_stride,
-                                              int64_t seg_stride, int n_ch, int M, int CH, int ch0, int64_t s0,
+                                              int64_t seg_stride, int64_t seg_base, int n_ch, int M, int CH,
+                                              int ch0, int64_t s0,
                                               int64_t s_end, float scale, int chan_stride, int plane_elems,
                                               _Float16 *ylds) {
   const int n_items = CH * (MF_KS / 8) * M;
@@ -940,7 +1034,7 @@ __device__ __forceinline__ void mf_load_stage(const float2 *__restrict__ spectra
     for (int kk = 0; kk < 8; kk++) {
       const int64_t seg = s0 + kg * 8 + kk;
       float2 v = make_float2(0.0f, 0.0f);
-      if (seg < s_end && ch0 + c < n_ch) v = spectra[st * stream_stride + seg * seg_stride + ch0 + c];
+      if (seg < s_end && ch0 + c < n_ch) v = spectra[st * stream_stride + (seg - seg_base) * seg_stride + ch0 + c];
       re[kk] = v.x * scale;
       im[kk] = v.y * scale;
     }
@@ -1017,8 +1111,8 @@ __device__ __forceinline__ void mf_mma(f16v_t &acc, const _Float16 *ylds, int oa
 
 template <int PREC>  // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
-    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT,
-    int CH, const int32_t *__restrict__ fold_start, float scale, const int32_t *__restrict__ bl_first,
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch, int M,
+    int RT, int CH, const int32_t *__restrict__ fold_start, float scale, const int32_t *__restrict__ bl_first,
     const int32_t *__restrict__ bl_second, int n_baselines, float inv_scale2, float2 *__restrict__ partial) {
   extern __shared__ _Float16 ylds[];
   constexpr int planes = PREC == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
@@ -1039,8 +1133,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
   __syncthreads();
   const int n_pairs = CH * nT;
   for (int64_t s0 = s_begin; s0 < s_end; s0 += MF_KS) {
-    mf_load_stage<PREC>(spectra, stream_stride, seg_stride, n_ch, M, CH, ch0, s0, s_end, scale, chan_stride,
-                        plane_elems, ylds);
+    mf_load_stage<PREC>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
+                        chan_stride, plane_elems, ylds);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MF_MAXP; q++) {
@@ -1116,7 +1210,7 @@ __device__ __forceinline__ int pulsar_bin(int64_t sgi, double off_ch, double sta
 }
 
 __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
-    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int64_t seg_base,
     int64_t n_segments, const int32_t *__restrict__ bl_first, const int32_t *__restrict__ bl_second,
     int n_baselines, const double *__restrict__ offsets, double start_phase, double phase_coef,
     double fft_duration, double gate_begin, double gate_end, int nbins, float2 *__restrict__ acc_out) {
@@ -1139,7 +1233,7 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
   const double off_ch = offsets[ch];
   const int64_t bin_stride = (int64_t)n_baselines * n_ch;
   int cur = -1;
-  for (int64_t sgi = 0; sgi < n_segments; sgi++) {
+  for (int64_t sgi = seg_base; sgi < seg_base + n_segments; sgi++) {
     const int bin = pulsar_bin(sgi, off_ch, start_phase, phase_coef, fft_duration, gate_begin, gate_end, nbins);
     if (bin != cur) {
 #pragma unroll
@@ -1151,7 +1245,7 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
       }
       cur = bin;
     }
-    const int64_t off = sgi * seg_stride;
+    const int64_t off = (sgi - seg_base) * seg_stride;
 #pragma unroll
     for (int q = 0; q < XB; q++) {
       const float2 x = pa[q][off], y = pb[q][off];
@@ -1185,7 +1279,7 @@ constexpr int MFP_MAXP = 4;  // (channel, tile) pairs per wave with pulsar binni
 template <int PREC>
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
     const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT,
-    int CH, int64_t n_segments, int64_t segs_per_z, float scale, const double *__restrict__ offsets,
+    int CH, int64_t seg_base, int64_t n_segments, int64_t segs_per_z, float scale, const double *__restrict__ offsets,
     double start_phase, double phase_coef, double fft_duration, double gate_begin, double gate_end, int nbins,
     float *__restrict__ gram) {
   extern __shared__ _Float16 ylds[];
@@ -1194,9 +1288,9 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
   const int nT = RT * (RT + 1) / 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ch0 = blockIdx.x * CH;
-  const int64_t s_begin = (int64_t)blockIdx.z * segs_per_z;
+  const int64_t s_begin = seg_base + (int64_t)blockIdx.z * segs_per_z;
   int64_t s_end = s_begin + segs_per_z;
-  if (s_end > n_segments) s_end = n_segments;
+  if (s_end > seg_base + n_segments) s_end = seg_base + n_segments;
   if (s_begin >= s_end) return;  // uniform over the workgroup
   const int chan_stride = R * MF_RS + 8;
   const int plane_elems = CH * chan_stride;
@@ -1223,8 +1317,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
       }
   };
   for (int64_t s0 = s_begin; s0 < s_end; s0 += MF_KS) {
-    mf_load_stage<PREC>(spectra, stream_stride, seg_stride, n_ch, M, CH, ch0, s0, s_end, scale, chan_stride,
-                        plane_elems, ylds);
+    mf_load_stage<PREC>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
+                        chan_stride, plane_elems, ylds);
     for (int i = threadIdx.x; i < CH * MF_KS; i += MF_THREADS) {
       const int c = i / MF_KS, k = i % MF_KS;
       const int64_t seg = s0 + k;
@@ -1318,6 +1412,11 @@ __global__ __launch_bounds__(256) void k_fill_zero(float4 *__restrict__ p, int64
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero_u32(uint32_t *__restrict__ p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1419,13 +1518,19 @@ hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_c
   return hipGetLastError();
 }
 
-hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const float2 *tw_nd, hipStream_t s) {
+hipError_t launch_block_consts(const DevStream *d_streams, const DevJob &job, hipStream_t s) {
   if (job.nd != 256) return hipErrorInvalidValue;
+  if (job.n_blocks <= 0) return hipSuccess;
   dim3 gridc((unsigned)((job.n_blocks + 255) / 256), job.n_streams);
   hipLaunchKernelGGL(k_block_consts, gridc, dim3(256), 0, s, d_streams, job.n_blocks, job.nd);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  dim3 grid((unsigned)((job.n_blocks + 7) / 8), job.n_streams);
+  return hipGetLastError();
+}
+
+hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const float2 *tw_nd, hipStream_t s) {
+  if (job.nd != 256) return hipErrorInvalidValue;
+  const int64_t end = job.blk_end < job.n_blocks ? job.blk_end : job.n_blocks;
+  if (end <= job.blk_base) return hipSuccess;
+  dim3 grid((unsigned)((end - job.blk_base + 7) / 8), job.n_streams);
   hipLaunchKernelGGL(k_delay, grid, dim3(256), 0, s, d_streams, job, tw_nd);
   return hipGetLastError();
 }
@@ -1453,9 +1558,31 @@ static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, c
   return hipGetLastError();
 }
 
+size_t channelise_scratch_elems(int nc) { return nc == LG_N ? (size_t)2 * 1024 * LG_N : 0; }
+
+static hipError_t launch_chan_large(const DevStream *d_streams, const DevJob &job, const float *window,
+                                    const float2 *tw_n, const float2 *tw_2n, float2 *scratch, hipStream_t s) {
+  // scratch: Y and Z for up to 1024 (stream, segment) items per launch
+  constexpr int64_t kItems = 1024;
+  if (job.seg_step != LG_N) return hipErrorInvalidValue;
+  float2 *Y = scratch, *Z = scratch + kItems * LG_N;
+  const int64_t total = (int64_t)job.n_streams * job.n_segments;
+  for (int64_t i0 = 0; i0 < total; i0 += kItems) {
+    const unsigned n = (unsigned)(total - i0 < kItems ? total - i0 : kItems);
+    hipLaunchKernelGGL(k_chan_cols, dim3(LG_N2 / 16, n), dim3(256), 0, s, d_streams, job, window, tw_n, i0, Y);
+    hipLaunchKernelGGL(k_chan_rows, dim3(LG_N1 / 16, n), dim3(256), 0, s, Y, tw_n, Z);
+    hipLaunchKernelGGL(k_chan_split, dim3(n), dim3(256), 0, s, d_streams, job, Z, tw_2n, i0);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
-                             const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, hipStream_t s) {
+                             const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
+                             hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
+  if (job.nc == LG_N) return launch_chan_large(d_streams, job, window, tw_n, tw_2n, scratch, s);
   switch (job.nc) {
     case 256: return launch_chan_n<256>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
     case 512: return launch_chan_n<512>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
@@ -1487,10 +1614,6 @@ hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const f
                           const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
   if (job.nd != 256) return hipErrorInvalidValue;
-  dim3 gridc((unsigned)((job.n_blocks + 255) / 256), job.n_streams);
-  hipLaunchKernelGGL(k_block_consts, gridc, dim3(256), 0, s, d_streams, job.n_blocks, job.nd);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   switch (job.nc) {
     case 256: return launch_fe_n<256, 64>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
     case 512: return launch_fe_n<512, 128>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
@@ -1501,9 +1624,9 @@ hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const f
   }
 }
 
-hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M,
-                        const int32_t *fold_start, int n_k, const int32_t *bl_first, const int32_t *bl_second,
-                        int n_baselines, float2 *partial, float2 *vis, hipStream_t s) {
+hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
+                        int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
+                        const int32_t *bl_second, int n_baselines, float2 *partial, float2 *vis, hipStream_t s) {
   if (n_k <= 0) return hipSuccess;
   int nw = (n_baselines + XBPW - 1) / XBPW;
   if (nw > 8) nw = 8;
@@ -1515,8 +1638,8 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
   const size_t lds = (size_t)sps * M * 64 * sizeof(float2);
   if (lds > 65536) return hipErrorInvalidValue;
   dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)bl_tiles, (unsigned)n_k);
-  hipLaunchKernelGGL(k_xstep_subint, grid, dim3(64 * nw), lds, s, spectra, stream_stride, seg_stride, n_ch, M,
-                     fold_start, bl_first, bl_second, n_baselines, sps, partial);
+  hipLaunchKernelGGL(k_xstep_subint, grid, dim3(64 * nw), lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch,
+                     M, fold_start, bl_first, bl_second, n_baselines, sps, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int64_t n = (int64_t)n_baselines * n_ch;
@@ -1525,9 +1648,9 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
 }
 
 hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
-                             int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
-                             const int32_t *bl_second, int n_baselines, float scale, float2 *partial, float2 *vis,
-                             hipStream_t s) {
+                             int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
+                             const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
+                             float2 *partial, float2 *vis, hipStream_t s) {
   if (n_k <= 0) return hipSuccess;
   const int RT = (2 * M + 31) / 32;
   const int R = 32 * RT;
@@ -1555,24 +1678,24 @@ hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t strea
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_FP16>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride,
-                         seg_stride, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2,
-                         partial);
+                         seg_stride, seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second,
+                         n_baselines, inv2, partial);
       break;
     case SFXG_PREC_BF16:
       e = hipFuncSetAttribute((const void *)k_xstep_mfma<SFXG_PREC_BF16>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_BF16>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride,
-                         seg_stride, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2,
-                         partial);
+                         seg_stride, seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second,
+                         n_baselines, inv2, partial);
       break;
     case SFXG_PREC_SPLIT_FP16:
       e = hipFuncSetAttribute((const void *)k_xstep_mfma<SFXG_PREC_SPLIT_FP16>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_SPLIT_FP16>, grid, dim3(MF_THREADS), lds, s, spectra,
-                         stream_stride, seg_stride, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second,
-                         n_baselines, inv2, partial);
+                         stream_stride, seg_stride, seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first,
+                         bl_second, n_baselines, inv2, partial);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1585,13 +1708,14 @@ hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t strea
 }
 
 hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int n_ch,
-                               int64_t n_segments, const int32_t *bl_first, const int32_t *bl_second,
-                               int n_baselines, const double *offsets, double start_phase, double phase_coef,
-                               double fft_duration, double gate_begin, double gate_end, int nbins, float2 *acc,
-                               hipStream_t s) {
+                               int64_t seg_base, int64_t n_segments, const int32_t *bl_first,
+                               const int32_t *bl_second, int n_baselines, const double *offsets, double start_phase,
+                               double phase_coef, double fft_duration, double gate_begin, double gate_end, int nbins,
+                               float2 *acc, hipStream_t s) {
+  if (n_segments <= 0) return hipSuccess;
   dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)((n_baselines + XB * XWAVES - 1) / (XB * XWAVES)));
   hipLaunchKernelGGL(k_xstep_pulsar, grid, dim3(64 * XWAVES), 0, s, spectra, stream_stride, seg_stride, n_ch,
-                     n_segments, bl_first, bl_second, n_baselines, offsets, start_phase, phase_coef,
+                     seg_base, n_segments, bl_first, bl_second, n_baselines, offsets, start_phase, phase_coef,
                      fft_duration, gate_begin, gate_end, nbins, acc);
   return hipGetLastError();
 }
@@ -1613,6 +1737,16 @@ static int mf_channels(int precision, int M, int maxp, size_t extra, size_t *lds
   return CH;
 }
 
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  if (bytes % 4 != 0) return hipErrorInvalidValue;
+  const int64_t n = (int64_t)(bytes / 4);
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill_zero_u32, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t *)p, n);
+  return hipGetLastError();
+}
+
 size_t pulsar_gram_floats(int M, int n_ch, int nbins) {
   const int RT = (2 * M + 31) / 32;
   return (size_t)nbins * n_ch * (RT * (RT + 1) / 2) * 1024;
@@ -1621,34 +1755,33 @@ size_t pulsar_gram_floats(int M, int n_ch, int nbins) {
 template <int PREC>
 static hipError_t launch_mfma_pulsar_t(dim3 grid, size_t lds, hipStream_t s, const float2 *spectra,
                                        int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT, int CH,
-                                       int64_t n_segments, int64_t segs_per_z, float scale, const double *offsets,
-                                       double start_phase, double phase_coef, double fft_duration,
-                                       double gate_begin, double gate_end, int nbins, float *gram) {
+                                       int64_t seg_base, int64_t n_segments, int64_t segs_per_z, float scale,
+                                       const double *offsets, double start_phase, double phase_coef,
+                                       double fft_duration, double gate_begin, double gate_end, int nbins,
+                                       float *gram) {
   hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma_pulsar<PREC>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_xstep_mfma_pulsar<PREC>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride, seg_stride,
-                     n_ch, M, RT, CH, n_segments, segs_per_z, scale, offsets, start_phase, phase_coef, fft_duration,
-                     gate_begin, gate_end, nbins, gram);
+                     n_ch, M, RT, CH, seg_base, n_segments, segs_per_z, scale, offsets, start_phase, phase_coef,
+                     fft_duration, gate_begin, gate_end, nbins, gram);
   return hipGetLastError();
 }
 
+hipError_t pulsar_gram_zero(int M, int n_ch_run, int nbins, float *gram, hipStream_t s) {
+  return launch_fill_zero(gram, pulsar_gram_floats(M, n_ch_run, nbins) * sizeof(float), s);
+}
+
 hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
-                                    int n_ch, int ch_base, int n_ch_run, int M, int64_t n_segments,
-                                    const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
-                                    const double *offsets, double start_phase, double phase_coef,
+                                    int ch_base, int n_ch_run, int M, int64_t seg_base, int64_t n_segments,
+                                    float scale, const double *offsets, double start_phase, double phase_coef,
                                     double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
-                                    float2 *vis, hipStream_t s) {
+                                    hipStream_t s) {
   if (n_segments <= 0 || n_ch_run <= 0) return hipSuccess;
   size_t lds = 0;
   const int CH = mf_channels(precision, M, MFP_MAXP, (size_t)64 * MF_KS * sizeof(int16_t), &lds);
   if (CH < 1) return hipErrorInvalidValue;
   const int RT = (2 * M + 31) / 32;
-  // zero the binned Gram tiles of this channel run
-  const int64_t n4 = (int64_t)(pulsar_gram_floats(M, n_ch_run, nbins) / 4);
-  hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, reinterpret_cast<float4 *>(gram), n4);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   // about 1024 workgroups: channel blocks x segment ranges (multiples of a stage)
   const int64_t nblk = (n_ch_run + CH - 1) / CH;
   int64_t nz = (1024 + nblk - 1) / nblk;
@@ -1660,23 +1793,26 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_
   const double *off = offsets + ch_base;
   switch (precision) {
     case SFXG_PREC_FP16:
-      e = launch_mfma_pulsar_t<SFXG_PREC_FP16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
-                                               n_segments, per, scale, off, start_phase, phase_coef, fft_duration,
-                                               gate_begin, gate_end, nbins, gram);
-      break;
+      return launch_mfma_pulsar_t<SFXG_PREC_FP16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
+                                                  seg_base, n_segments, per, scale, off, start_phase, phase_coef,
+                                                  fft_duration, gate_begin, gate_end, nbins, gram);
     case SFXG_PREC_BF16:
-      e = launch_mfma_pulsar_t<SFXG_PREC_BF16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
-                                               n_segments, per, scale, off, start_phase, phase_coef, fft_duration,
-                                               gate_begin, gate_end, nbins, gram);
-      break;
+      return launch_mfma_pulsar_t<SFXG_PREC_BF16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
+                                                  seg_base, n_segments, per, scale, off, start_phase, phase_coef,
+                                                  fft_duration, gate_begin, gate_end, nbins, gram);
     case SFXG_PREC_SPLIT_FP16:
-      e = launch_mfma_pulsar_t<SFXG_PREC_SPLIT_FP16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT,
-                                                     CH, n_segments, per, scale, off, start_phase, phase_coef,
-                                                     fft_duration, gate_begin, gate_end, nbins, gram);
-      break;
+      return launch_mfma_pulsar_t<SFXG_PREC_SPLIT_FP16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M,
+                                                        RT, CH, seg_base, n_segments, per, scale, off, start_phase,
+                                                        phase_coef, fft_duration, gate_begin, gate_end, nbins, gram);
     default: return hipErrorInvalidValue;
   }
-  if (e != hipSuccess) return e;
+}
+
+hipError_t pulsar_gram_baselines(const float *gram, int n_ch, int ch_base, int n_ch_run, int M, int nbins,
+                                 const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
+                                 float2 *vis, hipStream_t s) {
+  if (n_ch_run <= 0) return hipSuccess;
+  const int RT = (2 * M + 31) / 32;
   // vis rows keep the full channel stride; this run writes channels ch_base..
   hipLaunchKernelGGL(k_pulsar_gram_baselines_run, dim3((unsigned)n_ch_run, (unsigned)nbins), dim3(256), 0, s, gram,
                      n_ch_run, ch_base, n_ch, M, RT, bl_first, bl_second, n_baselines, 1.0f / (scale * scale), vis);

@@ -55,7 +55,7 @@ class PulsarC(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("lut", C.c_int32), ("precision", C.c_int32), ("keep_time_series", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("chunk_mib", C.c_int32)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -91,6 +91,19 @@ def lib() -> C.CDLL:
                                        C.POINTER(C.c_size_t)]
         L.sfxg_kernel_times.restype = C.c_int
         L.sfxg_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_int32]
+        L.sfxg_prepare.restype = C.c_int
+        L.sfxg_prepare.argtypes = [C.c_void_p]
+        L.sfxg_batch_create.restype = C.c_int
+        L.sfxg_batch_create.argtypes = [C.c_int, C.c_int32, C.POINTER(C.c_void_p)]
+        L.sfxg_batch_destroy.argtypes = [C.c_void_p]
+        L.sfxg_batch_last_error.restype = C.c_char_p
+        L.sfxg_batch_last_error.argtypes = [C.c_void_p]
+        for name in ("sfxg_batch_add", "sfxg_batch_clear", "sfxg_batch_launch", "sfxg_batch_synchronize"):
+            getattr(L, name).restype = C.c_int
+        L.sfxg_batch_add.argtypes = [C.c_void_p, C.c_void_p]
+        L.sfxg_batch_clear.argtypes = [C.c_void_p]
+        L.sfxg_batch_launch.argtypes = [C.c_void_p]
+        L.sfxg_batch_synchronize.argtypes = [C.c_void_p]
         L.sfxg_synthesize_raw.restype = C.c_int
         L.sfxg_synthesize_raw.argtypes = [C.c_int, C.c_void_p, C.c_size_t, C.c_int32, C.c_int64, C.c_int64,
                                           C.c_uint64, C.c_uint64, C.c_float, C.c_float]
@@ -130,8 +143,8 @@ class Context:
             msg = lib().sfxg_last_error(self._h)
             raise SfxgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def set_options(self, lut: int = 0, precision: int = 0, keep_time_series: bool = False):
-        o = Options(lut, precision, int(keep_time_series), 0)
+    def set_options(self, lut: int = 0, precision: int = 0, keep_time_series: bool = False, chunk_mib: int = 0):
+        o = Options(lut, precision, int(keep_time_series), chunk_mib)
         self._check(lib().sfxg_set_options(self._h, C.byref(o)), "sfxg_set_options")
 
     def set_parameters(self, job: CorrelationParameters, pulsar: Optional[Pulsar] = None):
@@ -188,6 +201,10 @@ class Context:
     def correlate(self):
         self._check(lib().sfxg_correlate(self._h), "sfxg_correlate")
 
+    def prepare(self):
+        """Host half of correlate; the device work runs in a Batch."""
+        self._check(lib().sfxg_prepare(self._h), "sfxg_prepare")
+
     def synchronize(self):
         self._check(lib().sfxg_synchronize(self._h), "sfxg_synchronize")
 
@@ -234,6 +251,48 @@ class Context:
         return out
 
 
+class Batch:
+    """Prepared contexts whose device work runs as one HIP graph launch."""
+
+    def __init__(self, device: int = 0, n_branches: int = 4):
+        self._h = C.c_void_p()
+        rc = lib().sfxg_batch_create(device, n_branches, C.byref(self._h))
+        if rc != SFXG_OK:
+            raise SfxgError(f"sfxg_batch_create failed ({rc})")
+        self.members: List[Context] = []
+
+    def _check(self, rc: int, what: str):
+        if rc != SFXG_OK:
+            msg = lib().sfxg_batch_last_error(self._h)
+            raise SfxgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def add(self, ctx: Context):
+        self._check(lib().sfxg_batch_add(self._h, ctx._h), "sfxg_batch_add")
+        self.members.append(ctx)
+
+    def clear(self):
+        self._check(lib().sfxg_batch_clear(self._h), "sfxg_batch_clear")
+        self.members = []
+
+    def launch(self):
+        self._check(lib().sfxg_batch_launch(self._h), "sfxg_batch_launch")
+
+    def synchronize(self):
+        self._check(lib().sfxg_batch_synchronize(self._h), "sfxg_batch_synchronize")
+
+    def close(self):
+        if self._h:
+            lib().sfxg_batch_destroy(self._h)
+            self._h = C.c_void_p()
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def synthesize_raw(device: int, ptr: int, n_bytes: int, bits: int, first_sample: int, delay_samples: int,
                    common_seed: int, stream_seed: int, rho: float, threshold: float = 0.9816):
     rc = lib().sfxg_synthesize_raw(device, C.c_void_p(ptr), n_bytes, bits, first_sample, delay_samples,
@@ -243,8 +302,9 @@ def synthesize_raw(device: int, ptr: int, n_bytes: int, bits: int, first_sample:
 
 
 def correlate_slice(ctx: Context, job: CorrelationParameters, headers: Sequence[bytes], lut: int = 0,
-                    pulsar: Optional[Pulsar] = None, keep_time_series: bool = False, precision: int = 0) -> bytes:
-    ctx.set_options(lut=lut, precision=precision, keep_time_series=keep_time_series)
+                    pulsar: Optional[Pulsar] = None, keep_time_series: bool = False, precision: int = 0,
+                    chunk_mib: int = 0) -> bytes:
+    ctx.set_options(lut=lut, precision=precision, keep_time_series=keep_time_series, chunk_mib=chunk_mib)
     ctx.set_parameters(job, pulsar)
     for i, h in enumerate(headers):
         ctx.push_stream(i, h)

@@ -135,6 +135,53 @@ def config2_jobs(subband: int, n_integrations: int = 8, t_int_usec: int = 500_00
                        sky=6_000_000_000 + 64_000_000 * subband)
 
 
+def config3_jobs(n_jobs: int = 48, seconds: float = 2.0, seed: int = 3):
+    """Configuration 3 (a mixed batch, no sample data): n_jobs seeded jobs of
+    one subband each, drawn per job: channels from {256, 1024, 4096, 16384,
+    32768}, stations uniform in 4..24 (one polarisation), 1- or 2-bit,
+    bandwidth from {8, 16, 32, 64} MHz (real sampling at twice that), integration
+    time from {0.25, 0.5, 1, 2} s, rho uniform in [0.005, 0.05], delay offsets
+    U[-5, 5] ms and rates U[-3e-6, 3e-6]; `seconds` of data per job. Returns
+    a list of dicts: jobs (one per integration), raw_start, rho and the draws."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for j in range(n_jobs):
+        nchan = int(rng.choice([256, 1024, 4096, 16384, 32768]))
+        n_st = int(rng.integers(4, 25))
+        bits = int(rng.choice([1, 2]))
+        bw = int(rng.choice([8, 16, 32, 64])) * 1_000_000
+        t_int = float(rng.choice([0.25, 0.5, 1.0, 2.0]))
+        rho = float(rng.uniform(0.005, 0.05))
+        offsets = rng.uniform(-5e-3, 5e-3, n_st)
+        rates = rng.uniform(-3e-6, 3e-6, n_st)
+        fs = 2 * bw
+        sky = 5_000_000_000 + 64_000_000 * j
+        start = mjd_to_ticks(START_MJD, 0.0)
+        tps = int(round(CLOCK_RATE / fs))
+        margin = int(math.ceil(5.1e-3 * fs)) + 4096
+        margin -= margin % 16
+        raw_start = start - margin * tps
+        n_int = max(1, int(round(seconds / t_int)))
+        secs = int(math.ceil(n_int * t_int + margin / fs + 2))
+        models = [DelayModel.linear(raw_start, o + r * (raw_start - start) / CLOCK_RATE, r, secs)
+                  for o, r in zip(offsets, rates)]
+        streams = [StationStream(station_number=s, station_stream=s, sample_rate=fs, channel_freq=sky, bandwidth=bw,
+                                 sideband="U", polarisation="R", bits_per_sample=bits) for s in range(n_st)]
+        t_int_ticks = int(round(t_int * CLOCK_RATE))
+        jobs = []
+        for k in range(n_int):
+            t0 = start + k * t_int_ticks
+            jobs.append(CorrelationParameters(
+                experiment_start=start, start_time=t0, stop_time=t0 + t_int_ticks, integration_time=t_int_ticks,
+                sub_integration_time=int(round(min(t_int, 0.125) * CLOCK_RATE)), number_channels=nchan,
+                fft_size_delaycor=256, fft_size_correlation=nchan, sample_rate=fs, channel_freq=sky, bandwidth=bw,
+                sideband="U", frequency_nr=j, station_streams=streams, integration_nr=k, slice_nr=k,
+                cross_polarize=False, delay_models=models))
+        out.append(dict(jobs=jobs, raw_start=raw_start, rho=rho, nchan=nchan, stations=n_st, bits=bits, fs=fs,
+                        t_int=t_int))
+    return out
+
+
 def config4_jobs(subband: int, n_integrations: int = 20, t_int_usec: int = 1_000_000, nchan: int = 1024,
                  n_stations: int = 12):
     """Job descriptions of configuration 4 (pulsar binning, no sample data):

@@ -136,7 +136,7 @@ def run_both(ctx, job, hdrs, lut=LUT_CONFIG, pulsar=None):
     return vis_g, vis_r, ref
 
 
-@pytest.mark.parametrize("nchan", [256, 2048, 8192, 16384])
+@pytest.mark.parametrize("nchan", [256, 2048, 8192, 16384, 32768])
 def test_channel_counts(ctx, nchan):
     """Power-of-four and odd-log2 FFT sizes (the radix-2 stage), up to the
     largest LDS-resident transform."""
@@ -390,3 +390,80 @@ def test_mixed_pulsar_vs_fp64(ctx, prec, nbins, gate):
         l2, worst = max(l2, e[0]), max(worst, e[1])
     print(f"pulsar precision {prec}, {nbins} bins: vs FP64 oracle, relative L2 {l2:.3e}, worst channel {worst:.3e}")
     assert l2 <= MIXED_BARS[prec][0] and worst <= MIXED_BARS[prec][1]
+
+
+def _multi_fold_job(nchan, seed=20, bits=2, window=4):
+    """0.25 s slice with 31.25 ms sub-integrations: eight folds."""
+    return configs.build_slices(seed=seed, n_stations=3, pols=["R"], fs=16_000_000, bw=8_000_000, nchan=nchan,
+                                seconds=0.25, t_int_usec=250_000, sub_int_usec=31_250, rho=0.05, bits=bits,
+                                sky_freq=5_000_000_000, offsets=[0.0, 2e-6, -1e-6], rates=[0.0, 1e-7, -2e-7],
+                                window=window)[0]
+
+
+@pytest.mark.parametrize("nchan,prec", [(1024, 0), (16384, 0), (1024, 1)])
+def test_chunked_matches_unchunked(ctx, nchan, prec):
+    """A 1 MiB spectra budget processes the slice one sub-integration per
+    chunk (eight chunks); the output bytes do not change."""
+    job, hdrs = _multi_fold_job(nchan)
+    a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=prec)
+    b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=prec, chunk_mib=1)
+    assert a == b
+    if prec == 0:
+        ref = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+        heads_g, vis_g = vis_from_bytes(job, b)
+        heads_r, vis_r = vis_from_bytes(job, ref["output"])
+        assert heads_g == heads_r
+        compare_vis(vis_g[0], vis_r[0], label=f"chunked nchan={nchan}")
+
+
+def test_chunked_pulsar_matches_unchunked(ctx):
+    """FP32 pulsar binning carries every bin's running sums across chunks."""
+    from sfxg.job import Pulsar, ticks_to_mjd
+    job, hdrs = _multi_fold_job(1024, seed=21)
+    psr = Pulsar(nbins=8, gate_start=0.1, gate_stop=0.6, tmid=ticks_to_mjd(job.start_time) - 0.001, DM=68.0,
+                 ref_phase=0.3, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+    a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, pulsar=psr)
+    b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, pulsar=psr, chunk_mib=1)
+    assert a == b
+
+
+def test_batch_graph_replays_match_eager():
+    """Slices of different shapes (channel counts, bit depths, pulsar,
+    mixed precision) as one captured graph: three replays, with eager work and
+    synchronisation between them, each give the eager bytes."""
+    from sfxg.job import Pulsar, ticks_to_mjd
+    cases = []
+    cases.append((_multi_fold_job(1024, seed=30), None, 0))
+    cases.append((_multi_fold_job(16384, seed=31), None, 0))
+    cases.append((_multi_fold_job(512, seed=32, bits=1), None, 0))
+    cases.append((_multi_fold_job(2048, seed=33), None, 1))
+    j4 = _multi_fold_job(1024, seed=34)
+    psr = Pulsar(nbins=4, gate_start=0.2, gate_stop=0.7, tmid=ticks_to_mjd(j4[0].start_time) - 0.001, DM=30.0,
+                 ref_phase=0.1, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+    cases.append((j4, psr, 0))
+    eager_ctx = capi.Context(0)
+    eager = [capi.correlate_slice(eager_ctx, job, hdrs, lut=LUT_CONFIG, pulsar=p, precision=prec)
+             for (job, hdrs), p, prec in cases]
+    batch = capi.Batch(0, n_branches=3)
+    ctxs = []
+    for (job, hdrs), p, prec in cases:
+        c = capi.Context(0)
+        c.set_options(lut=LUT_CONFIG, precision=prec)
+        c.set_parameters(job, p)
+        for i, h in enumerate(hdrs):
+            c.push_stream(i, h)
+        c.prepare()
+        batch.add(c)
+        ctxs.append(c)
+    for rep in range(3):
+        batch.launch()
+        batch.synchronize()
+        for k, c in enumerate(ctxs):
+            assert c.output() == eager[k], f"replay {rep}, slice {k}"
+        # eager work on another context between replays
+        (job, hdrs), p, prec = cases[0]
+        assert capi.correlate_slice(eager_ctx, job, hdrs, lut=LUT_CONFIG) == eager[0]
+    batch.close()
+    for c in ctxs:
+        c.close()
+    eager_ctx.close()
