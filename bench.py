@@ -46,9 +46,11 @@ def parse():
     ap.add_argument("--main-only", action="store_true", help="skip the mixed-precision and 32-station runs")
     ap.add_argument("--c2-steps", type=int, default=2, help="0.5 s integrations of the 32-station workload")
     ap.add_argument("--c2-precisions", default="fp32,split_fp16,fp16")
-    ap.add_argument("--only", choices=("all", "c1", "c2", "c4"), default="all",
+    ap.add_argument("--only", choices=("all", "c1", "c2", "c3", "c4"), default="all",
                     help="c1: the main 8-station workload only; c2: the 32-station workload only; "
-                         "c4: the pulsar-binning workload only")
+                         "c3: the mixed batch only; c4: the pulsar-binning workload only")
+    ap.add_argument("--c3-jobs", type=int, default=48, help="jobs of the mixed batch")
+    ap.add_argument("--c3-precisions", default="fp32,fp16")
     ap.add_argument("--c4-steps", type=int, default=1, help="1 s integrations of the pulsar workload")
     ap.add_argument("--c4-precisions", default="fp32,fp16,split_fp16")
     ap.add_argument("--cpu-cores", type=int, default=0, help="CPU baseline worker processes (default: up to 16)")
@@ -244,6 +246,78 @@ class Workload:
         return wall, kt
 
 
+class MixedBatch:
+    """The mixed batch: each job's 2-bit/1-bit payload for all its slices in
+    device memory, one context per job and round parity (two sets, so one
+    round's host preparation overlaps the previous round's graph), and per
+    round one captured graph of every job's next slice."""
+
+    def __init__(self, torch, capi, synth, configs, dev, n_jobs):
+        t0 = time.perf_counter()
+        self.jobs = configs.config3_jobs(n_jobs=n_jobs)
+        self.inputs = []
+        self.station_samples = 0
+        for j, d in enumerate(self.jobs):
+            per = []
+            for k, job in enumerate(d["jobs"]):
+                nsamp = job.n_ffts_per_integration(0) * job.fft_size_delaycor
+                lst = []
+                for i, st in enumerate(job.station_streams):
+                    ctrl, off, n = synth.header_stream(1 << 40, d["raw_start"], job.delay_models[i], job.start_time,
+                                                       nsamp, st.sample_rate, st.bits_per_sample, elided=True)
+                    buf = torch.empty(n, dtype=torch.uint8, device=f"cuda:{dev}")
+                    n_int = int(synth.integer_delay(job.delay_models[i], job.start_time, st.sample_rate))
+                    capi.synthesize_raw(dev, buf.data_ptr(), n, st.bits_per_sample, off * (8 // st.bits_per_sample),
+                                        n_int, common_seed=5000 + j, stream_seed=(5000 + j) * 4096 + i * 64 + k,
+                                        rho=d["rho"])
+                    lst.append((ctrl, buf, n))
+                    self.station_samples += nsamp
+                per.append(lst)
+            self.inputs.append(per)
+        torch.cuda.synchronize()
+        self.setup_s = time.perf_counter() - t0
+        self.rounds = max(len(d["jobs"]) for d in self.jobs)
+
+    def run(self, torch, capi, precision, dev, chunk_mib=512):
+        sets = [[capi.Context(dev) for _ in self.jobs] for _ in range(2)]
+        for cs in sets:
+            for c in cs:
+                c.set_options(lut=1, precision=precision, chunk_mib=chunk_mib)
+        batches = [capi.Batch(dev, n_branches=4) for _ in range(2)]
+
+        def prepare(r):
+            cs, b = sets[r % 2], batches[r % 2]
+            b.synchronize()
+            b.clear()
+            for j, d in enumerate(self.jobs):
+                if r < len(d["jobs"]):
+                    c = cs[j]
+                    c.set_parameters(d["jobs"][r])
+                    for i, (ctrl, buf, n) in enumerate(self.inputs[j][r]):
+                        c.push_stream_elided(i, ctrl, buf.data_ptr(), n)
+                    c.prepare()
+                    b.add(c)
+            return b
+
+        # warm-up round (captures, allocations), then the timed pass
+        prepare(0).launch()
+        batches[0].synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(self.rounds):
+            prepare(r).launch()
+        for b in batches:
+            b.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        for b in batches:
+            b.close()
+        for cs in sets:
+            for c in cs:
+                c.close()
+        return wall
+
+
 def summarise(wl, wall, kt):
     job = wl.job
     nseg = job.number_ffts_in_integration()
@@ -357,6 +431,26 @@ def main():
                     c2[key] = round(c2[name]["value"] / c2["fp32"]["value"], 3)
         line["config2"] = c2
         del w2
+        torch.cuda.empty_cache()
+    if args.only == "c3" or (args.only == "all" and not args.main_only):
+        mb = MixedBatch(torch, capi, synth, configs, dev, args.c3_jobs)
+        print(f"mixed batch setup done in {mb.setup_s:.1f} s", file=sys.stderr, flush=True)
+        from collections import Counter
+        c3 = {"workload": f"{args.c3_jobs} seeded jobs, 2 s each: channels from {{256, 1024, 4096, 16384, 32768}}, "
+                          "4..24 stations, 1- or 2-bit, 8/16/32/64 MHz, 0.25/0.5/1/2 s integrations; per round "
+                          "every job's next slice as one captured HIP graph",
+              "station_samples": mb.station_samples, "rounds": mb.rounds,
+              "slices": sum(len(d["jobs"]) for d in mb.jobs),
+              "channels": dict(Counter(str(d["nchan"]) for d in mb.jobs)),
+              "bits": dict(Counter(str(d["bits"]) for d in mb.jobs))}
+        for name in args.c3_precisions.split(","):
+            wall = mb.run(torch, capi, PREC[name], dev)
+            c3[name] = {"value": round(mb.station_samples / wall / 1e9, 4), "wall_s": round(wall, 3)}
+            print(f"mixed batch {name} done: {c3[name]['value']:.2f}", file=sys.stderr, flush=True)
+        if "fp32" in c3 and "fp16" in c3:
+            c3["fp16_vs_plain"] = round(c3["fp16"]["value"] / c3["fp32"]["value"], 3)
+        line["config3_batch"] = c3
+        del mb
         torch.cuda.empty_cache()
     if args.only == "c4" or (args.only == "all" and not args.main_only):
         w4 = Workload(torch, capi, synth, lambda sb, k: configs.config4_jobs(sb, n_integrations=k)[:2],
