@@ -258,6 +258,28 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
 #define SFXG_N_KERNELS 5
 int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n, int32_t reset);
 
+/* Autotuner of launch shapes: on a prepared context (sfxg_prepare), times the
+ * candidate FFT schedules of the F-engine (threads, i.e. points per thread and
+ * radix stages, for N_c <= 4096; workgroup count of the large-N channeliser)
+ * and the X-step tile shapes (waves per workgroup of the FP32 X-step, channels
+ * per workgroup of the MFMA X-step, workgroups of the MFMA pulsar X-step) on
+ * the slice's own data, keeps the fastest of each in a process-wide table
+ * keyed by the slice shape (channels, streams, precision, pulsar, fused), and
+ * uses the table for every later slice of that shape. reps: timed runs per
+ * candidate. report (may be NULL): text of the candidates' times. The trial
+ * runs overwrite the slice's buffers: prepare or correlate it again after.
+ * The reference's counterpart is the FFTW planner (src/sfxc_fft_float.cc:
+ * 161,181-183, plans made with FFTW_ESTIMATE). Results change only by the
+ * rounding of the FFT schedule. */
+int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap);
+/* The tuning table as text, one "shape|fe_threads|chan_wgs|xs_waves|mf_channels|
+ * mfp_wgs" line per shape (0: default); *written = bytes needed including the
+ * terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear empties
+ * the table. */
+int sfxg_tune_export(char *dst, size_t cap, size_t *written);
+int sfxg_tune_import(const char *text);
+void sfxg_tune_clear(void);
+
 /* Synthetic 2-bit/1-bit raw station data written straight into device
  * memory (for throughput measurements on device-resident input): n_bytes
  * bytes of stream `stream_seed` starting at raw sample `first_sample`,

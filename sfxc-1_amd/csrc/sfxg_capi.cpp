@@ -7,7 +7,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <string>
 #include <vector>
@@ -61,6 +63,10 @@ int nr_ffts_per_integration_slice(int64_t integration_ticks, int data_rate, int 
 }
 
 }  // namespace
+
+// Process-wide table of tuned launch shapes, by slice shape.
+std::mutex g_tune_mu;
+std::map<std::string, Tune> g_tune;
 
 // A run of whole sub-integrations processed together: segments [seg0, seg1)
 // (folds fold0 .. fold1 - 1); the real series of blocks [blk0, blk1).
@@ -125,6 +131,7 @@ struct sfxg_ctx {
   bool enqueued = false;
   bool have_results = false;  // the current slice ran (correlate or a batch launch)
   LaunchPlan lp;
+  Tune tune;              // launch shapes of the current slice (from the tuning table)
   bool prepared = false;  // the device work of the current slice can be enqueued
   int batch_member = 0;   // number of batches holding this context
   std::vector<struct sfxg_batch *> batches;  // those batches
@@ -453,6 +460,14 @@ static void make_tables(sfxg_ctx *ctx, std::vector<float2> &tnd, std::vector<flo
 
 }  // extern "C"
 
+// Shape of a slice as the tuning table sees it.
+static std::string tune_key(const sfxg_ctx *ctx, bool fused) {
+  char buf[160];
+  snprintf(buf, sizeof(buf), "nc=%d M=%d prec=%d pulsar=%d fused=%d", ctx->nc, ctx->M, ctx->opt.precision,
+           ctx->pulsar ? 1 : 0, fused ? 1 : 0);
+  return buf;
+}
+
 // Host part of sfxg_correlate: chunk and invalid-data bookkeeping, device
 // buffers, tables and the slice's uploads (asynchronous, on the context's
 // stream), and the launch parameters of the device work (ctx->lp).
@@ -606,6 +621,13 @@ static int prepare_slice(sfxg_ctx *ctx) {
       return ctx->fail(SFXG_ERR_ARG, "pulsar binning with mixed precision: the binned Gram tiles of %d channels "
                                      "exceed 8 GiB for a slice longer than one chunk", nc + 1);
     HIPCHK(ctx, ctx->gram.ensure(per_ch * psr_run));
+  }
+
+  {
+    const std::string key = tune_key(ctx, fused);
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    ctx->tune = it != g_tune.end() ? it->second : Tune();
   }
 
   // host staging stays alive in the context until the next slice, so the
@@ -775,13 +797,82 @@ static int prepare_slice(sfxg_ctx *ctx) {
   return SFXG_OK;
 }
 
+// F-engine of chunk c (block constants first, with the first chunk);
+// timing: the delay/channeliser split event of the chunk.
+static int enqueue_fengine(sfxg_ctx *ctx, size_t c, hipStream_t s, bool timing, const Tune &t) {
+  const LaunchPlan &lp = ctx->lp;
+  const Chunk &ch = lp.chunks[c];
+  DevJob cj = lp.job;
+  cj.seg_base = ch.seg0;
+  cj.n_segments = ch.seg1 - ch.seg0;
+  cj.blk_base = ch.blk0;
+  cj.blk_end = ch.blk1;
+  const DevStream *dstr = ctx->dstreams.as<DevStream>();
+  if (c == 0) HIPCHK(ctx, launch_block_consts(dstr, cj, s));
+  if (lp.fused) {
+    HIPCHK(ctx, launch_fengine(dstr, cj, ctx->window.as<float>(), ctx->tw_nd.as<float2>(), ctx->tw_n.as<float2>(),
+                               ctx->tw_2n.as<float2>(), t, s));
+    HIPCHK(ctx, timing ? hipEventRecord(ctx->cev[4 * c + 1], s) : hipSuccess);
+  } else {
+    HIPCHK(ctx, launch_delay(dstr, cj, ctx->tw_nd.as<float2>(), s));
+    HIPCHK(ctx, timing ? hipEventRecord(ctx->cev[4 * c + 1], s) : hipSuccess);
+    HIPCHK(ctx, launch_channelise(dstr, cj, ctx->window.as<float>(), ctx->tw_n.as<float2>(), ctx->tw_2n.as<float2>(),
+                                  ctx->tw_split.as<float2>(), ctx->chan_scratch.as<float2>(), t, s));
+  }
+  return SFXG_OK;
+}
+
+// X-step of chunk c: the chunk's sub-integrations into vis (in fold order),
+// or the pulsar bins of its segments.
+static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) {
+  const LaunchPlan &lp = ctx->lp;
+  const Chunk &ch = lp.chunks[c];
+  const int M = ctx->M, nc = ctx->nc, B = ctx->B;
+  const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
+  const int64_t sstride = lp.chunk_segs * ctx->S;  // spectra elements per stream
+  const sfxg_pulsar &ps = ctx->psr;
+  if (lp.mixed && !ctx->pulsar) {
+    for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
+      const int ng = std::min(lp.k_group, ch.fold1 - g0);
+      HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1,
+                                    M, ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, lp.mixed_scale,
+                                    ctx->partial.as<float2>(), ctx->vis.as<float2>(), t, s));
+    }
+  } else if (!ctx->pulsar) {
+    for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
+      const int ng = std::min(lp.k_group, ch.fold1 - g0);
+      HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
+                               ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),
+                               ctx->vis.as<float2>(), t, s));
+    }
+  } else if (lp.mixed) {
+    // with several chunks the channel run covers all channels (checked in prepare)
+    for (int c0 = 0; c0 < nc + 1; c0 += lp.psr_run) {
+      const int n_run = std::min(lp.psr_run, nc + 1 - c0);
+      if (c == 0 || lp.psr_run < nc + 1) HIPCHK(ctx, pulsar_gram_zero(M, n_run, ctx->nbins, ctx->gram.as<float>(), s));
+      HIPCHK(ctx, launch_xstep_mfma_pulsar(ctx->opt.precision, ctx->spectra.as<float2>(), sstride, ctx->S, c0, n_run,
+                                           M, ch.seg0, ch.seg1 - ch.seg0, lp.mixed_scale, ctx->offsets.as<double>(),
+                                           lp.start_phase, lp.phase_coef, lp.fft_duration, ps.gate_start,
+                                           ps.gate_stop, ctx->nbins, ctx->gram.as<float>(), t, s));
+      if (c + 1 == lp.chunks.size() || lp.psr_run < nc + 1)
+        HIPCHK(ctx, pulsar_gram_baselines(ctx->gram.as<float>(), nc + 1, c0, n_run, M, ctx->nbins, blf, bls, B,
+                                          lp.mixed_scale, ctx->vis.as<float2>(), s));
+    }
+  } else {
+    HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), sstride, ctx->S, nc + 1, ch.seg0, ch.seg1 - ch.seg0,
+                                    blf, bls, B, ctx->offsets.as<double>(), lp.start_phase, lp.phase_coef,
+                                    lp.fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins, ctx->vis.as<float2>(),
+                                    s));
+  }
+  return SFXG_OK;
+}
+
 // The device work of a prepared slice on stream s: no allocation, no host
 // synchronisation and no memset nodes, so that it can be captured into a
 // graph. timing: bracket the stages with the context's events.
 static int enqueue_slice(sfxg_ctx *ctx, hipStream_t s, bool timing) {
   const LaunchPlan &lp = ctx->lp;
   const int M = ctx->M, nc = ctx->nc, B = ctx->B;
-  const DevJob &job = lp.job;
   auto mark = [&](int k) { return timing ? hipEventRecord(ctx->ev[k], s) : hipSuccess; };
   HIPCHK(ctx, launch_fill_zero(ctx->hist.p, (size_t)M * 512 * sizeof(int32_t), s));
   HIPCHK(ctx, launch_fill_zero(ctx->vis.p, lp.vis_elems * sizeof(float2), s));
@@ -790,67 +881,16 @@ static int enqueue_slice(sfxg_ctx *ctx, hipStream_t s, bool timing) {
   HIPCHK(ctx, launch_stats(dstr, M, lp.max_counted, s));
   HIPCHK(ctx, mark(SFXG_KERNEL_DELAY));
   ctx->fused = lp.fused;
-  const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
-  const int64_t sstride = lp.chunk_segs * ctx->S;  // spectra elements per stream
-  auto cmark = [&](size_t c, int k) { return timing ? hipEventRecord(ctx->cev[4 * c + k], s) : hipSuccess; };
-  const sfxg_pulsar &ps = ctx->psr;
   for (size_t c = 0; c < lp.chunks.size(); c++) {
-    const Chunk &ch = lp.chunks[c];
-    DevJob cj = job;
-    cj.seg_base = ch.seg0;
-    cj.n_segments = ch.seg1 - ch.seg0;
-    cj.blk_base = ch.blk0;
-    cj.blk_end = ch.blk1;
-    HIPCHK(ctx, cmark(c, 0));
-    if (c == 0) HIPCHK(ctx, launch_block_consts(dstr, cj, s));
-    if (lp.fused) {
-      HIPCHK(ctx, launch_fengine(dstr, cj, ctx->window.as<float>(), ctx->tw_nd.as<float2>(), ctx->tw_n.as<float2>(),
-                                 ctx->tw_2n.as<float2>(), s));
-      HIPCHK(ctx, cmark(c, 1));
-    } else {
-      HIPCHK(ctx, launch_delay(dstr, cj, ctx->tw_nd.as<float2>(), s));
-      HIPCHK(ctx, cmark(c, 1));
-      HIPCHK(ctx, launch_channelise(dstr, cj, ctx->window.as<float>(), ctx->tw_n.as<float2>(),
-                                    ctx->tw_2n.as<float2>(), ctx->tw_split.as<float2>(),
-                                    ctx->chan_scratch.as<float2>(), s));
-    }
-    HIPCHK(ctx, cmark(c, 2));
-    if (lp.mixed && !ctx->pulsar) {
-      for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
-        const int ng = std::min(lp.k_group, ch.fold1 - g0);
-        HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0,
-                                      nc + 1, M, ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, lp.mixed_scale,
-                                      ctx->partial.as<float2>(), ctx->vis.as<float2>(), s));
-      }
-    } else if (!ctx->pulsar) {
-      for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
-        const int ng = std::min(lp.k_group, ch.fold1 - g0);
-        HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
-                                 ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),
-                                 ctx->vis.as<float2>(), s));
-      }
-    } else if (lp.mixed) {
-      // one channel run per chunk loop when there are several chunks (checked in prepare)
-      for (int c0 = 0; c0 < nc + 1; c0 += lp.psr_run) {
-        const int n_run = std::min(lp.psr_run, nc + 1 - c0);
-        if (c == 0 || lp.psr_run < nc + 1) HIPCHK(ctx, pulsar_gram_zero(M, n_run, ctx->nbins, ctx->gram.as<float>(), s));
-        HIPCHK(ctx, launch_xstep_mfma_pulsar(ctx->opt.precision, ctx->spectra.as<float2>(), sstride, ctx->S, c0,
-                                             n_run, M, ch.seg0, ch.seg1 - ch.seg0, lp.mixed_scale,
-                                             ctx->offsets.as<double>(), lp.start_phase, lp.phase_coef,
-                                             lp.fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins,
-                                             ctx->gram.as<float>(), s));
-        if (c + 1 == lp.chunks.size() || lp.psr_run < nc + 1)
-          HIPCHK(ctx, pulsar_gram_baselines(ctx->gram.as<float>(), nc + 1, c0, n_run, M, ctx->nbins, blf, bls, B,
-                                            lp.mixed_scale, ctx->vis.as<float2>(), s));
-      }
-    } else {
-      HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), sstride, ctx->S, nc + 1, ch.seg0, ch.seg1 - ch.seg0,
-                                      blf, bls, B, ctx->offsets.as<double>(), lp.start_phase, lp.phase_coef,
-                                      lp.fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins,
-                                      ctx->vis.as<float2>(), s));
-    }
-    HIPCHK(ctx, cmark(c, 3));
+    HIPCHK(ctx, timing ? hipEventRecord(ctx->cev[4 * c], s) : hipSuccess);
+    const int rc = enqueue_fengine(ctx, c, s, timing, ctx->tune);
+    if (rc != SFXG_OK) return rc;
+    HIPCHK(ctx, timing ? hipEventRecord(ctx->cev[4 * c + 2], s) : hipSuccess);
+    const int rc2 = enqueue_xstep(ctx, c, s, ctx->tune);
+    if (rc2 != SFXG_OK) return rc2;
+    HIPCHK(ctx, timing ? hipEventRecord(ctx->cev[4 * c + 3], s) : hipSuccess);
   }
+  const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
   HIPCHK(ctx, mark(SFXG_KERNEL_NORMALISE));
   HIPCHK(ctx, launch_auto_norms(ctx->vis.as<float2>(), M, nc + 1, nc, ctx->nbins, B, ctx->norms.as<double>(), s));
   HIPCHK(ctx, launch_normalise(ctx->vis.as<float2>(), ctx->norms.as<double>(), blf, bls, ctx->flags.as<double>(), M,
@@ -1293,6 +1333,166 @@ int sfxg_batch_synchronize(sfxg_batch *b) {
   const hipError_t e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) return b->fail(SFXG_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
   return SFXG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuning of launch shapes
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+// Minimum over reps runs (after one untimed run) of the GPU time of the work
+// f enqueues on the context's stream.
+template <class F>
+static int time_stage(sfxg_ctx *ctx, int reps, F f, float *best_ms) {
+  hipStream_t s = ctx->stream;
+  int rc = f();
+  if (rc != SFXG_OK) return rc;
+  float best = -1.0f;
+  for (int r = 0; r < reps; r++) {
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    rc = f();
+    if (rc != SFXG_OK) return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+    float ms = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    if (best < 0 || ms < best) best = ms;
+  }
+  *best_ms = best;
+  return SFXG_OK;
+}
+
+extern "C" {
+
+int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
+  if (!ctx || reps < 1) return SFXG_ERR_ARG;
+  if (!ctx->prepared) return ctx->fail(SFXG_ERR_STATE, "sfxg_prepare first");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  sync_slice(ctx);
+  const LaunchPlan &lp = ctx->lp;
+  const int nc = ctx->nc;
+  std::string rep;
+  char line[160];
+  Tune best = ctx->tune;
+  // F-engine: threads per workgroup (N_c <= 4096) or the large-N channeliser's workgroups
+  const bool small = nc <= 4096;
+  std::vector<int> fc = small ? fe_thread_candidates(nc) : chan_wgs_candidates(nc);
+  if (fc.size() > 1) {
+    float best_ms = -1.0f;
+    Tune pick = best;
+    for (int v : fc) {
+      Tune t = best;
+      (small ? t.fe_threads : t.chan_wgs) = v;
+      float ms = 0.0f;
+      const int rc = time_stage(ctx, reps, [&] { return enqueue_fengine(ctx, 0, ctx->stream, false, t); }, &ms);
+      if (rc != SFXG_OK) return rc;
+      snprintf(line, sizeof(line), "%s=%d: %.3f ms\n", small ? "fe_threads" : "chan_wgs", v, ms);
+      rep += line;
+      if (best_ms < 0 || ms < best_ms) {
+        best_ms = ms;
+        pick = t;
+      }
+    }
+    best = pick;
+  }
+  // spectra of the chosen F-engine for the X-step variants
+  {
+    const int rc = enqueue_fengine(ctx, 0, ctx->stream, false, best);
+    if (rc != SFXG_OK) return rc;
+  }
+  const bool mixed = ctx->opt.precision != SFXG_PREC_FP32;
+  std::vector<int> xc;
+  const char *xname = "";
+  if (mixed && !ctx->pulsar) {
+    xc = mf_channel_candidates(ctx->opt.precision, ctx->M);
+    xname = "mf_channels";
+  } else if (!ctx->pulsar) {
+    xc = xs_waves_candidates();
+    xname = "xs_waves";
+  } else if (mixed) {
+    xc = mfp_wgs_candidates();
+    xname = "mfp_wgs";
+  }
+  if (xc.size() > 1) {
+    float best_ms = -1.0f;
+    Tune pick = best;
+    for (int v : xc) {
+      Tune t = best;
+      if (mixed && !ctx->pulsar) t.mf_channels = v;
+      else if (!ctx->pulsar) t.xs_waves = v;
+      else t.mfp_wgs = v;
+      float ms = 0.0f;
+      const int rc = time_stage(ctx, reps, [&] { return enqueue_xstep(ctx, 0, ctx->stream, t); }, &ms);
+      if (rc != SFXG_OK) return rc;
+      snprintf(line, sizeof(line), "%s=%d: %.3f ms\n", xname, v, ms);
+      rep += line;
+      if (best_ms < 0 || ms < best_ms) {
+        best_ms = ms;
+        pick = t;
+      }
+    }
+    best = pick;
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[tune_key(ctx, lp.fused)] = best;
+  }
+  ctx->tune = best;
+  // the trial runs overwrote the slice's buffers
+  ctx->prepared = false;
+  ctx->have_results = false;
+  if (report && cap) {
+    const size_t n = std::min(cap - 1, rep.size());
+    memcpy(report, rep.data(), n);
+    report[n] = 0;
+  }
+  return SFXG_OK;
+}
+
+int sfxg_tune_export(char *dst, size_t cap, size_t *written) {
+  std::string out;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    char line[256];
+    for (const auto &kv : g_tune) {
+      const Tune &t = kv.second;
+      snprintf(line, sizeof(line), "%s|%d|%d|%d|%d|%d\n", kv.first.c_str(), t.fe_threads, t.chan_wgs, t.xs_waves,
+               t.mf_channels, t.mfp_wgs);
+      out += line;
+    }
+  }
+  if (written) *written = out.size() + 1;
+  if (!dst || cap < out.size() + 1) return SFXG_ERR_ARG;
+  memcpy(dst, out.c_str(), out.size() + 1);
+  return SFXG_OK;
+}
+
+int sfxg_tune_import(const char *text) {
+  if (!text) return SFXG_ERR_ARG;
+  std::map<std::string, Tune> add;
+  const char *p = text;
+  while (*p) {
+    const char *e = strchr(p, '\n');
+    std::string ln = e ? std::string(p, e) : std::string(p);
+    p = e ? e + 1 : p + ln.size();
+    if (ln.empty()) continue;
+    const size_t bar = ln.find('|');
+    if (bar == std::string::npos) return SFXG_ERR_ARG;
+    Tune t;
+    if (sscanf(ln.c_str() + bar + 1, "%d|%d|%d|%d|%d", &t.fe_threads, &t.chan_wgs, &t.xs_waves, &t.mf_channels,
+               &t.mfp_wgs) != 5)
+      return SFXG_ERR_ARG;
+    add[ln.substr(0, bar)] = t;
+  }
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  for (const auto &kv : add) g_tune[kv.first] = kv.second;
+  return SFXG_OK;
+}
+
+void sfxg_tune_clear(void) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune.clear();
 }
 
 }  // extern "C"

@@ -159,6 +159,22 @@ struct DevJob {
   int64_t blk_end;          // end of the chunk's blocks (k_delay)
 };
 
+// Launch shapes the autotuner chooses between; 0 selects the default.
+struct Tune {
+  int fe_threads = 0;   // F-engine threads per workgroup for N_c <= 4096 (fused and
+                        // unfused channeliser alike: N_c / threads points per thread)
+  int chan_wgs = 0;     // large-N channeliser (N_c 8192, 16384): workgroups in all
+  int xs_waves = 0;     // plain X-step: waves per workgroup
+  int mf_channels = 0;  // MFMA X-step: at most this many channels per workgroup
+  int mfp_wgs = 0;      // MFMA pulsar X-step: workgroups in all
+};
+// Candidates of each knob for a shape (the default first).
+std::vector<int> fe_thread_candidates(int nc);
+std::vector<int> chan_wgs_candidates(int nc);
+std::vector<int> xs_waves_candidates();
+std::vector<int> mf_channel_candidates(int precision, int M);
+std::vector<int> mfp_wgs_candidates();
+
 // kernel launchers (sfxg_kernels.hip)
 hipError_t launch_unpack(const DevStream *d_streams, int n_streams, int64_t n_samples, hipStream_t s);
 hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_counted_bytes,
@@ -172,7 +188,8 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
 // LDS) for N_c <= 4096.
 bool fengine_supported(int nc);
 hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window,
-                          const float2 *tw_nd, const float2 *tw_n, const float2 *tw_2n, hipStream_t s);
+                          const float2 *tw_nd, const float2 *tw_n, const float2 *tw_2n, const Tune &t,
+                          hipStream_t s);
 // tw_split: [2 N_c / 128] coarse then [128] fine twiddles of exp(-2 pi i j / 2 N_c)
 // (the LDS tables of the large-N channeliser).
 // N_c = 32768 goes through HBM scratch of channelise_scratch_elems(nc) float2
@@ -180,20 +197,21 @@ hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const f
 size_t channelise_scratch_elems(int nc);
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
                              const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
-                             hipStream_t s);
+                             const Tune &t, hipStream_t s);
 // Sub-integration partial sums (fold_start[0..n_k]: segment ranges) and
 // their in-order fold into vis.
 // spectra rows hold segments seg_base ..; fold_start holds slice segment indices.
 hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                         int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
-                        const int32_t *bl_second, int n_baselines, float2 *partial, float2 *vis, hipStream_t s);
+                        const int32_t *bl_second, int n_baselines, float2 *partial, float2 *vis, const Tune &t,
+                        hipStream_t s);
 // Mixed-precision X-step (SFXG_PREC_FP16 / BF16 / SPLIT_FP16): per-channel
 // Gram matrices on MFMA per sub-integration, formed into baselines in
 // partial (n_k * n_baselines * n_ch float2) and combined into vis in fold order.
 hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
                              int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
-                             float2 *partial, float2 *vis, hipStream_t s);
+                             float2 *partial, float2 *vis, const Tune &t, hipStream_t s);
 // segments seg_base .. seg_base + n_segments - 1 (spectra rows from seg_base),
 // continuing the bins' sums in acc.
 hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
@@ -216,7 +234,7 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_
                                     int ch_base, int n_ch_run, int M, int64_t seg_base, int64_t n_segments,
                                     float scale, const double *offsets, double start_phase, double phase_coef,
                                     double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
-                                    hipStream_t s);
+                                    const Tune &t, hipStream_t s);
 hipError_t pulsar_gram_baselines(const float *gram, int n_ch, int ch_base, int n_ch_run, int M, int nbins,
                                  const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
                                  float2 *vis, hipStream_t s);

@@ -1535,25 +1535,39 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
   return hipGetLastError();
 }
 
+template <int N, int P>
+static hipError_t launch_chan_small(const DevStream *d_streams, const DevJob &job, const float *window,
+                                    const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+  dim3 grid((unsigned)job.n_segments, job.n_streams);
+  hipLaunchKernelGGL((k_channelise<N, P>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_n, tw_2n);
+  return hipGetLastError();
+}
+
 template <int N>
 static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, const float *window,
-                                const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, hipStream_t s) {
+                                const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, const Tune &t,
+                                hipStream_t s) {
   if constexpr (N >= 8192) {
-    // persistent runs of consecutive segments, about 2048 workgroups in all
+    // persistent runs of consecutive segments, about chan_wgs (2048) workgroups in all
     constexpr int P = 16;
     if (job.seg_step != N) return hipErrorInvalidValue;
-    const int64_t per_stream = (2048 + job.n_streams - 1) / job.n_streams;
+    const int64_t wgs = t.chan_wgs > 0 ? t.chan_wgs : 2048;
+    const int64_t per_stream = (wgs + job.n_streams - 1) / job.n_streams;
     int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
     if (segs < 1) segs = 1;
     const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
     hipLaunchKernelGGL((k_channelise_run<N, P>), dim3(gx, job.n_streams), dim3(N / P), 0, s, d_streams, job, window,
                        tw_split, (int)segs);
   } else {
-    // the fused F-engine's schedule (512 threads at most), so both paths
-    // give the same bits
-    constexpr int P = N / 512 < 4 ? 4 : N / 512;
-    dim3 grid((unsigned)job.n_segments, job.n_streams);
-    hipLaunchKernelGGL((k_channelise<N, P>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_n, tw_2n);
+    // the fused F-engine's schedule, so both paths give the same bits:
+    // threads min(N / 4, 512) (PA points each) or half as many
+    constexpr int PA = N / (N / 4 < 512 ? N / 4 : 512);
+    const int T = t.fe_threads > 0 ? t.fe_threads : N / PA;
+    if (T == N / PA) return launch_chan_small<N, PA>(d_streams, job, window, tw_n, tw_2n, s);
+    if constexpr (2 * PA <= 16 && N / (2 * PA) >= 64) {
+      if (T == N / (2 * PA)) return launch_chan_small<N, 2 * PA>(d_streams, job, window, tw_n, tw_2n, s);
+    }
+    return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
@@ -1580,20 +1594,50 @@ static hipError_t launch_chan_large(const DevStream *d_streams, const DevJob &jo
 
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
                              const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
-                             hipStream_t s) {
+                             const Tune &t, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
   if (job.nc == LG_N) return launch_chan_large(d_streams, job, window, tw_n, tw_2n, scratch, s);
   switch (job.nc) {
-    case 256: return launch_chan_n<256>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
-    case 512: return launch_chan_n<512>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
-    case 1024: return launch_chan_n<1024>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
-    case 2048: return launch_chan_n<2048>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
-    case 4096: return launch_chan_n<4096>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
-    case 8192: return launch_chan_n<8192>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
-    case 16384: return launch_chan_n<16384>(d_streams, job, window, tw_n, tw_2n, tw_split, s);
+    case 256: return launch_chan_n<256>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 512: return launch_chan_n<512>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 1024: return launch_chan_n<1024>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 2048: return launch_chan_n<2048>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 4096: return launch_chan_n<4096>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 8192: return launch_chan_n<8192>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 16384: return launch_chan_n<16384>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
     default: return hipErrorInvalidValue;
   }
 }
+
+std::vector<int> fe_thread_candidates(int nc) {
+  switch (nc) {
+    case 256: return {64};
+    case 512: return {128, 64};
+    case 1024: return {256, 128};
+    case 2048: return {512, 256};
+    case 4096: return {512, 256};
+    default: return {0};
+  }
+}
+std::vector<int> chan_wgs_candidates(int nc) {
+  if (nc == 8192 || nc == 16384) return {2048, 1024, 4096};
+  return {0};
+}
+std::vector<int> xs_waves_candidates() { return {8, 4}; }
+std::vector<int> mf_channel_candidates(int precision, int M) {
+  const int RT = (2 * M + 31) / 32;
+  const int R = 32 * RT;
+  const int nT = RT * (RT + 1) / 2;
+  const int planes = precision == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
+  int CH = (MF_WAVES * MF_MAXP) / nT;
+  if (CH > 64) CH = 64;
+  while (CH > 1 && (size_t)planes * CH * (R * MF_RS + 8) * 2 > 150 * 1024) CH--;
+  std::vector<int> c{CH};
+  if (CH / 2 >= 1) c.push_back(CH / 2);
+  if (CH / 4 >= 1) c.push_back(CH / 4);
+  return c;
+}
+std::vector<int> mfp_wgs_candidates() { return {1024, 512, 2048}; }
 
 template <int N, int T>
 static hipError_t launch_fe_n(const DevStream *d_streams, const DevJob &job, const float *window,
@@ -1611,25 +1655,32 @@ static hipError_t launch_fe_n(const DevStream *d_streams, const DevJob &job, con
 bool fengine_supported(int nc) { return nc >= 256 && nc <= 4096; }
 
 hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window, const float2 *tw_nd,
-                          const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+                          const float2 *tw_n, const float2 *tw_2n, const Tune &t, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
   if (job.nd != 256) return hipErrorInvalidValue;
-  switch (job.nc) {
-    case 256: return launch_fe_n<256, 64>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 512: return launch_fe_n<512, 128>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 1024: return launch_fe_n<1024, 256>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 2048: return launch_fe_n<2048, 512>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 4096: return launch_fe_n<4096, 512>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+  const int T = t.fe_threads > 0 ? t.fe_threads : fe_thread_candidates(job.nc)[0];
+  switch (job.nc * 1024 + T) {
+    case 256 * 1024 + 64: return launch_fe_n<256, 64>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 512 * 1024 + 128: return launch_fe_n<512, 128>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 512 * 1024 + 64: return launch_fe_n<512, 64>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 1024 * 1024 + 256: return launch_fe_n<1024, 256>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 1024 * 1024 + 128: return launch_fe_n<1024, 128>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 2048 * 1024 + 512: return launch_fe_n<2048, 512>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 2048 * 1024 + 256: return launch_fe_n<2048, 256>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 4096 * 1024 + 512: return launch_fe_n<4096, 512>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 4096 * 1024 + 256: return launch_fe_n<4096, 256>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                         int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
-                        const int32_t *bl_second, int n_baselines, float2 *partial, float2 *vis, hipStream_t s) {
+                        const int32_t *bl_second, int n_baselines, float2 *partial, float2 *vis, const Tune &t,
+                        hipStream_t s) {
   if (n_k <= 0) return hipSuccess;
+  const int max_w = t.xs_waves > 0 ? t.xs_waves : 8;
   int nw = (n_baselines + XBPW - 1) / XBPW;
-  if (nw > 8) nw = 8;
+  if (nw > max_w) nw = max_w;
   const int bl_tiles = (n_baselines + nw * XBPW - 1) / (nw * XBPW);
   // a stage must fit the threads' prefetch registers (16 float2 each)
   int sps = (16 * 64 * nw) / (M * 64);
@@ -1650,7 +1701,7 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
 hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
                              int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
-                             float2 *partial, float2 *vis, hipStream_t s) {
+                             float2 *partial, float2 *vis, const Tune &t, hipStream_t s) {
   if (n_k <= 0) return hipSuccess;
   const int RT = (2 * M + 31) / 32;
   const int R = 32 * RT;
@@ -1666,6 +1717,7 @@ hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t strea
     return stage > gbuf ? stage : gbuf;
   };
   while (CH > 1 && lds_of(CH) > 150 * 1024) CH--;
+  if (t.mf_channels > 0 && CH > t.mf_channels) CH = t.mf_channels;
   if (CH < 1 || nT > MF_WAVES * MF_MAXP) return hipErrorInvalidValue;
   const size_t lds = lds_of(CH);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
@@ -1776,15 +1828,16 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_
                                     int ch_base, int n_ch_run, int M, int64_t seg_base, int64_t n_segments,
                                     float scale, const double *offsets, double start_phase, double phase_coef,
                                     double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
-                                    hipStream_t s) {
+                                    const Tune &t, hipStream_t s) {
   if (n_segments <= 0 || n_ch_run <= 0) return hipSuccess;
   size_t lds = 0;
   const int CH = mf_channels(precision, M, MFP_MAXP, (size_t)64 * MF_KS * sizeof(int16_t), &lds);
   if (CH < 1) return hipErrorInvalidValue;
   const int RT = (2 * M + 31) / 32;
-  // about 1024 workgroups: channel blocks x segment ranges (multiples of a stage)
+  // about mfp_wgs (1024) workgroups: channel blocks x segment ranges (multiples of a stage)
   const int64_t nblk = (n_ch_run + CH - 1) / CH;
-  int64_t nz = (1024 + nblk - 1) / nblk;
+  const int64_t wgs = t.mfp_wgs > 0 ? t.mfp_wgs : 1024;
+  int64_t nz = (wgs + nblk - 1) / nblk;
   int64_t per = (n_segments + nz - 1) / nz;
   per = (per + MF_KS - 1) / MF_KS * MF_KS;
   nz = (n_segments + per - 1) / per;

@@ -104,6 +104,14 @@ def lib() -> C.CDLL:
         L.sfxg_batch_clear.argtypes = [C.c_void_p]
         L.sfxg_batch_launch.argtypes = [C.c_void_p]
         L.sfxg_batch_synchronize.argtypes = [C.c_void_p]
+        L.sfxg_autotune.restype = C.c_int
+        L.sfxg_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_size_t]
+        L.sfxg_tune_export.restype = C.c_int
+        L.sfxg_tune_export.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.sfxg_tune_import.restype = C.c_int
+        L.sfxg_tune_import.argtypes = [C.c_char_p]
+        L.sfxg_tune_clear.restype = None
+        L.sfxg_tune_clear.argtypes = []
         L.sfxg_synthesize_raw.restype = C.c_int
         L.sfxg_synthesize_raw.argtypes = [C.c_int, C.c_void_p, C.c_size_t, C.c_int32, C.c_int64, C.c_int64,
                                           C.c_uint64, C.c_uint64, C.c_float, C.c_float]
@@ -205,6 +213,14 @@ class Context:
         """Host half of correlate; the device work runs in a Batch."""
         self._check(lib().sfxg_prepare(self._h), "sfxg_prepare")
 
+    def autotune(self, reps: int = 3) -> str:
+        """Times the launch-shape candidates on the prepared slice and keeps the
+        fastest for its shape; returns the candidates' times. The slice must be
+        prepared (or correlated) again afterwards."""
+        buf = C.create_string_buffer(4096)
+        self._check(lib().sfxg_autotune(self._h, reps, buf, len(buf)), "sfxg_autotune")
+        return buf.value.decode()
+
     def synchronize(self):
         self._check(lib().sfxg_synchronize(self._h), "sfxg_synchronize")
 
@@ -249,6 +265,31 @@ class Context:
         self._check(lib().sfxg_fetch_stage(self._h, stage, stream, out.ctypes.data, out.nbytes, C.byref(w)),
                     "sfxg_fetch_stage")
         return out
+
+
+def tune_key(nchan: int, n_streams: int, precision: int, pulsar: bool, fused: bool) -> str:
+    """The tuning table's shape key of a slice."""
+    return f"nc={nchan} M={n_streams} prec={precision} pulsar={int(pulsar)} fused={int(fused)}"
+
+
+def tune_export() -> str:
+    n = C.c_size_t()
+    lib().sfxg_tune_export(None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value)
+    rc = lib().sfxg_tune_export(buf, n.value, C.byref(n))
+    if rc != SFXG_OK:
+        raise SfxgError(f"sfxg_tune_export failed ({rc})")
+    return buf.value.decode()
+
+
+def tune_import(text: str):
+    rc = lib().sfxg_tune_import(text.encode())
+    if rc != SFXG_OK:
+        raise SfxgError(f"sfxg_tune_import failed ({rc})")
+
+
+def tune_clear():
+    lib().sfxg_tune_clear()
 
 
 class Batch:

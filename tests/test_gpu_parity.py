@@ -467,3 +467,100 @@ def test_batch_graph_replays_match_eager():
     for c in ctxs:
         c.close()
     eager_ctx.close()
+
+
+# (nchan, precision, pulsar, fields) with fields = (fe_threads, chan_wgs, xs_waves, mf_channels, mfp_wgs)
+TUNE_VARIANTS = [
+    (512, 0, False, (64, 0, 0, 0, 0)),
+    (1024, 0, False, (128, 0, 4, 0, 0)),
+    (2048, 0, False, (256, 0, 0, 0, 0)),
+    (4096, 0, False, (256, 0, 0, 0, 0)),
+    (16384, 0, False, (0, 1024, 0, 0, 0)),
+    (8192, 0, False, (0, 4096, 0, 0, 0)),
+    (1024, 1, False, (128, 0, 0, 8, 0)),
+    (1024, 1, True, (0, 0, 0, 0, 512)),
+]
+
+
+@pytest.mark.parametrize("nchan,prec,pulsar,fields", TUNE_VARIANTS)
+def test_tuned_launch_shapes(ctx, nchan, prec, pulsar, fields):
+    """Every launch shape the autotuner can pick, forced through the tuning
+    table, against the oracle (FP32) or the FP64 oracle (mixed)."""
+    from sfxg.job import Pulsar, ticks_to_mjd
+    job, hdrs = configs.build_slices(seed=40, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
+                                     seconds=0.125, t_int_usec=125000, rho=0.05, bits=2, sky_freq=1_400_000_000,
+                                     offsets=[0.0, 2e-6, -1e-6], rates=[0.0, 1e-7, -2e-7])[0]
+    psr = None
+    if pulsar:
+        psr = Pulsar(nbins=8, gate_start=0.1, gate_stop=0.6, tmid=ticks_to_mjd(job.start_time) - 0.001, DM=68.0,
+                     ref_phase=0.3, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+    capi.tune_clear()
+    key = capi.tune_key(nchan, 3, prec, pulsar, nchan <= 4096)
+    capi.tune_import(key + "|" + "|".join(str(v) for v in fields) + "\n")
+    assert key in capi.tune_export()
+    try:
+        out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, pulsar=psr, precision=prec)
+    finally:
+        capi.tune_clear()
+        ctx.set_options(lut=LUT_CONFIG, precision=0)
+    nb = psr.nbins + 1 if psr else 1
+    heads_g, vis_g = vis_from_bytes(job, out, nb)
+    if prec == 0:
+        ref = O.run_slice(job, hdrs, lut=LUT_CONFIG, pulsar=psr)
+        heads_r, vis_r = vis_from_bytes(job, ref["output"], nb)
+        assert heads_g == heads_r
+        for b in range(nb):
+            compare_vis(vis_g[b], vis_r[b], label=f"tuned {fields} bin {b}")
+    else:
+        r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64", pulsar=psr)
+        heads_r, vis_r = vis_from_bytes(job, r64["output"], nb)
+        assert heads_g == heads_r
+        for b in range(nb):
+            l2, worst = mixed_errors(vis_g[b], vis_r[b])
+            assert l2 <= MIXED_BARS[prec][0] and worst <= MIXED_BARS[prec][1]
+
+
+def test_tuned_fused_matches_unfused(ctx):
+    """With a tuned F-engine schedule the unfused channeliser follows it, and
+    both paths still give the same bytes."""
+    job, hdrs = configs.build_slices(seed=41, n_stations=2, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=1024,
+                                     seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=[0.0, 1e-6], rates=[0.0, 1e-7])[0]
+    capi.tune_clear()
+    capi.tune_import(capi.tune_key(1024, 2, 0, False, True) + "|128|0|0|0|0\n" +
+                     capi.tune_key(1024, 2, 0, False, False) + "|128|0|0|0|0\n")
+    try:
+        a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=True)
+        b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=False)
+        capi.tune_clear()
+        c = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    finally:
+        capi.tune_clear()
+    assert a == b
+    assert a != c  # another radix schedule: other rounding
+
+
+def test_autotune_then_correlate(ctx):
+    job, hdrs = configs.build_slices(seed=42, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=2048,
+                                     seconds=0.125, t_int_usec=125000, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=[0.0, 1e-6, 2e-6], rates=[0.0, 1e-7, 0.0])[0]
+    capi.tune_clear()
+    try:
+        ctx.set_options(lut=LUT_CONFIG)
+        ctx.set_parameters(job)
+        for i, h in enumerate(hdrs):
+            ctx.push_stream(i, h)
+        ctx.prepare()
+        report = ctx.autotune(reps=2)
+        print(report)
+        assert "fe_threads=512" in report and "fe_threads=256" in report and "xs_waves=4" in report
+        assert capi.tune_key(2048, 3, 0, False, True) in capi.tune_export()
+        ctx.correlate()
+        out = ctx.output()
+    finally:
+        capi.tune_clear()
+    ref = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+    heads_g, vis_g = vis_from_bytes(job, out)
+    heads_r, vis_r = vis_from_bytes(job, ref["output"])
+    assert heads_g == heads_r
+    compare_vis(vis_g[0], vis_r[0], label="autotuned")
