@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--c4-precisions", default="fp32,fp16,split_fp16")
     ap.add_argument("--cpu-cores", type=int, default=0, help="CPU baseline worker processes (default: up to 16)")
     ap.add_argument("--no-peaks", action="store_true", help="skip the achievable-peak microbenchmarks")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default launch shapes instead of the autotuner's (run before each timed pass)")
     ap.add_argument("--traffic-bytes", type=float, default=0.0,
                     help="HBM bytes per launch of the dominant kernel from the calibrated FETCH_SIZE + WRITE_SIZE "
                          "counters, reported as roofline.traffic")
@@ -204,11 +206,18 @@ class Workload:
         self.nsamp = job.n_ffts_per_integration(0) * job.fft_size_delaycor
         self.station_samples_per_step = NSB * self.M * self.nsamp
 
-    def run(self, torch, capi, precision, W, n_ctx, dev):
+    def run(self, torch, capi, precision, W, n_ctx, dev, autotune=True):
         ctxs = [capi.Context(dev) for _ in range(max(1, n_ctx))]
         for c in ctxs:
             c.set_options(lut=1, precision=precision)
         counter = [0]
+        if autotune:
+            c = ctxs[0]
+            c.set_parameters(self.per_sb[0][0][0], pulsar=self.pulsar)
+            for i, (ctrl, buf, n) in enumerate(self.inputs[(0, 0)]):
+                c.push_stream_elided(i, ctrl, buf.data_ptr(), n)
+            c.prepare()
+            self.tune_report = c.autotune(reps=2)
 
         def run_slice(k, sb, ctx=None):
             ctx = ctx or ctxs[counter[0] % len(ctxs)]
@@ -278,12 +287,28 @@ class MixedBatch:
         self.setup_s = time.perf_counter() - t0
         self.rounds = max(len(d["jobs"]) for d in self.jobs)
 
-    def run(self, torch, capi, precision, dev, chunk_mib=512):
+    def run(self, torch, capi, precision, dev, chunk_mib=512, autotune=True):
         sets = [[capi.Context(dev) for _ in self.jobs] for _ in range(2)]
         for cs in sets:
             for c in cs:
                 c.set_options(lut=1, precision=precision, chunk_mib=chunk_mib)
         batches = [capi.Batch(dev, n_branches=4) for _ in range(2)]
+        if autotune:
+            # one slice of every shape through the autotuner
+            seen = set()
+            for j, d in enumerate(self.jobs):
+                job = d["jobs"][0]
+                key = (job.fft_size_correlation, len(job.station_streams))
+                if key in seen:
+                    continue
+                seen.add(key)
+                c = sets[0][j]
+                c.set_parameters(job)
+                for i, (ctrl, buf, n) in enumerate(self.inputs[j][0]):
+                    c.push_stream_elided(i, ctrl, buf.data_ptr(), n)
+                c.prepare()
+                c.autotune(reps=1)
+            self.tuned_shapes = len(seen)
 
         def prepare(r):
             cs, b = sets[r % 2], batches[r % 2]
@@ -364,7 +389,7 @@ def main():
         wl = Workload(torch, capi, synth, lambda sb, k: configs.config1_jobs(sb, n_integrations=k), K, NSB, dev,
                       1000)
         print(f"setup done: {len(wl.inputs)} slices in {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
-        wall, kt = wl.run(torch, capi, PREC["fp32"], W, args.contexts, dev)
+        wall, kt = wl.run(torch, capi, PREC["fp32"], W, args.contexts, dev, autotune=not args.no_autotune)
         main_r = summarise(wl, wall, kt)
         print(f"plain FP32 done: {main_r['value']:.2f} station-Gsamples/s", file=sys.stderr, flush=True)
         dom = int(np.argmax(kt))
@@ -402,7 +427,7 @@ def main():
             "setup_s": round(wl.setup_s, 1),
         })
         if not args.main_only:
-            wall, kt = wl.run(torch, capi, PREC["split_fp16"], W, args.contexts, dev)
+            wall, kt = wl.run(torch, capi, PREC["split_fp16"], W, args.contexts, dev, autotune=not args.no_autotune)
             mixed = summarise(wl, wall, kt)
             mixed.pop("_kb")
             add_mfma_frac(mixed, peaks)
@@ -418,7 +443,7 @@ def main():
                           f"0.5 s integrations, {args.c2_steps * 0.5:g} s",
               "station_samples_per_step": w2.station_samples_per_step}
         for name in args.c2_precisions.split(","):
-            wall, kt = w2.run(torch, capi, PREC[name], 0, args.contexts, dev)
+            wall, kt = w2.run(torch, capi, PREC[name], 0, args.contexts, dev, autotune=not args.no_autotune)
             r = summarise(w2, wall, kt)
             r.pop("_kb")
             if name != "fp32":
@@ -444,7 +469,7 @@ def main():
               "channels": dict(Counter(str(d["nchan"]) for d in mb.jobs)),
               "bits": dict(Counter(str(d["bits"]) for d in mb.jobs))}
         for name in args.c3_precisions.split(","):
-            wall = mb.run(torch, capi, PREC[name], dev)
+            wall = mb.run(torch, capi, PREC[name], dev, autotune=not args.no_autotune)
             c3[name] = {"value": round(mb.station_samples / wall / 1e9, 4), "wall_s": round(wall, 3)}
             print(f"mixed batch {name} done: {c3[name]['value']:.2f}", file=sys.stderr, flush=True)
         if "fp32" in c3 and "fp16" in c3:
@@ -461,7 +486,7 @@ def main():
                           "+ off-gate bin, P = 0.0893 s, DM 68",
               "station_samples_per_step": w4.station_samples_per_step}
         for name in args.c4_precisions.split(","):
-            wall, kt = w4.run(torch, capi, PREC[name], 0, args.contexts, dev)
+            wall, kt = w4.run(torch, capi, PREC[name], 0, args.contexts, dev, autotune=not args.no_autotune)
             r = summarise(w4, wall, kt)
             r.pop("_kb")
             if name != "fp32":
@@ -476,6 +501,13 @@ def main():
         torch.cuda.empty_cache()
     if peaks:
         line["achievable_peaks"] = peaks
+    if not args.no_autotune:
+        table = capi.tune_export().strip().split("\n")
+        print("tuning table:\n" + "\n".join(table), file=sys.stderr, flush=True)
+        main_keys = [capi.tune_key(4096, 16, 0, False, True), capi.tune_key(16384, 64, 0, False, False),
+                     capi.tune_key(16384, 64, 1, False, False), capi.tune_key(1024, 24, 1, True, True)]
+        line["autotuner"] = {"shapes_tuned": len(table),
+                             "main_shapes": [t for t in table if t.split("|")[0] in main_keys]}
     if cpu:
         line["cpu_baseline"] = cpu
     print(json.dumps(line))
