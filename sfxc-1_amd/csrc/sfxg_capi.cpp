@@ -112,6 +112,7 @@ struct sfxg_ctx {
   int64_t n_ffts_per_integration = 0, n_blocks = 0, n_samples = 0, nseg = 0, S = 0;
   int n_integration = 0, n_sub = 0, nfft_max = 0;
   std::vector<int32_t> bl_first, bl_second;
+  std::vector<int32_t> h_tiles;  // register-tile table of the FP32 X-step
   std::vector<StreamPlan> plans;
   std::vector<int32_t> seg_per_chunk, folds;
   std::vector<double> flag_factor;
@@ -121,7 +122,7 @@ struct sfxg_ctx {
   // device buffers
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
       flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, tw_split, window, offsets, samples,
-      chan_scratch;
+      chan_scratch, tiles;
   // host staging of the enqueued slice (kept until the next slice)
   std::vector<double> h_model, h_offsets;
   std::vector<DevStream> h_streams;
@@ -228,7 +229,8 @@ void sfxg_destroy(sfxg_ctx *ctx) {
   DBuf *bufs[] = {&ctx->payload, &ctx->runs, &ctx->block_run, &ctx->counted, &ctx->dstreams,
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
                   &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
-                  &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples, &ctx->chan_scratch};
+                  &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples, &ctx->chan_scratch,
+                  &ctx->tiles};
   for (DBuf *b : bufs) b->release();
   sync_slice(ctx);
   if (ctx->done) (void)hipEventDestroy(ctx->done);
@@ -354,6 +356,7 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
     }
   }
   ctx->B = (int)ctx->bl_first.size();
+  xstep_tile_table(ctx->M, ctx->bl_first, ctx->bl_second, ctx->h_tiles);
   ctx->plans.assign(ctx->M, StreamPlan());
   ctx->have_params = true;
   return SFXG_OK;
@@ -604,6 +607,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->norms.ensure((size_t)ctx->nbins * M * sizeof(double)));
   HIPCHK(ctx, ctx->flags.ensure((size_t)B * sizeof(double)));
   HIPCHK(ctx, ctx->bl.ensure((size_t)2 * B * sizeof(int32_t)));
+  HIPCHK(ctx, ctx->tiles.ensure(std::max<size_t>(ctx->h_tiles.size(), 1) * sizeof(int32_t)));
   HIPCHK(ctx, ctx->fold_buf.ensure((ctx->folds.size() + 1) * sizeof(int32_t)));
   // sub-integration partial sums, at most ~1 GiB at a time
   const size_t part_one = (size_t)B * (nc + 1) * sizeof(float2);
@@ -716,6 +720,9 @@ static int prepare_slice(sfxg_ctx *ctx) {
   std::copy(ctx->bl_first.begin(), ctx->bl_first.end(), blv.begin());
   std::copy(ctx->bl_second.begin(), ctx->bl_second.end(), blv.begin() + B);
   HIPCHK(ctx, hipMemcpyAsync(ctx->bl.p, blv.data(), blv.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (!ctx->h_tiles.empty())
+    HIPCHK(ctx, hipMemcpyAsync(ctx->tiles.p, ctx->h_tiles.data(), ctx->h_tiles.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, s));
   HIPCHK(ctx, hipMemcpyAsync(ctx->flags.p, ctx->flag_factor.data(), B * sizeof(double), hipMemcpyHostToDevice, s));
   std::vector<int32_t> &fold_start = ctx->h_fold_start;
   fold_start.assign(1, 0);
@@ -841,9 +848,15 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
   } else if (!ctx->pulsar) {
     for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
       const int ng = std::min(lp.k_group, ch.fold1 - g0);
-      HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
-                               ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),
-                               ctx->vis.as<float2>(), t, s));
+      if (t.xs_waves == 0)
+        HIPCHK(ctx, launch_xstep_tile(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
+                                      ctx->fold_buf.as<int32_t>() + g0, ng, ctx->tiles.as<int32_t>(),
+                                      (int)(ctx->h_tiles.size() / 18), B, ctx->partial.as<float2>(),
+                                      ctx->vis.as<float2>(), s));
+      else
+        HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
+                                 ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),
+                                 ctx->vis.as<float2>(), t, s));
     }
   } else if (lp.mixed) {
     // with several chunks the channel run covers all channels (checked in prepare)

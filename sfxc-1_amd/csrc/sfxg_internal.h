@@ -164,7 +164,8 @@ struct Tune {
   int fe_threads = 0;   // F-engine threads per workgroup for N_c <= 4096 (fused and
                         // unfused channeliser alike: N_c / threads points per thread)
   int chan_wgs = 0;     // large-N channeliser (N_c 8192, 16384): workgroups in all
-  int xs_waves = 0;     // plain X-step: waves per workgroup
+  int xs_waves = 0;     // plain X-step: 0 register tiles, else the per-baseline kernel with
+                        // this many waves per workgroup
   int mf_channels = 0;  // MFMA X-step: at most this many channels per workgroup
   int mfp_wgs = 0;      // MFMA pulsar X-step: workgroups in all
 };
@@ -208,6 +209,13 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
 // Mixed-precision X-step (SFXG_PREC_FP16 / BF16 / SPLIT_FP16): per-channel
 // Gram matrices on MFMA per sub-integration, formed into baselines in
 // partial (n_k * n_baselines * n_ch float2) and combined into vis in fold order.
+// Register-tiled FP32 X-step: tiles from xstep_tile_table (2 + 16 ints per
+// tile: first streams a0, b0 and the baseline of each of the 4 x 4 pairs).
+void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::vector<int32_t> &bl_second,
+                      std::vector<int32_t> &table);
+hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
+                             int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
+                             int n_baselines, float2 *partial, float2 *vis, hipStream_t s);
 hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
                              int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,

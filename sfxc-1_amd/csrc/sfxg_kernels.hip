@@ -950,6 +950,111 @@ __global__ __launch_bounds__(512) void k_xstep_subint(const float2 *__restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_xstep_tile: the plain FP32 X-step as register tiles of stream pairs. A
+// wave owns a 4 x 4 tile (streams a0.. x b0..) of 64 channels (lane =
+// channel): per segment it reads 8 spectra from LDS for 16 products, where a
+// per-baseline scheme reads 2 per product. 1024-thread workgroups stage
+// segments x all streams x 64 channels in LDS (one workgroup per CU), so the
+// spectra of a channel tile are fetched ceil(tiles / 16) times. The tile
+// table maps each of the 16 pairs to its baseline: b >= 0 stores the sum,
+// ~b its conjugate (the baseline lists the pair the other way round: the
+// product of the swapped pair is exactly the conjugate), XT_NONE nothing.
+// Products and sums are the reference's, in its order, per baseline.
+// ---------------------------------------------------------------------------
+constexpr int XT = 4;          // streams per tile side
+constexpr int XT_WAVES = 16;   // tiles per workgroup
+constexpr int XT_NONE = INT32_MIN;
+constexpr int XT_PF = 8;       // prefetch registers per thread (float2)
+
+__global__ __launch_bounds__(64 * XT_WAVES) void k_xstep_tile(
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch, int M,
+    const int32_t *__restrict__ fold_start, const int32_t *__restrict__ tiles, int n_tiles, int n_baselines, int sps,
+    float2 *__restrict__ partial) {
+#pragma clang fp contract(off)
+  extern __shared__ float2 xlds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ch0 = blockIdx.x * 64;
+  const int ch = ch0 + lane;
+  const int k = blockIdx.z;
+  const int64_t s_begin = fold_start[k], s_end = fold_start[k + 1];
+  const int t = __builtin_amdgcn_readfirstlane(blockIdx.y * XT_WAVES + wave);
+  const bool active = t < n_tiles;
+  const int a0 = active ? tiles[t * (2 + XT * XT)] : 0;
+  const int b0 = active ? tiles[t * (2 + XT * XT) + 1] : 0;
+  float2 acc[XT * XT];
+#pragma unroll
+  for (int q = 0; q < XT * XT; q++) acc[q] = make_float2(0.0f, 0.0f);
+  const int rows_per_seg = M * 64;
+  constexpr int PF = XT_PF;  // per-thread prefetch registers (float2)
+  float2 pf[PF];
+  auto fetch = [&](int64_t s0, int ns) {
+    const int total = ns * rows_per_seg;
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int idx = threadIdx.x + u * (64 * XT_WAVES);
+      float2 v = make_float2(0.0f, 0.0f);
+      if (idx < total) {
+        const int c = idx & 63;
+        const int r = idx >> 6;
+        const int sl = r / M, st = r - sl * M;
+        if (ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + sl - seg_base) * seg_stride + ch0 + c];
+      }
+      pf[u] = v;
+    }
+  };
+  int64_t s0 = s_begin;
+  int ns = (int)((s_end - s0) < sps ? (s_end - s0) : sps);
+  if (s0 < s_end) fetch(s0, ns);
+  while (s0 < s_end) {
+    const int total = ns * rows_per_seg;
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int idx = threadIdx.x + u * (64 * XT_WAVES);
+      if (idx < total) xlds[idx] = pf[u];
+    }
+    __syncthreads();
+    const int64_t s1 = s0 + ns;
+    const int ns1 = (int)((s_end - s1) < sps ? (s_end - s1) : sps);
+    if (s1 < s_end) fetch(s1, ns1);
+    if (active) {
+#pragma unroll 1
+      for (int sl = 0; sl < ns; sl++) {
+        const float2 *row = xlds + sl * rows_per_seg + lane;
+        float2 x[XT], y[XT];
+#pragma unroll
+        for (int i = 0; i < XT; i++) {
+          x[i] = a0 + i < M ? row[(a0 + i) * 64] : make_float2(0.0f, 0.0f);
+          y[i] = b0 + i < M ? row[(b0 + i) * 64] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int i = 0; i < XT; i++)
+#pragma unroll
+          for (int j = 0; j < XT; j++) {
+            const float pr = x[i].x * y[j].x + x[i].y * y[j].y;
+            const float pi = x[i].y * y[j].x - x[i].x * y[j].y;
+            acc[XT * i + j].x = acc[XT * i + j].x + pr;
+            acc[XT * i + j].y = acc[XT * i + j].y + pi;
+          }
+      }
+    }
+    __syncthreads();
+    s0 = s1;
+    ns = ns1;
+  }
+  if (active && ch < n_ch) {
+    float2 *o = partial + (int64_t)k * n_baselines * n_ch + ch;
+#pragma unroll
+    for (int q = 0; q < XT * XT; q++) {
+      const int e = tiles[t * (2 + XT * XT) + 2 + q];
+      if (e == XT_NONE) continue;
+      if (e >= 0) o[(int64_t)e * n_ch] = acc[q];
+      else o[(int64_t)(~e) * n_ch] = make_float2(acc[q].x, -acc[q].y);
+    }
+  }
+}
+
 // vis += partial[k] for k = 0 .. n_k - 1, in fold order.
 __global__ __launch_bounds__(256) void k_fold_combine(const float2 *__restrict__ partial, int n_k, int64_t n,
                                                       float2 *__restrict__ vis) {
@@ -1623,7 +1728,7 @@ std::vector<int> chan_wgs_candidates(int nc) {
   if (nc == 8192 || nc == 16384) return {2048, 1024, 4096};
   return {0};
 }
-std::vector<int> xs_waves_candidates() { return {8, 4}; }
+std::vector<int> xs_waves_candidates() { return {0, 8, 4}; }  // 0: register tiles
 std::vector<int> mf_channel_candidates(int precision, int M) {
   const int RT = (2 * M + 31) / 32;
   const int R = 32 * RT;
@@ -1692,6 +1797,55 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
   hipLaunchKernelGGL(k_xstep_subint, grid, dim3(64 * nw), lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch,
                      M, fold_start, bl_first, bl_second, n_baselines, sps, partial);
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t n = (int64_t)n_baselines * n_ch;
+  hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
+  return hipGetLastError();
+}
+
+void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::vector<int32_t> &bl_second,
+                      std::vector<int32_t> &table) {
+  const int nt = (M + XT - 1) / XT;
+  std::vector<int32_t> idx((size_t)M * M, XT_NONE);  // (lo, hi) -> baseline (or ~b if listed as (hi, lo))
+  for (size_t b = 0; b < bl_first.size(); b++) {
+    const int f = bl_first[b], g = bl_second[b];
+    if (f <= g) idx[(size_t)f * M + g] = (int32_t)b;
+    else idx[(size_t)g * M + f] = ~(int32_t)b;
+  }
+  table.clear();
+  for (int ta = 0; ta < nt; ta++)
+    for (int tb = ta; tb < nt; tb++) {
+      std::vector<int32_t> e(2 + XT * XT, XT_NONE);
+      e[0] = ta * XT;
+      e[1] = tb * XT;
+      bool any = false;
+      for (int i = 0; i < XT; i++)
+        for (int j = 0; j < XT; j++) {
+          const int a = ta * XT + i, c = tb * XT + j;
+          if (a >= M || c >= M || a > c) continue;
+          e[2 + XT * i + j] = idx[(size_t)a * M + c];
+          any = any || e[2 + XT * i + j] != XT_NONE;
+        }
+      if (any) table.insert(table.end(), e.begin(), e.end());
+    }
+}
+
+hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
+                             int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
+                             int n_baselines, float2 *partial, float2 *vis, hipStream_t s) {
+  if (n_k <= 0) return hipSuccess;
+  // a stage fills the threads' prefetch registers
+  int sps = (XT_PF * 64 * XT_WAVES) / (M * 64);
+  if (sps > 16) sps = 16;
+  if (sps < 1) return hipErrorInvalidValue;  // more streams than one stage holds
+  const size_t lds = (size_t)sps * M * 64 * sizeof(float2);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void *)k_xstep_tile, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)((n_tiles + XT_WAVES - 1) / XT_WAVES), (unsigned)n_k);
+  hipLaunchKernelGGL(k_xstep_tile, grid, dim3(64 * XT_WAVES), lds, s, spectra, stream_stride, seg_stride, seg_base,
+                     n_ch, M, fold_start, tiles, n_tiles, n_baselines, sps, partial);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int64_t n = (int64_t)n_baselines * n_ch;
   hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);

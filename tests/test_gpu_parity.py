@@ -564,3 +564,29 @@ def test_autotune_then_correlate(ctx):
     heads_r, vis_r = vis_from_bytes(job, ref["output"])
     assert heads_g == heads_r
     compare_vis(vis_g[0], vis_r[0], label="autotuned")
+
+
+@pytest.mark.parametrize("xs_waves", [0, 8])
+def test_station_order_conjugates(ctx, xs_waves):
+    """Station numbers out of stream order: crosses list the lower station
+    first, so some products are taken the other way round (the register-tile
+    X-step stores their conjugates)."""
+    import dataclasses
+    job, hdrs = configs.build_slices(seed=43, n_stations=6, pols=["R"], fs=16_000_000, bw=8_000_000, nchan=512,
+                                     seconds=0.125, t_int_usec=125000, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=[0.0, 1e-6, -1e-6, 2e-6, 0.5e-6, -2e-6],
+                                     rates=[0.0, 1e-7, 0.0, -1e-7, 2e-7, 0.0])[0]
+    perm = [4, 0, 5, 2, 1, 3]
+    job.station_streams = [dataclasses.replace(s, station_number=perm[i]) for i, s in enumerate(job.station_streams)]
+    capi.tune_clear()
+    if xs_waves:
+        capi.tune_import(capi.tune_key(512, 6, 0, False, True) + f"|0|0|{xs_waves}|0|0\n")
+    try:
+        out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    finally:
+        capi.tune_clear()
+    ref = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+    heads_g, vis_g = vis_from_bytes(job, out)
+    heads_r, vis_r = vis_from_bytes(job, ref["output"])
+    assert heads_g == heads_r
+    compare_vis(vis_g[0], vis_r[0], label=f"station order, xs_waves={xs_waves}")
