@@ -817,8 +817,8 @@ static int enqueue_fengine(sfxg_ctx *ctx, size_t c, hipStream_t s, bool timing, 
   const DevStream *dstr = ctx->dstreams.as<DevStream>();
   if (c == 0) HIPCHK(ctx, launch_block_consts(dstr, cj, s));
   if (lp.fused) {
-    HIPCHK(ctx, launch_fengine(dstr, cj, ctx->window.as<float>(), ctx->tw_nd.as<float2>(), ctx->tw_n.as<float2>(),
-                               ctx->tw_2n.as<float2>(), t, s));
+    HIPCHK(ctx, launch_fengine(dstr, cj, ctx->window.as<float>(), ctx->tw_nd.as<float2>(), ctx->tw_split.as<float2>(),
+                               t, s));
     HIPCHK(ctx, timing ? hipEventRecord(ctx->cev[4 * c + 1], s) : hipSuccess);
   } else {
     HIPCHK(ctx, launch_delay(dstr, cj, ctx->tw_nd.as<float2>(), s));

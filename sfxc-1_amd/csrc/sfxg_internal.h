@@ -189,8 +189,7 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
 // LDS) for N_c <= 4096.
 bool fengine_supported(int nc);
 hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window,
-                          const float2 *tw_nd, const float2 *tw_n, const float2 *tw_2n, const Tune &t,
-                          hipStream_t s);
+                          const float2 *tw_nd, const float2 *tw_split, const Tune &t, hipStream_t s);
 // tw_split: [2 N_c / 128] coarse then [128] fine twiddles of exp(-2 pi i j / 2 N_c)
 // (the LDS tables of the large-N channeliser).
 // N_c = 32768 goes through HBM scratch of channelise_scratch_elems(nc) float2

@@ -569,10 +569,14 @@ __device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, floa
 template <int N, int P>
 __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restrict__ streams, DevJob job,
                                                       const float *__restrict__ window,
-                                                      const float2 *__restrict__ tw_n,
-                                                      const float2 *__restrict__ tw_2n) {
-  constexpr int T = N / P;
+                                                      const float2 *__restrict__ tw_split) {
+  constexpr int T = N / P, NHI = 2 * N / 128;
   __shared__ float2 lds[fft_lds_size<N, P>()];
+  __shared__ float2 thi[NHI], tlo[128];
+  for (int i = threadIdx.x; i < NHI + 128; i += T) {
+    const float2 t = tw_split[i];
+    if (i < NHI) thi[i] = t; else tlo[i - NHI] = t;
+  }
   const DevStream &s = streams[blockIdx.y];
   const int64_t seg = blockIdx.x;
   const float2 *x = reinterpret_cast<const float2 *>(s.time_series + seg * job.seg_step);
@@ -592,7 +596,8 @@ __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restric
     if (s.flip) z.y = -z.y;
     v[i] = z;
   }
-  const TableTw tw{tw_n, tw_2n};
+  __syncthreads();  // twiddle tables
+  const LdsTw tw{thi, tlo};
   reg_fft<N, P>(v, lds, tw, threadIdx.x);
   real_split<N, P, T>(lds, tw, s.spectra + seg * job.spectra_stride, threadIdx.x);
 }
@@ -780,13 +785,13 @@ __global__ __launch_bounds__(256) void k_chan_split(const DevStream *__restrict_
 template <int N, int T>
 __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ streams, DevJob job,
                                                const float *__restrict__ window, const float2 *__restrict__ tw_nd,
-                                               const float2 *__restrict__ tw_n, const float2 *__restrict__ tw_2n,
-                                               int segs_per_wg) {
+                                               const float2 *__restrict__ tw_split, int segs_per_wg) {
   static_assert(N >= 256 && (T / 64) * 256 <= N, "wave scratch must fit in the FFT buffer");
-  constexpr int P = N / T;
+  constexpr int P = N / T, NHI = 2 * N / 128;
   __shared__ float ring[2 * N];
   __shared__ float2 fbuf[fft_lds_size<N, P>()];
   __shared__ float2 tw[256];
+  __shared__ float2 thi[NHI], tlo[128];
   const DevStream &s = streams[blockIdx.y];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   constexpr int nwaves = T / 64;
@@ -796,7 +801,12 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
   if (sb > job.n_segments) sb = job.n_segments;
   if (sa >= sb) return;  // uniform over the workgroup
   for (int i = threadIdx.x; i < 256; i += T) tw[i] = tw_nd[i];
+  for (int i = threadIdx.x; i < NHI + 128; i += T) {
+    const float2 t = tw_split[i];
+    if (i < NHI) thi[i] = t; else tlo[i - NHI] = t;
+  }
   __syncthreads();
+  const LdsTw twl{thi, tlo};
   const bool none = job.window == SFXG_WINDOW_NONE;
   float2 *scratch = fbuf + wave * 256;
   // blocks go in pairs (blk, blk + 1) with blk even, as in k_delay
@@ -839,9 +849,8 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
       if (s.flip) z.y = -z.y;
       v[i] = z;
     }
-    const TableTw twt{tw_n, tw_2n};
-    reg_fft<N, P>(v, fbuf, twt, tid);
-    real_split<N, P, T>(fbuf, twt, s.spectra + seg * job.spectra_stride, tid);
+    reg_fft<N, P>(v, fbuf, twl, tid);
+    real_split<N, P, T>(fbuf, twl, s.spectra + seg * job.spectra_stride, tid);
     __syncthreads();
   }
 }
@@ -1642,9 +1651,9 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
 
 template <int N, int P>
 static hipError_t launch_chan_small(const DevStream *d_streams, const DevJob &job, const float *window,
-                                    const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+                                    const float2 *tw_split, hipStream_t s) {
   dim3 grid((unsigned)job.n_segments, job.n_streams);
-  hipLaunchKernelGGL((k_channelise<N, P>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_n, tw_2n);
+  hipLaunchKernelGGL((k_channelise<N, P>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_split);
   return hipGetLastError();
 }
 
@@ -1668,9 +1677,9 @@ static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, c
     // threads min(N / 4, 512) (PA points each) or half as many
     constexpr int PA = N / (N / 4 < 512 ? N / 4 : 512);
     const int T = t.fe_threads > 0 ? t.fe_threads : N / PA;
-    if (T == N / PA) return launch_chan_small<N, PA>(d_streams, job, window, tw_n, tw_2n, s);
+    if (T == N / PA) return launch_chan_small<N, PA>(d_streams, job, window, tw_split, s);
     if constexpr (2 * PA <= 16 && N / (2 * PA) >= 64) {
-      if (T == N / (2 * PA)) return launch_chan_small<N, 2 * PA>(d_streams, job, window, tw_n, tw_2n, s);
+      if (T == N / (2 * PA)) return launch_chan_small<N, 2 * PA>(d_streams, job, window, tw_split, s);
     }
     return hipErrorInvalidValue;
   }
@@ -1746,34 +1755,34 @@ std::vector<int> mfp_wgs_candidates() { return {1024, 512, 2048}; }
 
 template <int N, int T>
 static hipError_t launch_fe_n(const DevStream *d_streams, const DevJob &job, const float *window,
-                              const float2 *tw_nd, const float2 *tw_n, const float2 *tw_2n, hipStream_t s) {
+                              const float2 *tw_nd, const float2 *tw_split, hipStream_t s) {
   // about 1024 workgroups in flight: two per CU
   int per_stream = (1024 + job.n_streams - 1) / job.n_streams;
   int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
   if (segs < 1) segs = 1;
   const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
   hipLaunchKernelGGL((k_fengine<N, T>), dim3(gx, job.n_streams), dim3(T), 0, s, d_streams, job, window, tw_nd,
-                     tw_n, tw_2n, (int)segs);
+                     tw_split, (int)segs);
   return hipGetLastError();
 }
 
 bool fengine_supported(int nc) { return nc >= 256 && nc <= 4096; }
 
 hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window, const float2 *tw_nd,
-                          const float2 *tw_n, const float2 *tw_2n, const Tune &t, hipStream_t s) {
+                          const float2 *tw_split, const Tune &t, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
   if (job.nd != 256) return hipErrorInvalidValue;
   const int T = t.fe_threads > 0 ? t.fe_threads : fe_thread_candidates(job.nc)[0];
   switch (job.nc * 1024 + T) {
-    case 256 * 1024 + 64: return launch_fe_n<256, 64>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 512 * 1024 + 128: return launch_fe_n<512, 128>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 512 * 1024 + 64: return launch_fe_n<512, 64>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 1024 * 1024 + 256: return launch_fe_n<1024, 256>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 1024 * 1024 + 128: return launch_fe_n<1024, 128>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 2048 * 1024 + 512: return launch_fe_n<2048, 512>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 2048 * 1024 + 256: return launch_fe_n<2048, 256>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 4096 * 1024 + 512: return launch_fe_n<4096, 512>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
-    case 4096 * 1024 + 256: return launch_fe_n<4096, 256>(d_streams, job, window, tw_nd, tw_n, tw_2n, s);
+    case 256 * 1024 + 64: return launch_fe_n<256, 64>(d_streams, job, window, tw_nd, tw_split, s);
+    case 512 * 1024 + 128: return launch_fe_n<512, 128>(d_streams, job, window, tw_nd, tw_split, s);
+    case 512 * 1024 + 64: return launch_fe_n<512, 64>(d_streams, job, window, tw_nd, tw_split, s);
+    case 1024 * 1024 + 256: return launch_fe_n<1024, 256>(d_streams, job, window, tw_nd, tw_split, s);
+    case 1024 * 1024 + 128: return launch_fe_n<1024, 128>(d_streams, job, window, tw_nd, tw_split, s);
+    case 2048 * 1024 + 512: return launch_fe_n<2048, 512>(d_streams, job, window, tw_nd, tw_split, s);
+    case 2048 * 1024 + 256: return launch_fe_n<2048, 256>(d_streams, job, window, tw_nd, tw_split, s);
+    case 4096 * 1024 + 512: return launch_fe_n<4096, 512>(d_streams, job, window, tw_nd, tw_split, s);
+    case 4096 * 1024 + 256: return launch_fe_n<4096, 256>(d_streams, job, window, tw_nd, tw_split, s);
     default: return hipErrorInvalidValue;
   }
 }
