@@ -29,6 +29,11 @@ for _p in (ROOT, os.path.join(ROOT, "sfxc-1_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the main workload's dominant kernel, from the
+# FETCH_SIZE and WRITE_SIZE counters scaled by a 1 GiB copy kernel's readings
+# (profiles/fengine_xstep_plain_fp32_8st_8sb_4096ch.md); re-measured whenever
+# the kernel changes.
+MEASURED_TRAFFIC = {("fengine", 4096, 16): 8.70e9}
 FP16_DENSE_PEAK_TFLOPS = 2500.0
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "split_fp16": 3}
@@ -400,8 +405,12 @@ def main():
                 "bytes_per_launch": main_r["_kb"][dom_name], "launch_ms": round(float(kt[dom]), 3)}
         if peaks:
             roof["frac_of_achievable"] = round(achieved / peaks["hbm_copy_GBs"], 4)
+        key = (dom_name, wl.job.fft_size_correlation, wl.M)
         if args.traffic_bytes:
             roof["traffic"] = args.traffic_bytes
+        elif key in MEASURED_TRAFFIC and NSB == 8:
+            roof["traffic"] = MEASURED_TRAFFIC[key]
+            roof["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"
         line.update({
             "metric": "station-Gsamples/s correlated (plain FP32 GPU path)",
             "value": main_r["value"],
