@@ -273,7 +273,7 @@ int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n, int32_t reset);
  * rounding of the FFT schedule. */
 int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap);
 /* The tuning table as text, one "shape|fe_threads|chan_wgs|xs_waves|mf_channels|
- * mfp_wgs" line per shape (0: default); *written = bytes needed including the
+ * mfp_wgs|mf_ks" line per shape (0: default; a line without the last field is read as mf_ks 0); *written = bytes needed including the
  * terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear empties
  * the table. */
 int sfxg_tune_export(char *dst, size_t cap, size_t *written);

@@ -1470,8 +1470,8 @@ int sfxg_tune_export(char *dst, size_t cap, size_t *written) {
     char line[256];
     for (const auto &kv : g_tune) {
       const Tune &t = kv.second;
-      snprintf(line, sizeof(line), "%s|%d|%d|%d|%d|%d\n", kv.first.c_str(), t.fe_threads, t.chan_wgs, t.xs_waves,
-               t.mf_channels, t.mfp_wgs);
+      snprintf(line, sizeof(line), "%s|%d|%d|%d|%d|%d|%d\n", kv.first.c_str(), t.fe_threads, t.chan_wgs,
+               t.xs_waves, t.mf_channels, t.mfp_wgs, t.mf_ks);
       out += line;
     }
   }
@@ -1493,9 +1493,9 @@ int sfxg_tune_import(const char *text) {
     const size_t bar = ln.find('|');
     if (bar == std::string::npos) return SFXG_ERR_ARG;
     Tune t;
-    if (sscanf(ln.c_str() + bar + 1, "%d|%d|%d|%d|%d", &t.fe_threads, &t.chan_wgs, &t.xs_waves, &t.mf_channels,
-               &t.mfp_wgs) != 5)
-      return SFXG_ERR_ARG;
+    const int nf = sscanf(ln.c_str() + bar + 1, "%d|%d|%d|%d|%d|%d", &t.fe_threads, &t.chan_wgs, &t.xs_waves,
+                          &t.mf_channels, &t.mfp_wgs, &t.mf_ks);
+    if (nf != 5 && nf != 6) return SFXG_ERR_ARG;
     add[ln.substr(0, bar)] = t;
   }
   std::lock_guard<std::mutex> g(g_tune_mu);

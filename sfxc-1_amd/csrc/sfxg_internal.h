@@ -167,6 +167,7 @@ struct Tune {
   int xs_waves = 0;     // plain X-step: 0 register tiles, else the per-baseline kernel with
                         // this many waves per workgroup
   int mf_channels = 0;  // MFMA X-step: at most this many channels per workgroup
+  int mf_ks = 0;        // MFMA X-step: segments per LDS stage (16 or 32)
   int mfp_wgs = 0;      // MFMA pulsar X-step: workgroups in all
 };
 // Candidates of each knob for a shape (the default first).

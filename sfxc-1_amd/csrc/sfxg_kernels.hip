@@ -15,6 +15,9 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "../../include/sfxg.h"
 #include "sfxg_internal.h"
 
@@ -1131,18 +1134,18 @@ __device__ __forceinline__ void mf_pair(int p, int nT, int RT, int &c, int &t, i
 // into the LDS planes as Y[c][r][k]:
 // consecutive threads take consecutive channels, each thread 8 segments of
 // one stream, converted to the MFMA operand type and stored 16 bytes at a time.
-template <int PREC>
+template <int PREC, int KS = MF_KS>
 __device__ __forceinline__ void mf_load_stage(const float2 *__restrict__ spectra, int64_t stream_stride,
                                               int64_t seg_stride, int64_t seg_base, int n_ch, int M, int CH,
                                               int ch0, int64_t s0,
                                               int64_t s_end, float scale, int chan_stride, int plane_elems,
                                               _Float16 *ylds) {
-  const int n_items = CH * (MF_KS / 8) * M;
+  const int n_items = CH * (KS / 8) * M;
   for (int item = threadIdx.x; item < n_items; item += MF_THREADS) {
     const int c = item % CH;
     const int rest = item / CH;
-    const int kg = rest % (MF_KS / 8);
-    const int st = rest / (MF_KS / 8);
+    const int kg = rest % (KS / 8);
+    const int st = rest / (KS / 8);
     float re[8], im[8];
 #pragma unroll
     for (int kk = 0; kk < 8; kk++) {
@@ -1152,8 +1155,8 @@ __device__ __forceinline__ void mf_load_stage(const float2 *__restrict__ spectra
       re[kk] = v.x * scale;
       im[kk] = v.y * scale;
     }
-    const int o_re = c * chan_stride + st * MF_RS + kg * 8;
-    const int o_im = c * chan_stride + (M + st) * MF_RS + kg * 8;
+    const int o_re = c * chan_stride + st * (KS + 8) + kg * 8;
+    const int o_im = c * chan_stride + (M + st) * (KS + 8) + kg * 8;
     if (PREC == SFXG_PREC_BF16) {
       b8_t br, bi;
 #pragma unroll
@@ -1223,7 +1226,7 @@ __device__ __forceinline__ void mf_mma(f16v_t &acc, const _Float16 *ylds, int oa
   }
 }
 
-template <int PREC>  // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16
+template <int PREC, int KS>  // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16; segments per stage
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
     const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch, int M,
     int RT, int CH, const int32_t *__restrict__ fold_start, float scale, const int32_t *__restrict__ bl_first,
@@ -1236,7 +1239,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
   const int ch0 = blockIdx.x * CH;
   const int z = blockIdx.z;
   const int64_t s_begin = fold_start[z], s_end = fold_start[z + 1];
-  const int chan_stride = R * MF_RS + 8;  // halves, skewed by 16 bytes per channel
+  const int chan_stride = R * (KS + 8) + 8;  // halves, skewed by 16 bytes per channel
   const int plane_elems = CH * chan_stride;
   for (int i = threadIdx.x; i < planes * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
   f16v_t acc[MF_MAXP];
@@ -1246,9 +1249,9 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
     for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
   __syncthreads();
   const int n_pairs = CH * nT;
-  for (int64_t s0 = s_begin; s0 < s_end; s0 += MF_KS) {
-    mf_load_stage<PREC>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
-                        chan_stride, plane_elems, ylds);
+  for (int64_t s0 = s_begin; s0 < s_end; s0 += KS) {
+    mf_load_stage<PREC, KS>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
+                            chan_stride, plane_elems, ylds);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MF_MAXP; q++) {
@@ -1257,9 +1260,10 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
         int c, t, ti, tj;
         mf_pair(p, nT, RT, c, t, ti, tj);
         const int ko = 8 * (lane >> 5);
-        const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * MF_RS + ko;
-        const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * MF_RS + ko;
-        mf_mma<PREC>(acc[q], ylds, oa, ob, plane_elems, 0xffu);
+        const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * (KS + 8) + ko;
+        const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * (KS + 8) + ko;
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 16) mf_mma<PREC>(acc[q], ylds, oa + kk, ob + kk, plane_elems, 0xffu);
       }
     }
     __syncthreads();
@@ -1861,6 +1865,19 @@ hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64
   return hipGetLastError();
 }
 
+template <int PREC, int KS>
+static hipError_t launch_mfma_ks(dim3 grid, size_t lds, hipStream_t s, const float2 *spectra, int64_t stream_stride,
+                                 int64_t seg_stride, int64_t seg_base, int n_ch, int M, int RT, int CH,
+                                 const int32_t *fold_start, float scale, const int32_t *bl_first,
+                                 const int32_t *bl_second, int n_baselines, float inv2, float2 *partial) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma<PREC, KS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_xstep_mfma<PREC, KS>), grid, dim3(MF_THREADS), lds, s, spectra, stream_stride, seg_stride,
+                     seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial);
+  return hipGetLastError();
+}
+
 hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
                              int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
@@ -1870,52 +1887,44 @@ hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t strea
   const int R = 32 * RT;
   const int nT = RT * (RT + 1) / 2;
   const int planes = precision == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
+  if (nT > MF_WAVES * MF_MAXP) return hipErrorInvalidValue;
   // channels per workgroup: all (channel, tile) pairs fit MF_WAVES waves x
-  // MF_MAXP and the stage fits 150 KiB of LDS
-  int CH = (MF_WAVES * MF_MAXP) / nT;
-  if (CH > 64) CH = 64;
-  auto lds_of = [&](int ch) {
-    const size_t stage = (size_t)planes * ch * (R * MF_RS + 8) * 2;
+  // MF_MAXP and the stage fits 150 KiB of LDS. Stages of 32 segments (two
+  // MFMA K steps per barrier) where that keeps the channel count, else 16.
+  const int ch_pairs = std::min(64, (MF_WAVES * MF_MAXP) / nT);
+  auto lds_of = [&](int ch, int ks) {
+    const size_t stage = (size_t)planes * ch * (R * (ks + 8) + 8) * 2;
     const size_t gbuf = (size_t)R * R * 4;
     return stage > gbuf ? stage : gbuf;
   };
-  while (CH > 1 && lds_of(CH) > 150 * 1024) CH--;
+  auto fit = [&](int ks) {
+    int ch = ch_pairs;
+    while (ch > 1 && lds_of(ch, ks) > 150 * 1024) ch--;
+    return ch;
+  };
+  int KS = t.mf_ks > 0 ? t.mf_ks : (fit(32) >= fit(16) ? 32 : 16);
+  int CH = fit(KS);
   if (t.mf_channels > 0 && CH > t.mf_channels) CH = t.mf_channels;
-  if (CH < 1 || nT > MF_WAVES * MF_MAXP) return hipErrorInvalidValue;
-  const size_t lds = lds_of(CH);
+  if (CH < 1) return hipErrorInvalidValue;
+  const size_t lds = lds_of(CH, KS);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   dim3 grid((unsigned)((n_ch + CH - 1) / CH), 1, (unsigned)n_k);
   const float inv2 = 1.0f / (scale * scale);
-  hipError_t e;
+  hipError_t e = hipErrorInvalidValue;
+#define SFXG_MFMA_CASE(P)                                                                                        \
+  case P:                                                                                                        \
+    e = KS == 32 ? launch_mfma_ks<P, 32>(grid, lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch, M, RT, \
+                                         CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial) \
+                 : launch_mfma_ks<P, 16>(grid, lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch, M, RT, \
+                                         CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial); \
+    break;
   switch (precision) {
-    case SFXG_PREC_FP16:
-      e = hipFuncSetAttribute((const void *)k_xstep_mfma<SFXG_PREC_FP16>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_FP16>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride,
-                         seg_stride, seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second,
-                         n_baselines, inv2, partial);
-      break;
-    case SFXG_PREC_BF16:
-      e = hipFuncSetAttribute((const void *)k_xstep_mfma<SFXG_PREC_BF16>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_BF16>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride,
-                         seg_stride, seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second,
-                         n_baselines, inv2, partial);
-      break;
-    case SFXG_PREC_SPLIT_FP16:
-      e = hipFuncSetAttribute((const void *)k_xstep_mfma<SFXG_PREC_SPLIT_FP16>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(k_xstep_mfma<SFXG_PREC_SPLIT_FP16>, grid, dim3(MF_THREADS), lds, s, spectra,
-                         stream_stride, seg_stride, seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first,
-                         bl_second, n_baselines, inv2, partial);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    SFXG_MFMA_CASE(SFXG_PREC_FP16)
+    SFXG_MFMA_CASE(SFXG_PREC_BF16)
+    SFXG_MFMA_CASE(SFXG_PREC_SPLIT_FP16)
+    default: return hipErrorInvalidValue;
   }
-  e = hipGetLastError();
+#undef SFXG_MFMA_CASE
   if (e != hipSuccess) return e;
   const int64_t n = (int64_t)n_baselines * n_ch;
   hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
