@@ -43,6 +43,31 @@ __device__ __forceinline__ float2 twiddle(const float2 *tw, int idx) {
   return SIGN < 0 ? w : make_float2(w.x, -w.y);
 }
 
+// DevStream holds generic pointers to global memory. Reaching them through
+// these casts lets the compiler emit global loads and stores (scalar loads
+// for wave-uniform addresses of read-only data) instead of flat ones, which
+// also count against the LDS wait counter and would make every LDS wait
+// drain the memory loads in flight.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *cmem(const T *p) {
+  return (const __attribute__((address_space(4))) T *)p;
+}
+
+// float2 loads and stores in global memory through the same cast (HIP's
+// float2 has no members for address-space-qualified objects).
+typedef float vf2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gload2(const float2 *p) {
+  const vf2_t v = *(const __attribute__((address_space(1))) vf2_t *)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore2(float2 *p, float2 v) {
+  *(__attribute__((address_space(1))) vf2_t *)p = vf2_t{v.x, v.y};
+}
+
 // 4-point DFT in place, exp(SIGN 2 pi i jk/4)
 template <int SIGN>
 __device__ __forceinline__ void radix4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
@@ -71,19 +96,20 @@ __device__ double eval_model(const DevModel &m, const double *coef, int64_t tick
 
 __device__ __forceinline__ float decode_sample(const DevStream &s, int64_t psamp) {
   if (s.bits == 2) {
-    const uint32_t b = s.payload[psamp >> 2];
+    const uint32_t b = gmem(s.payload)[psamp >> 2];
     return s.lut[(b >> ((psamp & 3) * 2)) & 3];
   }
-  const uint32_t b = s.payload[psamp >> 3];
+  const uint32_t b = gmem(s.payload)[psamp >> 3];
   return s.lut[(b >> (psamp & 7)) & 1];
 }
 
 // Output sample o of a stream; r is a run index hint that only moves forward.
 __device__ __forceinline__ float sample_at(const DevStream &s, int64_t o, int64_t &r) {
-  while (r + 1 < s.n_runs && s.runs[r + 1].out_start <= o) r++;
-  const Run run = s.runs[r];
-  if (run.src < 0) return 0.0f;
-  return decode_sample(s, run.src + (o - run.out_start));
+  const auto *runs = gmem(s.runs);
+  while (r + 1 < s.n_runs && runs[r + 1].out_start <= o) r++;
+  const int64_t src = runs[r].src;
+  if (src < 0) return 0.0f;
+  return decode_sample(s, src + (o - runs[r].out_start));
 }
 
 }  // namespace
@@ -170,33 +196,69 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One Stockham radix-4 stage over 256 points for one wavefront: v[r] holds
-// data[lane + 64 r] on entry; on exit buf holds the stage output.
-template <int SIGN>
-__device__ __forceinline__ void stage256(float2 (&v)[4], float2 *buf, const float2 *tw, int lane, int ns) {
-  const int k = lane % ns;
-  const int step = 256 / (ns * 4);
-  v[1] = cmul(v[1], twiddle<SIGN>(tw, k * 1 * step));
-  v[2] = cmul(v[2], twiddle<SIGN>(tw, k * 2 * step));
-  v[3] = cmul(v[3], twiddle<SIGN>(tw, k * 3 * step));
-  radix4<SIGN>(v[0], v[1], v[2], v[3]);
-  const int idx = (lane / ns) * ns * 4 + k;
-  wave_lds_sync();
-  buf[idx] = v[0];
-  buf[idx + ns] = v[1];
-  buf[idx + 2 * ns] = v[2];
-  buf[idx + 3 * ns] = v[3];
-  wave_lds_sync();
+// Wave FFT256: one wavefront transforms 256 points, lane holding points
+// lane + 64 r (r < 4) in registers, in four radix-4 Stockham stages of span
+// ns = 1, 4, 16, 64. Each stage but the last exchanges its outputs through
+// the wave's LDS scratch; the last stage's outputs lane + 64 r are already
+// where the next step wants them and stay in registers.
+//
+// Stage twiddles come from LDS tables gathered from exp(-2 pi i j / 256):
+// for span ns (4, 16, 64) and r = 1..3 the entry tw256_base(ns) +
+// (r - 1) ns + k holds exp(-2 pi i k r / (4 ns)), so the lanes of a stage
+// (k = lane mod ns) read consecutive or identical entries. The first stage's
+// twiddles are all 1 and are not applied.
+constexpr int TW256_ELEMS = 3 * (4 + 16 + 64);
+constexpr int WAVE_FFT256_LDS = 320;  // float2 elements of one wave's scratch
+
+__device__ __forceinline__ constexpr int tw256_base(int ns) { return ns == 4 ? 0 : ns == 16 ? 12 : 60; }
+
+// Fills the per-stage twiddle tables from the 256-entry table tw_g.
+__device__ __forceinline__ void fill_tw256(float2 *tw, const float2 *__restrict__ tw_g, int tid, int nthreads) {
+  for (int i = tid; i < TW256_ELEMS; i += nthreads) {
+    const int ns = i < 12 ? 4 : i < 60 ? 16 : 64;
+    const int j = i - tw256_base(ns);
+    tw[i] = tw_g[(j % ns) * (j / ns + 1) * (64 / ns)];
+  }
 }
 
+// LDS position of element i written by the stage of span NS. The first two
+// stages write with strides of 4 and 16 elements; padding by one element
+// (NS = 1) or four (NS = 4) every 16 puts the 16 lanes of each 64-bit access
+// on distinct bank pairs, while the next stage's reads of lane + 64 r stay
+// consecutive within every 16 lanes. The third stage writes runs of 16 and
+// needs none.
+template <int NS>
+__device__ __forceinline__ int pad256(int i) {
+  return NS == 1 ? i + (i >> 4) : NS == 4 ? i + 4 * (i >> 4) : i;
+}
+
+template <int SIGN, int NS>
+__device__ __forceinline__ void wstage256(float2 (&v)[4], float2 *buf, const float2 *tw, int lane) {
+  const int k = lane % NS;
+  if (NS > 1) {
+    const float2 *t = tw + tw256_base(NS) + k;
+    v[1] = cmul(v[1], twiddle<SIGN>(t, 0));
+    v[2] = cmul(v[2], twiddle<SIGN>(t, NS));
+    v[3] = cmul(v[3], twiddle<SIGN>(t, 2 * NS));
+  }
+  radix4<SIGN>(v[0], v[1], v[2], v[3]);
+  if (NS == 64) return;
+  const int idx = (lane / NS) * NS * 4 + k;
+  wave_lds_sync();
+#pragma unroll
+  for (int r = 0; r < 4; r++) buf[pad256<NS>(idx + r * NS)] = v[r];
+  wave_lds_sync();
+#pragma unroll
+  for (int r = 0; r < 4; r++) v[r] = buf[pad256<NS>(lane + 64 * r)];
+}
+
+// v[r] = data[lane + 64 r] on entry and X[lane + 64 r] on exit.
 template <int SIGN>
 __device__ __forceinline__ void fft256(float2 (&v)[4], float2 *buf, const float2 *tw, int lane) {
-#pragma unroll
-  for (int ns = 1; ns < 256; ns *= 4) {
-    stage256<SIGN>(v, buf, tw, lane, ns);
-#pragma unroll
-    for (int r = 0; r < 4; r++) v[r] = buf[lane + 64 * r];
-  }
+  wstage256<SIGN, 1>(v, buf, tw, lane);
+  wstage256<SIGN, 4>(v, buf, tw, lane);
+  wstage256<SIGN, 16>(v, buf, tw, lane);
+  wstage256<SIGN, 64>(v, buf, tw, lane);
 }
 
 }  // namespace
@@ -261,8 +323,18 @@ __global__ __launch_bounds__(256) void k_block_consts(const DevStream *__restric
   s.block_consts[blk] = bc;
 }
 
-// The 256 samples of block blk, lane holding samples lane + 64 q (zeros
-// for a block past the end).
+// Decode table entry of a 2-bit code from registers.
+__device__ __forceinline__ float lut4(uint32_t c, float l0, float l1, float l2, float l3) {
+  const float lo = (c & 1) ? l1 : l0, hi = (c & 1) ? l3 : l2;
+  return (c & 2) ? hi : lo;
+}
+
+// The 256 samples of block blk (wave-uniform), lane holding samples
+// lane + 64 q (zeros for a block past the end). A block inside one payload
+// run, the common case, takes its four payload bytes per lane with loads
+// from one address (offsets 16 q bytes at 2 bit, 8 q at 1 bit) and decodes
+// them from the table in registers; a block that crosses a run boundary or
+// invalid data goes sample by sample.
 __device__ __forceinline__ void load_block(const DevStream &s, int64_t blk, int64_t n_blocks, int lane,
                                            float (&x)[4]) {
   constexpr int nd = 256;
@@ -272,13 +344,32 @@ __device__ __forceinline__ void load_block(const DevStream &s, int64_t blk, int6
     return;
   }
   const int64_t o0 = blk * nd;
-  int64_t r = s.block_run[blk];
-  const Run run = s.runs[r];
-  if (run.src >= 0 && run.out_start + run.count >= o0 + nd) {
-    // the whole block comes from one run: consecutive payload samples
-    const int64_t p0 = run.src + (o0 - run.out_start);
+  int64_t r = cmem(s.block_run)[blk];
+  const auto *run = cmem(s.runs) + r;
+  const int64_t out_start = run->out_start, src = run->src, count = run->count;
+  if (src >= 0 && out_start + count >= o0 + nd) {
+    const int64_t p0 = src + (o0 - out_start);
+    const auto *pay = gmem(s.payload);
+    uint32_t c[4];
+    if (s.bits == 2) {
+      const int u = (int)(p0 & 3) + lane;
+      const auto *b = pay + (p0 >> 2) + (u >> 2);
 #pragma unroll
-    for (int q = 0; q < 4; q++) x[q] = decode_sample(s, p0 + lane + 64 * q);
+      for (int q = 0; q < 4; q++) c[q] = b[16 * q];
+      const int sh = (u & 3) * 2;
+      const float l0 = s.lut[0], l1 = s.lut[1], l2 = s.lut[2], l3 = s.lut[3];
+#pragma unroll
+      for (int q = 0; q < 4; q++) x[q] = lut4((c[q] >> sh) & 3, l0, l1, l2, l3);
+    } else {
+      const int u = (int)(p0 & 7) + lane;
+      const auto *b = pay + (p0 >> 3) + (u >> 3);
+#pragma unroll
+      for (int q = 0; q < 4; q++) c[q] = b[8 * q];
+      const int sh = u & 7;
+      const float l0 = s.lut[0], l1 = s.lut[1];
+#pragma unroll
+      for (int q = 0; q < 4; q++) x[q] = ((c[q] >> sh) & 1) ? l1 : l0;
+    }
   } else {
 #pragma unroll
     for (int q = 0; q < 4; q++) x[q] = sample_at(s, o0 + lane + 64 * q, r);
@@ -290,10 +381,24 @@ __device__ __forceinline__ void load_block(const DevStream &s, int64_t blk, int6
 // negative frequencies, apply the ramp exp(-i (c + k l)) (one sincos per
 // lane, then rotations by exp(-i 64 l)), transform back, fringe-rotate by
 // phi + i dphi (the same sincos-and-rotate scheme) and write 256 samples.
+template <class OutT>
 __device__ __forceinline__ void finish_block(const DevStream &s, int64_t blk, float2 (&v)[4], float2 *buf,
-                                             const float2 *tw, int lane, float *out) {
+                                             const float2 *tw, int lane, OutT *out) {
   constexpr int nd = 256;
-  const BlockConst bc = s.block_consts[blk];
+  const auto *bcp = cmem(s.block_consts) + blk;
+  struct {
+    double c_term, l_term, phi, dphi;
+    float amp, ramp_step[2], fringe_step[2];
+  } bc;
+  bc.c_term = bcp->c_term;
+  bc.l_term = bcp->l_term;
+  bc.phi = bcp->phi;
+  bc.dphi = bcp->dphi;
+  bc.amp = bcp->amp;
+  bc.ramp_step[0] = bcp->ramp_step[0];
+  bc.ramp_step[1] = bcp->ramp_step[1];
+  bc.fringe_step[0] = bcp->fringe_step[0];
+  bc.fringe_step[1] = bcp->fringe_step[1];
   {
     float sn, cs;
     sincosf((float)(bc.c_term + lane * bc.l_term), &sn, &cs);
@@ -329,15 +434,20 @@ __device__ __forceinline__ void finish_block(const DevStream &s, int64_t blk, fl
 // real blocks share one complex forward FFT, z = a + i b, and are separated
 // by A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i. `buf`
 // is the wave's own LDS region; outb may be null when blk + 1 does not exist.
+template <class OutT>
 __device__ __forceinline__ void delay_block_pair(const DevStream &s, int64_t blk, int64_t n_blocks, float2 *buf,
-                                                 const float2 *tw, int lane, float *outa, float *outb) {
+                                                 const float2 *tw, int lane, OutT *outa, OutT *outb) {
   float xa[4], xb[4];
   load_block(s, blk, n_blocks, lane, xa);
   load_block(s, blk + 1, outb ? n_blocks : 0, lane, xb);
   float2 v[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) v[q] = make_float2(xa[q], xb[q]);
-  fft256<-1>(v, buf, tw, lane);  // buf keeps Z in natural order
+  fft256<-1>(v, buf, tw, lane);
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < 4; q++) buf[lane + 64 * q] = v[q];  // Z in natural order for Z[-k]
+  wave_lds_sync();
   float2 va[4], vb[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -350,22 +460,23 @@ __device__ __forceinline__ void delay_block_pair(const DevStream &s, int64_t blk
   if (outb) finish_block(s, blk + 1, vb, buf, tw, lane, outb);
 }
 
-// k_delay: one wavefront per pair of blocks, the real series to HBM
-// (unfused path, used for N_c above the fused F-engine's LDS budget and for
-// verification).
+// k_delay: each wavefront takes pairs of blocks in turn (grid-stride over
+// the chunk's blocks), the real series to HBM (unfused path, used for N_c
+// above the fused F-engine's LDS budget and for verification).
 __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ streams, DevJob job,
                                                const float2 *__restrict__ tw_g) {
-  __shared__ float2 lds[4][256];
-  __shared__ float2 tw[256];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  tw[threadIdx.x] = tw_g[threadIdx.x];
+  __shared__ float2 lds[4][WAVE_FFT256_LDS];
+  __shared__ float2 tw[TW256_ELEMS];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  fill_tw256(tw, tw_g, threadIdx.x, 256);
   __syncthreads();
   const DevStream &s = streams[blockIdx.y];
-  const int64_t blk = job.blk_base + ((int64_t)blockIdx.x * 4 + wave) * 2;
   const int64_t end = job.blk_end < job.n_blocks ? job.blk_end : job.n_blocks;
-  if (blk >= end) return;
-  float *base = s.time_series + (blk - job.blk_base) * 256;
-  delay_block_pair(s, blk, job.n_blocks, lds[wave], tw, lane, base, blk + 1 < end ? base + 256 : nullptr);
+  for (int64_t blk = job.blk_base + ((int64_t)blockIdx.x * 4 + wave) * 2; blk < end;
+       blk += (int64_t)gridDim.x * 8) {
+    auto *base = gmem(s.time_series) + (blk - job.blk_base) * 256;
+    delay_block_pair(s, blk, job.n_blocks, lds[wave], tw, lane, base, blk + 1 < end ? base + 256 : nullptr);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -556,7 +667,7 @@ __device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, floa
     const float2 zk = lds[lds_index<P>(k & (N - 1))], zn = lds[lds_index<P>((N - k) & (N - 1))];
     const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
     const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
-    out[k] = cadd(e, cmul(tw.split(k), o));
+    gstore2(out + k, cadd(e, cmul(tw.split(k), o)));
     __builtin_amdgcn_sched_barrier(0);  // one bin's loads in flight at a time
   }
 }
@@ -593,7 +704,7 @@ __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restric
     if (none && m >= N / 2) {
       z = make_float2(0.0f, 0.0f);
     } else {
-      const float2 xv = x[m], wv = w[m];
+      const float2 xv = gload2(x + m), wv = w[m];
       z = make_float2(xv.x * wv.x, xv.y * wv.y);
     }
     if (s.flip) z.y = -z.y;
@@ -639,8 +750,9 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
   float2 lo[H], hi[H];
 #pragma unroll
   for (int i = 0; i < H; i++) {
-    lo[i] = ts[sa * (N / 2) + threadIdx.x + i * T];
-    hi[i] = none ? make_float2(0.0f, 0.0f) : ts[sa * (N / 2) + N / 2 + threadIdx.x + i * T];
+    lo[i] = gload2(ts + sa * (N / 2) + threadIdx.x + i * T);
+    hi[i] = make_float2(0.0f, 0.0f);
+    if (!none) hi[i] = gload2(ts + sa * (N / 2) + N / 2 + threadIdx.x + i * T);
   }
   __syncthreads();  // twiddle tables
   for (int64_t seg = sa; seg < sb; seg++) {
@@ -664,7 +776,7 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
     if (more) {
       const int64_t base = (seg + 1) * (N / 2) + (none ? 0 : N / 2);
 #pragma unroll
-      for (int i = 0; i < H; i++) nx[i] = ts[base + tid + i * T];
+      for (int i = 0; i < H; i++) nx[i] = gload2(ts + base + tid + i * T);
     }
     reg_fft<N, P>(v, lds, tw, tid);
     real_split<N, P, T>(lds, tw, s.spectra + seg * job.spectra_stride, tid);
@@ -722,7 +834,7 @@ __global__ __launch_bounds__(256) void k_chan_cols(const DevStream *__restrict__
     const int m = LG_N2 * (tid + i * T1) + n2;
     float2 z = make_float2(0.0f, 0.0f);
     if (!(none && m >= LG_N / 2)) {
-      const float2 xv = x[m], wv = w[m];
+      const float2 xv = gload2(x + m), wv = w[m];
       z = make_float2(xv.x * wv.x, xv.y * wv.y);
     }
     if (s.flip) z.y = -z.y;
@@ -770,7 +882,7 @@ __global__ __launch_bounds__(256) void k_chan_split(const DevStream *__restrict_
     const float2 zk = z[k & (LG_N - 1)], zn = z[(LG_N - k) & (LG_N - 1)];
     const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
     const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
-    out[k] = cadd(e, cmul(tw_2n[k], o));
+    gstore2(out + k, cadd(e, cmul(tw_2n[k], o)));
   }
 }
 
@@ -789,21 +901,22 @@ template <int N, int T>
 __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ streams, DevJob job,
                                                const float *__restrict__ window, const float2 *__restrict__ tw_nd,
                                                const float2 *__restrict__ tw_split, int segs_per_wg) {
-  static_assert(N >= 256 && (T / 64) * 256 <= N, "wave scratch must fit in the FFT buffer");
+  static_assert(N >= 256 && (T / 64) * WAVE_FFT256_LDS <= fft_lds_size<N, N / T>(),
+                "wave scratch must fit in the FFT buffer");
   constexpr int P = N / T, NHI = 2 * N / 128;
   __shared__ float ring[2 * N];
   __shared__ float2 fbuf[fft_lds_size<N, P>()];
-  __shared__ float2 tw[256];
+  __shared__ float2 tw[TW256_ELEMS];
   __shared__ float2 thi[NHI], tlo[128];
   const DevStream &s = streams[blockIdx.y];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   constexpr int nwaves = T / 64;
   constexpr int BPH = N / 256;  // delay blocks per half
   const int64_t sa = (int64_t)blockIdx.x * segs_per_wg;
   int64_t sb = sa + segs_per_wg;
   if (sb > job.n_segments) sb = job.n_segments;
   if (sa >= sb) return;  // uniform over the workgroup
-  for (int i = threadIdx.x; i < 256; i += T) tw[i] = tw_nd[i];
+  fill_tw256(tw, tw_nd, threadIdx.x, T);
   for (int i = threadIdx.x; i < NHI + 128; i += T) {
     const float2 t = tw_split[i];
     if (i < NHI) thi[i] = t; else tlo[i - NHI] = t;
@@ -811,7 +924,7 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
   __syncthreads();
   const LdsTw twl{thi, tlo};
   const bool none = job.window == SFXG_WINDOW_NONE;
-  float2 *scratch = fbuf + wave * 256;
+  float2 *scratch = fbuf + wave * WAVE_FFT256_LDS;
   // blocks go in pairs (blk, blk + 1) with blk even, as in k_delay
   if (!none) {
     for (int b = 2 * wave; b < BPH; b += 2 * nwaves)
@@ -1648,7 +1761,10 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
   if (job.nd != 256) return hipErrorInvalidValue;
   const int64_t end = job.blk_end < job.n_blocks ? job.blk_end : job.n_blocks;
   if (end <= job.blk_base) return hipSuccess;
-  dim3 grid((unsigned)((end - job.blk_base + 7) / 8), job.n_streams);
+  // about 8192 workgroups over all streams; each wave then loops over pairs
+  const int64_t wgs = (end - job.blk_base + 7) / 8;
+  const int64_t per_stream = std::max<int64_t>(1, (8192 + job.n_streams - 1) / job.n_streams);
+  dim3 grid((unsigned)std::min(wgs, per_stream), job.n_streams);
   hipLaunchKernelGGL(k_delay, grid, dim3(256), 0, s, d_streams, job, tw_nd);
   return hipGetLastError();
 }
