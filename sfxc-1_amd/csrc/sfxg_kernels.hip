@@ -67,6 +67,17 @@ __device__ __forceinline__ float2 gload2(const float2 *p) {
 __device__ __forceinline__ void gstore2(float2 *p, float2 v) {
   *(__attribute__((address_space(1))) vf2_t *)p = vf2_t{v.x, v.y};
 }
+// base (wave-uniform) + a 32-bit byte offset: the scalar-base form of the
+// global load, one VGPR per address instead of two.
+__device__ __forceinline__ float2 gload2(const float2 *base, uint32_t byte_off) {
+  const auto *p = (const __attribute__((address_space(1))) char *)base + byte_off;
+  const vf2_t v = *(const __attribute__((address_space(1))) vf2_t *)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore2(float2 *base, uint32_t byte_off, float2 v) {
+  auto *p = (__attribute__((address_space(1))) char *)base + byte_off;
+  *(__attribute__((address_space(1))) vf2_t *)p = vf2_t{v.x, v.y};
+}
 
 // 4-point DFT in place, exp(SIGN 2 pi i jk/4)
 template <int SIGN>
@@ -661,15 +672,22 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[P], float2 *lds, const TW &t
 
 // Split of the N-point complex transform of 2N packed reals into bins
 // k = 0..N of the real transform: X_k = E_k + W_2N^k O_k.
+// Thread tid writes bins tid + j T (j < P) and thread 0 also bin N: a fixed
+// number of stores per thread, so that the compiler can wait for loads
+// issued before them without waiting for the stores too.
 template <int N, int P, int T, class TW>
 __device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, float2 *out, int tid) {
-  for (int k = tid; k <= N; k += T) {
+  static_assert(N == P * T, "one bin per point and thread");
+  auto bin = [&](int k) {
     const float2 zk = lds[lds_index<P>(k & (N - 1))], zn = lds[lds_index<P>((N - k) & (N - 1))];
     const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
     const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
-    gstore2(out + k, cadd(e, cmul(tw.split(k), o)));
+    gstore2(out, (uint32_t)k * (uint32_t)sizeof(float2), cadd(e, cmul(tw.split(k), o)));
     __builtin_amdgcn_sched_barrier(0);  // one bin's loads in flight at a time
-  }
+  };
+#pragma unroll 1
+  for (int j = 0; j < P; j++) bin(tid + j * T);
+  if (tid == 0) bin(N);
 }
 
 }  // namespace
@@ -745,53 +763,65 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
   const bool none = job.window == SFXG_WINDOW_NONE;
   const float2 *w2 = reinterpret_cast<const float2 *>(window);
   // complex point m of segment seg is the pair of reals 2m, 2m + 1 at
-  // seg * N reals; the segment's second half is the next one's first half
+  // seg * N reals; the segment's second half is the next one's first half.
+  // A segment's windowed points are formed right after the previous
+  // segment's FFT, before its spectrum stores (loads and stores share one
+  // in-order counter), from the carried raw half and the prefetched new
+  // reals; the loop top then waits for nothing.
   const float2 *ts = reinterpret_cast<const float2 *>(s.time_series);
-  float2 lo[H], hi[H];
+  const bool flip = s.flip;
+  auto win = [flip](float2 x, float2 w) {
+    float2 z = make_float2(x.x * w.x, x.y * w.y);
+    if (flip) z.y = -z.y;
+    return z;
+  };
+  auto off = [](int tid, int i) { return (uint32_t)(tid + i * T) * (uint32_t)sizeof(float2); };
+  float2 lo_w[H], hi_w[H], raw[H];
+  {
+    const float2 *seg0 = ts + sa * (N / 2);
 #pragma unroll
-  for (int i = 0; i < H; i++) {
-    lo[i] = gload2(ts + sa * (N / 2) + threadIdx.x + i * T);
-    hi[i] = make_float2(0.0f, 0.0f);
-    if (!none) hi[i] = gload2(ts + sa * (N / 2) + N / 2 + threadIdx.x + i * T);
+    for (int i = 0; i < H; i++) {
+      lo_w[i] = win(gload2(seg0, off(threadIdx.x, i)), gload2(w2, off(threadIdx.x, i)));
+      hi_w[i] = make_float2(0.0f, 0.0f);
+      if (!none) {
+        raw[i] = gload2(seg0 + N / 2, off(threadIdx.x, i));
+        hi_w[i] = win(raw[i], gload2(w2, off(threadIdx.x, i + H)));
+      }
+    }
   }
   __syncthreads();  // twiddle tables
   for (int64_t seg = sa; seg < sb; seg++) {
     const int tid = opaque(threadIdx.x);
     float2 v[P];
 #pragma unroll
-    for (int i = 0; i < P; i++) {
-      const float2 x = i < H ? lo[i] : hi[i - H];
-      const float2 wv = w2[tid + i * T];
-      float2 z = make_float2(x.x * wv.x, x.y * wv.y);
-      if (s.flip) z.y = -z.y;
-      v[i] = z;
+    for (int i = 0; i < H; i++) {
+      v[i] = lo_w[i];
+      v[i + H] = hi_w[i];
     }
-    if (none) {
-#pragma unroll
-      for (int i = H; i < P; i++) v[i] = make_float2(0.0f, 0.0f);
-    }
-    // prefetch the reals the next segment adds
+    // the reals the next segment adds (this segment's again after the last
+    // one, so that the loads need no branch)
+    const float2 *nseg = ts + (seg + 1 < sb ? seg + 1 : seg) * (N / 2) + (none ? 0 : N / 2);
     float2 nx[H];
-    const bool more = seg + 1 < sb;
-    if (more) {
-      const int64_t base = (seg + 1) * (N / 2) + (none ? 0 : N / 2);
 #pragma unroll
-      for (int i = 0; i < H; i++) nx[i] = gload2(ts + base + tid + i * T);
-    }
+    for (int i = 0; i < H; i++) nx[i] = gload2(nseg, off(tid, i));
     reg_fft<N, P>(v, lds, tw, tid);
+    // branch-free, so that the window loads issue together (without a
+    // window the second half is zero and the first takes the new reals)
+    const int tid2 = opaque(threadIdx.x);
+    float2 wlo[H], whi[H];
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+      wlo[i] = gload2(w2, off(tid2, i));
+      whi[i] = gload2(w2, off(tid2, i + H));
+    }
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+      lo_w[i] = win(none ? nx[i] : raw[i], wlo[i]);
+      hi_w[i] = none ? make_float2(0.0f, 0.0f) : win(nx[i], whi[i]);
+      raw[i] = nx[i];
+    }
     real_split<N, P, T>(lds, tw, s.spectra + seg * job.spectra_stride, tid);
     __syncthreads();
-    if (more) {
-#pragma unroll
-      for (int i = 0; i < H; i++) {
-        if (none) {
-          lo[i] = nx[i];
-        } else {
-          lo[i] = hi[i];
-          hi[i] = nx[i];
-        }
-      }
-    }
   }
 }
 
