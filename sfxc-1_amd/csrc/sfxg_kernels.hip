@@ -601,9 +601,14 @@ struct TableTw {
 struct LdsTw {
   const float2 *hi;
   const float2 *lo;
+  // lo[0] = 1 exactly, so the product is exact where j is a multiple of 128;
+  // it is skipped only where the compiler can prove that, and otherwise
+  // always taken (no per-twiddle select)
   __device__ __forceinline__ float2 at(int j) const {
+    const int l = j & 127;
     const float2 h = hi[j >> 7];
-    return (j & 127) ? cmul(h, lo[j & 127]) : h;
+    if (__builtin_constant_p(l == 0) && l == 0) return h;
+    return cmul(h, lo[l]);
   }
   __device__ __forceinline__ float2 fft(int m) const { return at(2 * m); }
   __device__ __forceinline__ float2 split(int k) const { return at(k); }
@@ -672,22 +677,36 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[P], float2 *lds, const TW &t
 
 // Split of the N-point complex transform of 2N packed reals into bins
 // k = 0..N of the real transform: X_k = E_k + W_2N^k O_k.
-// Thread tid writes bins tid + j T (j < P) and thread 0 also bin N: a fixed
-// number of stores per thread, so that the compiler can wait for loads
-// issued before them without waiting for the stores too.
+// Bins k and N - k come from the same two points: E_{N-k} = conj E_k,
+// O_{N-k} = conj O_k and W_2N^(N-k) = -conj W_2N^k, so
+// X_{N-k} = conj(E_k - W_2N^k O_k). Thread tid takes the pairs
+// k = tid + j T (j < P / 2, so k < N / 2, with k = 0 pairing bins 0 and N)
+// and thread 0 also bin N / 2: two LDS reads and one twiddle per two bins,
+// and a fixed number of stores per thread, so that the compiler can wait for
+// loads issued before them without waiting for the stores too.
 template <int N, int P, int T, class TW>
 __device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, float2 *out, int tid) {
-  static_assert(N == P * T, "one bin per point and thread");
-  auto bin = [&](int k) {
-    const float2 zk = lds[lds_index<P>(k & (N - 1))], zn = lds[lds_index<P>((N - k) & (N - 1))];
-    const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-    const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
-    gstore2(out, (uint32_t)k * (uint32_t)sizeof(float2), cadd(e, cmul(tw.split(k), o)));
-    __builtin_amdgcn_sched_barrier(0);  // one bin's loads in flight at a time
+  static_assert(N == P * T && P % 2 == 0, "pairs of bins per thread");
+  auto eo = [&](int k, float2 &e, float2 &o) {
+    const float2 zk = lds[lds_index<P>(k)], zn = lds[lds_index<P>((N - k) & (N - 1))];
+    e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+    o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
   };
 #pragma unroll 1
-  for (int j = 0; j < P; j++) bin(tid + j * T);
-  if (tid == 0) bin(N);
+  for (int j = 0; j < P / 2; j++) {
+    const int k = tid + j * T;
+    float2 e, o;
+    eo(k, e, o);
+    const float2 wo = cmul(tw.split(k), o);
+    gstore2(out, (uint32_t)k * (uint32_t)sizeof(float2), cadd(e, wo));
+    gstore2(out, (uint32_t)(N - k) * (uint32_t)sizeof(float2), make_float2(e.x - wo.x, -(e.y - wo.y)));
+    __builtin_amdgcn_sched_barrier(0);  // one pair's loads in flight at a time
+  }
+  if (tid == 0) {
+    float2 e, o;
+    eo(N / 2, e, o);
+    gstore2(out, (uint32_t)(N / 2) * (uint32_t)sizeof(float2), cadd(e, cmul(tw.split(N / 2), o)));
+  }
 }
 
 }  // namespace
