@@ -79,6 +79,14 @@ __device__ __forceinline__ void gstore2(float2 *base, uint32_t byte_off, float2 
   *(__attribute__((address_space(1))) vf2_t *)p = vf2_t{v.x, v.y};
 }
 
+// Windowed point x * w, per component and never contracted into the
+// transform's first additions: the reference forms the windowed series
+// before transforming it.
+__device__ __forceinline__ float2 wmul(float2 x, float2 w) {
+#pragma clang fp contract(off)
+  return make_float2(x.x * w.x, x.y * w.y);
+}
+
 // 4-point DFT in place, exp(SIGN 2 pi i jk/4)
 template <int SIGN>
 __device__ __forceinline__ void radix4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
@@ -741,8 +749,7 @@ __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restric
     if (none && m >= N / 2) {
       z = make_float2(0.0f, 0.0f);
     } else {
-      const float2 xv = gload2(x + m), wv = w[m];
-      z = make_float2(xv.x * wv.x, xv.y * wv.y);
+      z = wmul(gload2(x + m), w[m]);
     }
     if (s.flip) z.y = -z.y;
     v[i] = z;
@@ -790,7 +797,7 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
   const float2 *ts = reinterpret_cast<const float2 *>(s.time_series);
   const bool flip = s.flip;
   auto win = [flip](float2 x, float2 w) {
-    float2 z = make_float2(x.x * w.x, x.y * w.y);
+    float2 z = wmul(x, w);
     if (flip) z.y = -z.y;
     return z;
   };
@@ -883,8 +890,7 @@ __global__ __launch_bounds__(256) void k_chan_cols(const DevStream *__restrict__
     const int m = LG_N2 * (tid + i * T1) + n2;
     float2 z = make_float2(0.0f, 0.0f);
     if (!(none && m >= LG_N / 2)) {
-      const float2 xv = gload2(x + m), wv = w[m];
-      z = make_float2(xv.x * wv.x, xv.y * wv.y);
+      z = wmul(gload2(x + m), w[m]);
     }
     if (s.flip) z.y = -z.y;
     v[i] = z;
@@ -990,27 +996,24 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
       delay_block_pair(s, blk0 + b, job.n_blocks, scratch, tw, lane, ring + h_new * N + b * 256,
                        b + 1 < BPH ? ring + h_new * N + (b + 1) * 256 : nullptr);
     __syncthreads();
-    // window the segment's 2 N real samples into the FFT's registers
+    // window the segment's 2 N real samples into the FFT's registers:
+    // complex point m = tid + i T is the pair of reals 2m, 2m + 1, which lie
+    // in the segment's first half exactly when i < P / 2 (N = P T). Without a
+    // window the segment is the N reals of half 0 and N zeros. Branch-free,
+    // with every window load issued first.
     const int tid = opaque(threadIdx.x);
+    const float2 *ring2 = reinterpret_cast<const float2 *>(ring);
+    float2 wv[P];
+#pragma unroll
+    for (int i = 0; i < P; i++) wv[i] = gload2(w2, (uint32_t)(tid + i * T) * (uint32_t)sizeof(float2));
     float2 v[P];
 #pragma unroll
     for (int i = 0; i < P; i++) {
       const int m = tid + i * T;
-      float2 z;
-      if (none) {
-        if (m < N / 2) {
-          const float2 x = reinterpret_cast<const float2 *>(ring)[m], wv = w2[m];
-          z = make_float2(x.x * wv.x, x.y * wv.y);
-        } else {
-          z = make_float2(0.0f, 0.0f);
-        }
-      } else {
-        const int j = 2 * m;  // j and j + 1 lie in the same half
-        const int half = j < N ? h_old : 1 - h_old;
-        const float2 x = *reinterpret_cast<const float2 *>(ring + half * N + (j & (N - 1)));
-        const float2 wv = w2[m];
-        z = make_float2(x.x * wv.x, x.y * wv.y);
-      }
+      const int half = i < P / 2 ? h_old : 1 - h_old;
+      const float2 x = ring2[none ? m : half * (N / 2) + (m & (N / 2 - 1))];
+      float2 z = wmul(x, wv[i]);
+      if (none && i >= P / 2) z = make_float2(0.0f, 0.0f);
       if (s.flip) z.y = -z.y;
       v[i] = z;
     }
