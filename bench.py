@@ -68,12 +68,13 @@ def parse():
     return ap.parse_args()
 
 
-def kernel_bytes(job, M, nseg, n_samples, nfold, B, fused):
+def kernel_bytes(job, M, nseg, n_samples, nfold, B, fused, spec_bytes=8):
     """Algorithmic HBM bytes per launch of each kernel. fengine: unpack +
-    delay correction (+ channeliser when fused: payload in, spectra out)."""
+    delay correction (+ channeliser when fused: payload in, spectra out).
+    Spectra take spec_bytes per bin: 8 (FP32), 4 (FP16 / BF16 pairs)."""
     nc = job.fft_size_correlation
     bits = job.station_streams[0].bits_per_sample
-    spectra = M * nseg * (nc + 1) * 8
+    spectra = M * nseg * (nc + 1) * spec_bytes
     return {
         "stats": M * n_samples * bits / 8,
         "fengine": M * n_samples * bits / 8 + (spectra if fused else M * n_samples * 4),
@@ -348,13 +349,13 @@ class MixedBatch:
         return wall
 
 
-def summarise(wl, wall, kt):
+def summarise(wl, wall, kt, precision="fp32"):
     job = wl.job
     nseg = job.number_ffts_in_integration()
     B = len(job.baselines())
     nfold = max(1, round(job.integration_time / job.sub_integration_time))
     fused = kt[2] == 0.0
-    kb = kernel_bytes(job, wl.M, nseg, wl.nsamp, nfold, B, fused)
+    kb = kernel_bytes(job, wl.M, nseg, wl.nsamp, nfold, B, fused, 4 if precision in ("fp16", "bf16") else 8)
     value = wl.K * wl.station_samples_per_step / wall / 1e9
     xflops = 8.0 * B * (job.fft_size_correlation + 1) * nseg
     return {
@@ -437,7 +438,7 @@ def main():
         })
         if not args.main_only:
             wall, kt = wl.run(torch, capi, PREC["split_fp16"], W, args.contexts, dev, autotune=not args.no_autotune)
-            mixed = summarise(wl, wall, kt)
+            mixed = summarise(wl, wall, kt, "split_fp16")
             mixed.pop("_kb")
             add_mfma_frac(mixed, peaks)
             line["mixed_split_fp16"] = mixed
@@ -453,7 +454,7 @@ def main():
               "station_samples_per_step": w2.station_samples_per_step}
         for name in args.c2_precisions.split(","):
             wall, kt = w2.run(torch, capi, PREC[name], 0, args.contexts, dev, autotune=not args.no_autotune)
-            r = summarise(w2, wall, kt)
+            r = summarise(w2, wall, kt, name)
             r.pop("_kb")
             if name != "fp32":
                 add_mfma_frac(r, peaks)
@@ -496,7 +497,7 @@ def main():
               "station_samples_per_step": w4.station_samples_per_step}
         for name in args.c4_precisions.split(","):
             wall, kt = w4.run(torch, capi, PREC[name], 0, args.contexts, dev, autotune=not args.no_autotune)
-            r = summarise(w4, wall, kt)
+            r = summarise(w4, wall, kt, name)
             r.pop("_kb")
             if name != "fp32":
                 add_mfma_frac(r, peaks)

@@ -561,12 +561,18 @@ static int prepare_slice(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->model_buf.ensure((size_t)mod_tot * sizeof(double)));
   HIPCHK(ctx, ctx->dstreams.ensure((size_t)M * sizeof(DevStream)));
   const bool fused = !ctx->opt.keep_time_series && fengine_supported(nc);
+  // FP16 and BF16 X-steps take their operands straight from spectra the
+  // channeliser stores in that type (4 bytes per bin instead of 8); split
+  // FP16 and FP32 need the FP32 spectra
+  const int spec_prec = ctx->opt.precision == SFXG_PREC_FP16 || ctx->opt.precision == SFXG_PREC_BF16
+                            ? ctx->opt.precision : 0;
+  const size_t spec_bytes = spec_prec ? 4 : sizeof(float2);
   // chunks of whole sub-integrations whose spectra fit the budget (one
   // sub-integration at least)
   LaunchPlan &lp = ctx->lp;
   {
     const int64_t bph = nc / nd;  // delay blocks per segment advance
-    const size_t seg_bytes = (size_t)M * ctx->S * sizeof(float2);
+    const size_t seg_bytes = (size_t)M * ctx->S * spec_bytes;
     const size_t chunk_bytes = (size_t)(ctx->opt.chunk_mib > 0 ? ctx->opt.chunk_mib : 8192) << 20;
     const int64_t budget = std::max<int64_t>(1, (int64_t)(chunk_bytes / seg_bytes));
     lp.chunks.clear();
@@ -599,7 +605,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   if (!fused) HIPCHK(ctx, ctx->time_series.ensure((size_t)M * lp.ts_stride * sizeof(float)));
   if (channelise_scratch_elems(nc) > 0)
     HIPCHK(ctx, ctx->chan_scratch.ensure(channelise_scratch_elems(nc) * sizeof(float2)));
-  HIPCHK(ctx, ctx->spectra.ensure((size_t)M * lp.chunk_segs * ctx->S * sizeof(float2)));
+  HIPCHK(ctx, ctx->spectra.ensure((size_t)M * lp.chunk_segs * ctx->S * spec_bytes));
   const size_t vis_elems = (size_t)ctx->nbins * B * (nc + 1);
   HIPCHK(ctx, ctx->vis.ensure(vis_elems * sizeof(float2)));
   HIPCHK(ctx, ctx->norm_out.ensure(vis_elems * sizeof(float2)));
@@ -706,7 +712,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
     d.oversamp = (int)round(fs / (2.0 * st.bandwidth));
     d.flip = ctx->p.sideband != st.sideband;
     d.time_series = fused ? nullptr : ctx->time_series.as<float>() + (int64_t)i * lp.ts_stride;
-    d.spectra = ctx->spectra.as<float2>() + (int64_t)i * lp.chunk_segs * ctx->S;
+    d.spectra = reinterpret_cast<float2 *>(ctx->spectra.as<char>() + (size_t)i * lp.chunk_segs * ctx->S * spec_bytes);
     d.samples = nullptr;
     d.hist = ctx->hist.as<int32_t>() + 512 * i;
     d.block_consts = ctx->block_consts.as<BlockConst>() + blk_off[i];
@@ -745,6 +751,8 @@ static int prepare_slice(sfxg_ctx *ctx) {
   lp.job.n_segments = ctx->nseg;  // per chunk in enqueue_slice
   lp.job.spectra_stride = ctx->S;
   lp.job.seg_step = nc;  // fft_rot_size / 2
+  lp.job.spec_prec = spec_prec;
+  lp.job.spec_scale = 1.0f;
   lp.mixed_scale = 1.0f;
   if (mixed) {
     // power-of-two scale that brings the spectra's rms near 16: the delay
@@ -765,6 +773,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
     const double rms = (nd / 2.0) * lmax * sqrt(std::max(wsum, 1.0));
     lp.mixed_scale = (float)ldexp(1.0, (int)lround(log2(16.0 / rms)));
     ctx->mixed_scale = lp.mixed_scale;
+    if (spec_prec) lp.job.spec_scale = lp.mixed_scale;
   }
   if (ctx->pulsar) {
     // Correlation_core_pulsar::set_parameters (correlation_core_pulsar.cc:46-93)
@@ -841,7 +850,7 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
   if (lp.mixed && !ctx->pulsar) {
     for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
       const int ng = std::min(lp.k_group, ch.fold1 - g0);
-      HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1,
+      HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.p, lp.job.spec_prec != 0, sstride, ctx->S, ch.seg0, nc + 1,
                                     M, ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, lp.mixed_scale,
                                     ctx->partial.as<float2>(), ctx->vis.as<float2>(), t, s));
     }
@@ -863,7 +872,7 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
     for (int c0 = 0; c0 < nc + 1; c0 += lp.psr_run) {
       const int n_run = std::min(lp.psr_run, nc + 1 - c0);
       if (c == 0 || lp.psr_run < nc + 1) HIPCHK(ctx, pulsar_gram_zero(M, n_run, ctx->nbins, ctx->gram.as<float>(), s));
-      HIPCHK(ctx, launch_xstep_mfma_pulsar(ctx->opt.precision, ctx->spectra.as<float2>(), sstride, ctx->S, c0, n_run,
+      HIPCHK(ctx, launch_xstep_mfma_pulsar(ctx->opt.precision, ctx->spectra.p, lp.job.spec_prec != 0, sstride, ctx->S, c0, n_run,
                                            M, ch.seg0, ch.seg1 - ch.seg0, lp.mixed_scale, ctx->offsets.as<double>(),
                                            lp.start_phase, lp.phase_coef, lp.fft_duration, ps.gate_start,
                                            ps.gate_stop, ctx->nbins, ctx->gram.as<float>(), t, s));
@@ -1108,10 +1117,36 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
                          (int)ctx->lp.chunks.size());
       need = (size_t)ctx->nseg * (ctx->nc + 1) * sizeof(float2);
       if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
-      HIPCHK(ctx, hipMemcpy2D(dst, (ctx->nc + 1) * sizeof(float2),
-                              ctx->spectra.as<float2>() + (int64_t)stream_idx * ctx->lp.chunk_segs * ctx->S,
-                              ctx->S * sizeof(float2), (ctx->nc + 1) * sizeof(float2), ctx->nseg,
-                              hipMemcpyDeviceToHost));
+      if (ctx->lp.job.spec_prec == 0) {
+        HIPCHK(ctx, hipMemcpy2D(dst, (ctx->nc + 1) * sizeof(float2),
+                                ctx->spectra.as<float2>() + (int64_t)stream_idx * ctx->lp.chunk_segs * ctx->S,
+                                ctx->S * sizeof(float2), (ctx->nc + 1) * sizeof(float2), ctx->nseg,
+                                hipMemcpyDeviceToHost));
+      } else {
+        // packed FP16 / BF16 pairs, returned as float2 with the scale undone
+        const size_t row = (size_t)(ctx->nc + 1);
+        std::vector<uint32_t> packed((size_t)ctx->nseg * row);
+        HIPCHK(ctx, hipMemcpy2D(packed.data(), row * 4,
+                                ctx->spectra.as<uint32_t>() + (int64_t)stream_idx * ctx->lp.chunk_segs * ctx->S,
+                                ctx->S * 4, row * 4, ctx->nseg, hipMemcpyDeviceToHost));
+        const bool bf = ctx->lp.job.spec_prec == SFXG_PREC_BF16;
+        const float inv = 1.0f / ctx->lp.job.spec_scale;
+        auto half_to_float = [bf](uint32_t h) {
+          if (bf) {
+            const uint32_t b = h << 16;
+            float f;
+            memcpy(&f, &b, 4);
+            return f;
+          }
+          const uint16_t u = (uint16_t)h;
+          _Float16 x;
+          memcpy(&x, &u, 2);
+          return (float)x;
+        };
+        float2 *out = static_cast<float2 *>(dst);
+        for (size_t k = 0; k < packed.size(); k++)
+          out[k] = make_float2(half_to_float(packed[k] & 0xffffu) * inv, half_to_float(packed[k] >> 16) * inv);
+      }
       break;
     case SFXG_STAGE_ACCUMULATED:
     case SFXG_STAGE_NORMALISED:

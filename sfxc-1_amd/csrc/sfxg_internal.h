@@ -134,7 +134,7 @@ struct DevStream {
   int32_t oversamp;
   int32_t flip;      // sideband flip for the channeliser
   float *time_series;
-  float2 *spectra;   // [n_segments][spectra_stride]
+  float2 *spectra;   // [n_segments][spectra_stride] float2, or packed 32-bit pairs (DevJob::spec_prec)
   float *samples;    // optional unpack output (verification)
   int32_t *hist;     // [2][256] byte histograms (tsys on, off)
   BlockConst *block_consts;  // [n_blocks]
@@ -149,6 +149,8 @@ struct DevJob {
   int32_t nd;               // fft_size_delaycor
   int32_t nc;               // fft_size_correlation
   int32_t window;
+  int32_t spec_prec;        // spectra stored as float2 (0) or packed FP16 / BF16 pairs (SFXG_PREC_*)
+  float spec_scale;         // power-of-two scale of packed spectra
   int64_t n_blocks;         // N_d blocks per stream in the slice
   int64_t n_samples;        // n_blocks * nd
   int64_t n_segments;       // segments of this chunk
@@ -216,7 +218,9 @@ void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::ve
 hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
                              int n_baselines, float2 *partial, float2 *vis, hipStream_t s);
-hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+// half_spectra: spectra packed in the operand type (FP16 / BF16 only).
+hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spectra, int64_t stream_stride,
+                             int64_t seg_stride,
                              int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
                              float2 *partial, float2 *vis, const Tune &t, hipStream_t s);
@@ -238,7 +242,8 @@ size_t pulsar_gram_floats(int M, int n_ch, int nbins);
 // - 1 of each chunk (launch_xstep_mfma_pulsar), form the baselines after the
 // last chunk (pulsar_gram_baselines).
 hipError_t pulsar_gram_zero(int M, int n_ch_run, int nbins, float *gram, hipStream_t s);
-hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+hipError_t launch_xstep_mfma_pulsar(int precision, const void *spectra, bool half_spectra, int64_t stream_stride,
+                                    int64_t seg_stride,
                                     int ch_base, int n_ch_run, int M, int64_t seg_base, int64_t n_segments,
                                     float scale, const double *offsets, double start_phase, double phase_coef,
                                     double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,

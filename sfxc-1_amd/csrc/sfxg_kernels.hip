@@ -64,9 +64,6 @@ __device__ __forceinline__ float2 gload2(const float2 *p) {
   const vf2_t v = *(const __attribute__((address_space(1))) vf2_t *)p;
   return make_float2(v.x, v.y);
 }
-__device__ __forceinline__ void gstore2(float2 *p, float2 v) {
-  *(__attribute__((address_space(1))) vf2_t *)p = vf2_t{v.x, v.y};
-}
 // base (wave-uniform) + a 32-bit byte offset: the scalar-base form of the
 // global load, one VGPR per address instead of two.
 __device__ __forceinline__ float2 gload2(const float2 *base, uint32_t byte_off) {
@@ -86,6 +83,38 @@ __device__ __forceinline__ float2 wmul(float2 x, float2 w) {
 #pragma clang fp contract(off)
   return make_float2(x.x * w.x, x.y * w.y);
 }
+
+// Spectrum stores. SP = 0 writes float2 bins. SP = SFXG_PREC_FP16 or
+// SFXG_PREC_BF16 scales both parts by a power of two (exact) and rounds them
+// (to nearest even) to the MFMA X-step's operand type, packed in 32 bits
+// with the real part in the low half: the X-step then reads half the bytes
+// and converts nothing.
+template <int SP>
+struct SpecOut {
+  uint32_t *p;
+  float scale;
+  __device__ __forceinline__ SpecOut(float2 *base, int64_t off, float sc)
+      : p(reinterpret_cast<uint32_t *>(base) + off), scale(sc) {}
+  __device__ __forceinline__ void put(uint32_t k, float2 v) const {
+    uint32_t re, im;
+    if constexpr (SP == SFXG_PREC_BF16) {
+      re = __builtin_bit_cast(uint16_t, (__bf16)(v.x * scale));
+      im = __builtin_bit_cast(uint16_t, (__bf16)(v.y * scale));
+    } else {
+      static_assert(SP == SFXG_PREC_FP16, "FP16 or BF16 spectra");
+      re = __builtin_bit_cast(uint16_t, (_Float16)(v.x * scale));
+      im = __builtin_bit_cast(uint16_t, (_Float16)(v.y * scale));
+    }
+    auto *q = (__attribute__((address_space(1))) char *)p + k * 4u;
+    *(__attribute__((address_space(1))) uint32_t *)q = re | (im << 16);
+  }
+};
+template <>
+struct SpecOut<0> {
+  float2 *p;
+  __device__ __forceinline__ SpecOut(float2 *base, int64_t off, float) : p(base + off) {}
+  __device__ __forceinline__ void put(uint32_t k, float2 v) const { gstore2(p, k * (uint32_t)sizeof(float2), v); }
+};
 
 // 4-point DFT in place, exp(SIGN 2 pi i jk/4)
 template <int SIGN>
@@ -150,46 +179,68 @@ __global__ __launch_bounds__(256) void k_unpack(const DevStream *__restrict__ st
 }
 
 // ---------------------------------------------------------------------------
-// k_stats: histograms of the counted payload bytes, split by tsys phase
+// k_stats: histograms of the counted payload bytes, split by tsys phase.
+// Each workgroup takes a contiguous span of counting ordinals of one stream
+// (consecutive threads on consecutive bytes), so that its LDS histogram is
+// flushed to HBM once per span. A thread finds its first byte's range and
+// tsys phase once, then advances both incrementally: ordinals are
+// contiguous across ranges, and the phase flips at tsys_count0 and every
+// tsys_half bytes after it.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ streams) {
+__global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ streams, int64_t span) {
   __shared__ int hist[2][256];
   const DevStream &s = streams[blockIdx.y];
   for (int i = threadIdx.x; i < 512; i += blockDim.x) (&hist[0][0])[i] = 0;
   __syncthreads();
+  const int64_t n = s.n_counted;
   int64_t total = 0;
-  if (s.n_counted > 0) {
-    const CountRange last = s.counted[s.n_counted - 1];
+  if (n > 0) {
+    const CountRange last = s.counted[n - 1];
     total = last.ordinal + last.n_bytes;
   }
-  const int64_t per_block = (int64_t)blockDim.x * 16;
-  const int64_t g0 = (int64_t)blockIdx.x * per_block;
-  if (g0 < total) {
-    // range holding the first byte of this block
-    int64_t lo = 0, hi = s.n_counted - 1;
+  const int64_t g0 = (int64_t)blockIdx.x * span;
+  const int64_t g1 = g0 + span < total ? g0 + span : total;
+  const int64_t gt = g0 + threadIdx.x;
+  if (gt < g1) {
+    int64_t lo = 0, hi = n - 1;  // last range with ordinal <= gt
     while (lo < hi) {
       const int64_t mid = (lo + hi + 1) >> 1;
-      if (s.counted[mid].ordinal <= g0) lo = mid; else hi = mid - 1;
+      if (s.counted[mid].ordinal <= gt) lo = mid; else hi = mid - 1;
     }
-    for (int k = 0; k < 16; k++) {
-      const int64_t g = g0 + (int64_t)k * blockDim.x + threadIdx.x;
-      if (g >= total) break;
-      int64_t r = lo;
-      while (r + 1 < s.n_counted && s.counted[r + 1].ordinal <= g) r++;
-      const CountRange cr = s.counted[r];
-      const uint8_t b = s.payload[cr.byte_start + (g - cr.ordinal)];
-      int on = 1;
-      if (s.bits == 2) {
-        on = s.tsys_on0;
-        if (s.tsys_count0 > 0 && g >= s.tsys_count0) {
-          if (s.tsys_half > 0) {
-            const int64_t k2 = (g - s.tsys_count0) / s.tsys_half;
-            on = s.tsys_on0 ^ (int)((k2 + 1) & 1);
-          } else {
-            on = !s.tsys_on0;
-          }
+    int64_t r = lo;
+    int64_t next_ord = r + 1 < n ? s.counted[r + 1].ordinal : INT64_MAX;
+    int64_t delta = s.counted[r].byte_start - s.counted[r].ordinal;
+    // phase: tsys_on0 before tsys_count0, flipped at it and then every
+    // tsys_half bytes (once only without a half period); 1-bit data has none
+    int on = 1;
+    int64_t flip = INT64_MAX;
+    const int64_t c0 = s.tsys_count0, half = s.tsys_half;
+    if (s.bits == 2) {
+      on = s.tsys_on0;
+      if (c0 > 0) {
+        if (gt < c0) {
+          flip = c0;
+        } else if (half > 0) {
+          const int64_t k2 = (gt - c0) / half;
+          on = s.tsys_on0 ^ (int)((k2 + 1) & 1);
+          flip = c0 + (k2 + 1) * half;
+        } else {
+          on = !s.tsys_on0;
         }
       }
+    }
+    const uint8_t *pay = s.payload;
+    for (int64_t g = gt; g < g1; g += 256) {
+      while (g >= next_ord) {
+        r++;
+        delta = s.counted[r].byte_start - s.counted[r].ordinal;
+        next_ord = r + 1 < n ? s.counted[r + 1].ordinal : INT64_MAX;
+      }
+      while (g >= flip) {
+        on ^= 1;
+        flip = half > 0 ? flip + half : INT64_MAX;
+      }
+      const uint8_t b = pay[delta + g];
       atomicAdd(&hist[on ? 0 : 1][b], 1);
     }
   }
@@ -692,8 +743,8 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[P], float2 *lds, const TW &t
 // and thread 0 also bin N / 2: two LDS reads and one twiddle per two bins,
 // and a fixed number of stores per thread, so that the compiler can wait for
 // loads issued before them without waiting for the stores too.
-template <int N, int P, int T, class TW>
-__device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, float2 *out, int tid) {
+template <int N, int P, int T, class TW, class OUT>
+__device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, const OUT &out, int tid) {
   static_assert(N == P * T && P % 2 == 0, "pairs of bins per thread");
   auto eo = [&](int k, float2 &e, float2 &o) {
     const float2 zk = lds[lds_index<P>(k)], zn = lds[lds_index<P>((N - k) & (N - 1))];
@@ -706,14 +757,14 @@ __device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, floa
     float2 e, o;
     eo(k, e, o);
     const float2 wo = cmul(tw.split(k), o);
-    gstore2(out, (uint32_t)k * (uint32_t)sizeof(float2), cadd(e, wo));
-    gstore2(out, (uint32_t)(N - k) * (uint32_t)sizeof(float2), make_float2(e.x - wo.x, -(e.y - wo.y)));
+    out.put((uint32_t)k, cadd(e, wo));
+    out.put((uint32_t)(N - k), make_float2(e.x - wo.x, -(e.y - wo.y)));
     __builtin_amdgcn_sched_barrier(0);  // one pair's loads in flight at a time
   }
   if (tid == 0) {
     float2 e, o;
     eo(N / 2, e, o);
-    gstore2(out, (uint32_t)(N / 2) * (uint32_t)sizeof(float2), cadd(e, cmul(tw.split(N / 2), o)));
+    out.put((uint32_t)(N / 2), cadd(e, cmul(tw.split(N / 2), o)));
   }
 }
 
@@ -725,7 +776,7 @@ __device__ __forceinline__ void real_split(const float2 *lds, const TW &tw, floa
 // HBM straight into the FFT's registers, transformed and split into the
 // N + 1 bins of the real transform.
 // ---------------------------------------------------------------------------
-template <int N, int P>
+template <int N, int P, int SP>
 __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restrict__ streams, DevJob job,
                                                       const float *__restrict__ window,
                                                       const float2 *__restrict__ tw_split) {
@@ -757,7 +808,7 @@ __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restric
   __syncthreads();  // twiddle tables
   const LdsTw tw{thi, tlo};
   reg_fft<N, P>(v, lds, tw, threadIdx.x);
-  real_split<N, P, T>(lds, tw, s.spectra + seg * job.spectra_stride, threadIdx.x);
+  real_split<N, P, T>(lds, tw, SpecOut<SP>(s.spectra, seg * job.spectra_stride, job.spec_scale), threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -769,7 +820,7 @@ __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restric
 // segment ahead. The FFT takes its twiddles from two short LDS tables, so
 // the only global loads in flight during it are the prefetch.
 // ---------------------------------------------------------------------------
-template <int N, int P>
+template <int N, int P, int SP>
 __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__restrict__ streams, DevJob job,
                                                           const float *__restrict__ window,
                                                           const float2 *__restrict__ tw_split, int segs_per_wg) {
@@ -846,7 +897,7 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
       hi_w[i] = none ? make_float2(0.0f, 0.0f) : win(nx[i], whi[i]);
       raw[i] = nx[i];
     }
-    real_split<N, P, T>(lds, tw, s.spectra + seg * job.spectra_stride, tid);
+    real_split<N, P, T>(lds, tw, SpecOut<SP>(s.spectra, seg * job.spectra_stride, job.spec_scale), tid);
     __syncthreads();
   }
 }
@@ -925,6 +976,7 @@ __global__ __launch_bounds__(256) void k_chan_rows(const float2 *__restrict__ Y,
   }
 }
 
+template <int SP>
 __global__ __launch_bounds__(256) void k_chan_split(const DevStream *__restrict__ streams, DevJob job,
                                                     const float2 *__restrict__ Z, const float2 *__restrict__ tw_2n,
                                                     int64_t item0) {
@@ -932,12 +984,12 @@ __global__ __launch_bounds__(256) void k_chan_split(const DevStream *__restrict_
   const int st = (int)(item / job.n_segments);
   const int64_t seg = item - (int64_t)st * job.n_segments;
   const float2 *z = Z + (int64_t)blockIdx.x * LG_N;
-  float2 *out = streams[st].spectra + seg * job.spectra_stride;
+  const SpecOut<SP> out(streams[st].spectra, seg * job.spectra_stride, job.spec_scale);
   for (int k = threadIdx.x; k <= LG_N; k += 256) {
     const float2 zk = z[k & (LG_N - 1)], zn = z[(LG_N - k) & (LG_N - 1)];
     const float2 e = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
     const float2 o = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
-    gstore2(out + k, cadd(e, cmul(tw_2n[k], o)));
+    out.put((uint32_t)k, cadd(e, cmul(tw_2n[k], o)));
   }
 }
 
@@ -952,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_chan_split(const DevStream *__restrict_
 // the spectrum. The FFT buffer doubles as the waves' scratch for the 256-point
 // transforms.
 // ---------------------------------------------------------------------------
-template <int N, int T>
+template <int N, int T, int SP>
 __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ streams, DevJob job,
                                                const float *__restrict__ window, const float2 *__restrict__ tw_nd,
                                                const float2 *__restrict__ tw_split, int segs_per_wg) {
@@ -1018,7 +1070,7 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
       v[i] = z;
     }
     reg_fft<N, P>(v, fbuf, twl, tid);
-    real_split<N, P, T>(fbuf, twl, s.spectra + seg * job.spectra_stride, tid);
+    real_split<N, P, T>(fbuf, twl, SpecOut<SP>(s.spectra, seg * job.spectra_stride, job.spec_scale), tid);
     __syncthreads();
   }
 }
@@ -1299,8 +1351,10 @@ __device__ __forceinline__ void mf_pair(int p, int nT, int RT, int &c, int &t, i
 // into the LDS planes as Y[c][r][k]:
 // consecutive threads take consecutive channels, each thread 8 segments of
 // one stream, converted to the MFMA operand type and stored 16 bytes at a time.
-template <int PREC, int KS = MF_KS>
-__device__ __forceinline__ void mf_load_stage(const float2 *__restrict__ spectra, int64_t stream_stride,
+// HS: the spectra are already in the operand type (SpecOut), 32 bits per
+// bin, and only their halves are regrouped.
+template <int PREC, int KS = MF_KS, bool HS = false>
+__device__ __forceinline__ void mf_load_stage(const void *__restrict__ spectra_v, int64_t stream_stride,
                                               int64_t seg_stride, int64_t seg_base, int n_ch, int M, int CH,
                                               int ch0, int64_t s0,
                                               int64_t s_end, float scale, int chan_stride, int plane_elems,
@@ -1311,6 +1365,30 @@ __device__ __forceinline__ void mf_load_stage(const float2 *__restrict__ spectra
     const int rest = item / CH;
     const int kg = rest % (KS / 8);
     const int st = rest / (KS / 8);
+    const int o_re = c * chan_stride + st * (KS + 8) + kg * 8;
+    const int o_im = c * chan_stride + (M + st) * (KS + 8) + kg * 8;
+    if constexpr (HS) {
+      static_assert(PREC != SFXG_PREC_SPLIT_FP16, "split FP16 reads FP32 spectra");
+      const uint32_t *sp = static_cast<const uint32_t *>(spectra_v);
+      uint32_t w[8];
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) {
+        const int64_t seg = s0 + kg * 8 + kk;
+        w[kk] = 0u;
+        if (seg < s_end && ch0 + c < n_ch) w[kk] = sp[st * stream_stride + (seg - seg_base) * seg_stride + ch0 + c];
+      }
+      typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
+      u4_t r, i;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        r[e] = (w[2 * e] & 0xffffu) | (w[2 * e + 1] << 16);
+        i[e] = (w[2 * e] >> 16) | (w[2 * e + 1] & 0xffff0000u);
+      }
+      *reinterpret_cast<u4_t *>(ylds + o_re) = r;
+      *reinterpret_cast<u4_t *>(ylds + o_im) = i;
+      continue;
+    }
+    const float2 *spectra = static_cast<const float2 *>(spectra_v);
     float re[8], im[8];
 #pragma unroll
     for (int kk = 0; kk < 8; kk++) {
@@ -1320,8 +1398,6 @@ __device__ __forceinline__ void mf_load_stage(const float2 *__restrict__ spectra
       re[kk] = v.x * scale;
       im[kk] = v.y * scale;
     }
-    const int o_re = c * chan_stride + st * (KS + 8) + kg * 8;
-    const int o_im = c * chan_stride + (M + st) * (KS + 8) + kg * 8;
     if (PREC == SFXG_PREC_BF16) {
       b8_t br, bi;
 #pragma unroll
@@ -1391,9 +1467,11 @@ __device__ __forceinline__ void mf_mma(f16v_t &acc, const _Float16 *ylds, int oa
   }
 }
 
-template <int PREC, int KS>  // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16; segments per stage
+// SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16; segments per stage;
+// spectra in the operand type (HS) or FP32
+template <int PREC, int KS, bool HS>
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
-    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch, int M,
+    const void *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch, int M,
     int RT, int CH, const int32_t *__restrict__ fold_start, float scale, const int32_t *__restrict__ bl_first,
     const int32_t *__restrict__ bl_second, int n_baselines, float inv_scale2, float2 *__restrict__ partial) {
   extern __shared__ _Float16 ylds[];
@@ -1415,7 +1493,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
   __syncthreads();
   const int n_pairs = CH * nT;
   for (int64_t s0 = s_begin; s0 < s_end; s0 += KS) {
-    mf_load_stage<PREC, KS>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
+    mf_load_stage<PREC, KS, HS>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
                             chan_stride, plane_elems, ylds);
     __syncthreads();
 #pragma unroll
@@ -1559,9 +1637,9 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
 // ---------------------------------------------------------------------------
 constexpr int MFP_MAXP = 4;  // (channel, tile) pairs per wave with pulsar binning
 
-template <int PREC>
+template <int PREC, bool HS>
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
-    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT,
+    const void *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT,
     int CH, int64_t seg_base, int64_t n_segments, int64_t segs_per_z, float scale, const double *__restrict__ offsets,
     double start_phase, double phase_coef, double fft_duration, double gate_begin, double gate_end, int nbins,
     float *__restrict__ gram) {
@@ -1600,7 +1678,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
       }
   };
   for (int64_t s0 = s_begin; s0 < s_end; s0 += MF_KS) {
-    mf_load_stage<PREC>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
+    mf_load_stage<PREC, MF_KS, HS>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
                         chan_stride, plane_elems, ylds);
     for (int i = threadIdx.x; i < CH * MF_KS; i += MF_THREADS) {
       const int c = i / MF_KS, k = i % MF_KS;
@@ -1794,10 +1872,12 @@ hipError_t launch_unpack(const DevStream *d_streams, int n_streams, int64_t n_sa
 }
 
 hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_counted_bytes, hipStream_t s) {
-  const int64_t per_block = 256 * 16;
-  dim3 grid((unsigned)((max_counted_bytes + per_block - 1) / per_block), n_streams);
-  if (grid.x == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_stats, grid, dim3(256), 0, s, d_streams);
+  if (max_counted_bytes <= 0 || n_streams <= 0) return hipSuccess;
+  // about 4096 workgroups over all streams, each over at least 16 KiB
+  int64_t span = max_counted_bytes * n_streams / 4096;
+  span = std::max<int64_t>(16384, (span + 4095) / 4096 * 4096);
+  dim3 grid((unsigned)((max_counted_bytes + span - 1) / span), n_streams);
+  hipLaunchKernelGGL(k_stats, grid, dim3(256), 0, s, d_streams, span);
   return hipGetLastError();
 }
 
@@ -1821,15 +1901,15 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
   return hipGetLastError();
 }
 
-template <int N, int P>
+template <int N, int P, int SP>
 static hipError_t launch_chan_small(const DevStream *d_streams, const DevJob &job, const float *window,
                                     const float2 *tw_split, hipStream_t s) {
   dim3 grid((unsigned)job.n_segments, job.n_streams);
-  hipLaunchKernelGGL((k_channelise<N, P>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_split);
+  hipLaunchKernelGGL((k_channelise<N, P, SP>), grid, dim3(N / P), 0, s, d_streams, job, window, tw_split);
   return hipGetLastError();
 }
 
-template <int N>
+template <int N, int SP>
 static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, const float *window,
                                 const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, const Tune &t,
                                 hipStream_t s) {
@@ -1842,16 +1922,16 @@ static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, c
     int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
     if (segs < 1) segs = 1;
     const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
-    hipLaunchKernelGGL((k_channelise_run<N, P>), dim3(gx, job.n_streams), dim3(N / P), 0, s, d_streams, job, window,
+    hipLaunchKernelGGL((k_channelise_run<N, P, SP>), dim3(gx, job.n_streams), dim3(N / P), 0, s, d_streams, job, window,
                        tw_split, (int)segs);
   } else {
     // the fused F-engine's schedule, so both paths give the same bits:
     // threads min(N / 4, 512) (PA points each) or half as many
     constexpr int PA = N / (N / 4 < 512 ? N / 4 : 512);
     const int T = t.fe_threads > 0 ? t.fe_threads : N / PA;
-    if (T == N / PA) return launch_chan_small<N, PA>(d_streams, job, window, tw_split, s);
+    if (T == N / PA) return launch_chan_small<N, PA, SP>(d_streams, job, window, tw_split, s);
     if constexpr (2 * PA <= 16 && N / (2 * PA) >= 64) {
-      if (T == N / (2 * PA)) return launch_chan_small<N, 2 * PA>(d_streams, job, window, tw_split, s);
+      if (T == N / (2 * PA)) return launch_chan_small<N, 2 * PA, SP>(d_streams, job, window, tw_split, s);
     }
     return hipErrorInvalidValue;
   }
@@ -1860,6 +1940,7 @@ static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, c
 
 size_t channelise_scratch_elems(int nc) { return nc == LG_N ? (size_t)2 * 1024 * LG_N : 0; }
 
+template <int SP>
 static hipError_t launch_chan_large(const DevStream *d_streams, const DevJob &job, const float *window,
                                     const float2 *tw_n, const float2 *tw_2n, float2 *scratch, hipStream_t s) {
   // scratch: Y and Z for up to 1024 (stream, segment) items per launch
@@ -1871,26 +1952,40 @@ static hipError_t launch_chan_large(const DevStream *d_streams, const DevJob &jo
     const unsigned n = (unsigned)(total - i0 < kItems ? total - i0 : kItems);
     hipLaunchKernelGGL(k_chan_cols, dim3(LG_N2 / 16, n), dim3(256), 0, s, d_streams, job, window, tw_n, i0, Y);
     hipLaunchKernelGGL(k_chan_rows, dim3(LG_N1 / 16, n), dim3(256), 0, s, Y, tw_n, Z);
-    hipLaunchKernelGGL(k_chan_split, dim3(n), dim3(256), 0, s, d_streams, job, Z, tw_2n, i0);
+    hipLaunchKernelGGL(k_chan_split<SP>, dim3(n), dim3(256), 0, s, d_streams, job, Z, tw_2n, i0);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
-hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
+template <int SP>
+static hipError_t launch_channelise_sp(const DevStream *d_streams, const DevJob &job, const float *window,
                              const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
                              const Tune &t, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
-  if (job.nc == LG_N) return launch_chan_large(d_streams, job, window, tw_n, tw_2n, scratch, s);
+  if (job.nc == LG_N) return launch_chan_large<SP>(d_streams, job, window, tw_n, tw_2n, scratch, s);
   switch (job.nc) {
-    case 256: return launch_chan_n<256>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
-    case 512: return launch_chan_n<512>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
-    case 1024: return launch_chan_n<1024>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
-    case 2048: return launch_chan_n<2048>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
-    case 4096: return launch_chan_n<4096>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
-    case 8192: return launch_chan_n<8192>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
-    case 16384: return launch_chan_n<16384>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 256: return launch_chan_n<256, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 512: return launch_chan_n<512, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 1024: return launch_chan_n<1024, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 2048: return launch_chan_n<2048, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 4096: return launch_chan_n<4096, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 8192: return launch_chan_n<8192, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    case 16384: return launch_chan_n<16384, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
+                             const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
+                             const Tune &t, hipStream_t s) {
+  switch (job.spec_prec) {
+    case 0: return launch_channelise_sp<0>(d_streams, job, window, tw_n, tw_2n, tw_split, scratch, t, s);
+    case SFXG_PREC_FP16:
+      return launch_channelise_sp<SFXG_PREC_FP16>(d_streams, job, window, tw_n, tw_2n, tw_split, scratch, t, s);
+    case SFXG_PREC_BF16:
+      return launch_channelise_sp<SFXG_PREC_BF16>(d_streams, job, window, tw_n, tw_2n, tw_split, scratch, t, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1925,7 +2020,7 @@ std::vector<int> mf_channel_candidates(int precision, int M) {
 }
 std::vector<int> mfp_wgs_candidates() { return {1024, 512, 2048}; }
 
-template <int N, int T>
+template <int N, int T, int SP>
 static hipError_t launch_fe_n(const DevStream *d_streams, const DevJob &job, const float *window,
                               const float2 *tw_nd, const float2 *tw_split, hipStream_t s) {
   // about 1024 workgroups in flight: two per CU
@@ -1933,28 +2028,39 @@ static hipError_t launch_fe_n(const DevStream *d_streams, const DevJob &job, con
   int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
   if (segs < 1) segs = 1;
   const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
-  hipLaunchKernelGGL((k_fengine<N, T>), dim3(gx, job.n_streams), dim3(T), 0, s, d_streams, job, window, tw_nd,
+  hipLaunchKernelGGL((k_fengine<N, T, SP>), dim3(gx, job.n_streams), dim3(T), 0, s, d_streams, job, window, tw_nd,
                      tw_split, (int)segs);
   return hipGetLastError();
 }
 
 bool fengine_supported(int nc) { return nc >= 256 && nc <= 4096; }
 
-hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window, const float2 *tw_nd,
+template <int SP>
+static hipError_t launch_fengine_sp(const DevStream *d_streams, const DevJob &job, const float *window, const float2 *tw_nd,
                           const float2 *tw_split, const Tune &t, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
   if (job.nd != 256) return hipErrorInvalidValue;
   const int T = t.fe_threads > 0 ? t.fe_threads : fe_thread_candidates(job.nc)[0];
   switch (job.nc * 1024 + T) {
-    case 256 * 1024 + 64: return launch_fe_n<256, 64>(d_streams, job, window, tw_nd, tw_split, s);
-    case 512 * 1024 + 128: return launch_fe_n<512, 128>(d_streams, job, window, tw_nd, tw_split, s);
-    case 512 * 1024 + 64: return launch_fe_n<512, 64>(d_streams, job, window, tw_nd, tw_split, s);
-    case 1024 * 1024 + 256: return launch_fe_n<1024, 256>(d_streams, job, window, tw_nd, tw_split, s);
-    case 1024 * 1024 + 128: return launch_fe_n<1024, 128>(d_streams, job, window, tw_nd, tw_split, s);
-    case 2048 * 1024 + 512: return launch_fe_n<2048, 512>(d_streams, job, window, tw_nd, tw_split, s);
-    case 2048 * 1024 + 256: return launch_fe_n<2048, 256>(d_streams, job, window, tw_nd, tw_split, s);
-    case 4096 * 1024 + 512: return launch_fe_n<4096, 512>(d_streams, job, window, tw_nd, tw_split, s);
-    case 4096 * 1024 + 256: return launch_fe_n<4096, 256>(d_streams, job, window, tw_nd, tw_split, s);
+    case 256 * 1024 + 64: return launch_fe_n<256, 64, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    case 512 * 1024 + 128: return launch_fe_n<512, 128, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    case 512 * 1024 + 64: return launch_fe_n<512, 64, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    case 1024 * 1024 + 256: return launch_fe_n<1024, 256, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    case 1024 * 1024 + 128: return launch_fe_n<1024, 128, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    case 2048 * 1024 + 512: return launch_fe_n<2048, 512, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    case 2048 * 1024 + 256: return launch_fe_n<2048, 256, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    case 4096 * 1024 + 512: return launch_fe_n<4096, 512, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    case 4096 * 1024 + 256: return launch_fe_n<4096, 256, SP>(d_streams, job, window, tw_nd, tw_split, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const float *window, const float2 *tw_nd,
+                          const float2 *tw_split, const Tune &t, hipStream_t s) {
+  switch (job.spec_prec) {
+    case 0: return launch_fengine_sp<0>(d_streams, job, window, tw_nd, tw_split, t, s);
+    case SFXG_PREC_FP16: return launch_fengine_sp<SFXG_PREC_FP16>(d_streams, job, window, tw_nd, tw_split, t, s);
+    case SFXG_PREC_BF16: return launch_fengine_sp<SFXG_PREC_BF16>(d_streams, job, window, tw_nd, tw_split, t, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2033,20 +2139,20 @@ hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64
   return hipGetLastError();
 }
 
-template <int PREC, int KS>
-static hipError_t launch_mfma_ks(dim3 grid, size_t lds, hipStream_t s, const float2 *spectra, int64_t stream_stride,
+template <int PREC, int KS, bool HS>
+static hipError_t launch_mfma_ks(dim3 grid, size_t lds, hipStream_t s, const void *spectra, int64_t stream_stride,
                                  int64_t seg_stride, int64_t seg_base, int n_ch, int M, int RT, int CH,
                                  const int32_t *fold_start, float scale, const int32_t *bl_first,
                                  const int32_t *bl_second, int n_baselines, float inv2, float2 *partial) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma<PREC, KS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds);
+  hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma<PREC, KS, HS>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_xstep_mfma<PREC, KS>), grid, dim3(MF_THREADS), lds, s, spectra, stream_stride, seg_stride,
+  hipLaunchKernelGGL((k_xstep_mfma<PREC, KS, HS>), grid, dim3(MF_THREADS), lds, s, spectra, stream_stride, seg_stride,
                      seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial);
   return hipGetLastError();
 }
 
-hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spectra, int64_t stream_stride, int64_t seg_stride,
                              int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
                              float2 *partial, float2 *vis, const Tune &t, hipStream_t s) {
@@ -2079,18 +2185,20 @@ hipError_t launch_xstep_mfma(int precision, const float2 *spectra, int64_t strea
   dim3 grid((unsigned)((n_ch + CH - 1) / CH), 1, (unsigned)n_k);
   const float inv2 = 1.0f / (scale * scale);
   hipError_t e = hipErrorInvalidValue;
-#define SFXG_MFMA_CASE(P)                                                                                        \
-  case P:                                                                                                        \
-    e = KS == 32 ? launch_mfma_ks<P, 32>(grid, lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch, M, RT, \
-                                         CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial) \
-                 : launch_mfma_ks<P, 16>(grid, lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch, M, RT, \
-                                         CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial); \
-    break;
-  switch (precision) {
-    SFXG_MFMA_CASE(SFXG_PREC_FP16)
-    SFXG_MFMA_CASE(SFXG_PREC_BF16)
-    SFXG_MFMA_CASE(SFXG_PREC_SPLIT_FP16)
-    default: return hipErrorInvalidValue;
+#define SFXG_MFMA_CASE(P, H)                                                                                     \
+  e = KS == 32 ? launch_mfma_ks<P, 32, H>(grid, lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch, M, RT, \
+                                          CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial) \
+               : launch_mfma_ks<P, 16, H>(grid, lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch, M, RT, \
+                                          CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial);
+  if (half_spectra) {
+    if (precision == SFXG_PREC_FP16) SFXG_MFMA_CASE(SFXG_PREC_FP16, true)
+    else if (precision == SFXG_PREC_BF16) SFXG_MFMA_CASE(SFXG_PREC_BF16, true)
+    else return hipErrorInvalidValue;
+  } else {
+    if (precision == SFXG_PREC_FP16) SFXG_MFMA_CASE(SFXG_PREC_FP16, false)
+    else if (precision == SFXG_PREC_BF16) SFXG_MFMA_CASE(SFXG_PREC_BF16, false)
+    else if (precision == SFXG_PREC_SPLIT_FP16) SFXG_MFMA_CASE(SFXG_PREC_SPLIT_FP16, false)
+    else return hipErrorInvalidValue;
   }
 #undef SFXG_MFMA_CASE
   if (e != hipSuccess) return e;
@@ -2144,17 +2252,17 @@ size_t pulsar_gram_floats(int M, int n_ch, int nbins) {
   return (size_t)nbins * n_ch * (RT * (RT + 1) / 2) * 1024;
 }
 
-template <int PREC>
-static hipError_t launch_mfma_pulsar_t(dim3 grid, size_t lds, hipStream_t s, const float2 *spectra,
+template <int PREC, bool HS>
+static hipError_t launch_mfma_pulsar_t(dim3 grid, size_t lds, hipStream_t s, const void *spectra,
                                        int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT, int CH,
                                        int64_t seg_base, int64_t n_segments, int64_t segs_per_z, float scale,
                                        const double *offsets, double start_phase, double phase_coef,
                                        double fft_duration, double gate_begin, double gate_end, int nbins,
                                        float *gram) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma_pulsar<PREC>,
+  hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma_pulsar<PREC, HS>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_xstep_mfma_pulsar<PREC>, grid, dim3(MF_THREADS), lds, s, spectra, stream_stride, seg_stride,
+  hipLaunchKernelGGL((k_xstep_mfma_pulsar<PREC, HS>), grid, dim3(MF_THREADS), lds, s, spectra, stream_stride, seg_stride,
                      n_ch, M, RT, CH, seg_base, n_segments, segs_per_z, scale, offsets, start_phase, phase_coef,
                      fft_duration, gate_begin, gate_end, nbins, gram);
   return hipGetLastError();
@@ -2164,7 +2272,8 @@ hipError_t pulsar_gram_zero(int M, int n_ch_run, int nbins, float *gram, hipStre
   return launch_fill_zero(gram, pulsar_gram_floats(M, n_ch_run, nbins) * sizeof(float), s);
 }
 
-hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+hipError_t launch_xstep_mfma_pulsar(int precision, const void *spectra, bool half_spectra, int64_t stream_stride,
+                                    int64_t seg_stride,
                                     int ch_base, int n_ch_run, int M, int64_t seg_base, int64_t n_segments,
                                     float scale, const double *offsets, double start_phase, double phase_coef,
                                     double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
@@ -2182,19 +2291,30 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const float2 *spectra, int64_
   per = (per + MF_KS - 1) / MF_KS * MF_KS;
   nz = (n_segments + per - 1) / per;
   dim3 grid((unsigned)nblk, 1, (unsigned)nz);
-  const float2 *sp = spectra + ch_base;
+  const void *sp = static_cast<const char *>(spectra) + (size_t)ch_base * (half_spectra ? 4 : 8);
   const double *off = offsets + ch_base;
+  if (half_spectra) {
+    if (precision == SFXG_PREC_FP16)
+      return launch_mfma_pulsar_t<SFXG_PREC_FP16, true>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT,
+                                                        CH, seg_base, n_segments, per, scale, off, start_phase,
+                                                        phase_coef, fft_duration, gate_begin, gate_end, nbins, gram);
+    if (precision == SFXG_PREC_BF16)
+      return launch_mfma_pulsar_t<SFXG_PREC_BF16, true>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT,
+                                                        CH, seg_base, n_segments, per, scale, off, start_phase,
+                                                        phase_coef, fft_duration, gate_begin, gate_end, nbins, gram);
+    return hipErrorInvalidValue;
+  }
   switch (precision) {
     case SFXG_PREC_FP16:
-      return launch_mfma_pulsar_t<SFXG_PREC_FP16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
+      return launch_mfma_pulsar_t<SFXG_PREC_FP16, false>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
                                                   seg_base, n_segments, per, scale, off, start_phase, phase_coef,
                                                   fft_duration, gate_begin, gate_end, nbins, gram);
     case SFXG_PREC_BF16:
-      return launch_mfma_pulsar_t<SFXG_PREC_BF16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
+      return launch_mfma_pulsar_t<SFXG_PREC_BF16, false>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH,
                                                   seg_base, n_segments, per, scale, off, start_phase, phase_coef,
                                                   fft_duration, gate_begin, gate_end, nbins, gram);
     case SFXG_PREC_SPLIT_FP16:
-      return launch_mfma_pulsar_t<SFXG_PREC_SPLIT_FP16>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M,
+      return launch_mfma_pulsar_t<SFXG_PREC_SPLIT_FP16, false>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M,
                                                         RT, CH, seg_base, n_segments, per, scale, off, start_phase,
                                                         phase_coef, fft_duration, gate_begin, gate_end, nbins, gram);
     default: return hipErrorInvalidValue;
