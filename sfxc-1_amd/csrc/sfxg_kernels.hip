@@ -570,6 +570,11 @@ __device__ __forceinline__ int opaque(int x) {
   asm volatile("" : "+v"(x));
   return x;
 }
+// The same for a wave-uniform value, kept in a scalar register.
+__device__ __forceinline__ int opaque_s(int x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
 
 // Padded LDS position of FFT point i for a P-point-per-thread transform.
 template <int P>
@@ -1312,14 +1317,18 @@ __global__ __launch_bounds__(256) void k_fold_combine(const float2 *__restrict__
 // conversion (exact; undone when the baselines are formed).
 //
 // Workgroup: 8 waves, CH channels, one sub-integration (grid z). Stages of
-// MF_KS segments are loaded with coalesced reads (consecutive threads take
+// KS segments are loaded with coalesced reads (consecutive threads take
 // consecutive channels, each thread 8 segments), converted, and stored into
 // LDS as Y[c][r][k] with 16-byte stores (a per-channel skew keeps them
-// conflict-free). Each wave owns up to MF_MAXP (channel, tile) pairs and keeps
-// their accumulators for the whole sub-integration. At its end the tiles of
-// each channel go through LDS and the workgroup forms that channel's
-// baseline visibilities into partial[z][b][ch]; k_fold_combine adds the
-// sub-integrations in fold order.
+// conflict-free). The loads of the next stage are issued into registers
+// before the current stage's MFMAs and stay in flight across them (the
+// barriers order LDS only), so HBM latency hides behind the matrix work.
+// Each wave owns up to MF_MAXP (channel, tile) pairs and keeps their
+// accumulators for the whole sub-integration. At its end the tiles of each
+// channel go through LDS and the workgroup forms that channel's baseline
+// visibilities into partial[z][ch][b] (channel-major, so the stores are
+// coalesced); k_fold_combine_cm adds the sub-integrations in fold order into
+// vis[b][ch].
 // ---------------------------------------------------------------------------
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
@@ -1330,6 +1339,15 @@ constexpr int MF_RS = MF_KS + 8;   // row stride in halves (rows 16-byte aligned
 constexpr int MF_MAXP = 10;        // (channel, tile) pairs per wave
 constexpr int MF_THREADS = 512;
 constexpr int MF_WAVES = MF_THREADS / 64;
+constexpr int MF_PF_WORDS = 16;    // registers per thread holding the next stage's loads
+
+// Workgroup barrier that orders LDS accesses only: global loads issued
+// before it stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // (channel, tile) pair p of a workgroup: channel c and the tile (ti, tj),
 // ti <= tj, of the upper triangle; t is the tile's index in row-major order
@@ -1467,38 +1485,158 @@ __device__ __forceinline__ void mf_mma(f16v_t &acc, const _Float16 *ylds, int oa
   }
 }
 
+// 32-bit words one load item (8 segments of one channel and stream) takes:
+// packed operand pairs, or FP32 real and imaginary parts.
+template <bool HS>
+constexpr int mf_item_words() { return HS ? 8 : 16; }
+
+// Workgroup index of a one-dimensional grid after the XCD regrouping: the
+// hardware deals workgroups to the 8 XCDs in turn, so workgroup i runs on
+// XCD i % 8. Groups of G consecutive logical indices (channel blocks that
+// share 128-byte lines) are given to one XCD, where they run together and
+// their L2 fetches each line once. A bijection on [0, n): the last
+// n % (8 G) workgroups keep their index.
+__device__ __forceinline__ int xcd_grouped(int i, int n, int G) {
+  const int full = n / (8 * G) * (8 * G);
+  if (G <= 1 || i >= full) return i;
+  const int xcd = i & 7, slot = i >> 3;
+  return (slot / G) * (8 * G) + xcd * G + slot % G;
+}
+
 // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16; segments per stage;
-// spectra in the operand type (HS) or FP32
+// spectra in the operand type (HS) or FP32. The launcher guarantees
+// CH * (KS / 8) * M <= MF_THREADS * MF_PF_WORDS / mf_item_words<HS>().
+// One-dimensional grid of n_blk channel blocks x n_k sub-integrations,
+// regrouped by XCD in groups of G channel blocks.
 template <int PREC, int KS, bool HS>
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
     const void *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch, int M,
-    int RT, int CH, const int32_t *__restrict__ fold_start, float scale, const int32_t *__restrict__ bl_first,
-    const int32_t *__restrict__ bl_second, int n_baselines, float inv_scale2, float2 *__restrict__ partial) {
+    int RT, int CH, int n_blk, int G, const int32_t *__restrict__ fold_start, float scale,
+    const int32_t *__restrict__ bl_first, const int32_t *__restrict__ bl_second, int n_baselines, float inv_scale2,
+    float2 *__restrict__ partial) {
   extern __shared__ _Float16 ylds[];
   constexpr int planes = PREC == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
+  constexpr int W = mf_item_words<HS>(), IPT = MF_PF_WORDS / W;
   const int R = 32 * RT;
   const int nT = RT * (RT + 1) / 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ch0 = blockIdx.x * CH;
-  const int z = blockIdx.z;
+  const int L = xcd_grouped(blockIdx.x, gridDim.x, G);
+  const int ch0 = (L % n_blk) * CH;
+  const int z = L / n_blk;
   const int64_t s_begin = fold_start[z], s_end = fold_start[z + 1];
   const int chan_stride = R * (KS + 8) + 8;  // halves, skewed by 16 bytes per channel
   const int plane_elems = CH * chan_stride;
+  // this thread's load items j: channel c, segment group kg, stream st
+  // (element offset from the stage's first row, which the launcher keeps
+  // below 2^32; LDS offset of the real part's row times 64 plus the first
+  // segment in the stage, KS for an item that loads nothing; negative for
+  // an item past the end)
+  const int n_items = CH * (KS / 8) * M;
+  uint32_t goff[IPT];
+  int lk[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {
+    const int item = threadIdx.x + j * MF_THREADS;
+    const int c = item % CH, rest = item / CH;
+    const int kg = rest % (KS / 8), st = rest / (KS / 8);
+    goff[j] = (uint32_t)(st * stream_stride + (int64_t)kg * 8 * seg_stride + ch0 + c);
+    lk[j] = item < n_items ? (c * chan_stride + st * (KS + 8) + kg * 8) * 64 + (ch0 + c < n_ch ? kg * 8 : KS) : -1;
+  }
   for (int i = threadIdx.x; i < planes * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
   f16v_t acc[MF_MAXP];
 #pragma unroll
   for (int q = 0; q < MF_MAXP; q++)
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
-  __syncthreads();
+  uint32_t pf[IPT][W];
+  // global loads of the stage at s0 into pf (zeros past s_end and n_ch)
+  auto fetch = [&](int64_t s0) {
+    const int nv = (int)(s_end - s0 < KS ? s_end - s0 : KS);
+    const int64_t row = (s0 - seg_base) * seg_stride;
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      const int kg8 = lk[j] < 0 ? KS : lk[j] & 63;
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) {
+        const bool ok = kg8 + kk < nv;
+        const int64_t idx = row + goff[j] + kk * seg_stride;
+        if constexpr (HS) {
+          pf[j][kk] = ok ? gmem(static_cast<const uint32_t *>(spectra))[idx] : 0u;
+        } else {
+          const float2 v = ok ? gload2(static_cast<const float2 *>(spectra) + idx) : make_float2(0.0f, 0.0f);
+          pf[j][2 * kk] = __float_as_uint(v.x);
+          pf[j][2 * kk + 1] = __float_as_uint(v.y);
+        }
+      }
+    }
+  };
+  // pf into the LDS planes, converted to the operand type
+  auto put = [&]() {
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+      if (lk[j] < 0) continue;
+      const int o_re = lk[j] >> 6, o_im = o_re + M * (KS + 8);
+      if constexpr (HS) {
+        static_assert(PREC != SFXG_PREC_SPLIT_FP16, "split FP16 reads FP32 spectra");
+        typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
+        u4_t r, i;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          r[e] = (pf[j][2 * e] & 0xffffu) | (pf[j][2 * e + 1] << 16);
+          i[e] = (pf[j][2 * e] >> 16) | (pf[j][2 * e + 1] & 0xffff0000u);
+        }
+        *reinterpret_cast<u4_t *>(ylds + o_re) = r;
+        *reinterpret_cast<u4_t *>(ylds + o_im) = i;
+      } else {
+        float re[8], im[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          re[kk] = __uint_as_float(pf[j][2 * kk]) * scale;
+          im[kk] = __uint_as_float(pf[j][2 * kk + 1]) * scale;
+        }
+        if constexpr (PREC == SFXG_PREC_BF16) {
+          b8_t br, bi;
+#pragma unroll
+          for (int kk = 0; kk < 8; kk++) {
+            br[kk] = (__bf16)re[kk];
+            bi[kk] = (__bf16)im[kk];
+          }
+          *reinterpret_cast<b8_t *>(ylds + o_re) = br;
+          *reinterpret_cast<b8_t *>(ylds + o_im) = bi;
+        } else {
+          h8_t hr, hi_;
+#pragma unroll
+          for (int kk = 0; kk < 8; kk++) {
+            hr[kk] = (_Float16)re[kk];
+            hi_[kk] = (_Float16)im[kk];
+          }
+          *reinterpret_cast<h8_t *>(ylds + o_re) = hr;
+          *reinterpret_cast<h8_t *>(ylds + o_im) = hi_;
+          if constexpr (PREC == SFXG_PREC_SPLIT_FP16) {
+            h8_t lr, li;
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+              lr[kk] = (_Float16)(re[kk] - (float)hr[kk]);
+              li[kk] = (_Float16)(im[kk] - (float)hi_[kk]);
+            }
+            *reinterpret_cast<h8_t *>(ylds + plane_elems + o_re) = lr;
+            *reinterpret_cast<h8_t *>(ylds + plane_elems + o_im) = li;
+          }
+        }
+      }
+    }
+  };
   const int n_pairs = CH * nT;
+  if (s_begin < s_end) fetch(s_begin);
+  __syncthreads();  // LDS zeroed
   for (int64_t s0 = s_begin; s0 < s_end; s0 += KS) {
-    mf_load_stage<PREC, KS, HS>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0, s0, s_end, scale,
-                            chan_stride, plane_elems, ylds);
-    __syncthreads();
+    put();
+    lds_barrier();
+    if (s0 + KS < s_end) fetch(s0 + KS);
+    const int wv = opaque(wave);  // pair offsets recomputed per stage, not held in registers
 #pragma unroll
     for (int q = 0; q < MF_MAXP; q++) {
-      const int p = wave + MF_WAVES * q;
+      const int p = wv + MF_WAVES * q;
       if (p < n_pairs) {
         int c, t, ti, tj;
         mf_pair(p, nT, RT, c, t, ti, tj);
@@ -1509,7 +1647,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
         for (int kk = 0; kk < KS; kk += 16) mf_mma<PREC>(acc[q], ylds, oa + kk, ob + kk, plane_elems, 0xffu);
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   // baselines, one channel at a time through an R x R FP32 buffer in LDS
   float *g = reinterpret_cast<float *>(ylds);
@@ -1530,15 +1668,269 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
     }
     __syncthreads();
     if (ch0 + c < n_ch) {
+      float2 *out = partial + ((int64_t)z * n_ch + ch0 + c) * n_baselines;
       for (int b = threadIdx.x; b < n_baselines; b += MF_THREADS) {
         const int a = bl_first[b], d = bl_second[b];
         auto G = [&](int r, int s2) { return r <= s2 ? g[r * R + s2] : g[s2 * R + r]; };
         const float re = (G(a, d) + G(M + a, M + d)) * inv_scale2;
         const float im = (G(M + a, d) - G(a, M + d)) * inv_scale2;
-        partial[((int64_t)z * n_baselines + b) * n_ch + ch0 + c] = make_float2(re, im);
+        out[b] = make_float2(re, im);
       }
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_xstep_mfma_dma: the FP16 / BF16 X-step on packed spectra with the
+// spectra fetched by direct global-to-LDS loads (global_load_lds_dword), so
+// that two stages are in flight while a third is transformed and multiplied,
+// at no register cost. Stages are 16 segments. A stage's raw rows land in a
+// ring slot as 256-byte blocks (one load instruction each: 64 / CH rows of CH
+// channels), each block padded to 288 bytes so that the regrouping reads are
+// free of bank conflicts. Each wave issues exactly NPW block loads per stage
+// (blocks past the stage's count go to a dummy block), which lets it wait for
+// its own loads of a stage with a fixed vmcnt. The regrouping pass turns the
+// raw rows into the operand layout of k_xstep_mfma (zeros past n_ch and the
+// sub-integration's end); the MFMA work, the XCD grouping and the epilogue
+// are those of k_xstep_mfma.
+// ---------------------------------------------------------------------------
+constexpr int MFD_KS = 16;          // segments per stage
+constexpr int MFD_BLK = 288;        // bytes per raw block: 256 of data + 32 of skew
+constexpr int MFD_SLOTS = 3;        // ring slots
+
+// LDS bytes of the DMA X-step: ring, operand stage, dummy block; the
+// epilogue's R x R FP32 buffer reuses the ring.
+__host__ __device__ constexpr size_t mfd_lds_bytes(int npw, int CH, int R) {
+  return (size_t)MFD_SLOTS * npw * MF_WAVES * MFD_BLK + (size_t)CH * (R * (MFD_KS + 8) + 8) * 2 + 256;
+}
+
+// Workgroup barrier over LDS, with the waits written out: the compiler's
+// own LDS fence would also wait for every direct-to-LDS load in flight.
+__device__ __forceinline__ void dma_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+template <int PREC, int NPW, int CH>
+__global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
+    const uint32_t *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch,
+    int M, int RT, int n_blk, int G, const int32_t *__restrict__ fold_start, const int32_t *__restrict__ bl_first,
+    const int32_t *__restrict__ bl_second, int n_baselines, float inv_scale2, float2 *__restrict__ partial) {
+  static_assert(PREC == SFXG_PREC_FP16 || PREC == SFXG_PREC_BF16, "packed operand pairs");
+  static_assert(CH == 8 || CH == 16 || CH == 32 || CH == 64, "channels per workgroup");
+  constexpr int KS = MFD_KS;
+  constexpr int RPI = 64 / CH;                  // rows per block
+  constexpr int PARTS = CH / 8;                 // blocks per group of 8 rows
+  constexpr int SLOT = NPW * MF_WAVES * MFD_BLK;
+  extern __shared__ __attribute__((aligned(16))) char dlds[];
+  const int R = 32 * RT;
+  const int nT = RT * (RT + 1) / 2;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int L = xcd_grouped(blockIdx.x, gridDim.x, G);
+  const int ch0 = (L % n_blk) * CH;
+  const int z = L / n_blk;
+  const int64_t s_begin = fold_start[z], s_end = fold_start[z + 1];
+  const int chan_stride = R * (KS + 8) + 8;  // halves
+  char *ring = dlds;
+  _Float16 *ylds = reinterpret_cast<_Float16 *>(dlds + MFD_SLOTS * SLOT);
+  char *dummy = dlds + MFD_SLOTS * SLOT + CH * chan_stride * 2;
+  const int n_blocks = M * (KS / 8) * PARTS;  // real blocks per stage
+  const bool all_ch = ch0 + CH <= n_ch;
+  // lane part of every block load: row lane / CH of the block, channel
+  // lane % CH (clamped into the row; the regrouping masks it)
+  const int lc = ch0 + lane % CH < n_ch ? ch0 + lane % CH : n_ch - 1;
+  const int lr = lane / CH;
+  const uint32_t lane_off = (uint32_t)(lr * seg_stride + lc);
+  const uint32_t lane_boff = lane_off * 4u;  // bytes
+  // issue this wave's NPW block loads of the stage at s0 into ring slot sl:
+  // block i = wave + 8 q is part i % PARTS of group g = (st, kg); a block
+  // past the stage's count reads stream 0 into the dummy block. In a full
+  // stage, block q of the wave starts 4 q / PARTS streams after block 0 (for
+  // PARTS <= 4), so its wave-uniform base is one scalar add away; rows past
+  // s_end (the last stage only) read the sub-integration's last row.
+  auto issue = [&](int64_t s0, int sl) {
+    const int64_t row0 = s0 - seg_base;
+    const int wv = opaque_s(wave);  // per-block bases recomputed per stage, not held in registers
+    char *dst0 = ring + sl * SLOT;
+    auto dma = [&](const void *src, char *dst) {
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                       (__attribute__((address_space(3))) void *)dst, 4, 0, 0);
+    };
+    if (s0 + KS <= s_end) {
+      const int g0 = wv / PARTS, part0 = wv % PARTS;
+      const char *b0 = reinterpret_cast<const char *>(spectra + (g0 / 2) * stream_stride +
+                                                      (row0 + (g0 % 2) * 8 + part0 * RPI) * seg_stride);
+#pragma unroll
+      for (int q = 0; q < NPW; q++) {
+        const int i = wv + MF_WAVES * q;
+        const char *bq;
+        if constexpr (PARTS <= 4) {
+          bq = b0 + (int64_t)q * (4 / PARTS) * stream_stride * 4;
+        } else {  // PARTS == 8: block q is group q, part wave
+          bq = reinterpret_cast<const char *>(spectra + (q / 2) * stream_stride +
+                                              (row0 + (q % 2) * 8 + wv * RPI) * seg_stride);
+        }
+        const bool real = i < n_blocks;
+        dma(real ? bq + lane_boff : reinterpret_cast<const char *>(spectra) + lane_boff % (CH * 4),
+            real ? dst0 + i * MFD_BLK : dummy);
+      }
+    } else {
+      const int64_t last = s_end - 1 - seg_base;
+#pragma unroll
+      for (int q = 0; q < NPW; q++) {
+        const int i0 = wv + MF_WAVES * q;
+        const bool real = i0 < n_blocks;
+        const int i = real ? i0 : wv;
+        const int g = i / PARTS, part = i % PARTS;
+        const int st = g / (KS / 8), kg = g % (KS / 8);
+        int64_t row = row0 + kg * 8 + part * RPI + lr;
+        row = row < last ? row : last;
+        dma(spectra + st * stream_stride + row * seg_stride + lc, real ? dst0 + i * MFD_BLK : dummy);
+      }
+    }
+  };
+  // raw slot sl into the operand planes: item (c, kg, st) takes the 8 rows
+  // of its block group, real halves to row st, imaginary to row M + st;
+  // zeros past n_ch and past s_end (last stage only)
+  const int n_items = CH * (KS / 8) * M;
+  auto regroup = [&](int64_t s0, int sl) {
+    const int nv = (int)(s_end - s0 < KS ? s_end - s0 : KS);
+    const bool full = all_ch && nv == KS;
+    const char *slot = ring + sl * SLOT;
+    for (int item = threadIdx.x; item < n_items; item += MF_THREADS) {
+      const int c = item % CH, rest = item / CH;
+      const int kg = rest % (KS / 8), st = rest / (KS / 8);
+      const char *src = slot + (st * (KS / 8) + kg) * PARTS * MFD_BLK + c * 4;
+      uint32_t w[8];
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++)
+        w[kk] = *reinterpret_cast<const uint32_t *>(src + (kk / RPI) * MFD_BLK + (kk % RPI) * (CH * 4));
+      if (!full) {
+        const bool cok = ch0 + c < n_ch;
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) w[kk] = cok && kg * 8 + kk < nv ? w[kk] : 0u;
+      }
+      typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
+      u4_t re, im;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        re[e] = __builtin_amdgcn_perm(w[2 * e + 1], w[2 * e], 0x05040100u);  // low halves
+        im[e] = __builtin_amdgcn_perm(w[2 * e + 1], w[2 * e], 0x07060302u);  // high halves
+      }
+      const int o_re = c * chan_stride + st * (KS + 8) + kg * 8;
+      *reinterpret_cast<u4_t *>(ylds + o_re) = re;
+      *reinterpret_cast<u4_t *>(ylds + o_re + M * (KS + 8)) = im;
+    }
+  };
+  // padding rows (2M .. R) stay zero
+  for (int i = threadIdx.x; i < CH * chan_stride; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
+  f16v_t acc[MF_MAXP];
+#pragma unroll
+  for (int q = 0; q < MF_MAXP; q++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
+  // the wave's (channel, tile) pairs: operand offsets, wave-uniform part
+  const int n_pairs = CH * nT;
+  const int lane_ab = (lane & 31) * (KS + 8) + 8 * (lane >> 5);
+  int pa[MF_MAXP], pb[MF_MAXP];
+#pragma unroll
+  for (int q = 0; q < MF_MAXP; q++) {
+    const int p = wave + MF_WAVES * q;
+    pa[q] = pb[q] = -1;
+    if (p < n_pairs) {
+      int c, t, ti, tj;
+      mf_pair(p, nT, RT, c, t, ti, tj);
+      pa[q] = c * chan_stride + ti * 32 * (KS + 8);
+      pb[q] = c * chan_stride + tj * 32 * (KS + 8);
+    }
+  }
+  const int n_stages = (int)((s_end - s_begin + KS - 1) / KS);
+  if (n_stages > 0) issue(s_begin, 0);
+  if (n_stages > 1) issue(s_begin + KS, 1);
+  for (int k = 0; k < n_stages; k++) {
+    const int64_t s0 = s_begin + (int64_t)k * KS;
+    // stage k + 2 into the slot stage k - 1 used (regrouped before the
+    // previous iteration's second barrier)
+    if (k + 2 < n_stages) {
+      issue(s0 + 2 * KS, (k + 2) % MFD_SLOTS);
+      wait_vmcnt<2 * NPW>();
+    } else if (k + 1 < n_stages) {
+      wait_vmcnt<NPW>();
+    } else {
+      wait_vmcnt<0>();
+    }
+    dma_barrier();  // every wave's blocks of stage k are in LDS; the previous MFMAs are done
+    regroup(s0, k % MFD_SLOTS);
+    dma_barrier();
+#pragma unroll
+    for (int q = 0; q < MF_MAXP; q++)
+      if (pa[q] >= 0) mf_mma<PREC>(acc[q], ylds, pa[q] + lane_ab, pb[q] + lane_ab, 0, 0xffu);
+  }
+  __syncthreads();
+  // baselines, one channel at a time through an R x R FP32 buffer in LDS
+  float *gb = reinterpret_cast<float *>(dlds);
+  for (int c = 0; c < CH; c++) {
+#pragma unroll
+    for (int q = 0; q < MF_MAXP; q++) {
+      const int p = wave + MF_WAVES * q;
+      if (p < n_pairs && p / nT == c) {
+        int cc, t, ti, tj;
+        mf_pair(p, nT, RT, cc, t, ti, tj);
+        const int col = tj * 32 + (lane & 31);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) gb[(ti * 32 + 8 * i + 4 * (lane >> 5) + j) * R + col] = acc[q][4 * i + j];
+      }
+    }
+    __syncthreads();
+    if (ch0 + c < n_ch) {
+      float2 *out = partial + ((int64_t)z * n_ch + ch0 + c) * n_baselines;
+      for (int b = threadIdx.x; b < n_baselines; b += MF_THREADS) {
+        const int a = bl_first[b], d = bl_second[b];
+        auto Gm = [&](int r, int s2) { return r <= s2 ? gb[r * R + s2] : gb[s2 * R + r]; };
+        const float re = (Gm(a, d) + Gm(M + a, M + d)) * inv_scale2;
+        const float im = (Gm(M + a, d) - Gm(a, M + d)) * inv_scale2;
+        out[b] = make_float2(re, im);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// vis[b][ch] += partial[k][ch][b] for k = 0 .. n_k - 1 in order: the
+// channel-major partials of the MFMA X-step, transposed through LDS in
+// 64 x 64 tiles so that both sides are read and written coalesced.
+__global__ __launch_bounds__(256) void k_fold_combine_cm(const float2 *__restrict__ partial, int n_k, int n_ch,
+                                                         int n_baselines, float2 *__restrict__ vis) {
+#pragma clang fp contract(off)
+  __shared__ float2 tile[64][65];  // [baseline][channel]
+  const int b0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int b = b0 + r, c = c0 + tx;
+    tile[r][tx] = b < n_baselines && c < n_ch ? vis[(int64_t)b * n_ch + c] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const int c = c0 + r, b = b0 + tx;
+    if (b < n_baselines && c < n_ch) {
+      float2 t = tile[tx][r];
+      for (int k = 0; k < n_k; k++) {
+        const float2 p = partial[((int64_t)k * n_ch + c) * n_baselines + b];
+        t.x = t.x + p.x;
+        t.y = t.y + p.y;
+      }
+      tile[tx][r] = t;
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const int b = b0 + r, c = c0 + tx;
+    if (b < n_baselines && c < n_ch) vis[(int64_t)b * n_ch + c] = tile[r][tx];
   }
 }
 
@@ -2141,14 +2533,15 @@ hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64
 
 template <int PREC, int KS, bool HS>
 static hipError_t launch_mfma_ks(dim3 grid, size_t lds, hipStream_t s, const void *spectra, int64_t stream_stride,
-                                 int64_t seg_stride, int64_t seg_base, int n_ch, int M, int RT, int CH,
-                                 const int32_t *fold_start, float scale, const int32_t *bl_first,
+                                 int64_t seg_stride, int64_t seg_base, int n_ch, int M, int RT, int CH, int n_blk,
+                                 int G, const int32_t *fold_start, float scale, const int32_t *bl_first,
                                  const int32_t *bl_second, int n_baselines, float inv2, float2 *partial) {
   hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma<PREC, KS, HS>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_xstep_mfma<PREC, KS, HS>), grid, dim3(MF_THREADS), lds, s, spectra, stream_stride, seg_stride,
-                     seg_base, n_ch, M, RT, CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial);
+                     seg_base, n_ch, M, RT, CH, n_blk, G, fold_start, scale, bl_first, bl_second, n_baselines, inv2,
+                     partial);
   return hipGetLastError();
 }
 
@@ -2162,6 +2555,61 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
   const int nT = RT * (RT + 1) / 2;
   const int planes = precision == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
   if (nT > MF_WAVES * MF_MAXP) return hipErrorInvalidValue;
+  if (half_spectra && t.mf_ks != 32 && stream_stride * M < ((int64_t)1 << 32)) {
+    // direct-to-LDS form: CH a power of two from 8 to 64 whose (channel,
+    // tile) pairs fit the waves, NPW = 8 or 16 block loads per wave and
+    // stage, and the LDS budget
+    const int max_pairs_ch = (MF_WAVES * MF_MAXP) / nT;
+    int CH = 64;
+    auto npw_of = [&](int ch) { return (M * (MFD_KS / 8) * (ch / 8) + MF_WAVES - 1) / MF_WAVES; };
+    auto lds_of = [&](int ch) {
+      const int npw = npw_of(ch) <= 8 ? 8 : 16;
+      return std::max(mfd_lds_bytes(npw, ch, R), (size_t)R * R * 4);
+    };
+    while (CH >= 8 && (CH > max_pairs_ch || npw_of(CH) > 16 || lds_of(CH) > 160 * 1024 ||
+                       (t.mf_channels > 0 && CH > t.mf_channels)))
+      CH /= 2;
+    if (CH >= 8) {
+      const int npw = npw_of(CH) <= 8 ? 8 : 16;
+      const size_t lds = lds_of(CH);
+      const int n_blk = (n_ch + CH - 1) / CH;
+      const int G = std::max(1, 128 / (CH * 4));
+      dim3 grid((unsigned)(n_blk * n_k));
+      const float inv2 = 1.0f / (scale * scale);
+      const uint32_t *sp = static_cast<const uint32_t *>(spectra);
+      hipError_t e = hipErrorInvalidValue;
+#define SFXG_MFD3(P, W, C)                                                                                         \
+  do {                                                                                                             \
+    e = hipFuncSetAttribute((const void *)k_xstep_mfma_dma<P, W, C>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                            (int)lds);                                                                             \
+    if (e != hipSuccess) return e;                                                                                 \
+    hipLaunchKernelGGL((k_xstep_mfma_dma<P, W, C>), grid, dim3(MF_THREADS), lds, s, sp, stream_stride,            \
+                       seg_stride, seg_base, n_ch, M, RT, n_blk, G, fold_start, bl_first, bl_second, n_baselines,  \
+                       inv2, partial);                                                                             \
+    e = hipGetLastError();                                                                                         \
+  } while (0)
+#define SFXG_MFD(P, W)                   \
+  do {                                   \
+    if (CH == 8) SFXG_MFD3(P, W, 8);     \
+    else if (CH == 16) SFXG_MFD3(P, W, 16); \
+    else if (CH == 32) SFXG_MFD3(P, W, 32); \
+    else SFXG_MFD3(P, W, 64);            \
+  } while (0)
+      if (precision == SFXG_PREC_FP16) {
+        if (npw == 8) SFXG_MFD(SFXG_PREC_FP16, 8);
+        else SFXG_MFD(SFXG_PREC_FP16, 16);
+      } else if (precision == SFXG_PREC_BF16) {
+        if (npw == 8) SFXG_MFD(SFXG_PREC_BF16, 8);
+        else SFXG_MFD(SFXG_PREC_BF16, 16);
+      }
+#undef SFXG_MFD
+#undef SFXG_MFD3
+      if (e != hipSuccess) return e;
+      dim3 gc((unsigned)((n_baselines + 63) / 64), (unsigned)((n_ch + 63) / 64));
+      hipLaunchKernelGGL(k_fold_combine_cm, gc, dim3(256), 0, s, partial, n_k, n_ch, n_baselines, vis);
+      return hipGetLastError();
+    }
+  }
   // channels per workgroup: all (channel, tile) pairs fit MF_WAVES waves x
   // MF_MAXP and the stage fits 150 KiB of LDS. Stages of 32 segments (two
   // MFMA K steps per barrier) where that keeps the channel count, else 16.
@@ -2176,20 +2624,33 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
     while (ch > 1 && lds_of(ch, ks) > 150 * 1024) ch--;
     return ch;
   };
-  int KS = t.mf_ks > 0 ? t.mf_ks : (fit(32) >= fit(16) ? 32 : 16);
-  int CH = fit(KS);
+  // and a stage's loads fit the prefetch registers
+  const int words = half_spectra ? 8 : 16;
+  auto fit_pf = [&](int ks) {
+    int ch = fit(ks);
+    while (ch > 1 && ch * (ks / 8) * M * words > MF_THREADS * MF_PF_WORDS) ch--;
+    return ch;
+  };
+  int KS = t.mf_ks > 0 ? t.mf_ks : (fit_pf(32) >= fit_pf(16) ? 32 : 16);
+  int CH = fit_pf(KS);
   if (t.mf_channels > 0 && CH > t.mf_channels) CH = t.mf_channels;
-  if (CH < 1) return hipErrorInvalidValue;
+  if (CH < 1 || CH * (KS / 8) * M * words > MF_THREADS * MF_PF_WORDS) return hipErrorInvalidValue;
+  if (stream_stride * M >= ((int64_t)1 << 32)) return hipErrorInvalidValue;  // 32-bit item offsets
   const size_t lds = lds_of(CH, KS);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((n_ch + CH - 1) / CH), 1, (unsigned)n_k);
+  const int n_blk = (n_ch + CH - 1) / CH;
+  // channel blocks that share a 128-byte line of a spectra row
+  const int G = std::max(1, 128 / (CH * (half_spectra ? 4 : 8)));
+  dim3 grid((unsigned)(n_blk * n_k));
   const float inv2 = 1.0f / (scale * scale);
   hipError_t e = hipErrorInvalidValue;
 #define SFXG_MFMA_CASE(P, H)                                                                                     \
   e = KS == 32 ? launch_mfma_ks<P, 32, H>(grid, lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch, M, RT, \
-                                          CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial) \
+                                          CH, n_blk, G, fold_start, scale, bl_first, bl_second, n_baselines, inv2,  \
+                                          partial)                                                                  \
                : launch_mfma_ks<P, 16, H>(grid, lds, s, spectra, stream_stride, seg_stride, seg_base, n_ch, M, RT, \
-                                          CH, fold_start, scale, bl_first, bl_second, n_baselines, inv2, partial);
+                                          CH, n_blk, G, fold_start, scale, bl_first, bl_second, n_baselines, inv2,  \
+                                          partial);
   if (half_spectra) {
     if (precision == SFXG_PREC_FP16) SFXG_MFMA_CASE(SFXG_PREC_FP16, true)
     else if (precision == SFXG_PREC_BF16) SFXG_MFMA_CASE(SFXG_PREC_BF16, true)
@@ -2202,8 +2663,8 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
   }
 #undef SFXG_MFMA_CASE
   if (e != hipSuccess) return e;
-  const int64_t n = (int64_t)n_baselines * n_ch;
-  hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
+  dim3 gc((unsigned)((n_baselines + 63) / 64), (unsigned)((n_ch + 63) / 64));
+  hipLaunchKernelGGL(k_fold_combine_cm, gc, dim3(256), 0, s, partial, n_k, n_ch, n_baselines, vis);
   return hipGetLastError();
 }
 
