@@ -1683,26 +1683,32 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
 
 // ---------------------------------------------------------------------------
 // k_xstep_mfma_dma: the FP16 / BF16 X-step on packed spectra with the
-// spectra fetched by direct global-to-LDS loads (global_load_lds_dword), so
-// that two stages are in flight while a third is transformed and multiplied,
-// at no register cost. Stages are 16 segments. A stage's raw rows land in a
-// ring slot as 256-byte blocks (one load instruction each: 64 / CH rows of CH
-// channels), each block padded to 288 bytes so that the regrouping reads are
-// free of bank conflicts. Each wave issues exactly NPW block loads per stage
-// (blocks past the stage's count go to a dummy block), which lets it wait for
-// its own loads of a stage with a fixed vmcnt. The regrouping pass turns the
-// raw rows into the operand layout of k_xstep_mfma (zeros past n_ch and the
+// spectra fetched by direct global-to-LDS loads (global_load_lds_dwordx4:
+// 16 bytes, four channels, per lane), so that two stages are in flight while
+// a third is regrouped and multiplied, at no register cost. Stages are 16
+// segments. A stage's M x 16 rows (stream-major) land in a ring slot in
+// 1 KiB blocks, one load instruction each (256 / CH rows of CH channels),
+// padded to mfd_blk(CH) bytes so that the regrouping reads of a wave fall in
+// distinct banks. Each wave issues exactly NPW block loads per stage (blocks
+// past the stage's count go to a dummy block), which lets it wait for its
+// own loads of a stage with a fixed vmcnt. The regrouping pass turns the raw
+// rows into the operand layout of k_xstep_mfma (zeros past n_ch and past the
 // sub-integration's end); the MFMA work, the XCD grouping and the epilogue
 // are those of k_xstep_mfma.
 // ---------------------------------------------------------------------------
 constexpr int MFD_KS = 16;          // segments per stage
-constexpr int MFD_BLK = 288;        // bytes per raw block: 256 of data + 32 of skew
 constexpr int MFD_SLOTS = 3;        // ring slots
+
+// Bytes per raw block of CH-channel rows: 1 KiB of data, then 4 CH bytes of
+// skew so that consecutive blocks start CH banks apart.
+__host__ __device__ constexpr int mfd_blk(int CH) { return 1024 + (CH < 64 ? 4 * CH : 0); }
+// Block loads per stage for M streams.
+__host__ __device__ constexpr int mfd_blocks(int M, int CH) { return (M * MFD_KS * CH + 255) / 256; }
 
 // LDS bytes of the DMA X-step: ring, operand stage, dummy block; the
 // epilogue's R x R FP32 buffer reuses the ring.
 __host__ __device__ constexpr size_t mfd_lds_bytes(int npw, int CH, int R) {
-  return (size_t)MFD_SLOTS * npw * MF_WAVES * MFD_BLK + (size_t)CH * (R * (MFD_KS + 8) + 8) * 2 + 256;
+  return (size_t)MFD_SLOTS * npw * MF_WAVES * mfd_blk(CH) + (size_t)CH * (R * (MFD_KS + 8) + 8) * 2 + 1024;
 }
 
 // Workgroup barrier over LDS, with the waits written out: the compiler's
@@ -1722,9 +1728,10 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
   static_assert(PREC == SFXG_PREC_FP16 || PREC == SFXG_PREC_BF16, "packed operand pairs");
   static_assert(CH == 8 || CH == 16 || CH == 32 || CH == 64, "channels per workgroup");
   constexpr int KS = MFD_KS;
-  constexpr int RPI = 64 / CH;                  // rows per block
-  constexpr int PARTS = CH / 8;                 // blocks per group of 8 rows
-  constexpr int SLOT = NPW * MF_WAVES * MFD_BLK;
+  constexpr int CQ = CH / 4;                 // lanes per row
+  constexpr int RPX = 64 / CQ;               // rows per block
+  constexpr int BLK = mfd_blk(CH);
+  constexpr int SLOT = NPW * MF_WAVES * BLK;
   extern __shared__ __attribute__((aligned(16))) char dlds[];
   const int R = 32 * RT;
   const int nT = RT * (RT + 1) / 2;
@@ -1737,77 +1744,85 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
   char *ring = dlds;
   _Float16 *ylds = reinterpret_cast<_Float16 *>(dlds + MFD_SLOTS * SLOT);
   char *dummy = dlds + MFD_SLOTS * SLOT + CH * chan_stride * 2;
-  const int n_blocks = M * (KS / 8) * PARTS;  // real blocks per stage
+  const int NI = mfd_blocks(M, CH);             // real blocks per stage
+  const bool regular = NI * RPX == M * KS;      // every block inside the M streams
   const bool all_ch = ch0 + CH <= n_ch;
-  // lane part of every block load: row lane / CH of the block, channel
-  // lane % CH (clamped into the row; the regrouping masks it)
-  const int lc = ch0 + lane % CH < n_ch ? ch0 + lane % CH : n_ch - 1;
-  const int lr = lane / CH;
-  const uint32_t lane_off = (uint32_t)(lr * seg_stride + lc);
-  const uint32_t lane_boff = lane_off * 4u;  // bytes
+  // lane part of a block load: row r = lane / CQ of the block (stream
+  // r / 16, segment r % 16 from the block's first row), channels
+  // ch0 + 4 (lane % CQ) .. + 3, in bytes. Rows hold S >= n_blk CH bins, so
+  // the four channels stay inside the row (bins past n_ch are masked later).
+  const int lr = lane / CQ, lq = lane % CQ;
+  const uint32_t lane_boff =
+      (uint32_t)(((lr / KS) * stream_stride + (lr % KS) * seg_stride + ch0 + 4 * lq) * 4);
   // issue this wave's NPW block loads of the stage at s0 into ring slot sl:
-  // block i = wave + 8 q is part i % PARTS of group g = (st, kg); a block
-  // past the stage's count reads stream 0 into the dummy block. In a full
-  // stage, block q of the wave starts 4 q / PARTS streams after block 0 (for
-  // PARTS <= 4), so its wave-uniform base is one scalar add away; rows past
-  // s_end (the last stage only) read the sub-integration's last row.
+  // block j = wave + 8 q holds rows j RPX .. j RPX + RPX - 1 of the stage;
+  // a block past the stage's count reads the buffer's first KiB into the
+  // dummy block. In a full stage of a regular shape, block q of the wave
+  // starts RPX / 2 q streams after block 0, so its wave-uniform base is one
+  // scalar add away; otherwise (the last stage, or an odd stream count with
+  // 32-row blocks) every lane clamps its stream and row.
+  auto dma = [](const void *src, char *dst) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                     (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+  };
   auto issue = [&](int64_t s0, int sl) {
     const int64_t row0 = s0 - seg_base;
     const int wv = opaque_s(wave);  // per-block bases recomputed per stage, not held in registers
     char *dst0 = ring + sl * SLOT;
-    auto dma = [&](const void *src, char *dst) {
-      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
-                                       (__attribute__((address_space(3))) void *)dst, 4, 0, 0);
-    };
-    if (s0 + KS <= s_end) {
-      const int g0 = wv / PARTS, part0 = wv % PARTS;
-      const char *b0 = reinterpret_cast<const char *>(spectra + (g0 / 2) * stream_stride +
-                                                      (row0 + (g0 % 2) * 8 + part0 * RPI) * seg_stride);
+    const char *dsrc = reinterpret_cast<const char *>(spectra) + lane * 16;
+    if (regular && s0 + KS <= s_end) {
+      const int rj = wv * RPX;
+      const char *b0 = reinterpret_cast<const char *>(spectra + (rj / KS) * stream_stride +
+                                                      (row0 + rj % KS) * seg_stride);
 #pragma unroll
       for (int q = 0; q < NPW; q++) {
-        const int i = wv + MF_WAVES * q;
-        const char *bq;
-        if constexpr (PARTS <= 4) {
-          bq = b0 + (int64_t)q * (4 / PARTS) * stream_stride * 4;
-        } else {  // PARTS == 8: block q is group q, part wave
-          bq = reinterpret_cast<const char *>(spectra + (q / 2) * stream_stride +
-                                              (row0 + (q % 2) * 8 + wv * RPI) * seg_stride);
-        }
-        const bool real = i < n_blocks;
-        dma(real ? bq + lane_boff : reinterpret_cast<const char *>(spectra) + lane_boff % (CH * 4),
-            real ? dst0 + i * MFD_BLK : dummy);
+        const int j = wv + MF_WAVES * q;
+        const char *bq = b0 + (int64_t)q * (RPX / 2) * stream_stride * 4;
+        const bool real = j < NI;
+        dma(real ? bq + lane_boff : dsrc, real ? dst0 + j * BLK : dummy);
       }
     } else {
       const int64_t last = s_end - 1 - seg_base;
 #pragma unroll
       for (int q = 0; q < NPW; q++) {
-        const int i0 = wv + MF_WAVES * q;
-        const bool real = i0 < n_blocks;
-        const int i = real ? i0 : wv;
-        const int g = i / PARTS, part = i % PARTS;
-        const int st = g / (KS / 8), kg = g % (KS / 8);
-        int64_t row = row0 + kg * 8 + part * RPI + lr;
+        const int j = wv + MF_WAVES * q;
+        const int rho = j * RPX + lr;
+        const int st = rho / KS < M ? rho / KS : M - 1;
+        int64_t row = row0 + rho % KS;
         row = row < last ? row : last;
-        dma(spectra + st * stream_stride + row * seg_stride + lc, real ? dst0 + i * MFD_BLK : dummy);
+        const bool real = j < NI;
+        dma(real ? spectra + st * stream_stride + row * seg_stride + ch0 + 4 * lq : (const uint32_t *)dsrc,
+            real ? dst0 + j * BLK : dummy);
       }
     }
   };
-  // raw slot sl into the operand planes: item (c, kg, st) takes the 8 rows
-  // of its block group, real halves to row st, imaginary to row M + st;
-  // zeros past n_ch and past s_end (last stage only)
-  const int n_items = CH * (KS / 8) * M;
+  // raw slot sl into the operand planes: item (c, group gm = 2 st + kg)
+  // takes the 8 rows of its group, real halves to row st, imaginary to row
+  // M + st; zeros past n_ch and past s_end (last stage only). Items are
+  // ordered channel first, then block, so that a wave's reads fall in
+  // distinct banks.
+  constexpr int GPI = RPX >= 8 ? RPX / 8 : 1;        // groups per block
+  const int NJ = (2 * M + GPI - 1) / GPI;            // blocks holding groups
+  const int n_items = CH * NJ * GPI;
   auto regroup = [&](int64_t s0, int sl) {
     const int nv = (int)(s_end - s0 < KS ? s_end - s0 : KS);
     const bool full = all_ch && nv == KS;
     const char *slot = ring + sl * SLOT;
     for (int item = threadIdx.x; item < n_items; item += MF_THREADS) {
       const int c = item % CH, rest = item / CH;
-      const int kg = rest % (KS / 8), st = rest / (KS / 8);
-      const char *src = slot + (st * (KS / 8) + kg) * PARTS * MFD_BLK + c * 4;
+      const int jj = rest % NJ, sub = rest / NJ;
+      const int gm = jj * GPI + sub;
+      if (gm >= 2 * M) continue;
+      const char *src;
+      if constexpr (RPX >= 8) src = slot + jj * BLK + sub * 8 * (CH * 4) + c * 4;
+      else src = slot + gm * (8 / RPX) * BLK + c * 4;  // a group spans 8 / RPX blocks
       uint32_t w[8];
 #pragma unroll
-      for (int kk = 0; kk < 8; kk++)
-        w[kk] = *reinterpret_cast<const uint32_t *>(src + (kk / RPI) * MFD_BLK + (kk % RPI) * (CH * 4));
+      for (int kk = 0; kk < 8; kk++) {
+        if constexpr (RPX >= 8) w[kk] = *reinterpret_cast<const uint32_t *>(src + kk * (CH * 4));
+        else w[kk] = *reinterpret_cast<const uint32_t *>(src + (kk / RPX) * BLK + (kk % RPX) * (CH * 4));
+      }
+      const int st = gm >> 1, kg = gm & 1;
       if (!full) {
         const bool cok = ch0 + c < n_ch;
 #pragma unroll
@@ -2557,20 +2572,20 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
   if (nT > MF_WAVES * MF_MAXP) return hipErrorInvalidValue;
   if (half_spectra && t.mf_ks != 32 && stream_stride * M < ((int64_t)1 << 32)) {
     // direct-to-LDS form: CH a power of two from 8 to 64 whose (channel,
-    // tile) pairs fit the waves, NPW = 8 or 16 block loads per wave and
-    // stage, and the LDS budget
+    // tile) pairs fit the waves, NPW = 1, 2, 4 or 8 block loads per wave
+    // and stage, and the LDS budget
     const int max_pairs_ch = (MF_WAVES * MF_MAXP) / nT;
-    int CH = 64;
-    auto npw_of = [&](int ch) { return (M * (MFD_KS / 8) * (ch / 8) + MF_WAVES - 1) / MF_WAVES; };
-    auto lds_of = [&](int ch) {
-      const int npw = npw_of(ch) <= 8 ? 8 : 16;
-      return std::max(mfd_lds_bytes(npw, ch, R), (size_t)R * R * 4);
+    auto npw_of = [&](int ch) {
+      const int n = (mfd_blocks(M, ch) + MF_WAVES - 1) / MF_WAVES;
+      return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : n <= 8 ? 8 : 0;
     };
-    while (CH >= 8 && (CH > max_pairs_ch || npw_of(CH) > 16 || lds_of(CH) > 160 * 1024 ||
+    auto lds_of = [&](int ch) { return std::max(mfd_lds_bytes(npw_of(ch), ch, R), (size_t)R * R * 4); };
+    int CH = 64;
+    while (CH >= 8 && (CH > max_pairs_ch || npw_of(CH) == 0 || lds_of(CH) > 160 * 1024 ||
                        (t.mf_channels > 0 && CH > t.mf_channels)))
       CH /= 2;
-    if (CH >= 8) {
-      const int npw = npw_of(CH) <= 8 ? 8 : 16;
+    if (CH >= 8 && stream_stride * 8 < ((int64_t)1 << 32)) {
+      const int npw = npw_of(CH);
       const size_t lds = lds_of(CH);
       const int n_blk = (n_ch + CH - 1) / CH;
       const int G = std::max(1, 128 / (CH * 4));
@@ -2595,13 +2610,16 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
     else if (CH == 32) SFXG_MFD3(P, W, 32); \
     else SFXG_MFD3(P, W, 64);            \
   } while (0)
-      if (precision == SFXG_PREC_FP16) {
-        if (npw == 8) SFXG_MFD(SFXG_PREC_FP16, 8);
-        else SFXG_MFD(SFXG_PREC_FP16, 16);
-      } else if (precision == SFXG_PREC_BF16) {
-        if (npw == 8) SFXG_MFD(SFXG_PREC_BF16, 8);
-        else SFXG_MFD(SFXG_PREC_BF16, 16);
-      }
+#define SFXG_MFDP(P)                          \
+  do {                                        \
+    if (npw == 1) SFXG_MFD(P, 1);             \
+    else if (npw == 2) SFXG_MFD(P, 2);        \
+    else if (npw == 4) SFXG_MFD(P, 4);        \
+    else SFXG_MFD(P, 8);                      \
+  } while (0)
+      if (precision == SFXG_PREC_FP16) SFXG_MFDP(SFXG_PREC_FP16);
+      else if (precision == SFXG_PREC_BF16) SFXG_MFDP(SFXG_PREC_BF16);
+#undef SFXG_MFDP
 #undef SFXG_MFD
 #undef SFXG_MFD3
       if (e != hipSuccess) return e;
