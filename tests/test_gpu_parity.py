@@ -427,6 +427,56 @@ def test_packed_spectra(ctx, nchan, prec, keep_ts):
     assert l2 <= MIXED_BARS[prec][0] and worst <= MIXED_BARS[prec][1]
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+def test_packed_unaligned_folds(ctx, prec):
+    """Sub-integrations of 453 segments start and end inside the four-segment
+    quads of packed spectra: the X-step masks the neighbouring
+    sub-integration's segments out of a shared quad. Unchunked (folds inside
+    one chunk) and chunked (one sub-integration per chunk) both meet the
+    mixed-precision bars against the FP64 oracle."""
+    job, hdrs = configs.build_slices(seed=43, n_stations=3, pols=["R", "L"], fs=16_000_000, bw=8_000_000,
+                                     nchan=1024, seconds=0.25, t_int_usec=250_000, sub_int_usec=29_000, rho=0.05,
+                                     bits=2, sky_freq=5_000_000_000, offsets=[0.0, 2e-6, -1e-6],
+                                     rates=[0.0, 1e-7, -2e-7])[0]
+    r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
+    heads_r, vis_r = vis_from_bytes(job, r64["output"])
+    try:
+        for chunk in (0, 1):
+            out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=prec, chunk_mib=chunk)
+            heads_g, vis_g = vis_from_bytes(job, out)
+            assert heads_g == heads_r
+            l2, worst = mixed_errors(vis_g[0], vis_r[0])
+            print(f"unaligned folds precision {prec} chunk_mib={chunk}: relative L2 {l2:.3e}, worst {worst:.3e}")
+            assert l2 <= MIXED_BARS[prec][0] and worst <= MIXED_BARS[prec][1]
+    finally:
+        ctx.set_options(lut=LUT_CONFIG, precision=0)
+
+
+@pytest.mark.parametrize("prec", [1, 3])
+def test_mixed_32_stations(ctx, prec):
+    """The 32-station shape of the mixed-precision X-step (64 streams, 2048
+    baselines: the full LDS budget of the direct-to-LDS form, ring slots past
+    64 KiB) against the FP64 oracle: FP16 on packed spectra and split FP16 on
+    FP32 spectra."""
+    n = 32
+    offsets = [1e-7 * ((7 * i) % 13 - 6) for i in range(n)]
+    rates = [1e-8 * ((5 * i) % 9 - 4) for i in range(n)]
+    job, hdrs = configs.build_slices(seed=44, n_stations=n, pols=["R", "L"], fs=8_000_000, bw=4_000_000, nchan=256,
+                                     seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=offsets, rates=rates)[0]
+    r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
+    heads_r, vis_r = vis_from_bytes(job, r64["output"])
+    try:
+        out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=prec)
+    finally:
+        ctx.set_options(lut=LUT_CONFIG, precision=0)
+    heads_g, vis_g = vis_from_bytes(job, out)
+    assert heads_g == heads_r
+    l2, worst = mixed_errors(vis_g[0], vis_r[0])
+    print(f"32 stations precision {prec}: relative L2 {l2:.3e}, worst channel {worst:.3e}")
+    assert l2 <= MIXED_BARS[prec][0] and worst <= MIXED_BARS[prec][1]
+
+
 def _multi_fold_job(nchan, seed=20, bits=2, window=4):
     """0.25 s slice with 31.25 ms sub-integrations: eight folds."""
     return configs.build_slices(seed=seed, n_stations=3, pols=["R"], fs=16_000_000, bw=8_000_000, nchan=nchan,
