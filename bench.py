@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--main-only", action="store_true", help="skip the mixed-precision and 32-station runs")
     ap.add_argument("--c2-steps", type=int, default=2, help="0.5 s integrations of the 32-station workload")
     ap.add_argument("--c2-precisions", default="fp32,split_fp16,fp16")
+    ap.add_argument("--c2-contexts", type=int, default=4,
+                    help="contexts (HIP streams) the 32-station slices alternate over (~18 GB of HBM each)")
     ap.add_argument("--only", choices=("all", "c1", "c2", "c3", "c4"), default="all",
                     help="c1: the main 8-station workload only; c2: the 32-station workload only; "
                          "c3: the mixed batch only; c4: the pulsar-binning workload only")
@@ -451,9 +453,10 @@ def main():
         print(f"32-station setup done in {w2.setup_s:.1f} s", file=sys.stderr, flush=True)
         c2 = {"workload": "32 stations x 2 pol, 4 subbands x 64 MHz at 128 Ms/s, 2-bit, 16384 channels, "
                           f"0.5 s integrations, {args.c2_steps * 0.5:g} s",
+              "contexts": args.c2_contexts,
               "station_samples_per_step": w2.station_samples_per_step}
         for name in args.c2_precisions.split(","):
-            wall, kt = w2.run(torch, capi, PREC[name], 0, args.contexts, dev, autotune=not args.no_autotune)
+            wall, kt = w2.run(torch, capi, PREC[name], 0, args.c2_contexts, dev, autotune=not args.no_autotune)
             r = summarise(w2, wall, kt, name)
             r.pop("_kb")
             if name != "fp32":

@@ -180,13 +180,17 @@ __global__ __launch_bounds__(256) void k_unpack(const DevStream *__restrict__ st
 
 // ---------------------------------------------------------------------------
 // k_stats: histograms of the counted payload bytes, split by tsys phase.
-// Each workgroup takes a contiguous span of counting ordinals of one stream
-// (consecutive threads on consecutive bytes), so that its LDS histogram is
-// flushed to HBM once per span. A thread finds its first byte's range and
-// tsys phase once, then advances both incrementally: ordinals are
-// contiguous across ranges, and the phase flips at tsys_count0 and every
-// tsys_half bytes after it.
+// Each workgroup takes a contiguous span of counting ordinals of one stream,
+// so that its LDS histogram is flushed to HBM once per span; each thread
+// takes 16 consecutive ordinals at a time (the workgroup 4096). A thread
+// finds its first byte's range and tsys phase once and advances both
+// incrementally: ordinals are contiguous across ranges, and the phase flips
+// at tsys_count0 and every tsys_half bytes after it. Where the 16 bytes lie
+// in one range and one phase (nearly always), all 16 loads are issued before
+// the first histogram update.
 // ---------------------------------------------------------------------------
+constexpr int STATS_RUN = 16;  // ordinals per thread and step
+
 __global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ streams, int64_t span) {
   __shared__ int hist[2][256];
   const DevStream &s = streams[blockIdx.y];
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ str
   }
   const int64_t g0 = (int64_t)blockIdx.x * span;
   const int64_t g1 = g0 + span < total ? g0 + span : total;
-  const int64_t gt = g0 + threadIdx.x;
+  const int64_t gt = g0 + (int64_t)threadIdx.x * STATS_RUN;
   if (gt < g1) {
     int64_t lo = 0, hi = n - 1;  // last range with ordinal <= gt
     while (lo < hi) {
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ str
       }
     }
     const uint8_t *pay = s.payload;
-    for (int64_t g = gt; g < g1; g += 256) {
+    auto advance = [&](int64_t g) {
       while (g >= next_ord) {
         r++;
         delta = s.counted[r].byte_start - s.counted[r].ordinal;
@@ -240,8 +244,23 @@ __global__ __launch_bounds__(256) void k_stats(const DevStream *__restrict__ str
         on ^= 1;
         flip = half > 0 ? flip + half : INT64_MAX;
       }
-      const uint8_t b = pay[delta + g];
-      atomicAdd(&hist[on ? 0 : 1][b], 1);
+    };
+    for (int64_t g = gt; g < g1; g += 256 * STATS_RUN) {
+      advance(g);
+      const int64_t ge = g + STATS_RUN;
+      if (ge <= g1 && ge <= next_ord && ge <= flip) {
+        uint8_t b[STATS_RUN];
+#pragma unroll
+        for (int i = 0; i < STATS_RUN; i++) b[i] = pay[delta + g + i];
+        int *h = hist[on ? 0 : 1];
+#pragma unroll
+        for (int i = 0; i < STATS_RUN; i++) atomicAdd(&h[b[i]], 1);
+      } else {
+        for (int64_t gi = g; gi < ge && gi < g1; gi++) {
+          advance(gi);
+          atomicAdd(&hist[on ? 0 : 1][pay[delta + gi]], 1);
+        }
+      }
     }
   }
   __syncthreads();
@@ -2280,7 +2299,8 @@ hipError_t launch_unpack(const DevStream *d_streams, int n_streams, int64_t n_sa
 
 hipError_t launch_stats(const DevStream *d_streams, int n_streams, int64_t max_counted_bytes, hipStream_t s) {
   if (max_counted_bytes <= 0 || n_streams <= 0) return hipSuccess;
-  // about 4096 workgroups over all streams, each over at least 16 KiB
+  // about 4096 workgroups over all streams, each over at least 16 KiB (a
+  // multiple of the 4096 ordinals a workgroup takes per step)
   int64_t span = max_counted_bytes * n_streams / 4096;
   span = std::max<int64_t>(16384, (span + 4095) / 4096 * 4096);
   dim3 grid((unsigned)((max_counted_bytes + span - 1) / span), n_streams);
