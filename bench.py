@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE counters scaled by a 1 GiB copy kernel's readings
 # (profiles/fengine_xstep_plain_fp32_8st_8sb_4096ch.md); re-measured whenever
 # the kernel changes.
-MEASURED_TRAFFIC = {("fengine", 4096, 16): 8.70e9}
+MEASURED_TRAFFIC = {("fengine", 4096, 16): 8.94e9}
 FP16_DENSE_PEAK_TFLOPS = 2500.0
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "split_fp16": 3}
