@@ -34,6 +34,12 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/fengine_xstep_plain_fp32_8st_8sb_4096ch.md); re-measured whenever
 # the kernel changes.
 MEASURED_TRAFFIC = {("fengine", 4096, 16): 8.94e9}
+# Wave-level VALU instructions per launch of the same kernel (SQ_INSTS_VALU,
+# same profile), against the wave64 FP32 issue rate: 256 CUs x 1 instruction
+# per cycle (a wave64 instruction occupies a 16-lane SIMD for 4 cycles, 4
+# SIMDs per CU) x 2.4 GHz.
+MEASURED_VALU_INSTS = {("fengine", 4096, 16): 2.776e9}
+VALU_ISSUE_PEAK = 256 * 2.4e9
 FP16_DENSE_PEAK_TFLOPS = 2500.0
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "split_fp16": 3}
@@ -414,6 +420,11 @@ def main():
         elif key in MEASURED_TRAFFIC and NSB == 8:
             roof["traffic"] = MEASURED_TRAFFIC[key]
             roof["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"
+        if key in MEASURED_VALU_INSTS and NSB == 8:
+            rate = MEASURED_VALU_INSTS[key] / (kt[dom] * 1e-3)
+            roof["valu"] = {"insts_per_launch": MEASURED_VALU_INSTS[key], "achieved_insts_per_s": round(rate, -8),
+                            "peak_insts_per_s": VALU_ISSUE_PEAK, "frac": round(rate / VALU_ISSUE_PEAK, 4),
+                            "source": "rocprofv3 SQ_INSTS_VALU"}
         line.update({
             "metric": "station-Gsamples/s correlated (plain FP32 GPU path)",
             "value": main_r["value"],
