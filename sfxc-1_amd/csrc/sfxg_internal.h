@@ -105,6 +105,8 @@ struct BlockConst {
   float amp;
   float ramp_step[2];     // cos, sin of 64 l_term
   float fringe_step[2];   // cos, sin of 64 dphi
+  float ramp_step16[2];   // cos, sin of 16 l_term
+  float fringe_step16[2]; // cos, sin of 16 dphi
   float pad;
 };
 

@@ -101,15 +101,22 @@ def test_c0_slice_stages(ctx, c0_short, lut):
 
 
 def test_fused_matches_unfused(ctx, c0_short):
-    """The fused F-engine (series in LDS) and the two-kernel path give the
-    same bytes."""
+    """The fused F-engine (series in LDS) and the two-kernel path agree to
+    FP32 rounding: they share the channeliser's FFT schedule, and the
+    unfused delay correction factors its 256-point transforms as 16 x 16
+    where the fused one uses four radix-4 stages. Headers are identical."""
     job, hdrs = c0_short[0]
     a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=True)
     sp_a = ctx.stage(capi.STAGE_SPECTRA, 1)
     b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=False)
     sp_b = ctx.stage(capi.STAGE_SPECTRA, 1)
-    np.testing.assert_array_equal(sp_a, sp_b)
-    assert a == b
+    err = np.max(np.abs(sp_a - sp_b)) / np.sqrt(np.mean(np.abs(sp_b.astype(np.complex128)) ** 2))
+    print("fused vs unfused spectra", err)
+    assert err < 1e-5  # each path is within 2e-5 of the oracle (test_c0_slice_stages); they differ by ~2e-6
+    heads_a, vis_a = vis_from_bytes(job, a)
+    heads_b, vis_b = vis_from_bytes(job, b)
+    assert heads_a == heads_b
+    compare_vis(vis_a[0], vis_b[0], label="fused vs unfused")
 
 
 def test_c0_all_slices(ctx, c0_short):
@@ -607,7 +614,7 @@ def test_tuned_launch_shapes(ctx, nchan, prec, pulsar, fields):
 
 def test_tuned_fused_matches_unfused(ctx):
     """With a tuned F-engine schedule the unfused channeliser follows it, and
-    both paths still give the same bytes."""
+    both paths still agree to FP32 rounding."""
     job, hdrs = configs.build_slices(seed=41, n_stations=2, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=1024,
                                      seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2, sky_freq=5_000_000_000,
                                      offsets=[0.0, 1e-6], rates=[0.0, 1e-7])[0]
@@ -621,8 +628,11 @@ def test_tuned_fused_matches_unfused(ctx):
         c = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
     finally:
         capi.tune_clear()
-    assert a == b
-    assert a != c  # another radix schedule: other rounding
+    heads_a, vis_a = vis_from_bytes(job, a)
+    heads_b, vis_b = vis_from_bytes(job, b)
+    assert heads_a == heads_b
+    compare_vis(vis_a[0], vis_b[0], label="tuned fused vs unfused")
+    assert b != c  # another radix schedule: other rounding
 
 
 def test_autotune_then_correlate(ctx):
