@@ -63,7 +63,10 @@ extern "C" {
                                      spectra are kept in that type (scaled by a power of two) and
                                      are returned converted back to float2 */
 #define SFXG_STAGE_ACCUMULATED 3  /* phase-centre / bin accumulators, float2[n_bins][B][N_c + 1] */
-#define SFXG_STAGE_NORMALISED 4   /* normalised visibilities, float2[n_bins][B][nchan + 1] */
+#define SFXG_STAGE_NORMALISED 4   /* normalised visibilities, float2[n_bins][B][N_c + 1] */
+#define SFXG_STAGE_CHANNELS 5     /* the channels integration_write stores, float2[n_bins][B][nchan + 1]:
+                                     the normalised visibilities when nchan == N_c, their lag
+                                     re-channelisation when nchan < N_c */
 
 typedef struct sfxg_ctx sfxg_ctx;
 
@@ -91,9 +94,11 @@ typedef struct {
   int64_t stop_time;
   int64_t integration_time;
   int64_t sub_integration_time;
-  int32_t number_channels;
-  int32_t fft_size_delaycor;
-  int32_t fft_size_correlation;
+  int32_t number_channels;       /* output channels: fft_size_correlation, or a power of two
+                                    >= 2 below it (fft_size_correlation <= 16384), which
+                                    integration_write re-channelises to (see sfxg_set_mask) */
+  int32_t fft_size_delaycor;      /* 256 */
+  int32_t fft_size_correlation;   /* N_c, a power of two in [256, 32768] */
   int32_t integration_nr;
   int32_t slice_nr;
   int32_t slice_offset;
@@ -174,6 +179,22 @@ int sfxg_set_options(sfxg_ctx *ctx, const sfxg_options *opt);
 int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params,
                         const sfxg_delay_model *models, const double *uvw,
                         const sfxg_pulsar *pulsar);
+
+/* Mask_parameters (include/control_parameters.h:107-114) of the job: replaces
+ * the correlator node's receipt of MPI_TAG_MASK_PARAMETERS
+ * (src/correlator_node_controller.cc:72-77), after which every slice's
+ * Correlation_parameters points at them (:55-62). They apply to slices with
+ * fft_size_correlation > number_channels, whose integration_write
+ * re-channelises the normalised visibilities in the lag domain
+ * (src/correlation_core.cc:417-440): normalize != 0 scales every bin to unit
+ * magnitude; mask (n_mask = fft_size_correlation + 1 values, or NULL / 0 for
+ * ones) multiplies the bins (create_mask, :811-824); window (n_window =
+ * 2 * number_channels values, or NULL / 0 for the lag window derived from the
+ * job's window function, create_window :761-809) weights the lags. The values
+ * are copied; sizes are checked against the slice at sfxg_prepare /
+ * sfxg_correlate. Call with (0, NULL, 0, NULL, 0) to return to the defaults. */
+int sfxg_set_mask(sfxg_ctx *ctx, int32_t normalize, const double *mask, int32_t n_mask, const double *window,
+                  int32_t n_window);
 
 /* Hands the complete HEADER byte stream of stream `stream_idx` (index into
  * station_streams) for this slice to the context. Replaces the reader thread

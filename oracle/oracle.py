@@ -50,6 +50,11 @@ class CoreParams(C.Structure):
                 ("bl_first", C.POINTER(C.c_int32)), ("bl_second", C.POINTER(C.c_int32))]
 
 
+class RechanParamsC(C.Structure):
+    _fields_ = [("fft_size", C.c_int32), ("nchan", C.c_int32), ("normalize", C.c_int32), ("pad", C.c_int32),
+                ("mask", C.POINTER(C.c_float)), ("window", C.POINTER(C.c_float))]
+
+
 class PulsarParamsC(C.Structure):
     _fields_ = [("nbins", C.c_int32), ("pad", C.c_int32), ("start_phase", C.c_double),
                 ("phase_coef", C.c_double), ("fft_duration", C.c_double),
@@ -349,6 +354,61 @@ def write_output(job, sourcenr, integration_slice, vis_f32, stats5, weights, uvw
     return dst[:n].tobytes()
 
 
+def irfft(X: np.ndarray, n: int, precision="f32") -> np.ndarray:
+    """FFTW c2r of n real points from n/2 + 1 bins (unnormalised)."""
+    d = _dtype(precision)
+    a = np.empty(2 * (n // 2 + 1), dtype=d)
+    a[0::2] = np.real(X)
+    a[1::2] = np.imag(X)
+    out = np.empty(n, dtype=d)
+    getattr(lib(precision), f"or_irfft_{precision}")(_rptr(a, precision), _rptr(out, precision), C.c_int(n))
+    return out
+
+
+def lag_window(kind: int, nchan: int, fft_size: int, precision="f32") -> np.ndarray:
+    """create_window without a user window (correlation_core.cc:761-809)."""
+    w = np.empty(2 * nchan, dtype=np.float32)
+    f = getattr(lib(precision), f"or_lag_window_{precision}")
+    if f(C.c_int(kind), C.c_int(nchan), C.c_int(fft_size), _ptr(w, C.c_float)) != 0:
+        raise ValueError(f"invalid window {kind}")
+    return w
+
+
+def lag_tables(job, precision="f32"):
+    """create_mask / create_window (correlation_core.cc:761-824) from the
+    job's Mask_parameters: (normalize, mask[N_c + 1], window[2 nchan])."""
+    nc, nch = job.fft_size_correlation, job.number_channels
+    mp = getattr(job, "mask", None)
+    normalize = bool(mp.normalize) if mp is not None else False
+    if mp is not None and mp.mask is not None and len(mp.mask) > 0:
+        mask = np.asarray(mp.mask, dtype=np.float32)
+        if len(mask) != nc + 1:
+            raise ValueError("mask must hold fft_size_correlation + 1 values")
+    else:
+        mask = np.ones(nc + 1, dtype=np.float32)
+    if mp is not None and mp.window is not None and len(mp.window) > 0:
+        win = np.asarray(mp.window, dtype=np.float32)
+        if len(win) != 2 * nch:
+            raise ValueError("window must hold 2 * number_channels values")
+    else:
+        win = lag_window(job.window, nch, nc, precision)
+    return normalize, mask, win
+
+
+def rechannelize(job, norm: np.ndarray, precision="f32") -> np.ndarray:
+    """integration_write's re-channelisation of normalised rows
+    (correlation_core.cc:417-440): (rows, N_c + 1) -> (rows, nchan + 1, 2) float32."""
+    nc, nch = job.fft_size_correlation, job.number_channels
+    normalize, mask, win = lag_tables(job)
+    a = np.ascontiguousarray(norm, dtype=np.float32).reshape(-1, nc + 1, 2)
+    rows = a.shape[0]
+    out = np.zeros((rows, nch + 1, 2), dtype=np.float32)
+    rp = RechanParamsC(nc, nch, int(normalize), 0, _ptr(mask, C.c_float), _ptr(win, C.c_float))
+    getattr(lib(precision), f"or_rechannelize_{precision}")(C.byref(rp), C.c_int(rows), _ptr(a, C.c_float),
+                                                             _ptr(out, C.c_float))
+    return out
+
+
 # ----------------------------------------------------------------------------
 def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32",
               keep_intermediate: bool = False, pulsar=None, uvw=None):
@@ -386,22 +446,32 @@ def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32
             weights.append(max(total_samples - int(lv[4]) - int(nflag[b, 0]), 0))
     stats5 = [stats_header_levels(job.station_streams[i].bits_per_sample, stats[i][0]) for i in range(M)]
     outputs = []
+    rechan = job.number_channels != job.fft_size_correlation
+
+    def channels(norm):
+        # correlation_core.cc:417-444
+        return rechannelize(job, norm, precision) if rechan else norm
+
     if pulsar is None:
         vis = correlate(job, spectra, dc[0]["seg_per_chunk"], precision)
         norm = normalize(job, vis, levels4, nflag, precision)
-        outputs.append(write_output(job, 0, job.integration_nr, norm, stats5, weights, uvw))
+        chan = channels(norm)
+        outputs.append(write_output(job, 0, job.integration_nr, chan, stats5, weights, uvw))
         vis_bins = [vis]
         norm_bins = [norm]
+        chan_bins = [chan]
     else:
         acc = pulsar_correlate(job, pulsar, spectra, dc[0]["nseg"], precision)
-        vis_bins, norm_bins = [], []
+        vis_bins, norm_bins, chan_bins = [], [], []
         for b in range(pulsar.nbins + 1):
             norm = normalize(job, acc[b], levels4, nflag, precision)
-            outputs.append(write_output(job, b, job.integration_nr, norm, stats5, weights, uvw))
+            chan = channels(norm)
+            outputs.append(write_output(job, b, job.integration_nr, chan, stats5, weights, uvw))
             vis_bins.append(acc[b])
             norm_bins.append(norm)
+            chan_bins.append(chan)
     res = dict(output=b"".join(outputs), stats=stats, nflag=nflag, weights=weights,
-               vis=vis_bins, norm=norm_bins, seg_per_chunk=dc[0]["seg_per_chunk"],
+               vis=vis_bins, norm=norm_bins, channels=chan_bins, seg_per_chunk=dc[0]["seg_per_chunk"],
                chunks=b2f[0]["chunks"], invalid=invalid)
     if keep_intermediate:
         res["b2f"] = b2f

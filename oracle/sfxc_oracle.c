@@ -29,6 +29,21 @@
  *                      linear-ephemeris closed form of the bin index.
  *   or_write_output    correlation_core.cc:316-477, output_header.h:70-148;
  *                      pinned by the struct formats of utils/print_corfile.py.
+ *   or_irfft           FFTW3f c2r convention of sfxc_fft_float.cc:235-242
+ *                      (unnormalised, imaginary parts of bins 0 and n/2
+ *                      unused); pinned by numpy.fft.irfft and a brute-force
+ *                      FP64 sum.
+ *   or_lag_window      create_window (correlation_core.cc:668-710,761-809);
+ *                      pinned by the rectangular closed form and, for the
+ *                      COS / HAMMING / HANN forms, numpy.convolve of the
+ *                      window function with itself.
+ *   or_rechannelize    integration_write's fft_size != number_channels branch
+ *                      (correlation_core.cc:417-440); pinned by closed forms:
+ *                      a lag function inside +-nchan gives the spectrum
+ *                      sampled every N_c / nchan bins, lags at +-nchan give
+ *                      their mean times (-1)^k, normalisation of a pure delay
+ *                      gives the window's value at that lag, mask before
+ *                      normalisation is a no-op.
  */
 #include <math.h>
 #include <stdint.h>
@@ -825,4 +840,117 @@ int64_t FN(or_write_output)(int32_t sourcenr, int32_t integration_slice, int n_s
     o += sizeof(float) * 2 * (nchan + 1);
   }
   return (int64_t)(o - dst);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 7. Lag re-channelisation (fft_size_correlation > number_channels)         */
+/* ------------------------------------------------------------------------ */
+
+/* c2r of n real points from n/2 + 1 complex values (sfxc_fft_fftw_float::irfft,
+ * sfxc_fft_float.cc:235-242): FFTW's unnormalised backward transform of a
+ * Hermitian half spectrum, which does not use the imaginary parts of bins 0
+ * and n/2. Computed as an n/2-point complex transform of
+ * Z[k] = (X[k] + conj X[n/2-k]) + i exp(+2 pi i k/n) (X[k] - conj X[n/2-k]),
+ * whose output holds out[2m] + i out[2m+1]. */
+static void irfft(const cplx *in, real *out, int n, cplx *work) {
+  const int h = n / 2;
+  fft_plan *pl = get_plan(n, -1); /* exp(-2 pi i k / n), k < n/2 */
+  for (int k = 0; k < h; k++) {
+    cplx a = in[k], b = in[h - k];
+    if (k == 0) { a.im = 0; b.im = 0; }
+    real sr = a.re + b.re, si = a.im - b.im;
+    real dr = a.re - b.re, di = a.im + b.im;
+    real wr = pl->tw[k].re, wi = -pl->tw[k].im;
+    real tr = dr * wr - di * wi, ti = dr * wi + di * wr;
+    work[k].re = sr - ti;
+    work[k].im = si + tr;
+  }
+  fft_inplace(work, h, +1);
+  for (int m = 0; m < h; m++) { out[2 * m] = work[m].re; out[2 * m + 1] = work[m].im; }
+}
+
+void FN(or_irfft)(const real *in, real *out, int n) {
+  cplx *work = (cplx *)malloc(sizeof(cplx) * (n / 2));
+  irfft((const cplx *)in, out, n, work);
+  free(work);
+}
+
+/* The window functions and their self-convolution (correlation_core.cc:668-710). */
+static double lw_cos(int n, int i) { return sin(M_PI * i / n); }
+static double lw_hamming(int n, int i) { return 0.54 - 0.46 * cos(2 * M_PI * i / n); }
+static double lw_hann(int n, int i) { return 0.5 * (1 - cos(2 * M_PI * i / n)); }
+
+static double lw_convolve(double (*f)(int, int), int n, int i) {
+  double sum = 0.0;
+  i += n / 2;
+  for (int j = 0; j <= n; j++) {
+    if (i - j < 0 || i - j > n) continue;
+    sum += f(n, j) * f(n, i - j);
+  }
+  return sum;
+}
+
+/* create_window without a user window (correlation_core.cc:761-809): the lag
+ * window of 2 nchan points for the delay-correction window `window` applied
+ * over 2 fft_size points. (m / n) is an integer quotient, as in the
+ * reference. Returns -1 for an unknown window. */
+int FN(or_lag_window)(int window, int nchan, int fft_size, float *w) {
+  const int n = 2 * nchan, m = 2 * fft_size;
+  double (*f)(int, int);
+  switch (window) {
+    case OR_WINDOW_NONE:
+    case OR_WINDOW_RECT:
+      w[0] = 1.0f;
+      for (int i = 1; i < n / 2; i++) w[i] = w[n - i] = (float)((1. - 2. * i / n) / (1. - 2. * i / m));
+      w[n / 2] = 0.0f;
+      return 0;
+    case OR_WINDOW_COS: f = lw_cos; break;
+    case OR_WINDOW_HAMMING: f = lw_hamming; break;
+    case OR_WINDOW_HANN: f = lw_hann; break;
+    default: return -1;
+  }
+  w[0] = (float)((m / n) * lw_convolve(f, n, n / 2) / lw_convolve(f, m, m / 2));
+  for (int i = 1; i < n / 2; i++)
+    w[i] = w[n - i] = (float)((m / n) * lw_convolve(f, n, n / 2 + i) / lw_convolve(f, m, m / 2 + i));
+  w[n / 2] = (float)((m / n) * lw_convolve(f, n, n) / lw_convolve(f, m, (n + m) / 2));
+  return 0;
+}
+
+/* integration_write for fft_size != number_channels (correlation_core.cc:417-440),
+ * rows of normalised visibilities: optional normalisation of every bin to
+ * unit magnitude, the mask, a 2 N_c-point c2r transform to lags, the fold of
+ * lags -nchan..nchan into 2 nchan points (lag +-nchan averaged), the lag
+ * window, a 2 nchan-point r2c transform and division by 2 N_c.
+ * in: rows x (N_c + 1) complex<float>; out: rows x (nchan + 1) complex<float>. */
+void FN(or_rechannelize)(const or_rechan_params *p, int rows, const float *in, float *out) {
+  const int nc = p->fft_size, nch = p->nchan;
+  cplx *X = (cplx *)malloc(sizeof(cplx) * (nc + 1));
+  cplx *work = (cplx *)malloc(sizeof(cplx) * nc);
+  real *r = (real *)malloc(sizeof(real) * 2 * nc);
+  real *y = (real *)malloc(sizeof(real) * 2 * nch);
+  cplx *Y = (cplx *)malloc(sizeof(cplx) * (nch + 1));
+  for (int row = 0; row < rows; row++) {
+    const float *x = in + 2 * (size_t)row * (nc + 1);
+    for (int j = 0; j <= nc; j++) {
+      real re = x[2 * j], im = x[2 * j + 1];
+      if (p->normalize) {
+        real a = (real)hypot((double)re, (double)im);
+        if (a != 0) { re /= a; im /= a; }
+      }
+      X[j].re = re * (real)p->mask[j];
+      X[j].im = im * (real)p->mask[j];
+    }
+    irfft(X, r, 2 * nc, work);
+    for (int i = 0; i < nch; i++) y[i] = r[i];
+    y[nch] = (r[nch] + r[2 * nc - nch]) / 2;
+    for (int j = 1; j < nch; j++) y[nch + j] = r[2 * nc - nch + j];
+    for (int i = 0; i < 2 * nch; i++) y[i] *= (real)p->window[i];
+    rfft(y, Y, 2 * nch, work);
+    float *o = out + 2 * (size_t)row * (nch + 1);
+    for (int j = 0; j <= nch; j++) {
+      o[2 * j] = (float)(Y[j].re / (real)(2 * nc));
+      o[2 * j + 1] = (float)(Y[j].im / (real)(2 * nc));
+    }
+  }
+  free(X); free(work); free(r); free(y); free(Y);
 }

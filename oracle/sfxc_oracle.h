@@ -104,4 +104,17 @@ typedef struct {
   const double *offsets;  /* N_c + 1 dispersion offsets, pulsar periods */
 } or_pulsar_params;
 
+/* Lag re-channelisation of integration_write when fft_size_correlation >
+ * number_channels (correlation_core.cc:417-440): Mask_parameters
+ * (control_parameters.h:107-114) and the tables create_mask / create_window
+ * build from them (correlation_core.cc:761-824). */
+typedef struct {
+  int32_t fft_size;       /* N_c */
+  int32_t nchan;          /* number_channels */
+  int32_t normalize;      /* Mask_parameters::normalize */
+  int32_t pad;
+  const float *mask;      /* N_c + 1 values */
+  const float *window;    /* 2 * nchan values */
+} or_rechan_params;
+
 #endif

@@ -123,6 +123,15 @@ struct sfxg_ctx {
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
       flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, tw_split, window, offsets, samples,
       chan_scratch, tiles;
+  // Mask_parameters (control_parameters.h:107-114) and the lag
+  // re-channelisation tables built from them (correlation_core.cc:761-824)
+  bool mask_normalize = false;
+  std::vector<double> mask_param, window_param;
+  uint64_t mask_version = 0;
+  int rc_nc = -1, rc_nchan = -1, rc_window = -1;
+  uint64_t rc_version = ~(uint64_t)0;
+  bool rc_normalize = false;  // normalize as the prepared slice uses it
+  DBuf rc_tw, rc_mask, rc_window_buf, chan_out;
   // host staging of the enqueued slice (kept until the next slice)
   std::vector<double> h_model, h_offsets;
   std::vector<DevStream> h_streams;
@@ -230,7 +239,7 @@ void sfxg_destroy(sfxg_ctx *ctx) {
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
                   &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
                   &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples, &ctx->chan_scratch,
-                  &ctx->tiles};
+                  &ctx->tiles, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out};
   for (DBuf *b : bufs) b->release();
   sync_slice(ctx);
   if (ctx->done) (void)hipEventDestroy(ctx->done);
@@ -255,6 +264,19 @@ int sfxg_set_options(sfxg_ctx *ctx, const sfxg_options *opt) {
   return SFXG_OK;
 }
 
+int sfxg_set_mask(sfxg_ctx *ctx, int32_t normalize, const double *mask, int32_t n_mask, const double *window,
+                  int32_t n_window) {
+  if (!ctx) return SFXG_ERR_ARG;
+  if (n_mask < 0 || n_window < 0 || (n_mask > 0 && !mask) || (n_window > 0 && !window))
+    return ctx->fail(SFXG_ERR_ARG, "mask or window size without values");
+  ctx->mask_normalize = normalize != 0;
+  ctx->mask_param.assign(mask, mask + n_mask);
+  ctx->window_param.assign(window, window + n_window);
+  ctx->mask_version++;
+  ctx->prepared = false;
+  return SFXG_OK;
+}
+
 static int sb_sign(char c) { return c == 'L' ? -1 : 1; }
 
 int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params,
@@ -267,9 +289,6 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
   ctx->prepared = false;
   const sfxg_correlation_parameters &p = *params;
   if (p.n_streams <= 0 || !p.station_streams) return ctx->fail(SFXG_ERR_ARG, "no station streams");
-  if (p.fft_size_correlation != p.number_channels)
-    return ctx->fail(SFXG_ERR_ARG, "fft_size_correlation != number_channels (lag re-channelisation) "
-                                   "is not supported");
   if (p.n_phase_centers != 1) return ctx->fail(SFXG_ERR_ARG, "only one phase centre is supported");
   if (p.fft_size_delaycor != 256) return ctx->fail(SFXG_ERR_ARG, "fft_size_delaycor must be 256");
   const int nc = p.fft_size_correlation;
@@ -277,6 +296,12 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
     return ctx->fail(SFXG_ERR_ARG, "fft_size_correlation must be a power of two in [256, 32768]");
   if (p.window < SFXG_WINDOW_NONE || p.window > SFXG_WINDOW_HANN)
     return ctx->fail(SFXG_ERR_ARG, "unknown window %d", p.window);
+  // integration_write re-channelises in the lag domain when nchan < N_c
+  // (correlation_core.cc:408-440)
+  if (p.number_channels != nc && !rechannel_supported(nc, p.number_channels))
+    return ctx->fail(SFXG_ERR_ARG, "number_channels %d with fft_size_correlation %d: the lag re-channelisation "
+                                   "takes a power of two number_channels below fft_size_correlation <= 16384",
+                     p.number_channels, nc);
   for (int i = 0; i < p.n_streams; i++) {
     const sfxg_station_parameters &s = p.station_streams[i];
     if (s.sample_rate != p.sample_rate)
@@ -535,6 +560,47 @@ static int prepare_slice(sfxg_ctx *ctx) {
     ctx->tables_window = ctx->p.window;
   }
 
+  // lag re-channelisation tables (create_mask / create_window,
+  // correlation_core.cc:761-824), rebuilt when the shape, the window or the
+  // mask parameters change
+  if (ctx->nchan != nc && (ctx->rc_nc != nc || ctx->rc_nchan != ctx->nchan || ctx->rc_window != ctx->p.window ||
+                           ctx->rc_version != ctx->mask_version)) {
+    std::vector<float> mask, win;
+    if (!ctx->mask_param.empty()) {
+      if ((int)ctx->mask_param.size() != nc + 1)
+        return ctx->fail(SFXG_ERR_ARG, "the mask holds %zu values, fft_size_correlation + 1 = %d needed",
+                         ctx->mask_param.size(), nc + 1);
+      mask.assign(ctx->mask_param.begin(), ctx->mask_param.end());
+    } else {
+      mask.assign(nc + 1, 1.0f);
+    }
+    if (!ctx->window_param.empty()) {
+      if ((int)ctx->window_param.size() != 2 * ctx->nchan)
+        return ctx->fail(SFXG_ERR_ARG, "the lag window holds %zu values, 2 * number_channels = %d needed",
+                         ctx->window_param.size(), 2 * ctx->nchan);
+      win.assign(ctx->window_param.begin(), ctx->window_param.end());
+    } else if (!lag_window(ctx->p.window, ctx->nchan, nc, win)) {
+      return ctx->fail(SFXG_ERR_ARG, "unknown window %d", ctx->p.window);
+    }
+    std::vector<float2> tw(2 * (size_t)nc);
+    for (int k = 0; k < 2 * nc; k++) {
+      const double a = -M_PI * k / nc;
+      tw[k] = make_float2((float)cos(a), (float)sin(a));
+    }
+    HIPCHK(ctx, ctx->rc_tw.ensure(tw.size() * sizeof(float2)));
+    HIPCHK(ctx, ctx->rc_mask.ensure(mask.size() * sizeof(float)));
+    HIPCHK(ctx, ctx->rc_window_buf.ensure(win.size() * sizeof(float)));
+    HIPCHK(ctx, hipMemcpy(ctx->rc_tw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->rc_mask.p, mask.data(), mask.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->rc_window_buf.p, win.data(), win.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->rc_nc = nc;
+    ctx->rc_nchan = ctx->nchan;
+    ctx->rc_window = ctx->p.window;
+    ctx->rc_version = ctx->mask_version;
+  }
+
+  ctx->rc_normalize = ctx->mask_normalize;
+
   // per-stream tables, concatenated
   std::vector<int64_t> pay_off(M), run_off(M), cnt_off(M), blk_off(M), mod_off(M);
   int64_t pay_tot = 0, run_tot = 0, cnt_tot = 0, mod_tot = 0, max_counted = 0;
@@ -609,6 +675,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   const size_t vis_elems = (size_t)ctx->nbins * B * (nc + 1);
   HIPCHK(ctx, ctx->vis.ensure(vis_elems * sizeof(float2)));
   HIPCHK(ctx, ctx->norm_out.ensure(vis_elems * sizeof(float2)));
+  if (ctx->nchan != nc) HIPCHK(ctx, ctx->chan_out.ensure((size_t)ctx->nbins * B * (ctx->nchan + 1) * sizeof(float2)));
   HIPCHK(ctx, ctx->hist.ensure((size_t)M * 512 * sizeof(int32_t)));
   HIPCHK(ctx, ctx->norms.ensure((size_t)ctx->nbins * M * sizeof(double)));
   HIPCHK(ctx, ctx->flags.ensure((size_t)B * sizeof(double)));
@@ -917,6 +984,10 @@ static int enqueue_slice(sfxg_ctx *ctx, hipStream_t s, bool timing) {
   HIPCHK(ctx, launch_auto_norms(ctx->vis.as<float2>(), M, nc + 1, nc, ctx->nbins, B, ctx->norms.as<double>(), s));
   HIPCHK(ctx, launch_normalise(ctx->vis.as<float2>(), ctx->norms.as<double>(), blf, bls, ctx->flags.as<double>(), M,
                                B, nc + 1, ctx->nbins, ctx->norm_out.as<float2>(), s));
+  if (ctx->nchan != nc)
+    HIPCHK(ctx, launch_rechannel(ctx->norm_out.as<float2>(), ctx->chan_out.as<float2>(), ctx->nbins * B, nc,
+                                 ctx->nchan, ctx->rc_mask.as<float>(), ctx->rc_window_buf.as<float>(),
+                                 ctx->rc_normalize ? 1 : 0, ctx->rc_tw.as<float2>(), s));
   HIPCHK(ctx, mark(SFXG_N_KERNELS));
   return SFXG_OK;
 }
@@ -999,7 +1070,7 @@ int sfxg_number_of_baselines(sfxg_ctx *ctx, int32_t *n) {
   return SFXG_OK;
 }
 
-// integration_write (correlation_core.cc:316-477) with fft_size == number_channels
+// integration_write (correlation_core.cc:316-477)
 int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) {
   if (!ctx || !dst) return SFXG_ERR_ARG;
   if (!ctx->have_results) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
@@ -1009,7 +1080,8 @@ int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) 
   sync_slice(ctx);
   const int M = ctx->M, B = ctx->B, nch = ctx->nchan + 1;
   std::vector<float2> vis((size_t)ctx->nbins * B * nch);
-  HIPCHK(ctx, hipMemcpy(vis.data(), ctx->norm_out.p, vis.size() * sizeof(float2), hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(vis.data(), ctx->nchan != ctx->nc ? ctx->chan_out.p : ctx->norm_out.p,
+                        vis.size() * sizeof(float2), hipMemcpyDeviceToHost));
   std::vector<int32_t> lv, ts;
   int rc = stats_of(ctx, lv, ts);
   if (rc) return rc;
@@ -1153,6 +1225,12 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
       need = (size_t)ctx->nbins * ctx->B * (ctx->nc + 1) * sizeof(float2);
       if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
       HIPCHK(ctx, hipMemcpy(dst, stage == SFXG_STAGE_ACCUMULATED ? ctx->vis.p : ctx->norm_out.p, need,
+                            hipMemcpyDeviceToHost));
+      break;
+    case SFXG_STAGE_CHANNELS:
+      need = (size_t)ctx->nbins * ctx->B * (ctx->nchan + 1) * sizeof(float2);
+      if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
+      HIPCHK(ctx, hipMemcpy(dst, ctx->nchan != ctx->nc ? ctx->chan_out.p : ctx->norm_out.p, need,
                             hipMemcpyDeviceToHost));
       break;
     default:

@@ -308,4 +308,42 @@ void bit_statistics(int bits, int64_t sample_rate, int64_t base_sample_rate, con
   for (int k = 0; k < 5; k++) levels[k] = (int32_t)((base_sample_rate * (int64_t)stat[k]) / sample_rate);
 }
 
+// create_window (correlation_core.cc:761-809). The window functions of
+// correlation_core.cc:668-692 are tabulated once per length, and each
+// self-convolution (convolve, correlation_core.cc:694-710) adds the same
+// products in the same order as the reference's loop.
+bool lag_window(int window, int nchan, int fft_size, std::vector<float> &w) {
+  const int n = 2 * nchan, m = 2 * fft_size;
+  w.assign(n, 0.0f);
+  if (window == SFXG_WINDOW_NONE || window == SFXG_WINDOW_RECT) {
+    w[0] = 1.0f;
+    for (int i = 1; i < n / 2; i++) w[i] = w[n - i] = (float)((1. - 2. * i / n) / (1. - 2. * i / m));
+    w[n / 2] = 0.0f;
+    return true;
+  }
+  if (window != SFXG_WINDOW_COS && window != SFXG_WINDOW_HAMMING && window != SFXG_WINDOW_HANN) return false;
+  auto table = [window](int len) {
+    std::vector<double> f(len + 1);
+    for (int i = 0; i <= len; i++) {
+      if (window == SFXG_WINDOW_COS) f[i] = sin(M_PI * i / len);
+      else if (window == SFXG_WINDOW_HAMMING) f[i] = 0.54 - 0.46 * cos(2 * M_PI * i / len);
+      else f[i] = 0.5 * (1 - cos(2 * M_PI * i / len));
+    }
+    return f;
+  };
+  const std::vector<double> fn = table(n), fm = table(m);
+  auto convolve = [](const std::vector<double> &f, int len, int i) {
+    i += len / 2;
+    double sum = 0.0;
+    for (int j = std::max(0, i - len); j <= std::min(len, i); j++) sum += f[j] * f[i - j];
+    return sum;
+  };
+  const int ratio = m / n;
+  w[0] = (float)(ratio * convolve(fn, n, n / 2) / convolve(fm, m, m / 2));
+  for (int i = 1; i < n / 2; i++)
+    w[i] = w[n - i] = (float)(ratio * convolve(fn, n, n / 2 + i) / convolve(fm, m, m / 2 + i));
+  w[n / 2] = (float)(ratio * convolve(fn, n, n) / convolve(fm, m, (n + m) / 2));
+  return true;
+}
+
 }  // namespace sfxg

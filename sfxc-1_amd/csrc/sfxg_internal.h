@@ -260,6 +260,16 @@ hipError_t launch_auto_norms(const float2 *vis, int n_streams, int n_ch, int fft
 hipError_t launch_normalise(const float2 *vis, const double *norms, const int32_t *bl_first,
                             const int32_t *bl_second, const double *flag_factor, int n_streams,
                             int n_baselines, int n_ch, int n_bins, float2 *out, hipStream_t s);
+// Lag re-channelisation (correlation_core.cc:417-440, sfxg_rechan.hip): rows
+// of N_c + 1 normalised bins in, rows of nchan + 1 bins out. tw holds
+// exp(-2 pi i k / 2N_c) for k < 2N_c; mask N_c + 1 values; window 2 nchan.
+bool rechannel_supported(int nc, int nchan);
+hipError_t launch_rechannel(const float2 *in, float2 *out, int rows, int nc, int nchan, const float *mask,
+                            const float *window, int normalize, const float2 *tw, hipStream_t s);
+// create_window of Correlation_core (correlation_core.cc:668-710,761-809):
+// the 2 nchan-point lag window for the delay-correction window `window`
+// applied over 2 fft_size points. Returns false for an unknown window.
+bool lag_window(int window, int nchan, int fft_size, std::vector<float> &w);
 hipError_t launch_synthesize(void *dst, size_t n_bytes, int bits, int64_t first_sample,
                              int64_t delay_samples, uint64_t common_seed, uint64_t stream_seed,
                              float rho, float threshold, hipStream_t s);

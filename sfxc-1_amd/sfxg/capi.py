@@ -18,7 +18,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libsfxg.so")
 
 SFXG_OK = 0
-STAGE_SAMPLES, STAGE_TIME_SERIES, STAGE_SPECTRA, STAGE_ACCUMULATED, STAGE_NORMALISED = 0, 1, 2, 3, 4
+STAGE_SAMPLES, STAGE_TIME_SERIES, STAGE_SPECTRA, STAGE_ACCUMULATED, STAGE_NORMALISED, STAGE_CHANNELS = 0, 1, 2, 3, 4, 5
 
 
 class StationParameters(C.Structure):
@@ -112,6 +112,9 @@ def lib() -> C.CDLL:
         L.sfxg_tune_import.argtypes = [C.c_char_p]
         L.sfxg_tune_clear.restype = None
         L.sfxg_tune_clear.argtypes = []
+        L.sfxg_set_mask.restype = C.c_int
+        L.sfxg_set_mask.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_int32,
+                                    C.POINTER(C.c_double), C.c_int32]
         L.sfxg_synthesize_raw.restype = C.c_int
         L.sfxg_synthesize_raw.argtypes = [C.c_int, C.c_void_p, C.c_size_t, C.c_int32, C.c_int64, C.c_int64,
                                           C.c_uint64, C.c_uint64, C.c_float, C.c_float]
@@ -195,9 +198,26 @@ class Context:
             keep.append(ps)
         self._check(lib().sfxg_set_parameters(self._h, C.byref(p), models, uvw,
                                                C.byref(ps) if ps is not None else None), "sfxg_set_parameters")
+        self.set_mask(job.mask)
         self._keep = keep
         self.job = job
         self.nbins = pulsar.nbins + 1 if pulsar is not None else 1
+
+    def set_mask(self, mask=None):
+        """Mask_parameters of the job (sfxg.job.MaskParameters or None for the defaults)."""
+        keep = []
+
+        def arr(a):
+            if a is None or len(a) == 0:
+                return C.POINTER(C.c_double)(), 0
+            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_double)), len(a)
+
+        m, nm = arr(mask.mask if mask is not None else None)
+        w, nw = arr(mask.window if mask is not None else None)
+        norm = int(bool(mask.normalize)) if mask is not None else 0
+        self._check(lib().sfxg_set_mask(self._h, norm, m, nm, w, nw), "sfxg_set_mask")
 
     def push_stream(self, i: int, hdr: bytes):
         self._check(lib().sfxg_push_stream(self._h, i, hdr, len(hdr)), f"sfxg_push_stream({i})")
@@ -259,6 +279,8 @@ class Context:
             out = np.zeros(job.n_ffts_per_integration(stream) * nd, dtype=np.float32)
         elif stage == STAGE_SPECTRA:
             out = np.zeros((job.number_ffts_in_integration(), nc + 1), dtype=np.complex64)
+        elif stage == STAGE_CHANNELS:
+            out = np.zeros((self.nbins, B, job.number_channels + 1), dtype=np.complex64)
         else:
             out = np.zeros((self.nbins, B, nc + 1), dtype=np.complex64)
         w = C.c_size_t()

@@ -121,6 +121,19 @@ class Pulsar:
 
 
 @dataclasses.dataclass
+class MaskParameters:
+    """Mask_parameters (control_parameters.h:107-114): used by integration_write
+    when fft_size_correlation > number_channels (correlation_core.cc:417-440).
+    mask: fft_size_correlation + 1 values multiplying the spectrum (None: ones);
+    window: 2 * number_channels lag weights (None: derived from the job's
+    window by create_window, correlation_core.cc:761-809)."""
+
+    normalize: bool = False
+    mask: Optional[List[float]] = None
+    window: Optional[List[float]] = None
+
+
+@dataclasses.dataclass
 class CorrelationParameters:
     """Correlation_parameters (control_parameters.h:117-182), one integration
     slice of one subband."""
@@ -152,6 +165,7 @@ class CorrelationParameters:
     # one delay model per entry of station_streams (Delay_table_akima per stream)
     delay_models: List[DelayModel] = dataclasses.field(default_factory=list)
     uvw: Optional[np.ndarray] = None  # (n_streams, 3) per stream, metres
+    mask: Optional[MaskParameters] = None  # Correlation_parameters::mask_parameters
 
     # --- derived quantities (each cites the reference expression) ---------
     def n_ffts_per_integration(self, i: int = 0) -> int:

@@ -1,0 +1,127 @@
+"""Lag re-channelisation on the GPU (integration_write with
+fft_size_correlation > number_channels, correlation_core.cc:417-440, and the
+Mask_parameters of control_parameters.h:107-114) against the oracle, through
+the C ABI."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sfxg import capi, configs
+from sfxg.job import (LUT_CONFIG, WINDOW_COS, WINDOW_HAMMING, WINDOW_HANN, WINDOW_NONE, WINDOW_RECT,
+                      MaskParameters, Pulsar, ticks_to_mjd)
+
+from test_gpu_parity import compare_vis, vis_from_bytes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def make_job(nc, nchan, window, seed=21, stations=3, pols=("R",), seconds=0.0625):
+    job, hdrs = configs.build_slices(seed=seed, n_stations=stations, pols=list(pols), fs=32_000_000,
+                                     bw=16_000_000, nchan=nc, seconds=seconds, t_int_usec=int(seconds * 1e6),
+                                     rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=[0.0, 2.5e-6, -1.75e-6][:stations],
+                                     rates=[0.0, 2e-7, -1e-7][:stations], window=window)[0]
+    job.number_channels = nchan
+    return job, hdrs
+
+
+def mask_for(kind, nc, nchan, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "band":  # flag three bands of bins
+        m = np.ones(nc + 1)
+        for a, b in ((0, nc // 64), (nc // 3, nc // 3 + nc // 50), (nc - nc // 40, nc + 1)):
+            m[a:b] = 0.0
+        return list(m)
+    if kind == "random":
+        return list(rng.uniform(0.5, 1.5, nc + 1))
+    return None
+
+
+CASES = [
+    # nc, nchan, window, normalize, mask, user lag window
+    (1024, 256, WINDOW_HANN, False, None, False),
+    (4096, 64, WINDOW_COS, True, "band", False),
+    (16384, 4096, WINDOW_RECT, False, None, True),
+    (2048, 1024, WINDOW_NONE, False, None, False),
+    (8192, 2, WINDOW_HAMMING, True, None, False),
+    (512, 128, WINDOW_HANN, False, "random", False),
+]
+
+
+@pytest.mark.parametrize("nc,nchan,window,normalize,mask,user_window", CASES)
+def test_rechannelisation(ctx, nc, nchan, window, normalize, mask, user_window):
+    print(f"case nc={nc} nchan={nchan} window={window} normalize={normalize} mask={mask}", flush=True)
+    job, hdrs = make_job(nc, nchan, window)
+    win = list(O.lag_window(WINDOW_HANN, nchan, nc) ** 2) if user_window else None
+    job.mask = MaskParameters(normalize=normalize, mask=mask_for(mask, nc, nchan, nc + nchan), window=win)
+    ref = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+    out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    heads_g, vis_g = vis_from_bytes(job, out)
+    heads_r, vis_r = vis_from_bytes(job, ref["output"])
+    assert heads_g == heads_r
+    assert vis_g[0].shape[1] == nchan + 1
+    # the normalised visibilities before re-channelisation, and the stored channels
+    norm = ctx.stage(capi.STAGE_NORMALISED)[0]
+    nr = ref["norm"][0]
+    compare_vis(norm, nr[..., 0] + 1j * nr[..., 1], label="normalised")
+    chan = ctx.stage(capi.STAGE_CHANNELS)[0]
+    np.testing.assert_array_equal(chan, vis_g[0])
+    compare_vis(vis_g[0], vis_r[0], label=f"channels nc={nc} nchan={nchan}")
+
+
+def test_rechannelisation_pulsar(ctx):
+    """Correlation_core_pulsar builds the same mask and window
+    (correlation_core_pulsar.cc:182-185) and writes every bin through
+    integration_write."""
+    job, hdrs = make_job(1024, 128, WINDOW_HANN, seed=23, stations=2, pols=("R", "L"), seconds=0.125)
+    job.mask = MaskParameters(normalize=False)
+    psr = Pulsar(nbins=4, gate_start=0.1, gate_stop=0.6, tmid=ticks_to_mjd(job.start_time) - 0.001, DM=68.0,
+                 ref_phase=0.3, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+    ref = O.run_slice(job, hdrs, lut=LUT_CONFIG, pulsar=psr)
+    out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, pulsar=psr)
+    nb = psr.nbins + 1
+    heads_g, vis_g = vis_from_bytes(job, out, nb)
+    heads_r, vis_r = vis_from_bytes(job, ref["output"], nb)
+    assert heads_g == heads_r
+    for b in range(nb):
+        compare_vis(vis_g[b], vis_r[b], label=f"pulsar bin {b}")
+
+
+def test_mask_tables_rebuilt(ctx):
+    """A context reused for a job without mask parameters rebuilds its tables:
+    ones for the mask and the lag window derived from the job's window."""
+    job, hdrs = make_job(1024, 256, WINDOW_HANN, seed=24)
+    job.mask = MaskParameters(normalize=True, mask=mask_for("band", 1024, 256, 1))
+    a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    ref_a = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+    job.mask = None
+    b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    ref_b = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+    for got, ref in ((a, ref_a), (b, ref_b)):
+        _, vg = vis_from_bytes(job, got)
+        _, vr = vis_from_bytes(job, ref["output"])
+        compare_vis(vg[0], vr[0], label="mask reuse")
+    assert a != b
+
+
+def test_rechannelisation_rejects_unsupported(ctx):
+    job, hdrs = make_job(1024, 256, WINDOW_HANN)
+    for nchan in (384, 1, 2048):
+        job.number_channels = nchan
+        with pytest.raises(capi.SfxgError):
+            ctx.set_parameters(job)
+    job.number_channels = 256
+    job.mask = MaskParameters(mask=[1.0] * 100)
+    ctx.set_options(lut=LUT_CONFIG)
+    ctx.set_parameters(job)
+    for i, h in enumerate(hdrs):
+        ctx.push_stream(i, h)
+    with pytest.raises(capi.SfxgError):
+        ctx.correlate()
