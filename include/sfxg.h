@@ -160,6 +160,16 @@ typedef struct {
    * sub-integration per chunk). 0: 8192. The results do not depend on it
    * (the FP32 path gives the same bytes). */
   int32_t chunk_mib;
+  /* Nonzero: the context is a phased-array node, as Correlator_node builds
+   * Correlation_core_phased in place of Correlation_core
+   * (src/correlator_node.cc:22-47): per sub-integration the spectra of all
+   * streams are summed (src/correlation_core_phased.cc:116-126) and the
+   * number_ffts_in_integration / number_ffts_in_sub_integration + 1 sums
+   * are written as the slice's rows, each with the header of stream 0's
+   * auto-correlation (:13-50, :84-98; integration_write). No normalisation.
+   * Takes FP32, no pulsar binning (src/control_parameters.cc:161-166) and
+   * number_channels == fft_size_correlation. Set before sfxg_set_parameters. */
+  int32_t phased_array;
 } sfxg_options;
 
 /* Creates a context on HIP device `device`. Replaces the construction of

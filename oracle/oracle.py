@@ -232,6 +232,23 @@ def correlate(job, spectra: List[np.ndarray], seg_per_chunk, precision="f32"):
     return vis.reshape(len(bl), nf, 2)
 
 
+def phased_correlate(job, spectra: List[np.ndarray], nseg: int, precision="f32"):
+    """Correlation_core_phased: per sub-integration sums of all streams' spectra
+    -> (number_ffts_in_integration // number_ffts_in_sub_integration + 1, N_c + 1, 2)."""
+    keep = []
+    cp, _ = _core_params(job, keep)
+    nf = job.fft_size_correlation + 1
+    nrows = job.number_ffts_in_integration() // job.number_ffts_in_sub_integration() + 1
+    arrs = [np.ascontiguousarray(s, dtype=_dtype(precision)) for s in spectra]
+    ptrs = (C.POINTER(C.c_float if precision == "f32" else C.c_double) * len(arrs))(
+        *[_rptr(a, precision) for a in arrs])
+    rows = np.zeros(2 * nrows * nf, dtype=_dtype(precision))
+    f = getattr(lib(precision), f"or_phased_correlate_{precision}")
+    n = f(C.byref(cp), ptrs, C.c_int64(nseg), _rptr(rows, precision))
+    assert n == nrows
+    return rows.reshape(nrows, nf, 2)
+
+
 def find_invalid(job, invalid: List[tuple]):
     """n_flagged per baseline (correlation_core.cc:599-665)."""
     bl = job.baselines()
@@ -328,9 +345,10 @@ def stats_header_levels(bits, levels):
     return [0, levels[0], levels[1], 0, levels[4]]
 
 
-def write_output(job, sourcenr, integration_slice, vis_f32, stats5, weights, uvw=None):
+def write_output(job, sourcenr, integration_slice, vis_f32, stats5, weights, uvw=None, bl=None):
     M = len(job.station_streams)
-    bl = job.baselines()
+    if bl is None:
+        bl = job.baselines()
     station = np.array([s.station_number for s in job.station_streams], dtype=np.int32)
     pol = np.array([0 if s.polarisation == "R" else 1 for s in job.station_streams], dtype=np.int32)
     if uvw is None:
@@ -411,7 +429,7 @@ def rechannelize(job, norm: np.ndarray, precision="f32") -> np.ndarray:
 
 # ----------------------------------------------------------------------------
 def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32",
-              keep_intermediate: bool = False, pulsar=None, uvw=None):
+              keep_intermediate: bool = False, pulsar=None, uvw=None, phased: bool = False):
     """Full correlator-node path for one slice; returns dict with the output
     bytes and intermediates. One integration per slice (the reference asserts
     (stop - start) / integration_time == 1, correlator_node.cc:327-328)."""
@@ -435,7 +453,12 @@ def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32
     nflag = find_invalid(job, invalid)
     levels4 = np.array([s[0][4] for s in stats], dtype=np.int32)
     spectra = [d["spectra"] for d in dc]
-    bl = job.baselines()
+    if phased:
+        # Correlation_core_phased: one row per sub-integration plus one, each
+        # written as stream 0's auto-correlation (correlation_core_phased.cc:84-98)
+        bl = [(0, 0)] * (job.number_ffts_in_integration() // job.number_ffts_in_sub_integration() + 1)
+    else:
+        bl = job.baselines()
     total_samples = job.number_ffts_in_integration() * job.fft_size_correlation
     weights = []
     for b, (i, j) in enumerate(bl):
@@ -452,7 +475,11 @@ def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32
         # correlation_core.cc:417-444
         return rechannelize(job, norm, precision) if rechan else norm
 
-    if pulsar is None:
+    if phased:
+        rows = phased_correlate(job, spectra, dc[0]["nseg"], precision)
+        outputs.append(write_output(job, 0, job.integration_nr, rows, stats5, weights, uvw, bl=bl))
+        vis_bins = norm_bins = chan_bins = [rows]
+    elif pulsar is None:
         vis = correlate(job, spectra, dc[0]["seg_per_chunk"], precision)
         norm = normalize(job, vis, levels4, nflag, precision)
         chan = channels(norm)

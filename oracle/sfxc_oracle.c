@@ -37,6 +37,8 @@
  *                      pinned by the rectangular closed form and, for the
  *                      COS / HAMMING / HANN forms, numpy.convolve of the
  *                      window function with itself.
+ *   or_phased_correlate correlation_core_phased.cc:13-50,101-126; pinned by a
+ *                      FP64 numpy sum over the same spectra.
  *   or_rechannelize    integration_write's fft_size != number_channels branch
  *                      (correlation_core.cc:417-440); pinned by closed forms:
  *                      a lag function inside +-nchan gives the spectrum
@@ -953,4 +955,35 @@ void FN(or_rechannelize)(const or_rechan_params *p, int rows, const float *in, f
     }
   }
   free(X); free(work); free(r); free(y); free(Y);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 8. Phased-array core                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Correlation_core_phased (correlation_core_phased.cc:13-50,101-126): for
+ * every segment in order, the spectra of all streams added, in stream order,
+ * to the accumulator of its sub-integration (segment / number_ffts_in_sub_
+ * integration). rows receives number_ffts_in_integration /
+ * number_ffts_in_sub_integration + 1 accumulators of N_c + 1 complex values.
+ * Returns the number of rows. */
+int FN(or_phased_correlate)(const or_core_params *p, const real *const *spectra, int64_t nseg, real *rows) {
+  const int nf = p->fft_size + 1, M = p->n_streams;
+  const int nsub = p->number_ffts_in_sub_integration;
+  const int nrows = p->number_ffts_in_integration / nsub + 1;
+  cplx *acc = (cplx *)rows;
+  memset(acc, 0, sizeof(cplx) * (size_t)nrows * nf);
+  for (int64_t seg = 0; seg < nseg; seg++) {
+    const int64_t sub = seg / nsub;
+    if (sub >= nrows) break;
+    cplx *o = acc + sub * nf;
+    for (int i = 0; i < M; i++) {
+      const cplx *x = (const cplx *)spectra[i] + seg * (int64_t)p->stride;
+      for (int j = 0; j < nf; j++) {
+        o[j].re += x[j].re;
+        o[j].im += x[j].im;
+      }
+    }
+  }
+  return nrows;
 }

@@ -87,12 +87,14 @@ struct LaunchPlan {
   DevJob job{};
   float mixed_scale = 1.0f;
   double start_phase = 0.0, phase_coef = 0.0, fft_duration = 0.0;  // pulsar
+  // phased array: per chunk, its first block / row in the tables and their counts
+  std::vector<int32_t> ph_b0, ph_nb, ph_r0, ph_nr;
 };
 
 struct sfxg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  sfxg_options opt{SFXG_LUT_REFERENCE, SFXG_PREC_FP32, 0, 0};
+  sfxg_options opt{SFXG_LUT_REFERENCE, SFXG_PREC_FP32, 0, 0, 0};
   bool fused = false;  // the last slice ran the fused F-engine
   float mixed_scale = 1.0f;
   std::string err;
@@ -131,6 +133,11 @@ struct sfxg_ctx {
   int rc_nc = -1, rc_nchan = -1, rc_window = -1;
   uint64_t rc_version = ~(uint64_t)0;
   bool rc_normalize = false;  // normalize as the prepared slice uses it
+  // phased array (Correlation_core_phased) as of sfxg_set_parameters
+  bool phased = false;
+  DBuf ph_blocks, ph_rows;
+  std::vector<PhasedBlock> h_ph_blocks;  // host staging of the enqueued slice
+  std::vector<PhasedRow> h_ph_rows;
   DBuf rc_tw, rc_mask, rc_window_buf, chan_out;
   // host staging of the enqueued slice (kept until the next slice)
   std::vector<double> h_model, h_offsets;
@@ -239,7 +246,8 @@ void sfxg_destroy(sfxg_ctx *ctx) {
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
                   &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
                   &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples, &ctx->chan_scratch,
-                  &ctx->tiles, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out};
+                  &ctx->tiles, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out,
+                  &ctx->ph_blocks, &ctx->ph_rows};
   for (DBuf *b : bufs) b->release();
   sync_slice(ctx);
   if (ctx->done) (void)hipEventDestroy(ctx->done);
@@ -260,6 +268,7 @@ int sfxg_set_options(sfxg_ctx *ctx, const sfxg_options *opt) {
   if (opt->precision < SFXG_PREC_FP32 || opt->precision > SFXG_PREC_SPLIT_FP16)
     return ctx->fail(SFXG_ERR_ARG, "unsupported precision %d", opt->precision);
   if (opt->chunk_mib < 0) return ctx->fail(SFXG_ERR_ARG, "negative chunk_mib");
+  if (opt->phased_array != 0 && opt->phased_array != 1) return ctx->fail(SFXG_ERR_ARG, "phased_array is 0 or 1");
   ctx->opt = *opt;
   return SFXG_OK;
 }
@@ -296,6 +305,12 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
     return ctx->fail(SFXG_ERR_ARG, "fft_size_correlation must be a power of two in [256, 32768]");
   if (p.window < SFXG_WINDOW_NONE || p.window > SFXG_WINDOW_HANN)
     return ctx->fail(SFXG_ERR_ARG, "unknown window %d", p.window);
+  const bool phased = ctx->opt.phased_array != 0;
+  if (phased && pulsar)
+    return ctx->fail(SFXG_ERR_ARG, "pulsar binning cannot be used in phased array mode");
+  // Correlation_core_phased builds no lag window or mask (correlation_core_phased.cc:53-76)
+  if (phased && p.number_channels != nc)
+    return ctx->fail(SFXG_ERR_ARG, "phased array mode takes number_channels == fft_size_correlation");
   // integration_write re-channelises in the lag domain when nchan < N_c
   // (correlation_core.cc:408-440)
   if (p.number_channels != nc && !rechannel_supported(nc, p.number_channels))
@@ -359,14 +374,23 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
   ctx->total_samples = (int64_t)ctx->n_integration * nc;
   if (ctx->n_integration <= 0) return ctx->fail(SFXG_ERR_ARG, "integration shorter than two FFTs");
 
-  // baselines (correlation_core.cc:149-170)
+  // baselines (correlation_core.cc:149-170); the phased core's rows are one
+  // per sub-integration plus one, each written as stream 0's auto-correlation
+  // (correlation_core_phased.cc:84-98)
+  ctx->phased = phased;
   ctx->bl_first.clear();
   ctx->bl_second.clear();
-  for (int i = 0; i < ctx->M; i++) {
+  if (phased) {
+    if (ctx->n_sub <= 0) return ctx->fail(SFXG_ERR_ARG, "sub-integration shorter than one FFT");
+    const int rows = ctx->n_integration / ctx->n_sub + 1;
+    ctx->bl_first.assign(rows, 0);
+    ctx->bl_second.assign(rows, 0);
+  }
+  for (int i = 0; i < ctx->M && !phased; i++) {
     ctx->bl_first.push_back(i);
     ctx->bl_second.push_back(i);
   }
-  for (int i = 0; i < ctx->M; i++) {
+  for (int i = 0; i < ctx->M && !phased; i++) {
     for (int j = i + 1; j < ctx->M; j++) {
       const int s1 = ctx->st[i].station_number, s2 = ctx->st[j].station_number;
       if (s1 == s2) continue;
@@ -501,6 +525,8 @@ static std::string tune_key(const sfxg_ctx *ctx, bool fused) {
 // stream), and the launch parameters of the device work (ctx->lp).
 static int prepare_slice(sfxg_ctx *ctx) {
   if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
+  if ((ctx->opt.phased_array != 0) != ctx->phased)
+    return ctx->fail(SFXG_ERR_STATE, "phased_array changed after sfxg_set_parameters: set the parameters again");
   for (int i = 0; i < ctx->M; i++)
     if (!ctx->plans[i].ready) return ctx->fail(SFXG_ERR_STATE, "stream %d has not been pushed", i);
   for (int i = 1; i < ctx->M; i++)
@@ -802,6 +828,36 @@ static int prepare_slice(sfxg_ctx *ctx) {
   fold_start.insert(fold_start.end(), ctx->folds.begin(), ctx->folds.end());
   HIPCHK(ctx, hipMemcpyAsync(ctx->fold_buf.p, fold_start.data(), fold_start.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, s));
+  // phased array: blocks and rows of every chunk (correlation_core_phased.cc:116-126)
+  lp.ph_b0.clear();
+  lp.ph_nb.clear();
+  lp.ph_r0.clear();
+  lp.ph_nr.clear();
+  if (ctx->phased) {
+    if (ctx->opt.precision != SFXG_PREC_FP32)
+      return ctx->fail(SFXG_ERR_ARG, "phased array mode takes SFXG_PREC_FP32");
+    ctx->h_ph_blocks.clear();
+    ctx->h_ph_rows.clear();
+    size_t max_nb = 1;
+    for (const Chunk &ch : lp.chunks) {
+      const size_t b0 = ctx->h_ph_blocks.size(), r0 = ctx->h_ph_rows.size();
+      phased_tables(ch.seg0, ch.seg1, ctx->n_sub, ctx->h_ph_blocks, ctx->h_ph_rows);
+      lp.ph_b0.push_back((int32_t)b0);
+      lp.ph_nb.push_back((int32_t)(ctx->h_ph_blocks.size() - b0));
+      lp.ph_r0.push_back((int32_t)r0);
+      lp.ph_nr.push_back((int32_t)(ctx->h_ph_rows.size() - r0));
+      max_nb = std::max(max_nb, ctx->h_ph_blocks.size() - b0);
+    }
+    for (const PhasedRow &r : ctx->h_ph_rows)
+      if (r.row < 0 || r.row >= B) return ctx->fail(SFXG_ERR_STATE, "phased row %d outside %d rows", r.row, B);
+    HIPCHK(ctx, ctx->ph_blocks.ensure(std::max<size_t>(ctx->h_ph_blocks.size(), 1) * sizeof(PhasedBlock)));
+    HIPCHK(ctx, ctx->ph_rows.ensure(std::max<size_t>(ctx->h_ph_rows.size(), 1) * sizeof(PhasedRow)));
+    HIPCHK(ctx, ctx->partial.ensure(max_nb * (size_t)(nc + 1) * sizeof(float2)));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->ph_blocks.p, ctx->h_ph_blocks.data(),
+                               ctx->h_ph_blocks.size() * sizeof(PhasedBlock), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->ph_rows.p, ctx->h_ph_rows.data(), ctx->h_ph_rows.size() * sizeof(PhasedRow),
+                               hipMemcpyHostToDevice, s));
+  }
   // launch parameters of the device work
   lp.fused = fused;
   lp.mixed = mixed;
@@ -914,7 +970,12 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
   const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
   const int64_t sstride = lp.chunk_segs * ctx->S;  // spectra elements per stream
   const sfxg_pulsar &ps = ctx->psr;
-  if (lp.mixed && !ctx->pulsar) {
+  if (ctx->phased) {
+    HIPCHK(ctx, launch_phased(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
+                              ctx->ph_blocks.as<PhasedBlock>() + lp.ph_b0[c], lp.ph_nb[c],
+                              ctx->ph_rows.as<PhasedRow>() + lp.ph_r0[c], lp.ph_nr[c], ctx->partial.as<float2>(),
+                              ctx->vis.as<float2>(), s));
+  } else if (lp.mixed && !ctx->pulsar) {
     for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
       const int ng = std::min(lp.k_group, ch.fold1 - g0);
       HIPCHK(ctx, launch_xstep_mfma(ctx->opt.precision, ctx->spectra.p, lp.job.spec_prec != 0, sstride, ctx->S, ch.seg0, nc + 1,
@@ -981,6 +1042,10 @@ static int enqueue_slice(sfxg_ctx *ctx, hipStream_t s, bool timing) {
   }
   const int32_t *blf = ctx->bl.as<int32_t>(), *bls = ctx->bl.as<int32_t>() + B;
   HIPCHK(ctx, mark(SFXG_KERNEL_NORMALISE));
+  if (ctx->phased) {  // Correlation_core_phased writes the sums as they are
+    HIPCHK(ctx, mark(SFXG_N_KERNELS));
+    return SFXG_OK;
+  }
   HIPCHK(ctx, launch_auto_norms(ctx->vis.as<float2>(), M, nc + 1, nc, ctx->nbins, B, ctx->norms.as<double>(), s));
   HIPCHK(ctx, launch_normalise(ctx->vis.as<float2>(), ctx->norms.as<double>(), blf, bls, ctx->flags.as<double>(), M,
                                B, nc + 1, ctx->nbins, ctx->norm_out.as<float2>(), s));
@@ -1070,6 +1135,13 @@ int sfxg_number_of_baselines(sfxg_ctx *ctx, int32_t *n) {
   return SFXG_OK;
 }
 
+// The rows integration_write stores: the phased core's sums, the lag
+// re-channelised rows, or the normalised visibilities.
+static const void *written_rows(const sfxg_ctx *ctx) {
+  if (ctx->phased) return ctx->vis.p;
+  return ctx->nchan != ctx->nc ? ctx->chan_out.p : ctx->norm_out.p;
+}
+
 // integration_write (correlation_core.cc:316-477)
 int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) {
   if (!ctx || !dst) return SFXG_ERR_ARG;
@@ -1080,8 +1152,7 @@ int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) 
   sync_slice(ctx);
   const int M = ctx->M, B = ctx->B, nch = ctx->nchan + 1;
   std::vector<float2> vis((size_t)ctx->nbins * B * nch);
-  HIPCHK(ctx, hipMemcpy(vis.data(), ctx->nchan != ctx->nc ? ctx->chan_out.p : ctx->norm_out.p,
-                        vis.size() * sizeof(float2), hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(vis.data(), written_rows(ctx), vis.size() * sizeof(float2), hipMemcpyDeviceToHost));
   std::vector<int32_t> lv, ts;
   int rc = stats_of(ctx, lv, ts);
   if (rc) return rc;
@@ -1224,14 +1295,13 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
     case SFXG_STAGE_NORMALISED:
       need = (size_t)ctx->nbins * ctx->B * (ctx->nc + 1) * sizeof(float2);
       if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
-      HIPCHK(ctx, hipMemcpy(dst, stage == SFXG_STAGE_ACCUMULATED ? ctx->vis.p : ctx->norm_out.p, need,
-                            hipMemcpyDeviceToHost));
+      HIPCHK(ctx, hipMemcpy(dst, stage == SFXG_STAGE_ACCUMULATED || ctx->phased ? ctx->vis.p : ctx->norm_out.p,
+                            need, hipMemcpyDeviceToHost));
       break;
     case SFXG_STAGE_CHANNELS:
       need = (size_t)ctx->nbins * ctx->B * (ctx->nchan + 1) * sizeof(float2);
       if (cap < need) return ctx->fail(SFXG_ERR_ARG, "buffer too small");
-      HIPCHK(ctx, hipMemcpy(dst, ctx->nchan != ctx->nc ? ctx->chan_out.p : ctx->norm_out.p, need,
-                            hipMemcpyDeviceToHost));
+      HIPCHK(ctx, hipMemcpy(dst, written_rows(ctx), need, hipMemcpyDeviceToHost));
       break;
     default:
       return ctx->fail(SFXG_ERR_ARG, "unknown stage %d", stage);

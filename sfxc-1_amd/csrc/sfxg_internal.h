@@ -266,6 +266,25 @@ hipError_t launch_normalise(const float2 *vis, const double *norms, const int32_
 bool rechannel_supported(int nc, int nchan);
 hipError_t launch_rechannel(const float2 *in, float2 *out, int rows, int nc, int nchan, const float *mask,
                             const float *window, int normalize, const float2 *tw, hipStream_t s);
+// Phased-array core (sfxg_phased.hip, correlation_core_phased.cc:13-126).
+// A block is up to kPhasedBlock consecutive segments [seg0, seg1) of one
+// row (absolute segment indices); a row entry lists the row's blocks
+// [b0, b1) within its chunk's block table.
+constexpr int kPhasedBlock = 64;
+struct PhasedBlock {
+  int64_t seg0, seg1;
+};
+struct PhasedRow {
+  int32_t row, b0, b1, pad;
+};
+// Appends the blocks and rows of chunk [seg0, seg1) (rows of n_sub segments);
+// block indices in the row entries count from the chunk's first block.
+void phased_tables(int64_t seg0, int64_t seg1, int n_sub, std::vector<PhasedBlock> &blocks,
+                   std::vector<PhasedRow> &rows);
+// Adds the chunk's spectra (spectra[stream][seg - seg_base][ch]) to out[row][ch].
+hipError_t launch_phased(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
+                         int n_ch, int M, const PhasedBlock *blocks, int n_blocks, const PhasedRow *rows, int n_rows,
+                         float2 *partial, float2 *out, hipStream_t s);
 // create_window of Correlation_core (correlation_core.cc:668-710,761-809):
 // the 2 nchan-point lag window for the delay-correction window `window`
 // applied over 2 fft_size points. Returns false for an unknown window.

@@ -55,7 +55,7 @@ class PulsarC(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("lut", C.c_int32), ("precision", C.c_int32), ("keep_time_series", C.c_int32),
-                ("chunk_mib", C.c_int32)]
+                ("chunk_mib", C.c_int32), ("phased_array", C.c_int32)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -154,8 +154,9 @@ class Context:
             msg = lib().sfxg_last_error(self._h)
             raise SfxgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def set_options(self, lut: int = 0, precision: int = 0, keep_time_series: bool = False, chunk_mib: int = 0):
-        o = Options(lut, precision, int(keep_time_series), chunk_mib)
+    def set_options(self, lut: int = 0, precision: int = 0, keep_time_series: bool = False, chunk_mib: int = 0,
+                    phased_array: bool = False):
+        o = Options(lut, precision, int(keep_time_series), chunk_mib, int(phased_array))
         self._check(lib().sfxg_set_options(self._h, C.byref(o)), "sfxg_set_options")
 
     def set_parameters(self, job: CorrelationParameters, pulsar: Optional[Pulsar] = None):
@@ -366,8 +367,9 @@ def synthesize_raw(device: int, ptr: int, n_bytes: int, bits: int, first_sample:
 
 def correlate_slice(ctx: Context, job: CorrelationParameters, headers: Sequence[bytes], lut: int = 0,
                     pulsar: Optional[Pulsar] = None, keep_time_series: bool = False, precision: int = 0,
-                    chunk_mib: int = 0) -> bytes:
-    ctx.set_options(lut=lut, precision=precision, keep_time_series=keep_time_series, chunk_mib=chunk_mib)
+                    chunk_mib: int = 0, phased_array: bool = False) -> bytes:
+    ctx.set_options(lut=lut, precision=precision, keep_time_series=keep_time_series, chunk_mib=chunk_mib,
+                    phased_array=phased_array)
     ctx.set_parameters(job, pulsar)
     for i, h in enumerate(headers):
         ctx.push_stream(i, h)

@@ -1,7 +1,8 @@
-"""Lag re-channelisation on the GPU (integration_write with
+"""The two output forms beside the normal correlator, on the GPU against the
+oracle through the C ABI: lag re-channelisation (integration_write with
 fft_size_correlation > number_channels, correlation_core.cc:417-440, and the
-Mask_parameters of control_parameters.h:107-114) against the oracle, through
-the C ABI."""
+Mask_parameters of control_parameters.h:107-114) and the phased-array core
+(Correlation_core_phased, correlation_core_phased.cc:13-126)."""
 import numpy as np
 import pytest
 
@@ -123,5 +124,83 @@ def test_rechannelisation_rejects_unsupported(ctx):
     ctx.set_parameters(job)
     for i, h in enumerate(hdrs):
         ctx.push_stream(i, h)
+    with pytest.raises(capi.SfxgError):
+        ctx.correlate()
+
+
+def phased_job(nc, sub_usec, stations, pols, window=WINDOW_HANN, seed=31):
+    return configs.build_slices(seed=seed, n_stations=stations, pols=list(pols), fs=32_000_000, bw=16_000_000,
+                                nchan=nc, seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2,
+                                sky_freq=5_000_000_000, offsets=[0.0, 2.5e-6, -1.75e-6][:stations],
+                                rates=[0.0, 2e-7, -1e-7][:stations], window=window, sub_int_usec=sub_usec)[0]
+
+
+@pytest.mark.parametrize("nc,sub_usec,stations,pols,window", [
+    (1024, 10_000, 3, ("R",), WINDOW_HANN),     # 7 rows, the last one partial
+    (4096, 62_500, 2, ("R", "L"), WINDOW_HANN),  # one row
+    (2048, 15_616, 2, ("R",), WINDOW_NONE),     # 976 segments in rows of 244: four full rows, an empty fifth
+])
+def test_phased_array(ctx, nc, sub_usec, stations, pols, window):
+    print(f"case phased nc={nc} sub={sub_usec} stations={stations} pols={pols} window={window}", flush=True)
+    job, hdrs = phased_job(nc, sub_usec, stations, pols, window)
+    ref = O.run_slice(job, hdrs, lut=LUT_CONFIG, phased=True)
+    out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, phased_array=True)
+    rows = ref["vis"][0].shape[0]
+    assert ctx.number_of_baselines() == rows
+    heads_g, vis_g = _phased_vis(job, out, rows)
+    heads_r, vis_r = _phased_vis(job, ref["output"], rows)
+    assert heads_g == heads_r
+    full = [r for r in range(rows) if np.any(vis_r[r] != 0)]
+    if window == WINDOW_NONE:
+        assert full == list(range(rows - 1))
+    for r in range(rows):
+        if r not in full:
+            assert np.all(vis_g[r] == 0), r
+    compare_vis(vis_g[full], vis_r[full], label=f"phased nc={nc} rows={rows}")
+
+
+def _phased_vis(job, out, rows):
+    """integration_write bytes of a phased slice: headers and the rows."""
+    import copy
+    j = copy.copy(job)
+    j.baselines = lambda: [(0, 0)] * rows
+    heads, vis = vis_from_bytes(j, out)
+    return heads, vis[0]
+
+
+def test_phased_chunked_matches_unchunked(ctx):
+    """Rows carry over the chunks of a slice: with a 1 MiB spectra budget
+    (one sub-integration per chunk, boundaries at buffer ends, not at row
+    starts) the rows agree with one chunk to FP32 rounding; the block sums
+    split where a chunk ends."""
+    job, hdrs = phased_job(1024, 10_000, 3, ("R",))
+    a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, phased_array=True)
+    b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, phased_array=True, chunk_mib=1)
+    rows = ctx.number_of_baselines()
+    ha, va = _phased_vis(job, a, rows)
+    hb, vb = _phased_vis(job, b, rows)
+    assert ha == hb
+    compare_vis(vb, va, label="phased chunked")
+
+
+def test_phased_rejects(ctx):
+    job, hdrs = phased_job(1024, 10_000, 2, ("R",))
+    ctx.set_options(lut=LUT_CONFIG, phased_array=True)
+    job.number_channels = 256
+    with pytest.raises(capi.SfxgError):
+        ctx.set_parameters(job)
+    job.number_channels = 1024
+    psr = Pulsar(nbins=4, gate_start=0.1, gate_stop=0.6, tmid=ticks_to_mjd(job.start_time) - 0.001, DM=68.0,
+                 ref_phase=0.3, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+    with pytest.raises(capi.SfxgError):
+        ctx.set_parameters(job, psr)
+    ctx.set_options(lut=LUT_CONFIG, precision=1, phased_array=True)
+    ctx.set_parameters(job)
+    for i, h in enumerate(hdrs):
+        ctx.push_stream(i, h)
+    with pytest.raises(capi.SfxgError):
+        ctx.correlate()
+    # options changed after the parameters
+    ctx.set_options(lut=LUT_CONFIG, phased_array=False)
     with pytest.raises(capi.SfxgError):
         ctx.correlate()

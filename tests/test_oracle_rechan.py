@@ -216,3 +216,42 @@ def test_table_sizes_checked():
         O.lag_tables(_Job(32, 128, mask=MaskParameters(mask=[1.0] * 128)))
     with pytest.raises(ValueError):
         O.lag_tables(_Job(32, 128, mask=MaskParameters(window=[1.0] * 32)))
+
+
+# ---------------------------------------------------------------------------
+# phased-array core -- correlation_core_phased.cc:13-50,101-126
+# ---------------------------------------------------------------------------
+class _PhasedJob:
+    def __init__(self, M, fft, nffts, nsub):
+        self.station_streams = list(range(M))
+        self.fft_size_correlation = fft
+        self._n, self._s = nffts, nsub
+
+    def baselines(self):
+        return [(i, i) for i in range(len(self.station_streams))]
+
+    def number_ffts_in_integration(self):
+        return self._n
+
+    def number_ffts_in_sub_integration(self):
+        return self._s
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("nffts,nsub", [(50, 7), (48, 12), (30, 100)])
+def test_phased_sums(prec, nffts, nsub):
+    M, fft = 3, 64
+    job = _PhasedJob(M, fft, nffts, nsub)
+    rng = np.random.default_rng(nffts + nsub)
+    spectra = [rng.standard_normal((nffts, fft + 4, 2)) for _ in range(M)]
+    rows = O.phased_correlate(job, spectra, nffts, prec)
+    nrows = nffts // nsub + 1
+    assert rows.shape == (nrows, fft + 1, 2)
+    x = sum(s.astype(np.float32 if prec == "f32" else np.float64).astype(np.float64) for s in spectra)
+    ref = np.zeros((nrows, fft + 1, 2))
+    for seg in range(nffts):
+        ref[seg // nsub] += x[seg, :fft + 1]
+    tol = 1e-12 if prec == "f64" else 2e-6
+    assert np.max(np.abs(rows - ref)) <= tol * np.sqrt(np.mean(ref ** 2)) * 10
+    if nffts % nsub == 0:
+        assert np.all(rows[-1] == 0)
