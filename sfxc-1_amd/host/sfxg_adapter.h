@@ -8,7 +8,9 @@
 //   Correlation_core   (sfxc/include/correlation_core.h:27-59): set_parameters,
 //                      do_task (runs Delay_correction and the X-step for the
 //                      whole slice), finished, number_of_baselines and the
-//                      integration_write bytes.
+//                      integration_write bytes; set_mask_parameters takes the
+//                      Mask_parameters (sfxc/include/control_parameters.h:107-114)
+//                      that the node points every slice's parameters at.
 // Delay_correction has no front-end: its work runs inside Correlation_core's
 // do_task on the device. Errors throw sfxg_adapter::Error (the reference
 // aborts through sfxc_abort, sfxc/src/utils.cc:44-95).
@@ -64,6 +66,12 @@ class Bit2float_worker {
   std::vector<uint8_t> buf_;
 };
 
+// Mask_parameters (sfxc/include/control_parameters.h:107-114).
+struct Mask_parameters {
+  bool normalize = false;
+  std::vector<double> mask, window;
+};
+
 class Correlation_core {
  public:
   explicit Correlation_core(Context &ctx) : ctx_(ctx) {}
@@ -72,6 +80,14 @@ class Correlation_core {
     ctx_.check(sfxg_set_parameters(ctx_.get(), &p, models.data(), uvw.empty() ? nullptr : uvw.data(), pulsar),
                "sfxg_set_parameters");
     finished_ = false;
+  }
+  // Correlator_node_controller's MPI_TAG_MASK_PARAMETERS handler
+  // (sfxc/src/correlator_node_controller.cc:72-77).
+  void set_mask_parameters(const Mask_parameters &m) {
+    ctx_.check(sfxg_set_mask(ctx_.get(), m.normalize ? 1 : 0, m.mask.empty() ? nullptr : m.mask.data(),
+                             (int32_t)m.mask.size(), m.window.empty() ? nullptr : m.window.data(),
+                             (int32_t)m.window.size()),
+               "sfxg_set_mask");
   }
   void do_task() {
     ctx_.check(sfxg_correlate(ctx_.get()), "sfxg_correlate");
