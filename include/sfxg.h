@@ -190,6 +190,23 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
                         const sfxg_delay_model *models, const double *uvw,
                         const sfxg_pulsar *pulsar);
 
+/* The phase centres after the pointing centre of a multi-phase-centre slice
+ * (params->n_phase_centers > 1): replaces the further sources of each
+ * stream's Delay_table_akima (src/delay_table_akima.cc:114-136) and their
+ * uvw tables. Call after sfxg_set_parameters, with n_phase_centers equal to
+ * the parameters'. models: (n_phase_centers - 1) x n_streams, centre-major
+ * (models[(j - 1) * n_streams + i] is stream i towards centre j); the
+ * pointing centre's are those sfxg_set_parameters took. uvw: n_phase_centers
+ * x n_streams x 3 doubles (NULL for zeros), replacing the uvw of
+ * sfxg_set_parameters. source_nr: n_phase_centers source numbers written at
+ * the head of each centre's record (NULL for 0). At every sub-integration the
+ * cross-correlations go to the pointing centre as they are and to centre j
+ * uv-shifted (Correlation_core::sub_integration / uvshift,
+ * src/correlation_core.cc:515-592); each centre is then normalised and
+ * written as a record of its own (:52-66). FP32 only. */
+int sfxg_set_phase_centers(sfxg_ctx *ctx, int32_t n_phase_centers, const sfxg_delay_model *models,
+                           const double *uvw, const int32_t *source_nr);
+
 /* Mask_parameters (include/control_parameters.h:107-114) of the job: replaces
  * the correlator node's receipt of MPI_TAG_MASK_PARAMETERS
  * (src/correlator_node_controller.cc:72-77), after which every slice's

@@ -50,6 +50,12 @@ class CoreParams(C.Structure):
                 ("bl_first", C.POINTER(C.c_int32)), ("bl_second", C.POINTER(C.c_int32))]
 
 
+class PcParamsC(C.Structure):
+    _fields_ = [("n_phase_centers", C.c_int32), ("sideband", C.c_int32), ("sample_rate", C.c_int32),
+                ("pad", C.c_int32), ("start_ticks", C.c_int64), ("channel_freq", C.c_int64),
+                ("models", C.POINTER(DelayModelC)), ("weights", C.POINTER(C.c_float))]
+
+
 class RechanParamsC(C.Structure):
     _fields_ = [("fft_size", C.c_int32), ("nchan", C.c_int32), ("normalize", C.c_int32), ("pad", C.c_int32),
                 ("mask", C.POINTER(C.c_float)), ("window", C.POINTER(C.c_float))]
@@ -247,6 +253,49 @@ def phased_correlate(job, spectra: List[np.ndarray], nseg: int, precision="f32")
     n = f(C.byref(cp), ptrs, C.c_int64(nseg), _rptr(rows, precision))
     assert n == nrows
     return rows.reshape(nrows, nf, 2)
+
+
+def eval_rate(model, ticks: int, precision="f64") -> float:
+    keep = []
+    m = _model_c(model, keep)
+    f = getattr(lib(precision), f"or_eval_rate_{precision}")
+    f.restype = C.c_double
+    return f(C.byref(m), C.c_int64(ticks))
+
+
+def pc_weights(window: int, n: int, precision="f32") -> np.ndarray:
+    """create_weights (correlation_core.cc:712-759)."""
+    w = np.zeros(n, dtype=np.float32)
+    if getattr(lib(precision), f"or_pc_weights_{precision}")(C.c_int(window), C.c_int(n), _ptr(w, C.c_float)) != 0:
+        raise ValueError(f"invalid window {window}")
+    return w
+
+
+def correlate_pc(job, spectra: List[np.ndarray], seg_per_chunk, precision="f32"):
+    """Correlation_core with the job's extra phase centres:
+    (n_phase_centers, n_baselines, N_c + 1, 2) accumulators."""
+    keep = []
+    cp, bl = _core_params(job, keep)
+    nf = job.fft_size_correlation + 1
+    M = len(job.station_streams)
+    centres = [job.delay_models] + [pc.delay_models for pc in job.phase_centers]
+    n_pc = len(centres)
+    models = (DelayModelC * (n_pc * M))()
+    for j, ms in enumerate(centres):
+        assert len(ms) == M
+        for i, m in enumerate(ms):
+            models[j * M + i] = _model_c(m, keep)
+    w = pc_weights(job.window, job.fft_size_correlation, precision)
+    pp = PcParamsC(n_pc, sb_sign(job.sideband), job.sample_rate, 0, job.start_time, job.channel_freq, models,
+                   _ptr(w, C.c_float))
+    arrs = [np.ascontiguousarray(s, dtype=_dtype(precision)) for s in spectra]
+    ptrs = (C.POINTER(C.c_float if precision == "f32" else C.c_double) * len(arrs))(
+        *[_rptr(a, precision) for a in arrs])
+    vis = np.zeros(2 * n_pc * len(bl) * nf, dtype=_dtype(precision))
+    spc = np.ascontiguousarray(seg_per_chunk, dtype=np.int32)
+    getattr(lib(precision), f"or_correlate_pc_{precision}")(C.byref(cp), C.byref(pp), ptrs, _ptr(spc, C.c_int32),
+                                                            C.c_int32(len(spc)), _rptr(vis, precision))
+    return vis.reshape(n_pc, len(bl), nf, 2)
 
 
 def find_invalid(job, invalid: List[tuple]):
@@ -479,6 +528,20 @@ def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32
         rows = phased_correlate(job, spectra, dc[0]["nseg"], precision)
         outputs.append(write_output(job, 0, job.integration_nr, rows, stats5, weights, uvw, bl=bl))
         vis_bins = norm_bins = chan_bins = [rows]
+    elif getattr(job, "phase_centers", None):
+        # one record per phase centre (correlation_core.cc:52-66)
+        acc = correlate_pc(job, spectra, dc[0]["seg_per_chunk"], precision)
+        vis_bins, norm_bins, chan_bins = [], [], []
+        sources = [job.source_nr] + [pc.source_nr for pc in job.phase_centers]
+        uvws = [uvw] + [pc.uvw for pc in job.phase_centers]
+        for j in range(acc.shape[0]):
+            norm = normalize(job, acc[j], levels4, nflag, precision)
+            chan = channels(norm)
+            u = None if uvws[j] is None else np.asarray(uvws[j], dtype=np.float64).ravel()
+            outputs.append(write_output(job, sources[j], job.integration_nr, chan, stats5, weights, u))
+            vis_bins.append(acc[j])
+            norm_bins.append(norm)
+            chan_bins.append(chan)
     elif pulsar is None:
         vis = correlate(job, spectra, dc[0]["seg_per_chunk"], precision)
         norm = normalize(job, vis, levels4, nflag, precision)

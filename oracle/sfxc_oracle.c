@@ -22,6 +22,16 @@
  *                      tone to DC) and a numpy FP64 restatement on random data.
  *   or_correlate       correlation_core.cc:20-73,241-273,515-559; pinned by a
  *                      FP64 numpy einsum over the same spectra.
+ *   or_correlate_pc    the same with extra phase centres: sub_integration's
+ *                      uv shift (correlation_core.cc:515-592); pinned by a
+ *                      zero shift (copies of the pointing centre), a numpy
+ *                      FP64 restatement of the rotation (direct exponentials,
+ *                      not the recurrence) and, on the GPU path, by a second
+ *                      source that only the shifted centre fringe-stops.
+ *   or_pc_weights      create_weights (correlation_core.cc:712-759); pinned
+ *                      by numpy.correlate of the window in FP64.
+ *   or_eval_rate       gsl_spline_eval_deriv of the piecewise cubic; pinned
+ *                      by finite differences.
  *   or_normalize       correlation_core.cc:275-314; pinned by closed form.
  *   or_find_invalid    correlation_core.cc:599-665; pinned by brute-force
  *                      interval arithmetic on small invalid lists.
@@ -404,6 +414,18 @@ double FN(or_eval_delay)(const or_delay_model *m, int64_t ticks) {
   return eval_piece(m, m->delay, ticks, 0.0);
 }
 
+/* Delay_table_akima::rate (delay_table_akima.cc:128-136): the derivative of
+ * the piece at ticks, in seconds per second. */
+double FN(or_eval_rate)(const or_delay_model *m, int64_t ticks) {
+  double sec = (ticks - m->scan_begin_ticks) / CLOCK_RATE;
+  int i = (int)floor(sec / m->interval);
+  if (i < 0) i = 0;
+  if (i > m->n - 1) i = m->n - 1;
+  double u = sec - i * m->interval;
+  const double *c = m->delay + 4 * i;
+  return c[1] + u * (2 * c[2] + u * 3 * c[3]);
+}
+
 /* create_window (delay_correction.cc:299-341) */
 void FN(or_window)(int window, int n, real *w) {
   for (int i = 0; i < n; i++) {
@@ -561,12 +583,17 @@ int64_t FN(or_delay_correction)(const or_dc_params *p, const or_delay_model *m, 
  * consumed in chunks of seg_per_chunk segments. vis receives
  * phase_centers[0] (n_baselines x (N_c + 1) complex, before normalisation).
  */
-void FN(or_correlate)(const or_core_params *p, const real *const *spectra, const int32_t *seg_per_chunk,
-                      int32_t n_chunks, real *vis) {
+static void sub_integration(const or_core_params *p, const or_pc_params *pp, const cplx *acc, cplx *pc,
+                            int64_t previous_fft, int64_t current_fft);
+
+static void correlate(const or_core_params *p, const or_pc_params *pp, const real *const *spectra,
+                      const int32_t *seg_per_chunk, int32_t n_chunks, real *vis) {
   const int nf = p->fft_size + 1, B = p->n_baselines, M = p->n_streams;
+  const int n_pc = pp ? pp->n_phase_centers : 1;
   cplx *acc = (cplx *)calloc((size_t)B * nf, sizeof(cplx));
   cplx *pc = (cplx *)vis;
-  memset(pc, 0, sizeof(cplx) * (size_t)B * nf);
+  memset(pc, 0, sizeof(cplx) * (size_t)n_pc * B * nf);
+  int64_t previous_fft = 0;
   cplx *conj = (cplx *)malloc(sizeof(cplx) * nf);
   int64_t current_fft = 0, next_sub = 1, seg = 0;
   for (int c = 0; c < n_chunks; c++) {
@@ -594,17 +621,133 @@ void FN(or_correlate)(const or_core_params *p, const real *const *spectra, const
     if (current_fft == p->number_ffts_in_integration) fold = 1;
     else if (current_fft >= next_sub * p->number_ffts_in_sub_integration) { fold = 1; next_sub++; }
     if (fold) {
-      for (size_t i = 0; i < (size_t)B * nf; i++) {
-        pc[i].re += acc[i].re;
-        pc[i].im += acc[i].im;
-        acc[i].re = 0;
-        acc[i].im = 0;
+      if (n_pc > 1) {
+        sub_integration(p, pp, acc, pc, previous_fft, current_fft);
+      } else {
+        for (size_t i = 0; i < (size_t)B * nf; i++) {
+          pc[i].re += acc[i].re;
+          pc[i].im += acc[i].im;
+        }
       }
+      memset(acc, 0, sizeof(cplx) * (size_t)B * nf);
+      previous_fft = current_fft;
     }
   }
   (void)M;
   free(acc);
   free(conj);
+}
+
+void FN(or_correlate)(const or_core_params *p, const real *const *spectra, const int32_t *seg_per_chunk,
+                      int32_t n_chunks, real *vis) {
+  correlate(p, NULL, spectra, seg_per_chunk, n_chunks, vis);
+}
+
+/* or_correlate with pp->n_phase_centers phase centres: vis receives
+ * phase_centers[0 .. n - 1], each n_baselines x (N_c + 1) complex. */
+void FN(or_correlate_pc)(const or_core_params *p, const or_pc_params *pp, const real *const *spectra,
+                         const int32_t *seg_per_chunk, int32_t n_chunks, real *vis) {
+  correlate(p, pp, spectra, seg_per_chunk, n_chunks, vis);
+}
+
+/* uvshift (correlation_core.cc:561-592): out += amplitude * in * exp(i phi_k),
+ * phi_k advanced by an FP64 sine/cosine recurrence. */
+static void uvshift(const or_core_params *p, const or_pc_params *pp, const cplx *in, cplx *out, double ddelay1,
+                    double ddelay2, double rate1, double rate2) {
+  const int nf = p->fft_size + 1;
+  const int sb = pp->sideband;
+  const double base_freq = (double)pp->channel_freq;
+  const double dfreq = pp->sample_rate / (2. * p->fft_size);
+  real amplitude = 1;
+  const int lag = abs((int)round((ddelay1 - ddelay2) * pp->sample_rate));
+  if (lag < p->fft_size && pp->weights[lag] > 1e-4) amplitude = (real)(1. / pp->weights[lag]);
+  double phi = base_freq * (ddelay1 * (1 - rate1) - ddelay2 * (1 - rate2));
+  phi = 2 * M_PI * sb * (phi - floor(phi));
+  const double delta = 2 * M_PI * dfreq * (ddelay1 * (1 - rate1) - ddelay2 * (1 - rate2));
+  double temp = sin(delta / 2);
+  const double a = 2 * temp * temp, b = sin(delta);
+  double cos_phi = cos(phi), sin_phi = sin(phi);
+  for (int i = 0; i < nf; i++) {
+    const real ar = amplitude * in[i].re, ai = amplitude * in[i].im;
+    const real c = (real)cos_phi, s = (real)sin_phi;
+    out[i].re += ar * c - ai * s;
+    out[i].im += ar * s + ai * c;
+    temp = sin_phi - (a * sin_phi - b * cos_phi);
+    cos_phi = cos_phi - (a * cos_phi + b * sin_phi);
+    sin_phi = temp;
+  }
+}
+
+/* sub_integration (correlation_core.cc:515-559) with several phase centres:
+ * autos go to every centre, crosses to the pointing centre as they are and
+ * to centre j uv-shifted by the delay differences at the sub-integration's
+ * middle, tmid = start + tfft * (previous_fft + (current_fft - previous_fft) / 2). */
+static void sub_integration(const or_core_params *p, const or_pc_params *pp, const cplx *acc, cplx *pc,
+                            int64_t previous_fft, int64_t current_fft) {
+  const int nf = p->fft_size + 1, B = p->n_baselines, M = p->n_streams, n_pc = pp->n_phase_centers;
+  const size_t pcs = (size_t)B * nf;
+  const int64_t tfft = (int64_t)p->fft_size * (int64_t)round(CLOCK_RATE / pp->sample_rate);
+  const int64_t tmid =
+      pp->start_ticks + (int64_t)round(tfft * (previous_fft + (current_fft - previous_fft) / 2.));
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < n_pc; j++)
+      for (int k = 0; k < nf; k++) {
+        pc[j * pcs + (size_t)i * nf + k].re += acc[(size_t)i * nf + k].re;
+        pc[j * pcs + (size_t)i * nf + k].im += acc[(size_t)i * nf + k].im;
+      }
+  for (int i = M; i < B; i++) {
+    const int s1 = p->bl_first[i], s2 = p->bl_second[i];
+    for (int k = 0; k < nf; k++) {
+      pc[(size_t)i * nf + k].re += acc[(size_t)i * nf + k].re;
+      pc[(size_t)i * nf + k].im += acc[(size_t)i * nf + k].im;
+    }
+    for (int j = 1; j < n_pc; j++) {
+      const double delay1 = FN(or_eval_delay)(&pp->models[s1], tmid);
+      const double delay2 = FN(or_eval_delay)(&pp->models[s2], tmid);
+      const double ddelay1 = FN(or_eval_delay)(&pp->models[(size_t)j * M + s1], tmid) - delay1;
+      const double ddelay2 = FN(or_eval_delay)(&pp->models[(size_t)j * M + s2], tmid) - delay2;
+      const double rate1 = FN(or_eval_rate)(&pp->models[s1], tmid);
+      const double rate2 = FN(or_eval_rate)(&pp->models[s2], tmid);
+      uvshift(p, pp, acc + (size_t)i * nf, pc + j * pcs + (size_t)i * nf, ddelay1, ddelay2, rate1, rate2);
+    }
+  }
+}
+
+/* create_weights (correlation_core.cc:712-759): the autocorrelation of the
+ * window f(2n - 1, i), i < 2n, zero padded to 4n, through a 4n-point real
+ * transform, |.|^2 and the inverse transform; weights[i] = r[i] / r[0] for
+ * i < n. The window functions are those of create_window. Returns -1 for an
+ * unknown window. */
+static double lw_rect(int n, int i) { return (i >= n / 4 && i < (3 * n) / 4) ? 1.0 : 0.0; }
+static double lw_cos(int n, int i);
+static double lw_hamming(int n, int i);
+static double lw_hann(int n, int i);
+static void irfft(const cplx *in, real *out, int n, cplx *work);
+
+int FN(or_pc_weights)(int window, int n, float *weights) {
+  double (*f)(int, int);
+  switch (window) {
+    case OR_WINDOW_NONE:
+    case OR_WINDOW_RECT: f = lw_rect; break;
+    case OR_WINDOW_COS: f = lw_cos; break;
+    case OR_WINDOW_HAMMING: f = lw_hamming; break;
+    case OR_WINDOW_HANN: f = lw_hann; break;
+    default: return -1;
+  }
+  real *tbuf = (real *)calloc(4 * (size_t)n, sizeof(real));
+  cplx *fbuf = (cplx *)malloc(sizeof(cplx) * (2 * (size_t)n + 1));
+  cplx *work = (cplx *)malloc(sizeof(cplx) * 2 * (size_t)n);
+  for (int i = 0; i < 2 * n; i++) tbuf[i] = (real)f(2 * n - 1, i);
+  rfft(tbuf, fbuf, 4 * n, work);
+  for (int k = 0; k <= 2 * n; k++) {
+    const real re = fbuf[k].re, im = fbuf[k].im;
+    fbuf[k].re = re * re + im * im;  /* conj(F) F */
+    fbuf[k].im = re * im - im * re;
+  }
+  irfft(fbuf, tbuf, 4 * n, work);
+  for (int i = 0; i < n; i++) weights[i] = (float)(tbuf[i] / tbuf[0]);
+  free(tbuf); free(fbuf); free(work);
+  return 0;
 }
 
 /*

@@ -104,6 +104,19 @@ typedef struct {
   const double *offsets;  /* N_c + 1 dispersion offsets, pulsar periods */
 } or_pulsar_params;
 
+/* Extra phase centres of Correlation_core (correlation_core.cc:515-592):
+ * what sub_integration and uvshift read besides the accumulators. */
+typedef struct {
+  int32_t n_phase_centers;
+  int32_t sideband;              /* +1 'U', -1 'L' (correlation_parameters.sideband) */
+  int32_t sample_rate;           /* correlation_parameters.sample_rate */
+  int32_t pad;
+  int64_t start_ticks;           /* correlation_parameters.start_time */
+  int64_t channel_freq;          /* correlation_parameters.channel_freq, Hz */
+  const or_delay_model *models;  /* n_phase_centers x n_streams, centre-major; centre 0 the pointing centre */
+  const float *weights;          /* fft_size values of create_weights (correlation_core.cc:712-759) */
+} or_pc_params;
+
 /* Lag re-channelisation of integration_write when fft_size_correlation >
  * number_channels (correlation_core.cc:417-440): Mask_parameters
  * (control_parameters.h:107-114) and the tables create_mask / create_window

@@ -50,6 +50,19 @@ struct ModelCopy {
   std::vector<double> delay, phase, amp;
 };
 
+// Delay_table_akima::delay / rate (delay_table_akima.cc:114-136) on the
+// piecewise cubic: value and derivative of the piece holding ticks.
+static double model_eval(const ModelCopy &m, int64_t ticks, bool derivative) {
+  const double sec = (double)(ticks - m.scan_begin) / kClockRate;
+  int64_t i = (int64_t)floor(sec / m.interval);
+  if (i < 0) i = 0;
+  if (i > m.n - 1) i = m.n - 1;
+  const double u = sec - (double)i * m.interval;
+  const double *c = &m.delay[4 * i];
+  if (derivative) return c[1] + u * (2 * c[2] + u * 3 * c[3]);
+  return c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+}
+
 const double kLut2[2][4] = {{-7, -2, 2, 7}, {-3.3359, -1, 1, 3.3359}};  // bit2float_worker.cc:4-6
 const double kLut1[2][2] = {{-5, 5}, {-1, 1}};                          // bit2float_worker.cc:8-10
 
@@ -133,6 +146,17 @@ struct sfxg_ctx {
   int rc_nc = -1, rc_nchan = -1, rc_window = -1;
   uint64_t rc_version = ~(uint64_t)0;
   bool rc_normalize = false;  // normalize as the prepared slice uses it
+  // extra phase centres (sfxg_set_phase_centers)
+  int n_pc = 1;
+  bool have_pc = true;
+  std::vector<ModelCopy> pc_models;  // (n_pc - 1) x M, centre-major
+  std::vector<double> pc_uvw;        // n_pc x M x 3
+  std::vector<int32_t> pc_source;    // n_pc
+  std::vector<UvShift> h_uv;         // [fold][centre - 1][baseline] of the enqueued slice
+  DBuf uv_tab;
+  int pcw_window = -1, pcw_nc = -1;  // create_weights cache (window, N_c)
+  std::vector<float> pcw_win;        // the window f(2 N_c - 1, i) as stored (float)
+  std::vector<double> pcw_r;         // autocorrelation by lag, NaN until computed
   // phased array (Correlation_core_phased) as of sfxg_set_parameters
   bool phased = false;
   DBuf ph_blocks, ph_rows;
@@ -247,7 +271,7 @@ void sfxg_destroy(sfxg_ctx *ctx) {
                   &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
                   &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples, &ctx->chan_scratch,
                   &ctx->tiles, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out,
-                  &ctx->ph_blocks, &ctx->ph_rows};
+                  &ctx->ph_blocks, &ctx->ph_rows, &ctx->uv_tab};
   for (DBuf *b : bufs) b->release();
   sync_slice(ctx);
   if (ctx->done) (void)hipEventDestroy(ctx->done);
@@ -270,6 +294,34 @@ int sfxg_set_options(sfxg_ctx *ctx, const sfxg_options *opt) {
   if (opt->chunk_mib < 0) return ctx->fail(SFXG_ERR_ARG, "negative chunk_mib");
   if (opt->phased_array != 0 && opt->phased_array != 1) return ctx->fail(SFXG_ERR_ARG, "phased_array is 0 or 1");
   ctx->opt = *opt;
+  return SFXG_OK;
+}
+
+int sfxg_set_phase_centers(sfxg_ctx *ctx, int32_t n_phase_centers, const sfxg_delay_model *models,
+                           const double *uvw, const int32_t *source_nr) {
+  if (!ctx) return SFXG_ERR_ARG;
+  if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
+  if (n_phase_centers != ctx->n_pc || n_phase_centers < 2)
+    return ctx->fail(SFXG_ERR_ARG, "%d phase centres given, the parameters have %d", n_phase_centers, ctx->n_pc);
+  if (!models) return ctx->fail(SFXG_ERR_ARG, "no delay models");
+  const int M = ctx->M, n = n_phase_centers;
+  ctx->pc_models.assign((size_t)(n - 1) * M, ModelCopy());
+  for (size_t k = 0; k < ctx->pc_models.size(); k++) {
+    const sfxg_delay_model &src = models[k];
+    if (src.n_pieces <= 0 || !src.delay)
+      return ctx->fail(SFXG_ERR_ARG, "phase centre %d, stream %d: empty delay model", (int)(k / M) + 1, (int)(k % M));
+    ModelCopy &m = ctx->pc_models[k];
+    m.scan_begin = src.scan_begin;
+    m.interval = src.interval;
+    m.n = src.n_pieces;
+    m.delay.assign(src.delay, src.delay + 4 * m.n);
+  }
+  ctx->pc_uvw.assign((size_t)n * M * 3, 0.0);
+  if (uvw) std::copy(uvw, uvw + (size_t)n * M * 3, ctx->pc_uvw.begin());
+  ctx->pc_source.assign(n, 0);
+  if (source_nr) std::copy(source_nr, source_nr + n, ctx->pc_source.begin());
+  ctx->have_pc = true;
+  ctx->prepared = false;
   return SFXG_OK;
 }
 
@@ -298,7 +350,12 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
   ctx->prepared = false;
   const sfxg_correlation_parameters &p = *params;
   if (p.n_streams <= 0 || !p.station_streams) return ctx->fail(SFXG_ERR_ARG, "no station streams");
-  if (p.n_phase_centers != 1) return ctx->fail(SFXG_ERR_ARG, "only one phase centre is supported");
+  if (p.n_phase_centers < 1 || p.n_phase_centers > 256)
+    return ctx->fail(SFXG_ERR_ARG, "n_phase_centers %d outside 1 .. 256", p.n_phase_centers);
+  if (p.n_phase_centers > 1 && pulsar)  // control_parameters.cc:156-160
+    return ctx->fail(SFXG_ERR_ARG, "pulsar binning cannot be used together with multiple phase centres");
+  if (p.n_phase_centers > 1 && ctx->opt.phased_array)
+    return ctx->fail(SFXG_ERR_ARG, "the phased-array core has one phase centre");
   if (p.fft_size_delaycor != 256) return ctx->fail(SFXG_ERR_ARG, "fft_size_delaycor must be 256");
   const int nc = p.fft_size_correlation;
   if (nc < 256 || nc > 32768 || (nc & (nc - 1)))
@@ -361,7 +418,10 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
   ctx->nd = p.fft_size_delaycor;
   ctx->nc = nc;
   ctx->nchan = p.number_channels;
-  ctx->nbins = pulsar ? pulsar->nbins + 1 : 1;
+  // one record per pulsar bin or per phase centre
+  ctx->n_pc = p.n_phase_centers;
+  ctx->have_pc = ctx->n_pc == 1;
+  ctx->nbins = pulsar ? pulsar->nbins + 1 : ctx->n_pc;
   const int nr = nr_ffts_per_integration_slice(p.integration_time, p.sample_rate, nc);
   ctx->n_ffts_per_integration = (int64_t)(nc / ctx->nd) * nr;  // delay_correction.cc:244-249
   ctx->n_blocks = ctx->n_ffts_per_integration;
@@ -523,10 +583,88 @@ static std::string tune_key(const sfxg_ctx *ctx, bool fused) {
 // Host part of sfxg_correlate: chunk and invalid-data bookkeeping, device
 // buffers, tables and the slice's uploads (asynchronous, on the context's
 // stream), and the launch parameters of the device work (ctx->lp).
+// create_weights (correlation_core.cc:712-759) at one lag: the
+// autocorrelation of the window f(2N_c - 1, i), i < 2N_c, as stored in FLOAT,
+// over its value at lag 0. The reference takes the whole autocorrelation
+// through FP32 transforms; here the lags a slice needs are summed in FP64.
+static float pc_weight(sfxg_ctx *ctx, int lag) {
+  const int n = ctx->nc, w = ctx->p.window;
+  if (ctx->pcw_window != w || ctx->pcw_nc != n) {
+    ctx->pcw_win.assign(2 * (size_t)n, 0.0f);
+    const int L = 2 * n - 1;
+    for (int i = 0; i < 2 * n; i++) {
+      double v;
+      if (w == SFXG_WINDOW_COS) v = sin(M_PI * i / L);
+      else if (w == SFXG_WINDOW_HAMMING) v = 0.54 - 0.46 * cos(2 * M_PI * i / L);
+      else if (w == SFXG_WINDOW_HANN) v = 0.5 * (1 - cos(2 * M_PI * i / L));
+      else v = (i >= L / 4 && i < (3 * L) / 4) ? 1.0 : 0.0;  // NONE, RECT
+      ctx->pcw_win[i] = (float)v;
+    }
+    ctx->pcw_r.assign(n, std::nan(""));
+    ctx->pcw_window = w;
+    ctx->pcw_nc = n;
+  }
+  auto r = [ctx, n](int l) {
+    if (std::isnan(ctx->pcw_r[l])) {
+      double sum = 0.0;
+      for (int j = 0; j + l < 2 * n; j++) sum += (double)ctx->pcw_win[j] * ctx->pcw_win[j + l];
+      ctx->pcw_r[l] = sum;
+    }
+    return ctx->pcw_r[l];
+  };
+  return (float)(r(lag) / r(0));
+}
+
+// The uv shift of every sub-integration, extra phase centre and cross
+// baseline (Correlation_core::sub_integration and uvshift,
+// correlation_core.cc:515-592) into ctx->h_uv, [fold][centre - 1][baseline].
+static void uv_table(sfxg_ctx *ctx) {
+  const int M = ctx->M, B = ctx->B, npc = ctx->n_pc, nc = ctx->nc;
+  const int nf = (int)ctx->folds.size();
+  ctx->h_uv.assign((size_t)nf * (npc - 1) * B, UvShift{0.0, 0.0, 1.0f, 0});
+  // Time tfft(0., sample_rate); tfft.inc_samples(fft_size())
+  const int64_t tfft = (int64_t)nc * (int64_t)(int)round(kClockRate / ctx->p.sample_rate);
+  const int sb = ctx->p.sideband == 'L' ? -1 : 1;
+  const double base_freq = (double)ctx->p.channel_freq;
+  const double dfreq = ctx->p.sample_rate / (2. * nc);
+  int64_t previous_fft = 0;
+  for (int f = 0; f < nf; f++) {
+    const int64_t current_fft = ctx->folds[f];
+    const int64_t tmid =
+        ctx->p.start_time + (int64_t)round(tfft * (previous_fft + (current_fft - previous_fft) / 2.));
+    for (int b = M; b < B; b++) {
+      const int s1 = ctx->bl_first[b], s2 = ctx->bl_second[b];
+      const double delay1 = model_eval(ctx->models[s1], tmid, false);
+      const double delay2 = model_eval(ctx->models[s2], tmid, false);
+      const double rate1 = model_eval(ctx->models[s1], tmid, true);
+      const double rate2 = model_eval(ctx->models[s2], tmid, true);
+      for (int j = 1; j < npc; j++) {
+        const double ddelay1 = model_eval(ctx->pc_models[(size_t)(j - 1) * M + s1], tmid, false) - delay1;
+        const double ddelay2 = model_eval(ctx->pc_models[(size_t)(j - 1) * M + s2], tmid, false) - delay2;
+        UvShift &u = ctx->h_uv[((size_t)f * (npc - 1) + (j - 1)) * B + b];
+        const int lag = abs((int)round((ddelay1 - ddelay2) * ctx->p.sample_rate));
+        u.amp = 1.0f;
+        if (lag < nc) {
+          const float w = pc_weight(ctx, lag);
+          if (w > 1e-4) u.amp = (float)(1. / w);
+        }
+        const double x = ddelay1 * (1 - rate1) - ddelay2 * (1 - rate2);
+        const double phi = base_freq * x;
+        u.phi = 2 * M_PI * sb * (phi - floor(phi));
+        u.delta = 2 * M_PI * dfreq * x;
+      }
+    }
+    previous_fft = current_fft;
+  }
+}
+
 static int prepare_slice(sfxg_ctx *ctx) {
   if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
   if ((ctx->opt.phased_array != 0) != ctx->phased)
     return ctx->fail(SFXG_ERR_STATE, "phased_array changed after sfxg_set_parameters: set the parameters again");
+  if (!ctx->have_pc) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_phase_centers first (%d phase centres)", ctx->n_pc);
+  if (ctx->n_pc > 1 && ctx->opt.precision != SFXG_PREC_FP32)
+    return ctx->fail(SFXG_ERR_ARG, "multiple phase centres take SFXG_PREC_FP32");
   for (int i = 0; i < ctx->M; i++)
     if (!ctx->plans[i].ready) return ctx->fail(SFXG_ERR_STATE, "stream %d has not been pushed", i);
   for (int i = 1; i < ctx->M; i++)
@@ -828,6 +966,13 @@ static int prepare_slice(sfxg_ctx *ctx) {
   fold_start.insert(fold_start.end(), ctx->folds.begin(), ctx->folds.end());
   HIPCHK(ctx, hipMemcpyAsync(ctx->fold_buf.p, fold_start.data(), fold_start.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, s));
+  // extra phase centres: the uv shift of every sub-integration
+  if (ctx->n_pc > 1) {
+    uv_table(ctx);
+    HIPCHK(ctx, ctx->uv_tab.ensure(std::max<size_t>(ctx->h_uv.size(), 1) * sizeof(UvShift)));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->uv_tab.p, ctx->h_uv.data(), ctx->h_uv.size() * sizeof(UvShift),
+                               hipMemcpyHostToDevice, s));
+  }
   // phased array: blocks and rows of every chunk (correlation_core_phased.cc:116-126)
   lp.ph_b0.clear();
   lp.ph_nb.clear();
@@ -983,17 +1128,22 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
                                     ctx->partial.as<float2>(), ctx->vis.as<float2>(), t, s));
     }
   } else if (!ctx->pulsar) {
+    // with extra phase centres the partials are folded into every centre below
+    float2 *fold_vis = ctx->n_pc > 1 ? nullptr : ctx->vis.as<float2>();
     for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
       const int ng = std::min(lp.k_group, ch.fold1 - g0);
       if (t.xs_waves == 0)
         HIPCHK(ctx, launch_xstep_tile(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
                                       ctx->fold_buf.as<int32_t>() + g0, ng, ctx->tiles.as<int32_t>(),
-                                      (int)(ctx->h_tiles.size() / 18), B, ctx->partial.as<float2>(),
-                                      ctx->vis.as<float2>(), s));
+                                      (int)(ctx->h_tiles.size() / 18), B, ctx->partial.as<float2>(), fold_vis, s));
       else
         HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
                                  ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),
-                                 ctx->vis.as<float2>(), t, s));
+                                 fold_vis, t, s));
+      if (ctx->n_pc > 1)
+        HIPCHK(ctx, launch_fold_combine_pc(ctx->partial.as<float2>(), ng, B, nc + 1, M,
+                                           ctx->uv_tab.as<UvShift>() + (size_t)g0 * (ctx->n_pc - 1) * B, ctx->n_pc,
+                                           ctx->vis.as<float2>(), s));
     }
   } else if (lp.mixed) {
     // with several chunks the channel run covers all channels (checked in prepare)
@@ -1170,14 +1320,14 @@ int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) 
     o += n;
   };
   for (int bin = 0; bin < ctx->nbins; bin++) {
-    int32_t sourcenr = ctx->pulsar ? bin : (ctx->p.pulsar_binning ? 1 : 0);
+    int32_t sourcenr = ctx->pulsar ? bin : ctx->n_pc > 1 ? ctx->pc_source[bin] : (ctx->p.pulsar_binning ? 1 : 0);
     put(&sourcenr, 4);
     const int32_t ts_hdr[4] = {ctx->p.integration_nr, B, (int32_t)stations.size(), M};
     put(ts_hdr, 16);
     for (size_t k = 0; k < stations.size(); k++) {
       const int32_t h[2] = {stations[k], 0};
       put(h, 8);
-      put(&ctx->uvw[3 * first_stream[k]], 24);
+      put(ctx->n_pc > 1 ? &ctx->pc_uvw[((size_t)bin * M + first_stream[k]) * 3] : &ctx->uvw[3 * first_stream[k]], 24);
     }
     for (int i = 0; i < M; i++) {
       const sfxg_station_parameters &st = ctx->st[i];

@@ -204,7 +204,7 @@ hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, cons
                              const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
                              const Tune &t, hipStream_t s);
 // Sub-integration partial sums (fold_start[0..n_k]: segment ranges) and
-// their in-order fold into vis.
+// their in-order fold into vis (vis == nullptr: the partials only).
 // spectra rows hold segments seg_base ..; fold_start holds slice segment indices.
 hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                         int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *bl_first,
@@ -285,6 +285,17 @@ void phased_tables(int64_t seg0, int64_t seg1, int n_sub, std::vector<PhasedBloc
 hipError_t launch_phased(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                          int n_ch, int M, const PhasedBlock *blocks, int n_blocks, const PhasedRow *rows, int n_rows,
                          float2 *partial, float2 *out, hipStream_t s);
+// Extra phase centres (sfxg_pc.hip, correlation_core.cc:515-592): the uv
+// shift of one sub-integration, centre and baseline.
+struct UvShift {
+  double phi, delta;  // rotation exp(i (phi + ch * delta))
+  float amp;          // 1 / weights[lag] or 1
+  int32_t pad;
+};
+// vis[0] += partial[k] and vis[j] += shifted partial[k] for k = 0 .. n_k - 1
+// in fold order; uv holds [k][j - 1][baseline]; vis is [centre][baseline][ch].
+hipError_t launch_fold_combine_pc(const float2 *partial, int n_k, int n_baselines, int n_ch, int M,
+                                  const UvShift *uv, int n_pc, float2 *vis, hipStream_t s);
 // create_window of Correlation_core (correlation_core.cc:668-710,761-809):
 // the 2 nchan-point lag window for the delay-correction window `window`
 // applied over 2 fft_size points. Returns false for an unknown window.

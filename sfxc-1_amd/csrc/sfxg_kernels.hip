@@ -2681,6 +2681,7 @@ hipError_t launch_xstep(const float2 *spectra, int64_t stream_stride, int64_t se
                      M, fold_start, bl_first, bl_second, n_baselines, sps, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (!vis) return hipSuccess;  // the caller folds the partials itself
   const int64_t n = (int64_t)n_baselines * n_ch;
   hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
   return hipGetLastError();
@@ -2730,6 +2731,7 @@ hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64
                      n_ch, M, fold_start, tiles, n_tiles, n_baselines, sps, partial);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (!vis) return hipSuccess;  // the caller folds the partials itself
   const int64_t n = (int64_t)n_baselines * n_ch;
   hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
   return hipGetLastError();

@@ -112,6 +112,9 @@ def lib() -> C.CDLL:
         L.sfxg_tune_import.argtypes = [C.c_char_p]
         L.sfxg_tune_clear.restype = None
         L.sfxg_tune_clear.argtypes = []
+        L.sfxg_set_phase_centers.restype = C.c_int
+        L.sfxg_set_phase_centers.argtypes = [C.c_void_p, C.c_int32, C.POINTER(DelayModelC), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int32)]
         L.sfxg_set_mask.restype = C.c_int
         L.sfxg_set_mask.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_int32,
                                     C.POINTER(C.c_double), C.c_int32]
@@ -173,7 +176,7 @@ class Context:
             job.number_channels, job.fft_size_delaycor, job.fft_size_correlation, job.integration_nr,
             job.slice_nr, job.slice_offset, job.sample_rate, job.bandwidth, job.channel_freq,
             job.sideband.encode(), job.polarisation.encode(), 0, job.frequency_nr, int(job.cross_polarize),
-            job.reference_station, job.window, job.n_phase_centers, int(job.pulsar_binning or pulsar is not None),
+            job.reference_station, job.window, job.phase_center_count(), int(job.pulsar_binning or pulsar is not None),
             M, sts)
         models = (DelayModelC * M)()
 
@@ -199,10 +202,37 @@ class Context:
             keep.append(ps)
         self._check(lib().sfxg_set_parameters(self._h, C.byref(p), models, uvw,
                                                C.byref(ps) if ps is not None else None), "sfxg_set_parameters")
+        if job.phase_centers:
+            self.set_phase_centers(job)
         self.set_mask(job.mask)
         self._keep = keep
         self.job = job
-        self.nbins = pulsar.nbins + 1 if pulsar is not None else 1
+        self.nbins = pulsar.nbins + 1 if pulsar is not None else job.phase_center_count()
+
+    def set_phase_centers(self, job: CorrelationParameters):
+        """The job's phase centres after the pointing centre (sfxg_set_phase_centers)."""
+        keep = []
+        M = len(job.station_streams)
+        n = 1 + len(job.phase_centers)
+        models = (DelayModelC * ((n - 1) * M))()
+        for j, pc in enumerate(job.phase_centers):
+            assert len(pc.delay_models) == M
+            for i, m in enumerate(pc.delay_models):
+                c = np.ascontiguousarray(m.delay, dtype=np.float64).ravel()
+                keep.append(c)
+                models[j * M + i] = DelayModelC(m.scan_begin_ticks, m.interval, m.delay.shape[0], 0,
+                                                c.ctypes.data_as(C.POINTER(C.c_double)),
+                                                C.POINTER(C.c_double)(), C.POINTER(C.c_double)())
+        uvw = np.zeros((n, M, 3))
+        if job.uvw is not None:
+            uvw[0] = np.asarray(job.uvw, dtype=np.float64).reshape(M, 3)
+        for j, pc in enumerate(job.phase_centers):
+            if pc.uvw is not None:
+                uvw[j + 1] = np.asarray(pc.uvw, dtype=np.float64).reshape(M, 3)
+        src = np.array([job.source_nr] + [pc.source_nr for pc in job.phase_centers], dtype=np.int32)
+        self._check(lib().sfxg_set_phase_centers(self._h, n, models, uvw.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  src.ctypes.data_as(C.POINTER(C.c_int32))),
+                    "sfxg_set_phase_centers")
 
     def set_mask(self, mask=None):
         """Mask_parameters of the job (sfxg.job.MaskParameters or None for the defaults)."""

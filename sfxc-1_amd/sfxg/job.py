@@ -121,6 +121,18 @@ class Pulsar:
 
 
 @dataclasses.dataclass
+class PhaseCenter:
+    """A phase centre besides the pointing centre: every stream's delay
+    towards it (the further splines of Delay_table_akima,
+    delay_table_akima.cc:114-136), its uvw per stream and the source number its
+    records carry (correlation_core.cc:44-66)."""
+
+    delay_models: List["DelayModel"]
+    uvw: Optional[np.ndarray] = None  # (n_streams, 3) metres
+    source_nr: int = 0
+
+
+@dataclasses.dataclass
 class MaskParameters:
     """Mask_parameters (control_parameters.h:107-114): used by integration_write
     when fft_size_correlation > number_channels (correlation_core.cc:417-440).
@@ -166,6 +178,13 @@ class CorrelationParameters:
     delay_models: List[DelayModel] = dataclasses.field(default_factory=list)
     uvw: Optional[np.ndarray] = None  # (n_streams, 3) per stream, metres
     mask: Optional[MaskParameters] = None  # Correlation_parameters::mask_parameters
+    # phase centres 1 .. n - 1 (multi-phase-centre jobs); source_nr is the
+    # pointing centre's source number when there are any
+    phase_centers: List[PhaseCenter] = dataclasses.field(default_factory=list)
+    source_nr: int = 0
+
+    def phase_center_count(self) -> int:
+        return 1 + len(self.phase_centers) if self.phase_centers else self.n_phase_centers
 
     # --- derived quantities (each cites the reference expression) ---------
     def n_ffts_per_integration(self, i: int = 0) -> int:

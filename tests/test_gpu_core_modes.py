@@ -1,15 +1,16 @@
-"""The two output forms beside the normal correlator, on the GPU against the
+"""The forms of Correlation_core beside the plain one, on the GPU against the
 oracle through the C ABI: lag re-channelisation (integration_write with
 fft_size_correlation > number_channels, correlation_core.cc:417-440, and the
-Mask_parameters of control_parameters.h:107-114) and the phased-array core
-(Correlation_core_phased, correlation_core_phased.cc:13-126)."""
+Mask_parameters of control_parameters.h:107-114), the phased-array core
+(Correlation_core_phased, correlation_core_phased.cc:13-126) and extra phase
+centres (sub_integration / uvshift, correlation_core.cc:515-592)."""
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 from sfxg import capi, configs
-from sfxg.job import (LUT_CONFIG, WINDOW_COS, WINDOW_HAMMING, WINDOW_HANN, WINDOW_NONE, WINDOW_RECT,
-                      MaskParameters, Pulsar, ticks_to_mjd)
+from sfxg.job import (LUT_CONFIG, WINDOW_COS, WINDOW_HAMMING, WINDOW_HANN, WINDOW_NONE, WINDOW_RECT, DelayModel,
+                      MaskParameters, PhaseCenter, Pulsar, ticks_to_mjd)
 
 from test_gpu_parity import compare_vis, vis_from_bytes
 
@@ -204,3 +205,79 @@ def test_phased_rejects(ctx):
     ctx.set_options(lut=LUT_CONFIG, phased_array=False)
     with pytest.raises(capi.SfxgError):
         ctx.correlate()
+
+
+def pc_job(nc, sub_usec, n_extra, window, seed=51, second_source=False):
+    fs = 32_000_000
+    offsets, rates = [0.0, 2.5e-6, -1.75e-6], [0.0, 2e-7, -1e-7]
+    shifts = [[0.0, 8 / fs, -6 / fs], [0.0, -3.1e-8, 4.4e-8], [1e-8, 0.0, -2.5e-7]][:n_extra]
+    extra = (0.1, [o + d for o, d in zip(offsets, shifts[0])], rates) if second_source else None
+    job, hdrs = configs.build_slices(seed=seed, n_stations=3, pols=["R"], fs=fs, bw=fs // 2, nchan=nc,
+                                     seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2, sky_freq=5_000_000_000,
+                                     offsets=offsets, rates=rates, window=window, sub_int_usec=sub_usec,
+                                     extra=extra)[0]
+    rng = np.random.default_rng(seed)
+    pcs = []
+    for k, dl in enumerate(shifts):
+        ms = []
+        for i, m in enumerate(job.delay_models):
+            c = m.delay.copy()
+            c[:, 0] += dl[i]
+            c[:, 1] += 1e-9 * k  # a small rate difference as well
+            ms.append(DelayModel(m.scan_begin_ticks, m.interval, c))
+        pcs.append(PhaseCenter(ms, uvw=rng.standard_normal((3, 3)) * 1e6, source_nr=10 + k))
+    job.phase_centers = pcs
+    job.source_nr = 3
+    return job, hdrs
+
+
+@pytest.mark.parametrize("nc,sub_usec,n_extra,window,second,nchan", [
+    (1024, 10_000, 2, WINDOW_HANN, False, None),   # 6 sub-integrations, two shifted centres
+    (512, 15_625, 1, WINDOW_NONE, True, None),     # a second source in the shifted centre
+    (4096, 62_500, 3, WINDOW_COS, False, 1024),    # one sub-integration, re-channelised to 1024
+])
+def test_phase_centers(ctx, nc, sub_usec, n_extra, window, second, nchan):
+    print(f"case phase centres nc={nc} sub={sub_usec} extra={n_extra} window={window} nchan={nchan}", flush=True)
+    job, hdrs = pc_job(nc, sub_usec, n_extra, window, second_source=second)
+    if nchan:
+        job.number_channels = nchan
+    ref = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+    out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    n_pc = 1 + n_extra
+    heads_g, vis_g = vis_from_bytes(job, out, n_pc)
+    heads_r, vis_r = vis_from_bytes(job, ref["output"], n_pc)
+    assert heads_g == heads_r
+    for j in range(n_pc):
+        compare_vis(vis_g[j], vis_r[j], label=f"phase centre {j}")
+
+
+def test_phase_centers_chunked_matches_unchunked(ctx):
+    """Chunks hold whole sub-integrations, so every centre gets the same
+    folds in the same order: identical bytes."""
+    job, hdrs = pc_job(1024, 10_000, 2, WINDOW_HANN)
+    a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, chunk_mib=1)
+    assert a == b
+
+
+def test_phase_centers_reject(ctx):
+    job, hdrs = pc_job(1024, 10_000, 1, WINDOW_HANN)
+    # mixed precision
+    with pytest.raises(capi.SfxgError):
+        capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=1)
+    # pulsar binning with several centres
+    psr = Pulsar(nbins=4, gate_start=0.1, gate_stop=0.6, tmid=ticks_to_mjd(job.start_time) - 0.001, DM=68.0,
+                 ref_phase=0.3, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+    ctx.set_options(lut=LUT_CONFIG)
+    with pytest.raises(capi.SfxgError):
+        ctx.set_parameters(job, psr)
+    # centres announced but not given
+    pcs = job.phase_centers
+    job.phase_centers = []
+    job.n_phase_centers = 2
+    ctx.set_parameters(job)
+    for i, h in enumerate(hdrs):
+        ctx.push_stream(i, h)
+    with pytest.raises(capi.SfxgError):
+        ctx.correlate()
+    job.phase_centers = pcs
