@@ -81,6 +81,14 @@ class Correlation_core {
                "sfxg_set_parameters");
     finished_ = false;
   }
+  // The further phase centres of a multi-phase-centre slice: models
+  // (n - 1) x n_streams centre-major, uvw n x n_streams x 3, source_nr n.
+  void set_phase_centers(int n, const std::vector<sfxg_delay_model> &models, const std::vector<double> &uvw,
+                         const std::vector<int32_t> &source_nr) {
+    ctx_.check(sfxg_set_phase_centers(ctx_.get(), n, models.data(), uvw.empty() ? nullptr : uvw.data(),
+                                      source_nr.empty() ? nullptr : source_nr.data()),
+               "sfxg_set_phase_centers");
+  }
   // Correlator_node_controller's MPI_TAG_MASK_PARAMETERS handler
   // (sfxc/src/correlator_node_controller.cc:72-77).
   void set_mask_parameters(const Mask_parameters &m) {
