@@ -190,6 +190,33 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
                         const sfxg_delay_model *models, const double *uvw,
                         const sfxg_pulsar *pulsar);
 
+/* One station's raw VDIF stream for sfxg_push_vdif (multi-thread VDIF: one
+ * channel per thread, Input_node_parameters with n_tracks == 0,
+ * src/vdif_reader.cc:202-222). */
+typedef struct {
+  int32_t frame_bytes;        /* frame length with its header (dataframe_length * 8) */
+  int32_t channel;            /* the stream's channel: index into thread_ids */
+  int32_t n_threads;          /* 0: the stream holds one thread, whose id comes from the data */
+  int32_t ref_mjd;            /* the reader's reference day (VDIF_reader ref_jday) */
+  const int32_t *thread_ids;  /* thread id of every channel of the station */
+  int64_t offset;             /* Input_node_parameters::offset, ticks subtracted from frame times */
+} sfxg_vdif_stream;
+
+/* Takes stream `stream_idx` of the slice as raw VDIF frames that already
+ * live in device memory (device_frames, n_bytes: whole frames covering the
+ * slice). Replaces, for that stream, the input node's VDIF_reader
+ * (src/vdif_reader.cc:77-190: fill-pattern frames, the invalid flag, thread
+ * to channel, frame times), Channel_extractor_tasklet_VDIF
+ * (src/channel_extractor_tasklet_vdif.cc:25-49) and Input_node_data_writer
+ * (src/input_node_data_writer.cc:105-228: the integer delay in bytes, DELAY
+ * and INVALID records, one frame per block), and then the HEADER stream
+ * transfer: a kernel lifts the frames' header words, the host builds the
+ * stream's records from them alone, and a kernel gathers the DATA payload
+ * from the frames. No sample byte crosses PCIe. The frames must stay valid
+ * until this call returns; the gathered payload belongs to the context. */
+int sfxg_push_vdif(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                   const sfxg_vdif_stream *vdif);
+
 /* The phase centres after the pointing centre of a multi-phase-centre slice
  * (params->n_phase_centers > 1): replaces the further sources of each
  * stream's Delay_table_akima (src/delay_table_akima.cc:114-136) and their

@@ -477,6 +477,162 @@ def rechannelize(job, norm: np.ndarray, precision="f32") -> np.ndarray:
 
 
 # ----------------------------------------------------------------------------
+# Raw VDIF: the input node's reader and writer (test-side restatement)
+# ----------------------------------------------------------------------------
+def _mjd(day, month, year):
+    """utils.cc:255-263."""
+    a = (14 - month) // 12
+    y = year + 4800 - a
+    m = month + 12 * a - 3
+    jdn = day + (153 * m + 2) // 5 + 365 * y + y // 4 - y // 100 + y // 400 - 32045
+    return int(jdn - 2400000.5)
+
+
+def vdif_blocks(vdif: bytes, frame_bytes: int, thread_ids, channel: int, sample_rate: int, ref_mjd: int,
+                offset: int = 0):
+    """VDIF_reader::read_new_block with one frame per block
+    (vdif_reader.cc:77-190, set_parameters :202-222) and
+    Channel_extractor_tasklet_VDIF's routing: channel `channel`'s blocks as
+    (payload bytes, start ticks, invalid)."""
+    n = len(vdif) // frame_bytes
+    tmap = {int(t): c for c, t in enumerate(thread_ids)}
+    first = None
+    out = []
+    for f in range(n):
+        fr = vdif[f * frame_bytes:(f + 1) * frame_bytes]
+        w = np.frombuffer(fr[:16], dtype="<u4")
+        if np.any(w == 0x11223344):
+            continue
+        if first is None:
+            first = w.copy()
+        legacy = (int(first[0]) >> 30) & 1
+        hsize = 16 + 16 * (1 - legacy)
+        fsize = frame_bytes - hsize
+        invalid = (int(w[0]) >> 31) & 1
+        thread = (int(w[3]) >> 16) & 0x3FF
+        if invalid:
+            ch = tmap.get(thread, 0)
+        else:
+            if thread not in tmap:
+                if tmap:
+                    continue
+                tmap[thread] = 0
+            ch = tmap[thread]
+        if ch != channel:
+            continue
+        sec = int(w[0]) & 0x3FFFFFFF
+        epoch = (int(w[1]) >> 24) & 0x3F
+        jday_epoch = _mjd(1, 1 + 6 * (epoch % 2), 2000 + epoch // 2)
+        seconds = float(sec) - (ref_mjd - jday_epoch) * 24 * 60 * 60
+        bits = ((int(first[3]) >> 26) & 0x1F) + 1
+        nch = 1 << ((int(first[2]) >> 24) & 0x1F)
+        spf = 8 * fsize // (bits * nch)
+        subsec = (int(w[1]) & 0xFFFFFF) * spf / sample_rate
+        t = int((ref_mjd - 51544) * 86400 * float(CLOCK_RATE)) + int(round((seconds + subsec) * CLOCK_RATE)) - offset
+        out.append((fr[hsize:], t, bool(invalid)))
+    return out
+
+
+def vdif_header_stream(vdif: bytes, frame_bytes: int, thread_ids, channel: int, job, stream: int, ref_mjd: int,
+                       offset: int = 0) -> bytes:
+    """Input_node_data_writer::do_task (input_node_data_writer.cc:105-228) over
+    one slice of VDIF frame blocks: the HEADER stream channel `channel` of the
+    station sends for stream `stream` of the job."""
+    from sfxg import synth
+    st = job.station_streams[stream]
+    fs, bits = st.sample_rate, st.bits_per_sample
+    spb = 8 // bits
+    tps = int(round(CLOCK_RATE / fs))
+    byte_length = int(round(8 * 1_000_000.0 / (fs * bits) * (CLOCK_RATE / 1_000_000)))
+    nsamples = job.n_ffts_per_integration(stream) * job.fft_size_delaycor
+    changes = synth.delay_changes(job.delay_models[stream], job.start_time, nsamples, fs)
+    dl = []
+    for t, nlev in changes:
+        b, r = synth.delay_split(np.array([nlev]), bits)
+        dl.append((t, int(b[0]), int(r[0])))
+    blocks = vdif_blocks(vdif, frame_bytes, thread_ids, channel, fs, ref_mjd, offset)
+    out = bytearray()
+
+    def counted(kind, n):
+        while n > 0:
+            k = min(n, 32767)
+            out.append(kind)
+            out.extend(int(k).to_bytes(2, "little"))
+            n -= k
+
+    current = job.start_time
+    di = 0
+    while di < len(dl) - 1 and dl[di + 1][0] <= current + byte_length:
+        di += 1
+    slice_size = nsamples
+    sync = True
+    for payload, bstart, invalid in blocks:
+        block_size = len(payload)
+        byte_offset = 0
+        if sync:
+            diff = current - bstart
+            dsamples = abs(diff) // tps * (1 if diff >= 0 else -1)  # C++ division truncates toward zero
+            byte_offset = int(dsamples * bits / 8) if dsamples < 0 else dsamples * bits // 8
+            byte_offset += dl[di][1]
+            if byte_offset < 0:
+                raise ValueError("slice starts before the VDIF data")
+            if byte_offset >= block_size:
+                continue
+            slice_size += dl[di][2]
+            out.extend([2, dl[di][2] & 0xFF])
+            current -= dl[di][2] * tps
+            sync = False
+
+        def ndp_of():
+            if di < len(dl) - 1:
+                return int((dl[di + 1][0] - current) / byte_length)
+            return (2**31 - 1) // 2
+
+        index = byte_offset
+        inv = [(0, block_size)] if invalid else []
+        inv_i = 0
+        nip = inv[0][0] if inv else block_size + 1
+        ndp = ndp_of() + byte_offset
+        total = min(block_size - byte_offset, (slice_size + spb - 1) // spb)
+        end_index = total + byte_offset
+        while index < end_index:
+            if index >= ndp:
+                di += 1
+                out.extend([2, dl[di][2] & 0xFF])
+                d = dl[di][2] - dl[di - 1][2]
+                slice_size += -1 if (d > 1 or d == -1) else 1
+                ndp = ndp_of() + byte_offset
+                total = min(block_size - byte_offset, (slice_size + spb - 1) // spb)
+                end_index = total + byte_offset
+            elif index >= nip:
+                nrun = inv[inv_i][1]
+                end_pos = min(nip + nrun, ndp)
+                nr = min(max(0, end_pos - index), end_index - index)
+                if nr > 0:
+                    counted(1, nr * spb)
+                    index += nr
+                if end_pos == nip + nrun:
+                    inv_i += 1
+                    nip = inv[inv_i][0] if inv_i < len(inv) else block_size + 1
+            else:
+                k = min(ndp - index, end_index - index, nip - index)
+                done = 0
+                while done < k:
+                    c = min(k - done, 32767)
+                    out.append(0)
+                    out.extend(int(c).to_bytes(2, "little"))
+                    out.extend(payload[index + done:index + done + c])
+                    done += c
+                index += k
+        slice_size -= total * spb
+        current += total * spb * tps
+        if slice_size <= 0:
+            out.append(3)
+            return bytes(out)
+    raise ValueError("the VDIF data ends before the slice")
+
+
+# ----------------------------------------------------------------------------
 def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32",
               keep_intermediate: bool = False, pulsar=None, uvw=None, phased: bool = False):
     """Full correlator-node path for one slice; returns dict with the output

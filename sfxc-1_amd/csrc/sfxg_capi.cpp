@@ -157,6 +157,9 @@ struct sfxg_ctx {
   int pcw_window = -1, pcw_nc = -1;  // create_weights cache (window, N_c)
   std::vector<float> pcw_win;        // the window f(2 N_c - 1, i) as stored (float)
   std::vector<double> pcw_r;         // autocorrelation by lag, NaN until computed
+  // raw VDIF ingest: the gathered payload of each stream, header words, pieces
+  std::vector<DBuf> vdif_payload;
+  DBuf vdif_words, vdif_pieces, vdif_dst;
   // phased array (Correlation_core_phased) as of sfxg_set_parameters
   bool phased = false;
   DBuf ph_blocks, ph_rows;
@@ -273,6 +276,10 @@ void sfxg_destroy(sfxg_ctx *ctx) {
                   &ctx->tiles, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out,
                   &ctx->ph_blocks, &ctx->ph_rows, &ctx->uv_tab};
   for (DBuf *b : bufs) b->release();
+  for (DBuf &b : ctx->vdif_payload) b.release();
+  ctx->vdif_words.release();
+  ctx->vdif_pieces.release();
+  ctx->vdif_dst.release();
   sync_slice(ctx);
   if (ctx->done) (void)hipEventDestroy(ctx->done);
   if (ctx->prep_done) (void)hipEventDestroy(ctx->prep_done);
@@ -523,6 +530,90 @@ int sfxg_push_stream(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *bytes, si
 int sfxg_push_stream_elided(sfxg_ctx *ctx, int32_t stream_idx, const uint8_t *control, size_t n,
                             const void *device_payload, size_t payload_bytes) {
   return push_common(ctx, stream_idx, control, n, true, device_payload, payload_bytes);
+}
+
+// Input_node_tasklet::get_delays (input_node_tasklet.cc:235-262) for a
+// monotonic delay model: the slice start's integer delay and the first
+// sample instant of every later integer level, each found by bisection
+// over the slice's samples.
+static std::vector<std::pair<int64_t, int64_t>> input_node_delays(const ModelCopy &m, int64_t start,
+                                                                  int64_t nsamples, int fs) {
+  const int64_t tps = (int64_t)round(kClockRate / fs);
+  auto n_at = [&](int64_t t) { return (int64_t)floor(model_eval(m, t, false) * fs + 0.5); };
+  std::vector<std::pair<int64_t, int64_t>> out{{start, n_at(start)}};
+  const int64_t n0 = out[0].second, n1 = n_at(start + nsamples * tps);
+  if (n0 == n1) return out;
+  const int64_t step = n1 > n0 ? 1 : -1;
+  for (int64_t level = n0 + step; level != n1 + step; level += step) {
+    int64_t lo = 0, hi = nsamples;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) / 2;
+      if ((n_at(start + mid * tps) - level) * step >= 0) hi = mid;
+      else lo = mid;
+    }
+    out.push_back({start + hi * tps, level});
+  }
+  return out;
+}
+
+int sfxg_push_vdif(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                   const sfxg_vdif_stream *v) {
+  if (!ctx || !device_frames || !v) return SFXG_ERR_ARG;
+  if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
+  if (stream_idx < 0 || stream_idx >= ctx->M) return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
+  if (v->frame_bytes < 32 || v->frame_bytes % 8) return ctx->fail(SFXG_ERR_ARG, "frame_bytes %d", v->frame_bytes);
+  if (v->n_threads < 0 || (v->n_threads > 0 && !v->thread_ids) || v->channel < 0 ||
+      (v->n_threads > 0 && v->channel >= v->n_threads) || (v->n_threads == 0 && v->channel != 0))
+    return ctx->fail(SFXG_ERR_ARG, "channel %d of %d threads", v->channel, v->n_threads);
+  const int64_t n_frames = (int64_t)(n_bytes / (size_t)v->frame_bytes);
+  if (n_frames <= 0) return ctx->fail(SFXG_ERR_ARG, "no whole VDIF frame");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  sync_slice(ctx);
+  const sfxg_station_parameters &st = ctx->st[stream_idx];
+  // header words of every frame, to the host
+  std::vector<uint32_t> words((size_t)n_frames * 4);
+  HIPCHK(ctx, ctx->vdif_words.ensure(words.size() * sizeof(uint32_t)));
+  HIPCHK(ctx, launch_vdif_words(device_frames, n_frames, v->frame_bytes, ctx->vdif_words.as<uint32_t>(), ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(words.data(), ctx->vdif_words.p, words.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  // the channel's frame blocks and the slice's records
+  std::vector<int> threads(v->thread_ids, v->thread_ids + v->n_threads);
+  std::vector<VdifBlock> blocks;
+  int frame_payload = 0;
+  std::string err;
+  int rc = vdif_blocks(words.data(), n_frames, v->frame_bytes, st.sample_rate, v->ref_mjd, v->offset, threads,
+                       v->channel, blocks, frame_payload, err);
+  if (rc != SFXG_OK) return ctx->fail(rc, "stream %d: %s", stream_idx, err.c_str());
+  const int64_t nsamples = (int64_t)ctx->n_ffts_per_integration * ctx->nd;
+  const auto delays = input_node_delays(ctx->models[stream_idx], ctx->p.start_time, nsamples, st.sample_rate);
+  std::vector<uint8_t> control;
+  std::vector<VdifPiece> pieces;
+  rc = vdif_write_slice(blocks, frame_payload, st.bits_per_sample, st.sample_rate, ctx->p.start_time, nsamples,
+                        delays, control, pieces, err);
+  if (rc != SFXG_OK) return ctx->fail(rc, "stream %d: %s", stream_idx, err.c_str());
+  // gather the DATA payload on the device
+  std::vector<int64_t> dst_off(pieces.size());
+  int64_t total = 0;
+  for (size_t k = 0; k < pieces.size(); k++) {
+    if (pieces[k].src < 0 || pieces[k].src + pieces[k].n > (int64_t)n_bytes)
+      return ctx->fail(SFXG_ERR_STREAM, "stream %d: payload outside the frames", stream_idx);
+    dst_off[k] = total;
+    total += pieces[k].n;
+  }
+  if (ctx->vdif_payload.size() < (size_t)ctx->M) ctx->vdif_payload.resize(ctx->M);
+  DBuf &pay = ctx->vdif_payload[stream_idx];
+  HIPCHK(ctx, pay.ensure(std::max<int64_t>(total, 1)));
+  HIPCHK(ctx, ctx->vdif_pieces.ensure(std::max<size_t>(pieces.size(), 1) * sizeof(VdifPiece)));
+  HIPCHK(ctx, ctx->vdif_dst.ensure(std::max<size_t>(dst_off.size(), 1) * sizeof(int64_t)));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->vdif_pieces.p, pieces.data(), pieces.size() * sizeof(VdifPiece),
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->vdif_dst.p, dst_off.data(), dst_off.size() * sizeof(int64_t),
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, launch_vdif_gather(device_frames, ctx->vdif_pieces.as<VdifPiece>(), ctx->vdif_dst.as<int64_t>(),
+                                 (int)pieces.size(), pay.p, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return push_common(ctx, stream_idx, control.data(), control.size(), true, pay.p, (size_t)total);
 }
 
 static void make_tables(sfxg_ctx *ctx, std::vector<float2> &tnd, std::vector<float2> &tn,

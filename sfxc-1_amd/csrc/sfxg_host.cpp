@@ -346,4 +346,214 @@ bool lag_window(int window, int nchan, int fft_size, std::vector<float> &w) {
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// Raw VDIF ingest
+// ---------------------------------------------------------------------------
+
+namespace {
+// mjd (utils.cc:255-263): Julian day number minus 2400000.5, truncated to int.
+int mjd_of(int day, int month, int year) {
+  const int a = (14 - month) / 12;
+  const int y = year + 4800 - a;
+  const int m = month + 12 * a - 3;
+  const int jdn = day + ((153 * m + 2) / 5) + 365 * y + (y / 4) - (y / 100) + (y / 400) - 32045;
+  return (int)(jdn - 2400000.5);
+}
+constexpr uint32_t kVdifFill = 0x11223344u;
+constexpr int kShrtMax = 32767;
+}  // namespace
+
+int vdif_blocks(const uint32_t *words, int64_t n_frames, int frame_bytes, int sample_rate, int ref_mjd,
+                int64_t offset_ticks, const std::vector<int> &thread_ids, int channel,
+                std::vector<VdifBlock> &out, int &frame_payload, std::string &err) {
+  // Header fields (vdif_reader.h:27-40, little-endian bit fields):
+  // w0: seconds 0-29, legacy 30, invalid 31; w1: frame in second 0-23,
+  // reference epoch 24-29; w2: length in 8-byte units 0-23, log2 channels
+  // 24-28; w3: thread 16-25, bits per sample - 1 26-30.
+  std::vector<int> thread_map_key(thread_ids), thread_map_val;
+  for (size_t c = 0; c < thread_ids.size(); c++) thread_map_val.push_back((int)c);
+  auto lookup = [&](int thread) {
+    for (size_t k = 0; k < thread_map_key.size(); k++)
+      if (thread_map_key[k] == thread) return thread_map_val[k];
+    return -1;
+  };
+  bool first_seen = false;
+  int legacy = 0, bits_m1 = 0, log2_nchan = 0;
+  frame_payload = 0;
+  for (int64_t f = 0; f < n_frames; f++) {
+    const uint32_t *w = words + 4 * f;
+    if (w[0] == kVdifFill || w[1] == kVdifFill || w[2] == kVdifFill || w[3] == kVdifFill) continue;
+    if (!first_seen) {
+      first_seen = true;
+      legacy = (w[0] >> 30) & 1;
+      bits_m1 = (w[3] >> 26) & 0x1f;
+      log2_nchan = (w[2] >> 24) & 0x1f;
+      frame_payload = frame_bytes - (16 + 16 * (1 - legacy));
+      if ((int64_t)(w[2] & 0xffffff) * 8 != frame_bytes) {
+        err = "VDIF frame length in the header differs from frame_bytes";
+        return SFXG_ERR_STREAM;
+      }
+      if (frame_payload <= 0) {
+        err = "VDIF frame without payload";
+        return SFXG_ERR_STREAM;
+      }
+    }
+    const bool invalid = (w[0] >> 31) != 0;
+    const int thread = (int)((w[3] >> 16) & 0x3ff);
+    int ch = lookup(thread);
+    if (invalid) {
+      if (ch < 0) ch = 0;
+    } else if (ch < 0) {
+      // the only thread: its id comes from the data
+      if (!thread_map_key.empty()) continue;
+      thread_map_key.push_back(thread);
+      thread_map_val.push_back(0);
+      ch = 0;
+    }
+    if (ch != channel) continue;
+    // get_current_time: seconds since the reference day plus the frame's
+    // place in its second, then Time::set_time
+    const uint32_t sec = w[0] & 0x3fffffff;
+    const int ref_epoch = (int)((w[1] >> 24) & 0x3f);
+    const int jday_epoch = mjd_of(1, 1 + 6 * (ref_epoch % 2), 2000 + ref_epoch / 2);
+    const double seconds_since_reference = (double)sec - (double)((ref_mjd - jday_epoch) * 24 * 60 * 60);
+    double subsec = 0;
+    if (sample_rate > 0) {
+      const int samples_per_frame = 8 * frame_payload / ((bits_m1 + 1) * (1 << log2_nchan));
+      subsec = (double)(w[1] & 0xffffff) * samples_per_frame / sample_rate;
+    }
+    const int64_t t = (int64_t)((ref_mjd - 51544) * 86400 * kClockRate) +
+                      (int64_t)round((seconds_since_reference + subsec) * kClockRate) - offset_ticks;
+    out.push_back({f * (int64_t)frame_bytes + (16 + 16 * (1 - legacy)), t, invalid});
+  }
+  if (!first_seen) {
+    err = "no VDIF frames";
+    return SFXG_ERR_STREAM;
+  }
+  return SFXG_OK;
+}
+
+int vdif_write_slice(const std::vector<VdifBlock> &blocks, int frame_payload, int bits, int sample_rate,
+                     int64_t slice_start, int64_t nsamples, const std::vector<std::pair<int64_t, int64_t>> &delays,
+                     std::vector<uint8_t> &control, std::vector<VdifPiece> &pieces, std::string &err) {
+  const int spb = 8 / bits;
+  const int64_t tps = (int64_t)round(kClockRate / sample_rate);  // Time::sample_rate, ticks per sample
+  const int64_t byte_length = (int64_t)round(8 * 1000000. / ((double)sample_rate * bits) * (kClockRate / 1000000));
+  // Input_node_tasklet::get_delay (input_node_tasklet.cc:45-63): bytes and remaining samples
+  struct Delay {
+    int64_t time, bytes;
+    int remaining;
+  };
+  std::vector<Delay> dl;
+  for (const auto &d : delays) {
+    const int64_t n = d.second;
+    int64_t nb = (int64_t)floor((double)(n - 1) / spb);
+    int64_t rem = n - nb * spb;
+    if (rem * bits == 8) {
+      nb++;
+      rem = 0;
+    }
+    dl.push_back({d.first, nb, (int)rem});
+  }
+  if (dl.empty()) {
+    err = "no delays";
+    return SFXG_ERR_ARG;
+  }
+  auto w_delay = [&](int r) {
+    control.push_back(kHeaderDelay);
+    control.push_back((uint8_t)(int8_t)r);
+  };
+  auto w_count = [&](uint8_t type, int64_t n) {
+    int64_t done = 0;
+    while (done < n) {
+      const int k = (int)std::min<int64_t>(n - done, kShrtMax);
+      control.push_back(type);
+      control.push_back((uint8_t)(k & 0xff));
+      control.push_back((uint8_t)(k >> 8));
+      done += k;
+    }
+  };
+  int64_t current = slice_start;
+  size_t di = 0;
+  while (di + 1 < dl.size() && dl[di + 1].time <= current + byte_length) di++;
+  int64_t slice_size = nsamples;
+  bool sync = true;
+  const int64_t block_size = frame_payload;
+  auto next_delay_pos = [&](int64_t start_time) -> int64_t {
+    if (di + 1 < dl.size()) return (int64_t)((double)(dl[di + 1].time - start_time) / (double)byte_length);
+    return INT_MAX / 2;
+  };
+  for (size_t b = 0; b < blocks.size(); b++) {
+    const VdifBlock &blk = blocks[b];
+    int64_t byte_offset = 0;
+    if (sync) {
+      const int64_t dsamples = (current - blk.time) / tps;  // Time::diff_samples
+      byte_offset = dsamples * bits / 8 + dl[di].bytes;
+      if (byte_offset < 0) {
+        err = "the slice starts before the VDIF data";
+        return SFXG_ERR_STREAM;
+      }
+      if (byte_offset >= block_size) continue;  // data before the slice
+      slice_size += dl[di].remaining;
+      w_delay(dl[di].remaining);
+      current -= (int64_t)dl[di].remaining * tps;
+      sync = false;
+    }
+    int64_t index = byte_offset;
+    int64_t next_invalid_pos = blk.invalid ? 0 : block_size + 1;
+    const int64_t n_invalid = blk.invalid ? block_size : 0;
+    bool invalid_done = !blk.invalid;
+    int64_t ndp = next_delay_pos(current) + byte_offset;
+    int64_t total = std::min(block_size - byte_offset, (slice_size + spb - 1) / spb);
+    int64_t end_index = total + byte_offset;
+    while (index < end_index) {
+      if (index >= ndp) {
+        di++;
+        w_delay(dl[di].remaining);
+        const int d_delay = dl[di].remaining - dl[di - 1].remaining;
+        if (d_delay > 1 || d_delay == -1) slice_size -= 1;
+        else slice_size += 1;
+        ndp = next_delay_pos(current) + byte_offset;
+        total = std::min(block_size - byte_offset, (slice_size + spb - 1) / spb);
+        end_index = total + byte_offset;
+      } else if (index >= next_invalid_pos) {
+        const int64_t end_pos = std::min(next_invalid_pos + n_invalid, ndp);
+        int64_t nr = std::max<int64_t>(0, end_pos - index);
+        nr = std::min(nr, end_index - index);
+        if (nr > 0) {
+          w_count(kHeaderInvalid, nr * spb);
+          index += nr;
+        }
+        if (end_pos == next_invalid_pos + n_invalid && !invalid_done) {
+          invalid_done = true;
+          next_invalid_pos = block_size + 1;
+        }
+      } else {
+        int64_t k = std::min(ndp - index, end_index - index);
+        k = std::min(next_invalid_pos - index, k);
+        if (k > 0) {
+          int64_t done = 0;
+          while (done < k) {  // write_data: records of at most SHRT_MAX bytes
+            const int n = (int)std::min<int64_t>(k - done, kShrtMax);
+            control.push_back(kHeaderData);
+            control.push_back((uint8_t)(n & 0xff));
+            control.push_back((uint8_t)(n >> 8));
+            pieces.push_back({blk.payload_offset + index + done, n});
+            done += n;
+          }
+        }
+        index += k;
+      }
+    }
+    slice_size -= total * spb;
+    current += total * spb * tps;
+    if (slice_size <= 0) {
+      control.push_back(kHeaderEndstream);
+      return SFXG_OK;
+    }
+  }
+  err = "the VDIF data ends before the slice";
+  return SFXG_ERR_STREAM;
+}
+
 }  // namespace sfxg

@@ -64,6 +64,43 @@ struct StreamPlan {
 int parse_header_stream(const uint8_t *bytes, size_t n, bool elided, const B2fParams &p,
                         StreamPlan &out, std::string &err);
 
+// Raw VDIF ingest (multi-thread VDIF, one channel per thread).
+// One frame block as VDIF_reader::read_new_block delivers it to one channel
+// (vdif_reader.cc:77-190): the payload's byte offset in the frame buffer, its
+// start time in ticks and whether the frame is flagged invalid.
+struct VdifBlock {
+  int64_t payload_offset;
+  int64_t time;
+  bool invalid;
+};
+// A piece of DATA payload: bytes [src, src + n) of the frame buffer.
+struct VdifPiece {
+  int64_t src;
+  int64_t n;
+};
+// VDIF_reader over the frames' first four header words (4 per frame):
+// fill-pattern frames skipped, thread ids mapped to channels (thread_ids[c]
+// is channel c's thread; none given: the first valid frame's thread is
+// channel 0), times from the header (get_current_time, vdif_reader.cc:59-73).
+// Appends channel `channel`'s blocks. Returns 0 or a negative SFXG_ERR_*.
+int vdif_blocks(const uint32_t *words, int64_t n_frames, int frame_bytes, int sample_rate, int ref_mjd,
+                int64_t offset_ticks, const std::vector<int> &thread_ids, int channel,
+                std::vector<VdifBlock> &out, int &frame_payload, std::string &err);
+// Input_node_data_writer::do_task over one slice (input_node_data_writer.cc:105-228)
+// on frame blocks: the HEADER records with the DATA payload left out
+// (control) and the payload pieces in order. delays: (tick, integer delay
+// in samples) from the slice start, as Input_node_tasklet::get_delays finds them.
+int vdif_write_slice(const std::vector<VdifBlock> &blocks, int frame_payload, int bits, int sample_rate,
+                     int64_t slice_start, int64_t nsamples, const std::vector<std::pair<int64_t, int64_t>> &delays,
+                     std::vector<uint8_t> &control, std::vector<VdifPiece> &pieces, std::string &err);
+
+// Device side of the VDIF ingest (sfxg_vdif.hip): header words of n_frames
+// frames at a stride of frame_bytes; the pieces' bytes to dst + dst_off[i].
+hipError_t launch_vdif_words(const void *frames, int64_t n_frames, int frame_bytes, uint32_t *words,
+                             hipStream_t s);
+hipError_t launch_vdif_gather(const void *frames, const VdifPiece *pieces, const int64_t *dst_off, int n_pieces,
+                              void *dst, hipStream_t s);
+
 // Number of correlation segments Delay_correction pushes per bit2float
 // chunk (delay_correction.cc:24-36,58-69).
 std::vector<int32_t> segments_per_chunk(const std::vector<int32_t> &chunks, int fft_size_delaycor,

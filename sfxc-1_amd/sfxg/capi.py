@@ -53,6 +53,11 @@ class PulsarC(C.Structure):
                 ("ref_freq", C.c_double), ("obs_freq", C.c_double), ("coef", C.POINTER(C.c_double))]
 
 
+class VdifStreamC(C.Structure):
+    _fields_ = [("frame_bytes", C.c_int32), ("channel", C.c_int32), ("n_threads", C.c_int32), ("ref_mjd", C.c_int32),
+                ("thread_ids", C.POINTER(C.c_int32)), ("offset", C.c_int64)]
+
+
 class Options(C.Structure):
     _fields_ = [("lut", C.c_int32), ("precision", C.c_int32), ("keep_time_series", C.c_int32),
                 ("chunk_mib", C.c_int32), ("phased_array", C.c_int32)]
@@ -112,6 +117,8 @@ def lib() -> C.CDLL:
         L.sfxg_tune_import.argtypes = [C.c_char_p]
         L.sfxg_tune_clear.restype = None
         L.sfxg_tune_clear.argtypes = []
+        L.sfxg_push_vdif.restype = C.c_int
+        L.sfxg_push_vdif.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t, C.POINTER(VdifStreamC)]
         L.sfxg_set_phase_centers.restype = C.c_int
         L.sfxg_set_phase_centers.argtypes = [C.c_void_p, C.c_int32, C.POINTER(DelayModelC), C.POINTER(C.c_double),
                                              C.POINTER(C.c_int32)]
@@ -252,6 +259,14 @@ class Context:
 
     def push_stream(self, i: int, hdr: bytes):
         self._check(lib().sfxg_push_stream(self._h, i, hdr, len(hdr)), f"sfxg_push_stream({i})")
+
+    def push_vdif(self, i: int, device_ptr: int, nbytes: int, frame_bytes: int, channel: int, thread_ids,
+                  ref_mjd: int, offset: int = 0):
+        """Stream i from raw VDIF frames in device memory (sfxg_push_vdif)."""
+        th = np.ascontiguousarray(thread_ids, dtype=np.int32)
+        v = VdifStreamC(frame_bytes, channel, len(th), ref_mjd, th.ctypes.data_as(C.POINTER(C.c_int32)), offset)
+        self._check(lib().sfxg_push_vdif(self._h, i, C.c_void_p(device_ptr), nbytes, C.byref(v)),
+                    f"sfxg_push_vdif({i})")
 
     def push_stream_elided(self, i: int, control: bytes, device_ptr: int, nbytes: int):
         self._check(lib().sfxg_push_stream_elided(self._h, i, control, len(control), C.c_void_p(device_ptr),

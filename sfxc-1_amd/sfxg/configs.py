@@ -25,7 +25,8 @@ def build_slices(*, seed: int, n_stations: int, pols: Sequence[str], fs: int, bw
                  window: int = WINDOW_HANN, sub_int_usec: Optional[int] = None,
                  extra: Optional[Tuple[float, Sequence[float], Sequence[float]]] = None,
                  fft_size_delaycor: Optional[int] = None, sideband: str = "U",
-                 invalid: Optional[dict] = None, n_integrations: Optional[int] = None
+                 invalid: Optional[dict] = None, n_integrations: Optional[int] = None,
+                 raw_out: Optional[dict] = None, margin_multiple: int = 1
                  ) -> List[Tuple[CorrelationParameters, List[bytes]]]:
     """Synthesise the raw streams of one subband and cut them into slices.
 
@@ -33,6 +34,10 @@ def build_slices(*, seed: int, n_stations: int, pols: Sequence[str], fs: int, bw
     extra: (rho2, offsets2, rates2) adds a second common component.
     invalid: {stream_index: [(byte_start_in_slice_raw, nbytes), ...]} marks
     input bytes invalid (relative to the first byte sent for each slice).
+    raw_out: if given, receives the raw packed streams ("raw", one per stream)
+    and their first sample's time ("raw_start", ticks).
+    margin_multiple: the raw streams start a multiple of this many samples
+    before the first slice (frame-aligned raw data for VDIF frames).
     """
     start = mjd_to_ticks(START_MJD, 0.0)
     tps = int(round(CLOCK_RATE / fs))
@@ -45,6 +50,8 @@ def build_slices(*, seed: int, n_stations: int, pols: Sequence[str], fs: int, bw
     spb = 8 // bits
     margin = int(math.ceil(max_neg * fs)) + 4096
     margin -= margin % (4 * spb)
+    step = 4 * spb * margin_multiple // math.gcd(4 * spb, margin_multiple)
+    margin = -(-margin // step) * step
     raw_start = start - margin * tps
     n_raw = margin + int(total_sec * fs) + int(math.ceil(max_pos * fs)) + 8192
     n_raw += (-n_raw) % (4096 * spb)
@@ -101,6 +108,9 @@ def build_slices(*, seed: int, n_stations: int, pols: Sequence[str], fs: int, bw
                 inv = [(base + a, n) for a, n in invalid[i]]
             hdrs.append(synth.header_stream(raw[i], raw_start, models[i], t0, nsamp, fs, bits, inv))
         slices.append((job, hdrs))
+    if raw_out is not None:
+        raw_out["raw"] = raw
+        raw_out["raw_start"] = raw_start
     return slices
 
 

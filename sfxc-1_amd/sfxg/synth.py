@@ -21,7 +21,7 @@ correlator node's result does not depend on where DATA records are split).
 from __future__ import annotations
 
 import math
-from typing import List, Tuple
+from typing import List, Sequence, Tuple
 
 import numpy as np
 
@@ -208,6 +208,64 @@ def join_elided(control: bytes, payload: np.ndarray) -> bytes:
         else:
             out.append(h)
             i += 1
+    return bytes(out)
+
+
+# --------------------------------------------------------------------------
+# VDIF frames
+# --------------------------------------------------------------------------
+def vdif_epoch_mjd(ref_epoch: int) -> int:
+    """MJD of the first day of a VDIF reference epoch (half years from 2000)."""
+    year, month = 2000 + ref_epoch // 2, 1 + 6 * (ref_epoch % 2)
+    a = (14 - month) // 12
+    y = year + 4800 - a
+    m = month + 12 * a - 3
+    jdn = 1 + (153 * m + 2) // 5 + 365 * y + y // 4 - y // 100 + y // 400 - 32045
+    return jdn - 2400001
+
+
+def vdif_header(sec: int, frame: int, ref_epoch: int, frame_bytes: int, thread: int, bits: int,
+                invalid: bool = False, legacy: bool = False, station: int = 0x4a42) -> bytes:
+    """A VDIF data frame header (VDIF specification, 32 bytes, 16 in legacy
+    mode): word 0 seconds, legacy and invalid flags; word 1 frame number in the
+    second and reference epoch; word 2 frame length in 8-byte units, log2 of
+    the channel count, version; word 3 station, thread, bits per sample - 1."""
+    w0 = (sec & 0x3FFFFFFF) | (int(legacy) << 30) | (int(invalid) << 31)
+    w1 = (frame & 0xFFFFFF) | ((ref_epoch & 0x3F) << 24)
+    w2 = ((frame_bytes // 8) & 0xFFFFFF) | (0 << 24) | (0 << 29)
+    w3 = (station & 0xFFFF) | ((thread & 0x3FF) << 16) | (((bits - 1) & 0x1F) << 26)
+    words = [w0, w1, w2, w3] + ([] if legacy else [0, 0, 0, 0])
+    return np.array(words, dtype="<u4").tobytes()
+
+
+def vdif_stream(raws: Sequence[np.ndarray], thread_ids: Sequence[int], raw_start_ticks: int, fs: int, bits: int,
+                payload: int, ref_epoch: int, invalid=(), fill=(), legacy: bool = False) -> bytes:
+    """Multi-thread VDIF frames of one station: thread_ids[c] carries raws[c]
+    (packed bytes whose first sample is at raw_start_ticks, on a frame
+    boundary), one frame per thread in turn. invalid: (thread index, frame)
+    pairs flagged invalid (their payload is kept); fill: frame slots written
+    as fill pattern before the frame of that index."""
+    tps = int(round(CLOCK_RATE / fs))
+    frame_samples = payload * 8 // bits
+    fps = fs // frame_samples
+    assert fps * frame_samples == fs, "frames per second must be whole"
+    epoch_ticks = (vdif_epoch_mjd(ref_epoch) - 51544) * 86400 * CLOCK_RATE
+    t0 = raw_start_ticks - epoch_ticks
+    assert t0 >= 0 and t0 % (frame_samples * tps) == 0, "raw data must start on a frame boundary"
+    first = t0 // (frame_samples * tps)
+    n_frames = min(len(r) for r in raws) // payload
+    hdr_len = 16 if legacy else 32
+    out = bytearray()
+    inv = set(invalid)
+    for f in range(n_frames):
+        if f in set(fill):
+            out.extend(np.full((payload + hdr_len) // 4, 0x11223344, dtype="<u4").tobytes())
+        g = first + f
+        sec, fr = g // fps, g % fps
+        for c, (r, th) in enumerate(zip(raws, thread_ids)):
+            out.extend(vdif_header(sec, fr, ref_epoch, payload + hdr_len, th, bits, invalid=(c, f) in inv,
+                                   legacy=legacy))
+            out.extend(np.asarray(r[f * payload:(f + 1) * payload], dtype=np.uint8).tobytes())
     return bytes(out)
 
 
