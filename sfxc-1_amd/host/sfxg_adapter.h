@@ -52,6 +52,11 @@ class Bit2float_worker {
   Bit2float_worker(Context &ctx, int stream_idx) : ctx_(ctx), stream_(stream_idx) {}
   // The reader thread's bytes for this slice (the Channel_circular_input_buffer feed).
   void put_bytes(const uint8_t *p, size_t n) { buf_.insert(buf_.end(), p, p + n); }
+  // The station's raw VDIF frames in device memory instead of a HEADER
+  // stream: the input node's reader and writer run in the library.
+  void put_vdif(const void *device_frames, size_t n_bytes, const sfxg_vdif_stream &v) {
+    ctx_.check(sfxg_push_vdif(ctx_.get(), stream_, device_frames, n_bytes, &v), "sfxg_push_vdif");
+  }
   bool has_work() const { return !buf_.empty() && buf_.back() == 3 /* HEADER_ENDSTREAM */; }
   int64_t do_task() {
     ctx_.check(sfxg_push_stream(ctx_.get(), stream_, buf_.data(), buf_.size()), "sfxg_push_stream");
