@@ -314,6 +314,15 @@ int sfxg_get_statistics(sfxg_ctx *ctx, int32_t stream_idx, int32_t levels[5], in
 /* Correlation_core::number_of_baselines (include/correlation_core.h:50-52). */
 int sfxg_number_of_baselines(sfxg_ctx *ctx, int32_t *n);
 
+/* The tsys side product of the slice: Correlation_core::tsys_write
+ * (src/correlation_core.cc:479-513) sends the output node one message per
+ * stream; this writes their bytes back to back, 44 per stream in stream
+ * order, in the layout MPI_Pack gives them on one host: station, frequency
+ * number, sideband (LSB 0, USB 1) and polarisation (R 0, L 1) as uint8, the
+ * slice start in ticks as uint64, then tsys on (low, high) and off (low,
+ * high) counts as int64. */
+int sfxg_fetch_tsys(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written);
+
 /* Copies one stage's device output (SFXG_STAGE_*) for verification; waits
  * for the slice. stream_idx selects the stream for per-stream stages. */
 int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst, size_t cap,

@@ -476,6 +476,21 @@ def rechannelize(job, norm: np.ndarray, precision="f32") -> np.ndarray:
     return out
 
 
+def tsys_messages(job, stats) -> bytes:
+    """Correlation_core::tsys_write (correlation_core.cc:479-513): per stream
+    station, frequency number, sideband, polarisation (uint8), the slice start
+    (uint64 ticks) and tsys on low/high, off low/high (int64), as MPI_Pack
+    lays them out on one host."""
+    import struct
+    out = bytearray()
+    for i, st in enumerate(job.station_streams):
+        ts = stats[i][1]
+        out += struct.pack("<4BQ4q", st.station_number, job.frequency_nr, 0 if job.sideband == "L" else 1,
+                           0 if st.polarisation == "R" else 1, job.start_time, int(ts[0]), int(ts[1]), int(ts[2]),
+                           int(ts[3]))
+    return bytes(out)
+
+
 # ----------------------------------------------------------------------------
 # Raw VDIF: the input node's reader and writer (test-side restatement)
 # ----------------------------------------------------------------------------

@@ -1457,6 +1457,35 @@ int sfxg_fetch_output(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) 
   return SFXG_OK;
 }
 
+int sfxg_fetch_tsys(sfxg_ctx *ctx, uint8_t *dst, size_t cap, size_t *written) {
+  if (!ctx || !dst) return SFXG_ERR_ARG;
+  if (!ctx->have_results) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");
+  const size_t rec = 4 + 8 + 4 * 8, need = rec * (size_t)ctx->M;
+  if (cap < need) return ctx->fail(SFXG_ERR_ARG, "tsys buffer holds %zu bytes, %zu needed", cap, need);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  sync_slice(ctx);
+  std::vector<int32_t> lv, ts;
+  const int rc = stats_of(ctx, lv, ts);
+  if (rc) return rc;
+  uint8_t *o = dst;
+  for (int i = 0; i < ctx->M; i++) {
+    const sfxg_station_parameters &st = ctx->st[i];
+    const uint8_t h[4] = {(uint8_t)st.station_number, (uint8_t)ctx->p.frequency_nr,
+                          (uint8_t)(ctx->p.sideband == 'L' ? 0 : 1), (uint8_t)(st.polarisation == 'R' ? 0 : 1)};
+    memcpy(o, h, 4);
+    const uint64_t ticks = (uint64_t)ctx->p.start_time;
+    memcpy(o + 4, &ticks, 8);
+    // tsys_on_lo = tsys[0], tsys_on_hi = tsys[1], tsys_off_lo = tsys[2], tsys_off_hi = tsys[3]
+    for (int k = 0; k < 4; k++) {
+      const int64_t v = ts[4 * i + k];
+      memcpy(o + 12 + 8 * k, &v, 8);
+    }
+    o += rec;
+  }
+  if (written) *written = need;
+  return SFXG_OK;
+}
+
 int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst, size_t cap, size_t *written) {
   if (!ctx || !dst) return SFXG_ERR_ARG;
   if (!ctx->have_results) return ctx->fail(SFXG_ERR_STATE, "sfxg_correlate first");

@@ -117,6 +117,8 @@ def lib() -> C.CDLL:
         L.sfxg_tune_import.argtypes = [C.c_char_p]
         L.sfxg_tune_clear.restype = None
         L.sfxg_tune_clear.argtypes = []
+        L.sfxg_fetch_tsys.restype = C.c_int
+        L.sfxg_fetch_tsys.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.sfxg_push_vdif.restype = C.c_int
         L.sfxg_push_vdif.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t, C.POINTER(VdifStreamC)]
         L.sfxg_set_phase_centers.restype = C.c_int
@@ -310,6 +312,14 @@ class Context:
         ms = (C.c_double * 5)()
         self._check(lib().sfxg_kernel_times(self._h, ms, 5, int(reset)), "sfxg_kernel_times")
         return np.array(ms[:])
+
+    def tsys(self) -> bytes:
+        """The slice's tsys_write messages (sfxg_fetch_tsys), 44 bytes per stream."""
+        n = 44 * len(self.job.station_streams)
+        buf = (C.c_uint8 * n)()
+        w = C.c_size_t()
+        self._check(lib().sfxg_fetch_tsys(self._h, buf, n, C.byref(w)), "sfxg_fetch_tsys")
+        return bytes(buf)[:w.value]
 
     def number_of_baselines(self) -> int:
         n = C.c_int32()

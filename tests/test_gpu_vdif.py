@@ -69,6 +69,7 @@ def test_vdif_ingest(ctx, nchan, rates):
         ctx.correlate()
         out = ctx.output()
         stats = [ctx.statistics(i) for i in range(len(job.station_streams))]
+        tsys = ctx.tsys()
     finally:
         for L, p in dev:
             L.hipFree(p)
@@ -79,6 +80,8 @@ def test_vdif_ingest(ctx, nchan, rates):
         np.testing.assert_array_equal(lv, ref["stats"][i][0])
         np.testing.assert_array_equal(ts, ref["stats"][i][1])
     assert ref["stats"][2][0][4] > 0 or ref["stats"][3][0][4] > 0  # the invalid frame reached station 1
+    # the tsys side product (tsys_write) of the same slice
+    assert tsys == O.tsys_messages(job, ref["stats"])
     heads_g, vis_g = vis_from_bytes(job, out)
     heads_r, vis_r = vis_from_bytes(job, ref["output"])
     assert heads_g == heads_r
