@@ -357,6 +357,72 @@ class MixedBatch:
         return wall
 
 
+def vdif_ingest(torch, capi, configs, dev, reps=3):
+    """Raw VDIF ingest of one slice of the main workload: every station's
+    frames (two threads, R and L, 8000-byte payloads of random 2-bit data)
+    resident in HBM; times sfxg_push_vdif for all streams (header words lifted
+    on the device, records built on the host, payload gathered on the device),
+    then the correlation of that slice."""
+    jobs, _ = configs.config1_jobs(0, n_integrations=1)
+    job = jobs[0]
+    st0 = job.station_streams[0]
+    fs, bits, payload, epoch, ref_mjd = st0.sample_rate, st0.bits_per_sample, 8000, 46, 60000
+    fb = payload + 32
+    frame_samples = payload * 8 // bits
+    tps = int(round(512_000_000 / fs))
+    frame_ticks = frame_samples * tps
+    fps = fs // frame_samples
+    margin = 24  # frames before and after the slice: the delays reach 5 ms
+    n_frames = (job.stop_time - job.start_time) // frame_ticks + 2 * margin
+    epoch_ticks = (59945 - 51544) * 86400 * 512_000_000
+    first = (job.start_time - margin * frame_ticks - epoch_ticks) // frame_ticks
+    g = first + np.arange(n_frames, dtype=np.int64)
+    rng = np.random.default_rng(5)
+    stations = sorted({s.station_number for s in job.station_streams})
+    bufs, streams = [], {}
+    for sn in stations:
+        idx = [i for i, s in enumerate(job.station_streams) if s.station_number == sn]
+        fr = np.empty((n_frames, len(idx), fb), dtype=np.uint8)
+        w = np.zeros((n_frames, len(idx), 8), dtype="<u4")
+        for c in range(len(idx)):
+            w[:, c, 0] = (g // fps).astype(np.uint32)
+            w[:, c, 1] = ((g % fps) | (epoch << 24)).astype(np.uint32)
+            w[:, c, 2] = fb // 8
+            w[:, c, 3] = sn | ((1 + c) << 16) | ((bits - 1) << 26)
+        fr[:, :, :32] = w.view(np.uint8).reshape(n_frames, len(idx), 32)
+        fr[:, :, 32:] = rng.integers(0, 256, size=(n_frames, len(idx), payload), dtype=np.uint8)
+        t = torch.from_numpy(fr.reshape(-1)).to(f"cuda:{dev}")
+        bufs.append(t)
+        for c, i in enumerate(idx):
+            streams[i] = (len(bufs) - 1, c, [1 + k for k in range(len(idx))])
+    torch.cuda.synchronize()
+    ctx = capi.Context(dev)
+    ctx.set_options(lut=1, precision=0)
+    ingest, total = [], []
+    for _ in range(reps):
+        ctx.set_parameters(job)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(len(job.station_streams)):
+            b, c, th = streams[i]
+            ctx.push_vdif(i, bufs[b].data_ptr(), bufs[b].numel(), fb, c, th, ref_mjd)
+        t1 = time.perf_counter()
+        ctx.correlate()
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        ingest.append(t1 - t0)
+        total.append(t2 - t0)
+    ctx.close()
+    frames_bytes = sum(int(b.numel()) for b in bufs)
+    used = len(job.station_streams) * (job.n_ffts_per_integration(0) * job.fft_size_delaycor) * bits // 8
+    best = min(ingest)
+    return {"workload": "one slice of the main workload as 2-thread VDIF per station, 8000-byte payloads",
+            "streams": len(job.station_streams), "frames_bytes_in_hbm": frames_bytes,
+            "payload_bytes_of_slice": used, "header_bytes_to_host": 16 * frames_bytes // fb,
+            "ingest_ms": round(best * 1e3, 2), "ingest_payload_GBs": round(used / best / 1e9, 1),
+            "ingest_plus_correlate_ms": round(min(total) * 1e3, 2)}
+
+
 def summarise(wl, wall, kt, precision="fp32"):
     job = wl.job
     nseg = job.number_ffts_in_integration()
@@ -458,6 +524,10 @@ def main():
             print(f"mixed done: {mixed['value']:.2f}", file=sys.stderr, flush=True)
         del wl
         torch.cuda.empty_cache()
+        if not args.main_only:
+            line["vdif_ingest"] = vdif_ingest(torch, capi, configs, dev)
+            print(f"VDIF ingest done: {line['vdif_ingest']['ingest_ms']} ms per slice", file=sys.stderr, flush=True)
+            torch.cuda.empty_cache()
     if args.only == "c2" or (args.only == "all" and not args.main_only):
         w2 = Workload(torch, capi, synth, lambda sb, k: configs.config2_jobs(sb, n_integrations=k), args.c2_steps, 4,
                       dev, 2000)
