@@ -217,6 +217,22 @@ typedef struct {
 int sfxg_push_vdif(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
                    const sfxg_vdif_stream *vdif);
 
+/* The phase-cal side product of a VDIF stream: Input_node_data_writer::do_phasecal
+ * (src/input_node_data_writer.cc:232-306) over the frames of stream
+ * `stream_idx` (as sfxg_push_vdif takes them). Every raw sample adds the
+ * reference's integer value ({-7,-2,2,7} or {-5,5}) to bin (position mod
+ * P), P = sample_rate / 10 kHz, of its integration; integrations start at
+ * frames whose start time is a multiple of integration_ticks, and an
+ * integration is written once the next one starts. dst receives one message
+ * per finished integration as the node sends it to the output node, in
+ * MPI_Pack's layout on one host: station, frequency number, sideband (LSB
+ * 0, USB 1), polarisation (R 0, L 1) as uint8, start and duration as int64
+ * ticks, P as int32, then the P sums as int32. *written holds the bytes
+ * needed (also when cap is too small, which returns SFXG_ERR_ARG). */
+int sfxg_vdif_phasecal(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                       const sfxg_vdif_stream *vdif, int64_t integration_ticks, uint8_t *dst, size_t cap,
+                       size_t *written);
+
 /* The phase centres after the pointing centre of a multi-phase-centre slice
  * (params->n_phase_centers > 1): replaces the further sources of each
  * stream's Delay_table_akima (src/delay_table_akima.cc:114-136) and their

@@ -548,6 +548,45 @@ def vdif_blocks(vdif: bytes, frame_bytes: int, thread_ids, channel: int, sample_
     return out
 
 
+def vdif_phasecal(vdif: bytes, frame_bytes: int, thread_ids, channel: int, job, stream: int, ref_mjd: int,
+                  integration_ticks: int, offset: int = 0) -> bytes:
+    """Input_node_data_writer::do_phasecal (input_node_data_writer.cc:232-306)
+    over channel `channel`'s frame blocks: the messages of the finished
+    integrations (station, frequency, sideband, polarisation as uint8; start
+    and duration as int64 ticks; the sample count and the sums as int32)."""
+    import struct
+    st = job.station_streams[stream]
+    fs, bits = st.sample_rate, st.bits_per_sample
+    spb = 8 // bits
+    P = int(fs / 10e3)
+    values = np.array([-7, -2, 2, 7] if bits == 2 else [-5, 5], dtype=np.int64)  # sample_value_2 / _1
+    acc = None
+    count = 0
+    pc_time = 0
+    out = bytearray()
+    for payload, t, invalid in vdif_blocks(vdif, frame_bytes, thread_ids, channel, fs, ref_mjd, offset):
+        if t % integration_ticks == 0:
+            if acc is not None:
+                out += struct.pack("<4Bqqi", st.station_number, job.frequency_nr, 1 if st.sideband == "U" else 0,
+                                   1 if st.polarisation == "L" else 0, pc_time, t - pc_time, P)
+                out += acc.astype("<i4").tobytes()
+            acc = np.zeros(P, dtype=np.int64)
+            count = 0
+            pc_time = t
+        if acc is None:
+            continue
+        n = len(payload) * spb
+        if invalid:
+            count += n
+            continue
+        b = np.frombuffer(payload, dtype=np.uint8)
+        codes = (b[:, None] >> (bits * np.arange(spb))[None, :]) & ((1 << bits) - 1)
+        count %= P
+        np.add.at(acc, (count + np.arange(n)) % P, values[codes.reshape(-1)])
+        count += n
+    return bytes(out)
+
+
 def vdif_header_stream(vdif: bytes, frame_bytes: int, thread_ids, channel: int, job, stream: int, ref_mjd: int,
                        offset: int = 0) -> bytes:
     """Input_node_data_writer::do_task (input_node_data_writer.cc:105-228) over

@@ -556,35 +556,129 @@ static std::vector<std::pair<int64_t, int64_t>> input_node_delays(const ModelCop
   return out;
 }
 
-int sfxg_push_vdif(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
-                   const sfxg_vdif_stream *v) {
-  if (!ctx || !device_frames || !v) return SFXG_ERR_ARG;
-  if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
-  if (stream_idx < 0 || stream_idx >= ctx->M) return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
-  if (v->frame_bytes < 32 || v->frame_bytes % 8) return ctx->fail(SFXG_ERR_ARG, "frame_bytes %d", v->frame_bytes);
-  if (v->n_threads < 0 || (v->n_threads > 0 && !v->thread_ids) || v->channel < 0 ||
-      (v->n_threads > 0 && v->channel >= v->n_threads) || (v->n_threads == 0 && v->channel != 0))
-    return ctx->fail(SFXG_ERR_ARG, "channel %d of %d threads", v->channel, v->n_threads);
-  const int64_t n_frames = (int64_t)(n_bytes / (size_t)v->frame_bytes);
-  if (n_frames <= 0) return ctx->fail(SFXG_ERR_ARG, "no whole VDIF frame");
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  sync_slice(ctx);
-  const sfxg_station_parameters &st = ctx->st[stream_idx];
-  // header words of every frame, to the host
+// The frames' header words lifted on the device, then VDIF_reader's walk
+// on the host: channel v->channel's frame blocks.
+static int vdif_channel_blocks(sfxg_ctx *ctx, int stream_idx, const void *device_frames, int64_t n_frames,
+                               const sfxg_vdif_stream *v, std::vector<VdifBlock> &blocks, int &frame_payload) {
   std::vector<uint32_t> words((size_t)n_frames * 4);
   HIPCHK(ctx, ctx->vdif_words.ensure(words.size() * sizeof(uint32_t)));
   HIPCHK(ctx, launch_vdif_words(device_frames, n_frames, v->frame_bytes, ctx->vdif_words.as<uint32_t>(), ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(words.data(), ctx->vdif_words.p, words.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
                              ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  // the channel's frame blocks and the slice's records
   std::vector<int> threads(v->thread_ids, v->thread_ids + v->n_threads);
+  std::string err;
+  const int rc = vdif_blocks(words.data(), n_frames, v->frame_bytes, ctx->st[stream_idx].sample_rate, v->ref_mjd,
+                             v->offset, threads, v->channel, blocks, frame_payload, err);
+  if (rc != SFXG_OK) return ctx->fail(rc, "stream %d: %s", stream_idx, err.c_str());
+  return SFXG_OK;
+}
+
+static int vdif_check(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                      const sfxg_vdif_stream *v) {
+  if (!ctx->have_params) return ctx->fail(SFXG_ERR_STATE, "sfxg_set_parameters first");
+  if (stream_idx < 0 || stream_idx >= ctx->M) return ctx->fail(SFXG_ERR_ARG, "stream index %d", stream_idx);
+  if (v->frame_bytes < 32 || v->frame_bytes % 8) return ctx->fail(SFXG_ERR_ARG, "frame_bytes %d", v->frame_bytes);
+  if (v->n_threads < 0 || (v->n_threads > 0 && !v->thread_ids) || v->channel < 0 ||
+      (v->n_threads > 0 && v->channel >= v->n_threads) || (v->n_threads == 0 && v->channel != 0))
+    return ctx->fail(SFXG_ERR_ARG, "channel %d of %d threads", v->channel, v->n_threads);
+  if ((int64_t)(n_bytes / (size_t)v->frame_bytes) <= 0) return ctx->fail(SFXG_ERR_ARG, "no whole VDIF frame");
+  (void)device_frames;
+  return SFXG_OK;
+}
+
+int sfxg_vdif_phasecal(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                       const sfxg_vdif_stream *v, int64_t integration_ticks, uint8_t *dst, size_t cap,
+                       size_t *written) {
+  if (!ctx || !device_frames || !v || (!dst && cap)) return SFXG_ERR_ARG;
+  int rc = vdif_check(ctx, stream_idx, device_frames, n_bytes, v);
+  if (rc != SFXG_OK) return rc;
+  if (integration_ticks <= 0) return ctx->fail(SFXG_ERR_ARG, "phase-cal integration time %lld", (long long)integration_ticks);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  sync_slice(ctx);
+  const sfxg_station_parameters &st = ctx->st[stream_idx];
+  const int64_t n_frames = (int64_t)(n_bytes / (size_t)v->frame_bytes);
+  std::vector<VdifBlock> blocks;
+  int frame_payload = 0;
+  rc = vdif_channel_blocks(ctx, stream_idx, device_frames, n_frames, v, blocks, frame_payload);
+  if (rc != SFXG_OK) return rc;
+  // do_phasecal's walk over the blocks (input_node_data_writer.cc:232-306):
+  // an integration starts at a block whose start time is a multiple of the
+  // integration time; the fold position restarts there and advances over
+  // invalid blocks without adding
+  const int P = (int)(st.sample_rate / 10e3);
+  const int spb = 8 / st.bits_per_sample;
+  const int S = frame_payload * spb;
+  if (P <= 0 || P % spb) return ctx->fail(SFXG_ERR_ARG, "phase-cal period of %d samples", P);
+  std::vector<PcalFrame> table;
+  std::vector<std::pair<int64_t, int64_t>> closed;  // (start, duration) of the finished integrations
+  int integ = -1;
+  int64_t count = 0, pc_time = 0;
+  for (const VdifBlock &b : blocks) {
+    if (b.time % integration_ticks == 0) {
+      if (integ >= 0) closed.push_back({pc_time, b.time - pc_time});
+      integ++;
+      count = 0;
+      pc_time = b.time;
+    }
+    if (integ < 0) continue;
+    if (!b.invalid) table.push_back({b.payload_offset, (int32_t)(count % P), integ});
+    count = count % P + S;
+  }
+  const size_t rec = 4 + 8 + 8 + 4 + 4 * (size_t)P;
+  const size_t need = rec * closed.size();
+  if (written) *written = need;
+  if (cap < need) return ctx->fail(SFXG_ERR_ARG, "phase-cal buffer holds %zu bytes, %zu needed", cap, need);
+  if (closed.empty()) return SFXG_OK;
+  const size_t n_sums = (size_t)(integ + 1) * P;
+  DBuf sums, tab;
+  hipError_t e = sums.ensure(n_sums * sizeof(int32_t));
+  if (e == hipSuccess) e = tab.ensure(std::max<size_t>(table.size(), 1) * sizeof(PcalFrame));
+  if (e == hipSuccess) e = launch_fill_zero(sums.p, n_sums * sizeof(int32_t), ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(tab.p, table.data(), table.size() * sizeof(PcalFrame), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = launch_phasecal(device_frames, tab.as<PcalFrame>(), (int)table.size(), S, P, st.bits_per_sample,
+                        sums.as<int32_t>(), ctx->stream);
+  std::vector<int32_t> h(n_sums);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), sums.p, n_sums * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  sums.release();
+  tab.release();
+  HIPCHK(ctx, e);
+  // the MPI_Pack layout of the messages: station, frequency number,
+  // sideband, polarisation (uint8), start and duration (int64 ticks),
+  // sample count (int32), the sums (int32)
+  uint8_t *o = dst;
+  for (size_t i = 0; i < closed.size(); i++) {
+    const uint8_t hd[4] = {(uint8_t)st.station_number, (uint8_t)ctx->p.frequency_nr,
+                           (uint8_t)(st.sideband == 'U' ? 1 : 0), (uint8_t)(st.polarisation == 'L' ? 1 : 0)};
+    memcpy(o, hd, 4);
+    memcpy(o + 4, &closed[i].first, 8);
+    memcpy(o + 12, &closed[i].second, 8);
+    const int32_t n = P;
+    memcpy(o + 20, &n, 4);
+    memcpy(o + 24, &h[i * P], 4 * (size_t)P);
+    o += rec;
+  }
+  return SFXG_OK;
+}
+
+int sfxg_push_vdif(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                   const sfxg_vdif_stream *v) {
+  if (!ctx || !device_frames || !v) return SFXG_ERR_ARG;
+  const int vrc = vdif_check(ctx, stream_idx, device_frames, n_bytes, v);
+  if (vrc != SFXG_OK) return vrc;
+  const int64_t n_frames = (int64_t)(n_bytes / (size_t)v->frame_bytes);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  sync_slice(ctx);
+  const sfxg_station_parameters &st = ctx->st[stream_idx];
   std::vector<VdifBlock> blocks;
   int frame_payload = 0;
   std::string err;
-  int rc = vdif_blocks(words.data(), n_frames, v->frame_bytes, st.sample_rate, v->ref_mjd, v->offset, threads,
-                       v->channel, blocks, frame_payload, err);
-  if (rc != SFXG_OK) return ctx->fail(rc, "stream %d: %s", stream_idx, err.c_str());
+  int rc = vdif_channel_blocks(ctx, stream_idx, device_frames, n_frames, v, blocks, frame_payload);
+  if (rc != SFXG_OK) return rc;
   const int64_t nsamples = (int64_t)ctx->n_ffts_per_integration * ctx->nd;
   const auto delays = input_node_delays(ctx->models[stream_idx], ctx->p.start_time, nsamples, st.sample_rate);
   std::vector<uint8_t> control;

@@ -100,6 +100,16 @@ hipError_t launch_vdif_words(const void *frames, int64_t n_frames, int frame_byt
                              hipStream_t s);
 hipError_t launch_vdif_gather(const void *frames, const VdifPiece *pieces, const int64_t *dst_off, int n_pieces,
                               void *dst, hipStream_t s);
+// Phase-cal accumulation: per valid frame its payload offset, the fold
+// position of its first sample and its integration; sums[integration][P]
+// (zeroed by the caller) gain the reference's integer sample values.
+struct PcalFrame {
+  int64_t payload_offset;
+  int32_t count0;
+  int32_t integration;
+};
+hipError_t launch_phasecal(const void *frames, const PcalFrame *table, int n_frames, int samples_per_frame, int P,
+                           int bits, int32_t *sums, hipStream_t s);
 
 // Number of correlation segments Delay_correction pushes per bit2float
 // chunk (delay_correction.cc:24-36,58-69).

@@ -33,7 +33,56 @@ __global__ __launch_bounds__(256) void k_vdif_gather(const uint8_t *__restrict__
   }
 }
 
+// Phase-cal accumulation (Input_node_data_writer::do_phasecal,
+// input_node_data_writer.cc:232-306): sample j of a frame adds its value to
+// bin (count0 + j) mod P of the frame's integration. One thread per bin
+// walks a group of frames; its sum per integration is added to the bins
+// with an integer atomic, so the result does not depend on the order.
+__global__ __launch_bounds__(256) void k_phasecal(const uint8_t *__restrict__ frames,
+                                                  const PcalFrame *__restrict__ table, int n_frames,
+                                                  int frames_per_group, int samples_per_frame, int P, int bits,
+                                                  int32_t *__restrict__ sums) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= P) return;
+  const int f0 = blockIdx.y * frames_per_group;
+  const int f1 = min(n_frames, f0 + frames_per_group);
+  const int spb = 8 / bits;
+  int32_t acc = 0;
+  int cur = -1;
+  for (int f = f0; f < f1; f++) {
+    const PcalFrame t = table[f];
+    if (t.integration != cur) {
+      if (cur >= 0 && acc != 0) atomicAdd(&sums[(int64_t)cur * P + k], acc);
+      acc = 0;
+      cur = t.integration;
+    }
+    const uint8_t *pay = frames + t.payload_offset;
+    int j = k - t.count0;
+    if (j < 0) j += P;
+    for (; j < samples_per_frame; j += P) {
+      const int byte = pay[j / spb];
+      if (bits == 2) {
+        const int code = (byte >> (2 * (j & 3))) & 3;
+        acc += code == 0 ? -7 : code == 1 ? -2 : code == 2 ? 2 : 7;  // sample_value_2
+      } else {
+        acc += ((byte >> (j & 7)) & 1) ? 5 : -5;  // sample_value_1
+      }
+    }
+  }
+  if (cur >= 0 && acc != 0) atomicAdd(&sums[(int64_t)cur * P + k], acc);
+}
+
 }  // namespace
+
+hipError_t launch_phasecal(const void *frames, const PcalFrame *table, int n_frames, int samples_per_frame, int P,
+                           int bits, int32_t *sums, hipStream_t s) {
+  if (n_frames <= 0) return hipSuccess;
+  const int fpg = 16;
+  dim3 grid((unsigned)((P + 255) / 256), (unsigned)((n_frames + fpg - 1) / fpg));
+  hipLaunchKernelGGL(k_phasecal, grid, dim3(256), 0, s, static_cast<const uint8_t *>(frames), table, n_frames, fpg,
+                     samples_per_frame, P, bits, sums);
+  return hipGetLastError();
+}
 
 hipError_t launch_vdif_words(const void *frames, int64_t n_frames, int frame_bytes, uint32_t *words,
                              hipStream_t s) {

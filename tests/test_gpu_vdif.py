@@ -108,3 +108,30 @@ def test_vdif_rejects(ctx):
             ctx.push_vdif(0, p.value, len(v), PAYLOAD + 32, 0, [8], REF_MJD)
     finally:
         L.hipFree(p)
+
+
+@pytest.mark.parametrize("bits,frames_per_integration", [(2, 8), (2, 25), (1, 6)])
+def test_vdif_phasecal(ctx, bits, frames_per_integration):
+    """Phase-cal sums of the raw samples on the GPU (sfxg_vdif_phasecal)
+    against the oracle's do_phasecal: integer sums, identical bytes."""
+    print(f"case phasecal bits={bits} frames_per_integration={frames_per_integration}", flush=True)
+    raw = {}
+    spf = PAYLOAD * 8 // bits
+    job, _ = configs.build_slices(seed=73, n_stations=2, pols=["R", "L"], fs=FS, bw=FS // 2, nchan=512,
+                                  seconds=0.0625, t_int_usec=62500, rho=0.05, bits=bits, sky_freq=5_000_000_000,
+                                  offsets=[0.0, 1e-6], rates=[0.0, 0.0], raw_out=raw, margin_multiple=spf)[0]
+    idx = [i for i, s in enumerate(job.station_streams) if s.station_number == job.station_streams[0].station_number]
+    v = synth.vdif_stream([raw["raw"][i] for i in idx], [4, 6], raw["raw_start"], FS, bits, PAYLOAD, EPOCH,
+                          invalid=[(1, 11), (0, 30)], fill=[2])
+    T = frames_per_integration * spf * (512_000_000 // FS)
+    L, p = to_device(v)
+    try:
+        ctx.set_options(lut=LUT_CONFIG)
+        ctx.set_parameters(job)
+        for c, i in enumerate(idx):
+            got = ctx.vdif_phasecal(i, p.value, len(v), PAYLOAD + 32, c, [4, 6], REF_MJD, T)
+            ref = O.vdif_phasecal(v, PAYLOAD + 32, [4, 6], c, job, i, REF_MJD, T)
+            assert len(ref) > 0
+            assert got == ref
+    finally:
+        L.hipFree(p)

@@ -89,3 +89,45 @@ def test_threads_fill_and_invalid(vdif_job):
     frame_samples = PAYLOAD * 8 // BITS
     assert diff[-1] - diff[0] < frame_samples
     assert 0 < got["n_invalid"] <= frame_samples
+
+
+def test_phasecal_periodic_closed_form(vdif_job):
+    """Data periodic in the phase-cal period P: every bin of an integration
+    holds its value times the number of valid samples that fold onto it;
+    an invalid frame takes its samples out of the count (do_phasecal,
+    input_node_data_writer.cc:232-306)."""
+    job, _, raw = vdif_job
+    P = FS // 10_000                      # 3200 samples
+    spb = 8 // BITS
+    tps = 512_000_000 // FS
+    S = PAYLOAD * spb
+    n_frames = len(raw["raw"][0]) // PAYLOAD
+    rng = np.random.default_rng(3)
+    codes = rng.integers(0, 4, P)
+    k = np.arange(n_frames * S)
+    pattern = codes[k % P].reshape(-1, spb)
+    data = np.zeros(len(pattern), dtype=np.uint8)
+    for j in range(spb):
+        data |= (pattern[:, j] << (2 * j)).astype(np.uint8)
+    T = 8 * S * tps  # integrations of eight frames
+    # the first frame at a multiple of T opens the first integration
+    f0 = next(f for f in range(n_frames) if (raw["raw_start"] + f * S * tps) % T == 0)
+    bad = f0 + 3
+    v = synth.vdif_stream([data], [0], raw["raw_start"], FS, BITS, PAYLOAD, EPOCH, invalid=[(0, bad)])
+    msgs = O.vdif_phasecal(v, FRAME, [0], 0, job, 0, REF_MJD, T)
+    rec = 24 + 4 * P
+    assert len(msgs) % rec == 0 and len(msgs) // rec >= 2
+    values = np.array([-7, -2, 2, 7])
+    for m in range(len(msgs) // rec):
+        body = msgs[m * rec:(m + 1) * rec]
+        start = int(np.frombuffer(body[4:12], dtype="<i8")[0])
+        dur = int(np.frombuffer(body[12:20], dtype="<i8")[0])
+        n = int(np.frombuffer(body[20:24], dtype="<i4")[0])
+        assert n == P and dur == T and start == raw["raw_start"] + (f0 + 8 * m) * S * tps
+        sums = np.frombuffer(body[24:], dtype="<i4")
+        frames = range(f0 + 8 * m, f0 + 8 * m + 8)
+        pos = np.concatenate([np.arange(j * S, (j + 1) * S) for j, f in enumerate(frames) if f != bad])
+        cnt = np.bincount(pos % P, minlength=P)
+        # the data at fold position q is codes[(absolute sample) % P]; integrations start on multiples of P
+        absq = ((f0 + 8 * m) * S) % P
+        np.testing.assert_array_equal(sums, cnt * values[codes[(np.arange(P) + absq) % P]])
