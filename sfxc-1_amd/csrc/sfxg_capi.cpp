@@ -1320,7 +1320,8 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
       if (t.xs_waves == 0)
         HIPCHK(ctx, launch_xstep_tile(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
                                       ctx->fold_buf.as<int32_t>() + g0, ng, ctx->tiles.as<int32_t>(),
-                                      (int)(ctx->h_tiles.size() / 18), B, ctx->partial.as<float2>(), fold_vis, s));
+                                      (int)(ctx->h_tiles.size() / 18), blf, bls, B, ctx->partial.as<float2>(),
+                                      fold_vis, s));
       else
         HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
                                  ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),

@@ -266,7 +266,8 @@ void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::ve
                       std::vector<int32_t> &table);
 hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
-                             int n_baselines, float2 *partial, float2 *vis, hipStream_t s);
+                             const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float2 *partial,
+                             float2 *vis, hipStream_t s);
 // half_spectra: spectra packed in the operand type (FP16 / BF16 only).
 hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spectra, int64_t stream_stride,
                              int64_t seg_stride,

@@ -1389,9 +1389,9 @@ constexpr int XT_NONE = INT32_MIN;
 constexpr int XT_PF = 8;       // prefetch registers per thread (float2)
 
 __global__ __launch_bounds__(64 * XT_WAVES) void k_xstep_tile(
-    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch, int M,
-    const int32_t *__restrict__ fold_start, const int32_t *__restrict__ tiles, int n_tiles, int n_baselines, int sps,
-    float2 *__restrict__ partial) {
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch,
+    int n_run, int M, const int32_t *__restrict__ fold_start, const int32_t *__restrict__ tiles, int n_tiles,
+    int n_baselines, int sps, float2 *__restrict__ partial) {
 #pragma clang fp contract(off)
   extern __shared__ float2 xlds[];
   const int lane = threadIdx.x & 63;
@@ -1420,7 +1420,7 @@ __global__ __launch_bounds__(64 * XT_WAVES) void k_xstep_tile(
         const int c = idx & 63;
         const int r = idx >> 6;
         const int sl = r / M, st = r - sl * M;
-        if (ch0 + c < n_ch) v = spectra[st * stream_stride + (s0 + sl - seg_base) * seg_stride + ch0 + c];
+        if (ch0 + c < n_run) v = spectra[st * stream_stride + (s0 + sl - seg_base) * seg_stride + ch0 + c];
       }
       pf[u] = v;
     }
@@ -1464,7 +1464,7 @@ __global__ __launch_bounds__(64 * XT_WAVES) void k_xstep_tile(
     s0 = s1;
     ns = ns1;
   }
-  if (active && ch < n_ch) {
+  if (active && ch < n_run) {
     float2 *o = partial + (int64_t)k * n_baselines * n_ch + ch;
 #pragma unroll
     for (int q = 0; q < XT * XT; q++) {
@@ -1474,6 +1474,60 @@ __global__ __launch_bounds__(64 * XT_WAVES) void k_xstep_tile(
       else o[(int64_t)(~e) * n_ch] = make_float2(acc[q].x, -acc[q].y);
     }
   }
+}
+
+// The last bin (channel ch = N_c, the Nyquist bin) of every baseline and
+// sub-integration for the FP32 tile X-step, whose workgroups map 64
+// channels to the lanes: N_c + 1 bins would otherwise need one more block
+// of workgroups for a single channel. Thread = baseline; the bin of 16
+// segments of every stream is staged in LDS, then each baseline adds its
+// products in segment order with the tile kernel's expressions, so the
+// sums are the same bits. partial[k * k_stride + b * b_stride].
+constexpr int XL_SEGS = 16;
+__global__ __launch_bounds__(256) void k_xstep_last(const float2 *__restrict__ spectra, int64_t stream_stride,
+                                                    int64_t seg_stride, int64_t seg_base, int ch, int M,
+                                                    const int32_t *__restrict__ fold_start,
+                                                    const int32_t *__restrict__ bl_first,
+                                                    const int32_t *__restrict__ bl_second, int n_baselines,
+                                                    float2 *__restrict__ partial, int64_t k_stride, int64_t b_stride) {
+#pragma clang fp contract(off)
+  extern __shared__ float2 col[];  // [XL_SEGS][M]
+  const int k = blockIdx.x;
+  const int b = blockIdx.y * 256 + threadIdx.x;
+  const int64_t s_begin = fold_start[k], s_end = fold_start[k + 1];
+  const int a = b < n_baselines ? bl_first[b] : 0, c = b < n_baselines ? bl_second[b] : 0;
+  float2 acc = make_float2(0.0f, 0.0f);
+  for (int64_t s0 = s_begin; s0 < s_end; s0 += XL_SEGS) {
+    const int n = (int)(s_end - s0 < XL_SEGS ? s_end - s0 : XL_SEGS);
+    for (int i = threadIdx.x; i < n * M; i += 256) {
+      const int sl = i / M, st = i - sl * M;
+      col[i] = spectra[st * stream_stride + (s0 + sl - seg_base) * seg_stride + ch];
+    }
+    __syncthreads();
+    if (b < n_baselines) {
+      for (int sl = 0; sl < n; sl++) {
+        const float2 x = col[sl * M + a], y = col[sl * M + c];
+        const float pr = x.x * y.x + x.y * y.y;
+        const float pi = x.y * y.x - x.x * y.y;
+        acc.x = acc.x + pr;
+        acc.y = acc.y + pi;
+      }
+    }
+    __syncthreads();
+  }
+  if (b < n_baselines) partial[k * k_stride + b * b_stride] = acc;
+}
+
+static hipError_t launch_xstep_last(const float2 *spectra, int64_t stream_stride, int64_t seg_stride,
+                                    int64_t seg_base, int ch, int M, const int32_t *fold_start, int n_k,
+                                    const int32_t *bl_first, const int32_t *bl_second, int n_baselines,
+                                    float2 *partial, int64_t k_stride, int64_t b_stride, hipStream_t s) {
+  const size_t lds = (size_t)XL_SEGS * M * sizeof(float2);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  dim3 grid((unsigned)n_k, (unsigned)((n_baselines + 255) / 256));
+  hipLaunchKernelGGL(k_xstep_last, grid, dim3(256), lds, s, spectra, stream_stride, seg_stride, seg_base, ch, M,
+                     fold_start, bl_first, bl_second, n_baselines, partial, k_stride, b_stride);
+  return hipGetLastError();
 }
 
 // vis += partial[k] for k = 0 .. n_k - 1, in fold order.
@@ -2716,7 +2770,8 @@ void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::ve
 
 hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
-                             int n_baselines, float2 *partial, float2 *vis, hipStream_t s) {
+                             const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float2 *partial,
+                             float2 *vis, hipStream_t s) {
   if (n_k <= 0) return hipSuccess;
   // a stage fills the threads' prefetch registers
   int sps = (XT_PF * 64 * XT_WAVES) / (M * 64);
@@ -2726,11 +2781,19 @@ hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   hipError_t e = hipFuncSetAttribute((const void *)k_xstep_tile, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)((n_tiles + XT_WAVES - 1) / XT_WAVES), (unsigned)n_k);
+  // the last bin of N_c + 1 on its own, so that the tile grid covers whole
+  // blocks of 64 channels
+  const int n_run = n_ch > 1 && (n_ch - 1) % 64 == 0 ? n_ch - 1 : n_ch;
+  dim3 grid((unsigned)((n_run + 63) / 64), (unsigned)((n_tiles + XT_WAVES - 1) / XT_WAVES), (unsigned)n_k);
   hipLaunchKernelGGL(k_xstep_tile, grid, dim3(64 * XT_WAVES), lds, s, spectra, stream_stride, seg_stride, seg_base,
-                     n_ch, M, fold_start, tiles, n_tiles, n_baselines, sps, partial);
+                     n_ch, n_run, M, fold_start, tiles, n_tiles, n_baselines, sps, partial);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (n_run < n_ch) {
+    e = launch_xstep_last(spectra, stream_stride, seg_stride, seg_base, n_ch - 1, M, fold_start, n_k, bl_first,
+                          bl_second, n_baselines, partial + (n_ch - 1), (int64_t)n_baselines * n_ch, n_ch, s);
+    if (e != hipSuccess) return e;
+  }
   if (!vis) return hipSuccess;  // the caller folds the partials itself
   const int64_t n = (int64_t)n_baselines * n_ch;
   hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
