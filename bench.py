@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE counters scaled by a 1 GiB copy kernel's readings
 # (profiles/fengine_xstep_plain_fp32_8st_8sb_4096ch.md); re-measured whenever
 # the kernel changes.
-MEASURED_TRAFFIC = {("fengine", 4096, 16): 8.94e9}
+MEASURED_TRAFFIC = {("fengine", 4096, 16): 9.07e9}
 # Wave-level VALU instructions per launch of the same kernel (SQ_INSTS_VALU,
 # same profile), against the wave64 FP32 issue rate: 256 CUs x 1 instruction
 # per cycle (a wave64 instruction occupies a 16-lane SIMD for 4 cycles, 4

@@ -1479,26 +1479,26 @@ __global__ __launch_bounds__(64 * XT_WAVES) void k_xstep_tile(
 // The last bin (channel ch = N_c, the Nyquist bin) of every baseline and
 // sub-integration for the FP32 tile X-step, whose workgroups map 64
 // channels to the lanes: N_c + 1 bins would otherwise need one more block
-// of workgroups for a single channel. Thread = baseline; the bin of 16
-// segments of every stream is staged in LDS, then each baseline adds its
-// products in segment order with the tile kernel's expressions, so the
-// sums are the same bits. partial[k * k_stride + b * b_stride].
-constexpr int XL_SEGS = 16;
+// of workgroups for a single channel. Thread = baseline; the bin of `segs`
+// segments of every stream is staged in LDS (one scattered load per thread
+// and stream-segment), then each baseline adds its products in segment
+// order with the tile kernel's expressions, so the sums are the same bits.
+// partial[k * k_stride + b * b_stride].
 __global__ __launch_bounds__(256) void k_xstep_last(const float2 *__restrict__ spectra, int64_t stream_stride,
-                                                    int64_t seg_stride, int64_t seg_base, int ch, int M,
+                                                    int64_t seg_stride, int64_t seg_base, int ch, int M, int segs,
                                                     const int32_t *__restrict__ fold_start,
                                                     const int32_t *__restrict__ bl_first,
                                                     const int32_t *__restrict__ bl_second, int n_baselines,
                                                     float2 *__restrict__ partial, int64_t k_stride, int64_t b_stride) {
 #pragma clang fp contract(off)
-  extern __shared__ float2 col[];  // [XL_SEGS][M]
+  extern __shared__ float2 col[];  // [segs][M]
   const int k = blockIdx.x;
   const int b = blockIdx.y * 256 + threadIdx.x;
   const int64_t s_begin = fold_start[k], s_end = fold_start[k + 1];
   const int a = b < n_baselines ? bl_first[b] : 0, c = b < n_baselines ? bl_second[b] : 0;
   float2 acc = make_float2(0.0f, 0.0f);
-  for (int64_t s0 = s_begin; s0 < s_end; s0 += XL_SEGS) {
-    const int n = (int)(s_end - s0 < XL_SEGS ? s_end - s0 : XL_SEGS);
+  for (int64_t s0 = s_begin; s0 < s_end; s0 += segs) {
+    const int n = (int)(s_end - s0 < segs ? s_end - s0 : segs);
     for (int i = threadIdx.x; i < n * M; i += 256) {
       const int sl = i / M, st = i - sl * M;
       col[i] = spectra[st * stream_stride + (s0 + sl - seg_base) * seg_stride + ch];
@@ -1522,11 +1522,13 @@ static hipError_t launch_xstep_last(const float2 *spectra, int64_t stream_stride
                                     int64_t seg_base, int ch, int M, const int32_t *fold_start, int n_k,
                                     const int32_t *bl_first, const int32_t *bl_second, int n_baselines,
                                     float2 *partial, int64_t k_stride, int64_t b_stride, hipStream_t s) {
-  const size_t lds = (size_t)XL_SEGS * M * sizeof(float2);
+  // up to 64 segments per pass, within 64 KiB of LDS
+  const int segs = std::max(1, std::min(64, 8192 / std::max(M, 1)));
+  const size_t lds = (size_t)segs * M * sizeof(float2);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   dim3 grid((unsigned)n_k, (unsigned)((n_baselines + 255) / 256));
   hipLaunchKernelGGL(k_xstep_last, grid, dim3(256), lds, s, spectra, stream_stride, seg_stride, seg_base, ch, M,
-                     fold_start, bl_first, bl_second, n_baselines, partial, k_stride, b_stride);
+                     segs, fold_start, bl_first, bl_second, n_baselines, partial, k_stride, b_stride);
   return hipGetLastError();
 }
 
