@@ -2912,7 +2912,10 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
   if (stream_stride * M >= ((int64_t)1 << 32)) return hipErrorInvalidValue;  // 32-bit item offsets
   const size_t lds = lds_of(CH, KS);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  const int n_blk = (n_ch + CH - 1) / CH;
+  // FP32 spectra: the last bin of N_c + 1 on its own (in FP32), so that the
+  // grid covers whole blocks of CH channels
+  const int n_run = !half_spectra && n_ch > 1 && (n_ch - 1) % CH == 0 ? n_ch - 1 : n_ch;
+  const int n_blk = (n_run + CH - 1) / CH;
   // channel blocks that share a 128-byte line of a spectra row
   const int G = std::max(1, 128 / (CH * (half_spectra ? 4 : 8)));
   dim3 grid((unsigned)(n_blk * n_k));
@@ -2937,6 +2940,12 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
   }
 #undef SFXG_MFMA_CASE
   if (e != hipSuccess) return e;
+  if (n_run < n_ch) {
+    e = launch_xstep_last(static_cast<const float2 *>(spectra), stream_stride, seg_stride, seg_base, n_ch - 1, M,
+                          fold_start, n_k, bl_first, bl_second, n_baselines,
+                          partial + (int64_t)(n_ch - 1) * n_baselines, (int64_t)n_ch * n_baselines, 1, s);
+    if (e != hipSuccess) return e;
+  }
   dim3 gc((unsigned)((n_baselines + 63) / 64), (unsigned)((n_ch + 63) / 64));
   hipLaunchKernelGGL(k_fold_combine_cm, gc, dim3(256), 0, s, partial, n_k, n_ch, n_baselines, vis);
   return hipGetLastError();
