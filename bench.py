@@ -139,7 +139,7 @@ def _cpu_run(n, sample_usec):
 
 
 def cpu_cores():
-    """Host cores this process may use, at most 16 (the GPU box's CPU share)."""
+    """Host cores this process may use, at most 16 (a one-GPU host's CPU share)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:

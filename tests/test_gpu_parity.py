@@ -324,8 +324,8 @@ def test_cpp_host_adapter(ctx, c0_short, tmp_path):
 
 
 # Mixed-precision error bars against the FP64 oracle, per product: the
-# relative L2 error over channels, ||V - V64|| / ||V64|| (the north-star
-# "visibility error relative to the FP64 oracle", 1e-3 for FP16), and the
+# relative L2 error over channels, ||V - V64|| / ||V64|| (the visibility
+# error relative to the FP64 oracle, at most 1e-3 for FP16), and the
 # worst single channel relative to the product's rms, max |V - V64| / rms |V64|.
 MIXED_BARS = {1: (1e-3, 4e-3), 2: (8e-3, 3e-2), 3: (1e-5, 5e-5)}
 
