@@ -561,7 +561,8 @@ def test_batch_graph_replays_match_eager():
     eager_ctx.close()
 
 
-# (nchan, precision, pulsar, fields) with fields = (fe_threads, chan_wgs, xs_waves, mf_channels, mfp_wgs)
+# (nchan, precision, pulsar, fields) with fields = (fe_threads, chan_wgs, xs_waves, mf_channels, mfp_wgs);
+# split FP16 with 16-channel blocks hands bin N_c to the last-bin kernel (1024 = 64 x 16), with 24 it does not
 TUNE_VARIANTS = [
     (512, 0, False, (64, 0, 0, 0, 0)),
     (1024, 0, False, (128, 0, 4, 0, 0)),
@@ -570,6 +571,8 @@ TUNE_VARIANTS = [
     (16384, 0, False, (0, 1024, 0, 0, 0)),
     (8192, 0, False, (0, 4096, 0, 0, 0)),
     (1024, 1, False, (128, 0, 0, 8, 0)),
+    (1024, 3, False, (128, 0, 0, 16, 0)),
+    (1024, 3, False, (128, 0, 0, 24, 0)),
     (1024, 1, True, (0, 0, 0, 0, 512)),
 ]
 
