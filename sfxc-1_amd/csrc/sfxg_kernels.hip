@@ -2306,6 +2306,9 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
   const int chan_stride = R * MF_RS + 8;
   const int plane_elems = CH * chan_stride;
   int16_t *bins = reinterpret_cast<int16_t *>(ylds + planes * plane_elems);  // [CH][MF_KS]
+  // runs of equal bin per channel and stage: runs[c][0] = count, runs[c][1 + r]
+  // = bin << 16 | mask of the run's segments (bit k: segment s0 + k)
+  int32_t *runs = reinterpret_cast<int32_t *>(bins + 64 * MF_KS);  // [CH][MF_KS + 1]
   for (int i = threadIdx.x; i < planes * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
   f16v_t acc[MFP_MAXP];
   int cur[MFP_MAXP];
@@ -2339,6 +2342,24 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
                     : (int16_t)-1;
     }
     __syncthreads();
+    // one thread per channel splits its stage into runs of equal bin, in
+    // segment order (segments without a bin are in no run), so that every
+    // (channel, tile) pair reads its runs instead of scanning the bins
+    if (threadIdx.x < CH) {
+      const int16_t *bc = bins + threadIdx.x * MF_KS;
+      int32_t *rc = runs + threadIdx.x * (MF_KS + 1);
+      int n = 0;
+      for (int k = 0; k < MF_KS;) {
+        const int b = bc[k];
+        unsigned m = 0;
+        int k1 = k;
+        while (k1 < MF_KS && bc[k1] == b) m |= 1u << k1++;
+        if (b >= 0) rc[1 + n++] = (int32_t)(((unsigned)b << 16) | m);
+        k = k1;
+      }
+      rc[0] = n;
+    }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < MFP_MAXP; q++) {
       const int p = wave + MF_WAVES * q;
@@ -2348,31 +2369,17 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
         const int ko = 8 * (lane >> 5);
         const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * MF_RS + ko;
         const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * MF_RS + ko;
-        const int16_t *bc = bins + c * MF_KS;
-        const int b_first = bc[0];
-        bool one = true;
-        for (int k = 1; k < MF_KS; k++) one = one && bc[k] == b_first;
-        if (one && b_first == cur[q]) {
-          mf_mma<PREC>(acc[q], ylds, oa, ob, plane_elems, 0xffu);
-        } else {
-          // runs of equal bin, in segment order
-          int k = 0;
-          while (k < MF_KS) {
-            const int b = bc[k];
-            int k1 = k + 1;
-            while (k1 < MF_KS && bc[k1] == b) k1++;
-            if (b >= 0) {
-              if (b != cur[q]) {
-                if (cur[q] >= 0 && ch0 + c < n_ch) flush(q, c, t);
-                cur[q] = b;
-              }
-              unsigned keep = 0;
-#pragma unroll
-              for (int e = 0; e < 8; e++) keep |= (unsigned)(ko + e >= k && ko + e < k1) << e;
-              mf_mma<PREC>(acc[q], ylds, oa, ob, plane_elems, keep);
-            }
-            k = k1;
+        const int32_t *rc = runs + c * (MF_KS + 1);
+        const int n = rc[0];
+        for (int r = 0; r < n; r++) {
+          const int32_t e = rc[1 + r];
+          const int b = e >> 16;
+          if (b != cur[q]) {
+            if (cur[q] >= 0 && ch0 + c < n_ch) flush(q, c, t);
+            cur[q] = b;
           }
+          // this lane half's eight segments of the run
+          mf_mma<PREC>(acc[q], ylds, oa, ob, plane_elems, ((unsigned)e >> ko) & 0xffu);
         }
       }
     }
@@ -3024,7 +3031,8 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const void *spectra, bool hal
                                     const Tune &t, hipStream_t s) {
   if (n_segments <= 0 || n_ch_run <= 0) return hipSuccess;
   size_t lds = 0;
-  const int CH = mf_channels(precision, M, MFP_MAXP, (size_t)64 * MF_KS * sizeof(int16_t), &lds);
+  const int CH = mf_channels(precision, M, MFP_MAXP,
+                            (size_t)64 * MF_KS * sizeof(int16_t) + (size_t)64 * (MF_KS + 1) * sizeof(int32_t), &lds);
   if (CH < 1) return hipErrorInvalidValue;
   const int RT = (2 * M + 31) / 32;
   // about mfp_wgs (1024) workgroups: channel blocks x segment ranges (multiples of a stage)
