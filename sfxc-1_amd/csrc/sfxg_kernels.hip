@@ -2286,6 +2286,7 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
 // k_pulsar_gram_baselines_run then forms the baselines of every bin.
 // ---------------------------------------------------------------------------
 constexpr int MFP_MAXP = 4;  // (channel, tile) pairs per wave with pulsar binning
+constexpr int MFP_RUNS = 20;  // run-table entries per channel: count + up to MF_KS runs, padded to 16 bytes
 
 template <int PREC, bool HS>
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
@@ -2307,8 +2308,9 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
   const int plane_elems = CH * chan_stride;
   int16_t *bins = reinterpret_cast<int16_t *>(ylds + planes * plane_elems);  // [CH][MF_KS]
   // runs of equal bin per channel and stage: runs[c][0] = count, runs[c][1 + r]
-  // = bin << 16 | mask of the run's segments (bit k: segment s0 + k)
-  int32_t *runs = reinterpret_cast<int32_t *>(bins + 64 * MF_KS);  // [CH][MF_KS + 1]
+  // = bin << 16 | mask of the run's segments (bit k: segment s0 + k); rows of
+  // MFP_RUNS entries, 16-byte aligned
+  int32_t *runs = reinterpret_cast<int32_t *>(bins + 64 * MF_KS);  // [CH][MFP_RUNS]
   for (int i = threadIdx.x; i < planes * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
   f16v_t acc[MFP_MAXP];
   int cur[MFP_MAXP];
@@ -2347,16 +2349,22 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
     // (channel, tile) pair reads its runs instead of scanning the bins
     if (threadIdx.x < CH) {
       const int16_t *bc = bins + threadIdx.x * MF_KS;
-      int32_t *rc = runs + threadIdx.x * (MF_KS + 1);
-      int n = 0;
-      for (int k = 0; k < MF_KS;) {
-        const int b = bc[k];
-        unsigned m = 0;
-        int k1 = k;
-        while (k1 < MF_KS && bc[k1] == b) m |= 1u << k1++;
-        if (b >= 0) rc[1 + n++] = (int32_t)(((unsigned)b << 16) | m);
-        k = k1;
+      int32_t *rc = runs + threadIdx.x * MFP_RUNS;
+      int bv[MF_KS];
+#pragma unroll
+      for (int k = 0; k < MF_KS; k++) bv[k] = bc[k];
+      int n = 0, b = bv[0];
+      unsigned m = 1u;
+#pragma unroll
+      for (int k = 1; k < MF_KS; k++) {
+        if (bv[k] != b) {
+          if (b >= 0) rc[1 + n++] = (int32_t)(((unsigned)b << 16) | m);
+          b = bv[k];
+          m = 0;
+        }
+        m |= 1u << k;
       }
+      if (b >= 0) rc[1 + n++] = (int32_t)(((unsigned)b << 16) | m);
       rc[0] = n;
     }
     __syncthreads();
@@ -2369,10 +2377,12 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
         const int ko = 8 * (lane >> 5);
         const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * MF_RS + ko;
         const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * MF_RS + ko;
-        const int32_t *rc = runs + c * (MF_KS + 1);
-        const int n = rc[0];
+        const int32_t *rc = runs + c * MFP_RUNS;
+        // the count and the first three runs in one read
+        const int4 h = *reinterpret_cast<const int4 *>(rc);
+        const int n = h.x;
         for (int r = 0; r < n; r++) {
-          const int32_t e = rc[1 + r];
+          const int32_t e = r == 0 ? h.y : r == 1 ? h.z : r == 2 ? h.w : rc[1 + r];
           const int b = e >> 16;
           if (b != cur[q]) {
             if (cur[q] >= 0 && ch0 + c < n_ch) flush(q, c, t);
@@ -3032,7 +3042,7 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const void *spectra, bool hal
   if (n_segments <= 0 || n_ch_run <= 0) return hipSuccess;
   size_t lds = 0;
   const int CH = mf_channels(precision, M, MFP_MAXP,
-                            (size_t)64 * MF_KS * sizeof(int16_t) + (size_t)64 * (MF_KS + 1) * sizeof(int32_t), &lds);
+                            (size_t)64 * MF_KS * sizeof(int16_t) + (size_t)64 * MFP_RUNS * sizeof(int32_t), &lds);
   if (CH < 1) return hipErrorInvalidValue;
   const int RT = (2 * M + 31) / 32;
   // about mfp_wgs (1024) workgroups: channel blocks x segment ranges (multiples of a stage)
