@@ -50,7 +50,7 @@ extern "C" {
  * (src/correlation_core.cc:241-273). The others run the X-step as a
  * per-channel Hermitian rank-k update on MFMA with FP32 accumulation:
  * FP16 or BF16 operands, or FP16 operands split into hi + lo parts (three
- * products). Not available with pulsar binning. */
+ * products). Pulsar binning takes every precision (per-bin Gram tiles on MFMA). */
 #define SFXG_PREC_FP32 0
 #define SFXG_PREC_FP16 1
 #define SFXG_PREC_BF16 2

@@ -72,6 +72,34 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def _workers() -> int:
+    """Threads the driver runs independent C calls on (streams, groups of
+    baselines): the host cores this process may use, at most 16. The C
+    calls release the GIL, and each computes what it would alone."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _map(fn, items):
+    items = list(items)
+    if len(items) <= 1 or _workers() == 1:
+        return [fn(x) for x in items]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=_workers()) as ex:
+        return list(ex.map(fn, items))
+
+
+def _baseline_groups(n_baselines: int):
+    """Contiguous groups of baselines for the threads (each baseline's sums
+    do not depend on the others')."""
+    k = max(1, min(_workers() * 2, n_baselines))
+    edges = np.linspace(0, n_baselines, k + 1).astype(int)
+    return [(a, b) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+
+
 _libs: Dict[str, C.CDLL] = {}
 
 
@@ -224,18 +252,39 @@ def _core_params(job, keep):
                       _ptr(f, C.c_int32), _ptr(s, C.c_int32)), bl
 
 
-def correlate(job, spectra: List[np.ndarray], seg_per_chunk, precision="f32"):
-    keep = []
+def _core_params_range(job, a, b, keep):
+    """CoreParams of baselines a .. b - 1 of the job's list."""
     cp, bl = _core_params(job, keep)
+    f = np.array([x[0] for x in bl[a:b]], dtype=np.int32)
+    s = np.array([x[1] for x in bl[a:b]], dtype=np.int32)
+    keep += [f, s]
+    cp.n_baselines = b - a
+    cp.bl_first = _ptr(f, C.c_int32)
+    cp.bl_second = _ptr(s, C.c_int32)
+    return cp, bl
+
+
+def correlate(job, spectra: List[np.ndarray], seg_per_chunk, precision="f32"):
+    """or_correlate over groups of baselines on the driver's threads."""
+    bl = job.baselines()
     nf = job.fft_size_correlation + 1
     arrs = [np.ascontiguousarray(s, dtype=_dtype(precision)) for s in spectra]
     ptrs = (C.POINTER(C.c_float if precision == "f32" else C.c_double) * len(arrs))(
         *[_rptr(a, precision) for a in arrs])
-    vis = np.zeros(2 * len(bl) * nf, dtype=_dtype(precision))
+    vis = np.zeros((len(bl), nf, 2), dtype=_dtype(precision))
     spc = np.ascontiguousarray(seg_per_chunk, dtype=np.int32)
-    getattr(lib(precision), f"or_correlate_{precision}")(C.byref(cp), ptrs, _ptr(spc, C.c_int32),
-                                                         C.c_int32(len(spc)), _rptr(vis, precision))
-    return vis.reshape(len(bl), nf, 2)
+    fn = getattr(lib(precision), f"or_correlate_{precision}")
+
+    def run(rng):
+        a, b = rng
+        keep = []
+        cp, _ = _core_params_range(job, a, b, keep)
+        part = np.zeros((b - a, nf, 2), dtype=_dtype(precision))
+        fn(C.byref(cp), ptrs, _ptr(spc, C.c_int32), C.c_int32(len(spc)), _rptr(part, precision))
+        vis[a:b] = part
+
+    _map(run, _baseline_groups(len(bl)))
+    return vis
 
 
 def phased_correlate(job, spectra: List[np.ndarray], nseg: int, precision="f32"):
@@ -372,18 +421,27 @@ def pulsar_bins(job, psr, current_fft, precision="f64"):
 
 
 def pulsar_correlate(job, psr, spectra, nseg, precision="f32"):
-    keep = []
-    cp, bl = _core_params(job, keep)
+    """or_pulsar_correlate over groups of baselines on the driver's threads."""
+    bl = job.baselines()
     offsets, out3 = pulsar_setup(job, psr, "f64")
-    pp = pulsar_params_c(psr, offsets, out3, keep)
     nf = job.fft_size_correlation + 1
     arrs = [np.ascontiguousarray(s, dtype=_dtype(precision)) for s in spectra]
     ptrs = (C.POINTER(C.c_float if precision == "f32" else C.c_double) * len(arrs))(
         *[_rptr(a, precision) for a in arrs])
-    acc = np.zeros(2 * (psr.nbins + 1) * len(bl) * nf, dtype=_dtype(precision))
-    getattr(lib(precision), f"or_pulsar_correlate_{precision}")(
-        C.byref(cp), C.byref(pp), ptrs, C.c_int64(nseg), _rptr(acc, precision))
-    return acc.reshape(psr.nbins + 1, len(bl), nf, 2)
+    acc = np.zeros((psr.nbins + 1, len(bl), nf, 2), dtype=_dtype(precision))
+    fn = getattr(lib(precision), f"or_pulsar_correlate_{precision}")
+
+    def run(rng):
+        a, b = rng
+        keep = []
+        cp, _ = _core_params_range(job, a, b, keep)
+        pp = pulsar_params_c(psr, offsets, out3, keep)
+        part = np.zeros((psr.nbins + 1, b - a, nf, 2), dtype=_dtype(precision))
+        fn(C.byref(cp), C.byref(pp), ptrs, C.c_int64(nseg), _rptr(part, precision))
+        acc[:, a:b] = part
+
+    _map(run, _baseline_groups(len(bl)))
+    return acc
 
 
 def stats_header_levels(bits, levels):
@@ -693,15 +751,17 @@ def run_slice(job, headers: Sequence[bytes], lut: int = 0, precision: str = "f32
     bytes and intermediates. One integration per slice (the reference asserts
     (stop - start) / integration_time == 1, correlator_node.cc:327-328)."""
     M = len(job.station_streams)
-    b2f = []
-    dc = []
-    for i in range(M):
+
+    def stream(i):
         r = bit2float(headers[i], b2f_params(job, i, lut), precision)
-        b2f.append(r)
         d = delay_correction(job, i, r["samples"], r["chunks"], precision, keep_intermediate)
-        dc.append(d)
         if not keep_intermediate:
             r["samples"] = None
+        return r, d
+
+    per = _map(stream, range(M))
+    b2f = [r for r, _ in per]
+    dc = [d for _, d in per]
     stats = []
     for i in range(M):
         st = job.station_streams[i]

@@ -415,6 +415,9 @@ int sfxg_set_parameters(sfxg_ctx *ctx, const sfxg_correlation_parameters *params
   if (pulsar) {
     if (pulsar->nbins <= 0 || pulsar->n_coef < 2 || !pulsar->coef)
       return ctx->fail(SFXG_ERR_ARG, "invalid pulsar parameters");
+    // the MFMA pulsar X-step keeps bin indices in 16 bits (bin 0 off-gate)
+    if (pulsar->nbins + 1 > 32767)
+      return ctx->fail(SFXG_ERR_ARG, "pulsar with %d bins: at most 32766 on-pulse bins", pulsar->nbins);
     ctx->psr = *pulsar;
     ctx->psr_coef.assign(pulsar->coef, pulsar->coef + pulsar->n_coef);
     ctx->psr.coef = ctx->psr_coef.data();
@@ -1037,7 +1040,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   const bool mixed = ctx->opt.precision != SFXG_PREC_FP32;
   if (!ctx->pulsar) HIPCHK(ctx, ctx->partial.ensure(part_one * k_group));
   // mixed precision with pulsar binning: binned Gram tiles for runs of
-  // channels, at most ~2 GiB at a time
+  // channels, at most ~8 GiB at a time
   int psr_run = 0;
   if (mixed && ctx->pulsar) {
     // all channels in one run up to 8 GiB; longer slices than one chunk need that

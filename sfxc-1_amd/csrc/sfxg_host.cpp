@@ -230,56 +230,67 @@ std::vector<int32_t> fold_points(const std::vector<int32_t> &seg_per_chunk, int 
   return folds;
 }
 
+namespace {
+// One stream's Invalid list as a queue of half-open runs [b, e) in job-rate
+// samples; (b, e) is the unconsumed rest of the run at the head.
+struct InvalidQueue {
+  const std::vector<std::pair<int32_t, int32_t>> &runs;
+  double scale;
+  size_t next = 0;
+  int64_t b = 0, e = 0;
+  bool live = false;
+  void pop() {
+    live = next < runs.size();
+    if (!live) return;
+    b = (int64_t)floor(runs[next].first / scale);
+    e = b + (int64_t)floor(runs[next].second / scale);
+    next++;
+  }
+};
+}  // namespace
+
+// Counting rule: a sample at which one stream of the baseline is invalid and
+// the other is not is flagged for the other stream. For Invalid lists whose
+// runs are disjoint and increasing, nflagged[0] = |mask_1 \ mask_0| and
+// nflagged[1] = |mask_0 \ mask_1| (a brute-force per-sample test pins this).
+// The lists are consumed as two queues: when the heads start together their
+// common part is skipped and the head that ends first (both, if they end
+// together) is used up; otherwise the head that starts first gives the
+// samples before the other head's start (all of them when it ends first) to
+// the other stream. A stream's own runs can overlap by the samples that a
+// pending sample removal drops at the start of an INVALID record (the run
+// keeps the record's length, bit2float_worker.cc:92,105-112); each run is then
+// compared only with the other queue's head, which is how
+// Correlation_core::find_invalid (correlation_core.cc:599-665) counts them.
 void find_invalid(const std::vector<std::pair<int32_t, int32_t>> &inv0,
                   const std::vector<std::pair<int32_t, int32_t>> &inv1, double scale0,
                   double scale1, int64_t nflagged_out[2]) {
-  const std::vector<std::pair<int32_t, int32_t>> *inv[2] = {&inv0, &inv1};
-  const double scale[2] = {scale0, scale1};
-  int index[2] = {0, 0};
-  int nflagged[2] = {0, 0};
-  int st[2] = {INT_MAX, INT_MAX}, en[2] = {INT_MAX, INT_MAX};
-  const int size[2] = {(int)inv0.size(), (int)inv1.size()};
-  for (int i = 0; i < 2; i++) {
-    if (size[i] > 0) {
-      st[i] = (int)floor((*inv[i])[0].first / scale[i]);
-      en[i] = st[i] + (int)floor((*inv[i])[index[i]].second / scale[i]);
-      index[i]++;
-    }
-  }
-  int i = st[0] < st[1] ? 0 : 1;
-  while (st[i] < INT_MAX) {
-    if (st[0] == st[1]) {
-      if (en[0] == en[1]) {
-        st[0] = en[0] + 1;
-        st[1] = en[1] + 1;
-      } else {
-        const int j = en[0] < en[1] ? 0 : 1;
-        st[1 - j] = en[j];
-        st[j] = en[j] + 1;
+  InvalidQueue q[2] = {{inv0, scale0}, {inv1, scale1}};
+  q[0].pop();
+  q[1].pop();
+  int64_t flagged[2] = {0, 0};
+  while (q[0].live || q[1].live) {
+    if (q[0].live && q[1].live && q[0].b == q[1].b) {
+      const int64_t end = std::min(q[0].e, q[1].e);
+      const bool used[2] = {q[0].e == end, q[1].e == end};
+      for (int k = 0; k < 2; k++) {
+        if (used[k]) q[k].pop();
+        else q[k].b = end;
       }
-    } else if (en[i] < st[1 - i]) {
-      nflagged[1 - i] += en[i] - st[i];
-      st[i] = en[i] + 1;
+      continue;
+    }
+    const int first = (!q[1].live || (q[0].live && q[0].b < q[1].b)) ? 0 : 1;
+    InvalidQueue &a = q[first], &o = q[1 - first];
+    if (!o.live || a.e < o.b) {
+      flagged[1 - first] += a.e - a.b;
+      a.pop();
     } else {
-      nflagged[1 - i] += st[1 - i] - st[i];
-      st[i] = st[1 - i];
+      flagged[1 - first] += o.b - a.b;
+      a.b = o.b;
     }
-    for (int j = 0; j < 2; j++) {
-      if (st[j] > en[j]) {
-        if (index[j] < size[j]) {
-          st[j] = (int)floor((*inv[j])[index[j]].first / scale[j]);
-          en[j] = st[j] + (int)floor((*inv[j])[index[j]].second / scale[j]);
-          index[j]++;
-        } else {
-          st[j] = INT_MAX;
-          en[j] = INT_MAX;
-        }
-      }
-    }
-    i = st[0] < st[1] ? 0 : 1;
   }
-  nflagged_out[0] += nflagged[0];
-  nflagged_out[1] += nflagged[1];
+  nflagged_out[0] += flagged[0];
+  nflagged_out[1] += flagged[1];
 }
 
 void bit_statistics(int bits, int64_t sample_rate, int64_t base_sample_rate, const int32_t *on,

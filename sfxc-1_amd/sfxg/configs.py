@@ -26,8 +26,8 @@ def build_slices(*, seed: int, n_stations: int, pols: Sequence[str], fs: int, bw
                  extra: Optional[Tuple[float, Sequence[float], Sequence[float]]] = None,
                  fft_size_delaycor: Optional[int] = None, sideband: str = "U",
                  invalid: Optional[dict] = None, n_integrations: Optional[int] = None,
-                 raw_out: Optional[dict] = None, margin_multiple: int = 1
-                 ) -> List[Tuple[CorrelationParameters, List[bytes]]]:
+                 raw_out: Optional[dict] = None, margin_multiple: int = 1, pulsar=None,
+                 whole_streams: bool = False) -> List[Tuple[CorrelationParameters, List[bytes]]]:
     """Synthesise the raw streams of one subband and cut them into slices.
 
     offsets/rates: per-station delay polynomial tau = offset + rate * t.
@@ -38,6 +38,10 @@ def build_slices(*, seed: int, n_stations: int, pols: Sequence[str], fs: int, bw
     and their first sample's time ("raw_start", ticks).
     margin_multiple: the raw streams start a multiple of this many samples
     before the first slice (frame-aligned raw data for VDIF frames).
+    pulsar: (Pulsar, peak, duty) injects a dispersed pulsed component
+    (synth.synth_station_streams).
+    whole_streams: synthesise every raw sample; otherwise only those the
+    slices read (the rest of each raw stream is zero bytes).
     """
     start = mjd_to_ticks(START_MJD, 0.0)
     tps = int(round(CLOCK_RATE / fs))
@@ -80,8 +84,21 @@ def build_slices(*, seed: int, n_stations: int, pols: Sequence[str], fs: int, bw
         m2 = [DelayModel.linear(raw_start, o + r * (raw_start - start) / CLOCK_RATE, r, seconds_model)
               for o, r in zip(o2, r2)]
         extra_arg = [(rho2, [m2[streams[i].station_number] for i in order])]
+    active = None
+    if not whole_streams and raw_out is None:
+        # samples the slices read: from the first slice's start to the last
+        # one's end, moved by each stream's integer delay over that time
+        t_end = start + n_int * t_int_ticks
+        tt = np.linspace(start, t_end, 64).astype(np.int64)
+        active = []
+        for i in order:
+            n = np.floor(models[i].eval(tt) * fs + 0.5)
+            a0 = (start - raw_start) // tps + int(n.min()) - 8192
+            a1 = (t_end - raw_start) // tps + int(n.max()) + 8192
+            active.append((a0, a1 - a0))
     raw_ordered = synth.synth_station_streams(seed, [len(stream_models_by_pol[p]) for p in pols], fs, n_raw,
-                                              [models[i] for i in order], raw_start, rho, bits, sky_freq, extra_arg)
+                                              [models[i] for i in order], raw_start, rho, bits, sky_freq, extra_arg,
+                                              active=active, pulsar=pulsar)
     raw = [None] * len(streams)
     for k, i in enumerate(order):
         raw[i] = raw_ordered[k]
@@ -261,3 +278,85 @@ def config1_subband(subband: int, seconds: float = 10.0, t_int_usec: int = 1_000
                         bw=32_000_000, nchan=nchan, seconds=seconds, t_int_usec=t_int_usec, rho=0.02,
                         bits=2, sky_freq=6_000_000_000 + 32_000_000 * subband, offsets=list(offsets),
                         rates=list(rates), subband=subband)
+
+
+def second_source(n_stations: int, seed: int = 2):
+    """Configuration 2's second point source: its own delay per station, the
+    first source's polynomial moved by offsets U[-50, 50] us and rates
+    U[-1e-7, 1e-7] s/s (drawn after the first source's delays from the same
+    seeded generator). Returns (rho2, offsets2, rates2)."""
+    rng = np.random.default_rng(seed)
+    offsets = rng.uniform(-5e-3, 5e-3, n_stations)
+    rates = rng.uniform(-3e-6, 3e-6, n_stations)
+    d_off = rng.uniform(-50e-6, 50e-6, n_stations)
+    d_rate = rng.uniform(-1e-7, 1e-7, n_stations)
+    return 0.005, list(offsets + d_off), list(rates + d_rate)
+
+
+def config2_slices(seconds: float = 0.0078125, n_stations: int = 32, nchan: int = 16384, subband: int = 0):
+    """Configuration 2 with sample data, one integration of `seconds`: 32
+    stations x 2 pols, 64 MHz at 128 Ms/s, 2-bit, 16384 channels, common
+    signal rho 0.01 plus the second point source (rho 0.005, second_source),
+    delays of config2_jobs (seed 2), sky 6.0 GHz + 64 MHz * subband."""
+    rng = np.random.default_rng(2)
+    offsets = rng.uniform(-5e-3, 5e-3, n_stations)
+    rates = rng.uniform(-3e-6, 3e-6, n_stations)
+    return build_slices(seed=2000 + subband, n_stations=n_stations, pols=["R", "L"], fs=128_000_000,
+                        bw=64_000_000, nchan=nchan, seconds=seconds, t_int_usec=int(round(seconds * 1e6)),
+                        rho=0.01, bits=2, sky_freq=6_000_000_000 + 64_000_000 * subband, offsets=list(offsets),
+                        rates=list(rates), subband=subband, extra=second_source(n_stations))
+
+
+def config3_slices(job_index: int, seconds: float = 0.03125, n_jobs: int = 48, seed: int = 3):
+    """Job `job_index` of configuration 3's draws (config3_jobs) with sample
+    data, one integration of `seconds` (its sub-integration too): channels,
+    stations, bit depth, bandwidth, rho and delays as drawn."""
+    rng = np.random.default_rng(seed)
+    for j in range(n_jobs):
+        nchan = int(rng.choice([256, 1024, 4096, 16384, 32768]))
+        n_st = int(rng.integers(4, 25))
+        bits = int(rng.choice([1, 2]))
+        bw = int(rng.choice([8, 16, 32, 64])) * 1_000_000
+        float(rng.choice([0.25, 0.5, 1.0, 2.0]))
+        rho = float(rng.uniform(0.005, 0.05))
+        offsets = rng.uniform(-5e-3, 5e-3, n_st)
+        rates = rng.uniform(-3e-6, 3e-6, n_st)
+        if j == job_index:
+            usec = int(round(seconds * 1e6))
+            return build_slices(seed=5000 + j, n_stations=n_st, pols=["R"], fs=2 * bw, bw=bw, nchan=nchan,
+                                seconds=seconds, t_int_usec=usec, sub_int_usec=usec, rho=rho, bits=bits,
+                                sky_freq=5_000_000_000 + 64_000_000 * j, offsets=list(offsets), rates=list(rates),
+                                subband=j)
+    raise ValueError(f"job {job_index} of {n_jobs}")
+
+
+def config4_slices(seconds: float = 0.0625, n_stations: int = 12, subband: int = 0, nbins: int = 64):
+    """Configuration 4 with sample data, one integration of `seconds`: 12
+    stations x 2 pols, 16 MHz at 32 Ms/s, 2-bit, 1024 channels, continuum rho
+    0.005, delays of config4_jobs (seed 4), and the injected pulsar: period
+    0.0893 s, Gaussian pulse of 5 % duty (full width at half maximum) with
+    peak power 0.2, dispersed with DM 68 across the band relative to its
+    observing frequency (1.4 GHz). The ephemeris (pulsar4 with the reference
+    phase moved so that the integration starts at pulse phase 0.2) bins 64
+    phase bins over the gate [0.25, 0.75), whose centre the pulse peaks at.
+    Returns (slices, pulsar)."""
+    import dataclasses
+    rng = np.random.default_rng(4)
+    offsets = rng.uniform(-5e-3, 5e-3, n_stations)
+    rates = rng.uniform(-3e-6, 3e-6, n_stations)
+    sky = 1_400_000_000 + 16_000_000 * subband
+    probe, _ = survey_jobs(seed=4, subband=subband, n_integrations=1, t_int_usec=int(round(seconds * 1e6)),
+                           nchan=1024, n_stations=1, fs=32_000_000, bw=16_000_000, sky=sky)
+    psr = dataclasses.replace(pulsar4(probe[0]), nbins=nbins)
+    # phase at the start with ref_phase 0, moved to 0.2
+    from .job import REFERENCE_MJD, SECONDS_PER_DAY
+    tmid_ticks = (psr.tmid - REFERENCE_MJD) * SECONDS_PER_DAY * CLOCK_RATE
+    p0 = psr.ref_freq * (probe[0].start_time - tmid_ticks) / CLOCK_RATE
+    psr = dataclasses.replace(psr, ref_phase=(0.2 - p0) % 1.0)
+    slices = build_slices(seed=4000 + subband, n_stations=n_stations, pols=["R", "L"], fs=32_000_000,
+                          bw=16_000_000, nchan=1024, seconds=seconds, t_int_usec=int(round(seconds * 1e6)),
+                          rho=0.005, bits=2, sky_freq=sky, offsets=list(offsets), rates=list(rates),
+                          subband=subband, pulsar=(psr, 0.2, 0.05))
+    for job, _ in slices:
+        job.pulsar_binning = True
+    return slices, psr

@@ -333,15 +333,61 @@ def pack(codes: np.ndarray, bits: int) -> np.ndarray:
     return out
 
 
+def pulsar_power(ticks, psr, peak: float, duty: float):
+    """Pulse power at geocentric time `ticks` (int64 array) as it arrives at
+    the pulsar's observing frequency: a Gaussian in pulse phase centred on
+    phase 0.5 with full width at half maximum `duty` (in periods) and height
+    `peak`. The phase is the linear ephemeris Correlation_core_pulsar
+    evaluates (get_phase, correlation_core_pulsar.h:40-44, with the start
+    phase of correlation_core_pulsar.cc:46-64): frac(ref_phase) + c0 +
+    (c1 + 60 F0) minutes since tmid; higher polyco terms must be zero."""
+    from .job import REFERENCE_MJD, SECONDS_PER_DAY
+    if any(c != 0.0 for c in psr.coef[2:]):
+        raise ValueError("pulse injection takes a linear ephemeris")
+    tmid_ticks = (psr.tmid - REFERENCE_MJD) * SECONDS_PER_DAY * CLOCK_RATE
+    minutes = (np.asarray(ticks, dtype=np.float64) - tmid_ticks) / CLOCK_RATE / 60.0
+    phase = (psr.ref_phase - math.floor(psr.ref_phase)) + psr.coef[0] + (psr.coef[1] + 60.0 * psr.ref_freq) * minutes
+    x = phase - 0.5
+    x = x - np.floor(x + 0.5)
+    sigma = duty / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+    return peak * np.exp(-0.5 * (x / sigma) ** 2)
+
+
+def disperse(a: np.ndarray, fs: int, lo_freq: float, dm: float, obs_freq_mhz: float) -> np.ndarray:
+    """Cold-plasma dispersion of an analytic baseband signal (upper sideband,
+    sky frequency lo_freq + f): the component at sky frequency f is delayed
+    by 4148.808 s DM ((f/MHz)^-2 - (obs/MHz)^-2), so a pulse keeps its
+    arrival time at the observing frequency. Applied as the all-pass filter
+    exp(-2 pi i Phi(f)) with dPhi/df equal to that delay, over one FFT of
+    the whole signal (the caller pads it by the largest delay)."""
+    import scipy.fft as sf
+    n = len(a)
+    f = sf.fftfreq(n, 1.0 / fs)
+    k = 4148.808e12 * dm  # s Hz^2
+    f0 = float(lo_freq)
+    fobs = obs_freq_mhz * 1e6
+    # Phi(f) = k (1/f0 - 1/(f0 + f) - f / fobs^2), written without cancellation
+    phi = k * (f / (f0 * (f0 + f)) - f / (fobs * fobs))
+    A = sf.fft(a, workers=-1)
+    A *= np.exp(-2j * np.pi * (phi - np.floor(phi)))
+    return sf.ifft(A, workers=-1, overwrite_x=True)
+
+
 def synth_station_streams(seed: int, n_streams_per_signal: List[int], fs: int, n_samples: int,
                           delays: List[DelayModel], raw_start_ticks: int, rho: float, bits: int,
-                          lo_freq: float, extra_sources=None) -> List[np.ndarray]:
+                          lo_freq: float, extra_sources=None, active=None, pulsar=None) -> List[np.ndarray]:
     """Raw packed byte streams, one per entry of `delays` (upper sideband,
     LO at lo_freq Hz).
 
     n_streams_per_signal groups consecutive streams sharing one common signal
     (e.g. one group per polarisation). extra_sources: list of (rho2, [DelayModel
     per stream]) adding further common components (one per group).
+    active: per stream (first, count), the raw samples the slices will read;
+    only those are synthesised (the rest of the stream is zero bytes).
+    pulsar: (Pulsar, peak, duty) adds a pulsed component per group seen
+    through each stream's own delay model: noise whose power follows
+    pulsar_power at the source, dispersed with the pulsar's DM relative to its
+    observing frequency; it adds to the power of the continuum and noise.
     """
     rng = np.random.default_rng(seed)
     tps = int(round(CLOCK_RATE / fs))
@@ -358,30 +404,62 @@ def synth_station_streams(seed: int, n_streams_per_signal: List[int], fs: int, n
         fr = lo_freq * tau
         return d_blk, fr - np.floor(fr)
 
+    if active is None:
+        active = [(0, n_samples)] * len(delays)
+    spb = 8 // bits
+    act = []
+    for a0, na in active:
+        a0 = max(0, (int(a0) // block) * block)
+        a1 = min(n_samples, -(-(int(a0) + int(na)) // block) * block + block)
+        act.append((a0, a1))
     per = [per_stream(m) for m in delays]
-    dmin = min(int(math.floor(d.min())) for d, _ in per)
-    dmax = max(int(math.ceil(d.max())) for d, _ in per)
     extra_per = [[per_stream(m) for m in models] for _, models in (extra_sources or [])]
-    for ep in extra_per:
-        dmin = min(dmin, min(int(math.floor(d.min())) for d, _ in ep))
-        dmax = max(dmax, max(int(math.ceil(d.max())) for d, _ in ep))
+    # the common-signal indices j = k - d(k) (plus the FIR's reach) that the
+    # active samples k of every stream and source need
+    jmin, jmax = None, None
+    for ds in [per] + extra_per:
+        for (d, _), (a0, a1) in zip(ds, act):
+            dd = d[a0 // block:a1 // block + 1]
+            lo = a0 - int(math.ceil(dd.max())) - pad
+            hi = a1 - int(math.floor(dd.min())) + pad
+            jmin = lo if jmin is None else min(jmin, lo)
+            jmax = hi if jmax is None else max(jmax, hi)
+    psr_pad = 0
+    if pulsar is not None:
+        psr, peak, duty = pulsar
+        fhi = (lo_freq + fs / 2) / 1e6
+        flo = lo_freq / 1e6
+        worst = 4148.808 * psr.DM * max(abs(flo ** -2 - psr.obs_freq ** -2), abs(fhi ** -2 - psr.obs_freq ** -2))
+        psr_pad = int(math.ceil(worst * fs)) + pad
     out = []
     idx = 0
     for group in n_streams_per_signal:
-        cstart = -dmax - pad
-        clen = n_samples + (dmax - dmin) + 2 * pad
+        cstart = jmin
+        clen = jmax - jmin
         common = analytic(rng.standard_normal(clen))
         extras = [(rho2, analytic(rng.standard_normal(clen)), k)
                   for k, (rho2, _) in enumerate(extra_sources or [])]
+        psig = None
+        if pulsar is not None:
+            n_p = clen + 2 * psr_pad
+            src_ticks = raw_start_ticks + (cstart - psr_pad + np.arange(n_p, dtype=np.int64)) * tps
+            psig = analytic(rng.standard_normal(n_p)) * np.sqrt(pulsar_power(src_ticks, psr, peak, duty))
+            psig = disperse(psig, fs, lo_freq, psr.DM, psr.obs_freq)[psr_pad:psr_pad + clen]
         for _ in range(group):
             d, fr = per[idx]
-            x = np.sqrt(rho) * delayed_signal(common, cstart, n_samples, d, fr)
+            a0, a1 = act[idx]
+            n_act = a1 - a0
+            x = np.sqrt(rho) * delayed_signal(common, cstart - a0, n_act, d[a0 // block:], fr[a0:a1])
+            if psig is not None:
+                x += delayed_signal(psig, cstart - a0, n_act, d[a0 // block:], fr[a0:a1])
             rest = 1.0 - rho
             for rho2, sig2, k in extras:
                 d2, fr2 = extra_per[k][idx]
-                x += np.sqrt(rho2) * delayed_signal(sig2, cstart, n_samples, d2, fr2)
+                x += np.sqrt(rho2) * delayed_signal(sig2, cstart - a0, n_act, d2[a0 // block:], fr2[a0:a1])
                 rest -= rho2
-            x += np.sqrt(rest) * rng.standard_normal(n_samples)
-            out.append(pack(quantise(x, bits), bits))
+            x += np.sqrt(rest) * rng.standard_normal(n_act)
+            raw = np.zeros(n_samples // spb, dtype=np.uint8)
+            raw[a0 // spb:a1 // spb] = pack(quantise(x, bits), bits)
+            out.append(raw)
             idx += 1
     return out

@@ -45,15 +45,18 @@ def vis_from_bytes(job, out: bytes, nbins=1):
     return heads, vis
 
 
-def compare_vis(v_gpu, v_ref, tol_norm=2e-5, tol_chan=1e-5, label=""):
-    """Per channel relative amplitude and phase errors (bar: 1e-5 and 1e-5
-    rad) over channels whose amplitude is at least half the product's rms;
-    the error of a channel is set by the accumulation's absolute rounding, so
-    on weak channels (a Rayleigh tail of noise-dominated visibilities) a
-    relative error or a phase grows without bound and means nothing. Also per product max |dV| / rms |V|: two
-    FP32 correlators whose FFTs round differently differ there by about
-    sqrt(2 n / 3) * 2^-24 for n segments per sub-integration (1e-5 at 7800
-    segments), whatever their summation order, hence its bar of 2e-5."""
+def compare_vis(v_gpu, v_ref, tol_norm=1e-5, tol_chan=1e-5, label=""):
+    """The plain FP32 bars, per product:
+    * every channel: max |V_gpu - V_ref| / rms |V_ref| <= 1e-5 (normwise; the
+      error of a channel is the accumulations' absolute rounding, so this
+      bounds it on every channel);
+    * channels with |V_ref| >= 0.5 rms |V_ref|: relative amplitude error and
+      phase error (rad) <= 1e-5. Below half the rms the absolute bound above
+      allows a relative error beyond 1e-5: those channels are the Rayleigh
+      tail of noise-dominated visibilities, where amplitude and phase are
+      undefined to that precision for any two FP32 correlators. The fraction
+      of channels under the relative check is printed."""
+    checked = []
     worst = (0.0, 0.0, 0.0)
     for b in range(v_ref.shape[0]):
         r = v_ref[b].astype(np.complex128)
@@ -61,10 +64,12 @@ def compare_vis(v_gpu, v_ref, tol_norm=2e-5, tol_chan=1e-5, label=""):
         rms = np.sqrt(np.mean(np.abs(r) ** 2))
         e_norm = np.max(np.abs(g - r)) / rms
         m = np.abs(r) >= 0.5 * rms
+        checked.append(np.mean(m))
         e_amp = np.max(np.abs(np.abs(g[m]) - np.abs(r[m])) / np.abs(r[m]))
         e_ph = np.max(np.abs(np.angle(g[m] * np.conj(r[m]))))
         worst = tuple(max(a, c) for a, c in zip(worst, (e_norm, e_amp, e_ph)))
-    print(f"{label} normwise {worst[0]:.3e} amp {worst[1]:.3e} phase {worst[2]:.3e} rad")
+    print(f"{label} normwise {worst[0]:.3e} amp {worst[1]:.3e} phase {worst[2]:.3e} rad "
+          f"(relative check on {100 * np.mean(checked):.1f} % of channels)")
     assert worst[0] <= tol_norm, worst
     assert worst[1] <= tol_chan, worst
     assert worst[2] <= tol_chan, worst

@@ -189,25 +189,79 @@ def test_segments_and_folds(pc):
     assert list(folds[:nf]) == exp and exp[-1] == job.number_ffts_in_integration()
 
 
-@pytest.mark.parametrize("seed", range(20))
-def test_find_invalid_matches_oracle(pc, seed):
-    rng = np.random.default_rng(seed)
-    lists = []
-    for _ in range(2):
-        k = int(rng.integers(0, 6))
-        st = np.sort(rng.choice(np.arange(0, 2000, 10), k, replace=False)).astype(np.int32)
-        nn = rng.integers(1, 9, k).astype(np.int32)
-        lists.append((st, nn))
+def _invalid_runs(rng, own_overlaps):
+    """A stream's Invalid list: runs in increasing order, adjacent or apart;
+    with own_overlaps, a run may start up to 3 samples before the previous
+    run's end (the samples a pending sample removal drops at the start of an
+    INVALID record, which keeps its announced length)."""
+    k = int(rng.integers(0, 7))
+    pos = int(rng.integers(0, 30))
+    st, nn = [], []
+    for _ in range(k):
+        n = int(rng.integers(1, 14))
+        st.append(pos)
+        nn.append(n)
+        if own_overlaps and rng.random() < 0.4:
+            pos += n - min(n, int(rng.integers(1, 4)))
+        else:
+            pos += n + int(rng.integers(0, 12))
+    return np.array(st, dtype=np.int32), np.array(nn, dtype=np.int32)
+
+
+def _flagged(fn, a, b):
     out = np.zeros(2, dtype=np.int64)
-    pc.pc_find_invalid(ptr(lists[0][0], C.c_int32), ptr(lists[0][1], C.c_int32), len(lists[0][0]),
-                       ptr(lists[1][0], C.c_int32), ptr(lists[1][1], C.c_int32), len(lists[1][0]),
-                       C.c_double(1.0), C.c_double(1.0), ptr(out, C.c_int64))
-    acc = np.zeros(2, dtype=np.int64)
-    f = getattr(O.lib("f32"), "or_find_invalid_f32")
-    f(ptr(lists[0][0], C.c_int32), ptr(lists[0][1], C.c_int32), C.c_int32(len(lists[0][0])),
-      ptr(lists[1][0], C.c_int32), ptr(lists[1][1], C.c_int32), C.c_int32(len(lists[1][0])), C.c_double(1.0),
-      C.c_double(1.0), ptr(acc, C.c_int64))
-    np.testing.assert_array_equal(out, acc)
+    fn(ptr(a[0], C.c_int32), ptr(a[1], C.c_int32), C.c_int32(len(a[0])), ptr(b[0], C.c_int32),
+       ptr(b[1], C.c_int32), C.c_int32(len(b[0])), C.c_double(1.0), C.c_double(1.0), ptr(out, C.c_int64))
+    return out
+
+
+def _mask(runs, n=400):
+    m = np.zeros(n, dtype=bool)
+    for s, k in zip(*runs):
+        m[s:s + k] = True
+    return m
+
+
+def test_find_invalid_counts_samples_one_stream_lacks(pc):
+    """Per-sample brute force: with disjoint increasing runs (every list
+    bit2float builds without a sample removal pending at an INVALID record),
+    the samples flagged for stream 1 are those invalid in stream 0 and valid in
+    stream 1, and the other way round. Runs of the two streams start
+    together, end together, nest, touch and partly overlap. Both the library
+    and the oracle's restatement of Correlation_core::find_invalid are
+    checked against the masks."""
+    rng = np.random.default_rng(0)
+    oracle_fn = getattr(O.lib("f32"), "or_find_invalid_f32")
+    kinds = set()
+    for _ in range(3000):
+        a, b = _invalid_runs(rng, False), _invalid_runs(rng, False)
+        ma, mb = _mask(a), _mask(b)
+        expect = np.array([np.sum(mb & ~ma), np.sum(ma & ~mb)], dtype=np.int64)
+        np.testing.assert_array_equal(_flagged(pc.pc_find_invalid, a, b), expect)
+        np.testing.assert_array_equal(_flagged(oracle_fn, a, b), expect)
+        if np.any(ma & mb) and np.any(ma & ~mb) and np.any(mb & ~ma):
+            kinds.add("partial")
+        if len(a[0]) and len(b[0]) and a[0][0] == b[0][0]:
+            kinds.add("same start")
+    assert kinds == {"partial", "same start"}
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_find_invalid_own_overlaps_match_oracle(pc, seed):
+    """Runs of one stream that overlap by the samples a pending removal drops:
+    each run is compared only with the other stream's run at hand, so such
+    samples can count once per run. The library counts them as the oracle's
+    restatement of the reference's walk does."""
+    rng = np.random.default_rng(100 + seed)
+    oracle_fn = getattr(O.lib("f32"), "or_find_invalid_f32")
+    differs = 0
+    for _ in range(2000):
+        a, b = _invalid_runs(rng, True), _invalid_runs(rng, True)
+        got = _flagged(pc.pc_find_invalid, a, b)
+        np.testing.assert_array_equal(got, _flagged(oracle_fn, a, b))
+        ma, mb = _mask(a), _mask(b)
+        differs += int(got[1] != np.sum(ma & ~mb) or got[0] != np.sum(mb & ~ma))
+    assert differs > 0  # overlapping runs do leave the mask difference
 
 
 def test_library_exports_header_symbols():

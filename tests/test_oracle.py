@@ -145,6 +145,56 @@ def random_stream(rng, bits, n_out):
 
 
 @pytest.mark.parametrize("bits", [1, 2])
+@pytest.mark.parametrize("rate", [1.2e-4, -1.2e-4])
+def test_integer_shift_closed_form(bits, rate):
+    """The integer delay, independently of how the HEADER records express it:
+    the correlator's output sample k at time t_k = start + k / fs is the raw
+    sample recorded at t_k + tau(t_k) rounded to a sample, i.e. raw index
+    k0 + k + floor(tau(t_k) fs + 0.5), for a rising delay (a sample skipped at
+    each change) and a falling one (a zero inserted at each change). The
+    input node places a change at a byte boundary (input_node_data_writer.cc:
+    171-185) found from the change instant's distance to the slice start in
+    input bytes, which the earlier changes of the slice have moved by one
+    sample each (input_node_data_writer.cc:421-433): around change j the
+    samples within j + one byte of it are left out of the comparison, and
+    the count and places of the inserted zeros are checked there."""
+    fs, nsamp = 32_000_000, 1 << 18
+    tps = CLOCK_RATE // fs
+    spb = 8 // bits
+    rng = np.random.default_rng(11 + bits)
+    raw_start = mjd_to_ticks(60000, 0.0)
+    start = raw_start + 8192 * tps
+    model = DelayModel.linear(raw_start, 5.0e-6, rate, 2)
+    raw = rng.integers(0, 256, 2 * (nsamp // spb) + 16384, dtype=np.uint8)
+    hdr = synth.header_stream(raw, raw_start, model, start, nsamp, fs, bits)
+    delay0 = float(model.eval(start))
+    p = b2f_params(bits, nsamp // 256, nfft_max=64, fft_size=256, lut=1, rate=fs,
+                   delay_in_samples=int(delay0 * fs))
+    out = O.bit2float(hdr, p, "f64")["samples"]
+    levels = np.array([-3.3359, -1, 1, 3.3359] if bits == 2 else [-1.0, 1.0])
+    codes = (raw[:, None] >> (bits * np.arange(spb))[None, :]) & ((1 << bits) - 1)
+    decoded = levels[codes.reshape(-1)]
+    k = np.arange(nsamp)
+    n_k = np.floor(model.eval(start + k * tps) * fs + 0.5).astype(np.int64)
+    k0 = (start - raw_start) // tps
+    expect = decoded[k0 + k + n_k]
+    change = np.flatnonzero(np.diff(n_k)) + 1
+    assert len(change) >= 25
+    near = np.zeros(nsamp, dtype=bool)
+    for j, c in enumerate(change):
+        near[max(0, c - j - spb - 1):c + j + spb + 2] = True
+    np.testing.assert_array_equal(out[~near], expect[~near])
+    assert np.mean(near) < 0.01
+    # inserted zeros: one per falling change, inside that change's window; none when rising
+    zeros = np.flatnonzero(out == 0.0)
+    if rate < 0:
+        assert len(zeros) == len(change)
+        assert np.all(np.abs(zeros - change) <= np.arange(len(change)) + spb + 1)
+    else:
+        assert len(zeros) == 0
+
+
+@pytest.mark.parametrize("bits", [1, 2])
 @pytest.mark.parametrize("seed", range(6))
 def test_bit2float_random_streams(bits, seed):
     rng = np.random.default_rng(seed)
@@ -395,6 +445,79 @@ def test_find_invalid_disjoint_and_equal():
     assert list(out[b]) == [7, 25]
     out2 = O.find_invalid(job, [(np.array([10]), np.array([5])), none, (np.array([10]), np.array([5])), none])
     assert list(out2[b]) == [0, 0]
+
+
+# ---------------------------------------------------------------------------
+# injected pulsar: dispersion law, then the oracle's dedispersion
+# ---------------------------------------------------------------------------
+def test_synth_dispersion_law():
+    """The injector's dispersion against the cold-plasma law: a 0.2 ms pulse
+    of noise, dispersed with DM 68 across 1400-1416 MHz, arrives in each of
+    eight sub-bands at 4148.808 s DM ((f/MHz)^-2 - 1400^-2) after its time at
+    1400 MHz (lower frequencies later), to 40 us (the pulse's power centroid
+    per sub-band; the arrivals span 3.2 ms)."""
+    import scipy.fft as sf
+    fs, n = 32_000_000, 1 << 21
+    rng = np.random.default_rng(3)
+    t = np.arange(n) / fs
+    t0 = 0.03
+    a = synth.analytic(rng.standard_normal(n)) * np.exp(-0.5 * ((t - t0) / 2e-4) ** 2)
+    d = synth.disperse(a, fs, 1.4e9, 68.0, 1400.0)
+    D = sf.fft(d)
+    f = sf.fftfreq(n, 1 / fs)
+    for k in range(8):
+        lo, hi = 2e6 * k, 2e6 * (k + 1)
+        sub = sf.ifft(np.where((f >= lo) & (f < hi), D, 0))
+        p = np.abs(sub) ** 2
+        fc = 1400.0 + (lo + hi) / 2e6
+        expect = t0 + 4148.808 * 68.0 * (fc ** -2 - 1400.0 ** -2)
+        w = np.abs(t - expect) < 2e-3
+        arrival = np.sum(t[w] * p[w]) / np.sum(p[w])  # power centroid
+        assert abs(arrival - expect) < 4e-5, (k, arrival, expect)
+
+
+def test_dedispersion_aligns_injected_pulse():
+    """A pulse injected with DM 68 (lower frequencies later) lines up in the
+    same phase bins across the band once Correlation_core_pulsar
+    dedisperses it with the per-channel offsets 4148.808 DM (f^-2 -
+    f_obs^-2) F (correlation_core_pulsar.cc:80-93), at the bin of the pulse's
+    phase 0.5; dedispersed with the opposite sign (a planted error, given
+    as -DM to the oracle) its peak moves by about seven bins across the band.
+    Profile per quarter band: the mean real part of the cross-correlation
+    per bin; its centroid over the bins above the median."""
+    import dataclasses
+    from sfxg.job import REFERENCE_MJD, SECONDS_PER_DAY, Pulsar, ticks_to_mjd
+    seconds = 0.2
+    fs, sky = 32_000_000, 1_400_000_000
+    start = mjd_to_ticks(configs.START_MJD, 0.0)
+    psr = Pulsar(nbins=64, gate_start=0.25, gate_stop=0.75, tmid=ticks_to_mjd(start) - 0.001, DM=68.0,
+                 ref_phase=0.0, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+    tmid_ticks = (psr.tmid - REFERENCE_MJD) * SECONDS_PER_DAY * CLOCK_RATE
+    psr = dataclasses.replace(psr, ref_phase=(0.2 - psr.ref_freq * (start - tmid_ticks) / CLOCK_RATE) % 1.0)
+    job, hdrs = configs.build_slices(seed=44, n_stations=2, pols=["R"], fs=fs, bw=fs // 2, nchan=1024,
+                                     seconds=seconds, t_int_usec=int(seconds * 1e6), rho=0.005, bits=2,
+                                     sky_freq=sky, offsets=[0.0, 1.5e-6], rates=[0.0, 1e-7],
+                                     pulsar=(psr, 0.2, 0.05))[0]
+    nb = psr.nbins + 1
+    expect_bin = 1 + int((0.5 - psr.gate_start) * (nb - 1) / (psr.gate_stop - psr.gate_start))
+
+    def centroids(p):
+        r = O.run_slice(job, hdrs, lut=1, pulsar=p)
+        v = np.array([r["norm"][b][2, :, 0] for b in range(nb)])  # cross baseline, real part
+        out = []
+        for q in range(4):
+            prof = v[1:, 256 * q:256 * (q + 1)].mean(axis=1)
+            w = np.maximum(prof - np.median(prof), 0.0)
+            out.append(1 + np.sum(np.arange(nb - 1) * w) / np.sum(w))
+        return np.array(out), v
+
+    good, v = centroids(psr)
+    assert np.max(good) - np.min(good) < 1.0, good
+    assert np.all(np.abs(good - expect_bin) < 1.5), (good, expect_bin)
+    # the pulse is there: peak bins well above the off-gate bin 0
+    assert v[expect_bin, :].mean() > 5 * abs(v[0, :].mean()) + 0.02
+    bad, _ = centroids(dataclasses.replace(psr, DM=-psr.DM))
+    assert abs(bad[3] - bad[0]) > 4.0, bad
 
 
 # ---------------------------------------------------------------------------
