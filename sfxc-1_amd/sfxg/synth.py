@@ -396,9 +396,17 @@ def synth_station_streams(seed: int, n_streams_per_signal: List[int], fs: int, n
     tt_blk = raw_start_ticks + np.arange(0, n_samples + block, block) * tps
 
     def per_stream(m: DelayModel):
-        d_blk = m.eval(tt_blk) * fs
+        # The model gives the delay tau(t) at geocentric time t: the station
+        # records at t + tau(t) what the geocentre receives at t. The sample
+        # at station time t' therefore holds the signal of t' - tau(t_g) with
+        # t_g + tau(t_g) = t' (three fixed-point steps; the remainder is rate^3
+        # times the delay).
+        t_g = tt_blk
+        for _ in range(3):
+            t_g = tt_blk - np.round(m.eval(t_g) * CLOCK_RATE).astype(np.int64)
+        tau_blk = m.eval(t_g)
+        d_blk = tau_blk * fs
         # nu_LO * tau per sample, linear interpolation of tau inside a block
-        tau_blk = m.eval(tt_blk)
         k = np.arange(n_samples)
         tau = np.interp(k, np.arange(len(tau_blk)) * block, tau_blk)
         fr = lo_freq * tau
