@@ -1,15 +1,22 @@
 """Throughput of the correlator-node data path on one MI355X.
 
-Main workload (N=1, `value`): 8 stations x 2 circular polarisations, 8
-subbands of 32 MHz sampled at 64 Ms/s, 2-bit, 4096 channels, 1 s integrations,
-delay offsets U[-5, 5] ms and rates U[-3e-6, 3e-6], plain FP32 X-step. One
-step = one 1 s integration of all 8 subbands (8 slices x 16 streams x 64
-Msamples = 8.19e9 station-samples); --steps 10 covers the full 10 s job.
+Headline (`value`): the 32-station configuration -- 32 stations x 2 circular
+polarisations, 4 subbands of 64 MHz sampled at 128 Ms/s, 2-bit, 16384
+channels, 0.5 s integrations, 4 s of data, a common signal (rho 0.01) plus a
+second point source (rho 0.005) with its own delays, delay offsets U[-5, 5] ms
+and rates U[-3e-6, 3e-6] -- correlated by the batched mixed-precision form:
+each step's four subband slices run as one captured HIP graph, with the
+X-step on FP16 MFMA with FP32 accumulation. One step is one 0.5 s integration
+of the four subbands (4 slices x 64 streams x 64 Msamples = 1.64e10
+station-samples); steps cycle over the job's 8 integrations, so --steps 8
+times the whole 4 s job. The same line carries the plain FP32 GPU path on
+the same workload and the ratio of the two, split FP16 and BF16, each
+variant's visibility error on one slice against split FP16, the roofline of
+the dominant kernel, the X-step's MFMA fraction and the CPU baseline on a
+sample of the same workload.
 
-Also reported: the same workload with the mixed-precision (split-FP16 MFMA)
-X-step, and the 32-station workload (32 x 2 pols, 4 subbands of 64 MHz at 128
-Ms/s, 16384 channels, 0.5 s integrations) with the plain FP32 and the
-mixed-precision X-steps and their ratio.
+Extra keys: the 8-station configuration (plain FP32 and split FP16), raw VDIF
+ingest, the 48-job mixed batch and pulsar binning.
 
 The HEADER control records come from the input-node delay rule on the host;
 the 2-bit payload is synthesised straight into device memory, so the timed
@@ -18,6 +25,7 @@ region starts with the input resident in HBM. Prints one JSON line.
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -29,50 +37,47 @@ for _p in (ROOT, os.path.join(ROOT, "sfxc-1_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the main workload's dominant kernel, from the
-# FETCH_SIZE and WRITE_SIZE counters scaled by a 1 GiB copy kernel's readings
-# (profiles/fengine_xstep_plain_fp32_8st_8sb_4096ch.md); re-measured whenever
-# the kernel changes.
-MEASURED_TRAFFIC = {("fengine", 4096, 16): 9.07e9}
-# Wave-level VALU instructions per launch of the same kernel (SQ_INSTS_VALU,
-# same profile), against the wave64 FP32 issue rate: 256 CUs x 1 instruction
-# per cycle (a wave64 instruction occupies a 16-lane SIMD for 4 cycles, 4
-# SIMDs per CU) x 2.4 GHz.
-MEASURED_VALU_INSTS = {("fengine", 4096, 16): 2.776e9}
-VALU_ISSUE_PEAK = 256 * 2.4e9
 FP16_DENSE_PEAK_TFLOPS = 2500.0
+# HBM bytes per launch (one slice) of a kernel, from the FETCH_SIZE and
+# WRITE_SIZE counters scaled by a 1 GiB copy kernel's readings (the
+# summaries under profiles/ give the commands); re-measured whenever the
+# kernel changes. Key: (kernel, channels, streams, precision).
+MEASURED_TRAFFIC = {}
+# Wave-level VALU instructions per launch (SQ_INSTS_VALU, same profiles),
+# against the wave64 FP32 issue rate: 256 CUs x 1 instruction per cycle (a
+# wave64 instruction occupies a 16-lane SIMD for 4 cycles, 4 SIMDs per CU)
+# x 2.4 GHz.
+MEASURED_VALU_INSTS = {}
+VALU_ISSUE_PEAK = 256 * 2.4e9
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "split_fp16": 3}
+C2_SUBBANDS, C2_INTEGRATIONS = 4, 8
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=8, help="0.5 s integrations of the 32-station job (8 = all 4 s)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--subbands", type=int, default=8)
+    ap.add_argument("--precisions", default="fp16,fp32,split_fp16,bf16",
+                    help="32-station legs; the first is the headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-usec", type=int, default=125000)
-    ap.add_argument("--contexts", type=int, default=2, help="contexts (HIP streams) slices alternate over")
-    ap.add_argument("--main-only", action="store_true", help="skip the mixed-precision and 32-station runs")
-    ap.add_argument("--c2-steps", type=int, default=2, help="0.5 s integrations of the 32-station workload")
-    ap.add_argument("--c2-precisions", default="fp32,split_fp16,fp16")
-    ap.add_argument("--c2-contexts", type=int, default=4,
-                    help="contexts (HIP streams) the 32-station slices alternate over (~18 GB of HBM each)")
+    ap.add_argument("--cpu-sample-usec", type=int, default=7812,
+                    help="length of each CPU-baseline worker's 32-station slice")
+    ap.add_argument("--cpu-cores", type=int, default=0, help="CPU baseline worker processes (default: up to 16)")
     ap.add_argument("--only", choices=("all", "c1", "c2", "c3", "c4"), default="all",
-                    help="c1: the main 8-station workload only; c2: the 32-station workload only; "
-                         "c3: the mixed batch only; c4: the pulsar-binning workload only")
+                    help="c2: the headline only; c1, c3, c4: that extra leg only (no headline)")
+    ap.add_argument("--no-extras", action="store_true", help="headline only (same as --only c2)")
+    ap.add_argument("--c1-steps", type=int, default=2, help="1 s integrations of the 8-station workload")
+    ap.add_argument("--c1-contexts", type=int, default=2)
     ap.add_argument("--c3-jobs", type=int, default=48, help="jobs of the mixed batch")
     ap.add_argument("--c3-precisions", default="fp32,fp16")
     ap.add_argument("--c4-steps", type=int, default=1, help="1 s integrations of the pulsar workload")
     ap.add_argument("--c4-precisions", default="fp32,fp16,split_fp16")
-    ap.add_argument("--cpu-cores", type=int, default=0, help="CPU baseline worker processes (default: up to 16)")
     ap.add_argument("--no-peaks", action="store_true", help="skip the achievable-peak microbenchmarks")
     ap.add_argument("--no-autotune", action="store_true",
                     help="default launch shapes instead of the autotuner's (run before each timed pass)")
-    ap.add_argument("--traffic-bytes", type=float, default=0.0,
-                    help="HBM bytes per launch of the dominant kernel from the calibrated FETCH_SIZE + WRITE_SIZE "
-                         "counters, reported as roofline.traffic")
+    ap.add_argument("--no-errors", action="store_true", help="skip the per-variant visibility errors")
     return ap.parse_args()
 
 
@@ -92,33 +97,35 @@ def kernel_bytes(job, M, nseg, n_samples, nfold, B, fused, spec_bytes=8):
     }
 
 
-def _cpu_slice_inputs(sb, sample_usec):
+def _c2_cpu_inputs(k, sample_usec):
+    """A slice of the 32-station workload (subband k % 4) shortened to
+    sample_usec, with random 2-bit payload behind the input-node records."""
     from sfxg import configs, synth
 
-    jobs, raw_start = configs.config1_jobs(sb, n_integrations=1, t_int_usec=sample_usec)
+    jobs, raw_start = configs.config2_jobs(k % C2_SUBBANDS, n_integrations=1, t_int_usec=sample_usec)
     job = jobs[0]
-    rng = np.random.default_rng(7 + sb)
+    rng = np.random.default_rng(70 + k)
     hdrs = []
     nsamp = job.n_ffts_per_integration(0) * job.fft_size_delaycor
     for i, st in enumerate(job.station_streams):
         ctrl, off, n = synth.header_stream(1 << 40, raw_start, job.delay_models[i], job.start_time, nsamp,
                                            st.sample_rate, st.bits_per_sample, elided=True)
-        payload = rng.integers(0, 256, n, dtype=np.uint8)
-        hdrs.append(synth.join_elided(ctrl, payload))
+        hdrs.append(synth.join_elided(ctrl, rng.integers(0, 256, n, dtype=np.uint8)))
     return job, hdrs, len(job.station_streams) * nsamp
 
 
-def _cpu_worker(sb, sample_usec, barrier, out):
-    """One correlator-node worker: builds its slice, waits for the others,
-    then times the FP32 oracle on it."""
+def _cpu_worker(k, sample_usec, barrier, out):
+    """One correlator-node worker on one core: builds its slice, waits for the
+    others, then times the FP32 oracle on it."""
     from oracle import oracle as O
 
-    job, hdrs, ss = _cpu_slice_inputs(sb, sample_usec)
+    O.set_threads(1)
+    job, hdrs, ss = _c2_cpu_inputs(k, sample_usec)
     O.lib("f32")
     barrier.wait()
     t0 = time.perf_counter()
     O.run_slice(job, hdrs, lut=0)
-    out.put((sb, t0, time.perf_counter(), ss))
+    out.put((k, t0, time.perf_counter(), ss))
 
 
 def _cpu_run(n, sample_usec):
@@ -126,7 +133,7 @@ def _cpu_run(n, sample_usec):
 
     ctx = mp.get_context("fork")
     barrier, out = ctx.Barrier(n), ctx.Queue()
-    procs = [ctx.Process(target=_cpu_worker, args=(k % 8, sample_usec, barrier, out)) for k in range(n)]
+    procs = [ctx.Process(target=_cpu_worker, args=(k, sample_usec, barrier, out)) for k in range(n)]
     for p in procs:
         p.start()
     res = [out.get() for _ in procs]
@@ -147,19 +154,31 @@ def cpu_cores():
     return max(1, min(16, n))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(sample_usec, cores):
     """The FP32 oracle (C restatement, -O2 -ffast-math) as the reference's
     correlator nodes run it: one worker process per host core, each on an
-    independent slice (subbands 0..7 of the main workload, shortened to
-    sample_usec), plus the same slice on one core alone. Runs before the GPU
-    is initialised (the workers are forked)."""
+    independent slice of the 32-station workload (subbands 0..3, shortened to
+    sample_usec, all 64 streams and 16384 channels), plus one such slice on
+    one core alone. Runs before the GPU is initialised (the workers are
+    forked)."""
     ss1, wall1 = _cpu_run(1, sample_usec)
     ssn, walln = _cpu_run(cores, sample_usec)
     return {"value": ssn / walln / 1e9, "unit": "station-Gsamples/s", "cores": cores, "kind": "port",
-            "single_core_value": ss1 / wall1 / 1e9,
+            "single_core_value": ss1 / wall1 / 1e9, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "fft_backend": "the oracle's own iterative radix-2 FFT (no FFTW in this image)",
             "sample": f"{cores} worker processes, each one {sample_usec / 1e6:g} s slice of one subband of the "
-                      f"main workload (16 streams, {ss1:.3g} station-samples per worker), FP32 oracle; "
-                      f"{walln:.1f} s wall for all, {wall1:.1f} s for one slice on one core"}
+                      f"32-station workload (64 streams, 16384 channels, {ss1:.3g} station-samples per worker), "
+                      f"FP32 oracle; {walln:.1f} s wall for all, {wall1:.1f} s for one slice on one core"}
 
 
 def achievable_peaks(torch, dev):
@@ -267,6 +286,131 @@ class Workload:
         for c in ctxs:
             c.close()
         return wall, kt
+
+
+class BatchedWorkload:
+    """The 32-station job in device memory (every subband and integration),
+    correlated step by step: a step's four subband slices are prepared on
+    one of two context sets (host work of the next step overlaps the graph
+    of the previous one) and run as one captured graph of four branches."""
+
+    def __init__(self, torch, capi, synth, configs, dev, n_int=C2_INTEGRATIONS, nsb=C2_SUBBANDS):
+        t0 = time.perf_counter()
+        self.per_sb = [configs.config2_jobs(sb, n_integrations=n_int) for sb in range(nsb)]
+        rho2, off2, rate2 = configs.second_source(32)
+        self.K, self.NSB = n_int, nsb
+        self.inputs = {}
+        for sb in range(nsb):
+            jobs, raw_start = self.per_sb[sb]
+            for k, job in enumerate(jobs):
+                nsamp = job.n_ffts_per_integration(0) * job.fft_size_delaycor
+                tsec = (job.start_time - jobs[0].start_time) / 512e6
+                lst = []
+                for i, st in enumerate(job.station_streams):
+                    ctrl, off, n = synth.header_stream(1 << 40, raw_start, job.delay_models[i], job.start_time,
+                                                       nsamp, st.sample_rate, st.bits_per_sample, elided=True)
+                    buf = torch.empty(n, dtype=torch.uint8, device=f"cuda:{dev}")
+                    n_int = int(synth.integer_delay(job.delay_models[i], job.start_time, st.sample_rate))
+                    s = st.station_number
+                    n2 = int(round((off2[s] + rate2[s] * tsec) * st.sample_rate))
+                    pol = st.polarisation == "L"
+                    capi.synthesize_raw(dev, buf.data_ptr(), n, st.bits_per_sample, off * 4, n_int,
+                                        common_seed=2000 + 2 * sb + pol, stream_seed=(2000 + sb * 64 + i) * 4096 + k,
+                                        rho=0.01, delay2_samples=n2, common2_seed=3000 + 2 * sb + pol, rho2=rho2)
+                    lst.append((ctrl, buf, n))
+                self.inputs[(k, sb)] = lst
+        torch.cuda.synchronize()
+        self.setup_s = time.perf_counter() - t0
+        self.job = self.per_sb[0][0][0]
+        self.M = len(self.job.station_streams)
+        self.nsamp = self.job.n_ffts_per_integration(0) * self.job.fft_size_delaycor
+        self.station_samples_per_step = nsb * self.M * self.nsamp
+
+    def _fill(self, ctx, k, sb):
+        ctx.set_parameters(self.per_sb[sb][0][k])
+        for i, (ctrl, buf, n) in enumerate(self.inputs[(k, sb)]):
+            ctx.push_stream_elided(i, ctrl, buf.data_ptr(), n)
+
+    def run(self, torch, capi, precision, K, W, dev, autotune=True):
+        sets = [[capi.Context(dev) for _ in range(self.NSB)] for _ in range(2)]
+        for cs in sets:
+            for c in cs:
+                c.set_options(lut=1, precision=precision)
+        batches = [capi.Batch(dev, n_branches=self.NSB) for _ in range(2)]
+        if autotune:
+            c = sets[0][0]
+            self._fill(c, 0, 0)
+            c.prepare()
+            self.tune_report = c.autotune(reps=2)
+
+        def step(r):
+            cs, b = sets[r % 2], batches[r % 2]
+            b.synchronize()
+            b.clear()
+            k = r % self.K
+            for sb, c in enumerate(cs):
+                self._fill(c, k, sb)
+                c.prepare()
+                b.add(c)
+            b.launch()
+
+        for r in range(W):
+            step(r)
+        for b in batches:
+            b.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(K):
+            step(W + r)
+        for b in batches:
+            b.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        for b in batches:
+            b.close()
+        # kernel durations without overlap: the four slices of one step again,
+        # eagerly on one context, each waited for
+        c = sets[0][0]
+        c.kernel_times(reset=True)
+        for sb in range(self.NSB):
+            self._fill(c, 0, sb)
+            c.correlate()
+            c.synchronize()
+        kt = c.kernel_times(reset=True)
+        # one slice's visibilities for the error report, and its read-back time
+        self._fill(c, 0, 0)
+        c.correlate()
+        c.synchronize()
+        t1 = time.perf_counter()
+        vis = c.stage(capi.STAGE_NORMALISED)
+        t2 = time.perf_counter()
+        out = c.output()
+        t3 = time.perf_counter()
+        self.readback = {"normalised_bytes": int(vis.nbytes), "stage_ms": round((t2 - t1) * 1e3, 2),
+                         "output_bytes": len(out), "output_ms": round((t3 - t2) * 1e3, 2),
+                         "output_GBs": round(len(out) / max(t3 - t2, 1e-9) / 1e9, 2)}
+        for cs in sets:
+            for c in cs:
+                c.close()
+        return wall, kt, vis
+
+
+def errors_vs(vis, ref):
+    """Per product (rows) of one slice: relative L2 error over channels and
+    the worst channel relative to the product's rms, against ref."""
+    l2, worst = [], []
+    for k in range(ref.shape[0]):
+        r = ref[k].astype(np.complex128)
+        g = vis[k].astype(np.complex128)
+        rms = np.sqrt(np.mean(np.abs(r) ** 2))
+        if rms == 0.0:
+            continue
+        d = np.abs(g - r)
+        l2.append(np.sqrt(np.mean(d ** 2)) / rms)
+        worst.append(np.max(d) / rms)
+    l2, worst = np.array(l2), np.array(worst)
+    return {"relative_l2_max": float(f"{l2.max():.3e}"), "relative_l2_median": float(f"{np.median(l2):.3e}"),
+            "worst_channel_max": float(f"{worst.max():.3e}"), "products": int(len(l2))}
 
 
 class MixedBatch:
@@ -423,23 +567,28 @@ def vdif_ingest(torch, capi, configs, dev, reps=3):
             "ingest_plus_correlate_ms": round(min(total) * 1e3, 2)}
 
 
-def summarise(wl, wall, kt, precision="fp32"):
+def summarise(wl, wall, kt, precision="fp32", steps=None):
+    """steps: steps in the timed region (default: the workload's own count)."""
+    steps = wl.K if steps is None else steps
     job = wl.job
     nseg = job.number_ffts_in_integration()
     B = len(job.baselines())
     nfold = max(1, round(job.integration_time / job.sub_integration_time))
     fused = kt[2] == 0.0
     kb = kernel_bytes(job, wl.M, nseg, wl.nsamp, nfold, B, fused, 4 if precision in ("fp16", "bf16") else 8)
-    value = wl.K * wl.station_samples_per_step / wall / 1e9
+    value = steps * wl.station_samples_per_step / wall / 1e9
     xflops = 8.0 * B * (job.fft_size_correlation + 1) * nseg
+    kt_timed = getattr(wl, "kt_timed", None)
     return {
         "value": round(value, 4),
-        "ms_per_step": round(wall / wl.K * 1e3, 3),
+        "ms_per_step": round(wall / steps * 1e3, 3),
         "kernel_ms_isolated": {n: round(float(a), 3) for n, a in zip(KERNELS, kt)},
-        "kernel_ms_timed_region": {n: round(float(a), 3) for n, a in zip(KERNELS, wl.kt_timed)},
+        "kernel_ms_timed_region": ({n: round(float(a), 3) for n, a in zip(KERNELS, kt_timed)}
+                                   if kt_timed is not None else None),
         "kernel_GBs": {n: round(kb[n] / (a * 1e-3) / 1e9, 1) for n, a in zip(KERNELS, kt) if a > 0},
         "xstep_TFLOPs": round(xflops / (kt[3] * 1e-3) / 1e12, 2) if kt[3] > 0 else None,
         "_kb": kb,
+        "_kt": kt,
     }
 
 
@@ -449,8 +598,11 @@ def main():
         print("bench.py measures one GPU: every box in this pool has one GPU and the path shards by "
               "(slice x subband) with no collectives; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    headline = args.only in ("all", "c2")
+    extras = set() if args.no_extras or args.only == "c2" else (
+        {"c1", "c3", "c4"} if args.only == "all" else {args.only})
     cpu = None
-    if not args.no_cpu_baseline and args.only in ("all", "c1"):
+    if headline and not args.no_cpu_baseline:
         # before the GPU is touched: the workers are forked
         cpu = cpu_baseline(args.cpu_sample_usec, args.cpu_cores or cpu_cores())
         print(f"CPU baseline done: {cpu['value']:.4f} station-Gsamples/s on {cpu['cores']} cores",
@@ -463,95 +615,117 @@ def main():
     peaks = None if args.no_peaks else achievable_peaks(torch, dev)
     if peaks:
         print(f"peaks: {peaks}", file=sys.stderr, flush=True)
-    K, W, NSB = args.steps, args.warmup, args.subbands
+    K, W = args.steps, args.warmup
     line = {}
-    if args.only in ("all", "c1"):
-        wl = Workload(torch, capi, synth, lambda sb, k: configs.config1_jobs(sb, n_integrations=k), K, NSB, dev,
-                      1000)
-        print(f"setup done: {len(wl.inputs)} slices in {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
-        wall, kt = wl.run(torch, capi, PREC["fp32"], W, args.contexts, dev, autotune=not args.no_autotune)
-        main_r = summarise(wl, wall, kt)
-        print(f"plain FP32 done: {main_r['value']:.2f} station-Gsamples/s", file=sys.stderr, flush=True)
+    if headline:
+        wl = BatchedWorkload(torch, capi, synth, configs, dev)
+        print(f"32-station setup done: {len(wl.inputs)} slices in {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
+        legs, vis = {}, {}
+        precs = args.precisions.split(",")
+        for name in precs:
+            wall, kt, v = wl.run(torch, capi, PREC[name], K, W, dev, autotune=not args.no_autotune)
+            r = summarise(wl, wall, kt, name, steps=K)
+            r["readback"] = wl.readback
+            legs[name] = r
+            vis[name] = v
+            print(f"32-station {name} done: {r['value']:.2f} station-Gsamples/s", file=sys.stderr, flush=True)
+        head = precs[0]
+        h = legs[head]
+        kt = h["_kt"]
         dom = int(np.argmax(kt))
         dom_name = KERNELS[dom]
-        achieved = main_r["_kb"][dom_name] / (kt[dom] * 1e-3) / 1e9
+        achieved = h["_kb"][dom_name] / (kt[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": main_r["_kb"][dom_name], "launch_ms": round(float(kt[dom]), 3)}
+                "bytes_per_launch": h["_kb"][dom_name], "launch": "one slice (64 streams x 0.5 s)",
+                "launch_ms": round(float(kt[dom]), 3)}
         if peaks:
             roof["frac_of_achievable"] = round(achieved / peaks["hbm_copy_GBs"], 4)
-        key = (dom_name, wl.job.fft_size_correlation, wl.M)
-        if args.traffic_bytes:
-            roof["traffic"] = args.traffic_bytes
-        elif key in MEASURED_TRAFFIC and NSB == 8:
+        key = (dom_name, 16384, wl.M, head)
+        if key in MEASURED_TRAFFIC:
             roof["traffic"] = MEASURED_TRAFFIC[key]
             roof["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"
-        if key in MEASURED_VALU_INSTS and NSB == 8:
+        if key in MEASURED_VALU_INSTS:
             rate = MEASURED_VALU_INSTS[key] / (kt[dom] * 1e-3)
             roof["valu"] = {"insts_per_launch": MEASURED_VALU_INSTS[key], "achieved_insts_per_s": round(rate, -8),
                             "peak_insts_per_s": VALU_ISSUE_PEAK, "frac": round(rate / VALU_ISSUE_PEAK, 4),
                             "source": "rocprofv3 SQ_INSTS_VALU"}
+        xs = {"kernel": "xstep", "TFLOPs": h["xstep_TFLOPs"], "mfma_frac": h.get("xstep_mfma_frac"),
+              "mfma_frac_of_achievable": h.get("xstep_mfma_frac_of_achievable"),
+              "GBs": h["kernel_GBs"].get("xstep"), "launch_ms": h["kernel_ms_isolated"]["xstep"]}
+        errs = None
+        if not args.no_errors and "split_fp16" in vis:
+            errs = {"reference": "split FP16 of the same slice (itself within 1e-5 relative L2 of the FP64 oracle "
+                                 "in the GPU tests); one slice: 2080 products x 16385 channels, 3905 segments"}
+            for name, v in vis.items():
+                if name != "split_fp16":
+                    errs[name] = errors_vs(v[0], vis["split_fp16"][0])
         line.update({
-            "metric": "station-Gsamples/s correlated (plain FP32 GPU path)",
-            "value": main_r["value"],
+            "metric": "station-Gsamples/s correlated (mixed vs FP32 GPU), X-step MFMA %, HBM GB/s",
+            "value": h["value"],
             "unit": "station-Gsamples/s",
             "n_gpus": 1,
             "steps": K,
             "warmup": W,
-            "ms_per_step": main_r["ms_per_step"],
+            "ms_per_step": h["ms_per_step"],
             "higher_is_better": True,
             "scaling": "weak",
             "vs_baseline": None,
-            "dtype": "fp32",
-            "data": "synthetic: seeded 2-bit station data generated in device memory, HEADER records from the "
-                    "input-node delay rule",
-            "config": {"workload": "8 stations x 2 pol, 8 subbands x 32 MHz at 64 Ms/s, 2-bit, 4096 channels, "
-                                   "1 s integrations, 10 s", "stations": 8, "polarisations": 2, "subbands": NSB,
-                       "sample_rate": 64_000_000, "channels": 4096, "integration_s": 1.0,
-                       "station_samples_per_step": wl.station_samples_per_step},
+            "dtype": head,
+            "dtype_detail": f"{head} X-step operands on MFMA with FP32 accumulation; FP32 F-engine; FP64 delay "
+                            "model and phase arguments",
+            "data": "synthetic: seeded 2-bit station data generated in device memory (common signal rho 0.01 and a "
+                    "second source rho 0.005 with its own delays), HEADER records from the input-node delay rule",
+            "config": {"workload": "32 stations x 2 pol, 4 subbands x 64 MHz at 128 Ms/s, 2-bit, 16384 channels, "
+                                   "0.5 s integrations, 4 s; each step's 4 subband slices as one captured graph",
+                       "stations": 32, "polarisations": 2, "subbands": wl.NSB, "sample_rate": 128_000_000,
+                       "channels": 16384, "integration_s": 0.5, "integrations_in_job": wl.K,
+                       "station_samples_per_step": wl.station_samples_per_step,
+                       "job_station_samples": wl.K * wl.station_samples_per_step},
             "roofline": roof,
-            "kernel_ms_isolated": main_r["kernel_ms_isolated"],
-            "kernel_ms_timed_region": main_r["kernel_ms_timed_region"],
-            "kernel_GBs": main_r["kernel_GBs"],
+            "xstep": xs,
+            "mixed_vs_plain": round(h["value"] / legs["fp32"]["value"], 3) if "fp32" in legs else None,
             "setup_s": round(wl.setup_s, 1),
         })
-        if not args.main_only:
-            wall, kt = wl.run(torch, capi, PREC["split_fp16"], W, args.contexts, dev, autotune=not args.no_autotune)
-            mixed = summarise(wl, wall, kt, "split_fp16")
-            mixed.pop("_kb")
-            add_mfma_frac(mixed, peaks)
-            line["mixed_split_fp16"] = mixed
-            print(f"mixed done: {mixed['value']:.2f}", file=sys.stderr, flush=True)
-        del wl
-        torch.cuda.empty_cache()
-        if not args.main_only:
-            line["vdif_ingest"] = vdif_ingest(torch, capi, configs, dev)
-            print(f"VDIF ingest done: {line['vdif_ingest']['ingest_ms']} ms per slice", file=sys.stderr, flush=True)
-            torch.cuda.empty_cache()
-    if args.only == "c2" or (args.only == "all" and not args.main_only):
-        w2 = Workload(torch, capi, synth, lambda sb, k: configs.config2_jobs(sb, n_integrations=k), args.c2_steps, 4,
-                      dev, 2000)
-        print(f"32-station setup done in {w2.setup_s:.1f} s", file=sys.stderr, flush=True)
-        c2 = {"workload": "32 stations x 2 pol, 4 subbands x 64 MHz at 128 Ms/s, 2-bit, 16384 channels, "
-                          f"0.5 s integrations, {args.c2_steps * 0.5:g} s",
-              "contexts": args.c2_contexts,
-              "station_samples_per_step": w2.station_samples_per_step}
-        for name in args.c2_precisions.split(","):
-            wall, kt = w2.run(torch, capi, PREC[name], 0, args.c2_contexts, dev, autotune=not args.no_autotune)
-            r = summarise(w2, wall, kt, name)
+        for name, r in legs.items():
             r.pop("_kb")
+            r.pop("_kt")
             if name != "fp32":
                 add_mfma_frac(r, peaks)
-            c2[name] = r
-            print(f"32-station {name} done: {r['value']:.2f}", file=sys.stderr, flush=True)
-        if "fp32" in c2:
-            for name, key in (("split_fp16", "mixed_vs_plain"), ("fp16", "fp16_vs_plain"), ("bf16", "bf16_vs_plain")):
-                if name in c2:
-                    c2[key] = round(c2[name]["value"] / c2["fp32"]["value"], 3)
-        line["config2"] = c2
-        del w2
+            if name != head:
+                line[name] = r
+                if "fp32" in legs and name != "fp32":
+                    r["vs_plain"] = round(r["value"] / legs["fp32"]["value"], 3)
+        line["kernel_ms_isolated"] = h["kernel_ms_isolated"]
+        line["kernel_GBs"] = h["kernel_GBs"]
+        line["readback"] = h["readback"]
+        if errs:
+            line["visibility_errors"] = errs
+        del wl, vis
         torch.cuda.empty_cache()
-    if args.only == "c3" or (args.only == "all" and not args.main_only):
+    if "c1" in extras:
+        wl = Workload(torch, capi, synth, lambda sb, k: configs.config1_jobs(sb, n_integrations=k), args.c1_steps, 8,
+                      dev, 1000)
+        print(f"8-station setup done: {len(wl.inputs)} slices in {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
+        c1 = {"workload": "8 stations x 2 pol, 8 subbands x 32 MHz at 64 Ms/s, 2-bit, 4096 channels, 1 s "
+                          f"integrations, {args.c1_steps} s", "station_samples_per_step": wl.station_samples_per_step}
+        for name in ("fp32", "split_fp16"):
+            wall, kt = wl.run(torch, capi, PREC[name], 1, args.c1_contexts, dev, autotune=not args.no_autotune)
+            r = summarise(wl, wall, kt, name)
+            r.pop("_kb")
+            r.pop("_kt")
+            if name != "fp32":
+                add_mfma_frac(r, peaks)
+            c1[name] = r
+            print(f"8-station {name} done: {r['value']:.2f}", file=sys.stderr, flush=True)
+        c1["mixed_vs_plain"] = round(c1["split_fp16"]["value"] / c1["fp32"]["value"], 3)
+        line["config1"] = c1
+        del wl
+        torch.cuda.empty_cache()
+        line["vdif_ingest"] = vdif_ingest(torch, capi, configs, dev)
+        print(f"VDIF ingest done: {line['vdif_ingest']['ingest_ms']} ms per slice", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    if "c3" in extras:
         mb = MixedBatch(torch, capi, synth, configs, dev, args.c3_jobs)
         print(f"mixed batch setup done in {mb.setup_s:.1f} s", file=sys.stderr, flush=True)
         from collections import Counter
@@ -571,18 +745,20 @@ def main():
         line["config3_batch"] = c3
         del mb
         torch.cuda.empty_cache()
-    if args.only == "c4" or (args.only == "all" and not args.main_only):
+    if "c4" in extras:
         w4 = Workload(torch, capi, synth, lambda sb, k: configs.config4_jobs(sb, n_integrations=k)[:2],
                       args.c4_steps, 8, dev, 4000, rho=0.005, pulsar_fn=configs.pulsar4)
         print(f"pulsar setup done in {w4.setup_s:.1f} s", file=sys.stderr, flush=True)
         c4 = {"workload": "12 stations x 2 pol, 8 subbands x 16 MHz at 32 Ms/s, 2-bit, 1024 channels, 1 s "
                           f"integrations, {args.c4_steps} s, pulsar binning: 64 bins over the gate [0.25, 0.75) "
-                          "+ off-gate bin, P = 0.0893 s, DM 68",
+                          "+ off-gate bin, P = 0.0893 s, DM 68 (device-synthesised continuum: the binning cost "
+                          "does not depend on the data)",
               "station_samples_per_step": w4.station_samples_per_step}
         for name in args.c4_precisions.split(","):
-            wall, kt = w4.run(torch, capi, PREC[name], 0, args.contexts, dev, autotune=not args.no_autotune)
+            wall, kt = w4.run(torch, capi, PREC[name], 0, 2, dev, autotune=not args.no_autotune)
             r = summarise(w4, wall, kt, name)
             r.pop("_kb")
+            r.pop("_kt")
             if name != "fp32":
                 add_mfma_frac(r, peaks)
             c4[name] = r
@@ -597,13 +773,13 @@ def main():
         line["achievable_peaks"] = peaks
     if not args.no_autotune:
         table = capi.tune_export().strip().split("\n")
-        print("tuning table:\n" + "\n".join(table), file=sys.stderr, flush=True)
-        main_keys = [capi.tune_key(4096, 16, 0, False, True), capi.tune_key(16384, 64, 0, False, False),
-                     capi.tune_key(16384, 64, 1, False, False), capi.tune_key(1024, 24, 1, True, True)]
+        main_keys = [capi.tune_key(16384, 64, p, False, False) for p in range(4)]
         line["autotuner"] = {"shapes_tuned": len(table),
-                             "main_shapes": [t for t in table if t.split("|")[0] in main_keys]}
+                             "headline_shapes": [t for t in table if t.split("|")[0] in main_keys]}
     if cpu:
         line["cpu_baseline"] = cpu
+        if "value" in line:
+            line["cpu_baseline"]["gpu_over_cpu"] = round(line["value"] / cpu["value"], 1)
     print(json.dumps(line))
 
 

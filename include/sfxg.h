@@ -385,11 +385,14 @@ void sfxg_tune_clear(void);
 /* Synthetic 2-bit/1-bit raw station data written straight into device
  * memory (for throughput measurements on device-resident input): n_bytes
  * bytes of stream `stream_seed` starting at raw sample `first_sample`,
- * x = sqrt(rho) * s[k - delay_samples] + sqrt(1 - rho) * n_stream[k], s shared
- * by all streams with the same common_seed. No reference counterpart. */
+ * x = sqrt(rho) s[k - delay_samples] + sqrt(rho2) s2[k - delay2_samples]
+ *     + sqrt(1 - rho - rho2) n_stream[k],
+ * s (s2) shared by all streams with the same common_seed (common2_seed);
+ * rho2 = 0 leaves the second component out. No reference counterpart. */
 int sfxg_synthesize_raw(int device, void *device_dst, size_t n_bytes, int32_t bits_per_sample,
                         int64_t first_sample, int64_t delay_samples, uint64_t common_seed,
-                        uint64_t stream_seed, float rho, float threshold);
+                        uint64_t stream_seed, float rho, float threshold, int64_t delay2_samples,
+                        uint64_t common2_seed, float rho2);
 
 #ifdef __cplusplus
 }

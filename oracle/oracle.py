@@ -72,10 +72,21 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+_threads: Optional[int] = None
+
+
+def set_threads(n: Optional[int]) -> None:
+    """Fixes the driver's thread count (None: the default of _workers)."""
+    global _threads
+    _threads = n
+
+
 def _workers() -> int:
     """Threads the driver runs independent C calls on (streams, groups of
     baselines): the host cores this process may use, at most 16. The C
     calls release the GIL, and each computes what it would alone."""
+    if _threads:
+        return _threads
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:

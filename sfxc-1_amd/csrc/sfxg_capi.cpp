@@ -1695,11 +1695,12 @@ int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n, int32_t reset) {
 
 int sfxg_synthesize_raw(int device, void *device_dst, size_t n_bytes, int32_t bits_per_sample, int64_t first_sample,
                         int64_t delay_samples, uint64_t common_seed, uint64_t stream_seed, float rho,
-                        float threshold) {
-  if (!device_dst || (bits_per_sample != 1 && bits_per_sample != 2) || rho < 0 || rho > 1) return SFXG_ERR_ARG;
+                        float threshold, int64_t delay2_samples, uint64_t common2_seed, float rho2) {
+  if (!device_dst || (bits_per_sample != 1 && bits_per_sample != 2) || rho < 0 || rho2 < 0 || rho + rho2 > 1)
+    return SFXG_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return SFXG_ERR_DEVICE;
   if (launch_synthesize(device_dst, n_bytes, bits_per_sample, first_sample, delay_samples, common_seed, stream_seed,
-                        rho, threshold, nullptr) != hipSuccess)
+                        rho, threshold, delay2_samples, common2_seed, rho2, nullptr) != hipSuccess)
     return SFXG_ERR_DEVICE;
   if (hipDeviceSynchronize() != hipSuccess) return SFXG_ERR_DEVICE;
   return SFXG_OK;

@@ -348,8 +348,8 @@ hipError_t launch_fold_combine_pc(const float2 *partial, int n_k, int n_baseline
 // the 2 nchan-point lag window for the delay-correction window `window`
 // applied over 2 fft_size points. Returns false for an unknown window.
 bool lag_window(int window, int nchan, int fft_size, std::vector<float> &w);
-hipError_t launch_synthesize(void *dst, size_t n_bytes, int bits, int64_t first_sample,
-                             int64_t delay_samples, uint64_t common_seed, uint64_t stream_seed,
-                             float rho, float threshold, hipStream_t s);
+hipError_t launch_synthesize(void *dst, size_t n_bytes, int bits, int64_t first_sample, int64_t delay_samples,
+                             uint64_t common_seed, uint64_t stream_seed, float rho, float threshold,
+                             int64_t delay2_samples, uint64_t common2_seed, float rho2, hipStream_t s);
 
 }  // namespace sfxg

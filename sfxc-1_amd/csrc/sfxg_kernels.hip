@@ -2512,15 +2512,17 @@ __device__ __forceinline__ float gauss(uint64_t seed, int64_t k) {
 __global__ __launch_bounds__(256) void k_synthesize(uint8_t *__restrict__ dst, int64_t n_bytes, int bits,
                                                     int64_t first_sample, int64_t delay_samples,
                                                     uint64_t common_seed, uint64_t stream_seed, float rho,
-                                                    float threshold) {
+                                                    float threshold, int64_t delay2_samples, uint64_t common2_seed,
+                                                    float rho2) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_bytes) return;
   const int spb = 8 / bits;
-  const float a = sqrtf(rho), b = sqrtf(1.0f - rho);
+  const float a = sqrtf(rho), a2 = sqrtf(rho2), b = sqrtf(1.0f - rho - rho2);
   uint32_t byte = 0;
   for (int t = 0; t < spb; t++) {
     const int64_t k = first_sample + i * spb + t;
-    const float x = a * gauss(common_seed, k - delay_samples) + b * gauss(stream_seed, k);
+    float x = a * gauss(common_seed, k - delay_samples) + b * gauss(stream_seed, k);
+    if (rho2 > 0.0f) x += a2 * gauss(common2_seed, k - delay2_samples);
     uint32_t c;
     if (bits == 2) c = x < -threshold ? 0u : (x < 0.0f ? 1u : (x < threshold ? 2u : 3u));
     else c = x >= 0.0f ? 1u : 0u;
@@ -3113,10 +3115,10 @@ hipError_t launch_normalise(const float2 *vis, const double *norms, const int32_
 
 hipError_t launch_synthesize(void *dst, size_t n_bytes, int bits, int64_t first_sample, int64_t delay_samples,
                              uint64_t common_seed, uint64_t stream_seed, float rho, float threshold,
-                             hipStream_t s) {
+                             int64_t delay2_samples, uint64_t common2_seed, float rho2, hipStream_t s) {
   dim3 grid((unsigned)((n_bytes + 255) / 256));
   hipLaunchKernelGGL(k_synthesize, grid, dim3(256), 0, s, (uint8_t *)dst, (int64_t)n_bytes, bits, first_sample,
-                     delay_samples, common_seed, stream_seed, rho, threshold);
+                     delay_samples, common_seed, stream_seed, rho, threshold, delay2_samples, common2_seed, rho2);
   return hipGetLastError();
 }
 

@@ -132,7 +132,8 @@ def lib() -> C.CDLL:
                                     C.POINTER(C.c_double), C.c_int32]
         L.sfxg_synthesize_raw.restype = C.c_int
         L.sfxg_synthesize_raw.argtypes = [C.c_int, C.c_void_p, C.c_size_t, C.c_int32, C.c_int64, C.c_int64,
-                                          C.c_uint64, C.c_uint64, C.c_float, C.c_float]
+                                          C.c_uint64, C.c_uint64, C.c_float, C.c_float, C.c_int64, C.c_uint64,
+                                          C.c_float]
         _lib = L
     return _lib
 
@@ -432,9 +433,10 @@ class Batch:
 
 
 def synthesize_raw(device: int, ptr: int, n_bytes: int, bits: int, first_sample: int, delay_samples: int,
-                   common_seed: int, stream_seed: int, rho: float, threshold: float = 0.9816):
+                   common_seed: int, stream_seed: int, rho: float, threshold: float = 0.9816,
+                   delay2_samples: int = 0, common2_seed: int = 0, rho2: float = 0.0):
     rc = lib().sfxg_synthesize_raw(device, C.c_void_p(ptr), n_bytes, bits, first_sample, delay_samples,
-                                   common_seed, stream_seed, rho, threshold)
+                                   common_seed, stream_seed, rho, threshold, delay2_samples, common2_seed, rho2)
     if rc != SFXG_OK:
         raise SfxgError(f"sfxg_synthesize_raw failed ({rc})")
 
