@@ -313,8 +313,10 @@ typedef struct {
   int *rev;   /* bit reversal permutation */
 } fft_plan;
 
-static fft_plan plans[64];
-static int n_plans = 0;
+/* Per-thread plan cache: the Python driver runs independent calls on
+ * several threads. */
+static _Thread_local fft_plan plans[64];
+static _Thread_local int n_plans = 0;
 
 static fft_plan *get_plan(int n, int sign) {
   for (int i = 0; i < n_plans; i++)
