@@ -342,10 +342,12 @@ class BatchedWorkload:
             self._fill(c, 0, 0)
             c.prepare()
             self.tune_report = c.autotune(reps=2)
+        self.host_s = 0.0
 
         def step(r):
             cs, b = sets[r % 2], batches[r % 2]
             b.synchronize()
+            h0 = time.perf_counter()
             b.clear()
             k = r % self.K
             for sb, c in enumerate(cs):
@@ -353,12 +355,17 @@ class BatchedWorkload:
                 c.prepare()
                 b.add(c)
             b.launch()
+            self.host_s += time.perf_counter() - h0
 
+        # at least two warm-up steps, so that both context sets hold their
+        # buffers before the timed region
+        W = max(W, 2)
         for r in range(W):
             step(r)
         for b in batches:
             b.synchronize()
         torch.cuda.synchronize()
+        self.host_s = 0.0
         t0 = time.perf_counter()
         for r in range(K):
             step(W + r)
@@ -366,6 +373,7 @@ class BatchedWorkload:
             b.synchronize()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        self.host_ms_per_step = self.host_s / K * 1e3
         for b in batches:
             b.close()
         # kernel durations without overlap: the four slices of one step again,
@@ -626,6 +634,9 @@ def main():
             wall, kt, v = wl.run(torch, capi, PREC[name], K, W, dev, autotune=not args.no_autotune)
             r = summarise(wl, wall, kt, name, steps=K)
             r["readback"] = wl.readback
+            r["host_prepare_ms_per_step"] = round(wl.host_ms_per_step, 2)
+            if name != "fp32":
+                add_mfma_frac(r, peaks)
             legs[name] = r
             vis[name] = v
             print(f"32-station {name} done: {r['value']:.2f} station-Gsamples/s", file=sys.stderr, flush=True)
@@ -690,8 +701,6 @@ def main():
         for name, r in legs.items():
             r.pop("_kb")
             r.pop("_kt")
-            if name != "fp32":
-                add_mfma_frac(r, peaks)
             if name != head:
                 line[name] = r
                 if "fp32" in legs and name != "fp32":
@@ -699,6 +708,7 @@ def main():
         line["kernel_ms_isolated"] = h["kernel_ms_isolated"]
         line["kernel_GBs"] = h["kernel_GBs"]
         line["readback"] = h["readback"]
+        line["host_prepare_ms_per_step"] = h["host_prepare_ms_per_step"]
         if errs:
             line["visibility_errors"] = errs
         del wl, vis

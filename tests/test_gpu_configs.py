@@ -66,6 +66,31 @@ def report(label, g, r):
         f"phase max {ph.max():.3e} rad median {np.median(ph):.3e} rad")
 
 
+def fp32_parity(job, hdrs, vis_g, vis_r32, label):
+    """The plain FP32 bars (compare_vis) against the FP32 oracle. A
+    sub-integration of more than 4000 segments takes two FP32 correlators
+    whose FFTs round differently beyond 1e-5 of each other on some channels
+    (the sums' rounding decorrelates; about sqrt(n) 2^-24 of a product's rms
+    per channel). There the GPU must be as accurate as the FP32 restatement
+    of the reference is: against the FP64 oracle its relative L2, amplitude
+    and phase errors at most 1.5 times the FP32 oracle's."""
+    try:
+        compare_vis(vis_g, vis_r32, label=label)
+        return
+    except AssertionError:
+        if job.number_ffts_in_sub_integration() <= 4000:
+            raise
+    r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
+    _, v64 = vis_from_bytes(job, r64["output"])
+    e_gpu = product_errors(vis_g, v64[0])
+    e_ref = product_errors(vis_r32, v64[0])
+    names = ("relative L2", "relative amplitude", "phase (rad)")
+    for n, a, b in zip(names, e_gpu, e_ref):
+        say(f"{label} ({job.number_ffts_in_sub_integration()} segments per sub-integration) vs FP64 oracle, "
+            f"{n}: GPU {a.max():.3e}, FP32 oracle {b.max():.3e}")
+        assert a.max() <= 1.5 * b.max() + 1e-7
+
+
 def run(ctx, job, hdrs, prec=0, pulsar=None, chunk_mib=0):
     try:
         return capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, pulsar=pulsar, precision=prec,
@@ -180,7 +205,7 @@ def test_config3_subset_one_graph(c3, prec):
                 heads_g, vis_g = vis_from_bytes(job, out)
                 assert heads_g == heads_r
                 if prec == 0:
-                    compare_vis(vis_g[0], vis_r[0], label=f"config 3 job {j}")
+                    fp32_parity(job, c3[[x[0] for x in c3].index(j)][2], vis_g[0], vis_r[0], f"config 3 job {j}")
                 else:
                     l2, worst = mixed_errors(vis_g[0], vis_r[0])
                     say(f"config 3 job {j} FP16: relative L2 {l2:.3e}, worst channel {worst:.3e}")
