@@ -233,6 +233,23 @@ int sfxg_vdif_phasecal(sfxg_ctx *ctx, int32_t stream_idx, const void *device_fra
                        const sfxg_vdif_stream *vdif, int64_t integration_ticks, uint8_t *dst, size_t cap,
                        size_t *written);
 
+/* sfxg_vdif_phasecal without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). The
+ * frames' header words, the frame table and the sums live in one static
+ * device buffer of SFXG_PHASECAL_SCRATCH_BYTES per GPU (held until the process
+ * ends) instead of buffers allocated per call; the output is identical.
+ * Limits: 16 bytes of header words per frame, and 16 bytes per valid frame
+ * plus 4 * P bytes per integration, each within the buffer; a larger call
+ * returns SFXG_ERR_ARG and says what it needed. Every call on a GPU shares
+ * that GPU's buffer: calls that overlap in time on one GPU (from several
+ * threads or contexts) race and must be serialised by the caller. Host-side
+ * bookkeeping (the frame walk) uses the C++ heap. */
+#define SFXG_PHASECAL_SCRATCH_BYTES (16u << 20)
+int sfxg_vdif_phasecal_static(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                              const sfxg_vdif_stream *vdif, int64_t integration_ticks, uint8_t *dst, size_t cap,
+                              size_t *written);
+
 /* The phase centres after the pointing centre of a multi-phase-centre slice
  * (params->n_phase_centers > 1): replaces the further sources of each
  * stream's Delay_table_akima (src/delay_table_akima.cc:114-136) and their

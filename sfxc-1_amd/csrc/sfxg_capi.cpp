@@ -562,11 +562,15 @@ static std::vector<std::pair<int64_t, int64_t>> input_node_delays(const ModelCop
 // The frames' header words lifted on the device, then VDIF_reader's walk
 // on the host: channel v->channel's frame blocks.
 static int vdif_channel_blocks(sfxg_ctx *ctx, int stream_idx, const void *device_frames, int64_t n_frames,
-                               const sfxg_vdif_stream *v, std::vector<VdifBlock> &blocks, int &frame_payload) {
+                               const sfxg_vdif_stream *v, std::vector<VdifBlock> &blocks, int &frame_payload,
+                               uint32_t *dev_words = nullptr) {
   std::vector<uint32_t> words((size_t)n_frames * 4);
-  HIPCHK(ctx, ctx->vdif_words.ensure(words.size() * sizeof(uint32_t)));
-  HIPCHK(ctx, launch_vdif_words(device_frames, n_frames, v->frame_bytes, ctx->vdif_words.as<uint32_t>(), ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(words.data(), ctx->vdif_words.p, words.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+  if (!dev_words) {
+    HIPCHK(ctx, ctx->vdif_words.ensure(words.size() * sizeof(uint32_t)));
+    dev_words = ctx->vdif_words.as<uint32_t>();
+  }
+  HIPCHK(ctx, launch_vdif_words(device_frames, n_frames, v->frame_bytes, dev_words, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(words.data(), dev_words, words.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
                              ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<int> threads(v->thread_ids, v->thread_ids + v->n_threads);
@@ -590,9 +594,13 @@ static int vdif_check(sfxg_ctx *ctx, int32_t stream_idx, const void *device_fram
   return SFXG_OK;
 }
 
-int sfxg_vdif_phasecal(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
-                       const sfxg_vdif_stream *v, int64_t integration_ticks, uint8_t *dst, size_t cap,
-                       size_t *written) {
+// do_phasecal over a stream's frames. scratch == nullptr: the device tables
+// and sums go to the context's word table and to buffers allocated for the
+// call; otherwise they go to `scratch` (scratch_bytes long), and nothing is
+// allocated on the device.
+static int vdif_phasecal_impl(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                              const sfxg_vdif_stream *v, int64_t integration_ticks, uint8_t *dst, size_t cap,
+                              size_t *written, uint8_t *scratch, size_t scratch_bytes) {
   if (!ctx || !device_frames || !v || (!dst && cap)) return SFXG_ERR_ARG;
   int rc = vdif_check(ctx, stream_idx, device_frames, n_bytes, v);
   if (rc != SFXG_OK) return rc;
@@ -603,7 +611,11 @@ int sfxg_vdif_phasecal(sfxg_ctx *ctx, int32_t stream_idx, const void *device_fra
   const int64_t n_frames = (int64_t)(n_bytes / (size_t)v->frame_bytes);
   std::vector<VdifBlock> blocks;
   int frame_payload = 0;
-  rc = vdif_channel_blocks(ctx, stream_idx, device_frames, n_frames, v, blocks, frame_payload);
+  if (scratch && (size_t)n_frames * 16 > scratch_bytes)
+    return ctx->fail(SFXG_ERR_ARG, "%lld frames: their header words need %lld bytes, the static scratch holds %zu",
+                     (long long)n_frames, (long long)n_frames * 16, scratch_bytes);
+  rc = vdif_channel_blocks(ctx, stream_idx, device_frames, n_frames, v, blocks, frame_payload,
+                           reinterpret_cast<uint32_t *>(scratch));
   if (rc != SFXG_OK) return rc;
   // do_phasecal's walk over the blocks (input_node_data_writer.cc:232-306):
   // an integration starts at a block whose start time is a multiple of the
@@ -634,18 +646,33 @@ int sfxg_vdif_phasecal(sfxg_ctx *ctx, int32_t stream_idx, const void *device_fra
   if (cap < need) return ctx->fail(SFXG_ERR_ARG, "phase-cal buffer holds %zu bytes, %zu needed", cap, need);
   if (closed.empty()) return SFXG_OK;
   const size_t n_sums = (size_t)(integ + 1) * P;
+  const size_t tab_bytes = std::max<size_t>(table.size(), 1) * sizeof(PcalFrame);
+  const size_t sums_at = (tab_bytes + 255) / 256 * 256;  // the header words are no longer needed
   DBuf sums, tab;
-  hipError_t e = sums.ensure(n_sums * sizeof(int32_t));
-  if (e == hipSuccess) e = tab.ensure(std::max<size_t>(table.size(), 1) * sizeof(PcalFrame));
-  if (e == hipSuccess) e = launch_fill_zero(sums.p, n_sums * sizeof(int32_t), ctx->stream);
+  void *tab_p, *sums_p;
+  hipError_t e = hipSuccess;
+  if (scratch) {
+    if (sums_at + n_sums * sizeof(int32_t) > scratch_bytes)
+      return ctx->fail(SFXG_ERR_ARG, "%zu frames in %zu integrations of %d bins need %zu bytes, the static scratch "
+                       "holds %zu", table.size(), (size_t)integ + 1, P, sums_at + n_sums * sizeof(int32_t),
+                       scratch_bytes);
+    tab_p = scratch;
+    sums_p = scratch + sums_at;
+  } else {
+    e = sums.ensure(n_sums * sizeof(int32_t));
+    if (e == hipSuccess) e = tab.ensure(tab_bytes);
+    tab_p = tab.p;
+    sums_p = sums.p;
+  }
+  if (e == hipSuccess) e = launch_fill_zero(sums_p, n_sums * sizeof(int32_t), ctx->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(tab.p, table.data(), table.size() * sizeof(PcalFrame), hipMemcpyHostToDevice, ctx->stream);
+    e = hipMemcpyAsync(tab_p, table.data(), table.size() * sizeof(PcalFrame), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
-    e = launch_phasecal(device_frames, tab.as<PcalFrame>(), (int)table.size(), S, P, st.bits_per_sample,
-                        sums.as<int32_t>(), ctx->stream);
+    e = launch_phasecal(device_frames, static_cast<const PcalFrame *>(tab_p), (int)table.size(), S, P,
+                        st.bits_per_sample, static_cast<int32_t *>(sums_p), ctx->stream);
   std::vector<int32_t> h(n_sums);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(h.data(), sums.p, n_sums * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+    e = hipMemcpyAsync(h.data(), sums_p, n_sums * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   sums.release();
   tab.release();
@@ -666,6 +693,57 @@ int sfxg_vdif_phasecal(sfxg_ctx *ctx, int32_t stream_idx, const void *device_fra
     o += rec;
   }
   return SFXG_OK;
+}
+
+int sfxg_vdif_phasecal(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                       const sfxg_vdif_stream *v, int64_t integration_ticks, uint8_t *dst, size_t cap,
+                       size_t *written) {
+  return vdif_phasecal_impl(ctx, stream_idx, device_frames, n_bytes, v, integration_ticks, dst, cap, written,
+                            nullptr, 0);
+}
+
+}  // extern "C"
+
+// Device scratch of sfxg_vdif_phasecal_static: one copy per GPU, held until
+// the process ends. Host code never passes `scratch` itself to a kernel or
+// a copy (in host code the name is a host-side stand-in): it uses the device
+// address hipGetSymbolAddress gives for the device that is current.
+static __device__ __align__(256) uint8_t scratch[SFXG_PHASECAL_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's `scratch`, looked up once per device
+// with that device current (the caller has made it current).
+int scratch_of(sfxg_ctx *ctx, uint8_t **out) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return ctx->fail(SFXG_ERR_ARG, "device %d", ctx->device);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[ctx->device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return ctx->fail(SFXG_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    scratch_ptr[ctx->device] = ptr;
+  }
+  *out = static_cast<uint8_t *>(scratch_ptr[ctx->device]);
+  return SFXG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sfxg_vdif_phasecal_static(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,
+                              const sfxg_vdif_stream *v, int64_t integration_ticks, uint8_t *dst, size_t cap,
+                              size_t *written) {
+  if (!ctx || !device_frames || !v || (!dst && cap)) return SFXG_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  uint8_t *buf = nullptr;
+  const int rc = scratch_of(ctx, &buf);
+  if (rc != SFXG_OK) return rc;
+  return vdif_phasecal_impl(ctx, stream_idx, device_frames, n_bytes, v, integration_ticks, dst, cap, written, buf,
+                            SFXG_PHASECAL_SCRATCH_BYTES);
 }
 
 int sfxg_push_vdif(sfxg_ctx *ctx, int32_t stream_idx, const void *device_frames, size_t n_bytes,

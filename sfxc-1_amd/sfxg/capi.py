@@ -122,6 +122,10 @@ def lib() -> C.CDLL:
         L.sfxg_vdif_phasecal.restype = C.c_int
         L.sfxg_vdif_phasecal.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t, C.POINTER(VdifStreamC),
                                          C.c_int64, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.sfxg_vdif_phasecal_static.restype = C.c_int
+        L.sfxg_vdif_phasecal_static.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t,
+                                                C.POINTER(VdifStreamC), C.c_int64, C.c_void_p, C.c_size_t,
+                                                C.POINTER(C.c_size_t)]
         L.sfxg_push_vdif.restype = C.c_int
         L.sfxg_push_vdif.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t, C.POINTER(VdifStreamC)]
         L.sfxg_set_phase_centers.restype = C.c_int
@@ -275,19 +279,21 @@ class Context:
                     f"sfxg_push_vdif({i})")
 
     def vdif_phasecal(self, i: int, device_ptr: int, nbytes: int, frame_bytes: int, channel: int, thread_ids,
-                      ref_mjd: int, integration_ticks: int, offset: int = 0) -> bytes:
-        """Phase-cal messages of stream i's VDIF frames (sfxg_vdif_phasecal)."""
+                      ref_mjd: int, integration_ticks: int, offset: int = 0, static: bool = False) -> bytes:
+        """Phase-cal messages of stream i's VDIF frames (sfxg_vdif_phasecal, or
+        sfxg_vdif_phasecal_static with static=True)."""
+        name = "sfxg_vdif_phasecal_static" if static else "sfxg_vdif_phasecal"
+        fn = getattr(lib(), name)
         th = np.ascontiguousarray(thread_ids, dtype=np.int32)
         v = VdifStreamC(frame_bytes, channel, len(th), ref_mjd, th.ctypes.data_as(C.POINTER(C.c_int32)), offset)
         need = C.c_size_t()
-        rc = lib().sfxg_vdif_phasecal(self._h, i, C.c_void_p(device_ptr), nbytes, C.byref(v), integration_ticks,
-                                      None, 0, C.byref(need))
+        rc = fn(self._h, i, C.c_void_p(device_ptr), nbytes, C.byref(v), integration_ticks, None, 0, C.byref(need))
         if rc != SFXG_OK and need.value == 0:
-            self._check(rc, "sfxg_vdif_phasecal")
+            self._check(rc, name)
         buf = (C.c_uint8 * max(1, need.value))()
         w = C.c_size_t()
-        self._check(lib().sfxg_vdif_phasecal(self._h, i, C.c_void_p(device_ptr), nbytes, C.byref(v),
-                                             integration_ticks, buf, need.value, C.byref(w)), "sfxg_vdif_phasecal")
+        self._check(fn(self._h, i, C.c_void_p(device_ptr), nbytes, C.byref(v), integration_ticks, buf, need.value,
+                       C.byref(w)), name)
         return bytes(buf)[:w.value]
 
     def push_stream_elided(self, i: int, control: bytes, device_ptr: int, nbytes: int):
