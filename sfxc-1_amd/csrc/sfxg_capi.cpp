@@ -1060,9 +1060,9 @@ static int prepare_slice(sfxg_ctx *ctx) {
   // FP16 and BF16 X-steps take their operands straight from spectra the
   // channeliser stores in that type (4 bytes per bin instead of 8); split
   // FP16 and FP32 need the FP32 spectra
-  const int spec_prec = ctx->opt.precision == SFXG_PREC_FP16 || ctx->opt.precision == SFXG_PREC_BF16
-                            ? ctx->opt.precision : 0;
-  const size_t spec_bytes = spec_prec ? 4 : sizeof(float2);
+  int spec_prec = ctx->opt.precision == SFXG_PREC_FP16 || ctx->opt.precision == SFXG_PREC_BF16
+                      ? ctx->opt.precision : 0;
+  const size_t spec_bytes = spec_prec ? 4 : sizeof(float2);  // split FP16 planes take 8, as FP32
   // chunks of whole sub-integrations whose spectra fit the budget (one
   // sub-integration at least)
   LaunchPlan &lp = ctx->lp;
@@ -1098,6 +1098,12 @@ static int prepare_slice(sfxg_ctx *ctx) {
       ctx->cev.push_back(e);
     }
   }
+  // split FP16 without pulsar binning: the channelisers store the hi and lo
+  // operand pairs (8 bytes per bin, as FP32), which the direct-to-LDS X-step
+  // reads without converting anything
+  if (ctx->opt.precision == SFXG_PREC_SPLIT_FP16 && !ctx->pulsar &&
+      xstep_split_planes_fit(M, lp.chunk_segs * ctx->S))
+    spec_prec = SFXG_PREC_SPLIT_FP16;
   if (!fused) HIPCHK(ctx, ctx->time_series.ensure((size_t)M * lp.ts_stride * sizeof(float)));
   if (channelise_scratch_elems(nc) > 0)
     HIPCHK(ctx, ctx->chan_scratch.ensure(channelise_scratch_elems(nc) * sizeof(float2)));
@@ -1712,12 +1718,14 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
                                 ctx->S * sizeof(float2), (ctx->nc + 1) * sizeof(float2), ctx->nseg,
                                 hipMemcpyDeviceToHost));
       } else {
-        // packed FP16 / BF16 pairs, returned as float2 with the scale undone
+        // packed FP16 / BF16 pairs (split FP16: hi and lo pairs, summed),
+        // returned as float2 with the scale undone
+        const int W = ctx->lp.job.spec_prec == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
         const size_t row = (size_t)(ctx->nc + 1);
-        std::vector<uint32_t> packed((size_t)ctx->nseg * row);
-        HIPCHK(ctx, hipMemcpy2D(packed.data(), row * 4,
-                                ctx->spectra.as<uint32_t>() + (int64_t)stream_idx * ctx->lp.chunk_segs * ctx->S,
-                                ctx->S * 4, row * 4, ctx->nseg, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> words((size_t)ctx->nseg * row * W);
+        HIPCHK(ctx, hipMemcpy2D(words.data(), row * 4 * W,
+                                ctx->spectra.as<uint32_t>() + (int64_t)stream_idx * ctx->lp.chunk_segs * ctx->S * W,
+                                ctx->S * 4 * W, row * 4 * W, ctx->nseg, hipMemcpyDeviceToHost));
         const bool bf = ctx->lp.job.spec_prec == SFXG_PREC_BF16;
         const float inv = 1.0f / ctx->lp.job.spec_scale;
         auto half_to_float = [bf](uint32_t h) {
@@ -1733,8 +1741,14 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
           return (float)x;
         };
         float2 *out = static_cast<float2 *>(dst);
-        for (size_t k = 0; k < packed.size(); k++)
-          out[k] = make_float2(half_to_float(packed[k] & 0xffffu) * inv, half_to_float(packed[k] >> 16) * inv);
+        for (size_t k = 0; k < words.size() / W; k++) {
+          float re = 0.0f, im = 0.0f;
+          for (int w = W - 1; w >= 0; w--) {  // lo, then hi: the pair's sum rounds once
+            re += half_to_float(words[W * k + w] & 0xffffu);
+            im += half_to_float(words[W * k + w] >> 16);
+          }
+          out[k] = make_float2(re * inv, im * inv);
+        }
       }
       break;
     case SFXG_STAGE_ACCUMULATED:

@@ -269,6 +269,9 @@ hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float2 *partial,
                              float2 *vis, hipStream_t s);
 // half_spectra: spectra packed in the operand type (FP16 / BF16 only).
+// Whether the direct-to-LDS X-step takes split FP16 hi / lo planes for M
+// streams with spectra rows stream_stride bins apart.
+bool xstep_split_planes_fit(int M, int64_t stream_stride);
 hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spectra, int64_t stream_stride,
                              int64_t seg_stride,
                              int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,

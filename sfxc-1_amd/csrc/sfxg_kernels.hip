@@ -88,25 +88,38 @@ __device__ __forceinline__ float2 wmul(float2 x, float2 w) {
 // SFXG_PREC_BF16 scales both parts by a power of two (exact) and rounds them
 // (to nearest even) to the MFMA X-step's operand type, packed in 32 bits
 // with the real part in the low half: the X-step then reads half the bytes
-// and converts nothing.
+// and converts nothing. SP = SFXG_PREC_SPLIT_FP16 writes the split operands
+// of the scaled value x: hi = fp16(x) and lo = fp16(x - hi), as two such
+// words per bin (hi pair, then lo pair), 8 bytes like a float2.
 template <int SP>
 struct SpecOut {
+  static constexpr uint32_t W = SP == SFXG_PREC_SPLIT_FP16 ? 2 : 1;  // 32-bit words per bin
   uint32_t *p;
   float scale;
   __device__ __forceinline__ SpecOut(float2 *base, int64_t off, float sc)
-      : p(reinterpret_cast<uint32_t *>(base) + off), scale(sc) {}
+      : p(reinterpret_cast<uint32_t *>(base) + W * off), scale(sc) {}
   __device__ __forceinline__ void put(uint32_t k, float2 v) const {
-    uint32_t re, im;
-    if constexpr (SP == SFXG_PREC_BF16) {
-      re = __builtin_bit_cast(uint16_t, (__bf16)(v.x * scale));
-      im = __builtin_bit_cast(uint16_t, (__bf16)(v.y * scale));
+    auto *q = (__attribute__((address_space(1))) char *)p + k * (4u * W);
+    if constexpr (SP == SFXG_PREC_SPLIT_FP16) {
+      const float xr = v.x * scale, xi = v.y * scale;
+      const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+      const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi);
+      const uint32_t h = (uint32_t)__builtin_bit_cast(uint16_t, hr) | ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+      const uint32_t l = (uint32_t)__builtin_bit_cast(uint16_t, lr) | ((uint32_t)__builtin_bit_cast(uint16_t, li) << 16);
+      typedef uint32_t u2_t __attribute__((ext_vector_type(2)));
+      *(__attribute__((address_space(1))) u2_t *)q = u2_t{h, l};
     } else {
-      static_assert(SP == SFXG_PREC_FP16, "FP16 or BF16 spectra");
-      re = __builtin_bit_cast(uint16_t, (_Float16)(v.x * scale));
-      im = __builtin_bit_cast(uint16_t, (_Float16)(v.y * scale));
+      uint32_t re, im;
+      if constexpr (SP == SFXG_PREC_BF16) {
+        re = __builtin_bit_cast(uint16_t, (__bf16)(v.x * scale));
+        im = __builtin_bit_cast(uint16_t, (__bf16)(v.y * scale));
+      } else {
+        static_assert(SP == SFXG_PREC_FP16, "FP16, BF16 or split FP16 spectra");
+        re = __builtin_bit_cast(uint16_t, (_Float16)(v.x * scale));
+        im = __builtin_bit_cast(uint16_t, (_Float16)(v.y * scale));
+      }
+      *(__attribute__((address_space(1))) uint32_t *)q = re | (im << 16);
     }
-    auto *q = (__attribute__((address_space(1))) char *)p + k * 4u;
-    *(__attribute__((address_space(1))) uint32_t *)q = re | (im << 16);
   }
 };
 template <>
@@ -1942,16 +1955,17 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
 constexpr int MFD_KS = 16;          // segments per stage
 constexpr int MFD_SLOTS = 3;        // ring slots
 
-// Bytes per raw block of CH-channel rows: 1 KiB of data, then 4 CH bytes of
-// skew so that consecutive blocks start CH banks apart.
-__host__ __device__ constexpr int mfd_blk(int CH) { return 1024 + (CH < 64 ? 4 * CH : 0); }
+// Bytes per raw block of rows of CW 32-bit words: 1 KiB of data, then 4 CW
+// bytes of skew so that consecutive blocks start CW banks apart.
+__host__ __device__ constexpr int mfd_blk(int CW) { return 1024 + (CW < 64 ? 4 * CW : 0); }
 // Block loads per stage for M streams.
-__host__ __device__ constexpr int mfd_blocks(int M, int CH) { return (M * MFD_KS * CH + 255) / 256; }
+__host__ __device__ constexpr int mfd_blocks(int M, int CW) { return (M * MFD_KS * CW + 255) / 256; }
 
-// LDS bytes of the DMA X-step: ring, operand stage, dummy block; the
-// epilogue's R x R FP32 buffer reuses the ring.
-__host__ __device__ constexpr size_t mfd_lds_bytes(int npw, int CH, int R) {
-  return (size_t)MFD_SLOTS * npw * MF_WAVES * mfd_blk(CH) + (size_t)CH * (R * (MFD_KS + 8) + 8) * 2 + 1024;
+// LDS bytes of the DMA X-step with rows of CW words (CW / W channels of W
+// operand planes each): ring, operand stage, dummy block; the epilogue's
+// R x R FP32 buffer reuses the ring.
+__host__ __device__ constexpr size_t mfd_lds_bytes(int npw, int CW, int R) {
+  return (size_t)MFD_SLOTS * npw * MF_WAVES * mfd_blk(CW) + (size_t)CW * (R * (MFD_KS + 8) + 8) * 2 + 1024;
 }
 
 // Workgroup barrier over LDS, with the waits written out: the compiler's
@@ -1963,17 +1977,24 @@ __device__ __forceinline__ void wait_vmcnt() {
   asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
-template <int PREC, int NPW, int CH>
+// PREC: FP16 or BF16 on packed pairs (one 32-bit word per bin), or split
+// FP16 on hi and lo pairs (two words per bin, SpecOut). CW: 32-bit words
+// per row of the workgroup's channels (CH = CW / W channels). Strides are in
+// bins.
+template <int PREC, int NPW, int CW>
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
     const uint32_t *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch,
     int M, int RT, int n_blk, int G, const int32_t *__restrict__ fold_start, const int32_t *__restrict__ bl_first,
     const int32_t *__restrict__ bl_second, int n_baselines, float inv_scale2, float2 *__restrict__ partial) {
-  static_assert(PREC == SFXG_PREC_FP16 || PREC == SFXG_PREC_BF16, "packed operand pairs");
-  static_assert(CH == 8 || CH == 16 || CH == 32 || CH == 64, "channels per workgroup");
+  static_assert(PREC == SFXG_PREC_FP16 || PREC == SFXG_PREC_BF16 || PREC == SFXG_PREC_SPLIT_FP16,
+                "packed operand pairs");
+  static_assert(CW == 8 || CW == 16 || CW == 32 || CW == 64, "words per row");
+  constexpr int W = PREC == SFXG_PREC_SPLIT_FP16 ? 2 : 1;  // words per bin, operand planes
+  constexpr int CH = CW / W;                 // channels per workgroup
   constexpr int KS = MFD_KS;
-  constexpr int CQ = CH / 4;                 // lanes per row
+  constexpr int CQ = CW / 4;                 // lanes per row
   constexpr int RPX = 64 / CQ;               // rows per block
-  constexpr int BLK = mfd_blk(CH);
+  constexpr int BLK = mfd_blk(CW);
   constexpr int SLOT = NPW * MF_WAVES * BLK;
   extern __shared__ __attribute__((aligned(16))) char dlds[];
   const int R = 32 * RT;
@@ -1986,8 +2007,9 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
   const int chan_stride = R * (KS + 8) + 8;  // halves
   char *ring = dlds;
   _Float16 *ylds = reinterpret_cast<_Float16 *>(dlds + MFD_SLOTS * SLOT);
-  char *dummy = dlds + MFD_SLOTS * SLOT + CH * chan_stride * 2;
-  const int NI = mfd_blocks(M, CH);             // real blocks per stage
+  const int plane_elems = CH * chan_stride;     // halves per operand plane
+  char *dummy = dlds + MFD_SLOTS * SLOT + W * plane_elems * 2;
+  const int NI = mfd_blocks(M, CW);             // real blocks per stage
   const bool regular = NI * RPX == M * KS;      // every block inside the M streams
   const bool all_ch = ch0 + CH <= n_ch;
   // lane part of a block load: row r = lane / CQ of the block (stream
@@ -1996,7 +2018,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
   // the four channels stay inside the row (bins past n_ch are masked later).
   const int lr = lane / CQ, lq = lane % CQ;
   const uint32_t lane_boff =
-      (uint32_t)(((lr / KS) * stream_stride + (lr % KS) * seg_stride + ch0 + 4 * lq) * 4);
+      (uint32_t)((((lr / KS) * stream_stride + (lr % KS) * seg_stride + ch0) * W + 4 * lq) * 4);
   // issue this wave's NPW block loads of the stage at s0 into ring slot sl:
   // block j = wave + 8 q holds rows j RPX .. j RPX + RPX - 1 of the stage;
   // a block past the stage's count reads the buffer's first KiB into the
@@ -2015,12 +2037,12 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
     const char *dsrc = reinterpret_cast<const char *>(spectra) + lane * 16;
     if (regular && s0 + KS <= s_end) {
       const int rj = wv * RPX;
-      const char *b0 = reinterpret_cast<const char *>(spectra + (rj / KS) * stream_stride +
-                                                      (row0 + rj % KS) * seg_stride);
+      const char *b0 = reinterpret_cast<const char *>(spectra + ((rj / KS) * stream_stride +
+                                                                 (row0 + rj % KS) * seg_stride) * W);
 #pragma unroll
       for (int q = 0; q < NPW; q++) {
         const int j = wv + MF_WAVES * q;
-        const char *bq = b0 + (int64_t)q * (RPX / 2) * stream_stride * 4;
+        const char *bq = b0 + (int64_t)q * (RPX / 2) * stream_stride * W * 4;
         const bool real = j < NI;
         dma(real ? bq + lane_boff : dsrc, real ? dst0 + j * BLK : dummy);
       }
@@ -2034,7 +2056,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
         int64_t row = row0 + rho % KS;
         row = row < last ? row : last;
         const bool real = j < NI;
-        dma(real ? spectra + st * stream_stride + row * seg_stride + ch0 + 4 * lq : (const uint32_t *)dsrc,
+        dma(real ? spectra + (st * stream_stride + row * seg_stride + ch0) * W + 4 * lq : (const uint32_t *)dsrc,
             real ? dst0 + j * BLK : dummy);
       }
     }
@@ -2057,34 +2079,38 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
       const int gm = jj * GPI + sub;
       if (gm >= 2 * M) continue;
       const char *src;
-      if constexpr (RPX >= 8) src = slot + jj * BLK + sub * 8 * (CH * 4) + c * 4;
-      else src = slot + gm * (8 / RPX) * BLK + c * 4;  // a group spans 8 / RPX blocks
-      uint32_t w[8];
-#pragma unroll
-      for (int kk = 0; kk < 8; kk++) {
-        if constexpr (RPX >= 8) w[kk] = *reinterpret_cast<const uint32_t *>(src + kk * (CH * 4));
-        else w[kk] = *reinterpret_cast<const uint32_t *>(src + (kk / RPX) * BLK + (kk % RPX) * (CH * 4));
-      }
+      if constexpr (RPX >= 8) src = slot + jj * BLK + sub * 8 * (CW * 4) + c * W * 4;
+      else src = slot + gm * (8 / RPX) * BLK + c * W * 4;  // a group spans 8 / RPX blocks
       const int st = gm >> 1, kg = gm & 1;
-      if (!full) {
-        const bool cok = ch0 + c < n_ch;
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++) w[kk] = cok && kg * 8 + kk < nv ? w[kk] : 0u;
-      }
-      typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
-      u4_t re, im;
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        re[e] = __builtin_amdgcn_perm(w[2 * e + 1], w[2 * e], 0x05040100u);  // low halves
-        im[e] = __builtin_amdgcn_perm(w[2 * e + 1], w[2 * e], 0x07060302u);  // high halves
-      }
       const int o_re = c * chan_stride + st * (KS + 8) + kg * 8;
-      *reinterpret_cast<u4_t *>(ylds + o_re) = re;
-      *reinterpret_cast<u4_t *>(ylds + o_re + M * (KS + 8)) = im;
+#pragma unroll
+      for (int pl = 0; pl < W; pl++) {  // hi (or the only) pair, then lo
+        uint32_t w[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+          if constexpr (RPX >= 8) w[kk] = *reinterpret_cast<const uint32_t *>(src + pl * 4 + kk * (CW * 4));
+          else
+            w[kk] = *reinterpret_cast<const uint32_t *>(src + pl * 4 + (kk / RPX) * BLK + (kk % RPX) * (CW * 4));
+        }
+        if (!full) {
+          const bool cok = ch0 + c < n_ch;
+#pragma unroll
+          for (int kk = 0; kk < 8; kk++) w[kk] = cok && kg * 8 + kk < nv ? w[kk] : 0u;
+        }
+        typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
+        u4_t re, im;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          re[e] = __builtin_amdgcn_perm(w[2 * e + 1], w[2 * e], 0x05040100u);  // low halves
+          im[e] = __builtin_amdgcn_perm(w[2 * e + 1], w[2 * e], 0x07060302u);  // high halves
+        }
+        *reinterpret_cast<u4_t *>(ylds + pl * plane_elems + o_re) = re;
+        *reinterpret_cast<u4_t *>(ylds + pl * plane_elems + o_re + M * (KS + 8)) = im;
+      }
     }
   };
   // padding rows (2M .. R) stay zero
-  for (int i = threadIdx.x; i < CH * chan_stride; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
+  for (int i = threadIdx.x; i < W * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
   f16v_t acc[MF_MAXP];
 #pragma unroll
   for (int q = 0; q < MF_MAXP; q++)
@@ -2125,7 +2151,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
     dma_barrier();
 #pragma unroll
     for (int q = 0; q < MF_MAXP; q++)
-      if (pa[q] >= 0) mf_mma<PREC>(acc[q], ylds, pa[q] + lane_ab, pb[q] + lane_ab, 0, 0xffu);
+      if (pa[q] >= 0) mf_mma<PREC>(acc[q], ylds, pa[q] + lane_ab, pb[q] + lane_ab, plane_elems, 0xffu);
   }
   __syncthreads();
   // baselines, one channel at a time through an R x R FP32 buffer in LDS
@@ -2657,6 +2683,8 @@ hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, cons
       return launch_channelise_sp<SFXG_PREC_FP16>(d_streams, job, window, tw_n, tw_2n, tw_split, scratch, t, s);
     case SFXG_PREC_BF16:
       return launch_channelise_sp<SFXG_PREC_BF16>(d_streams, job, window, tw_n, tw_2n, tw_split, scratch, t, s);
+    case SFXG_PREC_SPLIT_FP16:
+      return launch_channelise_sp<SFXG_PREC_SPLIT_FP16>(d_streams, job, window, tw_n, tw_2n, tw_split, scratch, t, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2732,6 +2760,8 @@ hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const f
     case 0: return launch_fengine_sp<0>(d_streams, job, window, tw_nd, tw_split, t, s);
     case SFXG_PREC_FP16: return launch_fengine_sp<SFXG_PREC_FP16>(d_streams, job, window, tw_nd, tw_split, t, s);
     case SFXG_PREC_BF16: return launch_fengine_sp<SFXG_PREC_BF16>(d_streams, job, window, tw_nd, tw_split, t, s);
+    case SFXG_PREC_SPLIT_FP16:
+      return launch_fengine_sp<SFXG_PREC_SPLIT_FP16>(d_streams, job, window, tw_nd, tw_split, t, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2835,6 +2865,15 @@ static hipError_t launch_mfma_ks(dim3 grid, size_t lds, hipStream_t s, const voi
   return hipGetLastError();
 }
 
+bool xstep_split_planes_fit(int M, int64_t stream_stride) {
+  const int RT = (2 * M + 31) / 32, R = 32 * RT, nT = RT * (RT + 1) / 2;
+  if (nT > MF_WAVES * MF_MAXP || (MF_WAVES * MF_MAXP) / nT < 4) return false;
+  if (stream_stride * 2 * M >= ((int64_t)1 << 32) || stream_stride * 2 * 8 >= ((int64_t)1 << 32)) return false;
+  const int n = (mfd_blocks(M, 8) + MF_WAVES - 1) / MF_WAVES;  // CW = 8: four channels
+  const int npw = n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : n <= 8 ? 8 : 0;
+  return npw > 0 && std::max(mfd_lds_bytes(npw, 8, R), (size_t)R * R * 4) <= 160 * 1024;
+}
+
 hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spectra, int64_t stream_stride, int64_t seg_stride,
                              int64_t seg_base, int n_ch, int M, const int32_t *fold_start, int n_k,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
@@ -2845,45 +2884,50 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
   const int nT = RT * (RT + 1) / 2;
   const int planes = precision == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
   if (nT > MF_WAVES * MF_MAXP) return hipErrorInvalidValue;
-  if (half_spectra && t.mf_ks != 32 && stream_stride * M < ((int64_t)1 << 32)) {
-    // direct-to-LDS form: CH a power of two from 8 to 64 whose (channel,
-    // tile) pairs fit the waves, NPW = 1, 2, 4 or 8 block loads per wave
-    // and stage, and the LDS budget
+  // split FP16 on hi / lo planes (two words per bin) has only the
+  // direct-to-LDS form
+  const bool split_planes = half_spectra && precision == SFXG_PREC_SPLIT_FP16;
+  const int W = split_planes ? 2 : 1;
+  if (half_spectra && (t.mf_ks != 32 || split_planes) && stream_stride * W * M < ((int64_t)1 << 32)) {
+    // direct-to-LDS form: CW = W CH words per row, a power of two from 8 to
+    // 64, whose (channel, tile) pairs fit the waves, NPW = 1, 2, 4 or 8 block
+    // loads per wave and stage, and the LDS budget
     const int max_pairs_ch = (MF_WAVES * MF_MAXP) / nT;
-    auto npw_of = [&](int ch) {
-      const int n = (mfd_blocks(M, ch) + MF_WAVES - 1) / MF_WAVES;
+    auto npw_of = [&](int cw) {
+      const int n = (mfd_blocks(M, cw) + MF_WAVES - 1) / MF_WAVES;
       return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : n <= 8 ? 8 : 0;
     };
-    auto lds_of = [&](int ch) { return std::max(mfd_lds_bytes(npw_of(ch), ch, R), (size_t)R * R * 4); };
-    int CH = 64;
-    while (CH >= 8 && (CH > max_pairs_ch || npw_of(CH) == 0 || lds_of(CH) > 160 * 1024 ||
-                       (t.mf_channels > 0 && CH > t.mf_channels)))
-      CH /= 2;
-    if (CH >= 8 && stream_stride * 8 < ((int64_t)1 << 32)) {
-      const int npw = npw_of(CH);
-      const size_t lds = lds_of(CH);
+    auto lds_of = [&](int cw) { return std::max(mfd_lds_bytes(npw_of(cw), cw, R), (size_t)R * R * 4); };
+    int CW = 64;
+    const int cap = t.mf_channels > 0 ? std::max(t.mf_channels, 8 / W) : 64;
+    while (CW >= 8 && (CW / W > max_pairs_ch || npw_of(CW) == 0 || lds_of(CW) > 160 * 1024 || CW / W > cap))
+      CW /= 2;
+    if (CW >= 8 && stream_stride * W * 8 < ((int64_t)1 << 32)) {
+      const int npw = npw_of(CW);
+      const size_t lds = lds_of(CW);
+      const int CH = CW / W;
       const int n_blk = (n_ch + CH - 1) / CH;
-      const int G = std::max(1, 128 / (CH * 4));
+      const int G = std::max(1, 128 / (CW * 4));
       dim3 grid((unsigned)(n_blk * n_k));
       const float inv2 = 1.0f / (scale * scale);
       const uint32_t *sp = static_cast<const uint32_t *>(spectra);
       hipError_t e = hipErrorInvalidValue;
-#define SFXG_MFD3(P, W, C)                                                                                         \
+#define SFXG_MFD3(P, NW, C)                                                                                        \
   do {                                                                                                             \
-    e = hipFuncSetAttribute((const void *)k_xstep_mfma_dma<P, W, C>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+    e = hipFuncSetAttribute((const void *)k_xstep_mfma_dma<P, NW, C>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             (int)lds);                                                                             \
     if (e != hipSuccess) return e;                                                                                 \
-    hipLaunchKernelGGL((k_xstep_mfma_dma<P, W, C>), grid, dim3(MF_THREADS), lds, s, sp, stream_stride,            \
+    hipLaunchKernelGGL((k_xstep_mfma_dma<P, NW, C>), grid, dim3(MF_THREADS), lds, s, sp, stream_stride,           \
                        seg_stride, seg_base, n_ch, M, RT, n_blk, G, fold_start, bl_first, bl_second, n_baselines,  \
                        inv2, partial);                                                                             \
     e = hipGetLastError();                                                                                         \
   } while (0)
-#define SFXG_MFD(P, W)                   \
-  do {                                   \
-    if (CH == 8) SFXG_MFD3(P, W, 8);     \
-    else if (CH == 16) SFXG_MFD3(P, W, 16); \
-    else if (CH == 32) SFXG_MFD3(P, W, 32); \
-    else SFXG_MFD3(P, W, 64);            \
+#define SFXG_MFD(P, NW)                     \
+  do {                                      \
+    if (CW == 8) SFXG_MFD3(P, NW, 8);       \
+    else if (CW == 16) SFXG_MFD3(P, NW, 16); \
+    else if (CW == 32) SFXG_MFD3(P, NW, 32); \
+    else SFXG_MFD3(P, NW, 64);              \
   } while (0)
 #define SFXG_MFDP(P)                          \
   do {                                        \
@@ -2894,6 +2938,7 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
   } while (0)
       if (precision == SFXG_PREC_FP16) SFXG_MFDP(SFXG_PREC_FP16);
       else if (precision == SFXG_PREC_BF16) SFXG_MFDP(SFXG_PREC_BF16);
+      else if (precision == SFXG_PREC_SPLIT_FP16) SFXG_MFDP(SFXG_PREC_SPLIT_FP16);
 #undef SFXG_MFDP
 #undef SFXG_MFD
 #undef SFXG_MFD3
@@ -2903,6 +2948,7 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
       return hipGetLastError();
     }
   }
+  if (split_planes) return hipErrorInvalidValue;  // the caller keeps FP32 spectra where this form does not fit
   // channels per workgroup: all (channel, tile) pairs fit MF_WAVES waves x
   // MF_MAXP and the stage fits 150 KiB of LDS. Stages of 32 segments (two
   // MFMA K steps per barrier) where that keeps the channel count, else 16.

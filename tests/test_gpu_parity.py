@@ -412,13 +412,16 @@ def test_mixed_pulsar_vs_fp64(ctx, prec, nbins, gate):
 
 
 @pytest.mark.parametrize("nchan,prec,keep_ts", [(1024, 1, False), (1024, 2, True), (4096, 2, False),
-                                                (16384, 1, False), (32768, 1, False)])
+                                                (16384, 1, False), (32768, 1, False), (1024, 3, False),
+                                                (2048, 3, True), (16384, 3, False), (32768, 3, False)])
 def test_packed_spectra(ctx, nchan, prec, keep_ts):
-    """FP16 and BF16 X-steps read spectra that the channeliser stored in that
-    type (fused F-engine, unfused channeliser, large-N runs, four-step): each
-    part is the FP32 spectrum value rounded to nearest in that type (unit
-    roundoff 2^-11 for FP16, 2^-8 for BF16, relative), and the visibilities
-    meet the mixed-precision bars against the FP64 oracle."""
+    """FP16, BF16 and split FP16 X-steps read spectra that the channeliser
+    stored as their operands (fused F-engine, unfused channeliser, large-N
+    runs, four-step): each part is the FP32 spectrum value rounded to nearest
+    in that type (unit roundoff 2^-11 for FP16, 2^-8 for BF16, relative; split
+    FP16 stores hi = fp16(x) and lo = fp16(x - hi), whose sum is x to 2^-21),
+    and the visibilities meet the mixed-precision bars against the FP64
+    oracle."""
     job, hdrs = configs.build_slices(seed=42, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
                                      seconds=0.125, t_int_usec=125000, rho=0.05, bits=2, sky_freq=1_400_000_000,
                                      offsets=[0.0, 2e-6, -1e-6], rates=[0.0, 1e-7, -2e-7])[0]
@@ -429,14 +432,14 @@ def test_packed_spectra(ctx, nchan, prec, keep_ts):
         sph = ctx.stage(capi.STAGE_SPECTRA, 1)
     finally:
         ctx.set_options(lut=LUT_CONFIG, precision=0)
-    u = 2.0 ** -11 if prec == 1 else 2.0 ** -8
+    u = {1: 2.0 ** -11, 2: 2.0 ** -8, 3: 2.0 ** -21}[prec]
     rms = np.sqrt(np.mean(np.abs(sp32) ** 2))
     for part in (np.real, np.imag):
         f = part(sp32).astype(np.float64)
         h = part(sph).astype(np.float64)
         assert np.all(np.abs(h - f) <= 1.01 * u * np.abs(f) + 1e-6 * rms)
-    # the rounding is not a no-op: most values moved
-    assert np.mean(sph != sp32) > 0.5
+    # the rounding is not a no-op: many values moved
+    assert np.mean(sph != sp32) > (0.5 if prec != 3 else 0.2)
     r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
     heads_g, vis_g = vis_from_bytes(job, out)
     heads_r, vis_r = vis_from_bytes(job, r64["output"])
