@@ -312,8 +312,11 @@ def test_config0_full(ctx):
 
 
 def test_cpp_host_adapter(ctx, c0_short, tmp_path):
-    """A C++ host (host/correlate_slice.cpp through host/sfxg_adapter.h)
-    produces the same integration_write bytes as the oracle's layout."""
+    """A C++ host (host/correlate_slice.cpp) wiring the drop-in classes with
+    the reference's signatures (host/sfxc_dropin.h: Bit2float_worker,
+    Delay_correction, Correlation_core) the way Correlator_node does, its
+    delay tables built as Akima splines over 1 s samples, produces the
+    oracle's headers and visibilities (and the statistics it prints)."""
     import os
     import subprocess
     from sfxg import slicefile
@@ -323,16 +326,29 @@ def test_cpp_host_adapter(ctx, c0_short, tmp_path):
     sl = tmp_path / "slice.bin"
     cor = tmp_path / "out.cor"
     slicefile.write_slice(str(sl), job, hdrs)
+    print("case C++ drop-in host", flush=True)
     r = subprocess.run([exe, str(sl), str(cor)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     out_cpp = cor.read_bytes()
-    out_py = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
-    assert out_cpp == out_py
     ref = O.run_slice(job, hdrs, lut=LUT_CONFIG)
     heads_g, vis_g = vis_from_bytes(job, out_cpp)
     heads_r, vis_r = vis_from_bytes(job, ref["output"])
     assert heads_g == heads_r
-    compare_vis(vis_g[0], vis_r[0], label="C++ host")
+    compare_vis(vis_g[0], vis_r[0], label="C++ drop-in host")
+    levels = [int(v) for v in r.stdout.split(":")[1].split()]
+    assert levels == list(ref["stats"][0][0])
+    # the library through the Python binding, on the same slice
+    out_py = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+    heads_p, vis_p = vis_from_bytes(job, out_py)
+    assert heads_p == heads_g
+    compare_vis(vis_g[0], vis_p[0], label="C++ drop-in host vs Python binding")
+    # split FP16 through the same host, against the FP64 oracle
+    r = subprocess.run([exe, str(sl), str(cor), "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    _, vis_s = vis_from_bytes(job, cor.read_bytes())
+    _, vis64 = vis_from_bytes(job, O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")["output"])
+    l2, worst = mixed_errors(vis_s[0], vis64[0])
+    assert l2 <= MIXED_BARS[3][0] and worst <= MIXED_BARS[3][1]
 
 
 # Mixed-precision error bars against the FP64 oracle, per product: the
