@@ -18,6 +18,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "sfxc_dropin.h"
@@ -100,18 +101,25 @@ int main(int argc, char **argv) {
     if (!rd(f, scan_begin) || !rd(f, interval) || !rd(f, n)) return 3;
     std::vector<double> coef(4 * (size_t)n);
     if (fread(coef.data(), sizeof(double), coef.size(), f) != coef.size()) return 3;
-    // the delay table's samples: the value at every piece start and at the
-    // end of the last piece
-    std::vector<double> t(n + 1), y(n + 1);
-    for (int k = 0; k <= n; k++) {
+    // the delay table's samples: the value at every piece start, and past
+    // the last piece its continuation, so that the spline has the five
+    // points a .del table's scan gives it at least
+    const int n_pts = std::max(n + 1, 5);
+    std::vector<double> t(n_pts), y(n_pts);
+    for (int k = 0; k < n_pts; k++) {
       t[k] = k * interval;
       const int piece = k < n ? k : n - 1;
-      const double u = k < n ? 0.0 : interval;
+      const double u = (k - piece) * interval;
       const double *c = &coef[4 * piece];
       y[k] = c[0] + u * (c[1] + u * (c[2] + u * c[3]));
     }
-    delays.push_back(
-        Delay_table_akima::from_samples(Time::from_ticks(scan_begin), t, {y}, {}, {}, Time::from_ticks(scan_begin), 0, 0));
+    try {
+      delays.push_back(Delay_table_akima::from_samples(Time::from_ticks(scan_begin), t, {y}, {}, {},
+                                                       Time::from_ticks(scan_begin), 0, 0));
+    } catch (const std::exception &e) {
+      fprintf(stderr, "error: stream %d: %s\n", i, e.what());
+      return 1;
+    }
     uint64_t nb = 0;
     if (!rd(f, nb)) return 3;
     hdr[i].resize(nb);
