@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/sfxg.h"
@@ -682,6 +683,7 @@ struct TableTw {
   __device__ __forceinline__ float2 split(int k) const { return n2[k]; }
 };
 struct LdsTw {
+  static constexpr bool kChain = false;
   const float2 *hi;
   const float2 *lo;
   // lo[0] = 1 exactly, so the product is exact where j is a multiple of 128;
@@ -696,6 +698,34 @@ struct LdsTw {
   __device__ __forceinline__ float2 fft(int m) const { return at(2 * m); }
   __device__ __forceinline__ float2 split(int k) const { return at(k); }
 };
+// The same tables, with the twiddles w^q (q = 2 .. R - 1) of a radix-R stage
+// formed from w = W^(k step) by products of powers (at most four products
+// deep): one table product per R points instead of R - 1. Their rounding
+// error, a few units of 2^-24, is far below the FP16 / BF16 rounding of the
+// spectra that take it.
+struct LdsTwChain : LdsTw {
+  static constexpr bool kChain = true;
+};
+// The twiddle source of a channeliser storing spectra of type SP: chained
+// powers for packed FP16 / BF16, table products otherwise.
+template <int SP>
+__device__ __forceinline__ auto channeliser_tw(const float2 *hi, const float2 *lo) {
+  std::conditional_t<SP == SFXG_PREC_FP16 || SP == SFXG_PREC_BF16, LdsTwChain, LdsTw> t;
+  t.hi = hi;
+  t.lo = lo;
+  return t;
+}
+
+// w[q] = w[1]^q for q < R, from w[1]: squares for the powers of two, one
+// product for the others (w[a + b] = w[a] w[b], a the largest power of two).
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2 (&w)[R]) {
+#pragma unroll
+  for (int q = 2; q < R; q++) {
+    const int a = 1 << (31 - __builtin_clz(q));
+    w[q] = a == q ? cmul(w[q / 2], w[q / 2]) : cmul(w[a], w[q - a]);
+  }
+}
 
 // One radix-R stage over the registers of all threads; ns = product of the
 // earlier radices. Outputs go to LDS.
@@ -712,12 +742,20 @@ __device__ __forceinline__ void fft_stage(float2 (&v)[P], float2 *lds, const TW 
     for (int q = 0; q < R; q++) u[q] = v[b + q * B];
     if (ns > 1) {
       const int step = N / (ns * R);
+      if constexpr (TW::kChain) {
+        float2 w[R];
+        w[1] = tw.fft(k * step);
+        twiddle_powers<R>(w);
 #pragma unroll
-      for (int q = 1; q < R; q++) {
-        u[q] = cmul(u[q], tw.fft(q * k * step));
-        // at most four twiddles in flight, so their loads do not hold
-        // registers needed by data the caller keeps live across the FFT
-        if (q % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+        for (int q = 1; q < R; q++) u[q] = cmul(u[q], w[q]);
+      } else {
+#pragma unroll
+        for (int q = 1; q < R; q++) {
+          u[q] = cmul(u[q], tw.fft(q * k * step));
+          // at most four twiddles in flight, so their loads do not hold
+          // registers needed by data the caller keeps live across the FFT
+          if (q % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+        }
       }
     }
     dft<R, -1>(u);
@@ -1011,7 +1049,7 @@ __global__ __launch_bounds__(N / P) void k_channelise(const DevStream *__restric
     v[i] = z;
   }
   __syncthreads();  // twiddle tables
-  const LdsTw tw{thi, tlo};
+  const auto tw = channeliser_tw<SP>(thi, tlo);
   reg_fft<N, P>(v, lds, tw, threadIdx.x);
   real_split<N, P, T>(lds, tw, SpecOut<SP>(s.spectra, seg * job.spectra_stride, job.spec_scale), threadIdx.x);
 }
@@ -1041,7 +1079,7 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
     const float2 t = tw_split[i];
     if (i < NHI) thi[i] = t; else tlo[i - NHI] = t;
   }
-  const LdsTw tw{thi, tlo};
+  const auto tw = channeliser_tw<SP>(thi, tlo);
   const bool none = job.window == SFXG_WINDOW_NONE;
   const float2 *w2 = reinterpret_cast<const float2 *>(window);
   // complex point m of segment seg is the pair of reals 2m, 2m + 1 at
@@ -1120,6 +1158,7 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
 constexpr int LG_N1 = 128, LG_N2 = 256, LG_N = LG_N1 * LG_N2;
 
 struct StrideTw {  // exp(-2 pi i m / n) = tw_N[m * stride], stride = N / n
+  static constexpr bool kChain = false;
   const float2 *__restrict__ tw;
   int stride;
   __device__ __forceinline__ float2 fft(int m) const { return tw[m * stride]; }
@@ -1234,7 +1273,7 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
     if (i < NHI) thi[i] = t; else tlo[i - NHI] = t;
   }
   __syncthreads();
-  const LdsTw twl{thi, tlo};
+  const auto twl = channeliser_tw<SP>(thi, tlo);
   const bool none = job.window == SFXG_WINDOW_NONE;
   float2 *scratch = fbuf + wave * WAVE_FFT256_LDS;
   // blocks go in pairs (blk, blk + 1) with blk even, as in k_delay
