@@ -436,8 +436,10 @@ def test_packed_spectra(ctx, nchan, prec, keep_ts):
     runs, four-step): each part is the FP32 spectrum value rounded to nearest
     in that type (unit roundoff 2^-11 for FP16, 2^-8 for BF16, relative; split
     FP16 stores hi = fp16(x) and lo = fp16(x - hi), whose sum is x to 2^-21),
-    and the visibilities meet the mixed-precision bars against the FP64
-    oracle."""
+    where the FP16 / BF16 channelisers' own transform (twiddles as products
+    of powers) may differ from the FP32 one by a few 2^-24 of the rms per
+    stage (4e-6 of the rms is allowed), and the visibilities meet the
+    mixed-precision bars against the FP64 oracle."""
     job, hdrs = configs.build_slices(seed=42, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
                                      seconds=0.125, t_int_usec=125000, rho=0.05, bits=2, sky_freq=1_400_000_000,
                                      offsets=[0.0, 2e-6, -1e-6], rates=[0.0, 1e-7, -2e-7])[0]
@@ -453,7 +455,7 @@ def test_packed_spectra(ctx, nchan, prec, keep_ts):
     for part in (np.real, np.imag):
         f = part(sp32).astype(np.float64)
         h = part(sph).astype(np.float64)
-        assert np.all(np.abs(h - f) <= 1.01 * u * np.abs(f) + 1e-6 * rms)
+        assert np.all(np.abs(h - f) <= 1.01 * u * np.abs(f) + (4e-6 if prec in (1, 2) else 1e-6) * rms)
     # the rounding is not a no-op: many values moved
     assert np.mean(sph != sp32) > (0.5 if prec != 3 else 0.2)
     r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
