@@ -453,6 +453,37 @@ class MixedBatch:
         self.setup_s = time.perf_counter() - t0
         self.rounds = max(len(d["jobs"]) for d in self.jobs)
 
+    def run_one_graph(self, torch, capi, precision, dev, chunk_mib=128, reps=2):
+        """Every slice of every job prepared on a context of its own, all
+        captured into one graph (16 branches) and run as one launch; the
+        first launch captures and is not timed, then `reps` launches are."""
+        ctxs = []
+        batch = capi.Batch(dev, n_branches=16)
+        try:
+            for j, d in enumerate(self.jobs):
+                for r, job in enumerate(d["jobs"]):
+                    c = capi.Context(dev)
+                    ctxs.append(c)
+                    c.set_options(lut=1, precision=precision, chunk_mib=chunk_mib)
+                    c.set_parameters(job)
+                    for i, (ctrl, buf, n) in enumerate(self.inputs[j][r]):
+                        c.push_stream_elided(i, ctrl, buf.data_ptr(), n)
+                    c.prepare()
+                    batch.add(c)
+            batch.launch()
+            batch.synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                batch.launch()
+            batch.synchronize()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / reps, len(ctxs)
+        finally:
+            batch.close()
+            for c in ctxs:
+                c.close()
+
     def run(self, torch, capi, precision, dev, chunk_mib=512, autotune=True):
         sets = [[capi.Context(dev) for _ in self.jobs] for _ in range(2)]
         for cs in sets:
@@ -741,7 +772,8 @@ def main():
         from collections import Counter
         c3 = {"workload": f"{args.c3_jobs} seeded jobs, 2 s each: channels from {{256, 1024, 4096, 16384, 32768}}, "
                           "4..24 stations, 1- or 2-bit, 8/16/32/64 MHz, 0.25/0.5/1/2 s integrations; per round "
-                          "every job's next slice as one captured HIP graph",
+                          "every job's next slice as one captured HIP graph (value), and all slices as one graph "
+                          "launch (one_graph)",
               "station_samples": mb.station_samples, "rounds": mb.rounds,
               "slices": sum(len(d["jobs"]) for d in mb.jobs),
               "channels": dict(Counter(str(d["nchan"]) for d in mb.jobs)),
@@ -750,6 +782,18 @@ def main():
             wall = mb.run(torch, capi, PREC[name], dev, autotune=not args.no_autotune)
             c3[name] = {"value": round(mb.station_samples / wall / 1e9, 4), "wall_s": round(wall, 3)}
             print(f"mixed batch {name} done: {c3[name]['value']:.2f}", file=sys.stderr, flush=True)
+            torch.cuda.empty_cache()
+            try:
+                wall1, n_ctx = mb.run_one_graph(torch, capi, PREC[name], dev)
+            except capi.SfxgError as e:  # e.g. device memory for every slice's buffers at once
+                c3[name]["one_graph"] = {"error": str(e)[:200]}
+                print(f"mixed batch {name} one graph failed: {e}", file=sys.stderr, flush=True)
+                continue
+            c3[name]["one_graph"] = {"value": round(mb.station_samples / wall1 / 1e9, 4), "wall_s": round(wall1, 3),
+                                     "slices": n_ctx, "launches": 1,
+                                     "note": "every slice prepared beforehand on its own context; one graph "
+                                             "launch runs all of them"}
+            print(f"mixed batch {name} one graph: {c3[name]['one_graph']['value']:.2f}", file=sys.stderr, flush=True)
         if "fp32" in c3 and "fp16" in c3:
             c3["fp16_vs_plain"] = round(c3["fp16"]["value"] / c3["fp32"]["value"], 3)
         line["config3_batch"] = c3
