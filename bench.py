@@ -38,16 +38,21 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP16_DENSE_PEAK_TFLOPS = 2500.0
-# HBM bytes per launch (one slice) of a kernel, from the FETCH_SIZE and
+# HBM bytes per launch (one chunk of a slice) of a kernel, from the FETCH_SIZE and
 # WRITE_SIZE counters scaled by a 1 GiB copy kernel's readings (the
 # summaries under profiles/ give the commands); re-measured whenever the
 # kernel changes. Key: (kernel, channels, streams, precision).
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {
+    # 8.320e9 read + 8.193e9 written per chunk of 1952 segments x 64 streams
+    ("channelise", 16384, 64, "fp16"): 16513000000,
+}
 # Wave-level VALU instructions per launch (SQ_INSTS_VALU, same profiles),
 # against the wave64 FP32 issue rate: 256 CUs x 1 instruction per cycle (a
 # wave64 instruction occupies a 16-lane SIMD for 4 cycles, 4 SIMDs per CU)
 # x 2.4 GHz.
-MEASURED_VALU_INSTS = {}
+MEASURED_VALU_INSTS = {
+    ("channelise", 16384, 64, "fp16"): 3.075e9,
+}
 VALU_ISSUE_PEAK = 256 * 2.4e9
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "split_fp16": 3}
@@ -385,6 +390,7 @@ class BatchedWorkload:
             c.correlate()
             c.synchronize()
         kt = c.kernel_times(reset=True)
+        self.chunks, self.chunk_segs = c.chunk_plan()
         # one slice's visibilities for the error report, and its read-back time
         self._fill(c, 0, 0)
         c.correlate()
@@ -677,10 +683,15 @@ def main():
         dom = int(np.argmax(kt))
         dom_name = KERNELS[dom]
         achieved = h["_kb"][dom_name] / (kt[dom] * 1e-3) / 1e9
+        # the delay, channeliser and X-step kernels run once per chunk of the
+        # slice; stats and normalise once per slice
+        per = wl.chunks if dom_name in ("fengine", "channelise", "xstep") else 1
         roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": h["_kb"][dom_name], "launch": "one slice (64 streams x 0.5 s)",
-                "launch_ms": round(float(kt[dom]), 3)}
+                "bytes_per_launch": int(h["_kb"][dom_name] / per),
+                "launch": (f"one chunk of {wl.chunk_segs} segments x 64 streams ({wl.chunks} per 0.5 s slice)"
+                           if per > 1 else "one slice (64 streams x 0.5 s)"),
+                "launch_ms": round(float(kt[dom]) / per, 3)}
         if peaks:
             roof["frac_of_achievable"] = round(achieved / peaks["hbm_copy_GBs"], 4)
         key = (dom_name, 16384, wl.M, head)
@@ -688,13 +699,14 @@ def main():
             roof["traffic"] = MEASURED_TRAFFIC[key]
             roof["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated"
         if key in MEASURED_VALU_INSTS:
-            rate = MEASURED_VALU_INSTS[key] / (kt[dom] * 1e-3)
+            rate = MEASURED_VALU_INSTS[key] / (roof["launch_ms"] * 1e-3)
             roof["valu"] = {"insts_per_launch": MEASURED_VALU_INSTS[key], "achieved_insts_per_s": round(rate, -8),
                             "peak_insts_per_s": VALU_ISSUE_PEAK, "frac": round(rate / VALU_ISSUE_PEAK, 4),
                             "source": "rocprofv3 SQ_INSTS_VALU"}
         xs = {"kernel": "xstep", "TFLOPs": h["xstep_TFLOPs"], "mfma_frac": h.get("xstep_mfma_frac"),
               "mfma_frac_of_achievable": h.get("xstep_mfma_frac_of_achievable"),
-              "GBs": h["kernel_GBs"].get("xstep"), "launch_ms": h["kernel_ms_isolated"]["xstep"]}
+              "GBs": h["kernel_GBs"].get("xstep"), "launch_ms": round(h["kernel_ms_isolated"]["xstep"] / wl.chunks, 3),
+              "launches_per_slice": wl.chunks}
         errs = None
         if not args.no_errors and "split_fp16" in vis:
             errs = {"reference": "split FP16 of the same slice (itself within 1e-5 relative L2 of the FP64 oracle "

@@ -376,6 +376,13 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
 #define SFXG_KERNEL_NORMALISE 4
 #define SFXG_N_KERNELS 5
 int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n, int32_t reset);
+/* The chunks the prepared slice runs in (sfxg_options.chunk_mib): the delay,
+ * channeliser and X-step kernels are launched once per chunk, so a kernel's
+ * duration per launch is its sfxg_kernel_times figure / *n_chunks.
+ * *max_segments: FFT segments per stream of the largest chunk. No counterpart
+ * in the reference, which processes one FFT segment at a time
+ * (src/delay_correction.cc:21-80). */
+int sfxg_chunk_plan(const sfxg_ctx *ctx, int32_t *n_chunks, int64_t *max_segments);
 
 /* Autotuner of launch shapes: on a prepared context (sfxg_prepare), times the
  * candidate FFT schedules of the F-engine (threads, i.e. points per thread and

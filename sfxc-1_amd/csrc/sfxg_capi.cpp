@@ -1785,6 +1785,14 @@ int sfxg_kernel_times(sfxg_ctx *ctx, double *ms, int32_t n, int32_t reset) {
   return SFXG_OK;
 }
 
+int sfxg_chunk_plan(const sfxg_ctx *ctx, int32_t *n_chunks, int64_t *max_segments) {
+  if (!ctx || !n_chunks || !max_segments) return SFXG_ERR_ARG;
+  if (!ctx->prepared) return SFXG_ERR_STATE;
+  *n_chunks = (int32_t)ctx->lp.chunks.size();
+  *max_segments = ctx->lp.chunk_segs;
+  return SFXG_OK;
+}
+
 int sfxg_synthesize_raw(int device, void *device_dst, size_t n_bytes, int32_t bits_per_sample, int64_t first_sample,
                         int64_t delay_samples, uint64_t common_seed, uint64_t stream_seed, float rho,
                         float threshold, int64_t delay2_samples, uint64_t common2_seed, float rho2) {

@@ -96,6 +96,8 @@ def lib() -> C.CDLL:
                                        C.POINTER(C.c_size_t)]
         L.sfxg_kernel_times.restype = C.c_int
         L.sfxg_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.c_int32]
+        L.sfxg_chunk_plan.restype = C.c_int
+        L.sfxg_chunk_plan.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.sfxg_prepare.restype = C.c_int
         L.sfxg_prepare.argtypes = [C.c_void_p]
         L.sfxg_batch_create.restype = C.c_int
@@ -338,6 +340,13 @@ class Context:
         ms = (C.c_double * 5)()
         self._check(lib().sfxg_kernel_times(self._h, ms, 5, int(reset)), "sfxg_kernel_times")
         return np.array(ms[:])
+
+    def chunk_plan(self) -> tuple:
+        """(chunks, segments per stream of the largest chunk) of the prepared
+        slice: the delay, channeliser and X-step kernels run once per chunk."""
+        n, segs = C.c_int32(), C.c_int64()
+        self._check(lib().sfxg_chunk_plan(self._h, C.byref(n), C.byref(segs)), "sfxg_chunk_plan")
+        return n.value, segs.value
 
     def tsys(self) -> bytes:
         """The slice's tsys_write messages (sfxg_fetch_tsys), 44 bytes per stream."""
