@@ -1319,6 +1319,24 @@ __global__ __launch_bounds__(T) void k_fengine(const DevStream *__restrict__ str
   }
 }
 
+namespace {
+
+// Workgroup index of a one-dimensional grid after the XCD regrouping: the
+// hardware deals workgroups to the 8 XCDs in turn, so workgroup i runs on
+// XCD i % 8. Groups of G consecutive logical indices (workgroups that read the
+// same spectra: channel blocks sharing 128-byte lines, or the stream-pair
+// groups of one channel block) are given to one XCD, where they run
+// together and its L2 fetches each line once. A bijection on [0, n): the last
+// n % (8 G) workgroups keep their index.
+__device__ __forceinline__ int xcd_grouped(int i, int n, int G) {
+  const int full = n / (8 * G) * (8 * G);
+  if (G <= 1 || i >= full) return i;
+  const int xcd = i & 7, slot = i >> 3;
+  return (slot / G) * (8 * G) + xcd * G + slot % G;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // k_xstep_subint: plain FP32 X-step for one sub-integration per workgroup
 // slice of the grid. Lane = channel (64-channel tile), each wave owns up to
@@ -1428,8 +1446,10 @@ __global__ __launch_bounds__(512) void k_xstep_subint(const float2 *__restrict__
 // wave owns a 4 x 4 tile (streams a0.. x b0..) of 64 channels (lane =
 // channel): per segment it reads 8 spectra from LDS for 16 products, where a
 // per-baseline scheme reads 2 per product. 1024-thread workgroups stage
-// segments x all streams x 64 channels in LDS (one workgroup per CU), so the
-// spectra of a channel tile are fetched ceil(tiles / 16) times. The tile
+// segments x all streams x 64 channels in LDS (one workgroup per CU). The
+// ceil(tiles / 16) workgroups of one channel block and sub-integration read
+// the same spectra: the one-dimensional grid puts them on one XCD
+// (xcd_grouped), where they run together and share its L2. The tile
 // table maps each of the 16 pairs to its baseline: b >= 0 stores the sum,
 // ~b its conjugate (the baseline lists the pair the other way round: the
 // product of the swapped pair is exactly the conjugate), XT_NONE nothing.
@@ -1443,16 +1463,18 @@ constexpr int XT_PF = 8;       // prefetch registers per thread (float2)
 __global__ __launch_bounds__(64 * XT_WAVES) void k_xstep_tile(
     const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch,
     int n_run, int M, const int32_t *__restrict__ fold_start, const int32_t *__restrict__ tiles, int n_tiles,
-    int n_baselines, int sps, float2 *__restrict__ partial) {
+    int n_baselines, int sps, int n_cb, int n_tg, float2 *__restrict__ partial) {
 #pragma clang fp contract(off)
   extern __shared__ float2 xlds[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ch0 = blockIdx.x * 64;
+  // (tile group, channel block, sub-integration), tile groups innermost
+  const int L = xcd_grouped(blockIdx.x, gridDim.x, n_tg);
+  const int tg = L % n_tg, cb = (L / n_tg) % n_cb, k = L / n_tg / n_cb;
+  const int ch0 = cb * 64;
   const int ch = ch0 + lane;
-  const int k = blockIdx.z;
   const int64_t s_begin = fold_start[k], s_end = fold_start[k + 1];
-  const int t = __builtin_amdgcn_readfirstlane(blockIdx.y * XT_WAVES + wave);
+  const int t = __builtin_amdgcn_readfirstlane(tg * XT_WAVES + wave);
   const bool active = t < n_tiles;
   const int a0 = active ? tiles[t * (2 + XT * XT)] : 0;
   const int b0 = active ? tiles[t * (2 + XT * XT) + 1] : 0;
@@ -1784,19 +1806,6 @@ __device__ __forceinline__ void mf_mma(f16v_t &acc, const _Float16 *ylds, int oa
 // packed operand pairs, or FP32 real and imaginary parts.
 template <bool HS>
 constexpr int mf_item_words() { return HS ? 8 : 16; }
-
-// Workgroup index of a one-dimensional grid after the XCD regrouping: the
-// hardware deals workgroups to the 8 XCDs in turn, so workgroup i runs on
-// XCD i % 8. Groups of G consecutive logical indices (channel blocks that
-// share 128-byte lines) are given to one XCD, where they run together and
-// their L2 fetches each line once. A bijection on [0, n): the last
-// n % (8 G) workgroups keep their index.
-__device__ __forceinline__ int xcd_grouped(int i, int n, int G) {
-  const int full = n / (8 * G) * (8 * G);
-  if (G <= 1 || i >= full) return i;
-  const int xcd = i & 7, slot = i >> 3;
-  return (slot / G) * (8 * G) + xcd * G + slot % G;
-}
 
 // SFXG_PREC_FP16, SFXG_PREC_BF16, SFXG_PREC_SPLIT_FP16; segments per stage;
 // spectra in the operand type (HS) or FP32. The launcher guarantees
@@ -2289,11 +2298,14 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
     const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int64_t seg_base,
     int64_t n_segments, const int32_t *__restrict__ bl_first, const int32_t *__restrict__ bl_second,
     int n_baselines, const double *__restrict__ offsets, double start_phase, double phase_coef,
-    double fft_duration, double gate_begin, double gate_end, int nbins, float2 *__restrict__ acc_out) {
+    double fft_duration, double gate_begin, double gate_end, int nbins, int n_bg, float2 *__restrict__ acc_out) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ch = blockIdx.x * 64 + lane;
-  const int b0 = (blockIdx.y * XWAVES + wave) * XB;
+  // (baseline group, channel block), baseline groups innermost: the groups
+  // of one channel block read the same spectra and run on one XCD
+  const int L = xcd_grouped(blockIdx.x, gridDim.x, n_bg);
+  const int ch = (L / n_bg) * 64 + lane;
+  const int b0 = ((L % n_bg) * XWAVES + wave) * XB;
   if (ch >= n_ch || b0 >= n_baselines) return;
   const float2 *pa[XB];
   const float2 *pb[XB];
@@ -2874,9 +2886,11 @@ hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64
   // the last bin of N_c + 1 on its own, so that the tile grid covers whole
   // blocks of 64 channels
   const int n_run = n_ch > 1 && (n_ch - 1) % 64 == 0 ? n_ch - 1 : n_ch;
-  dim3 grid((unsigned)((n_run + 63) / 64), (unsigned)((n_tiles + XT_WAVES - 1) / XT_WAVES), (unsigned)n_k);
+  const int n_cb = (n_run + 63) / 64, n_tg = (n_tiles + XT_WAVES - 1) / XT_WAVES;
+  if ((int64_t)n_cb * n_tg * n_k >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)(n_cb * n_tg * n_k));
   hipLaunchKernelGGL(k_xstep_tile, grid, dim3(64 * XT_WAVES), lds, s, spectra, stream_stride, seg_stride, seg_base,
-                     n_ch, n_run, M, fold_start, tiles, n_tiles, n_baselines, sps, partial);
+                     n_ch, n_run, M, fold_start, tiles, n_tiles, n_baselines, sps, n_cb, n_tg, partial);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (n_run < n_ch) {
@@ -3061,10 +3075,12 @@ hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int
                                double phase_coef, double fft_duration, double gate_begin, double gate_end, int nbins,
                                float2 *acc, hipStream_t s) {
   if (n_segments <= 0) return hipSuccess;
-  dim3 grid((unsigned)((n_ch + 63) / 64), (unsigned)((n_baselines + XB * XWAVES - 1) / (XB * XWAVES)));
+  const int n_cb = (n_ch + 63) / 64, n_bg = (n_baselines + XB * XWAVES - 1) / (XB * XWAVES);
+  if ((int64_t)n_cb * n_bg >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)(n_cb * n_bg));
   hipLaunchKernelGGL(k_xstep_pulsar, grid, dim3(64 * XWAVES), 0, s, spectra, stream_stride, seg_stride, n_ch,
                      seg_base, n_segments, bl_first, bl_second, n_baselines, offsets, start_phase, phase_coef,
-                     fft_duration, gate_begin, gate_end, nbins, acc);
+                     fft_duration, gate_begin, gate_end, nbins, n_bg, acc);
   return hipGetLastError();
 }
 
