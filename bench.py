@@ -187,8 +187,9 @@ def cpu_baseline(sample_usec, cores):
 
 
 def achievable_peaks(torch, dev):
-    """Copy bandwidth (read + write bytes) and dense FP16 MFMA rate measured
-    on this GPU by the microbenchmarks in libsfxg_peaks.so."""
+    """Copy bandwidth (read + write bytes), dense FP16 MFMA rate and FP32 VALU
+    issue rate (wave-level instructions per second) measured on this GPU by
+    the microbenchmarks in libsfxg_peaks.so."""
     import ctypes as C
 
     path = os.path.join(ROOT, "sfxc-1_amd", "lib", "libsfxg_peaks.so")
@@ -208,8 +209,12 @@ def achievable_peaks(torch, dev):
     rc = lib.sfxg_peak_mfma_fp16(20000, 8, C.byref(tf))
     if rc != 0:
         raise RuntimeError(f"sfxg_peak_mfma_fp16 failed: {rc}")
+    vi = C.c_double()
+    rc = lib.sfxg_peak_valu(20000, 16, C.byref(vi))
+    if rc != 0:
+        raise RuntimeError(f"sfxg_peak_valu failed: {rc}")
     return {"hbm_copy_GBs": round(2 * n / (ms.value * 1e-3) / 1e9, 1), "copy_bytes": n,
-            "mfma_fp16_dense_TFLOPs": round(tf.value, 1)}
+            "mfma_fp16_dense_TFLOPs": round(tf.value, 1), "valu_fp32_insts_per_s": float(f"{vi.value:.4g}")}
 
 
 class Workload:
@@ -703,6 +708,9 @@ def main():
             roof["valu"] = {"insts_per_launch": MEASURED_VALU_INSTS[key], "achieved_insts_per_s": round(rate, -8),
                             "peak_insts_per_s": VALU_ISSUE_PEAK, "frac": round(rate / VALU_ISSUE_PEAK, 4),
                             "source": "rocprofv3 SQ_INSTS_VALU"}
+            if peaks:
+                roof["valu"]["achievable_insts_per_s"] = peaks["valu_fp32_insts_per_s"]
+                roof["valu"]["frac_of_achievable"] = round(rate / peaks["valu_fp32_insts_per_s"], 4)
         xs = {"kernel": "xstep", "TFLOPs": h["xstep_TFLOPs"], "mfma_frac": h.get("xstep_mfma_frac"),
               "mfma_frac_of_achievable": h.get("xstep_mfma_frac_of_achievable"),
               "GBs": h["kernel_GBs"].get("xstep"), "launch_ms": round(h["kernel_ms_isolated"]["xstep"] / wl.chunks, 3),

@@ -387,7 +387,8 @@ int sfxg_chunk_plan(const sfxg_ctx *ctx, int32_t *n_chunks, int64_t *max_segment
 /* Autotuner of launch shapes: on a prepared context (sfxg_prepare), times the
  * candidate FFT schedules of the F-engine (threads, i.e. points per thread and
  * radix stages, for N_c <= 4096; workgroup count of the large-N channeliser)
- * and the X-step tile shapes (waves per workgroup of the FP32 X-step, channels
+ * and the X-step tile shapes (FP32 X-step: register tiles on 16- or 64-channel
+ * blocks, or the per-baseline kernel and its waves per workgroup; channels
  * per workgroup of the MFMA X-step, workgroups of the MFMA pulsar X-step) on
  * the slice's own data, keeps the fastest of each in a process-wide table
  * keyed by the slice shape (channels, streams, precision, pulsar, fused), and
@@ -399,8 +400,10 @@ int sfxg_chunk_plan(const sfxg_ctx *ctx, int32_t *n_chunks, int64_t *max_segment
  * rounding of the FFT schedule. */
 int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap);
 /* The tuning table as text, one "shape|fe_threads|chan_wgs|xs_waves|mf_channels|
- * mfp_wgs|mf_ks" line per shape (0: default; a line without the last field is read as mf_ks 0); *written = bytes needed including the
- * terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear empties
+ * mfp_wgs|mf_ks" line per shape (0: default; a line without the last field
+ * is read as mf_ks 0; xs_waves 1 and 2 force the register tiles on 64- and
+ * 16-channel blocks, 4 and 8 the per-baseline kernel); *written = bytes
+ * needed including the terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear empties
  * the table. */
 int sfxg_tune_export(char *dst, size_t cap, size_t *written);
 int sfxg_tune_import(const char *text);

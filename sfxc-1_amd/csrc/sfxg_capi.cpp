@@ -1404,11 +1404,12 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
     float2 *fold_vis = ctx->n_pc > 1 ? nullptr : ctx->vis.as<float2>();
     for (int g0 = ch.fold0; g0 < ch.fold1; g0 += lp.k_group) {
       const int ng = std::min(lp.k_group, ch.fold1 - g0);
-      if (t.xs_waves == 0)
+      const int n_tiles = (int)(ctx->h_tiles.size() / 18);
+      if (t.xs_waves >= 0 && t.xs_waves <= 2)
         HIPCHK(ctx, launch_xstep_tile(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
-                                      ctx->fold_buf.as<int32_t>() + g0, ng, ctx->tiles.as<int32_t>(),
-                                      (int)(ctx->h_tiles.size() / 18), blf, bls, B, ctx->partial.as<float2>(),
-                                      fold_vis, s));
+                                      ctx->fold_buf.as<int32_t>() + g0, ng, ctx->tiles.as<int32_t>(), n_tiles, blf,
+                                      bls, B, ctx->partial.as<float2>(), fold_vis,
+                                      t.xs_waves == 2 || (t.xs_waves == 0 && xstep_tile16_default(n_tiles)), s));
       else
         HIPCHK(ctx, launch_xstep(ctx->spectra.as<float2>(), sstride, ctx->S, ch.seg0, nc + 1, M,
                                  ctx->fold_buf.as<int32_t>() + g0, ng, blf, bls, B, ctx->partial.as<float2>(),

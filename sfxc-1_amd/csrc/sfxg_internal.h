@@ -215,8 +215,10 @@ struct Tune {
   int fe_threads = 0;   // F-engine threads per workgroup for N_c <= 4096 (fused and
                         // unfused channeliser alike: N_c / threads points per thread)
   int chan_wgs = 0;     // large-N channeliser (N_c 8192, 16384): workgroups in all
-  int xs_waves = 0;     // plain X-step: 0 register tiles, else the per-baseline kernel with
-                        // this many waves per workgroup
+  int xs_waves = 0;     // plain X-step: 0 register tiles (16-channel blocks holding every
+                        // tile when there are many tiles, else 64-channel blocks), 1 register
+                        // tiles on 64-channel blocks, 2 on 16-channel blocks, else the
+                        // per-baseline kernel with this many waves per workgroup
   int mf_channels = 0;  // MFMA X-step: at most this many channels per workgroup
   int mf_ks = 0;        // MFMA X-step: segments per LDS stage (16 or 32)
   int mfp_wgs = 0;      // MFMA pulsar X-step: workgroups in all
@@ -225,6 +227,8 @@ struct Tune {
 std::vector<int> fe_thread_candidates(int nc);
 std::vector<int> chan_wgs_candidates(int nc);
 std::vector<int> xs_waves_candidates();
+// whether the plain X-step takes k_xstep_tile16 by default for this many tiles
+bool xstep_tile16_default(int n_tiles);
 std::vector<int> mf_channel_candidates(int precision, int M);
 std::vector<int> mfp_wgs_candidates();
 
@@ -267,7 +271,7 @@ void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::ve
 hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float2 *partial,
-                             float2 *vis, hipStream_t s);
+                             float2 *vis, bool tile16, hipStream_t s);
 // half_spectra: spectra packed in the operand type (FP16 / BF16 only).
 // Whether the direct-to-LDS X-step takes split FP16 hi / lo planes for M
 // streams with spectra rows stream_stride bins apart.

@@ -1550,6 +1550,131 @@ __global__ __launch_bounds__(64 * XT_WAVES) void k_xstep_tile(
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_xstep_tile16: the plain FP32 X-step for many stream-pair tiles (large
+// station counts). A workgroup of 12 waves holds every tile of a block of
+// 16 channels at once: lane = (tile slot u, channel c), 4 slots of 16 lanes
+// per wave, 48 slots of TPS tiles each (up to 144 tiles of 4 x 4 pairs, all
+// of 64 streams). A stage of segments is staged as [segment][stream][16
+// channels], 128-byte rows, so each spectrum line is fetched from HBM once
+// and read from LDS by every tile that needs it. More tiles than one
+// workgroup holds go to further tile groups on the same XCD. Products,
+// sums and their order per (baseline, channel) are those of k_xstep_tile,
+// so the results are the same bits.
+// ---------------------------------------------------------------------------
+constexpr int XQ_WAVES = 12;              // waves per workgroup (three per SIMD)
+constexpr int XQ_SLOTS = 4 * XQ_WAVES;    // tile slots of 16 lanes
+constexpr int XQ_PF = 3;                  // prefetch registers per thread (float4)
+
+template <int TPS>
+__global__ __launch_bounds__(64 * XQ_WAVES) void k_xstep_tile16(
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base, int n_ch,
+    int n_run, int M, const int32_t *__restrict__ fold_start, const int32_t *__restrict__ tiles, int n_tiles,
+    int n_baselines, int sps, int n_cb, int n_tg, float2 *__restrict__ partial) {
+#pragma clang fp contract(off)
+  extern __shared__ float4 xq[];  // [sps][M][8]: 16 channels of a stream-segment as 8 float4
+  const float2 *xq2 = reinterpret_cast<const float2 *>(xq);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = lane & 15;
+  const int slot = wave * 4 + (lane >> 4);
+  // (tile group, channel block, sub-integration), tile groups innermost
+  const int L = xcd_grouped(blockIdx.x, gridDim.x, n_tg);
+  const int tg = L % n_tg, cb = (L / n_tg) % n_cb, k = L / n_tg / n_cb;
+  const int ch0 = cb * 16;
+  const int64_t s_begin = fold_start[k], s_end = fold_start[k + 1];
+  int a0[TPS], b0[TPS];
+  bool act[TPS];
+#pragma unroll
+  for (int j = 0; j < TPS; j++) {
+    const int t = (tg * TPS + j) * XQ_SLOTS + slot;
+    act[j] = t < n_tiles;
+    a0[j] = act[j] ? tiles[t * (2 + XT * XT)] : 0;
+    b0[j] = act[j] ? tiles[t * (2 + XT * XT) + 1] : 0;
+  }
+  float2 acc[TPS][XT * XT];
+#pragma unroll
+  for (int j = 0; j < TPS; j++)
+#pragma unroll
+    for (int q = 0; q < XT * XT; q++) acc[j][q] = make_float2(0.0f, 0.0f);
+  // stage loads: item idx = (segment, stream, float4 column): 16 bytes each;
+  // the 16-channel block lies inside the row (rows hold a multiple of 64
+  // bins), channels past n_run are read but never stored
+  const int row_items = M * 8;
+  float4 pf[XQ_PF];
+  auto fetch = [&](int64_t s0, int ns) {
+    const int total = ns * row_items;
+#pragma unroll
+    for (int v = 0; v < XQ_PF; v++) {
+      const int idx = threadIdx.x + v * (64 * XQ_WAVES);
+      float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (idx < total) {
+        const int sl = idx / row_items, r = idx - sl * row_items;
+        const int st = r >> 3, q = r & 7;
+        x = *reinterpret_cast<const float4 *>(spectra + st * stream_stride + (s0 + sl - seg_base) * seg_stride +
+                                              ch0 + 2 * q);
+      }
+      pf[v] = x;
+    }
+  };
+  int64_t s0 = s_begin;
+  int ns = (int)((s_end - s0) < sps ? (s_end - s0) : sps);
+  if (s0 < s_end) fetch(s0, ns);
+  while (s0 < s_end) {
+    const int total = ns * row_items;
+#pragma unroll
+    for (int v = 0; v < XQ_PF; v++) {
+      const int idx = threadIdx.x + v * (64 * XQ_WAVES);
+      if (idx < total) xq[idx] = pf[v];
+    }
+    __syncthreads();
+    const int64_t s1 = s0 + ns;
+    const int ns1 = (int)((s_end - s1) < sps ? (s_end - s1) : sps);
+    if (s1 < s_end) fetch(s1, ns1);
+#pragma unroll 1
+    for (int sl = 0; sl < ns; sl++) {
+      const float2 *row = xq2 + sl * M * 16 + c;
+#pragma unroll
+      for (int j = 0; j < TPS; j++) {
+        if (!act[j]) continue;
+        float2 x[XT], y[XT];
+#pragma unroll
+        for (int i = 0; i < XT; i++) {
+          x[i] = a0[j] + i < M ? row[(a0[j] + i) * 16] : make_float2(0.0f, 0.0f);
+          y[i] = b0[j] + i < M ? row[(b0[j] + i) * 16] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int i = 0; i < XT; i++)
+#pragma unroll
+          for (int m = 0; m < XT; m++) {
+            const float pr = x[i].x * y[m].x + x[i].y * y[m].y;
+            const float pi = x[i].y * y[m].x - x[i].x * y[m].y;
+            acc[j][XT * i + m].x = acc[j][XT * i + m].x + pr;
+            acc[j][XT * i + m].y = acc[j][XT * i + m].y + pi;
+          }
+      }
+    }
+    __syncthreads();
+    s0 = s1;
+    ns = ns1;
+  }
+  const int ch = ch0 + c;
+  if (ch >= n_run) return;
+  float2 *o = partial + (int64_t)k * n_baselines * n_ch + ch;
+#pragma unroll
+  for (int j = 0; j < TPS; j++) {
+    if (!act[j]) continue;
+    const int t = (tg * TPS + j) * XQ_SLOTS + slot;
+#pragma unroll
+    for (int q = 0; q < XT * XT; q++) {
+      const int e = tiles[t * (2 + XT * XT) + 2 + q];
+      if (e == XT_NONE) continue;
+      if (e >= 0) o[(int64_t)e * n_ch] = acc[j][q];
+      else o[(int64_t)(~e) * n_ch] = make_float2(acc[j][q].x, -acc[j][q].y);
+    }
+  }
+}
+
 // The last bin (channel ch = N_c, the Nyquist bin) of every baseline and
 // sub-integration for the FP32 tile X-step, whose workgroups map 64
 // channels to the lanes: N_c + 1 bins would otherwise need one more block
@@ -2754,7 +2879,10 @@ std::vector<int> chan_wgs_candidates(int nc) {
   if (nc == 8192 || nc == 16384) return {2048, 1024, 4096};
   return {0};
 }
-std::vector<int> xs_waves_candidates() { return {0, 8, 4}; }  // 0: register tiles
+// 0: register tiles (16-channel blocks for many tiles, else 64-channel blocks),
+// 1: register tiles on 64-channel blocks, 2: on 16-channel blocks, else the
+// per-baseline kernel's waves per workgroup
+std::vector<int> xs_waves_candidates() { return {0, 1, 2, 8, 4}; }
 std::vector<int> mf_channel_candidates(int precision, int M) {
   const int RT = (2 * M + 31) / 32;
   const int R = 32 * RT;
@@ -2870,11 +2998,49 @@ void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::ve
     }
 }
 
+bool xstep_tile16_default(int n_tiles) { return n_tiles > 64; }
+
 hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float2 *partial,
-                             float2 *vis, hipStream_t s) {
+                             float2 *vis, bool tile16, hipStream_t s) {
   if (n_k <= 0) return hipSuccess;
+  if (tile16) {
+    // a stage of sps segments fills the threads' prefetch registers
+    int sps = (XQ_PF * 64 * XQ_WAVES) / (M * 8);
+    if (sps > 8) sps = 8;
+    if (sps < 1) return hipErrorInvalidValue;
+    if (stream_stride % 2 || seg_stride % 64) return hipErrorInvalidValue;  // 16-byte loads inside the rows
+    const size_t lds = (size_t)sps * M * 8 * sizeof(float4);
+    const int TPS = n_tiles <= XQ_SLOTS ? 1 : n_tiles <= 2 * XQ_SLOTS ? 2 : 3;
+    const int n_run = n_ch > 1 && (n_ch - 1) % 16 == 0 ? n_ch - 1 : n_ch;
+    const int n_cb = (n_run + 15) / 16, n_tg = (n_tiles + TPS * XQ_SLOTS - 1) / (TPS * XQ_SLOTS);
+    if ((int64_t)n_cb * n_tg * n_k >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(n_cb * n_tg * n_k));
+    hipError_t e = hipErrorInvalidValue;
+#define SFXG_XQ(T)                                                                                                \
+  do {                                                                                                            \
+    e = hipFuncSetAttribute((const void *)k_xstep_tile16<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    if (e != hipSuccess) return e;                                                                                \
+    hipLaunchKernelGGL(k_xstep_tile16<T>, grid, dim3(64 * XQ_WAVES), lds, s, spectra, stream_stride, seg_stride,  \
+                       seg_base, n_ch, n_run, M, fold_start, tiles, n_tiles, n_baselines, sps, n_cb, n_tg, partial); \
+    e = hipGetLastError();                                                                                        \
+  } while (0)
+    if (TPS == 1) SFXG_XQ(1);
+    else if (TPS == 2) SFXG_XQ(2);
+    else SFXG_XQ(3);
+#undef SFXG_XQ
+    if (e != hipSuccess) return e;
+    if (n_run < n_ch) {
+      e = launch_xstep_last(spectra, stream_stride, seg_stride, seg_base, n_ch - 1, M, fold_start, n_k, bl_first,
+                            bl_second, n_baselines, partial + (n_ch - 1), (int64_t)n_baselines * n_ch, n_ch, s);
+      if (e != hipSuccess) return e;
+    }
+    if (!vis) return hipSuccess;
+    const int64_t n = (int64_t)n_baselines * n_ch;
+    hipLaunchKernelGGL(k_fold_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, n_k, n, vis);
+    return hipGetLastError();
+  }
   // a stage fills the threads' prefetch registers
   int sps = (XT_PF * 64 * XT_WAVES) / (M * 64);
   if (sps > 16) sps = 16;

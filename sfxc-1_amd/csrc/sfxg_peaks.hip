@@ -1,6 +1,8 @@
 // Achievable-peak microbenchmarks for the roofline fractions the benchmark
-// reports beside the vendor peaks: a streaming copy (HBM read + write) and a
-// dense FP16 MFMA loop (v_mfma_f32_32x32x16_f16, FP32 accumulation). Built
+// reports beside the vendor peaks: a streaming copy (HBM read + write), a
+// dense FP16 MFMA loop (v_mfma_f32_32x32x16_f16, FP32 accumulation) and an
+// FP32 VALU loop (v_fma_f32, not packed: the issue rate of wave-level VALU
+// instructions the correlator's FFTs and FP32 X-step run at). Built
 // into its own library (libsfxg_peaks.so); not part of the correlator ABI.
 #include <stdint.h>
 
@@ -37,6 +39,22 @@ __global__ __launch_bounds__(256) void k_mfma(int iters, float *out) {
   float s = 0.0f;
   for (int q = 0; q < 4; q++)
     for (int k = 0; k < 16; k++) s += acc[q][k];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// Eight independent FMA chains per thread, so that a SIMD always has a
+// ready instruction from one of its waves; the sum is stored so the loop is
+// not removed.
+__global__ __launch_bounds__(256) void k_valu(int iters, float *out) {
+  float a[8];
+  const float b = 0.999f + 1e-7f * threadIdx.x, c = 1e-4f;
+  for (int k = 0; k < 8; k++) a[k] = 0.001f * (threadIdx.x + k);
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] = __builtin_fmaf(a[k], b, c);
+  }
+  float s = 0.0f;
+  for (int k = 0; k < 8; k++) s += a[k];
   out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
@@ -101,6 +119,35 @@ int sfxg_peak_mfma_fp16(int iters, int waves_per_cu, double *tflops) {
   if (hipGetLastError() != hipSuccess || t <= 0.0f) return -3;
   const double waves = (double)grid.x * 4;
   *tflops = waves * iters * 4 * 32768.0 / (t * 1e-3) / 1e12;
+  return 0;
+}
+
+// Runs the FMA loop on `waves_per_cu` waves per CU; *insts_per_s = wave-level
+// FP32 VALU instructions per second (8 per wave and iteration).
+int sfxg_peak_valu(int iters, int waves_per_cu, double *insts_per_s) {
+  if (iters <= 0 || waves_per_cu <= 0 || waves_per_cu % 4 != 0 || insts_per_s == nullptr) return -1;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -2;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -2;
+  const dim3 grid((unsigned)(cus * waves_per_cu / 4)), block(256);
+  float *out = nullptr;
+  if (hipMalloc(&out, (size_t)grid.x * 256 * sizeof(float)) != hipSuccess) return -2;
+  hipEvent_t e0, e1;
+  PK_CHECK(hipEventCreate(&e0));
+  PK_CHECK(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_valu, grid, block, 0, 0, iters / 10 + 1, out);
+  PK_CHECK(hipEventRecord(e0, 0));
+  hipLaunchKernelGGL(k_valu, grid, block, 0, 0, iters, out);
+  PK_CHECK(hipEventRecord(e1, 0));
+  PK_CHECK(hipEventSynchronize(e1));
+  float t = 0.0f;
+  PK_CHECK(hipEventElapsedTime(&t, e0, e1));
+  PK_CHECK(hipEventDestroy(e0));
+  PK_CHECK(hipEventDestroy(e1));
+  PK_CHECK(hipFree(out));
+  if (hipGetLastError() != hipSuccess || t <= 0.0f) return -3;
+  const double waves = (double)grid.x * 4;
+  *insts_per_s = waves * iters * 8.0 / (t * 1e-3);
   return 0;
 }
 

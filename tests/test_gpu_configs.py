@@ -24,7 +24,7 @@ import pytest
 from oracle import oracle as O
 from sfxg import capi, configs
 from sfxg.job import LUT_CONFIG
-from test_gpu_parity import MIXED_BARS, compare_vis, mixed_errors, vis_from_bytes
+from test_gpu_parity import MIXED_BARS, compare_vis, fp32_parity, mixed_errors, product_errors, vis_from_bytes
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
@@ -42,53 +42,11 @@ def ctx():
     c.close()
 
 
-def product_errors(g, r):
-    """Per product against the reference r: relative L2 error over channels,
-    and on channels with |r| >= 0.5 rms the largest relative amplitude error
-    and phase error (rad). Returns arrays over products (rows)."""
-    g, r = g.astype(np.complex128), r.astype(np.complex128)
-    l2, amp, ph = [], [], []
-    for k in range(r.shape[0]):
-        rms = np.sqrt(np.mean(np.abs(r[k]) ** 2))
-        if rms == 0.0:
-            continue
-        l2.append(np.sqrt(np.mean(np.abs(g[k] - r[k]) ** 2)) / rms)
-        m = np.abs(r[k]) >= 0.5 * rms
-        amp.append(np.max(np.abs(np.abs(g[k][m]) - np.abs(r[k][m])) / np.abs(r[k][m])))
-        ph.append(np.max(np.abs(np.angle(g[k][m] * np.conj(r[k][m])))))
-    return np.array(l2), np.array(amp), np.array(ph)
-
-
 def report(label, g, r):
     l2, amp, ph = product_errors(g, r)
     say(f"{label}: {len(l2)} products; relative L2 max {l2.max():.3e} median {np.median(l2):.3e}; "
         f"relative amplitude max {amp.max():.3e} median {np.median(amp):.3e}; "
         f"phase max {ph.max():.3e} rad median {np.median(ph):.3e} rad")
-
-
-def fp32_parity(job, hdrs, vis_g, vis_r32, label):
-    """The plain FP32 bars (compare_vis) against the FP32 oracle. A
-    sub-integration of more than 4000 segments takes two FP32 correlators
-    whose FFTs round differently beyond 1e-5 of each other on some channels
-    (the sums' rounding decorrelates; about sqrt(n) 2^-24 of a product's rms
-    per channel). There the GPU must be as accurate as the FP32 restatement
-    of the reference is: against the FP64 oracle its relative L2, amplitude
-    and phase errors at most 1.5 times the FP32 oracle's."""
-    try:
-        compare_vis(vis_g, vis_r32, label=label)
-        return
-    except AssertionError:
-        if job.number_ffts_in_sub_integration() <= 4000:
-            raise
-    r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
-    _, v64 = vis_from_bytes(job, r64["output"])
-    e_gpu = product_errors(vis_g, v64[0])
-    e_ref = product_errors(vis_r32, v64[0])
-    names = ("relative L2", "relative amplitude", "phase (rad)")
-    for n, a, b in zip(names, e_gpu, e_ref):
-        say(f"{label} ({job.number_ffts_in_sub_integration()} segments per sub-integration) vs FP64 oracle, "
-            f"{n}: GPU {a.max():.3e}, FP32 oracle {b.max():.3e}")
-        assert a.max() <= 1.5 * b.max() + 1e-7
 
 
 def run(ctx, job, hdrs, prec=0, pulsar=None, chunk_mib=0):

@@ -351,6 +351,50 @@ def test_cpp_host_adapter(ctx, c0_short, tmp_path):
     assert l2 <= MIXED_BARS[3][0] and worst <= MIXED_BARS[3][1]
 
 
+def product_errors(g, r):
+    """Per product against the reference r: relative L2 error over channels,
+    and on channels with |r| >= 0.5 rms the largest relative amplitude error
+    and phase error (rad). Returns arrays over products (rows)."""
+    g, r = g.astype(np.complex128), r.astype(np.complex128)
+    l2, amp, ph = [], [], []
+    for k in range(r.shape[0]):
+        rms = np.sqrt(np.mean(np.abs(r[k]) ** 2))
+        if rms == 0.0:
+            continue
+        l2.append(np.sqrt(np.mean(np.abs(g[k] - r[k]) ** 2)) / rms)
+        m = np.abs(r[k]) >= 0.5 * rms
+        amp.append(np.max(np.abs(np.abs(g[k][m]) - np.abs(r[k][m])) / np.abs(r[k][m])))
+        ph.append(np.max(np.abs(np.angle(g[k][m] * np.conj(r[k][m])))))
+    return np.array(l2), np.array(amp), np.array(ph)
+
+
+def fp32_parity(job, hdrs, vis_g, vis_r32, label):
+    """The plain FP32 bars (compare_vis) against the FP32 oracle. Where the
+    rounding floor of two FP32 correlators reaches 1e-5 on some checked
+    channel (about sqrt(n) 2^-24 of a product's rms per channel after n
+    segments of a sub-integration, and its largest value grows with the
+    number of channels checked): a sub-integration of more than 4000
+    segments, or more than 2e5 (product, channel) pairs, the GPU must instead
+    be as accurate as the FP32 restatement of the reference is: against the
+    FP64 oracle its relative L2, amplitude and phase errors at most 1.5 times
+    the FP32 oracle's."""
+    try:
+        compare_vis(vis_g, vis_r32, label=label)
+        return
+    except AssertionError:
+        if job.number_ffts_in_sub_integration() <= 4000 and vis_g.shape[0] * vis_g.shape[1] <= 200_000:
+            raise
+    r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
+    _, v64 = vis_from_bytes(job, r64["output"])
+    e_gpu = product_errors(vis_g, v64[0])
+    e_ref = product_errors(vis_r32, v64[0])
+    names = ("relative L2", "relative amplitude", "phase (rad)")
+    for n, a, b in zip(names, e_gpu, e_ref):
+        print(f"{label} ({job.number_ffts_in_sub_integration()} segments per sub-integration, {vis_g.shape[0]} "
+              f"products) vs FP64 oracle, {n}: GPU {a.max():.3e}, FP32 oracle {b.max():.3e}", flush=True)
+        assert a.max() <= 1.5 * b.max() + 1e-7
+
+
 # Mixed-precision error bars against the FP64 oracle, per product: the
 # relative L2 error over channels, ||V - V64|| / ||V64|| (the visibility
 # error relative to the FP64 oracle, at most 1e-3 for FP16), and the
@@ -697,7 +741,38 @@ def test_autotune_then_correlate(ctx):
     compare_vis(vis_g[0], vis_r[0], label="autotuned")
 
 
-@pytest.mark.parametrize("xs_waves", [0, 8])
+@pytest.mark.parametrize("n_stations,nchan", [(32, 256), (24, 1024), (3, 512)])
+def test_tile16_matches_tile64(ctx, n_stations, nchan):
+    """The FP32 X-step on 16-channel blocks holding every stream-pair tile
+    (k_xstep_tile16: 136, 78 and 3 tiles, so three, two and one tiles per
+    slot) gives the same bits as the register tiles on 64-channel blocks
+    (k_xstep_tile): both form the reference's products and sums in its order.
+    Both match the oracle within the FP32 bars (fp32_parity: with 2080
+    products the largest relative error of the FP32 floor reaches 1e-5)."""
+    rng = np.random.default_rng(n_stations)
+    job, hdrs = configs.build_slices(seed=44, n_stations=n_stations, pols=["R", "L"], fs=16_000_000, bw=8_000_000,
+                                     nchan=nchan, seconds=0.03125, t_int_usec=31250, rho=0.05, bits=2,
+                                     sky_freq=1_400_000_000, offsets=list(rng.uniform(-3e-6, 3e-6, n_stations)),
+                                     rates=list(rng.uniform(-2e-7, 2e-7, n_stations)))[0]
+    M = len(job.station_streams)
+    print(f"case tile16 vs tile64: {M} streams, {nchan} channels", flush=True)
+    outs = {}
+    for xs in (1, 2):
+        capi.tune_clear()
+        capi.tune_import(capi.tune_key(nchan, M, 0, False, nchan <= 4096) + f"|0|0|{xs}|0|0\n")
+        try:
+            outs[xs] = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG)
+        finally:
+            capi.tune_clear()
+    assert outs[1] == outs[2]
+    ref = O.run_slice(job, hdrs, lut=LUT_CONFIG)
+    heads_g, vis_g = vis_from_bytes(job, outs[2])
+    heads_r, vis_r = vis_from_bytes(job, ref["output"])
+    assert heads_g == heads_r
+    fp32_parity(job, hdrs, vis_g[0], vis_r[0], f"tile16, {M} streams")
+
+
+@pytest.mark.parametrize("xs_waves", [0, 2, 8])
 def test_station_order_conjugates(ctx, xs_waves):
     """Station numbers out of stream order: crosses list the lower station
     first, so some products are taken the other way round (the register-tile
