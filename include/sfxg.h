@@ -402,8 +402,9 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap);
 /* The tuning table as text, one "shape|fe_threads|chan_wgs|xs_waves|mf_channels|
  * mfp_wgs|mf_ks" line per shape (0: default; a line without the last field
  * is read as mf_ks 0; xs_waves 1 and 2 force the register tiles on 64- and
- * 16-channel blocks, 4 and 8 the per-baseline kernel); *written = bytes
- * needed including the terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear empties
+ * 16-channel blocks, 4 and 8 the per-baseline kernel; with pulsar binning in
+ * FP32, 2 forces the register tiles and 1, 4 or 8 the per-baseline kernel);
+ * *written = bytes needed including the terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear empties
  * the table. */
 int sfxg_tune_export(char *dst, size_t cap, size_t *written);
 int sfxg_tune_import(const char *text);

@@ -137,7 +137,7 @@ struct sfxg_ctx {
   // device buffers
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
       flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, tw_split, window, offsets, samples,
-      chan_scratch, tiles;
+      chan_scratch, tiles, psr_bins;
   // Mask_parameters (control_parameters.h:107-114) and the lag
   // re-channelisation tables built from them (correlation_core.cc:761-824)
   bool mask_normalize = false;
@@ -273,7 +273,7 @@ void sfxg_destroy(sfxg_ctx *ctx) {
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
                   &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
                   &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples, &ctx->chan_scratch,
-                  &ctx->tiles, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out,
+                  &ctx->tiles, &ctx->psr_bins, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out,
                   &ctx->ph_blocks, &ctx->ph_rows, &ctx->uv_tab};
   for (DBuf *b : bufs) b->release();
   for (DBuf &b : ctx->vdif_payload) b.release();
@@ -1135,6 +1135,9 @@ static int prepare_slice(sfxg_ctx *ctx) {
                                      "exceed 8 GiB for a slice longer than one chunk", nc + 1);
     HIPCHK(ctx, ctx->gram.ensure(per_ch * psr_run));
   }
+  // plain FP32 with pulsar binning: the bin of every (segment, channel) of a
+  // chunk, evaluated once before the X-step
+  if (!mixed && ctx->pulsar) HIPCHK(ctx, ctx->psr_bins.ensure((size_t)lp.chunk_segs * (nc + 1) * sizeof(int16_t)));
 
   {
     const std::string key = tune_key(ctx, fused);
@@ -1435,7 +1438,11 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
   } else {
     HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), sstride, ctx->S, nc + 1, ch.seg0, ch.seg1 - ch.seg0,
                                     blf, bls, B, ctx->offsets.as<double>(), lp.start_phase, lp.phase_coef,
-                                    lp.fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins, ctx->vis.as<float2>(),
+                                    lp.fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins,
+                                    ctx->psr_bins.as<int16_t>(), ctx->vis.as<float2>(), ctx->tiles.as<int32_t>(),
+                                    (int)(ctx->h_tiles.size() / 18), M,
+                                    t.xs_waves == 2 ||
+                                        (t.xs_waves == 0 && pulsar_tile16_default(nc + 1, (int)(ctx->h_tiles.size() / 18))),
                                     s));
   }
   return SFXG_OK;
@@ -2078,6 +2085,9 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
   } else if (mixed) {
     xc = mfp_wgs_candidates();
     xname = "mfp_wgs";
+  } else {
+    xc = {0, 2, 4};  // default, register tiles, one thread per (baseline pair, channel)
+    xname = "xs_waves";
   }
   if (xc.size() > 1) {
     float best_ms = -1.0f;
@@ -2085,7 +2095,7 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
     for (int v : xc) {
       Tune t = best;
       if (mixed && !ctx->pulsar) t.mf_channels = v;
-      else if (!ctx->pulsar) t.xs_waves = v;
+      else if (!mixed) t.xs_waves = v;
       else t.mfp_wgs = v;
       float ms = 0.0f;
       const int rc = time_stage(ctx, reps, [&] { return enqueue_xstep(ctx, 0, ctx->stream, t); }, &ms);

@@ -2392,13 +2392,13 @@ __global__ __launch_bounds__(256) void k_fold_combine_cm(const float2 *__restric
 }
 
 // ---------------------------------------------------------------------------
-// k_xstep_pulsar: per segment, every channel's product goes to the bin its
-// pulse phase selects. Runs of equal bins accumulate in registers; at a bin
-// change the running sums are written back and the new bin's sums loaded,
+// k_xstep_pulsar: one thread per (baseline, channel); per segment its
+// product goes to the bin its pulse phase selects (from the chunk's bin
+// table, k_pulsar_bin_table). Runs of equal bins accumulate in registers;
+// at a bin change the running sum is written back and the new bin's loaded,
 // which keeps the reference's per-segment addition order.
 // ---------------------------------------------------------------------------
-constexpr int XB = 2;  // baselines per thread in the pulsar kernel
-constexpr int XWAVES = 4;
+constexpr int XWAVES = 4;  // baselines per workgroup of the pulsar kernel (one per wave)
 
 // Pulse-phase bin of segment sgi in a channel with dispersion offset off_ch
 // (get_phase and the gate test of Correlation_core_pulsar::integration_step,
@@ -2419,60 +2419,207 @@ __device__ __forceinline__ int pulsar_bin(int64_t sgi, double off_ch, double sta
   return 0;
 }
 
+// The bin of every (segment, channel) of a chunk into bins[segment][channel]
+// (pulsar_bin, once per pair instead of once per baseline group).
+__global__ __launch_bounds__(256) void k_pulsar_bin_table(int16_t *__restrict__ bins, int n_ch, int64_t seg_base,
+                                                          int64_t n_segments, const double *__restrict__ offsets,
+                                                          double start_phase, double phase_coef, double fft_duration,
+                                                          double gate_begin, double gate_end, int nbins) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_segments * n_ch) return;
+  const int64_t sl = i / n_ch;
+  const int ch = (int)(i - sl * n_ch);
+  bins[i] = (int16_t)pulsar_bin(seg_base + sl, offsets[ch], start_phase, phase_coef, fft_duration, gate_begin,
+                                gate_end, nbins);
+}
+
 __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
     const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int64_t seg_base,
     int64_t n_segments, const int32_t *__restrict__ bl_first, const int32_t *__restrict__ bl_second,
-    int n_baselines, const double *__restrict__ offsets, double start_phase, double phase_coef,
-    double fft_duration, double gate_begin, double gate_end, int nbins, int n_bg, float2 *__restrict__ acc_out) {
+    int n_baselines, const int16_t *__restrict__ bins, int n_bg, float2 *__restrict__ acc_out) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // (baseline group, channel block), baseline groups innermost: the groups
   // of one channel block read the same spectra and run on one XCD
   const int L = xcd_grouped(blockIdx.x, gridDim.x, n_bg);
   const int ch = (L / n_bg) * 64 + lane;
-  const int b0 = ((L % n_bg) * XWAVES + wave) * XB;
-  if (ch >= n_ch || b0 >= n_baselines) return;
-  const float2 *pa[XB];
-  const float2 *pb[XB];
-  float2 acc[XB];
-  int nb = n_baselines - b0 < XB ? n_baselines - b0 : XB;
-#pragma unroll
-  for (int q = 0; q < XB; q++) {
-    const int b = b0 + (q < nb ? q : 0);
-    pa[q] = spectra + bl_first[b] * stream_stride + ch;
-    pb[q] = spectra + bl_second[b] * stream_stride + ch;
-    acc[q] = make_float2(0.0f, 0.0f);
-  }
-  const double off_ch = offsets[ch];
+  const int b = (L % n_bg) * XWAVES + wave;
+  if (ch >= n_ch || b >= n_baselines) return;
+  const float2 *pa = spectra + bl_first[b] * stream_stride + ch;
+  const float2 *pb = spectra + bl_second[b] * stream_stride + ch;
+  const int16_t *pbin = bins + ch;
+  float2 acc = make_float2(0.0f, 0.0f);
+  float2 *out = acc_out + (int64_t)b * n_ch + ch;
   const int64_t bin_stride = (int64_t)n_baselines * n_ch;
+  // segment i + 1 and i + 2 (spectra and bin) are in flight while segment
+  // i's product is added (the last segment's again past the end, so that
+  // the loads need no branch)
+  auto at = [&](int64_t i) { return i < n_segments ? i : n_segments - 1; };
+  float2 x0 = pa[0], y0 = pb[0];
+  int k0 = pbin[0];
+  float2 x1 = pa[at(1) * seg_stride], y1 = pb[at(1) * seg_stride];
+  int k1 = pbin[at(1) * n_ch];
   int cur = -1;
-  for (int64_t sgi = seg_base; sgi < seg_base + n_segments; sgi++) {
-    const int bin = pulsar_bin(sgi, off_ch, start_phase, phase_coef, fft_duration, gate_begin, gate_end, nbins);
-    if (bin != cur) {
+  for (int64_t i = 0; i < n_segments; i++) {
+    const int64_t i2 = at(i + 2);
+    const float2 x2 = pa[i2 * seg_stride], y2 = pb[i2 * seg_stride];
+    const int k2 = pbin[i2 * n_ch];
+    if (k0 != cur) {
+      if (cur >= 0) out[cur * bin_stride] = acc;
+      acc = out[k0 * bin_stride];
+      cur = k0;
+    }
+    const float pr = x0.x * y0.x + x0.y * y0.y;
+    const float pi = x0.y * y0.x - x0.x * y0.y;
+    acc.x = acc.x + pr;
+    acc.y = acc.y + pi;
+    x0 = x1;
+    y0 = y1;
+    k0 = k1;
+    x1 = x2;
+    y1 = y2;
+    k1 = k2;
+  }
+  if (cur >= 0) out[cur * bin_stride] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// k_xstep_pulsar_tile16: k_xstep_pulsar on the register tiles of
+// k_xstep_tile16 (16 channels per workgroup, every stream-pair tile held at
+// once, spectra staged through LDS once per workgroup). The stage also
+// holds its segments' bins of the 16 channels. A lane keeps the running sums
+// of its tiles for its channel's current bin; where the bin changes they are
+// written back to acc_out[bin] and the new bin's sums loaded, so every
+// (bin, baseline, channel) still adds its products one segment after the
+// other, as the reference does.
+// ---------------------------------------------------------------------------
+template <int TPS>
+__global__ __launch_bounds__(64 * XQ_WAVES) void k_xstep_pulsar_tile16(
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int64_t seg_base,
+    int64_t n_segments, int M, const int32_t *__restrict__ tiles, int n_tiles, int n_baselines,
+    const int16_t *__restrict__ bins, int sps, int n_cb, int n_tg, float2 *__restrict__ acc_out) {
+#pragma clang fp contract(off)
+  extern __shared__ float4 xq[];  // [sps][M][8] float4, then [sps][16] bins (int)
+  const float2 *xq2 = reinterpret_cast<const float2 *>(xq);
+  int *bl = reinterpret_cast<int *>(xq + (size_t)sps * M * 8);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = lane & 15;
+  const int slot = wave * 4 + (lane >> 4);
+  // (tile group, channel block), tile groups innermost
+  const int L = xcd_grouped(blockIdx.x, gridDim.x, n_tg);
+  const int tg = L % n_tg, cb = L / n_tg;
+  const int ch0 = cb * 16;
+  const int ch = ch0 + c;
+  const bool own = ch < n_ch;  // lanes past the last channel never touch acc_out
+  int a0[TPS], b0[TPS];
+  bool act[TPS];
 #pragma unroll
-      for (int q = 0; q < XB; q++) {
-        if (q < nb) {
-          if (cur >= 0) acc_out[cur * bin_stride + (int64_t)(b0 + q) * n_ch + ch] = acc[q];
-          acc[q] = acc_out[bin * bin_stride + (int64_t)(b0 + q) * n_ch + ch];
+  for (int j = 0; j < TPS; j++) {
+    const int t = (tg * TPS + j) * XQ_SLOTS + slot;
+    act[j] = t < n_tiles;
+    a0[j] = act[j] ? tiles[t * (2 + XT * XT)] : 0;
+    b0[j] = act[j] ? tiles[t * (2 + XT * XT) + 1] : 0;
+  }
+  const int64_t bin_stride = (int64_t)n_baselines * n_ch;
+  float2 acc[TPS][XT * XT];
+  // running sums of the current bin to acc_out, or from it
+  auto sums = [&](int bin, bool store) {
+    float2 *o = acc_out + bin * bin_stride + ch;
+#pragma unroll
+    for (int j = 0; j < TPS; j++) {
+      if (!act[j]) continue;
+      const int t = (tg * TPS + j) * XQ_SLOTS + slot;
+#pragma unroll
+      for (int q = 0; q < XT * XT; q++) {
+        const int e = tiles[t * (2 + XT * XT) + 2 + q];
+        if (e == XT_NONE) continue;
+        float2 *p = o + (int64_t)(e >= 0 ? e : ~e) * n_ch;
+        if (store) {
+          *p = e >= 0 ? acc[j][q] : make_float2(acc[j][q].x, -acc[j][q].y);
+        } else {
+          const float2 v = *p;
+          acc[j][q] = e >= 0 ? v : make_float2(v.x, -v.y);
         }
       }
-      cur = bin;
     }
-    const int64_t off = (sgi - seg_base) * seg_stride;
+  };
 #pragma unroll
-    for (int q = 0; q < XB; q++) {
-      const float2 x = pa[q][off], y = pb[q][off];
-      const float pr = x.x * y.x + x.y * y.y;
-      const float pi = x.y * y.x - x.x * y.y;
-      acc[q].x = acc[q].x + pr;
-      acc[q].y = acc[q].y + pi;
+  for (int j = 0; j < TPS; j++)
+#pragma unroll
+    for (int q = 0; q < XT * XT; q++) acc[j][q] = make_float2(0.0f, 0.0f);
+  const int row_items = M * 8;
+  float4 pf[XQ_PF];
+  int pb = 0;
+  auto fetch = [&](int64_t s0, int ns) {
+    const int total = ns * row_items;
+#pragma unroll
+    for (int v = 0; v < XQ_PF; v++) {
+      const int idx = threadIdx.x + v * (64 * XQ_WAVES);
+      float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (idx < total) {
+        const int sl = idx / row_items, r = idx - sl * row_items;
+        const int st = r >> 3, q = r & 7;
+        x = *reinterpret_cast<const float4 *>(spectra + st * stream_stride + (s0 + sl) * seg_stride + ch0 + 2 * q);
+      }
+      pf[v] = x;
     }
-  }
-  if (cur >= 0) {
+    pb = 0;
+    if ((int)threadIdx.x < ns * 16) {
+      const int sl = threadIdx.x >> 4, cc = threadIdx.x & 15;
+      pb = ch0 + cc < n_ch ? bins[(s0 + sl) * n_ch + ch0 + cc] : 0;
+    }
+  };
+  int cur = -1;
+  int64_t s0 = 0;
+  int ns = (int)(n_segments < sps ? n_segments : sps);
+  if (s0 < n_segments) fetch(s0, ns);
+  while (s0 < n_segments) {
+    const int total = ns * row_items;
 #pragma unroll
-    for (int q = 0; q < XB; q++)
-      if (q < nb) acc_out[cur * bin_stride + (int64_t)(b0 + q) * n_ch + ch] = acc[q];
+    for (int v = 0; v < XQ_PF; v++) {
+      const int idx = threadIdx.x + v * (64 * XQ_WAVES);
+      if (idx < total) xq[idx] = pf[v];
+    }
+    if ((int)threadIdx.x < ns * 16) bl[threadIdx.x] = pb;
+    __syncthreads();
+    const int64_t s1 = s0 + ns;
+    const int ns1 = (int)((n_segments - s1) < sps ? (n_segments - s1) : sps);
+    if (s1 < n_segments) fetch(s1, ns1);
+#pragma unroll 1
+    for (int sl = 0; sl < ns; sl++) {
+      const int bin = bl[sl * 16 + c];
+      if (bin != cur && own) {
+        if (cur >= 0) sums(cur, true);
+        sums(bin, false);
+        cur = bin;
+      }
+      const float2 *row = xq2 + sl * M * 16 + c;
+#pragma unroll
+      for (int j = 0; j < TPS; j++) {
+        if (!act[j]) continue;
+        float2 x[XT], y[XT];
+#pragma unroll
+        for (int i = 0; i < XT; i++) {
+          x[i] = a0[j] + i < M ? row[(a0[j] + i) * 16] : make_float2(0.0f, 0.0f);
+          y[i] = b0[j] + i < M ? row[(b0[j] + i) * 16] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int i = 0; i < XT; i++)
+#pragma unroll
+          for (int m = 0; m < XT; m++) {
+            const float pr = x[i].x * y[m].x + x[i].y * y[m].y;
+            const float pi = x[i].y * y[m].x - x[i].x * y[m].y;
+            acc[j][XT * i + m].x = acc[j][XT * i + m].x + pr;
+            acc[j][XT * i + m].y = acc[j][XT * i + m].y + pi;
+          }
+      }
+    }
+    __syncthreads();
+    s0 = s1;
+    ns = ns1;
   }
+  if (own && cur >= 0) sums(cur, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -3000,6 +3147,15 @@ void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::ve
 
 bool xstep_tile16_default(int n_tiles) { return n_tiles > 64; }
 
+// The pulsar X-step holds its running sums for the whole chunk, so its
+// register tiles parallelise over channel blocks and tile groups only: they
+// are the default where that gives at least two workgroups per CU.
+bool pulsar_tile16_default(int n_ch, int n_tiles) {
+  const int TPS = n_tiles <= XQ_SLOTS ? 1 : n_tiles <= 2 * XQ_SLOTS ? 2 : 3;
+  const int n_tg = (n_tiles + TPS * XQ_SLOTS - 1) / (TPS * XQ_SLOTS);
+  return (int64_t)((n_ch + 15) / 16) * n_tg >= 512;
+}
+
 hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
                              const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float2 *partial,
@@ -3239,14 +3395,45 @@ hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int
                                int64_t seg_base, int64_t n_segments, const int32_t *bl_first,
                                const int32_t *bl_second, int n_baselines, const double *offsets, double start_phase,
                                double phase_coef, double fft_duration, double gate_begin, double gate_end, int nbins,
-                               float2 *acc, hipStream_t s) {
+                               int16_t *bins, float2 *acc, const int32_t *tiles, int n_tiles, int M, bool tile16,
+                               hipStream_t s) {
   if (n_segments <= 0) return hipSuccess;
-  const int n_cb = (n_ch + 63) / 64, n_bg = (n_baselines + XB * XWAVES - 1) / (XB * XWAVES);
+  const int64_t n_pairs = n_segments * n_ch;
+  hipLaunchKernelGGL(k_pulsar_bin_table, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0, s, bins, n_ch,
+                     seg_base, n_segments, offsets, start_phase, phase_coef, fft_duration, gate_begin, gate_end, nbins);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (tile16) {
+    // spectra of the chunk from its first row: the kernel counts segments from 0
+    const float2 *sp = spectra;
+    int sps = (XQ_PF * 64 * XQ_WAVES) / (M * 8);
+    if (sps > 8) sps = 8;
+    if (sps < 1 || stream_stride % 2 || seg_stride % 64) return hipErrorInvalidValue;
+    const size_t lds = (size_t)sps * M * 8 * sizeof(float4) + (size_t)sps * 16 * sizeof(int);
+    const int TPS = n_tiles <= XQ_SLOTS ? 1 : n_tiles <= 2 * XQ_SLOTS ? 2 : 3;
+    const int n_cb = (n_ch + 15) / 16, n_tg = (n_tiles + TPS * XQ_SLOTS - 1) / (TPS * XQ_SLOTS);
+    if ((int64_t)n_cb * n_tg >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(n_cb * n_tg));
+#define SFXG_XQP(T)                                                                                                 \
+  do {                                                                                                              \
+    e = hipFuncSetAttribute((const void *)k_xstep_pulsar_tile16<T>, hipFuncAttributeMaxDynamicSharedMemorySize,       \
+                            (int)lds);                                                                              \
+    if (e != hipSuccess) return e;                                                                                  \
+    hipLaunchKernelGGL(k_xstep_pulsar_tile16<T>, grid, dim3(64 * XQ_WAVES), lds, s, sp, stream_stride, seg_stride,  \
+                       n_ch, seg_base, n_segments, M, tiles, n_tiles, n_baselines, bins, sps, n_cb, n_tg, acc);    \
+    e = hipGetLastError();                                                                                          \
+  } while (0)
+    if (TPS == 1) SFXG_XQP(1);
+    else if (TPS == 2) SFXG_XQP(2);
+    else SFXG_XQP(3);
+#undef SFXG_XQP
+    return e;
+  }
+  const int n_cb = (n_ch + 63) / 64, n_bg = (n_baselines + XWAVES - 1) / XWAVES;
   if ((int64_t)n_cb * n_bg >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
   dim3 grid((unsigned)(n_cb * n_bg));
   hipLaunchKernelGGL(k_xstep_pulsar, grid, dim3(64 * XWAVES), 0, s, spectra, stream_stride, seg_stride, n_ch,
-                     seg_base, n_segments, bl_first, bl_second, n_baselines, offsets, start_phase, phase_coef,
-                     fft_duration, gate_begin, gate_end, nbins, n_bg, acc);
+                     seg_base, n_segments, bl_first, bl_second, n_baselines, bins, n_bg, acc);
   return hipGetLastError();
 }
 
