@@ -47,13 +47,13 @@ MEASURED_TRAFFIC = {
     ("channelise", 16384, 64, "fp16"): 16513000000,
 }
 # Wave-level VALU instructions per launch (SQ_INSTS_VALU, same profiles),
-# against the wave64 FP32 issue rate: 256 CUs x 1 instruction per cycle (a
-# wave64 instruction occupies a 16-lane SIMD for 4 cycles, 4 SIMDs per CU)
-# x 2.4 GHz.
+# against the wave64 FP32 issue rate: a gfx950 SIMD issues up to two
+# independent VALU instructions per quad-cycle (SQ_ACTIVE_INST_VALU2 counts
+# those), so 256 CUs x 4 SIMDs x 2 / 4 cycles x 2.4 GHz.
 MEASURED_VALU_INSTS = {
     ("channelise", 16384, 64, "fp16"): 3.075e9,
 }
-VALU_ISSUE_PEAK = 256 * 2.4e9
+VALU_ISSUE_PEAK = 256 * 4 * 2 / 4 * 2.4e9
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "split_fp16": 3}
 C2_SUBBANDS, C2_INTEGRATIONS = 4, 8
@@ -210,7 +210,7 @@ def achievable_peaks(torch, dev):
     if rc != 0:
         raise RuntimeError(f"sfxg_peak_mfma_fp16 failed: {rc}")
     vi = C.c_double()
-    rc = lib.sfxg_peak_valu(20000, 16, C.byref(vi))
+    rc = lib.sfxg_peak_valu(20000, 32, C.byref(vi))
     if rc != 0:
         raise RuntimeError(f"sfxg_peak_valu failed: {rc}")
     return {"hbm_copy_GBs": round(2 * n / (ms.value * 1e-3) / 1e9, 1), "copy_bytes": n,

@@ -1,8 +1,10 @@
 // Achievable-peak microbenchmarks for the roofline fractions the benchmark
 // reports beside the vendor peaks: a streaming copy (HBM read + write), a
 // dense FP16 MFMA loop (v_mfma_f32_32x32x16_f16, FP32 accumulation) and an
-// FP32 VALU loop (v_fma_f32, not packed: the issue rate of wave-level VALU
-// instructions the correlator's FFTs and FP32 X-step run at). Built
+// FP32 VALU loop (v_mul_f32 / v_add_f32, not packed: the issue rate of
+// wave-level VALU instructions the correlator's FFTs and FP32 X-step run at;
+// a gfx950 SIMD can issue two independent VALU instructions per quad-cycle).
+// Built
 // into its own library (libsfxg_peaks.so); not part of the correlator ABI.
 #include <stdint.h>
 
@@ -42,16 +44,17 @@ __global__ __launch_bounds__(256) void k_mfma(int iters, float *out) {
   out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
-// Eight independent FMA chains per thread, so that a SIMD always has a
-// ready instruction from one of its waves; the sum is stored so the loop is
-// not removed.
+// Eight independent chains per thread of a multiply and an add each (16
+// VALU instructions per iteration), so that a SIMD always has independent
+// ready instructions; the sum is stored so the loop is not removed.
 __global__ __launch_bounds__(256) void k_valu(int iters, float *out) {
+#pragma clang fp contract(off)
   float a[8];
   const float b = 0.999f + 1e-7f * threadIdx.x, c = 1e-4f;
   for (int k = 0; k < 8; k++) a[k] = 0.001f * (threadIdx.x + k);
   for (int it = 0; it < iters; it++) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) a[k] = __builtin_fmaf(a[k], b, c);
+    for (int k = 0; k < 8; k++) a[k] = a[k] * b + c;
   }
   float s = 0.0f;
   for (int k = 0; k < 8; k++) s += a[k];
@@ -122,8 +125,8 @@ int sfxg_peak_mfma_fp16(int iters, int waves_per_cu, double *tflops) {
   return 0;
 }
 
-// Runs the FMA loop on `waves_per_cu` waves per CU; *insts_per_s = wave-level
-// FP32 VALU instructions per second (8 per wave and iteration).
+// Runs the multiply-add loop on `waves_per_cu` waves per CU; *insts_per_s =
+// wave-level FP32 VALU instructions per second (16 per wave and iteration).
 int sfxg_peak_valu(int iters, int waves_per_cu, double *insts_per_s) {
   if (iters <= 0 || waves_per_cu <= 0 || waves_per_cu % 4 != 0 || insts_per_s == nullptr) return -1;
   int dev = 0, cus = 0;
@@ -147,7 +150,7 @@ int sfxg_peak_valu(int iters, int waves_per_cu, double *insts_per_s) {
   PK_CHECK(hipFree(out));
   if (hipGetLastError() != hipSuccess || t <= 0.0f) return -3;
   const double waves = (double)grid.x * 4;
-  *insts_per_s = waves * iters * 8.0 / (t * 1e-3);
+  *insts_per_s = waves * iters * 16.0 / (t * 1e-3);
   return 0;
 }
 
