@@ -1052,7 +1052,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->payload.ensure((size_t)pay_tot));
   HIPCHK(ctx, ctx->runs.ensure((size_t)run_tot * sizeof(Run)));
   HIPCHK(ctx, ctx->counted.ensure((size_t)std::max<int64_t>(cnt_tot, 1) * sizeof(CountRange)));
-  HIPCHK(ctx, ctx->block_run.ensure((size_t)M * ctx->n_blocks * sizeof(int32_t)));
+  HIPCHK(ctx, ctx->block_run.ensure((size_t)M * ctx->n_blocks * sizeof(int64_t)));
   HIPCHK(ctx, ctx->block_consts.ensure((size_t)M * ctx->n_blocks * sizeof(BlockConst)));
   HIPCHK(ctx, ctx->model_buf.ensure((size_t)mod_tot * sizeof(double)));
   HIPCHK(ctx, ctx->dstreams.ensure((size_t)M * sizeof(DevStream)));
@@ -1175,7 +1175,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
     memset(&d, 0, sizeof(d));
     d.payload = pl.ext_payload ? (const uint8_t *)pl.ext_payload : ctx->payload.as<uint8_t>() + pay_off[i];
     d.runs = ctx->runs.as<Run>() + run_off[i];
-    d.block_run = ctx->block_run.as<int32_t>() + blk_off[i];  // filled by k_block_consts
+    d.block_src = ctx->block_run.as<int64_t>() + blk_off[i];  // filled by k_block_consts
     d.counted = ctx->counted.as<CountRange>() + cnt_off[i];
     d.n_runs = (int64_t)pl.runs.size();
     d.n_counted = (int64_t)pl.counted.size();

@@ -160,7 +160,8 @@ struct BlockConst {
 struct DevStream {
   const uint8_t *payload;
   const Run *runs;
-  int32_t *block_run;        // index of the run holding the first sample of each N_d block
+  int64_t *block_src;        // per N_d block: the payload sample of its first sample where the
+                             // block lies in one payload run, else ~(index of the run holding it)
   const CountRange *counted;
   int64_t n_runs;
   int64_t n_counted;

@@ -379,14 +379,16 @@ __global__ __launch_bounds__(256) void k_block_consts(const DevStream *__restric
   const int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (blk >= n_blocks) return;
   {
-    // index of the run holding the block's first sample
+    // the run holding the block's first sample: the payload sample behind
+    // it where the whole block lies in that payload run, else ~run index
     const int64_t o = blk * nd;
     int64_t lo = 0, hi = s.n_runs - 1;
     while (lo < hi) {
       const int64_t mid = (lo + hi + 1) >> 1;
       if (s.runs[mid].out_start <= o) lo = mid; else hi = mid - 1;
     }
-    s.block_run[blk] = (int32_t)lo;
+    const Run &run = s.runs[lo];
+    s.block_src[blk] = run.src >= 0 && run.out_start + run.count >= o + nd ? run.src + (o - run.out_start) : ~lo;
   }
   const int64_t current = s.start_ticks + blk * nd * s.tps;
   const double delay = eval_model(s.model, s.model.delay, current + s.fl_half, 0.0);
@@ -453,11 +455,9 @@ __device__ __forceinline__ void load_block(const DevStream &s, int64_t blk, int6
     return;
   }
   const int64_t o0 = blk * nd;
-  int64_t r = cmem(s.block_run)[blk];
-  const auto *run = cmem(s.runs) + r;
-  const int64_t out_start = run->out_start, src = run->src, count = run->count;
-  if (src >= 0 && out_start + count >= o0 + nd) {
-    const int64_t p0 = src + (o0 - out_start);
+  const int64_t bs = cmem(s.block_src)[blk];
+  if (bs >= 0) {
+    const int64_t p0 = bs;
     const auto *pay = gmem(s.payload);
     uint32_t c[4];
     if (s.bits == 2) {
@@ -480,6 +480,7 @@ __device__ __forceinline__ void load_block(const DevStream &s, int64_t blk, int6
       for (int q = 0; q < 4; q++) x[q] = ((c[q] >> sh) & 1) ? l1 : l0;
     }
   } else {
+    int64_t r = ~bs;
 #pragma unroll
     for (int q = 0; q < 4; q++) x[q] = sample_at(s, o0 + lane + 64 * q, r);
   }
@@ -892,11 +893,9 @@ __device__ __forceinline__ void load_block16(const DevStream &s, int64_t blk, in
     return;
   }
   const int64_t o0 = blk * nd;
-  int64_t r = cmem(s.block_run)[blk];
-  const auto *run = cmem(s.runs) + r;
-  const int64_t out_start = run->out_start, src = run->src, count = run->count;
-  if (src >= 0 && out_start + count >= o0 + nd) {
-    const int64_t p0 = src + (o0 - out_start);
+  const int64_t bs = cmem(s.block_src)[blk];
+  if (bs >= 0) {
+    const int64_t p0 = bs;
     const auto *pay = gmem(s.payload);
     uint32_t c[16];
     if (s.bits == 2) {
@@ -919,6 +918,7 @@ __device__ __forceinline__ void load_block16(const DevStream &s, int64_t blk, in
       for (int q = 0; q < 16; q++) x[q] = ((c[q] >> sh) & 1) ? l1 : l0;
     }
   } else {
+    int64_t r = ~bs;
 #pragma unroll
     for (int q = 0; q < 16; q++) x[q] = sample_at(s, o0 + 16 * q + l, r);
   }
