@@ -180,7 +180,8 @@ def cpu_baseline(sample_usec, cores):
     ssn, walln = _cpu_run(cores, sample_usec)
     return {"value": ssn / walln / 1e9, "unit": "station-Gsamples/s", "cores": cores, "kind": "port",
             "single_core_value": ss1 / wall1 / 1e9, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
-            "fft_backend": "the oracle's own iterative radix-2 FFT (no FFTW in this image)",
+            "fft_backend": "the oracle's own radix-4 Stockham FFT with per-stage twiddle tables (no FFTW or "
+                           "MKL headers in this image)",
             "sample": f"{cores} worker processes, each one {sample_usec / 1e6:g} s slice of one subband of the "
                       f"32-station workload (64 streams, 16384 channels, {ss1:.3g} station-samples per worker), "
                       f"FP32 oracle; {walln:.1f} s wall for all, {wall1:.1f} s for one slice on one core"}

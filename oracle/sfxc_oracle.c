@@ -310,7 +310,8 @@ void FN(or_bit_statistics)(int bits_per_sample, int64_t sample_rate, int64_t bas
 typedef struct {
   int n, sign;
   cplx *tw;   /* n/2 twiddles exp(sign 2 pi i k / n), computed in double */
-  int *rev;   /* bit reversal permutation */
+  cplx *st;   /* per radix-4 stage of length m = n, n/4, ...: w^p, w^2p, w^3p (p < m/4), w = exp(sign 2 pi i / m) */
+  cplx *work; /* the second buffer of the autosort passes */
 } fft_plan;
 
 /* Per-thread plan cache: the Python driver runs independent calls on
@@ -322,52 +323,91 @@ static fft_plan *get_plan(int n, int sign) {
   for (int i = 0; i < n_plans; i++)
     if (plans[i].n == n && plans[i].sign == sign) return &plans[i];
   if (n_plans == 64) { /* evict everything; plans are cheap to rebuild */
-    for (int i = 0; i < n_plans; i++) { free(plans[i].tw); free(plans[i].rev); }
+    for (int i = 0; i < n_plans; i++) { free(plans[i].tw); free(plans[i].st); free(plans[i].work); }
     n_plans = 0;
   }
   fft_plan *pl = &plans[n_plans++];
   pl->n = n;
   pl->sign = sign;
   pl->tw = (cplx *)malloc(sizeof(cplx) * (n / 2 > 0 ? n / 2 : 1));
-  pl->rev = (int *)malloc(sizeof(int) * n);
   for (int k = 0; k < n / 2; k++) {
     double a = sign * 2.0 * M_PI * k / n;
     pl->tw[k].re = (real)cos(a);
     pl->tw[k].im = (real)sin(a);
   }
-  int lg = 0;
-  while ((1 << lg) < n) lg++;
-  for (int i = 0; i < n; i++) {
-    int r = 0;
-    for (int b = 0; b < lg; b++) r |= ((i >> b) & 1) << (lg - 1 - b);
-    pl->rev[i] = r;
+  /* stage tables: 3 m/4 entries for each stage length m >= 4 */
+  size_t tot = 0;
+  for (int m = n; m >= 4; m /= 4) tot += 3 * (size_t)(m / 4);
+  pl->st = (cplx *)malloc(sizeof(cplx) * (tot > 0 ? tot : 1));
+  cplx *t = pl->st;
+  for (int m = n; m >= 4; m /= 4) {
+    for (int p = 0; p < m / 4; p++)
+      for (int r = 1; r <= 3; r++) {
+        double a = sign * 2.0 * M_PI * (double)r * p / m;
+        t[3 * p + r - 1].re = (real)cos(a);
+        t[3 * p + r - 1].im = (real)sin(a);
+      }
+    t += 3 * (m / 4);
   }
+  pl->work = (cplx *)malloc(sizeof(cplx) * (n > 0 ? n : 1));
   return pl;
 }
 
-/* In-place iterative radix-2 complex FFT; n a power of two. */
+static inline cplx cmulr(cplx a, cplx b) {
+  cplx r;
+  r.re = a.re * b.re - a.im * b.im;
+  r.im = a.re * b.im + a.im * b.re;
+  return r;
+}
+
+/* Stockham autosort FFT, radix 4 with a last radix-2 pass when log2 n is odd
+ * (no bit reversal): a pass over sub-transforms of length m at stride s
+ * reads x[q + s (p + j m/4)] (j < 4) and writes the butterfly's outputs,
+ * times w^(jp), to y[q + s (4 p + j)]; the next pass has length m/4 and
+ * stride 4 s with the buffers swapped. out_in_x: whether the result must end
+ * in x. */
+static void fft_pass(cplx *x, cplx *y, int m, int s, int out_in_x, int sign, const cplx *st) {
+  if (m == 1) {
+    if (!out_in_x) memcpy(y, x, sizeof(cplx) * (size_t)s);
+    return;
+  }
+  cplx *z = out_in_x ? x : y;
+  if (m == 2) {
+    for (int q = 0; q < s; q++) {
+      cplx a = x[q], b = x[q + s];
+      z[q].re = a.re + b.re; z[q].im = a.im + b.im;
+      z[q + s].re = a.re - b.re; z[q + s].im = a.im - b.im;
+    }
+    return;
+  }
+  const int m4 = m / 4;
+  for (int p = 0; p < m4; p++) {
+    const cplx w1 = st[3 * p], w2 = st[3 * p + 1], w3 = st[3 * p + 2];
+    for (int q = 0; q < s; q++) {
+      const cplx a = x[q + s * p], b = x[q + s * (p + m4)], c = x[q + s * (p + 2 * m4)], d = x[q + s * (p + 3 * m4)];
+      cplx apc = {a.re + c.re, a.im + c.im}, amc = {a.re - c.re, a.im - c.im};
+      cplx bpd = {b.re + d.re, b.im + d.im};
+      /* jbmd = -sign i (b - d): i (b - d) for the forward transform */
+      cplx bmd = {b.re - d.re, b.im - d.im};
+      cplx jbmd = sign < 0 ? (cplx){-bmd.im, bmd.re} : (cplx){bmd.im, -bmd.re};
+      cplx y0 = {apc.re + bpd.re, apc.im + bpd.im};
+      cplx y1 = {amc.re - jbmd.re, amc.im - jbmd.im};
+      cplx y2 = {apc.re - bpd.re, apc.im - bpd.im};
+      cplx y3 = {amc.re + jbmd.re, amc.im + jbmd.im};
+      y[q + s * (4 * p)] = y0;
+      y[q + s * (4 * p + 1)] = cmulr(y1, w1);
+      y[q + s * (4 * p + 2)] = cmulr(y2, w2);
+      y[q + s * (4 * p + 3)] = cmulr(y3, w3);
+    }
+  }
+  fft_pass(y, x, m4, 4 * s, !out_in_x, sign, st + 3 * m4);
+}
+
+/* In-place complex FFT; n a power of two. */
 static void fft_inplace(cplx *x, int n, int sign) {
   if (n <= 1) return;
   fft_plan *pl = get_plan(n, sign);
-  for (int i = 0; i < n; i++) {
-    int r = pl->rev[i];
-    if (r > i) { cplx t = x[i]; x[i] = x[r]; x[r] = t; }
-  }
-  for (int len = 2; len <= n; len <<= 1) {
-    int h = len / 2, step = n / len;
-    for (int i = 0; i < n; i += len) {
-      for (int k = 0; k < h; k++) {
-        cplx w = pl->tw[k * step];
-        cplx a = x[i + k], b = x[i + k + h];
-        real tr = b.re * w.re - b.im * w.im;
-        real ti = b.re * w.im + b.im * w.re;
-        x[i + k].re = a.re + tr;
-        x[i + k].im = a.im + ti;
-        x[i + k + h].re = a.re - tr;
-        x[i + k + h].im = a.im - ti;
-      }
-    }
-  }
+  fft_pass(x, pl->work, n, 1, 1, sign, pl->st);
 }
 
 void FN(or_fft_c2c)(real *data, int n, int sign) { fft_inplace((cplx *)data, n, sign); }
