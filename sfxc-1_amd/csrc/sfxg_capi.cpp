@@ -137,7 +137,7 @@ struct sfxg_ctx {
   // device buffers
   DBuf payload, runs, block_run, counted, dstreams, time_series, spectra, vis, norm_out, hist, norms,
       flags, bl, fold_buf, partial, gram, block_consts, model_buf, tw_nd, tw_n, tw_2n, tw_split, window, offsets, samples,
-      chan_scratch, tiles, psr_bins;
+      chan_scratch, tiles, psr_bins, psr_masks;
   // Mask_parameters (control_parameters.h:107-114) and the lag
   // re-channelisation tables built from them (correlation_core.cc:761-824)
   bool mask_normalize = false;
@@ -273,7 +273,7 @@ void sfxg_destroy(sfxg_ctx *ctx) {
                   &ctx->time_series, &ctx->spectra, &ctx->vis, &ctx->norm_out, &ctx->hist,
                   &ctx->norms, &ctx->flags, &ctx->bl, &ctx->fold_buf, &ctx->partial, &ctx->gram, &ctx->block_consts, &ctx->model_buf, &ctx->tw_nd,
                   &ctx->tw_n, &ctx->tw_2n, &ctx->tw_split, &ctx->window, &ctx->offsets, &ctx->samples, &ctx->chan_scratch,
-                  &ctx->tiles, &ctx->psr_bins, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out,
+                  &ctx->tiles, &ctx->psr_bins, &ctx->psr_masks, &ctx->rc_tw, &ctx->rc_mask, &ctx->rc_window_buf, &ctx->chan_out,
                   &ctx->ph_blocks, &ctx->ph_rows, &ctx->uv_tab};
   for (DBuf *b : bufs) b->release();
   for (DBuf &b : ctx->vdif_payload) b.release();
@@ -1135,9 +1135,14 @@ static int prepare_slice(sfxg_ctx *ctx) {
                                      "exceed 8 GiB for a slice longer than one chunk", nc + 1);
     HIPCHK(ctx, ctx->gram.ensure(per_ch * psr_run));
   }
-  // plain FP32 with pulsar binning: the bin of every (segment, channel) of a
-  // chunk, evaluated once before the X-step
-  if (!mixed && ctx->pulsar) HIPCHK(ctx, ctx->psr_bins.ensure((size_t)lp.chunk_segs * (nc + 1) * sizeof(int16_t)));
+  // pulsar binning: the bin of every (segment, channel) of a chunk, evaluated
+  // once before the X-step; with mixed precision also the bins present in
+  // each stage of each channel block
+  if (ctx->pulsar) HIPCHK(ctx, ctx->psr_bins.ensure((size_t)lp.chunk_segs * (nc + 1) * sizeof(int16_t)));
+  if (mixed && ctx->pulsar) {
+    const size_t words = pulsar_bin_mask_words(ctx->opt.precision, M, psr_run, ctx->nbins, lp.chunk_segs);
+    HIPCHK(ctx, ctx->psr_masks.ensure(std::max<size_t>(words, 1) * sizeof(uint32_t)));
+  }
 
   {
     const std::string key = tune_key(ctx, fused);
@@ -1430,7 +1435,8 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
       HIPCHK(ctx, launch_xstep_mfma_pulsar(ctx->opt.precision, ctx->spectra.p, lp.job.spec_prec != 0, sstride, ctx->S, c0, n_run,
                                            M, ch.seg0, ch.seg1 - ch.seg0, lp.mixed_scale, ctx->offsets.as<double>(),
                                            lp.start_phase, lp.phase_coef, lp.fft_duration, ps.gate_start,
-                                           ps.gate_stop, ctx->nbins, ctx->gram.as<float>(), t, s));
+                                           ps.gate_stop, ctx->nbins, ctx->gram.as<float>(),
+                                           ctx->psr_bins.as<int16_t>(), ctx->psr_masks.as<uint32_t>(), t, s));
       if (c + 1 == lp.chunks.size() || lp.psr_run < nc + 1)
         HIPCHK(ctx, pulsar_gram_baselines(ctx->gram.as<float>(), nc + 1, c0, n_run, M, ctx->nbins, blf, bls, B,
                                           lp.mixed_scale, ctx->vis.as<float2>(), s));

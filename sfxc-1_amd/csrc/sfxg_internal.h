@@ -222,7 +222,7 @@ struct Tune {
                         // per-baseline kernel with this many waves per workgroup
   int mf_channels = 0;  // MFMA X-step: at most this many channels per workgroup
   int mf_ks = 0;        // MFMA X-step: segments per LDS stage (16 or 32)
-  int mfp_wgs = 0;      // MFMA pulsar X-step: workgroups in all
+  int mfp_wgs = 0;      // MFMA pulsar X-step: 0 by bin, else the time-sliced kernel's workgroups
 };
 // Candidates of each knob for a shape (the default first).
 std::vector<int> fe_thread_candidates(int nc);
@@ -307,7 +307,11 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const void *spectra, bool hal
                                     int ch_base, int n_ch_run, int M, int64_t seg_base, int64_t n_segments,
                                     float scale, const double *offsets, double start_phase, double phase_coef,
                                     double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
-                                    const Tune &t, hipStream_t s);
+                                    int16_t *bins, uint32_t *masks, const Tune &t, hipStream_t s);
+// 32-bit words of stage bin masks the by-bin kernel needs for a chunk of
+// n_segments (0: the shape does not fit it); its bin table takes
+// n_segments x n_ch_run int16.
+size_t pulsar_bin_mask_words(int precision, int M, int n_ch_run, int nbins, int64_t n_segments);
 hipError_t pulsar_gram_baselines(const float *gram, int n_ch, int ch_base, int n_ch_run, int M, int nbins,
                                  const int32_t *bl_first, const int32_t *bl_second, int n_baselines, float scale,
                                  float2 *vis, hipStream_t s);

@@ -2755,6 +2755,143 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar(
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_xstep_mfma_pulsar_bin: the mixed-precision X-step with pulsar binning by
+// bin. A workgroup takes one bin of one channel block (the off-gate bin 0,
+// which holds most segments, in MFB_Z0 parts of the chunk) and walks only
+// the 16-segment stages where that bin occurs: the stages come from the
+// per-stage bin masks (k_pulsar_stage_masks), compacted a window at a time,
+// and each channel's segments of the bin from the bin table
+// (k_pulsar_bin_table). The Gram tiles of its (channel, tile) pairs stay in
+// registers for the whole chunk and are added into gram once at the end,
+// instead of at every bin change.
+// ---------------------------------------------------------------------------
+constexpr int MFB_Z0 = 8;     // parts of the off-gate bin
+constexpr int MFB_WIN = MF_THREADS;  // stages examined per pass, one per thread
+
+// Bins present in each stage of each channel block: bit b of
+// masks[(g n_cb + cb) W + b / 32]; one thread per (stage, block).
+__global__ __launch_bounds__(256) void k_pulsar_stage_masks(const int16_t *__restrict__ bins, int64_t n_segments,
+                                                            int n_ch, int CH, int n_cb, int64_t n_stages, int W,
+                                                            uint32_t *__restrict__ masks) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_stages * n_cb) return;
+  const int64_t g = i / n_cb;
+  const int cb = (int)(i - g * n_cb);
+  uint32_t *m = masks + i * W;
+  for (int w = 0; w < W; w++) m[w] = 0u;
+  int prev = -1;
+  for (int k = 0; k < MF_KS; k++) {
+    const int64_t seg = g * MF_KS + k;
+    if (seg >= n_segments) break;
+    for (int c = 0; c < CH && cb * CH + c < n_ch; c++) {
+      const int b = bins[seg * n_ch + cb * CH + c];
+      if (b != prev) {
+        m[b >> 5] |= 1u << (b & 31);
+        prev = b;
+      }
+    }
+  }
+}
+
+template <int PREC, bool HS>
+__global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar_bin(
+    const void *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT, int CH,
+    int64_t seg_base, int64_t n_segments, float scale, const int16_t *__restrict__ bins,
+    const uint32_t *__restrict__ masks, int W, float *__restrict__ gram) {
+  extern __shared__ _Float16 ylds[];
+  constexpr int planes = PREC == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
+  const int R = 32 * RT;
+  const int nT = RT * (RT + 1) / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cb = blockIdx.x, n_cb = gridDim.x;
+  const int j = blockIdx.y;
+  const int b = j < MFB_Z0 ? 0 : j - MFB_Z0 + 1;
+  const int part = j < MFB_Z0 ? j : 0, parts = j < MFB_Z0 ? MFB_Z0 : 1;
+  const int64_t n_stages = (n_segments + MF_KS - 1) / MF_KS;
+  const int64_t g_lo = n_stages * part / parts, g_hi = n_stages * (part + 1) / parts;
+  const int ch0 = cb * CH;
+  const int chan_stride = R * MF_RS + 8;
+  const int plane_elems = CH * chan_stride;
+  uint32_t *cmask = reinterpret_cast<uint32_t *>(ylds + planes * plane_elems);  // [64]: the bin's segments per channel
+  int32_t *list = reinterpret_cast<int32_t *>(cmask + 64);                      // [MFB_WIN]
+  int32_t *wcnt = list + MFB_WIN;                                               // [MF_WAVES]
+  for (int i = threadIdx.x; i < planes * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
+  f16v_t acc[MFP_MAXP];
+#pragma unroll
+  for (int q = 0; q < MFP_MAXP; q++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
+  const int n_pairs = CH * nT;
+  const int ko = 8 * (lane >> 5);
+  for (int64_t w0 = g_lo; w0 < g_hi; w0 += MFB_WIN) {
+    // this window's stages holding the bin, in order
+    const int64_t g = w0 + threadIdx.x;
+    const bool has = g < g_hi && ((masks[(g * n_cb + cb) * W + (b >> 5)] >> (b & 31)) & 1u);
+    const uint64_t bal = __ballot(has);
+    __syncthreads();  // the previous window's list is no longer read
+    if (lane == 0) wcnt[wave] = __popcll(bal);
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < MF_WAVES; w++) {
+      base += w < wave ? wcnt[w] : 0;
+      total += wcnt[w];
+    }
+    if (has) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)(g - w0);
+    __syncthreads();
+    for (int e = 0; e < total; e++) {
+      const int64_t g_rel = w0 + list[e];
+      mf_load_stage<PREC, MF_KS, HS>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0,
+                                     seg_base + g_rel * MF_KS, seg_base + n_segments, scale, chan_stride,
+                                     plane_elems, ylds);
+      if ((int)threadIdx.x < CH) {
+        uint32_t mk = 0u;
+        const int ch = ch0 + threadIdx.x;
+        if (ch < n_ch) {
+#pragma unroll
+          for (int k = 0; k < MF_KS; k++) {
+            const int64_t seg = g_rel * MF_KS + k;
+            if (seg < n_segments && bins[seg * n_ch + ch] == b) mk |= 1u << k;
+          }
+        }
+        cmask[threadIdx.x] = mk;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < MFP_MAXP; q++) {
+        const int p = wave + MF_WAVES * q;
+        if (p < n_pairs) {
+          int c, t, ti, tj;
+          mf_pair(p, nT, RT, c, t, ti, tj);
+          const uint32_t mk = cmask[c];
+          if (mk != 0u) {
+            const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * MF_RS + ko;
+            const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * MF_RS + ko;
+            mf_mma<PREC>(acc[q], ylds, oa, ob, plane_elems, (mk >> ko) & 0xffu);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MFP_MAXP; q++) {
+    const int p = wave + MF_WAVES * q;
+    if (p < n_pairs) {
+      int c, t, ti, tj;
+      mf_pair(p, nT, RT, c, t, ti, tj);
+      if (ch0 + c < n_ch) {
+        float *o = gram + (((int64_t)b * n_ch + ch0 + c) * nT + t) * 1024 + (lane & 31);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int jj = 0; jj < 4; jj++) unsafeAtomicAdd(o + (8 * i + 4 * (lane >> 5) + jj) * 32, acc[q][4 * i + jj]);
+      }
+    }
+  }
+}
+
 // Visibilities of every bin from the binned Gram tiles:
 // vis[bin][b][ch] = (G[a][d] + G[M+a][M+d] + i (G[M+a][d] - G[a][M+d])) / scale^2.
 // gram holds channels ch_base .. ch_base + n_run - 1; vis rows have n_ch channels.
@@ -3043,7 +3180,8 @@ std::vector<int> mf_channel_candidates(int precision, int M) {
   if (CH / 4 >= 1) c.push_back(CH / 4);
   return c;
 }
-std::vector<int> mfp_wgs_candidates() { return {1024, 512, 2048}; }
+// 0: by bin (k_xstep_mfma_pulsar_bin), else the time-sliced kernel's workgroups
+std::vector<int> mfp_wgs_candidates() { return {0, 1024, 512}; }
 
 template <int N, int T, int SP>
 static hipError_t launch_fe_n(const DevStream *d_streams, const DevJob &job, const float *window,
@@ -3485,6 +3623,30 @@ static hipError_t launch_mfma_pulsar_t(dim3 grid, size_t lds, hipStream_t s, con
   return hipGetLastError();
 }
 
+// LDS beyond the stage planes of the by-bin kernel: channel masks, stage list, wave counts
+constexpr size_t MFB_EXTRA = 64 * sizeof(uint32_t) + (MFB_WIN + MF_WAVES) * sizeof(int32_t);
+
+size_t pulsar_bin_mask_words(int precision, int M, int n_ch_run, int nbins, int64_t n_segments) {
+  size_t lds = 0;
+  const int CH = mf_channels(precision, M, MFP_MAXP, MFB_EXTRA, &lds);
+  if (CH < 1) return 0;
+  const int64_t n_cb = (n_ch_run + CH - 1) / CH, n_stages = (n_segments + MF_KS - 1) / MF_KS;
+  return (size_t)n_stages * n_cb * ((nbins + 31) / 32);
+}
+
+template <int PREC, bool HS>
+static hipError_t launch_mfma_pulsar_bin_t(dim3 grid, size_t lds, hipStream_t s, const void *spectra,
+                                           int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT, int CH,
+                                           int64_t seg_base, int64_t n_segments, float scale, const int16_t *bins,
+                                           const uint32_t *masks, int W, float *gram) {
+  hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma_pulsar_bin<PREC, HS>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_xstep_mfma_pulsar_bin<PREC, HS>), grid, dim3(MF_THREADS), lds, s, spectra, stream_stride,
+                     seg_stride, n_ch, M, RT, CH, seg_base, n_segments, scale, bins, masks, W, gram);
+  return hipGetLastError();
+}
+
 hipError_t pulsar_gram_zero(int M, int n_ch_run, int nbins, float *gram, hipStream_t s) {
   return launch_fill_zero(gram, pulsar_gram_floats(M, n_ch_run, nbins) * sizeof(float), s);
 }
@@ -3494,8 +3656,44 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const void *spectra, bool hal
                                     int ch_base, int n_ch_run, int M, int64_t seg_base, int64_t n_segments,
                                     float scale, const double *offsets, double start_phase, double phase_coef,
                                     double fft_duration, double gate_begin, double gate_end, int nbins, float *gram,
-                                    const Tune &t, hipStream_t s) {
+                                    int16_t *bins, uint32_t *masks, const Tune &t, hipStream_t s) {
   if (n_segments <= 0 || n_ch_run <= 0) return hipSuccess;
+  if (t.mfp_wgs == 0) {
+    // by bin: the bin table and the stages' bin masks of the chunk, then one
+    // workgroup per (channel block, bin)
+    size_t lds = 0;
+    const int CH = mf_channels(precision, M, MFP_MAXP, MFB_EXTRA, &lds);
+    if (CH < 1 || nbins < 2) return hipErrorInvalidValue;
+    const int RT = (2 * M + 31) / 32;
+    const int n_cb = (n_ch_run + CH - 1) / CH;
+    const int64_t n_stages = (n_segments + MF_KS - 1) / MF_KS;
+    const int W = (nbins + 31) / 32;
+    const double *off = offsets + ch_base;
+    const int64_t n_pairs = n_segments * n_ch_run;
+    hipLaunchKernelGGL(k_pulsar_bin_table, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0, s, bins, n_ch_run,
+                       seg_base, n_segments, off, start_phase, phase_coef, fft_duration, gate_begin, gate_end, nbins);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pulsar_stage_masks, dim3((unsigned)((n_stages * n_cb + 255) / 256)), dim3(256), 0, s, bins,
+                       n_segments, n_ch_run, CH, n_cb, n_stages, W, masks);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 grid((unsigned)n_cb, (unsigned)(nbins - 1 + MFB_Z0));
+    const void *sp = static_cast<const char *>(spectra) + (size_t)ch_base * (half_spectra ? 4 : 8);
+#define SFXG_MFB(P, H)                                                                                            \
+  return launch_mfma_pulsar_bin_t<P, H>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH, seg_base, \
+                                        n_segments, scale, bins, masks, W, gram)
+    if (half_spectra) {
+      if (precision == SFXG_PREC_FP16) SFXG_MFB(SFXG_PREC_FP16, true);
+      if (precision == SFXG_PREC_BF16) SFXG_MFB(SFXG_PREC_BF16, true);
+      return hipErrorInvalidValue;
+    }
+    if (precision == SFXG_PREC_FP16) SFXG_MFB(SFXG_PREC_FP16, false);
+    if (precision == SFXG_PREC_BF16) SFXG_MFB(SFXG_PREC_BF16, false);
+    if (precision == SFXG_PREC_SPLIT_FP16) SFXG_MFB(SFXG_PREC_SPLIT_FP16, false);
+#undef SFXG_MFB
+    return hipErrorInvalidValue;
+  }
   size_t lds = 0;
   const int CH = mf_channels(precision, M, MFP_MAXP,
                             (size_t)64 * MF_KS * sizeof(int16_t) + (size_t)64 * MFP_RUNS * sizeof(int32_t), &lds);
@@ -3503,7 +3701,7 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const void *spectra, bool hal
   const int RT = (2 * M + 31) / 32;
   // about mfp_wgs (1024) workgroups: channel blocks x segment ranges (multiples of a stage)
   const int64_t nblk = (n_ch_run + CH - 1) / CH;
-  const int64_t wgs = t.mfp_wgs > 0 ? t.mfp_wgs : 1024;
+  const int64_t wgs = t.mfp_wgs;
   int64_t nz = (wgs + nblk - 1) / nblk;
   int64_t per = (n_segments + nz - 1) / nz;
   per = (per + MF_KS - 1) / MF_KS * MF_KS;
