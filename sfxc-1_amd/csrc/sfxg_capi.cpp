@@ -1139,8 +1139,10 @@ static int prepare_slice(sfxg_ctx *ctx) {
   // once before the X-step; with mixed precision also the bins present in
   // each stage of each channel block
   if (ctx->pulsar) HIPCHK(ctx, ctx->psr_bins.ensure((size_t)lp.chunk_segs * (nc + 1) * sizeof(int16_t)));
-  if (mixed && ctx->pulsar) {
-    const size_t words = pulsar_bin_mask_words(ctx->opt.precision, M, psr_run, ctx->nbins, lp.chunk_segs);
+  if (ctx->pulsar) {
+    const size_t words =
+        mixed ? pulsar_bin_mask_words(ctx->opt.precision, M, psr_run, ctx->nbins, lp.chunk_segs)
+              : (size_t)((lp.chunk_segs + 15) / 16) * ((nc + 1 + 15) / 16) * ((ctx->nbins + 31) / 32);
     HIPCHK(ctx, ctx->psr_masks.ensure(std::max<size_t>(words, 1) * sizeof(uint32_t)));
   }
 
@@ -1445,11 +1447,9 @@ static int enqueue_xstep(sfxg_ctx *ctx, size_t c, hipStream_t s, const Tune &t) 
     HIPCHK(ctx, launch_xstep_pulsar(ctx->spectra.as<float2>(), sstride, ctx->S, nc + 1, ch.seg0, ch.seg1 - ch.seg0,
                                     blf, bls, B, ctx->offsets.as<double>(), lp.start_phase, lp.phase_coef,
                                     lp.fft_duration, ps.gate_start, ps.gate_stop, ctx->nbins,
-                                    ctx->psr_bins.as<int16_t>(), ctx->vis.as<float2>(), ctx->tiles.as<int32_t>(),
-                                    (int)(ctx->h_tiles.size() / 18), M,
-                                    t.xs_waves == 2 ||
-                                        (t.xs_waves == 0 && pulsar_tile16_default(nc + 1, (int)(ctx->h_tiles.size() / 18))),
-                                    s));
+                                    ctx->psr_bins.as<int16_t>(), ctx->psr_masks.as<uint32_t>(), ctx->vis.as<float2>(),
+                                    ctx->tiles.as<int32_t>(), (int)(ctx->h_tiles.size() / 18), M,
+                                    t.xs_waves == 0 || t.xs_waves == 2, s));
   }
   return SFXG_OK;
 }
@@ -2092,7 +2092,7 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
     xc = mfp_wgs_candidates();
     xname = "mfp_wgs";
   } else {
-    xc = {0, 2, 4};  // default, register tiles, one thread per (baseline pair, channel)
+    xc = {0, 4};  // by bin on register tiles, or one thread per (baseline, channel)
     xname = "xs_waves";
   }
   if (xc.size() > 1) {

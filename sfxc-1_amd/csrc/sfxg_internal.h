@@ -230,8 +230,6 @@ std::vector<int> chan_wgs_candidates(int nc);
 std::vector<int> xs_waves_candidates();
 // whether the plain X-step takes k_xstep_tile16 by default for this many tiles
 bool xstep_tile16_default(int n_tiles);
-// whether the FP32 pulsar X-step takes k_xstep_pulsar_tile16 by default
-bool pulsar_tile16_default(int n_ch, int n_tiles);
 std::vector<int> mf_channel_candidates(int precision, int M);
 std::vector<int> mfp_wgs_candidates();
 
@@ -290,8 +288,8 @@ hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int
                                int n_ch, int64_t seg_base, int64_t n_segments, const int32_t *bl_first,
                                const int32_t *bl_second, int n_baselines, const double *offsets,
                                double start_phase, double phase_coef, double fft_duration,
-                               double gate_begin, double gate_end, int nbins, int16_t *bins, float2 *acc,
-                               const int32_t *tiles, int n_tiles, int M, bool tile16, hipStream_t s);
+                               double gate_begin, double gate_end, int nbins, int16_t *bins, uint32_t *masks,
+                               float2 *acc, const int32_t *tiles, int n_tiles, int M, bool by_bin, hipStream_t s);
 // Mixed-precision X-step with pulsar binning for channels ch_base ..
 // ch_base + n_ch_run - 1 of n_ch (offsets indexed by channel): binned Gram
 // tiles in gram (pulsar_gram_floats(M, n_ch_run, nbins) floats, zeroed here),

@@ -2484,56 +2484,59 @@ __global__ __launch_bounds__(64 * XWAVES) void k_xstep_pulsar(
 }
 
 // ---------------------------------------------------------------------------
-// k_xstep_pulsar_tile16: k_xstep_pulsar on the register tiles of
-// k_xstep_tile16 (16 channels per workgroup, every stream-pair tile held at
-// once, spectra staged through LDS once per workgroup). The stage also
-// holds its segments' bins of the 16 channels. A lane keeps the running sums
-// of its tiles for its channel's current bin; where the bin changes they are
-// written back to acc_out[bin] and the new bin's sums loaded, so every
-// (bin, baseline, channel) still adds its products one segment after the
-// other, as the reference does.
+// k_xstep_pulsar_bin: the FP32 pulsar X-step by bin, on the register tiles
+// of k_xstep_tile16 (16 channels per workgroup, lane = tile slot x channel).
+// A workgroup takes one bin of one channel block and walks, in order, the
+// 16-segment stages where the bin occurs (per-stage bin masks, compacted a
+// window at a time); each lane adds the products of its channel's segments
+// of that bin. Every (bin, baseline, channel) thus still adds its products
+// one segment after the other, from the sum the previous chunks left, as
+// the reference does, and its sum is read and written once per chunk
+// instead of at every bin change. NW waves per workgroup, TPS tiles per slot.
 // ---------------------------------------------------------------------------
-template <int TPS>
-__global__ __launch_bounds__(64 * XQ_WAVES) void k_xstep_pulsar_tile16(
-    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int64_t seg_base,
-    int64_t n_segments, int M, const int32_t *__restrict__ tiles, int n_tiles, int n_baselines,
-    const int16_t *__restrict__ bins, int sps, int n_cb, int n_tg, float2 *__restrict__ acc_out) {
+template <int NW, int TPS>
+__global__ __launch_bounds__(64 * NW) void k_xstep_pulsar_bin(
+    const float2 *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int64_t n_segments,
+    int M, const int32_t *__restrict__ tiles, int n_tiles, int n_baselines, const int16_t *__restrict__ bins,
+    const uint32_t *__restrict__ masks, int W, int n_tg, float2 *__restrict__ acc_out) {
 #pragma clang fp contract(off)
-  extern __shared__ float4 xq[];  // [sps][M][8] float4, then [sps][16] bins (int)
+  constexpr int SLOTS = 4 * NW, NT = 64 * NW;
+  extern __shared__ float4 xq[];                                            // [MF_KS][M][8]
+  int *sb = reinterpret_cast<int *>(xq + (size_t)MF_KS * M * 8);           // [MF_KS][16]: bin per (segment, channel)
+  int *list = sb + MF_KS * 16;                                             // [NT]
+  int *wcnt = list + NT;                                                   // [NW]
   const float2 *xq2 = reinterpret_cast<const float2 *>(xq);
-  int *bl = reinterpret_cast<int *>(xq + (size_t)sps * M * 8);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = lane & 15;
   const int slot = wave * 4 + (lane >> 4);
-  // (tile group, channel block), tile groups innermost
-  const int L = xcd_grouped(blockIdx.x, gridDim.x, n_tg);
-  const int tg = L % n_tg, cb = L / n_tg;
-  const int ch0 = cb * 16;
-  const int ch = ch0 + c;
-  const bool own = ch < n_ch;  // lanes past the last channel never touch acc_out
+  const int tg = blockIdx.x % n_tg, cb = blockIdx.x / n_tg, b = blockIdx.y;
+  const int n_cb = gridDim.x / n_tg;
+  const int ch0 = cb * 16, ch = ch0 + c;
+  const bool own = ch < n_ch;
   int a0[TPS], b0[TPS];
   bool act[TPS];
 #pragma unroll
   for (int j = 0; j < TPS; j++) {
-    const int t = (tg * TPS + j) * XQ_SLOTS + slot;
+    const int t = (tg * TPS + j) * SLOTS + slot;
     act[j] = t < n_tiles;
     a0[j] = act[j] ? tiles[t * (2 + XT * XT)] : 0;
     b0[j] = act[j] ? tiles[t * (2 + XT * XT) + 1] : 0;
   }
-  const int64_t bin_stride = (int64_t)n_baselines * n_ch;
   float2 acc[TPS][XT * XT];
-  // running sums of the current bin to acc_out, or from it
-  auto sums = [&](int bin, bool store) {
-    float2 *o = acc_out + bin * bin_stride + ch;
+  // the running sums of the bin from acc_out (store = false) or to it
+  auto sums = [&](bool store) {
+    float2 *o = acc_out + (int64_t)b * n_baselines * n_ch + ch;
 #pragma unroll
     for (int j = 0; j < TPS; j++) {
-      if (!act[j]) continue;
-      const int t = (tg * TPS + j) * XQ_SLOTS + slot;
+      const int t = (tg * TPS + j) * SLOTS + slot;
 #pragma unroll
       for (int q = 0; q < XT * XT; q++) {
-        const int e = tiles[t * (2 + XT * XT) + 2 + q];
-        if (e == XT_NONE) continue;
+        const int e = act[j] && own ? tiles[t * (2 + XT * XT) + 2 + q] : XT_NONE;
+        if (e == XT_NONE) {
+          if (!store) acc[j][q] = make_float2(0.0f, 0.0f);
+          continue;
+        }
         float2 *p = o + (int64_t)(e >= 0 ? e : ~e) * n_ch;
         if (store) {
           *p = e >= 0 ? acc[j][q] : make_float2(acc[j][q].x, -acc[j][q].y);
@@ -2544,82 +2547,68 @@ __global__ __launch_bounds__(64 * XQ_WAVES) void k_xstep_pulsar_tile16(
       }
     }
   };
-#pragma unroll
-  for (int j = 0; j < TPS; j++)
-#pragma unroll
-    for (int q = 0; q < XT * XT; q++) acc[j][q] = make_float2(0.0f, 0.0f);
+  sums(false);
+  const int64_t n_stages = (n_segments + MF_KS - 1) / MF_KS;
   const int row_items = M * 8;
-  float4 pf[XQ_PF];
-  int pb = 0;
-  auto fetch = [&](int64_t s0, int ns) {
-    const int total = ns * row_items;
+  for (int64_t w0 = 0; w0 < n_stages; w0 += NT) {
+    const int64_t g = w0 + threadIdx.x;
+    const bool has = g < n_stages && ((masks[(g * n_cb + cb) * W + (b >> 5)] >> (b & 31)) & 1u);
+    const uint64_t bal = __ballot(has);
+    __syncthreads();  // the previous window's list is no longer read
+    if (lane == 0) wcnt[wave] = __popcll(bal);
+    __syncthreads();
+    int base = 0, total = 0;
 #pragma unroll
-    for (int v = 0; v < XQ_PF; v++) {
-      const int idx = threadIdx.x + v * (64 * XQ_WAVES);
-      float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (idx < total) {
+    for (int w = 0; w < NW; w++) {
+      base += w < wave ? wcnt[w] : 0;
+      total += wcnt[w];
+    }
+    if (has) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)(g - w0);
+    __syncthreads();
+    for (int e = 0; e < total; e++) {
+      const int64_t seg0 = (w0 + list[e]) * MF_KS;
+      for (int idx = threadIdx.x; idx < MF_KS * row_items; idx += NT) {
         const int sl = idx / row_items, r = idx - sl * row_items;
         const int st = r >> 3, q = r & 7;
-        x = *reinterpret_cast<const float4 *>(spectra + st * stream_stride + (s0 + sl) * seg_stride + ch0 + 2 * q);
+        float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (seg0 + sl < n_segments)
+          x = *reinterpret_cast<const float4 *>(spectra + st * stream_stride + (seg0 + sl) * seg_stride + ch0 + 2 * q);
+        xq[idx] = x;
       }
-      pf[v] = x;
-    }
-    pb = 0;
-    if ((int)threadIdx.x < ns * 16) {
-      const int sl = threadIdx.x >> 4, cc = threadIdx.x & 15;
-      pb = ch0 + cc < n_ch ? bins[(s0 + sl) * n_ch + ch0 + cc] : 0;
-    }
-  };
-  int cur = -1;
-  int64_t s0 = 0;
-  int ns = (int)(n_segments < sps ? n_segments : sps);
-  if (s0 < n_segments) fetch(s0, ns);
-  while (s0 < n_segments) {
-    const int total = ns * row_items;
-#pragma unroll
-    for (int v = 0; v < XQ_PF; v++) {
-      const int idx = threadIdx.x + v * (64 * XQ_WAVES);
-      if (idx < total) xq[idx] = pf[v];
-    }
-    if ((int)threadIdx.x < ns * 16) bl[threadIdx.x] = pb;
-    __syncthreads();
-    const int64_t s1 = s0 + ns;
-    const int ns1 = (int)((n_segments - s1) < sps ? (n_segments - s1) : sps);
-    if (s1 < n_segments) fetch(s1, ns1);
+      if ((int)threadIdx.x < MF_KS * 16) {
+        const int sl = threadIdx.x >> 4, cc = threadIdx.x & 15;
+        sb[threadIdx.x] = seg0 + sl < n_segments && ch0 + cc < n_ch ? bins[(seg0 + sl) * n_ch + ch0 + cc] : -1;
+      }
+      __syncthreads();
 #pragma unroll 1
-    for (int sl = 0; sl < ns; sl++) {
-      const int bin = bl[sl * 16 + c];
-      if (bin != cur && own) {
-        if (cur >= 0) sums(cur, true);
-        sums(bin, false);
-        cur = bin;
-      }
-      const float2 *row = xq2 + sl * M * 16 + c;
+      for (int sl = 0; sl < MF_KS; sl++) {
+        const bool mine = sb[sl * 16 + c] == b;
+        if (!__any(mine)) continue;
+        const float2 *row = xq2 + sl * M * 16 + c;
 #pragma unroll
-      for (int j = 0; j < TPS; j++) {
-        if (!act[j]) continue;
-        float2 x[XT], y[XT];
+        for (int j = 0; j < TPS; j++) {
+          if (!act[j]) continue;
+          float2 x[XT], y[XT];
 #pragma unroll
-        for (int i = 0; i < XT; i++) {
-          x[i] = a0[j] + i < M ? row[(a0[j] + i) * 16] : make_float2(0.0f, 0.0f);
-          y[i] = b0[j] + i < M ? row[(b0[j] + i) * 16] : make_float2(0.0f, 0.0f);
-        }
-#pragma unroll
-        for (int i = 0; i < XT; i++)
-#pragma unroll
-          for (int m = 0; m < XT; m++) {
-            const float pr = x[i].x * y[m].x + x[i].y * y[m].y;
-            const float pi = x[i].y * y[m].x - x[i].x * y[m].y;
-            acc[j][XT * i + m].x = acc[j][XT * i + m].x + pr;
-            acc[j][XT * i + m].y = acc[j][XT * i + m].y + pi;
+          for (int i = 0; i < XT; i++) {
+            x[i] = a0[j] + i < M ? row[(a0[j] + i) * 16] : make_float2(0.0f, 0.0f);
+            y[i] = b0[j] + i < M ? row[(b0[j] + i) * 16] : make_float2(0.0f, 0.0f);
           }
+#pragma unroll
+          for (int i = 0; i < XT; i++)
+#pragma unroll
+            for (int m = 0; m < XT; m++) {
+              const float pr = x[i].x * y[m].x + x[i].y * y[m].y;
+              const float pi = x[i].y * y[m].x - x[i].x * y[m].y;
+              const float2 s2 = make_float2(acc[j][XT * i + m].x + pr, acc[j][XT * i + m].y + pi);
+              acc[j][XT * i + m] = mine ? s2 : acc[j][XT * i + m];
+            }
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
-    s0 = s1;
-    ns = ns1;
   }
-  if (own && cur >= 0) sums(cur, true);
+  sums(true);
 }
 
 // ---------------------------------------------------------------------------
@@ -3285,14 +3274,6 @@ void xstep_tile_table(int M, const std::vector<int32_t> &bl_first, const std::ve
 
 bool xstep_tile16_default(int n_tiles) { return n_tiles > 64; }
 
-// The pulsar X-step holds its running sums for the whole chunk, so its
-// register tiles parallelise over channel blocks and tile groups only: they
-// are the default where that gives at least two workgroups per CU.
-bool pulsar_tile16_default(int n_ch, int n_tiles) {
-  const int TPS = n_tiles <= XQ_SLOTS ? 1 : n_tiles <= 2 * XQ_SLOTS ? 2 : 3;
-  const int n_tg = (n_tiles + TPS * XQ_SLOTS - 1) / (TPS * XQ_SLOTS);
-  return (int64_t)((n_ch + 15) / 16) * n_tg >= 512;
-}
 
 hipError_t launch_xstep_tile(const float2 *spectra, int64_t stream_stride, int64_t seg_stride, int64_t seg_base,
                              int n_ch, int M, const int32_t *fold_start, int n_k, const int32_t *tiles, int n_tiles,
@@ -3533,38 +3514,55 @@ hipError_t launch_xstep_pulsar(const float2 *spectra, int64_t stream_stride, int
                                int64_t seg_base, int64_t n_segments, const int32_t *bl_first,
                                const int32_t *bl_second, int n_baselines, const double *offsets, double start_phase,
                                double phase_coef, double fft_duration, double gate_begin, double gate_end, int nbins,
-                               int16_t *bins, float2 *acc, const int32_t *tiles, int n_tiles, int M, bool tile16,
-                               hipStream_t s) {
+                               int16_t *bins, uint32_t *masks, float2 *acc, const int32_t *tiles, int n_tiles, int M,
+                               bool by_bin, hipStream_t s) {
   if (n_segments <= 0) return hipSuccess;
   const int64_t n_pairs = n_segments * n_ch;
   hipLaunchKernelGGL(k_pulsar_bin_table, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0, s, bins, n_ch,
                      seg_base, n_segments, offsets, start_phase, phase_coef, fft_duration, gate_begin, gate_end, nbins);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (tile16) {
-    // spectra of the chunk from its first row: the kernel counts segments from 0
-    const float2 *sp = spectra;
-    int sps = (XQ_PF * 64 * XQ_WAVES) / (M * 8);
-    if (sps > 8) sps = 8;
-    if (sps < 1 || stream_stride % 2 || seg_stride % 64) return hipErrorInvalidValue;
-    const size_t lds = (size_t)sps * M * 8 * sizeof(float4) + (size_t)sps * 16 * sizeof(int);
-    const int TPS = n_tiles <= XQ_SLOTS ? 1 : n_tiles <= 2 * XQ_SLOTS ? 2 : 3;
-    const int n_cb = (n_ch + 15) / 16, n_tg = (n_tiles + TPS * XQ_SLOTS - 1) / (TPS * XQ_SLOTS);
-    if ((int64_t)n_cb * n_tg >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
-    dim3 grid((unsigned)(n_cb * n_tg));
-#define SFXG_XQP(T)                                                                                                 \
-  do {                                                                                                              \
-    e = hipFuncSetAttribute((const void *)k_xstep_pulsar_tile16<T>, hipFuncAttributeMaxDynamicSharedMemorySize,       \
-                            (int)lds);                                                                              \
-    if (e != hipSuccess) return e;                                                                                  \
-    hipLaunchKernelGGL(k_xstep_pulsar_tile16<T>, grid, dim3(64 * XQ_WAVES), lds, s, sp, stream_stride, seg_stride,  \
-                       n_ch, seg_base, n_segments, M, tiles, n_tiles, n_baselines, bins, sps, n_cb, n_tg, acc);    \
-    e = hipGetLastError();                                                                                          \
+  if (by_bin) {
+    // stage bin masks of 16-channel blocks, then one workgroup per (channel
+    // block, tile group, bin)
+    const int W = (nbins + 31) / 32;
+    const int n_cb = (n_ch + 15) / 16;
+    const int64_t n_stages = (n_segments + MF_KS - 1) / MF_KS;
+    hipLaunchKernelGGL(k_pulsar_stage_masks, dim3((unsigned)((n_stages * n_cb + 255) / 256)), dim3(256), 0, s, bins,
+                       n_segments, n_ch, 16, n_cb, n_stages, W, masks);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (stream_stride % 2 || seg_stride % 64) return hipErrorInvalidValue;  // 16-byte loads inside the rows
+    const int NW = n_tiles <= 24 ? 6 : 12;
+    const int slots = 4 * NW;
+    const int TPS = n_tiles <= slots ? 1 : n_tiles <= 2 * slots ? 2 : 3;
+    const int n_tg = (n_tiles + TPS * slots - 1) / (TPS * slots);
+    const size_t lds = (size_t)MF_KS * M * 8 * sizeof(float4) + (size_t)(MF_KS * 16 + 64 * NW + NW) * sizeof(int);
+    if (lds > 160 * 1024 || (int64_t)n_cb * n_tg >= ((int64_t)1 << 31)) by_bin = false;  // per baseline below
+  }
+  if (by_bin) {
+    const int W = (nbins + 31) / 32;
+    const int n_cb = (n_ch + 15) / 16;
+    const int NW = n_tiles <= 24 ? 6 : 12;
+    const int slots = 4 * NW;
+    const int TPS = n_tiles <= slots ? 1 : n_tiles <= 2 * slots ? 2 : 3;
+    const int n_tg = (n_tiles + TPS * slots - 1) / (TPS * slots);
+    const size_t lds = (size_t)MF_KS * M * 8 * sizeof(float4) + (size_t)(MF_KS * 16 + 64 * NW + NW) * sizeof(int);
+    dim3 grid((unsigned)(n_cb * n_tg), (unsigned)nbins);
+#define SFXG_XPB(NWV, T)                                                                                          \
+  do {                                                                                                            \
+    e = hipFuncSetAttribute((const void *)k_xstep_pulsar_bin<NWV, T>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                            (int)lds);                                                                            \
+    if (e != hipSuccess) return e;                                                                                \
+    hipLaunchKernelGGL((k_xstep_pulsar_bin<NWV, T>), grid, dim3(64 * NWV), lds, s, spectra, stream_stride,        \
+                       seg_stride, n_ch, n_segments, M, tiles, n_tiles, n_baselines, bins, masks, W, n_tg, acc);  \
+    e = hipGetLastError();                                                                                        \
   } while (0)
-    if (TPS == 1) SFXG_XQP(1);
-    else if (TPS == 2) SFXG_XQP(2);
-    else SFXG_XQP(3);
-#undef SFXG_XQP
+    if (NW == 6) SFXG_XPB(6, 1);
+    else if (TPS == 1) SFXG_XPB(12, 1);
+    else if (TPS == 2) SFXG_XPB(12, 2);
+    else SFXG_XPB(12, 3);
+#undef SFXG_XPB
     return e;
   }
   const int n_cb = (n_ch + 63) / 64, n_bg = (n_baselines + XWAVES - 1) / XWAVES;

@@ -238,8 +238,8 @@ def test_pulsar_bins(ctx):
 
 @pytest.mark.parametrize("n_stations,nchan,chunk_mib", [(12, 512, 0), (12, 512, 1), (3, 256, 0)])
 def test_pulsar_tiles_match_per_baseline(ctx, n_stations, nchan, chunk_mib):
-    """The FP32 pulsar X-step on register tiles of 16 channels
-    (k_xstep_pulsar_tile16; 513 and 257 channels, so a last block of one
+    """The FP32 pulsar X-step by bin on register tiles of 16 channels
+    (k_xstep_pulsar_bin; 513 and 257 channels, so a last block of one
     channel) gives the same bits as the per-baseline kernel (k_xstep_pulsar):
     both add each (bin, baseline, channel)'s products segment by segment,
     across chunks too (chunk_mib 1: one sub-integration per chunk). Both match
