@@ -676,6 +676,7 @@ def main():
         for name in precs:
             wall, kt, v = wl.run(torch, capi, PREC[name], K, W, dev, autotune=not args.no_autotune)
             r = summarise(wl, wall, kt, name, steps=K)
+            r["chunks_per_slice"], r["chunk_segments"] = wl.chunks, wl.chunk_segs
             r["readback"] = wl.readback
             r["host_prepare_ms_per_step"] = round(wl.host_ms_per_step, 2)
             if name != "fp32":
@@ -691,11 +692,14 @@ def main():
         achieved = h["_kb"][dom_name] / (kt[dom] * 1e-3) / 1e9
         # the delay, channeliser and X-step kernels run once per chunk of the
         # slice; stats and normalise once per slice
-        per = wl.chunks if dom_name in ("fengine", "channelise", "xstep") else 1
+        # this leg's chunk plan (packed FP16 / BF16 spectra take half the
+        # bytes of FP32 ones, so fewer chunks)
+        chunks, chunk_segs = h["chunks_per_slice"], h["chunk_segments"]
+        per = chunks if dom_name in ("fengine", "channelise", "xstep") else 1
         roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": int(h["_kb"][dom_name] / per),
-                "launch": (f"one chunk of {wl.chunk_segs} segments x 64 streams ({wl.chunks} per 0.5 s slice)"
+                "launch": (f"one chunk of {chunk_segs} segments x 64 streams ({chunks} per 0.5 s slice)"
                            if per > 1 else "one slice (64 streams x 0.5 s)"),
                 "launch_ms": round(float(kt[dom]) / per, 3)}
         if peaks:
@@ -714,8 +718,8 @@ def main():
                 roof["valu"]["frac_of_achievable"] = round(rate / peaks["valu_fp32_insts_per_s"], 4)
         xs = {"kernel": "xstep", "TFLOPs": h["xstep_TFLOPs"], "mfma_frac": h.get("xstep_mfma_frac"),
               "mfma_frac_of_achievable": h.get("xstep_mfma_frac_of_achievable"),
-              "GBs": h["kernel_GBs"].get("xstep"), "launch_ms": round(h["kernel_ms_isolated"]["xstep"] / wl.chunks, 3),
-              "launches_per_slice": wl.chunks}
+              "GBs": h["kernel_GBs"].get("xstep"), "launch_ms": round(h["kernel_ms_isolated"]["xstep"] / chunks, 3),
+              "launches_per_slice": chunks}
         errs = None
         if not args.no_errors and "split_fp16" in vis:
             errs = {"reference": "split FP16 of the same slice (itself within 1e-5 relative L2 of the FP64 oracle "

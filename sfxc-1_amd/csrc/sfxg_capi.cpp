@@ -2053,19 +2053,33 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
   std::string rep;
   char line[160];
   Tune best = ctx->tune;
-  // F-engine: threads per workgroup (N_c <= 4096) or the large-N channeliser's workgroups
+  // F-engine: threads per workgroup (N_c <= 4096), or the large-N
+  // channeliser's workgroups; packed FP16 / BF16 spectra at N_c = 16384 also
+  // between the MFMA transform (chan_fp32 = 0) and the FP32 one
   const bool small = nc <= 4096;
-  std::vector<int> fc = small ? fe_thread_candidates(nc) : chan_wgs_candidates(nc);
+  std::vector<std::pair<int, int>> fc;  // (chan_fp32, threads or workgroups)
+  if (small) {
+    for (int v : fe_thread_candidates(nc)) fc.push_back({0, v});
+  } else {
+    const bool mf = chan_mfma_supported(nc, ctx->opt.precision);
+    if (mf)
+      for (int v : chan_mfma_wgs_candidates()) fc.push_back({0, v});
+    for (int v : chan_wgs_candidates(nc)) fc.push_back({mf ? 1 : 0, v});
+  }
   if (fc.size() > 1) {
     float best_ms = -1.0f;
     Tune pick = best;
-    for (int v : fc) {
+    for (const auto &c : fc) {
       Tune t = best;
-      (small ? t.fe_threads : t.chan_wgs) = v;
+      (small ? t.fe_threads : t.chan_wgs) = c.second;
+      t.chan_fp32 = c.first;
       float ms = 0.0f;
       const int rc = time_stage(ctx, reps, [&] { return enqueue_fengine(ctx, 0, ctx->stream, false, t); }, &ms);
       if (rc != SFXG_OK) return rc;
-      snprintf(line, sizeof(line), "%s=%d: %.3f ms\n", small ? "fe_threads" : "chan_wgs", v, ms);
+      snprintf(line, sizeof(line), "%s%s=%d: %.3f ms\n", small ? "fe_threads" : "chan_wgs",
+               !small && chan_mfma_supported(nc, ctx->opt.precision) ? (c.first ? " (FP32 transform)" : " (MFMA)")
+                                                                     : "",
+               c.second, ms);
       rep += line;
       if (best_ms < 0 || ms < best_ms) {
         best_ms = ms;
@@ -2139,8 +2153,8 @@ int sfxg_tune_export(char *dst, size_t cap, size_t *written) {
     char line[256];
     for (const auto &kv : g_tune) {
       const Tune &t = kv.second;
-      snprintf(line, sizeof(line), "%s|%d|%d|%d|%d|%d|%d\n", kv.first.c_str(), t.fe_threads, t.chan_wgs,
-               t.xs_waves, t.mf_channels, t.mfp_wgs, t.mf_ks);
+      snprintf(line, sizeof(line), "%s|%d|%d|%d|%d|%d|%d|%d\n", kv.first.c_str(), t.fe_threads, t.chan_wgs,
+               t.xs_waves, t.mf_channels, t.mfp_wgs, t.mf_ks, t.chan_fp32);
       out += line;
     }
   }
@@ -2162,9 +2176,9 @@ int sfxg_tune_import(const char *text) {
     const size_t bar = ln.find('|');
     if (bar == std::string::npos) return SFXG_ERR_ARG;
     Tune t;
-    const int nf = sscanf(ln.c_str() + bar + 1, "%d|%d|%d|%d|%d|%d", &t.fe_threads, &t.chan_wgs, &t.xs_waves,
-                          &t.mf_channels, &t.mfp_wgs, &t.mf_ks);
-    if (nf != 5 && nf != 6) return SFXG_ERR_ARG;
+    const int nf = sscanf(ln.c_str() + bar + 1, "%d|%d|%d|%d|%d|%d|%d", &t.fe_threads, &t.chan_wgs, &t.xs_waves,
+                          &t.mf_channels, &t.mfp_wgs, &t.mf_ks, &t.chan_fp32);
+    if (nf < 5) return SFXG_ERR_ARG;
     add[ln.substr(0, bar)] = t;
   }
   std::lock_guard<std::mutex> g(g_tune_mu);

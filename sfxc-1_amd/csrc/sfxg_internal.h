@@ -223,10 +223,13 @@ struct Tune {
   int mf_channels = 0;  // MFMA X-step: at most this many channels per workgroup
   int mf_ks = 0;        // MFMA X-step: segments per LDS stage (16 or 32)
   int mfp_wgs = 0;      // MFMA pulsar X-step: 0 by bin, else the time-sliced kernel's workgroups
+  int chan_fp32 = 0;    // packed FP16 / BF16 spectra at N_c = 16384: 0 transform on MFMA (FP16
+                        // operands), 1 the FP32 transform of the other precisions
 };
 // Candidates of each knob for a shape (the default first).
 std::vector<int> fe_thread_candidates(int nc);
 std::vector<int> chan_wgs_candidates(int nc);
+std::vector<int> chan_mfma_wgs_candidates();
 std::vector<int> xs_waves_candidates();
 // whether the plain X-step takes k_xstep_tile16 by default for this many tiles
 bool xstep_tile16_default(int n_tiles);
@@ -252,6 +255,8 @@ hipError_t launch_fengine(const DevStream *d_streams, const DevJob &job, const f
 // N_c = 32768 goes through HBM scratch of channelise_scratch_elems(nc) float2
 // (0 for the LDS-resident sizes).
 size_t channelise_scratch_elems(int nc);
+// whether packed spectra of this precision at this N_c have the MFMA channeliser
+bool chan_mfma_supported(int nc, int spec_prec);
 hipError_t launch_channelise(const DevStream *d_streams, const DevJob &job, const float *window,
                              const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
                              const Tune &t, hipStream_t s);

@@ -1146,6 +1146,397 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
 }
 
 // ---------------------------------------------------------------------------
+// k_chan_mfma: the N_c = 16384 channeliser of the mixed-precision X-steps
+// (spectra stored as packed FP16 or BF16), its transform on MFMA. The
+// 2 N_c windowed reals of a segment are N = 16384 complex points z[n],
+//   n = 512 m2 + 16 m3 + n1,  q = k2 + 32 k3,  k = q + 1024 k1:
+//   V[n1][q]      = sum_m3 W32^(m3 k3) W1024^(m3 k2) sum_m2 z[n] W32^(m2 k2)
+//   Z[q + 1024 k1] = sum_n1 W16^(n1 k1) W_N^(n1 q) V[n1][q].
+// Each of the 16 residues n1 is a 32 x 32 tile (rows m3, columns m2). Stage
+// B1 (over m2) takes the tile as the A operand of v_mfma_f32_32x32x16_f16
+// (K = the real and imaginary parts of 32 m2) against the real form of the
+// 32-point DFT matrix; its 32 x 32 FP32 result, times W1024^(m3 k2) and
+// rounded to FP16, is the B operand of stage B2 (over m3) as it lies in the
+// registers: a lane holds the result rows 8 g + 4 h + j (h = lane / 32), so
+// B1 orders the K slots of its operands the same way (cm_kslot) and the two
+// stages exchange nothing. The DFT matrix is symmetric, so one set of
+// fragments serves as B1's B operand and B2's A operand. The last stage (16
+// points over n1, twiddles W_N^(n1 q)) and the split of the N-point
+// transform into the N + 1 bins of the real one run in FP32 registers: the
+// thread of row q also takes row 1024 - q, which holds the bins N - k.
+// Phases per segment: z from HBM into the tiles' LDS operands (16-byte
+// chunks, XOR-swizzled); B1 and B2 for two tiles per wave, whose V (packed
+// FP16 pairs) goes where those two tiles lay; then the last stage and the
+// spectrum stores. 64 KiB of LDS and at most 128 VGPRs: two workgroups per CU.
+// The operands are FP16 (unit roundoff 2^-11) with FP32 accumulation: z,
+// the B1 results and V are rounded once each, which puts the transform's
+// error near that of the FP16 spectra themselves, within the mixed-precision
+// visibility bar.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int CM_N = 16384;
+constexpr int CM_T = 512;
+typedef _Float16 cm_h8 __attribute__((ext_vector_type(8)));
+typedef float cm_f16 __attribute__((ext_vector_type(16)));
+
+// Row of the B1 / B2 result that K slot 8 h + e of a 16-wide K slice holds.
+__device__ __forceinline__ constexpr int cm_kslot(int h, int e) { return 4 * h + (e & 3) + 8 * (e >> 2); }
+
+// LDS position (16-byte units) of chunk c (K slice s = c / 2, lane half
+// h = c % 2) of row m3 of tile n1.
+__device__ __forceinline__ int cm_chunk(int n1, int m3, int c) { return n1 * 256 + m3 * 8 + (c ^ ((m3 + n1) & 7)); }
+
+// exp(-2 pi i x / n), evaluated in FP64 and rounded once
+__device__ __forceinline__ float2 cm_w(int x, int n) {
+  double sn, cs;
+  sincospi(2.0 * (double)(x % n) / (double)n, &sn, &cs);
+  return make_float2((float)cs, (float)-sn);
+}
+
+// cos(2 pi k / 32) for k = 0..8
+__device__ __forceinline__ constexpr float cm_c32(int k) {
+  constexpr float c[9] = {1.0f, 0.98078528040323044913f, 0.92387953251128675613f, 0.83146961230254523708f,
+                          0.70710678118654752440f, 0.55557023301960222474f, 0.38268343236508977173f,
+                          0.19509032201612826785f, 0.0f};
+  return c[k];
+}
+// exp(-2 pi i k / 32) for k = 0..15 (compile-time k)
+__device__ __forceinline__ constexpr float2 cm_w32(int k) {
+  return make_float2(k <= 8 ? cm_c32(k) : -cm_c32(16 - k), -cm_c32(k <= 8 ? 8 - k : k - 8));
+}
+
+__device__ __forceinline__ uint32_t cm_pack(float re, float im) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)re) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)im) << 16);
+}
+__device__ __forceinline__ float2 cm_unpack(uint32_t w) {
+  return make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)),
+                     (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)));
+}
+
+// Row q of V from LDS: wave w's region holds the pairs (tiles 2 w, 2 w + 1)
+// of every q, 8 bytes each.
+__device__ __forceinline__ void cm_read_row(const uint2 *vlds, int q, float2 (&z)[16]) {
+#pragma unroll
+  for (int w = 0; w < 8; w++) {
+    const uint2 u = vlds[w * 1024 + q];
+    z[2 * w] = cm_unpack(u.x);
+    z[2 * w + 1] = cm_unpack(u.y);
+  }
+}
+
+// z[n1] *= w^n1 for n1 = 1..15 (running powers), then the 16-point DFT:
+// z[k1] = Z[q + 1024 k1].
+__device__ __forceinline__ void cm_last_stage(float2 (&z)[16], float2 w) {
+  float2 p = w;
+#pragma unroll
+  for (int i = 1; i < 16; i++) {
+    z[i] = cmul(z[i], p);
+    if (i < 15) p = cmul(p, w);
+  }
+  dft<16, -1>(z);
+}
+
+// Real-transform bins from Z[k] and Z[N - k]: X[k] = E + W_2N^k O and
+// X[N - k] = conj(E - W_2N^k O), E = (Z[k] + conj Z[N - k]) / 2,
+// O = -i (Z[k] - conj Z[N - k]) / 2.
+// hs = scale / 2 with the spectra's power-of-two scale: the bins come out
+// scaled (exactly as scaling them afterwards would).
+__device__ __forceinline__ void cm_split(float2 zk, float2 zn, float2 w, float hs, float2 &xk, float2 &xn) {
+  const float2 e = make_float2(hs * (zk.x + zn.x), hs * (zk.y - zn.y));
+  const float2 o = make_float2(hs * (zk.y + zn.y), -hs * (zk.x - zn.x));
+  const float2 wo = cmul(w, o);
+  xk = cadd(e, wo);
+  xn = make_float2(e.x - wo.x, -(e.y - wo.y));
+}
+
+// Bin k of a packed spectrum row: both parts of the (already scaled) value
+// rounded to nearest even in the operand type, one conversion of the pair.
+template <int SP>
+__device__ __forceinline__ void cm_put(uint32_t *row, uint32_t k, float2 v) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  uint32_t wd;
+  if constexpr (SP == SFXG_PREC_BF16) {
+    typedef __bf16 b2v __attribute__((ext_vector_type(2)));
+    wd = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){v.x, v.y}, b2v));
+  } else {
+    typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+    wd = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){v.x, v.y}, h2v));
+  }
+  *((__attribute__((address_space(1))) uint32_t *)row + k) = wd;
+}
+
+}  // namespace
+
+// The window at real sample i = 1024 m2 + c (c < 1024) of the 2 N_c: NONE 1
+// on the first half, RECT 1 on the middle half, and the cosine family of
+// theta_i = 2 pi i / (2 N_c - 1) (delay_correction.cc:299-341) through
+// s = sin(theta_i / 2): HANN 0.5 (1 - cos theta) = s^2, HAMMING 0.54 - 0.46
+// cos theta = 0.08 + 0.92 s^2, COS s; s is sin(A + B) of the half angles of
+// theta_A(m2) (a 32-entry table) and theta_B(c) (per thread), each evaluated
+// in FP64 and rounded once. The window differs from the FP32 table by a few
+// 2^-24, far below the FP16 rounding of the windowed points.
+struct CmWindow {
+  float k0, k1, k2;    // w = k0 + k1 s + k2 s^2
+  bool zero_hi, rect;  // NONE: second half zero; RECT: m2 outside [8, 24) zero
+};
+
+// Phase 0 of k_chan_mfma: thread t = 16 m3 + n1 turns its 32 points xp
+// (point i = 16 g + 8 hh + e at m2 = 16 g + cm_kslot(hh, e)) into the FP16 K
+// chunks of row m3 of tile n1, windowed and sideband-flipped.
+__device__ __forceinline__ void cm_phase0(uint4 *lds, const float2 (&xp)[32], const float2 *wtab,
+                                          const float2 (*cth)[CM_T], const CmWindow &win, bool flip) {
+  const int to = opaque(threadIdx.x);
+  const int n1 = to & 15, m3 = to >> 4;
+  const float2 b0 = cth[4][to], b1 = cth[5][to];
+  const float2 *wt = wtab + opaque_s(0);  // read in the loop, not hoisted into registers
+#pragma unroll
+  for (int g = 0; g < 2; g++) {
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++) {
+      cm_h8 re, im;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int i = 16 * g + 8 * hh + e, m2 = 16 * g + cm_kslot(hh, e);
+        // s = sin(theta / 2) from the half angles A / 2 and B / 2
+        const float2 a = wt[m2];
+        const float s0 = a.y * b0.x + a.x * b0.y, s1 = a.y * b1.x + a.x * b1.y;
+        float w0 = win.k0 + (win.k1 + win.k2 * s0) * s0, w1 = win.k0 + (win.k1 + win.k2 * s1) * s1;
+        if ((win.zero_hi && g == 1) || (win.rect && (m2 < 8 || m2 >= 24))) w0 = w1 = 0.0f;
+        const float2 z = wmul(xp[i], make_float2(w0, w1));
+        re[e] = (_Float16)z.x;
+        im[e] = (_Float16)(flip ? -z.y : z.y);
+      }
+      *reinterpret_cast<cm_h8 *>(&lds[cm_chunk(n1, m3, 2 * g + hh)]) = re;
+      *reinterpret_cast<cm_h8 *>(&lds[cm_chunk(n1, m3, 4 + 2 * g + hh)]) = im;
+    }
+  }
+}
+
+// A workgroup walks a run of consecutive segments of one stream. With a
+// window, the second half of a segment (points 512 m2 + t, m2 >= 16) is the
+// first half of the next one in the same thread, so a thread carries it in
+// registers and reads only the 16 new points; they are loaded before this
+// segment's spectrum stores (the vector memory counter is in order, so
+// waiting for them does not wait for the stores). Without a window a segment
+// uses only its first half and nothing carries. One workgroup per CU.
+template <int SP>
+__global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restrict__ streams, DevJob job,
+                                                                int segs_per_wg) {
+  static_assert(SP == SFXG_PREC_FP16 || SP == SFXG_PREC_BF16, "packed FP16 / BF16 spectra");
+  __shared__ uint4 lds[4096];  // 64 KiB: the 16 tiles' operands, then V
+  __shared__ float2 wtab[32];  // window: (cos, sin) of theta_A(m2) / 2
+  // per-thread constants (kept in LDS, not in registers held across the
+  // segment loop): W_N^q0, W_N^q1, W_2N^q0, W_2N^q1 and the window angle
+  // theta_B of the thread's two reals; per lane: W1024^((4 h + j) col) and
+  // W1024^(8 j col)
+  __shared__ float2 cth[6][CM_T];
+  __shared__ float4 ctw[8][64];  // per lane: W1024^(m3 col) of registers 2 p, 2 p + 1
+  const DevStream &s = streams[blockIdx.y];
+  const int64_t sa = (int64_t)blockIdx.x * segs_per_wg;
+  int64_t sb = sa + segs_per_wg;
+  if (sb > job.n_segments) sb = job.n_segments;
+  if (sa >= sb) return;  // uniform over the workgroup
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, h = lane >> 5, col = lane & 31;
+  // theta / 2 = pi i / (2 N - 1): table of the half of theta_A = 2 pi 1024
+  // m2 / (2 N - 1), and the half of this thread's theta_B (c = 2 t + b)
+  if (t < 32) {
+    double sn, cs;
+    sincospi(1024.0 * (double)t / (double)(2 * CM_N - 1), &sn, &cs);
+    wtab[t] = make_float2((float)cs, (float)sn);
+  }
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    double sn, cs;
+    sincospi((double)(2 * t + b) / (double)(2 * CM_N - 1), &sn, &cs);
+    cth[4 + b][t] = make_float2((float)cs, (float)sn);
+  }
+  CmWindow win{0.0f, 0.0f, 0.0f, job.window == SFXG_WINDOW_NONE, job.window == SFXG_WINDOW_RECT};
+  if (job.window == SFXG_WINDOW_NONE || job.window == SFXG_WINDOW_RECT) win.k0 = 1.0f;
+  else if (job.window == SFXG_WINDOW_HANN) win.k2 = 1.0f;
+  else if (job.window == SFXG_WINDOW_HAMMING) win.k0 = 0.08f, win.k2 = 0.92f;
+  else win.k1 = 1.0f;  // COS
+  // DFT-32 fragments: lane holds F[16 g + cm_kslot(h, e)][col] (e = 0..7),
+  // F[a][b] = W32^(a b): real parts, imaginary parts and their negatives
+  cm_h8 fr[2], fi[2], nfi[2];
+#pragma unroll
+  for (int g = 0; g < 2; g++)
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const float2 w = cm_w((16 * g + cm_kslot(h, e)) * col, 32);
+      fr[g][e] = (_Float16)w.x;
+      fi[g][e] = (_Float16)w.y;
+      nfi[g][e] = (_Float16)-w.y;
+    }
+  // W1024^(m3 col) of the B1 result in register r (m3 = 8 (r / 4) + 4 h + r % 4)
+  if (t < 64) {
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+      const float2 a = cm_w((8 * (p >> 1) + 4 * h + 2 * (p & 1)) * col, 1024);
+      const float2 b = cm_w((8 * (p >> 1) + 4 * h + 2 * (p & 1) + 1) * col, 1024);
+      ctw[p][t] = make_float4(a.x, a.y, b.x, b.y);
+    }
+  }
+  // the last stage's rows: q0 = t and q1 = 1024 - t (thread 0: 0 and 512)
+  {
+    const int q0 = t, q1 = t ? 1024 - t : 512;
+    cth[0][t] = cm_w(q0, CM_N);
+    cth[1][t] = cm_w(q1, CM_N);
+    cth[2][t] = cm_w(q0, 2 * CM_N);
+    cth[3][t] = cm_w(q1, 2 * CM_N);
+  }
+  const bool flip = s.flip;
+  const float2 *ts = reinterpret_cast<const float2 *>(s.time_series);
+  uint2 *vlds = reinterpret_cast<uint2 *>(lds);
+  // thread t = 16 m3 + n1 takes z[512 m2 + t] for the 32 m2 of its row: its
+  // reals 1024 m2 + 2 t + b; point i = 16 g + 8 hh + e has m2 = 16 g +
+  // cm_kslot(hh, e). xp[0..15]: the first half, xp[16..31]: the second.
+  float2 xp[32], xn[16];
+  // the 16 points of one half (g) of segment sg into dst
+  auto load_half = [&](float2 (&dst)[16], int64_t sg, int g) {
+    const int to = opaque(threadIdx.x);
+    const float2 *zs = ts + sg * (CM_N / 2);
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      dst[i] = gload2(zs, (uint32_t)(512 * (16 * g + cm_kslot(i >> 3, i & 7)) + to) * (uint32_t)sizeof(float2));
+  };
+  const bool carry = !win.zero_hi;
+  __syncthreads();  // wtab
+  {
+    float2 tmp[16];
+    load_half(tmp, sa, 0);
+#pragma unroll
+    for (int i = 0; i < 16; i++) xp[i] = tmp[i];
+    if (carry) load_half(xn, sa, 1);
+  }
+  if (carry) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) xp[16 + i] = xn[i];
+  }
+  // complete here, so that no wait at the loop top (where the previous
+  // segment's stores are in flight) is needed for them
+#pragma unroll
+  for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i].x), "v"(xp[i].y));
+  for (int64_t seg = sa; seg < sb; seg++) {
+    // phase 0: windowed (and sideband-flipped) points as FP16 K chunks
+    cm_phase0(lds, xp, wtab, cth, win, flip);
+    // with a window this segment's second half is the next one's first
+    if (carry) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) xp[i] = xp[16 + i];
+    }
+    __syncthreads();
+    // the next segment's new points: the second half with a window, else the
+    // whole used first half; they complete before this segment's spectrum
+    // stores, so no wait for them also waits for stores
+    const bool more = seg + 1 < sb;
+    if (more) load_half(xn, seg + 1, carry ? 1 : 0);
+    // wave w takes tiles 2 w and 2 w + 1 (K slices of their A operands: real
+    // parts of m2 0..15, 16..31, then the imaginary parts) and leaves their V
+    // in its own 8 KiB of LDS, which held just those two tiles: no other
+    // wave reads it before the next barrier
+    {
+      const int lo = opaque(lane), colo = lo & 31, ho = lo >> 5;
+      uint32_t v0[16];
+#pragma unroll
+      for (int tt = 0; tt < 2; tt++) {
+        cm_h8 a[4];
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++)
+          a[sl] = *reinterpret_cast<const cm_h8 *>(&lds[cm_chunk(2 * wave + tt, colo, 2 * sl + ho)]);
+        cm_f16 ur = {}, ui = {};
+        ur = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0], fr[0], ur, 0, 0, 0);
+        ur = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[1], fr[1], ur, 0, 0, 0);
+        ur = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[2], nfi[0], ur, 0, 0, 0);
+        ur = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[3], nfi[1], ur, 0, 0, 0);
+        ui = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0], fi[0], ui, 0, 0, 0);
+        ui = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[1], fi[1], ui, 0, 0, 0);
+        ui = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[2], fr[0], ui, 0, 0, 0);
+        ui = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[3], fr[1], ui, 0, 0, 0);
+        // U[m3][col] W1024^(m3 col) as B2's B operand: K slice g2 holds
+        // registers 8 g2 .. 8 g2 + 7 (rows m3 = 16 g2 + cm_kslot(h, e))
+        cm_h8 br[2], bi[2];
+#pragma unroll
+        for (int p = 0; p < 8; p++) {
+          const float4 w = ctw[p][lo];
+#pragma unroll
+          for (int q = 0; q < 2; q++) {
+            const int r = 2 * p + q, g = r >> 2, j = r & 3;
+            const float2 u = cmul(make_float2(ur[r], ui[r]), q ? make_float2(w.z, w.w) : make_float2(w.x, w.y));
+            br[g >> 1][4 * (g & 1) + j] = (_Float16)u.x;
+            bi[g >> 1][4 * (g & 1) + j] = (_Float16)u.y;
+          }
+        }
+        cm_f16 vr = {}, vi = {};
+        vr = __builtin_amdgcn_mfma_f32_32x32x16_f16(fr[0], br[0], vr, 0, 0, 0);
+        vr = __builtin_amdgcn_mfma_f32_32x32x16_f16(fr[1], br[1], vr, 0, 0, 0);
+        vr = __builtin_amdgcn_mfma_f32_32x32x16_f16(nfi[0], bi[0], vr, 0, 0, 0);
+        vr = __builtin_amdgcn_mfma_f32_32x32x16_f16(nfi[1], bi[1], vr, 0, 0, 0);
+        vi = __builtin_amdgcn_mfma_f32_32x32x16_f16(fi[0], br[0], vi, 0, 0, 0);
+        vi = __builtin_amdgcn_mfma_f32_32x32x16_f16(fi[1], br[1], vi, 0, 0, 0);
+        vi = __builtin_amdgcn_mfma_f32_32x32x16_f16(fr[0], bi[0], vi, 0, 0, 0);
+        vi = __builtin_amdgcn_mfma_f32_32x32x16_f16(fr[1], bi[1], vi, 0, 0, 0);
+        // V[q] with q = col + 32 k3, k3 = 8 g + 4 h + j in register 4 g + j
+        if (tt == 0) {
+#pragma unroll
+          for (int r = 0; r < 16; r++) v0[r] = cm_pack(vr[r], vi[r]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const int q = colo + 32 * (8 * (r >> 2) + 4 * ho + (r & 3));
+            vlds[wave * 1024 + q] = make_uint2(v0[r], cm_pack(vr[r], vi[r]));
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // last stage and split: rows q0 and q1 hold bins k and N - k
+    {
+      const int tq = opaque(threadIdx.x), r0 = tq, r1 = tq ? 1024 - tq : 512;
+      float2 z0[16], z1[16];
+      cm_read_row(vlds, r0, z0);
+      cm_read_row(vlds, r1, z1);
+      cm_last_stage(z0, cth[0][tq]);
+      cm_last_stage(z1, cth[1][tq]);
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          asm volatile("" ::"v"(xn[i].x), "v"(xn[i].y));  // the loads complete here
+          if (carry) xp[16 + i] = xn[i];
+          else xp[i] = xn[i];
+        }
+      }
+      uint32_t *row = reinterpret_cast<uint32_t *>(s.spectra) + seg * job.spectra_stride;
+      const float hs = 0.5f * job.spec_scale;
+      if (tq != 0) {
+        const float2 w0 = cth[2][tq];
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) {
+          float2 xk, xn;
+          cm_split(z0[k1], z1[15 - k1], cmul(w0, cm_w32(k1)), hs, xk, xn);
+          cm_put<SP>(row, (uint32_t)(r0 + 1024 * k1), xk);
+          cm_put<SP>(row, (uint32_t)(r1 + 1024 * (15 - k1)), xn);
+        }
+      } else {
+        // q = 0 pairs bins 1024 k1 and 1024 (16 - k1), and gives bin N;
+        // q = 512 pairs 512 + 1024 k1 and 512 + 1024 (15 - k1)
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) {
+          float2 xk, xn;
+          cm_split(z0[k1], z0[(16 - k1) & 15], cm_w32(k1), hs, xk, xn);
+          cm_put<SP>(row, (uint32_t)(1024 * k1), xk);
+          if (k1 == 0) cm_put<SP>(row, (uint32_t)CM_N, xn);
+          cm_split(z1[k1], z1[15 - k1], cmul(cth[3][tq], cm_w32(k1)), hs, xk, xn);
+          cm_put<SP>(row, (uint32_t)(512 + 1024 * k1), xk);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Channeliser for N = 32768 complex points (2 N_c = 65536 reals), which do
 // not fit in LDS: a four-step FFT through HBM scratch, N = 128 x 256,
 //   X[k1 + 128 k2] = sum_n2 W256^(n2 k2) W_N^(n2 k1) sum_n1 x[256 n1 + n2] W128^(n1 k1).
@@ -3084,6 +3475,24 @@ static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, c
   return hipGetLastError();
 }
 
+// the MFMA channeliser (N_c = 16384, packed FP16 / BF16 spectra): runs of
+// consecutive segments, about chan_wgs (1024) workgroups in all, two per CU
+template <int SP>
+static hipError_t launch_chan_mfma(const DevStream *d_streams, const DevJob &job, const Tune &t, hipStream_t s) {
+  if (job.seg_step != CM_N) return hipErrorInvalidValue;
+  const int64_t wgs = t.chan_wgs > 0 ? t.chan_wgs : 512;
+  const int64_t per_stream = (wgs + job.n_streams - 1) / job.n_streams;
+  int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
+  if (segs < 1) segs = 1;
+  const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
+  hipLaunchKernelGGL(k_chan_mfma<SP>, dim3(gx, job.n_streams), dim3(CM_T), 0, s, d_streams, job, (int)segs);
+  return hipGetLastError();
+}
+
+bool chan_mfma_supported(int nc, int spec_prec) {
+  return nc == CM_N && (spec_prec == SFXG_PREC_FP16 || spec_prec == SFXG_PREC_BF16);
+}
+
 size_t channelise_scratch_elems(int nc) { return nc == LG_N ? (size_t)2 * 1024 * LG_N : 0; }
 
 template <int SP>
@@ -3110,6 +3519,9 @@ static hipError_t launch_channelise_sp(const DevStream *d_streams, const DevJob 
                              const float2 *tw_n, const float2 *tw_2n, const float2 *tw_split, float2 *scratch,
                              const Tune &t, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
+  if constexpr (SP == SFXG_PREC_FP16 || SP == SFXG_PREC_BF16) {
+    if (job.nc == CM_N && t.chan_fp32 != 1) return launch_chan_mfma<SP>(d_streams, job, t, s);
+  }
   if (job.nc == LG_N) return launch_chan_large<SP>(d_streams, job, window, tw_n, tw_2n, scratch, s);
   switch (job.nc) {
     case 256: return launch_chan_n<256, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
@@ -3148,6 +3560,7 @@ std::vector<int> fe_thread_candidates(int nc) {
     default: return {0};
   }
 }
+std::vector<int> chan_mfma_wgs_candidates() { return {512, 256, 1024}; }
 std::vector<int> chan_wgs_candidates(int nc) {
   if (nc == 8192 || nc == 16384) return {2048, 1024, 4096};
   return {0};
