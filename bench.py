@@ -43,15 +43,16 @@ FP16_DENSE_PEAK_TFLOPS = 2500.0
 # summaries under profiles/ give the commands); re-measured whenever the
 # kernel changes. Key: (kernel, channels, streams, precision).
 MEASURED_TRAFFIC = {
-    # 8.320e9 read + 8.193e9 written per chunk of 1952 segments x 64 streams
-    ("channelise", 16384, 64, "fp16"): 16513000000,
+    # MFMA channeliser: 8.210e9 read + 8.198e9 written per chunk of 1953
+    # segments x 64 streams
+    ("channelise", 16384, 64, "fp16"): 16408000000,
 }
 # Wave-level VALU instructions per launch (SQ_INSTS_VALU, same profiles),
 # against the wave64 FP32 issue rate: a gfx950 SIMD issues up to two
 # independent VALU instructions per quad-cycle (SQ_ACTIVE_INST_VALU2 counts
 # those), so 256 CUs x 4 SIMDs x 2 / 4 cycles x 2.4 GHz.
 MEASURED_VALU_INSTS = {
-    ("channelise", 16384, 64, "fp16"): 3.075e9,
+    ("channelise", 16384, 64, "fp16"): 1.940e9,
 }
 VALU_ISSUE_PEAK = 256 * 4 * 2 / 4 * 2.4e9
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
@@ -739,8 +740,9 @@ def main():
             "scaling": "weak",
             "vs_baseline": None,
             "dtype": head,
-            "dtype_detail": f"{head} X-step operands on MFMA with FP32 accumulation; FP32 F-engine; FP64 delay "
-                            "model and phase arguments",
+            "dtype_detail": f"{head} X-step operands on MFMA with FP32 accumulation; FP32 delay correction; "
+                            "channeliser transform on FP16 MFMA operands with FP32 accumulation (packed FP16 / BF16 "
+                            "spectra; FP32 transform for plain and split FP16); FP64 delay model and phase arguments",
             "data": "synthetic: seeded 2-bit station data generated in device memory (common signal rho 0.01 and a "
                     "second source rho 0.005 with its own delays), HEADER records from the input-node delay rule",
             "config": {"workload": "32 stations x 2 pol, 4 subbands x 64 MHz at 128 Ms/s, 2-bit, 16384 channels, "

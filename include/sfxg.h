@@ -50,7 +50,11 @@ extern "C" {
  * (src/correlation_core.cc:241-273). The others run the X-step as a
  * per-channel Hermitian rank-k update on MFMA with FP32 accumulation:
  * FP16 or BF16 operands, or FP16 operands split into hi + lo parts (three
- * products). Pulsar binning takes every precision (per-bin Gram tiles on MFMA). */
+ * products). Pulsar binning takes every precision (per-bin Gram tiles on MFMA).
+ * With FP16 or BF16 at fft_size_correlation 16384 the channeliser's transform
+ * also runs on MFMA with FP16 operands and FP32 accumulation (unless the
+ * tuning table's chan_fp32 selects the FP32 transform); the visibility error
+ * stays within the FP16 bar of 1e-3 relative to the FP64 pipeline. */
 #define SFXG_PREC_FP32 0
 #define SFXG_PREC_FP16 1
 #define SFXG_PREC_BF16 2
@@ -400,12 +404,15 @@ int sfxg_chunk_plan(const sfxg_ctx *ctx, int32_t *n_chunks, int64_t *max_segment
  * rounding of the FFT schedule. */
 int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap);
 /* The tuning table as text, one "shape|fe_threads|chan_wgs|xs_waves|mf_channels|
- * mfp_wgs|mf_ks" line per shape (0: default; a line without the last field
- * is read as mf_ks 0; xs_waves 1 and 2 force the register tiles on 64- and
- * 16-channel blocks, 4 and 8 the per-baseline kernel; with pulsar binning in
- * FP32, 2 forces the register tiles and 1, 4 or 8 the per-baseline kernel);
- * *written = bytes needed including the terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear empties
- * the table. */
+ * mfp_wgs|mf_ks|chan_fp32" line per shape (0: default; fields missing at the
+ * end of a line (at least five given) are read as 0; xs_waves 1 and 2 force
+ * the register tiles on 64- and 16-channel blocks, 4 and 8 the per-baseline
+ * kernel; with pulsar binning in FP32, 2 forces the register tiles and 1, 4
+ * or 8 the per-baseline kernel; chan_fp32 1 gives packed FP16 / BF16 spectra
+ * at 16384 channels the FP32 transform instead of the MFMA one, whose
+ * workgroup count chan_wgs then sets); *written = bytes needed including the
+ * terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear
+ * empties the table. */
 int sfxg_tune_export(char *dst, size_t cap, size_t *written);
 int sfxg_tune_import(const char *text);
 void sfxg_tune_clear(void);
