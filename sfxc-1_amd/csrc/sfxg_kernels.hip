@@ -884,8 +884,9 @@ __device__ __forceinline__ void fft256_g16(float2 (&v)[16], float2 *buf, const f
 // end): a block inside one payload run takes them with 16 byte loads at a
 // fixed stride and decodes them from the table in registers; a block that
 // crosses a run boundary or invalid data goes sample by sample.
+// bs: the block's block_src entry (loaded ahead by the caller)
 __device__ __forceinline__ void load_block16(const DevStream &s, int64_t blk, int64_t n_blocks, int l,
-                                             float (&x)[16]) {
+                                             float (&x)[16], int64_t bs) {
   constexpr int nd = 256;
   if (blk >= n_blocks) {
 #pragma unroll
@@ -893,7 +894,6 @@ __device__ __forceinline__ void load_block16(const DevStream &s, int64_t blk, in
     return;
   }
   const int64_t o0 = blk * nd;
-  const int64_t bs = cmem(s.block_src)[blk];
   if (bs >= 0) {
     const int64_t p0 = bs;
     const auto *pay = gmem(s.payload);
@@ -929,17 +929,29 @@ __device__ __forceinline__ void load_block16(const DevStream &s, int64_t blk, in
 // exp(-i (c + k l_term)) (sincos of bin l, then rotations by exp(-i 16
 // l_term)), transform back, fringe-rotate (the same scheme) and write the
 // 256 samples when `write` is set.
-__device__ __forceinline__ void finish_block16(const DevStream &s, int64_t blk, float2 (&v)[16], float2 *buf,
+// The per-block constants finish_block16 takes, loaded by the caller before
+// any store of the iteration (the vector memory counter is in order, so
+// waiting for loads issued after stores would also wait for the stores).
+struct BlockC16 {
+  double c_term, l_term, phi, dphi;
+  float amp, ramp16c, ramp16s, fringe16c, fringe16s;
+};
+__device__ __forceinline__ BlockC16 block_c16(const DevStream &s, int64_t blk) {
+  const auto *bcp = cmem(s.block_consts) + blk;
+  return BlockC16{bcp->c_term, bcp->l_term, bcp->phi, bcp->dphi, bcp->amp,
+                  bcp->ramp_step16[0], bcp->ramp_step16[1], bcp->fringe_step16[0], bcp->fringe_step16[1]};
+}
+
+__device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[16], float2 *buf,
                                                const float2 *tw, int l, float *out, bool write) {
   constexpr int nd = 256;
-  const auto *bcp = cmem(s.block_consts) + blk;
-  const double c_term = bcp->c_term, l_term = bcp->l_term, phi = bcp->phi, dphi = bcp->dphi;
-  const float amp = bcp->amp;
+  const double c_term = bc.c_term, l_term = bc.l_term, phi = bc.phi, dphi = bc.dphi;
+  const float amp = bc.amp;
   {
     float sn, cs;
     sincosf((float)(c_term + l * l_term), &sn, &cs);
     float2 e = make_float2(cs, -sn);
-    const float2 step = make_float2(bcp->ramp_step16[0], -bcp->ramp_step16[1]);
+    const float2 step = make_float2(bc.ramp16c, -bc.ramp16s);
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int k = l + 16 * q;
@@ -956,7 +968,7 @@ __device__ __forceinline__ void finish_block16(const DevStream &s, int64_t blk, 
   float sn, cs;
   sincosf((float)(phi + l * dphi), &sn, &cs);
   float2 r = make_float2(cs, sn);
-  const float2 rstep = make_float2(bcp->fringe_step16[0], bcp->fringe_step16[1]);
+  const float2 rstep = make_float2(bc.fringe16c, bc.fringe16s);
   auto *o = gmem(out);
 #pragma unroll
   for (int q = 0; q < 16; q++) {
@@ -982,12 +994,20 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
   float2 *buf = lds[wave][g];
   const DevStream &s = streams[blockIdx.y];
   const int64_t end = job.blk_end < job.n_blocks ? job.blk_end : job.n_blocks;
-  for (int64_t blk = job.blk_base + ((int64_t)blockIdx.x * 4 + wave) * 8; blk < end;
-       blk += (int64_t)gridDim.x * 32) {
+  // block_src of this group's pair, loaded one iteration ahead, so that the
+  // payload loads at the top of an iteration depend on no load of their own
+  const int64_t step = (int64_t)gridDim.x * 32;
+  auto src_of = [&](int64_t b) { return b < end ? cmem(s.block_src)[b] : (int64_t)0; };
+  int64_t blk = job.blk_base + ((int64_t)blockIdx.x * 4 + wave) * 8;
+  int64_t bsa = src_of(blk + 2 * g), bsb = src_of(blk + 2 * g + 1);
+  for (; blk < end; blk += step) {
     const int64_t ba = blk + 2 * g;  // this group's pair (ba, ba + 1)
     float xa[16], xb[16];
-    load_block16(s, ba, ba < end ? job.n_blocks : 0, l, xa);
-    load_block16(s, ba + 1, ba + 1 < end ? job.n_blocks : 0, l, xb);
+    load_block16(s, ba, ba < end ? job.n_blocks : 0, l, xa, bsa);
+    load_block16(s, ba + 1, ba + 1 < end ? job.n_blocks : 0, l, xb, bsb);
+    bsa = src_of(ba + step);
+    bsb = src_of(ba + step + 1);
+    const BlockC16 ca = block_c16(s, ba < end ? ba : 0), cb = block_c16(s, ba + 1 < end ? ba + 1 : 0);
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) v[q] = make_float2(xa[q], xb[q]);
@@ -1008,8 +1028,8 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
       vb[q] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
     }
     float *base = s.time_series + (ba - job.blk_base) * 256;
-    finish_block16(s, ba < end ? ba : 0, va, buf, tw, l, base, ba < end);
-    finish_block16(s, ba + 1 < end ? ba + 1 : 0, vb, buf, tw, l, base + 256, ba + 1 < end);
+    finish_block16(ca, va, buf, tw, l, base, ba < end);
+    finish_block16(cb, vb, buf, tw, l, base + 256, ba + 1 < end);
   }
 }
 
@@ -1410,10 +1430,11 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
     for (int i = 0; i < 16; i++) xp[i] = tmp[i];
     if (carry) load_half(xn, sa, 1);
   }
-  if (carry) {
+  // without a window the second half is zero padding: never loaded, and
+  // zero (not left undefined: a non-finite value times the zero weight would
+  // give NaN)
 #pragma unroll
-    for (int i = 0; i < 16; i++) xp[16 + i] = xn[i];
-  }
+  for (int i = 0; i < 16; i++) xp[16 + i] = carry ? xn[i] : make_float2(0.0f, 0.0f);
   // complete here, so that no wait at the loop top (where the previous
   // segment's stores are in flight) is needed for them
 #pragma unroll

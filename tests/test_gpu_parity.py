@@ -236,6 +236,25 @@ def test_pulsar_bins(ctx):
         compare_vis(vis_g[b], vis_r[b], label=f"pulsar bin {b}")
 
 
+def test_pulsar_bin_limit(ctx):
+    """More than 32766 on-pulse bins are refused at set-up, before any
+    launch (the mixed-precision pulsar kernels hold bin indices in 16 bits);
+    the context then correlates the next slice as usual."""
+    from sfxg.job import Pulsar, ticks_to_mjd
+    job, hdrs = configs.build_slices(seed=10, n_stations=2, pols=["R"], fs=32_000_000, bw=16_000_000,
+                                     nchan=256, seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2,
+                                     sky_freq=1_400_000_000, offsets=[0.0, 1e-6], rates=[0.0, 1e-7])[0]
+    for prec in (0, 1):
+        ctx.set_options(lut=LUT_CONFIG, precision=prec)
+        psr = Pulsar(nbins=32767, gate_start=0.1, gate_stop=0.6, tmid=ticks_to_mjd(job.start_time) - 0.001,
+                     DM=68.0, ref_phase=0.3, ref_freq=1 / 0.0893, obs_freq=1400.0, coef=[0.0, 0.0])
+        with pytest.raises(capi.SfxgError, match="32766"):
+            ctx.set_parameters(job, pulsar=psr)
+    ctx.set_options(lut=LUT_CONFIG, precision=0)
+    vis_g, vis_r, _ = run_both(ctx, job, hdrs)
+    compare_vis(vis_g[0], vis_r[0], label="after a refused pulsar")
+
+
 @pytest.mark.parametrize("n_stations,nchan,chunk_mib", [(12, 512, 0), (12, 512, 1), (3, 256, 0)])
 def test_pulsar_tiles_match_per_baseline(ctx, n_stations, nchan, chunk_mib):
     """The FP32 pulsar X-step by bin on register tiles of 16 channels
