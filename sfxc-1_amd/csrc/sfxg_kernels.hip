@@ -1398,13 +1398,19 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
       ctw[p][t] = make_float4(a.x, a.y, b.x, b.y);
     }
   }
-  // the last stage's rows: q0 = t and q1 = 1024 - t (thread 0: 0 and 512)
+  // the last stage's rows: q0 = t and q1 = 1024 - t. Thread 0 takes row 0
+  // twice, the second time as "row 1024": its twiddles W_N^(1024 n1) =
+  // W16^n1 shift the 16-point result by one bin, so that the pairing of
+  // bins k1 and 15 - k1 below pairs row 0's bins k1 and 16 - k1 and yields
+  // bin N. Row 512 (its bins k1 and 15 - k1 pair with each other) goes to
+  // lane 0 of wave 0 besides.
   {
-    const int q0 = t, q1 = t ? 1024 - t : 512;
+    const int q0 = t, q1 = 1024 - t;
     cth[0][t] = cm_w(q0, CM_N);
     cth[1][t] = cm_w(q1, CM_N);
     cth[2][t] = cm_w(q0, 2 * CM_N);
-    cth[3][t] = cm_w(q1, 2 * CM_N);
+    // row 512's constants: W_N^512 (last stage) and W_2N^512 (split)
+    cth[3][t] = t == 0 ? cm_w(512, CM_N) : t == 1 ? cm_w(512, 2 * CM_N) : make_float2(0.0f, 0.0f);
   }
   const bool flip = s.flip;
   const float2 *ts = reinterpret_cast<const float2 *>(s.time_series);
@@ -1514,10 +1520,10 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
     __syncthreads();
     // last stage and split: rows q0 and q1 hold bins k and N - k
     {
-      const int tq = opaque(threadIdx.x), r0 = tq, r1 = tq ? 1024 - tq : 512;
+      const int tq = opaque(threadIdx.x), r0 = tq, r1 = 1024 - tq;
       float2 z0[16], z1[16];
       cm_read_row(vlds, r0, z0);
-      cm_read_row(vlds, r1, z1);
+      cm_read_row(vlds, r1 & 1023, z1);
       cm_last_stage(z0, cth[0][tq]);
       cm_last_stage(z1, cth[1][tq]);
       if (more) {
@@ -1530,26 +1536,25 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
       }
       uint32_t *row = reinterpret_cast<uint32_t *>(s.spectra) + seg * job.spectra_stride;
       const float hs = 0.5f * job.spec_scale;
-      if (tq != 0) {
-        const float2 w0 = cth[2][tq];
+      const float2 w0 = cth[2][tq];
 #pragma unroll
-        for (int k1 = 0; k1 < 16; k1++) {
-          float2 xk, xn;
-          cm_split(z0[k1], z1[15 - k1], cmul(w0, cm_w32(k1)), hs, xk, xn);
-          cm_put<SP>(row, (uint32_t)(r0 + 1024 * k1), xk);
-          cm_put<SP>(row, (uint32_t)(r1 + 1024 * (15 - k1)), xn);
-        }
-      } else {
-        // q = 0 pairs bins 1024 k1 and 1024 (16 - k1), and gives bin N;
-        // q = 512 pairs 512 + 1024 k1 and 512 + 1024 (15 - k1)
+      for (int k1 = 0; k1 < 16; k1++) {
+        float2 xk, xn;
+        cm_split(z0[k1], z1[15 - k1], cmul(w0, cm_w32(k1)), hs, xk, xn);
+        cm_put<SP>(row, (uint32_t)(r0 + 1024 * k1), xk);
+        cm_put<SP>(row, (uint32_t)(r1 + 1024 * (15 - k1)), xn);
+      }
+      if (tq == 0) {
+        // row 512: bins 512 + 1024 k1 and 512 + 1024 (15 - k1) pair
+        cm_read_row(vlds, 512, z1);
+        cm_last_stage(z1, cth[3][0]);
+        const float2 w5 = cth[3][1];
 #pragma unroll
-        for (int k1 = 0; k1 < 16; k1++) {
+        for (int k1 = 0; k1 < 8; k1++) {
           float2 xk, xn;
-          cm_split(z0[k1], z0[(16 - k1) & 15], cm_w32(k1), hs, xk, xn);
-          cm_put<SP>(row, (uint32_t)(1024 * k1), xk);
-          if (k1 == 0) cm_put<SP>(row, (uint32_t)CM_N, xn);
-          cm_split(z1[k1], z1[15 - k1], cmul(cth[3][tq], cm_w32(k1)), hs, xk, xn);
+          cm_split(z1[k1], z1[15 - k1], cmul(w5, cm_w32(k1)), hs, xk, xn);
           cm_put<SP>(row, (uint32_t)(512 + 1024 * k1), xk);
+          cm_put<SP>(row, (uint32_t)(512 + 1024 * (15 - k1)), xn);
         }
       }
     }
