@@ -94,6 +94,7 @@ struct LaunchPlan {
   int64_t chunk_segs = 0;   // spectra rows per stream (largest chunk)
   int64_t ts_stride = 0;    // real-series floats per stream (largest chunk)
   bool fused = false, mixed = false;
+  bool key_fused = false;   // the shape's tuning key: N_c within the fused F-engine's range
   int k_group = 1, psr_run = 0;
   size_t vis_elems = 0;
   int64_t max_counted = 0;
@@ -1056,7 +1057,19 @@ static int prepare_slice(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->block_consts.ensure((size_t)M * ctx->n_blocks * sizeof(BlockConst)));
   HIPCHK(ctx, ctx->model_buf.ensure((size_t)mod_tot * sizeof(double)));
   HIPCHK(ctx, ctx->dstreams.ensure((size_t)M * sizeof(DevStream)));
-  const bool fused = !ctx->opt.keep_time_series && fengine_supported(nc);
+  // the fused F-engine for N_c <= 4096, unless packed FP16 / BF16 spectra
+  // take the MFMA channeliser (delay stage and channeliser unfused; the
+  // tuning table's chan_fp32 = 1 keeps the fused one)
+  const bool key_fused = !ctx->opt.keep_time_series && fengine_supported(nc);
+  {
+    const std::string key = tune_key(ctx, key_fused);
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    ctx->tune = it != g_tune.end() ? it->second : Tune();
+  }
+  const int packed = ctx->opt.precision == SFXG_PREC_FP16 || ctx->opt.precision == SFXG_PREC_BF16
+                         ? ctx->opt.precision : 0;
+  const bool fused = key_fused && !(chan_mfma_supported(nc, packed) && ctx->tune.chan_fp32 == 0);
   // FP16 and BF16 X-steps take their operands straight from spectra the
   // channeliser stores in that type (4 bytes per bin instead of 8); split
   // FP16 and FP32 need the FP32 spectra
@@ -1146,12 +1159,6 @@ static int prepare_slice(sfxg_ctx *ctx) {
     HIPCHK(ctx, ctx->psr_masks.ensure(std::max<size_t>(words, 1) * sizeof(uint32_t)));
   }
 
-  {
-    const std::string key = tune_key(ctx, fused);
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    auto it = g_tune.find(key);
-    ctx->tune = it != g_tune.end() ? it->second : Tune();
-  }
 
   // host staging stays alive in the context until the next slice, so the
   // copies and kernels below run asynchronously
@@ -1287,6 +1294,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   }
   // launch parameters of the device work
   lp.fused = fused;
+  lp.key_fused = key_fused;
   lp.mixed = mixed;
   lp.k_group = k_group;
   lp.psr_run = psr_run;
@@ -2056,30 +2064,33 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
   // F-engine: threads per workgroup (N_c <= 4096), or the large-N
   // channeliser's workgroups; packed FP16 / BF16 spectra at N_c = 16384 also
   // between the MFMA transform (chan_fp32 = 0) and the FP32 one
+  // (the fused F-engine or not is the prepared plan's: chan_fp32 = 1 in the
+  // table gives packed spectra at N_c <= 4096 the fused one at the next
+  // preparation)
   const bool small = nc <= 4096;
-  std::vector<std::pair<int, int>> fc;  // (chan_fp32, threads or workgroups)
-  if (small) {
-    for (int v : fe_thread_candidates(nc)) fc.push_back({0, v});
+  const bool mf = chan_mfma_supported(nc, ctx->opt.precision);
+  const bool fe = lp.fused || (small && !mf);  // candidates are F-engine thread counts
+  std::vector<std::pair<int, int>> fc;          // (chan_fp32, threads or workgroups)
+  if (fe) {
+    for (int v : fe_thread_candidates(nc)) fc.push_back({ctx->tune.chan_fp32, v});
   } else {
-    const bool mf = chan_mfma_supported(nc, ctx->opt.precision);
     if (mf)
       for (int v : chan_mfma_wgs_candidates()) fc.push_back({0, v});
-    for (int v : chan_wgs_candidates(nc)) fc.push_back({mf ? 1 : 0, v});
+    if (!small)
+      for (int v : chan_wgs_candidates(nc)) fc.push_back({mf ? 1 : 0, v});
   }
   if (fc.size() > 1) {
     float best_ms = -1.0f;
     Tune pick = best;
     for (const auto &c : fc) {
       Tune t = best;
-      (small ? t.fe_threads : t.chan_wgs) = c.second;
+      (fe ? t.fe_threads : t.chan_wgs) = c.second;
       t.chan_fp32 = c.first;
       float ms = 0.0f;
       const int rc = time_stage(ctx, reps, [&] { return enqueue_fengine(ctx, 0, ctx->stream, false, t); }, &ms);
       if (rc != SFXG_OK) return rc;
-      snprintf(line, sizeof(line), "%s%s=%d: %.3f ms\n", small ? "fe_threads" : "chan_wgs",
-               !small && chan_mfma_supported(nc, ctx->opt.precision) ? (c.first ? " (FP32 transform)" : " (MFMA)")
-                                                                     : "",
-               c.second, ms);
+      snprintf(line, sizeof(line), "%s%s=%d: %.3f ms\n", fe ? "fe_threads" : "chan_wgs",
+               !fe && mf ? (c.first ? " (FP32 transform)" : " (MFMA)") : "", c.second, ms);
       rep += line;
       if (best_ms < 0 || ms < best_ms) {
         best_ms = ms;
@@ -2132,7 +2143,7 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[tune_key(ctx, lp.fused)] = best;
+    g_tune[tune_key(ctx, lp.key_fused)] = best;
   }
   ctx->tune = best;
   // the trial runs overwrote the slice's buffers

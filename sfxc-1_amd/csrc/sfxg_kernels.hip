@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 #include <vector>
 
@@ -1195,7 +1196,6 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int CM_N = 16384;
 constexpr int CM_T = 512;
 typedef _Float16 cm_h8 __attribute__((ext_vector_type(8)));
 typedef float cm_f16 __attribute__((ext_vector_type(16)));
@@ -1235,27 +1235,37 @@ __device__ __forceinline__ float2 cm_unpack(uint32_t w) {
                      (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)));
 }
 
-// Row q of V from LDS: wave w's region holds the pairs (tiles 2 w, 2 w + 1)
-// of every q, 8 bytes each.
-__device__ __forceinline__ void cm_read_row(const uint2 *vlds, int q, float2 (&z)[16]) {
+// Row q of segment sig of the pass from LDS: tile tau = NT sig + n1 lies in
+// wave tau / 2's region, which holds the pairs (tiles 2 w, 2 w + 1) of every
+// q, 8 bytes each.
+template <int NT>
+__device__ __forceinline__ void cm_read_rows(const uint2 *vlds, int sig, int q, float2 (&z)[NT]) {
+  if constexpr (NT == 1) {
+    const uint2 u = vlds[(sig >> 1) * 1024 + q];
+    z[0] = cm_unpack(sig & 1 ? u.y : u.x);
+  } else {
 #pragma unroll
-  for (int w = 0; w < 8; w++) {
-    const uint2 u = vlds[w * 1024 + q];
-    z[2 * w] = cm_unpack(u.x);
-    z[2 * w + 1] = cm_unpack(u.y);
+    for (int w = 0; w < NT / 2; w++) {
+      const uint2 u = vlds[(sig * NT / 2 + w) * 1024 + q];
+      z[2 * w] = cm_unpack(u.x);
+      z[2 * w + 1] = cm_unpack(u.y);
+    }
   }
 }
 
-// z[n1] *= w^n1 for n1 = 1..15 (running powers), then the 16-point DFT:
+// z[n1] *= w^n1 for n1 = 1..NT - 1 (running powers), then the NT-point DFT:
 // z[k1] = Z[q + 1024 k1].
-__device__ __forceinline__ void cm_last_stage(float2 (&z)[16], float2 w) {
-  float2 p = w;
+template <int NT>
+__device__ __forceinline__ void cm_last_stage(float2 (&z)[NT], float2 w) {
+  if constexpr (NT > 1) {
+    float2 p = w;
 #pragma unroll
-  for (int i = 1; i < 16; i++) {
-    z[i] = cmul(z[i], p);
-    if (i < 15) p = cmul(p, w);
+    for (int i = 1; i < NT; i++) {
+      z[i] = cmul(z[i], p);
+      if (i < NT - 1) p = cmul(p, w);
+    }
+    dft<NT, -1>(z);
   }
-  dft<16, -1>(z);
 }
 
 // Real-transform bins from Z[k] and Z[N - k]: X[k] = E + W_2N^k O and
@@ -1341,16 +1351,18 @@ __device__ __forceinline__ void cm_phase0(uint4 *lds, const float2 (&xp)[32], co
 // segment's spectrum stores (the vector memory counter is in order, so
 // waiting for them does not wait for the stores). Without a window a segment
 // uses only its first half and nothing carries. One workgroup per CU.
-template <int SP>
+template <int SP, int NT>
 __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restrict__ streams, DevJob job,
-                                                                int segs_per_wg) {
+                                                      int segs_per_wg) {
   static_assert(SP == SFXG_PREC_FP16 || SP == SFXG_PREC_BF16, "packed FP16 / BF16 spectra");
+  static_assert(NT == 1 || NT == 4 || NT == 16, "1024, 4096 or 16384 complex points");
+  constexpr int N = 1024 * NT;  // complex points of a segment (N_c)
+  constexpr int SG = 16 / NT;   // segments per pass (16 tiles)
   __shared__ uint4 lds[4096];  // 64 KiB: the 16 tiles' operands, then V
   __shared__ float2 wtab[32];  // window: (cos, sin) of theta_A(m2) / 2
   // per-thread constants (kept in LDS, not in registers held across the
-  // segment loop): W_N^q0, W_N^q1, W_2N^q0, W_2N^q1 and the window angle
-  // theta_B of the thread's two reals; per lane: W1024^((4 h + j) col) and
-  // W1024^(8 j col)
+  // segment loop): W_N^q0, W_N^q1, W_2N^q0, row 512's constants and the
+  // window angle theta_B / 2 of the thread's two reals
   __shared__ float2 cth[6][CM_T];
   __shared__ float4 ctw[8][64];  // per lane: W1024^(m3 col) of registers 2 p, 2 p + 1
   const DevStream &s = streams[blockIdx.y];
@@ -1359,18 +1371,24 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
   if (sb > job.n_segments) sb = job.n_segments;
   if (sa >= sb) return;  // uniform over the workgroup
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, h = lane >> 5, col = lane & 31;
-  // theta / 2 = pi i / (2 N - 1): table of the half of theta_A = 2 pi 1024
-  // m2 / (2 N - 1), and the half of this thread's theta_B (c = 2 t + b)
+  // thread t = 16 m3 + tau takes tile tau = NT sigma + n1 (segment sigma of
+  // the pass, residue n1): points n = NT (32 m2 + m3) + n1, reals
+  // i = 64 NT m2 + 2 (NT m3 + n1) + b. theta / 2 = pi i / (2 N - 1): table of
+  // the half of theta_A = 2 pi 64 NT m2 / (2 N - 1), and the half of this
+  // thread's theta_B
   if (t < 32) {
     double sn, cs;
-    sincospi(1024.0 * (double)t / (double)(2 * CM_N - 1), &sn, &cs);
+    sincospi(64.0 * NT * (double)t / (double)(2 * N - 1), &sn, &cs);
     wtab[t] = make_float2((float)cs, (float)sn);
   }
+  {
+    const int tau = t & 15, m3 = t >> 4, n1 = tau % NT;
 #pragma unroll
-  for (int b = 0; b < 2; b++) {
-    double sn, cs;
-    sincospi((double)(2 * t + b) / (double)(2 * CM_N - 1), &sn, &cs);
-    cth[4 + b][t] = make_float2((float)cs, (float)sn);
+    for (int b = 0; b < 2; b++) {
+      double sn, cs;
+      sincospi((double)(2 * (NT * m3 + n1) + b) / (double)(2 * N - 1), &sn, &cs);
+      cth[4 + b][t] = make_float2((float)cs, (float)sn);
+    }
   }
   CmWindow win{0.0f, 0.0f, 0.0f, job.window == SFXG_WINDOW_NONE, job.window == SFXG_WINDOW_RECT};
   if (job.window == SFXG_WINDOW_NONE || job.window == SFXG_WINDOW_RECT) win.k0 = 1.0f;
@@ -1400,65 +1418,95 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
   }
   // the last stage's rows: q0 = t and q1 = 1024 - t. Thread 0 takes row 0
   // twice, the second time as "row 1024": its twiddles W_N^(1024 n1) =
-  // W16^n1 shift the 16-point result by one bin, so that the pairing of
-  // bins k1 and 15 - k1 below pairs row 0's bins k1 and 16 - k1 and yields
-  // bin N. Row 512 (its bins k1 and 15 - k1 pair with each other) goes to
-  // lane 0 of wave 0 besides.
+  // W_NT^n1 shift the NT-point result by one bin, so that the pairing of
+  // bins k1 and NT - 1 - k1 below pairs row 0's bins k1 and NT - k1 and
+  // yields bin N. Row 512 (its bins k1 and NT - 1 - k1 pair with each other)
+  // goes to lane 0 of wave 0 besides.
   {
     const int q0 = t, q1 = 1024 - t;
-    cth[0][t] = cm_w(q0, CM_N);
-    cth[1][t] = cm_w(q1, CM_N);
-    cth[2][t] = cm_w(q0, 2 * CM_N);
+    cth[0][t] = cm_w(q0, N);
+    cth[1][t] = cm_w(q1, N);
+    cth[2][t] = cm_w(q0, 2 * N);
     // row 512's constants: W_N^512 (last stage) and W_2N^512 (split)
-    cth[3][t] = t == 0 ? cm_w(512, CM_N) : t == 1 ? cm_w(512, 2 * CM_N) : make_float2(0.0f, 0.0f);
+    cth[3][t] = t == 0 ? cm_w(512, N) : t == 1 ? cm_w(512, 2 * N) : make_float2(0.0f, 0.0f);
   }
   const bool flip = s.flip;
   const float2 *ts = reinterpret_cast<const float2 *>(s.time_series);
   uint2 *vlds = reinterpret_cast<uint2 *>(lds);
-  // thread t = 16 m3 + n1 takes z[512 m2 + t] for the 32 m2 of its row: its
-  // reals 1024 m2 + 2 t + b; point i = 16 g + 8 hh + e has m2 = 16 g +
-  // cm_kslot(hh, e). xp[0..15]: the first half, xp[16..31]: the second.
+  // point i = 16 g + 8 hh + e of the thread has m2 = 16 g + cm_kslot(hh, e);
+  // xp[0..15]: the first half of its segment, xp[16..31]: the second.
   float2 xp[32], xn[16];
-  // the 16 points of one half (g) of segment sg into dst
-  auto load_half = [&](float2 (&dst)[16], int64_t sg, int g) {
-    const int to = opaque(threadIdx.x);
-    const float2 *zs = ts + sg * (CM_N / 2);
-#pragma unroll
-    for (int i = 0; i < 16; i++)
-      dst[i] = gload2(zs, (uint32_t)(512 * (16 * g + cm_kslot(i >> 3, i & 7)) + to) * (uint32_t)sizeof(float2));
-  };
+  // With a window the second half of a segment is the first half of the
+  // next one, so a pass reads only the second halves of its segments (16
+  // points per thread): for NT = 16 (one segment per pass) the thread
+  // carries its second half into the next pass in registers; for NT < 16
+  // the first half of segment sigma comes from the thread of segment
+  // sigma - 1 (NT lanes lower in the same wave), and for sigma = 0 from the
+  // previous pass's last segment (NT (SG - 1) lanes higher), by ds_bpermute.
+  // Without a window every pass reads the first halves only (the second is
+  // zero padding).
   const bool carry = !win.zero_hi;
-  __syncthreads();  // wtab
-  {
-    float2 tmp[16];
-    load_half(tmp, sa, 0);
+  auto load_half = [&](float2 (&dst)[16], int64_t s0, int g) {
+    const int to = opaque(threadIdx.x);
+    const int tau = to & 15, m3 = to >> 4, sig = tau / NT, n1 = tau % NT;
+    const bool ok = s0 + sig < sb;
+    const float2 *zs = ts + (ok ? s0 + sig : s0) * (N / 2);
 #pragma unroll
-    for (int i = 0; i < 16; i++) xp[i] = tmp[i];
-    if (carry) load_half(xn, sa, 1);
-  }
+    for (int i = 0; i < 16; i++) {
+      const float2 v = gload2(
+          zs, (uint32_t)(NT * (32 * (16 * g + cm_kslot(i >> 3, i & 7)) + m3) + n1) * (uint32_t)sizeof(float2));
+      dst[i] = ok ? v : make_float2(0.0f, 0.0f);
+    }
+  };
+  auto load_pass = [&](int64_t s0) {  // without carry
+    float2 a[16];
+    load_half(a, s0, 0);
+#pragma unroll
+    for (int i = 0; i < 16; i++) xp[i] = a[i];
+    if (!win.zero_hi) {
+      load_half(a, s0, 1);
+#pragma unroll
+      for (int i = 0; i < 16; i++) xp[16 + i] = a[i];
+    }
+  };
   // without a window the second half is zero padding: never loaded, and
   // zero (not left undefined: a non-finite value times the zero weight would
   // give NaN)
 #pragma unroll
-  for (int i = 0; i < 16; i++) xp[16 + i] = carry ? xn[i] : make_float2(0.0f, 0.0f);
+  for (int i = 0; i < 16; i++) xp[16 + i] = make_float2(0.0f, 0.0f);
+  __syncthreads();  // wtab
+  if (carry && NT == 16) {
+    float2 a[16];
+    load_half(a, sa, 0);
+    load_half(xn, sa, 1);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      xp[i] = a[i];
+      xp[16 + i] = xn[i];
+    }
+  } else {
+    load_pass(sa);
+  }
   // complete here, so that no wait at the loop top (where the previous
-  // segment's stores are in flight) is needed for them
+  // pass's stores are in flight) is needed for them
 #pragma unroll
   for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i].x), "v"(xp[i].y));
-  for (int64_t seg = sa; seg < sb; seg++) {
+  for (int64_t seg = sa; seg < sb; seg += SG) {
     // phase 0: windowed (and sideband-flipped) points as FP16 K chunks
     cm_phase0(lds, xp, wtab, cth, win, flip);
-    // with a window this segment's second half is the next one's first
-    if (carry) {
+    // with carry this segment's second half is the next one's first
+    if (carry && NT == 16) {
 #pragma unroll
       for (int i = 0; i < 16; i++) xp[i] = xp[16 + i];
     }
     __syncthreads();
-    // the next segment's new points: the second half with a window, else the
-    // whole used first half; they complete before this segment's spectrum
+    // the next pass's points: they complete before this pass's spectrum
     // stores, so no wait for them also waits for stores
-    const bool more = seg + 1 < sb;
-    if (more) load_half(xn, seg + 1, carry ? 1 : 0);
+    const bool more = seg + SG < sb;
+    if (more) {
+      if (carry) load_half(xn, seg + SG, 1);
+      else load_pass(seg + SG);
+    }
     // wave w takes tiles 2 w and 2 w + 1 (K slices of their A operands: real
     // parts of m2 0..15, 16..31, then the imaginary parts) and leaves their V
     // in its own 8 KiB of LDS, which held just those two tiles: no other
@@ -1518,43 +1566,68 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
       }
     }
     __syncthreads();
-    // last stage and split: rows q0 and q1 hold bins k and N - k
+    // last stage and split: in each segment of the pass, rows q0 and q1
+    // hold bins k and N - k
     {
       const int tq = opaque(threadIdx.x), r0 = tq, r1 = 1024 - tq;
-      float2 z0[16], z1[16];
-      cm_read_row(vlds, r0, z0);
-      cm_read_row(vlds, r1 & 1023, z1);
-      cm_last_stage(z0, cth[0][tq]);
-      cm_last_stage(z1, cth[1][tq]);
       if (more) {
+        if (carry && NT == 16) {
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-          asm volatile("" ::"v"(xn[i].x), "v"(xn[i].y));  // the loads complete here
-          if (carry) xp[16 + i] = xn[i];
-          else xp[i] = xn[i];
+          for (int i = 0; i < 16; i++) {
+            asm volatile("" ::"v"(xn[i].x), "v"(xn[i].y));  // the loads complete here
+            xp[16 + i] = xn[i];
+          }
+        } else if (carry) {
+          // first halves from the neighbouring segments' second halves: the
+          // lane of segment SG - 1 hands on this pass's, the others the
+          // next pass's
+          const int lo = opaque(lane), sg = (lo & 15) / NT;
+          const int src = 4 * (sg == 0 ? lo + NT * (SG - 1) : lo - NT);
+          const bool last = sg == SG - 1;
+#pragma unroll
+          for (int i = 0; i < 16; i++) {
+            asm volatile("" ::"v"(xn[i].x), "v"(xn[i].y));  // the loads complete here
+            const float2 v = last ? xp[16 + i] : xn[i];
+            xp[i] = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src, __builtin_bit_cast(int, v.x))),
+                                __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src, __builtin_bit_cast(int, v.y))));
+            xp[16 + i] = xn[i];
+          }
+        }
+        if (!carry) {
+#pragma unroll
+          for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i].x), "v"(xp[i].y));
         }
       }
-      uint32_t *row = reinterpret_cast<uint32_t *>(s.spectra) + seg * job.spectra_stride;
       const float hs = 0.5f * job.spec_scale;
-      const float2 w0 = cth[2][tq];
+      const float2 wn0 = cth[0][tq], wn1 = cth[1][tq], w0 = cth[2][tq];
 #pragma unroll
-      for (int k1 = 0; k1 < 16; k1++) {
-        float2 xk, xn;
-        cm_split(z0[k1], z1[15 - k1], cmul(w0, cm_w32(k1)), hs, xk, xn);
-        cm_put<SP>(row, (uint32_t)(r0 + 1024 * k1), xk);
-        cm_put<SP>(row, (uint32_t)(r1 + 1024 * (15 - k1)), xn);
-      }
-      if (tq == 0) {
-        // row 512: bins 512 + 1024 k1 and 512 + 1024 (15 - k1) pair
-        cm_read_row(vlds, 512, z1);
-        cm_last_stage(z1, cth[3][0]);
-        const float2 w5 = cth[3][1];
+      for (int sig = 0; sig < SG; sig++) {
+        if (seg + sig >= sb) break;  // uniform
+        float2 z0[NT], z1[NT];
+        cm_read_rows<NT>(vlds, sig, r0, z0);
+        cm_read_rows<NT>(vlds, sig, r1 & 1023, z1);
+        cm_last_stage<NT>(z0, wn0);
+        cm_last_stage<NT>(z1, wn1);
+        uint32_t *row = reinterpret_cast<uint32_t *>(s.spectra) + (seg + sig) * job.spectra_stride;
 #pragma unroll
-        for (int k1 = 0; k1 < 8; k1++) {
+        for (int k1 = 0; k1 < NT; k1++) {
           float2 xk, xn;
-          cm_split(z1[k1], z1[15 - k1], cmul(w5, cm_w32(k1)), hs, xk, xn);
-          cm_put<SP>(row, (uint32_t)(512 + 1024 * k1), xk);
-          cm_put<SP>(row, (uint32_t)(512 + 1024 * (15 - k1)), xn);
+          cm_split(z0[k1], z1[NT - 1 - k1], cmul(w0, cm_w32((16 / NT) * k1)), hs, xk, xn);
+          cm_put<SP>(row, (uint32_t)(r0 + 1024 * k1), xk);
+          cm_put<SP>(row, (uint32_t)(r1 + 1024 * (NT - 1 - k1)), xn);
+        }
+        if (tq == 0) {
+          // row 512: bins 512 + 1024 k1 and 512 + 1024 (NT - 1 - k1) pair
+          cm_read_rows<NT>(vlds, sig, 512, z1);
+          cm_last_stage<NT>(z1, cth[3][0]);
+          const float2 w5 = cth[3][1];
+#pragma unroll
+          for (int k1 = 0; k1 < (NT + 1) / 2; k1++) {
+            float2 xk, xn;
+            cm_split(z1[k1], z1[NT - 1 - k1], cmul(w5, cm_w32((16 / NT) * k1)), hs, xk, xn);
+            cm_put<SP>(row, (uint32_t)(512 + 1024 * k1), xk);
+            if (NT - 1 - k1 != k1) cm_put<SP>(row, (uint32_t)(512 + 1024 * (NT - 1 - k1)), xn);
+          }
         }
       }
     }
@@ -3501,22 +3574,61 @@ static hipError_t launch_chan_n(const DevStream *d_streams, const DevJob &job, c
   return hipGetLastError();
 }
 
-// the MFMA channeliser (N_c = 16384, packed FP16 / BF16 spectra): runs of
-// consecutive segments, about chan_wgs (1024) workgroups in all, two per CU
+// compute units of the current device (cached per device)
+static int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int n = cache[dev].load(std::memory_order_relaxed);
+  if (n <= 0) {
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cache[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
+
+// the MFMA channeliser (N_c = 1024, 4096, 16384, packed FP16 / BF16
+// spectra): each workgroup a run of consecutive segments of one stream, the
+// same number of workgroups per stream, at most chan_wgs in all. One
+// workgroup fits a CU (its LDS), so by default chan_wgs is 2, 4 or 1 times
+// the CU count, whichever leaves the fewest CUs idle in the last round.
+template <int SP, int NT>
+static hipError_t launch_chan_mfma_nt(const DevStream *d_streams, const DevJob &job, const Tune &t, hipStream_t s) {
+  constexpr int SG = 16 / NT;  // segments per pass
+  if (job.seg_step != 1024 * NT) return hipErrorInvalidValue;
+  int64_t wgs = t.chan_wgs;
+  if (wgs <= 0) {
+    const int cus = device_cus();
+    double best = -1.0;
+    for (int k : {2, 4, 1}) {
+      const int64_t g = std::max<int64_t>(1, (int64_t)k * cus / job.n_streams) * job.n_streams;
+      const double eff = (double)g / (double)((g + cus - 1) / cus * cus);
+      if (eff > best + 1e-9) {
+        best = eff;
+        wgs = (int64_t)k * cus;
+      }
+    }
+  }
+  const int64_t per_stream = std::max<int64_t>(1, wgs / job.n_streams);
+  int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
+  segs = (segs + SG - 1) / SG * SG;  // whole passes
+  if (segs < SG) segs = SG;
+  const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
+  hipLaunchKernelGGL((k_chan_mfma<SP, NT>), dim3(gx, job.n_streams), dim3(CM_T), 0, s, d_streams, job, (int)segs);
+  return hipGetLastError();
+}
 template <int SP>
 static hipError_t launch_chan_mfma(const DevStream *d_streams, const DevJob &job, const Tune &t, hipStream_t s) {
-  if (job.seg_step != CM_N) return hipErrorInvalidValue;
-  const int64_t wgs = t.chan_wgs > 0 ? t.chan_wgs : 512;
-  const int64_t per_stream = (wgs + job.n_streams - 1) / job.n_streams;
-  int64_t segs = (job.n_segments + per_stream - 1) / per_stream;
-  if (segs < 1) segs = 1;
-  const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
-  hipLaunchKernelGGL(k_chan_mfma<SP>, dim3(gx, job.n_streams), dim3(CM_T), 0, s, d_streams, job, (int)segs);
-  return hipGetLastError();
+  switch (job.nc) {
+    case 1024: return launch_chan_mfma_nt<SP, 1>(d_streams, job, t, s);
+    case 4096: return launch_chan_mfma_nt<SP, 4>(d_streams, job, t, s);
+    case 16384: return launch_chan_mfma_nt<SP, 16>(d_streams, job, t, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 bool chan_mfma_supported(int nc, int spec_prec) {
-  return nc == CM_N && (spec_prec == SFXG_PREC_FP16 || spec_prec == SFXG_PREC_BF16);
+  return (nc == 1024 || nc == 4096 || nc == 16384) && (spec_prec == SFXG_PREC_FP16 || spec_prec == SFXG_PREC_BF16);
 }
 
 size_t channelise_scratch_elems(int nc) { return nc == LG_N ? (size_t)2 * 1024 * LG_N : 0; }
@@ -3546,7 +3658,7 @@ static hipError_t launch_channelise_sp(const DevStream *d_streams, const DevJob 
                              const Tune &t, hipStream_t s) {
   if (job.n_segments <= 0) return hipSuccess;
   if constexpr (SP == SFXG_PREC_FP16 || SP == SFXG_PREC_BF16) {
-    if (job.nc == CM_N && t.chan_fp32 != 1) return launch_chan_mfma<SP>(d_streams, job, t, s);
+    if (chan_mfma_supported(job.nc, SP) && t.chan_fp32 != 1) return launch_chan_mfma<SP>(d_streams, job, t, s);
   }
   if (job.nc == LG_N) return launch_chan_large<SP>(d_streams, job, window, tw_n, tw_2n, scratch, s);
   switch (job.nc) {

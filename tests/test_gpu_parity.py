@@ -547,11 +547,11 @@ def test_packed_spectra(ctx, nchan, prec, keep_ts):
     job, hdrs = configs.build_slices(seed=42, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
                                      seconds=0.125, t_int_usec=125000, rho=0.05, bits=2, sky_freq=1_400_000_000,
                                      offsets=[0.0, 2e-6, -1e-6], rates=[0.0, 1e-7, -2e-7])[0]
-    # packed FP16 / BF16 at 16384 channels take the FP32 transform here (the
-    # MFMA one has its own test, test_mfma_channeliser)
+    # packed FP16 / BF16 at 1024, 4096 and 16384 channels take the FP32
+    # transform here (the MFMA one has its own test, test_mfma_channeliser)
     capi.tune_clear()
-    if nchan == 16384 and prec in (1, 2):
-        capi.tune_import(capi.tune_key(nchan, 3, prec, False, False) + "|0|0|0|0|0|0|1\n")
+    if nchan in (1024, 4096, 16384) and prec in (1, 2):
+        capi.tune_import(capi.tune_key(nchan, 3, prec, False, nchan <= 4096 and not keep_ts) + "|0|0|0|0|0|0|1\n")
     try:
         capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=0, keep_time_series=keep_ts)
         sp32 = ctx.stage(capi.STAGE_SPECTRA, 1)
@@ -577,24 +577,29 @@ def test_packed_spectra(ctx, nchan, prec, keep_ts):
     assert l2 <= MIXED_BARS[prec][0] and worst <= MIXED_BARS[prec][1]
 
 
-@pytest.mark.parametrize("prec,window,sideband", [(1, 4, "U"), (2, 4, "U"), (1, 0, "U"), (1, 3, "L")])
-def test_mfma_channeliser(ctx, prec, window, sideband):
-    """Packed FP16 / BF16 spectra at 16384 channels come from the MFMA
-    channeliser: its transform takes FP16 operands with FP32 accumulation
-    (the windowed points, the first stage's results and the second stage's
-    results are each rounded to FP16 once). Against the FP32 channeliser's
+@pytest.mark.parametrize("nchan,prec,window,sideband", [(16384, 1, 4, "U"), (16384, 2, 4, "U"), (16384, 1, 0, "U"),
+                                                        (16384, 1, 3, "L"), (4096, 1, 4, "U"), (4096, 2, 2, "L"),
+                                                        (1024, 1, 4, "U"), (1024, 1, 1, "U"), (1024, 2, 0, "L")])
+def test_mfma_channeliser(ctx, nchan, prec, window, sideband):
+    """Packed FP16 / BF16 spectra at 1024, 4096 and 16384 channels come from
+    the MFMA channeliser (after the unfused delay stage at 1024 and 4096): its
+    transform of N = 1024 NT points takes NT tiles of 32 x 32 on MFMA with
+    FP16 operands and FP32 accumulation (the windowed points, the first
+    stage's results and the second stage's results are each rounded to FP16
+    once) and a last NT-point stage in FP32. Against the FP32 channeliser's
     spectra, per part: every value within 3e-3 of the spectrum rms plus the
     spectrum type's own rounding, and the transform's excess over that
-    rounding at most 4e-4 of the rms (a numpy model of the same
+    rounding at most 4e-4 of the rms (a numpy model of the 16384-point
     decomposition with the same roundings gives 2.8e-4 rms and 1.4e-3 at
     most over 2^16 values). The spectra differ from those of the FP32
     transform (chan_fp32 = 1 in the tuning table) in most values (FP16) or
     many (BF16, whose rounding hides most of the difference), so the MFMA
-    kernel is the one that ran, and the visibilities meet the mixed-precision bars against
-    the FP64 oracle. 0.25 s at 32 Ms/s is 488 segments per stream, runs of
-    two segments per workgroup; window NONE pads with zeros, lower sideband
-    flips."""
-    job, hdrs = configs.build_slices(seed=44, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=16384,
+    kernel is the one that ran, and the visibilities meet the mixed-precision
+    bars against the FP64 oracle. 0.25 s at 32 Ms/s is 488 segments per
+    stream at 16384 channels (runs of two segments per workgroup), 7812 at
+    1024 (passes of 16 segments, the last pass partial); windows NONE (zero
+    padding), RECT, COS, HAMMING and HANN, lower sideband flips."""
+    job, hdrs = configs.build_slices(seed=44, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
                                      seconds=0.25, t_int_usec=250000, rho=0.05, bits=2, sky_freq=1_400_000_000,
                                      offsets=[0.0, 2e-6, -1e-6], rates=[0.0, 1e-7, -2e-7], window=window,
                                      sideband=sideband)[0]
@@ -604,7 +609,7 @@ def test_mfma_channeliser(ctx, prec, window, sideband):
         sp32 = ctx.stage(capi.STAGE_SPECTRA, 1)
         out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=prec)
         sph = ctx.stage(capi.STAGE_SPECTRA, 1)
-        capi.tune_import(capi.tune_key(16384, 3, prec, False, False) + "|0|0|0|0|0|0|1\n")
+        capi.tune_import(capi.tune_key(nchan, 3, prec, False, nchan <= 4096) + "|0|0|0|0|0|0|1\n")
         capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=prec)
         spv = ctx.stage(capi.STAGE_SPECTRA, 1)
     finally:
@@ -616,7 +621,7 @@ def test_mfma_channeliser(ctx, prec, window, sideband):
         f = part(sp32).astype(np.float64)
         d = np.abs(part(sph).astype(np.float64) - f)
         excess = np.maximum(d - 1.01 * u * np.abs(f), 0.0) / rms
-        print(f"MFMA channeliser precision {prec} window {window} {sideband}: transform error rms "
+        print(f"MFMA channeliser {nchan} precision {prec} window {window} {sideband}: transform error rms "
               f"{np.sqrt(np.mean(excess ** 2)):.2e}, max {excess.max():.2e} of the spectrum rms")
         assert excess.max() <= 3e-3
         assert np.sqrt(np.mean(excess ** 2)) <= 4e-4
@@ -627,7 +632,7 @@ def test_mfma_channeliser(ctx, prec, window, sideband):
     heads_r, vis_r = vis_from_bytes(job, r64["output"])
     assert heads_g == heads_r
     l2, worst = mixed_errors(vis_g[0], vis_r[0])
-    print(f"MFMA channeliser precision {prec}: vs FP64 oracle, relative L2 {l2:.3e}, worst channel {worst:.3e}")
+    print(f"MFMA channeliser {nchan} precision {prec}: vs FP64 oracle, relative L2 {l2:.3e}, worst channel {worst:.3e}")
     assert l2 <= MIXED_BARS[prec][0] and worst <= MIXED_BARS[prec][1]
 
 
