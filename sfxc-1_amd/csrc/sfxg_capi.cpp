@@ -1069,7 +1069,13 @@ static int prepare_slice(sfxg_ctx *ctx) {
   }
   const int packed = ctx->opt.precision == SFXG_PREC_FP16 || ctx->opt.precision == SFXG_PREC_BF16
                          ? ctx->opt.precision : 0;
-  const bool fused = key_fused && !(chan_mfma_supported(nc, packed) && ctx->tune.chan_fp32 == 0);
+  const bool mfma_chan = chan_mfma_supported(nc, packed) && ctx->tune.chan_fp32 == 0;
+  const bool fused = key_fused && !mfma_chan;
+  // the MFMA channeliser rounds its input points to FP16 anyway: the delay
+  // stage hands it an FP16 series (half the bytes each way), unless the
+  // series is kept for reading back
+  const bool ts_half = mfma_chan && !ctx->opt.keep_time_series;
+  const size_t ts_bytes = ts_half ? sizeof(uint16_t) : sizeof(float);
   // FP16 and BF16 X-steps take their operands straight from spectra the
   // channeliser stores in that type (4 bytes per bin instead of 8); split
   // FP16 and FP32 need the FP32 spectra
@@ -1117,7 +1123,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   if (ctx->opt.precision == SFXG_PREC_SPLIT_FP16 && !ctx->pulsar &&
       xstep_split_planes_fit(M, lp.chunk_segs * ctx->S))
     spec_prec = SFXG_PREC_SPLIT_FP16;
-  if (!fused) HIPCHK(ctx, ctx->time_series.ensure((size_t)M * lp.ts_stride * sizeof(float)));
+  if (!fused) HIPCHK(ctx, ctx->time_series.ensure((size_t)M * lp.ts_stride * ts_bytes));
   if (channelise_scratch_elems(nc) > 0)
     HIPCHK(ctx, ctx->chan_scratch.ensure(channelise_scratch_elems(nc) * sizeof(float2)));
   HIPCHK(ctx, ctx->spectra.ensure((size_t)M * lp.chunk_segs * ctx->S * spec_bytes));
@@ -1231,7 +1237,8 @@ static int prepare_slice(sfxg_ctx *ctx) {
     d.fl_sec = d.fl / kClockRate;
     d.oversamp = (int)round(fs / (2.0 * st.bandwidth));
     d.flip = ctx->p.sideband != st.sideband;
-    d.time_series = fused ? nullptr : ctx->time_series.as<float>() + (int64_t)i * lp.ts_stride;
+    d.time_series = fused ? nullptr
+                          : reinterpret_cast<float *>(ctx->time_series.as<char>() + (size_t)i * lp.ts_stride * ts_bytes);
     d.spectra = reinterpret_cast<float2 *>(ctx->spectra.as<char>() + (size_t)i * lp.chunk_segs * ctx->S * spec_bytes);
     d.samples = nullptr;
     d.hist = ctx->hist.as<int32_t>() + 512 * i;
@@ -1310,6 +1317,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   lp.job.spectra_stride = ctx->S;
   lp.job.seg_step = nc;  // fft_rot_size / 2
   lp.job.spec_prec = spec_prec;
+  lp.job.ts_half = ts_half ? 1 : 0;
   lp.job.spec_scale = 1.0f;
   lp.mixed_scale = 1.0f;
   if (mixed) {
@@ -2061,12 +2069,12 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
   std::string rep;
   char line[160];
   Tune best = ctx->tune;
-  // F-engine: threads per workgroup (N_c <= 4096), or the large-N
-  // channeliser's workgroups; packed FP16 / BF16 spectra at N_c = 16384 also
-  // between the MFMA transform (chan_fp32 = 0) and the FP32 one
-  // (the fused F-engine or not is the prepared plan's: chan_fp32 = 1 in the
-  // table gives packed spectra at N_c <= 4096 the fused one at the next
-  // preparation)
+  // F-engine: threads per workgroup (N_c <= 4096), or the channeliser's
+  // workgroups. Packed FP16 / BF16 spectra at N_c 1024, 4096 and 16384 take
+  // the MFMA channeliser, fixed by the prepared plan with its FP16 series
+  // (chan_fp32 = 1 in the table selects the FP32 transform, and at N_c <=
+  // 4096 the fused F-engine, at the next preparation); with an FP32 series
+  // (keep_time_series) the FP32 transform at 16384 is among the candidates
   const bool small = nc <= 4096;
   const bool mf = chan_mfma_supported(nc, ctx->opt.precision);
   const bool fe = lp.fused || (small && !mf);  // candidates are F-engine thread counts
@@ -2076,7 +2084,9 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
   } else {
     if (mf)
       for (int v : chan_mfma_wgs_candidates()) fc.push_back({0, v});
-    if (!small)
+    // (the FP32 transform needs an FP32 series: not among the candidates
+    // when the MFMA channeliser's FP16 one was prepared)
+    if (!small && !lp.job.ts_half)
       for (int v : chan_wgs_candidates(nc)) fc.push_back({mf ? 1 : 0, v});
   }
   if (fc.size() > 1) {

@@ -201,6 +201,7 @@ struct DevJob {
   int32_t window;
   int32_t spec_prec;        // spectra stored as float2 (0) or packed FP16 / BF16 pairs (SFXG_PREC_*)
   float spec_scale;         // power-of-two scale of packed spectra
+  int32_t ts_half;          // real series stored as FP16 (k_delay out, MFMA channeliser in), else FP32
   int64_t n_blocks;         // N_d blocks per stream in the slice
   int64_t n_samples;        // n_blocks * nd
   int64_t n_segments;       // segments of this chunk

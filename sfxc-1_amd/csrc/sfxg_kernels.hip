@@ -866,12 +866,14 @@ __device__ __forceinline__ void dft16s(float2 (&u)[16]) {
 }
 
 // v[n1] = x[16 n1 + l] on entry, X[l + 16 k2] in v[k2] on exit; tw holds
-// exp(-2 pi i j / 256) for j < 256; buf is the group's LDS region.
+// exp(-2 pi i l k1 / 256) at 16 k1 + l (lane-major: the 16 lanes of a group
+// read 16 consecutive entries, free of bank conflicts); buf is the group's
+// LDS region.
 template <int SIGN>
 __device__ __forceinline__ void fft256_g16(float2 (&v)[16], float2 *buf, const float2 *tw, int l) {
   dft16s<SIGN>(v);
 #pragma unroll
-  for (int k1 = 1; k1 < 16; k1++) v[k1] = cmul(v[k1], twiddle<SIGN>(tw, (l * k1) & 255));
+  for (int k1 = 1; k1 < 16; k1++) v[k1] = cmul(v[k1], twiddle<SIGN>(tw, 16 * k1 + l));
   wave_lds_sync();
 #pragma unroll
   for (int k1 = 0; k1 < 16; k1++) buf[k1 * G16_ROW + l] = v[k1];
@@ -933,24 +935,26 @@ __device__ __forceinline__ void load_block16(const DevStream &s, int64_t blk, in
 // The per-block constants finish_block16 takes, loaded by the caller before
 // any store of the iteration (the vector memory counter is in order, so
 // waiting for loads issued after stores would also wait for the stores).
+// The FP64 phases of lane l are rounded to FP32 here, so that only FP32
+// values stay live across the transforms.
 struct BlockC16 {
-  double c_term, l_term, phi, dphi;
+  float ramp_arg, fringe_arg;  // c_term + l l_term, phi + l dphi
   float amp, ramp16c, ramp16s, fringe16c, fringe16s;
 };
-__device__ __forceinline__ BlockC16 block_c16(const DevStream &s, int64_t blk) {
+__device__ __forceinline__ BlockC16 block_c16(const DevStream &s, int64_t blk, int l) {
   const auto *bcp = cmem(s.block_consts) + blk;
-  return BlockC16{bcp->c_term, bcp->l_term, bcp->phi, bcp->dphi, bcp->amp,
+  return BlockC16{(float)(bcp->c_term + l * bcp->l_term), (float)(bcp->phi + l * bcp->dphi), bcp->amp,
                   bcp->ramp_step16[0], bcp->ramp_step16[1], bcp->fringe_step16[0], bcp->fringe_step16[1]};
 }
 
+template <bool H>
 __device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[16], float2 *buf,
-                                               const float2 *tw, int l, float *out, bool write) {
+                                               const float2 *tw, int l, void *out, bool write) {
   constexpr int nd = 256;
-  const double c_term = bc.c_term, l_term = bc.l_term, phi = bc.phi, dphi = bc.dphi;
   const float amp = bc.amp;
   {
     float sn, cs;
-    sincosf((float)(c_term + l * l_term), &sn, &cs);
+    sincosf(bc.ramp_arg, &sn, &cs);
     float2 e = make_float2(cs, -sn);
     const float2 step = make_float2(bc.ramp16c, -bc.ramp16s);
 #pragma unroll
@@ -967,14 +971,35 @@ __device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[1
   }
   fft256_g16<+1>(v, buf, tw, l);
   float sn, cs;
-  sincosf((float)(phi + l * dphi), &sn, &cs);
+  sincosf(bc.fringe_arg, &sn, &cs);
   float2 r = make_float2(cs, sn);
   const float2 rstep = make_float2(bc.fringe16c, bc.fringe16s);
-  auto *o = gmem(out);
+  if constexpr (!H) {
+    auto *o = gmem(static_cast<float *>(out));
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    if (write) o[l + 16 * q] = amp * (v[q].x * r.x + v[q].y * r.y);
-    r = cmul(r, rstep);
+    for (int q = 0; q < 16; q++) {
+      if (write) o[l + 16 * q] = amp * (v[q].x * r.x + v[q].y * r.y);
+      r = cmul(r, rstep);
+    }
+  } else {
+    // FP16 samples (rounded to nearest even), whole 32-bit words per lane:
+    // for the rows q0 = 2 j and q1 = 2 j + 1, an even lane l stores samples
+    // 16 q0 + l and 16 q0 + l + 1, an odd lane 16 q1 + l - 1 and 16 q1 + l;
+    // neighbouring lanes swap the other half (DPP quad_perm [1, 0, 3, 2])
+    auto *o = gmem(static_cast<uint32_t *>(out));
+    const bool odd = l & 1;
+    const int wb = (l >> 1) + (odd ? 8 : 0);  // word 16 j + wb
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const float y0 = amp * (v[2 * j].x * r.x + v[2 * j].y * r.y);
+      r = cmul(r, rstep);
+      const float y1 = amp * (v[2 * j + 1].x * r.x + v[2 * j + 1].y * r.y);
+      r = cmul(r, rstep);
+      const uint32_t h0 = __builtin_bit_cast(uint16_t, (_Float16)y0), h1 = __builtin_bit_cast(uint16_t, (_Float16)y1);
+      const uint32_t recv = (uint32_t)__builtin_amdgcn_mov_dpp((int)(odd ? h0 : h1), 0xB1, 0xF, 0xF, false);
+      const uint32_t w = odd ? (recv | (h1 << 16)) : (h0 | (recv << 16));
+      if (write) o[16 * j + wb] = w;
+    }
   }
 }
 
@@ -982,13 +1007,15 @@ __device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[1
 
 // k_delay: each wavefront takes four pairs of blocks at a time (grid-stride
 // over the chunk's blocks), the real series to HBM (unfused path, used for
-// N_c above the fused F-engine's LDS budget and for verification). The two
-// real blocks of a pair share one complex forward transform, z = a + i b.
-__global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ streams, DevJob job,
+// N_c above the fused F-engine's LDS budget, before the MFMA channeliser and
+// for verification) as FP32 or, H, FP16. The two real blocks of a pair
+// share one complex forward transform, z = a + i b.
+template <bool H>
+__global__ __launch_bounds__(256, 3) void k_delay(const DevStream *__restrict__ streams, DevJob job,
                                                const float2 *__restrict__ tw_g) {
   __shared__ float2 lds[4][4][G16_LDS];
-  __shared__ float2 tw[256];
-  for (int i = threadIdx.x; i < 256; i += 256) tw[i] = tw_g[i];
+  __shared__ float2 tw[256];  // exp(-2 pi i l k1 / 256) at 16 k1 + l
+  for (int i = threadIdx.x; i < 256; i += 256) tw[i] = tw_g[((i & 15) * (i >> 4)) & 255];
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int g = lane >> 4, l = lane & 15;
@@ -1008,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
     load_block16(s, ba + 1, ba + 1 < end ? job.n_blocks : 0, l, xb, bsb);
     bsa = src_of(ba + step);
     bsb = src_of(ba + step + 1);
-    const BlockC16 ca = block_c16(s, ba < end ? ba : 0), cb = block_c16(s, ba + 1 < end ? ba + 1 : 0);
+    const BlockC16 ca = block_c16(s, ba < end ? ba : 0, l), cb = block_c16(s, ba + 1 < end ? ba + 1 : 0, l);
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) v[q] = make_float2(xa[q], xb[q]);
@@ -1028,9 +1055,10 @@ __global__ __launch_bounds__(256) void k_delay(const DevStream *__restrict__ str
       va[q] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
       vb[q] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
     }
-    float *base = s.time_series + (ba - job.blk_base) * 256;
-    finish_block16(ca, va, buf, tw, l, base, ba < end);
-    finish_block16(cb, vb, buf, tw, l, base + 256, ba + 1 < end);
+    constexpr int64_t bb = H ? 512 : 1024;  // bytes per block
+    char *base = reinterpret_cast<char *>(s.time_series) + (ba - job.blk_base) * bb;
+    finish_block16<H>(ca, va, buf, tw, l, base, ba < end);
+    finish_block16<H>(cb, vb, buf, tw, l, base + bb, ba + 1 < end);
   }
 }
 
@@ -1351,7 +1379,7 @@ __device__ __forceinline__ void cm_phase0(uint4 *lds, const float2 (&xp)[32], co
 // segment's spectrum stores (the vector memory counter is in order, so
 // waiting for them does not wait for the stores). Without a window a segment
 // uses only its first half and nothing carries. One workgroup per CU.
-template <int SP, int NT>
+template <int SP, int NT, bool H>
 __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restrict__ streams, DevJob job,
                                                       int segs_per_wg) {
   static_assert(SP == SFXG_PREC_FP16 || SP == SFXG_PREC_BF16, "packed FP16 / BF16 spectra");
@@ -1431,7 +1459,10 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
     cth[3][t] = t == 0 ? cm_w(512, N) : t == 1 ? cm_w(512, 2 * N) : make_float2(0.0f, 0.0f);
   }
   const bool flip = s.flip;
+  // the real series as pairs of reals: float2 (FP32) or one 32-bit word of
+  // two FP16 (H)
   const float2 *ts = reinterpret_cast<const float2 *>(s.time_series);
+  const uint32_t *tsh = reinterpret_cast<const uint32_t *>(s.time_series);
   uint2 *vlds = reinterpret_cast<uint2 *>(lds);
   // point i = 16 g + 8 hh + e of the thread has m2 = 16 g + cm_kslot(hh, e);
   // xp[0..15]: the first half of its segment, xp[16..31]: the second.
@@ -1450,11 +1481,18 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
     const int to = opaque(threadIdx.x);
     const int tau = to & 15, m3 = to >> 4, sig = tau / NT, n1 = tau % NT;
     const bool ok = s0 + sig < sb;
-    const float2 *zs = ts + (ok ? s0 + sig : s0) * (N / 2);
+    const int64_t p0 = (ok ? s0 + sig : s0) * (N / 2);  // the segment's first point
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-      const float2 v = gload2(
-          zs, (uint32_t)(NT * (32 * (16 * g + cm_kslot(i >> 3, i & 7)) + m3) + n1) * (uint32_t)sizeof(float2));
+      const uint32_t p = (uint32_t)(NT * (32 * (16 * g + cm_kslot(i >> 3, i & 7)) + m3) + n1);
+      float2 v;
+      if constexpr (H) {
+        const uint32_t w = *((const __attribute__((address_space(1))) uint32_t *)(tsh + p0) + p);
+        v = make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)),
+                        (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)));
+      } else {
+        v = gload2(ts + p0, p * (uint32_t)sizeof(float2));
+      }
       dst[i] = ok ? v : make_float2(0.0f, 0.0f);
     }
   };
@@ -3533,7 +3571,8 @@ hipError_t launch_delay(const DevStream *d_streams, const DevJob &job, const flo
   const int64_t wgs = (end - job.blk_base + 31) / 32;
   const int64_t per_stream = std::max<int64_t>(1, (8192 + job.n_streams - 1) / job.n_streams);
   dim3 grid((unsigned)std::min(wgs, per_stream), job.n_streams);
-  hipLaunchKernelGGL(k_delay, grid, dim3(256), 0, s, d_streams, job, tw_nd);
+  if (job.ts_half) hipLaunchKernelGGL(k_delay<true>, grid, dim3(256), 0, s, d_streams, job, tw_nd);
+  else hipLaunchKernelGGL(k_delay<false>, grid, dim3(256), 0, s, d_streams, job, tw_nd);
   return hipGetLastError();
 }
 
@@ -3614,7 +3653,10 @@ static hipError_t launch_chan_mfma_nt(const DevStream *d_streams, const DevJob &
   segs = (segs + SG - 1) / SG * SG;  // whole passes
   if (segs < SG) segs = SG;
   const unsigned gx = (unsigned)((job.n_segments + segs - 1) / segs);
-  hipLaunchKernelGGL((k_chan_mfma<SP, NT>), dim3(gx, job.n_streams), dim3(CM_T), 0, s, d_streams, job, (int)segs);
+  if (job.ts_half)
+    hipLaunchKernelGGL((k_chan_mfma<SP, NT, true>), dim3(gx, job.n_streams), dim3(CM_T), 0, s, d_streams, job, (int)segs);
+  else
+    hipLaunchKernelGGL((k_chan_mfma<SP, NT, false>), dim3(gx, job.n_streams), dim3(CM_T), 0, s, d_streams, job, (int)segs);
   return hipGetLastError();
 }
 template <int SP>
@@ -3660,6 +3702,7 @@ static hipError_t launch_channelise_sp(const DevStream *d_streams, const DevJob 
   if constexpr (SP == SFXG_PREC_FP16 || SP == SFXG_PREC_BF16) {
     if (chan_mfma_supported(job.nc, SP) && t.chan_fp32 != 1) return launch_chan_mfma<SP>(d_streams, job, t, s);
   }
+  if (job.ts_half) return hipErrorInvalidValue;  // an FP16 series only feeds the MFMA channeliser
   if (job.nc == LG_N) return launch_chan_large<SP>(d_streams, job, window, tw_n, tw_2n, scratch, s);
   switch (job.nc) {
     case 256: return launch_chan_n<256, SP>(d_streams, job, window, tw_n, tw_2n, tw_split, t, s);
