@@ -2480,6 +2480,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma(
   const int L = xcd_grouped(blockIdx.x, gridDim.x, G);
   const int ch0 = (L % n_blk) * CH;
   const int z = L / n_blk;
+  if (ch0 >= n_ch) return;  // a block that pads n_blk to whole XCD groups
   const int64_t s_begin = fold_start[z], s_end = fold_start[z + 1];
   const int chan_stride = R * (KS + 8) + 8;  // halves, skewed by 16 bytes per channel
   const int plane_elems = CH * chan_stride;
@@ -2704,6 +2705,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_dma(
   const int L = xcd_grouped(blockIdx.x, gridDim.x, G);
   const int ch0 = (L % n_blk) * CH;
   const int z = L / n_blk;
+  if (ch0 >= n_ch) return;  // a block that pads n_blk to whole XCD groups
   const int64_t s_begin = fold_start[z], s_end = fold_start[z + 1];
   const int chan_stride = R * (KS + 8) + 8;  // halves
   char *ring = dlds;
@@ -3994,8 +3996,12 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
       const int npw = npw_of(CW);
       const size_t lds = lds_of(CW);
       const int CH = CW / W;
-      const int n_blk = (n_ch + CH - 1) / CH;
+      // G channel blocks share a 128-byte line of a spectra row; n_blk is
+      // padded to whole groups, so that no group straddles two
+      // sub-integrations (and two XCDs fetch no line twice); padding blocks
+      // return at once
       const int G = std::max(1, 128 / (CW * 4));
+      const int n_blk = ((n_ch + CH - 1) / CH + G - 1) / G * G;
       dim3 grid((unsigned)(n_blk * n_k));
       const float inv2 = 1.0f / (scale * scale);
       const uint32_t *sp = static_cast<const uint32_t *>(spectra);
@@ -4068,9 +4074,12 @@ hipError_t launch_xstep_mfma(int precision, const void *spectra, bool half_spect
   // FP32 spectra: the last bin of N_c + 1 on its own (in FP32), so that the
   // grid covers whole blocks of CH channels
   const int n_run = !half_spectra && n_ch > 1 && (n_ch - 1) % CH == 0 ? n_ch - 1 : n_ch;
-  const int n_blk = (n_run + CH - 1) / CH;
-  // channel blocks that share a 128-byte line of a spectra row
+  // channel blocks that share a 128-byte line of a spectra row; with packed
+  // spectra (n_run = n_ch) n_blk is padded to whole groups, so that no group
+  // straddles two sub-integrations, and padding blocks return at once
   const int G = std::max(1, 128 / (CH * (half_spectra ? 4 : 8)));
+  int n_blk = (n_run + CH - 1) / CH;
+  if (n_run == n_ch) n_blk = (n_blk + G - 1) / G * G;
   dim3 grid((unsigned)(n_blk * n_k));
   const float inv2 = 1.0f / (scale * scale);
   hipError_t e = hipErrorInvalidValue;
