@@ -1327,47 +1327,80 @@ __device__ __forceinline__ void cm_put(uint32_t *row, uint32_t k, float2 v) {
 
 }  // namespace
 
-// The window at real sample i = 1024 m2 + c (c < 1024) of the 2 N_c: NONE 1
-// on the first half, RECT 1 on the middle half, and the cosine family of
-// theta_i = 2 pi i / (2 N_c - 1) (delay_correction.cc:299-341) through
-// s = sin(theta_i / 2): HANN 0.5 (1 - cos theta) = s^2, HAMMING 0.54 - 0.46
-// cos theta = 0.08 + 0.92 s^2, COS s; s is sin(A + B) of the half angles of
-// theta_A(m2) (a 32-entry table) and theta_B(c) (per thread), each evaluated
-// in FP64 and rounded once. The window differs from the FP32 table by a few
-// 2^-24, far below the FP16 rounding of the windowed points.
-struct CmWindow {
-  float k0, k1, k2;    // w = k0 + k1 s + k2 s^2
-  bool zero_hi, rect;  // NONE: second half zero; RECT: m2 outside [8, 24) zero
-};
+// The window as a table of FP32 pairs: entry e < N / 2 holds the weights
+// of reals 2 e and 2 e + 1 of the 2 N (NONE 1 on the first half, RECT 1 on
+// the middle half, and the cosine family of theta_i = 2 pi i / (2 N - 1),
+// delay_correction.cc:299-341, through s = sin(theta_i / 2): HANN
+// 0.5 (1 - cos theta) = s^2, HAMMING 0.54 - 0.46 cos theta = 0.08 + 0.92 s^2,
+// COS s), each evaluated in FP64 and rounded once. Every window but NONE is
+// symmetric, w(2 N - 1 - i) = w(i), so point n >= N / 2 (reals 2 n, 2 n + 1)
+// takes entry N - 1 - n with its halves swapped; NONE's second half is zero.
+template <int N>
+__device__ __forceinline__ void cm_window_table(float2 *wt, int window) {
+  for (int e = threadIdx.x; e < N / 2; e += CM_T) {
+    float w[2];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int i = 2 * e + b;
+      double v = 1.0;
+      if (window == SFXG_WINDOW_RECT) {
+        v = i >= N / 2 ? 1.0 : 0.0;
+      } else if (window != SFXG_WINDOW_NONE) {
+        const double sn = sinpi((double)i / (double)(2 * N - 1));
+        v = window == SFXG_WINDOW_HANN ? sn * sn : window == SFXG_WINDOW_HAMMING ? 0.08 + 0.92 * sn * sn : sn;
+      }
+      w[b] = (float)v;
+    }
+    wt[e] = make_float2(w[0], w[1]);
+  }
+}
 
-// Phase 0 of k_chan_mfma: thread t = 16 m3 + n1 turns its 32 points xp
-// (point i = 16 g + 8 hh + e at m2 = 16 g + cm_kslot(hh, e)) into the FP16 K
-// chunks of row m3 of tile n1, windowed and sideband-flipped.
-__device__ __forceinline__ void cm_phase0(uint4 *lds, const float2 (&xp)[32], const float2 *wtab,
-                                          const float2 (*cth)[CM_T], const CmWindow &win, bool flip) {
+// Phase 0 of k_chan_mfma: thread t = 16 m3 + tau turns its 32 points xp (FP16
+// pairs; point i = 16 g + 8 hh + e at m2 = 16 g + cm_kslot(hh, e), point
+// n = NT (32 m2 + m3) + tau % NT of its segment) into the FP16 K chunks of
+// row m3 of tile tau: the FP32 product with the window pair (exact for the
+// FP16 point) rounded once to FP16, the sideband flip as the sign of the
+// imaginary half, then the real halves and the imaginary halves of 8 points
+// regrouped into the two chunks.
+template <int NT>
+__device__ __forceinline__ void cm_phase0(uint4 *lds, const uint32_t (&xp)[32], const float2 *wt,
+                                          uint32_t flipmask, bool zero_hi) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+  constexpr int N = 1024 * NT;
   const int to = opaque(threadIdx.x);
-  const int n1 = to & 15, m3 = to >> 4;
-  const float2 b0 = cth[4][to], b1 = cth[5][to];
-  const float2 *wt = wtab + opaque_s(0);  // read in the loop, not hoisted into registers
+  const int tau = to & 15, m3 = to >> 4, nb = NT * m3 + tau % NT;
+  const float2 *wd = wt + nb, *wm = wt + (N - 1 - nb);  // direct and mirrored entries at m2 = 0
 #pragma unroll
   for (int g = 0; g < 2; g++) {
 #pragma unroll
     for (int hh = 0; hh < 2; hh++) {
-      cm_h8 re, im;
+      uint32_t pr[8];
 #pragma unroll
       for (int e = 0; e < 8; e++) {
-        const int i = 16 * g + 8 * hh + e, m2 = 16 * g + cm_kslot(hh, e);
-        // s = sin(theta / 2) from the half angles A / 2 and B / 2
-        const float2 a = wt[m2];
-        const float s0 = a.y * b0.x + a.x * b0.y, s1 = a.y * b1.x + a.x * b1.y;
-        float w0 = win.k0 + (win.k1 + win.k2 * s0) * s0, w1 = win.k0 + (win.k1 + win.k2 * s1) * s1;
-        if ((win.zero_hi && g == 1) || (win.rect && (m2 < 8 || m2 >= 24))) w0 = w1 = 0.0f;
-        const float2 z = wmul(xp[i], make_float2(w0, w1));
-        re[e] = (_Float16)z.x;
-        im[e] = (_Float16)(flip ? -z.y : z.y);
+        const int m2 = 16 * g + cm_kslot(hh, e);
+        float2 w;
+        if (g == 0) {
+          w = wd[32 * NT * m2];
+        } else {
+          const float2 u = wm[-32 * NT * m2];
+          w = make_float2(u.y, u.x);
+        }
+        const h2v x = __builtin_bit_cast(h2v, xp[16 * g + 8 * hh + e]);
+        const h2v z = __builtin_convertvector((f2v){(float)x.x * w.x, (float)x.y * w.y}, h2v);
+        pr[e] = g == 1 && zero_hi ? 0u : __builtin_bit_cast(uint32_t, z) ^ flipmask;
       }
-      *reinterpret_cast<cm_h8 *>(&lds[cm_chunk(n1, m3, 2 * g + hh)]) = re;
-      *reinterpret_cast<cm_h8 *>(&lds[cm_chunk(n1, m3, 4 + 2 * g + hh)]) = im;
+      uint4 re, im;
+      re.x = __builtin_amdgcn_perm(pr[1], pr[0], 0x05040100u);
+      re.y = __builtin_amdgcn_perm(pr[3], pr[2], 0x05040100u);
+      re.z = __builtin_amdgcn_perm(pr[5], pr[4], 0x05040100u);
+      re.w = __builtin_amdgcn_perm(pr[7], pr[6], 0x05040100u);
+      im.x = __builtin_amdgcn_perm(pr[1], pr[0], 0x07060302u);
+      im.y = __builtin_amdgcn_perm(pr[3], pr[2], 0x07060302u);
+      im.z = __builtin_amdgcn_perm(pr[5], pr[4], 0x07060302u);
+      im.w = __builtin_amdgcn_perm(pr[7], pr[6], 0x07060302u);
+      lds[cm_chunk(tau, m3, 2 * g + hh)] = re;
+      lds[cm_chunk(tau, m3, 4 + 2 * g + hh)] = im;
     }
   }
 }
@@ -1387,11 +1420,10 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
   constexpr int N = 1024 * NT;  // complex points of a segment (N_c)
   constexpr int SG = 16 / NT;   // segments per pass (16 tiles)
   __shared__ uint4 lds[4096];  // 64 KiB: the 16 tiles' operands, then V
-  __shared__ float2 wtab[32];  // window: (cos, sin) of theta_A(m2) / 2
+  __shared__ float2 wt[N / 2];  // the window as FP32 pairs (cm_window_table)
   // per-thread constants (kept in LDS, not in registers held across the
-  // segment loop): W_N^q0, W_N^q1, W_2N^q0, row 512's constants and the
-  // window angle theta_B / 2 of the thread's two reals
-  __shared__ float2 cth[6][CM_T];
+  // segment loop): W_N^q0, W_N^q1, W_2N^q0 and row 512's constants
+  __shared__ float2 cth[4][CM_T];
   __shared__ float4 ctw[8][64];  // per lane: W1024^(m3 col) of registers 2 p, 2 p + 1
   const DevStream &s = streams[blockIdx.y];
   const int64_t sa = (int64_t)blockIdx.x * segs_per_wg;
@@ -1400,29 +1432,9 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
   if (sa >= sb) return;  // uniform over the workgroup
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, h = lane >> 5, col = lane & 31;
   // thread t = 16 m3 + tau takes tile tau = NT sigma + n1 (segment sigma of
-  // the pass, residue n1): points n = NT (32 m2 + m3) + n1, reals
-  // i = 64 NT m2 + 2 (NT m3 + n1) + b. theta / 2 = pi i / (2 N - 1): table of
-  // the half of theta_A = 2 pi 64 NT m2 / (2 N - 1), and the half of this
-  // thread's theta_B
-  if (t < 32) {
-    double sn, cs;
-    sincospi(64.0 * NT * (double)t / (double)(2 * N - 1), &sn, &cs);
-    wtab[t] = make_float2((float)cs, (float)sn);
-  }
-  {
-    const int tau = t & 15, m3 = t >> 4, n1 = tau % NT;
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      double sn, cs;
-      sincospi((double)(2 * (NT * m3 + n1) + b) / (double)(2 * N - 1), &sn, &cs);
-      cth[4 + b][t] = make_float2((float)cs, (float)sn);
-    }
-  }
-  CmWindow win{0.0f, 0.0f, 0.0f, job.window == SFXG_WINDOW_NONE, job.window == SFXG_WINDOW_RECT};
-  if (job.window == SFXG_WINDOW_NONE || job.window == SFXG_WINDOW_RECT) win.k0 = 1.0f;
-  else if (job.window == SFXG_WINDOW_HANN) win.k2 = 1.0f;
-  else if (job.window == SFXG_WINDOW_HAMMING) win.k0 = 0.08f, win.k2 = 0.92f;
-  else win.k1 = 1.0f;  // COS
+  // the pass, residue n1): points n = NT (32 m2 + m3) + n1
+  cm_window_table<N>(wt, job.window);
+  const bool zero_hi = job.window == SFXG_WINDOW_NONE;
   // DFT-32 fragments: lane holds F[16 g + cm_kslot(h, e)][col] (e = 0..7),
   // F[a][b] = W32^(a b): real parts, imaginary parts and their negatives
   cm_h8 fr[2], fi[2], nfi[2];
@@ -1458,15 +1470,16 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
     // row 512's constants: W_N^512 (last stage) and W_2N^512 (split)
     cth[3][t] = t == 0 ? cm_w(512, N) : t == 1 ? cm_w(512, 2 * N) : make_float2(0.0f, 0.0f);
   }
-  const bool flip = s.flip;
+  const uint32_t flipmask = s.flip ? 0x80000000u : 0u;  // sign of the imaginary half
   // the real series as pairs of reals: float2 (FP32) or one 32-bit word of
   // two FP16 (H)
   const float2 *ts = reinterpret_cast<const float2 *>(s.time_series);
   const uint32_t *tsh = reinterpret_cast<const uint32_t *>(s.time_series);
   uint2 *vlds = reinterpret_cast<uint2 *>(lds);
   // point i = 16 g + 8 hh + e of the thread has m2 = 16 g + cm_kslot(hh, e);
-  // xp[0..15]: the first half of its segment, xp[16..31]: the second.
-  float2 xp[32], xn[16];
+  // xp[0..15]: the first half of its segment, xp[16..31]: the second, as
+  // FP16 pairs (an FP32 series is rounded to FP16 as it is loaded)
+  uint32_t xp[32], xn[16];
   // With a window the second half of a segment is the first half of the
   // next one, so a pass reads only the second halves of its segments (16
   // points per thread): for NT = 16 (one segment per pass) the thread
@@ -1476,8 +1489,8 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
   // previous pass's last segment (NT (SG - 1) lanes higher), by ds_bpermute.
   // Without a window every pass reads the first halves only (the second is
   // zero padding).
-  const bool carry = !win.zero_hi;
-  auto load_half = [&](float2 (&dst)[16], int64_t s0, int g) {
+  const bool carry = !zero_hi;
+  auto load_half = [&](uint32_t (&dst)[16], int64_t s0, int g) {
     const int to = opaque(threadIdx.x);
     const int tau = to & 15, m3 = to >> 4, sig = tau / NT, n1 = tau % NT;
     const bool ok = s0 + sig < sb;
@@ -1485,23 +1498,24 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const uint32_t p = (uint32_t)(NT * (32 * (16 * g + cm_kslot(i >> 3, i & 7)) + m3) + n1);
-      float2 v;
+      uint32_t v;
       if constexpr (H) {
-        const uint32_t w = *((const __attribute__((address_space(1))) uint32_t *)(tsh + p0) + p);
-        v = make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)),
-                        (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)));
+        v = *((const __attribute__((address_space(1))) uint32_t *)(tsh + p0) + p);
       } else {
-        v = gload2(ts + p0, p * (uint32_t)sizeof(float2));
+        typedef float f2v __attribute__((ext_vector_type(2)));
+        typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+        const float2 x = gload2(ts + p0, p * (uint32_t)sizeof(float2));
+        v = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){x.x, x.y}, h2v));
       }
-      dst[i] = ok ? v : make_float2(0.0f, 0.0f);
+      dst[i] = ok ? v : 0u;
     }
   };
   auto load_pass = [&](int64_t s0) {  // without carry
-    float2 a[16];
+    uint32_t a[16];
     load_half(a, s0, 0);
 #pragma unroll
     for (int i = 0; i < 16; i++) xp[i] = a[i];
-    if (!win.zero_hi) {
+    if (!zero_hi) {
       load_half(a, s0, 1);
 #pragma unroll
       for (int i = 0; i < 16; i++) xp[16 + i] = a[i];
@@ -1511,10 +1525,10 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
   // zero (not left undefined: a non-finite value times the zero weight would
   // give NaN)
 #pragma unroll
-  for (int i = 0; i < 16; i++) xp[16 + i] = make_float2(0.0f, 0.0f);
-  __syncthreads();  // wtab
+  for (int i = 0; i < 16; i++) xp[16 + i] = 0u;
+  __syncthreads();  // the window table
   if (carry && NT == 16) {
-    float2 a[16];
+    uint32_t a[16];
     load_half(a, sa, 0);
     load_half(xn, sa, 1);
 #pragma unroll
@@ -1528,10 +1542,10 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
   // complete here, so that no wait at the loop top (where the previous
   // pass's stores are in flight) is needed for them
 #pragma unroll
-  for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i].x), "v"(xp[i].y));
+  for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i]));
   for (int64_t seg = sa; seg < sb; seg += SG) {
     // phase 0: windowed (and sideband-flipped) points as FP16 K chunks
-    cm_phase0(lds, xp, wtab, cth, win, flip);
+    cm_phase0<NT>(lds, xp, wt, flipmask, zero_hi);
     // with carry this segment's second half is the next one's first
     if (carry && NT == 16) {
 #pragma unroll
@@ -1612,7 +1626,7 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
         if (carry && NT == 16) {
 #pragma unroll
           for (int i = 0; i < 16; i++) {
-            asm volatile("" ::"v"(xn[i].x), "v"(xn[i].y));  // the loads complete here
+            asm volatile("" ::"v"(xn[i]));  // the loads complete here
             xp[16 + i] = xn[i];
           }
         } else if (carry) {
@@ -1624,16 +1638,14 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
           const bool last = sg == SG - 1;
 #pragma unroll
           for (int i = 0; i < 16; i++) {
-            asm volatile("" ::"v"(xn[i].x), "v"(xn[i].y));  // the loads complete here
-            const float2 v = last ? xp[16 + i] : xn[i];
-            xp[i] = make_float2(__builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src, __builtin_bit_cast(int, v.x))),
-                                __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src, __builtin_bit_cast(int, v.y))));
+            asm volatile("" ::"v"(xn[i]));  // the loads complete here
+            xp[i] = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(last ? xp[16 + i] : xn[i]));
             xp[16 + i] = xn[i];
           }
         }
         if (!carry) {
 #pragma unroll
-          for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i].x), "v"(xp[i].y));
+          for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i]));
         }
       }
       const float hs = 0.5f * job.spec_scale;
