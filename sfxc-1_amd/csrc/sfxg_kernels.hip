@@ -952,9 +952,16 @@ __device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[1
                                                const float2 *tw, int l, void *out, bool write) {
   constexpr int nd = 256;
   const float amp = bc.amp;
+  // the FP16 series (H) takes the hardware sine and cosine (absolute error
+  // near 1e-6 for these arguments of a few radians, far below its 2^-12
+  // rounding); the FP32 series the full-precision sincosf
+  auto sc = [](float x, float *sn, float *cs) {
+    if constexpr (H) __sincosf(x, sn, cs);
+    else sincosf(x, sn, cs);
+  };
   {
     float sn, cs;
-    sincosf(bc.ramp_arg, &sn, &cs);
+    sc(bc.ramp_arg, &sn, &cs);
     float2 e = make_float2(cs, -sn);
     const float2 step = make_float2(bc.ramp16c, -bc.ramp16s);
 #pragma unroll
@@ -971,7 +978,7 @@ __device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[1
   }
   fft256_g16<+1>(v, buf, tw, l);
   float sn, cs;
-  sincosf(bc.fringe_arg, &sn, &cs);
+  sc(bc.fringe_arg, &sn, &cs);
   float2 r = make_float2(cs, sn);
   const float2 rstep = make_float2(bc.fringe16c, bc.fringe16s);
   if constexpr (!H) {
