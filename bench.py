@@ -43,16 +43,16 @@ FP16_DENSE_PEAK_TFLOPS = 2500.0
 # summaries under profiles/ give the commands); re-measured whenever the
 # kernel changes. Key: (kernel, channels, streams, precision).
 MEASURED_TRAFFIC = {
-    # MFMA channeliser: 8.210e9 read + 8.198e9 written per chunk of 1953
-    # segments x 64 streams
-    ("channelise", 16384, 64, "fp16"): 16408000000,
+    # MFMA channeliser: 4.106e9 read (FP16 series) + 8.198e9 written per
+    # chunk of 1953 segments x 64 streams
+    ("channelise", 16384, 64, "fp16"): 12304000000,
 }
 # Wave-level VALU instructions per launch (SQ_INSTS_VALU, same profiles),
 # against the wave64 FP32 issue rate: a gfx950 SIMD issues up to two
 # independent VALU instructions per quad-cycle (SQ_ACTIVE_INST_VALU2 counts
 # those), so 256 CUs x 4 SIMDs x 2 / 4 cycles x 2.4 GHz.
 MEASURED_VALU_INSTS = {
-    ("channelise", 16384, 64, "fp16"): 1.940e9,
+    ("channelise", 16384, 64, "fp16"): 1.740e9,
 }
 VALU_ISSUE_PEAK = 256 * 4 * 2 / 4 * 2.4e9
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
@@ -87,17 +87,19 @@ def parse():
     return ap.parse_args()
 
 
-def kernel_bytes(job, M, nseg, n_samples, nfold, B, fused, spec_bytes=8):
+def kernel_bytes(job, M, nseg, n_samples, nfold, B, fused, spec_bytes=8, ts_bytes=4):
     """Algorithmic HBM bytes per launch of each kernel. fengine: unpack +
     delay correction (+ channeliser when fused: payload in, spectra out).
-    Spectra take spec_bytes per bin: 8 (FP32), 4 (FP16 / BF16 pairs)."""
+    Spectra take spec_bytes per bin: 8 (FP32), 4 (FP16 / BF16 pairs); the
+    real series between the delay stage and the channeliser ts_bytes per
+    sample: 4 (FP32), 2 (FP16, before the MFMA channeliser)."""
     nc = job.fft_size_correlation
     bits = job.station_streams[0].bits_per_sample
     spectra = M * nseg * (nc + 1) * spec_bytes
     return {
         "stats": M * n_samples * bits / 8,
-        "fengine": M * n_samples * bits / 8 + (spectra if fused else M * n_samples * 4),
-        "channelise": M * n_samples * 4 + spectra,
+        "fengine": M * n_samples * bits / 8 + (spectra if fused else M * n_samples * ts_bytes),
+        "channelise": M * n_samples * ts_bytes + spectra,
         "xstep": spectra + nfold * B * (nc + 1) * 16,
         "normalise": B * (nc + 1) * 8 * 2,
     }
@@ -627,7 +629,10 @@ def summarise(wl, wall, kt, precision="fp32", steps=None):
     B = len(job.baselines())
     nfold = max(1, round(job.integration_time / job.sub_integration_time))
     fused = kt[2] == 0.0
-    kb = kernel_bytes(job, wl.M, nseg, wl.nsamp, nfold, B, fused, 4 if precision in ("fp16", "bf16") else 8)
+    packed = precision in ("fp16", "bf16")
+    # packed spectra at 1024, 4096 and 16384 channels: MFMA channeliser fed an FP16 series
+    ts_bytes = 2 if packed and not fused and job.fft_size_correlation in (1024, 4096, 16384) else 4
+    kb = kernel_bytes(job, wl.M, nseg, wl.nsamp, nfold, B, fused, 4 if packed else 8, ts_bytes)
     value = steps * wl.station_samples_per_step / wall / 1e9
     xflops = 8.0 * B * (job.fft_size_correlation + 1) * nseg
     kt_timed = getattr(wl, "kt_timed", None)
@@ -740,7 +745,8 @@ def main():
             "scaling": "weak",
             "vs_baseline": None,
             "dtype": head,
-            "dtype_detail": f"{head} X-step operands on MFMA with FP32 accumulation; FP32 delay correction; "
+            "dtype_detail": f"{head} X-step operands on MFMA with FP32 accumulation; FP32 delay correction "
+                            "(its real series stored as FP16 for packed FP16 / BF16 spectra, FP32 otherwise); "
                             "channeliser transform on FP16 MFMA operands with FP32 accumulation (packed FP16 / BF16 "
                             "spectra; FP32 transform for plain and split FP16); FP64 delay model and phase arguments",
             "data": "synthetic: seeded 2-bit station data generated in device memory (common signal rho 0.01 and a "

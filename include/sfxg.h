@@ -51,10 +51,12 @@ extern "C" {
  * per-channel Hermitian rank-k update on MFMA with FP32 accumulation:
  * FP16 or BF16 operands, or FP16 operands split into hi + lo parts (three
  * products). Pulsar binning takes every precision (per-bin Gram tiles on MFMA).
- * With FP16 or BF16 at fft_size_correlation 16384 the channeliser's transform
- * also runs on MFMA with FP16 operands and FP32 accumulation (unless the
- * tuning table's chan_fp32 selects the FP32 transform); the visibility error
- * stays within the FP16 bar of 1e-3 relative to the FP64 pipeline. */
+ * With FP16 or BF16 at fft_size_correlation 1024, 4096 or 16384 the
+ * channeliser's transform also runs on MFMA with FP16 operands and FP32
+ * accumulation, fed an FP16 real series (unless the tuning table's chan_fp32
+ * selects the FP32 transform, fused with the delay stage at 1024 and 4096);
+ * the visibility error stays within the FP16 bar of 1e-3 relative to the
+ * FP64 pipeline. */
 #define SFXG_PREC_FP32 0
 #define SFXG_PREC_FP16 1
 #define SFXG_PREC_BF16 2
@@ -155,9 +157,14 @@ typedef struct {
   int32_t lut;        /* SFXG_LUT_* */
   int32_t precision;  /* SFXG_PREC_* */
   /* Nonzero: run delay correction and the channeliser as two kernels and
-   * keep the real series in HBM for SFXG_STAGE_TIME_SERIES (verification).
-   * Zero (default): for N_c <= 4096 they run fused and the series stays in
-   * LDS. Both give identical results. */
+   * keep the FP32 real series in HBM for SFXG_STAGE_TIME_SERIES
+   * (verification). Zero (default): for N_c <= 4096 they run fused and the
+   * series stays in LDS, except with packed FP16 / BF16 spectra at N_c 1024
+   * and 4096, which take the MFMA channeliser; that channeliser (N_c 1024,
+   * 4096, 16384) gets the series as FP16 (it rounds its points to FP16
+   * anyway). FP32 and split FP16 give identical results either way; packed
+   * FP16 / BF16 agree within their rounding (the FP16 series' phasors come
+   * from the hardware sine and cosine). */
   int32_t keep_time_series;
   /* Spectra budget in MiB of one chunk: a slice is processed in chunks of
    * whole sub-integrations whose spectra fit it (at least one
@@ -409,8 +416,10 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap);
  * the register tiles on 64- and 16-channel blocks, 4 and 8 the per-baseline
  * kernel; with pulsar binning in FP32, 2 forces the register tiles and 1, 4
  * or 8 the per-baseline kernel; chan_fp32 1 gives packed FP16 / BF16 spectra
- * at 16384 channels the FP32 transform instead of the MFMA one, whose
- * workgroup count chan_wgs then sets); *written = bytes needed including the
+ * at 1024, 4096 and 16384 channels the FP32 transform instead of the MFMA
+ * one (the fused F-engine at 1024 and 4096, whose threads fe_threads sets;
+ * at 16384 chan_wgs sets either channeliser's workgroups); *written = bytes
+ * needed including the
  * terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear
  * empties the table. */
 int sfxg_tune_export(char *dst, size_t cap, size_t *written);

@@ -1202,32 +1202,37 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
 }
 
 // ---------------------------------------------------------------------------
-// k_chan_mfma: the N_c = 16384 channeliser of the mixed-precision X-steps
-// (spectra stored as packed FP16 or BF16), its transform on MFMA. The
-// 2 N_c windowed reals of a segment are N = 16384 complex points z[n],
-//   n = 512 m2 + 16 m3 + n1,  q = k2 + 32 k3,  k = q + 1024 k1:
+// k_chan_mfma: the channeliser of the mixed-precision X-steps (spectra
+// stored as packed FP16 or BF16) for N_c = 1024 NT (NT = 1, 4, 16), its
+// transform on MFMA. The 2 N_c windowed reals of a segment are N = N_c
+// complex points z[n],
+//   n = NT (32 m2 + m3) + n1,  q = k2 + 32 k3,  k = q + 1024 k1:
 //   V[n1][q]      = sum_m3 W32^(m3 k3) W1024^(m3 k2) sum_m2 z[n] W32^(m2 k2)
-//   Z[q + 1024 k1] = sum_n1 W16^(n1 k1) W_N^(n1 q) V[n1][q].
-// Each of the 16 residues n1 is a 32 x 32 tile (rows m3, columns m2). Stage
-// B1 (over m2) takes the tile as the A operand of v_mfma_f32_32x32x16_f16
-// (K = the real and imaginary parts of 32 m2) against the real form of the
-// 32-point DFT matrix; its 32 x 32 FP32 result, times W1024^(m3 k2) and
-// rounded to FP16, is the B operand of stage B2 (over m3) as it lies in the
-// registers: a lane holds the result rows 8 g + 4 h + j (h = lane / 32), so
-// B1 orders the K slots of its operands the same way (cm_kslot) and the two
-// stages exchange nothing. The DFT matrix is symmetric, so one set of
-// fragments serves as B1's B operand and B2's A operand. The last stage (16
-// points over n1, twiddles W_N^(n1 q)) and the split of the N-point
-// transform into the N + 1 bins of the real one run in FP32 registers: the
-// thread of row q also takes row 1024 - q, which holds the bins N - k.
-// Phases per segment: z from HBM into the tiles' LDS operands (16-byte
+//   Z[q + 1024 k1] = sum_n1 W_NT^(n1 k1) W_N^(n1 q) V[n1][q].
+// Each of the NT residues n1 is a 32 x 32 tile (rows m3, columns m2); a
+// pass holds 16 tiles, those of 16 / NT segments. Stage B1 (over m2) takes
+// the tile as the A operand of v_mfma_f32_32x32x16_f16 (K = the real and
+// imaginary parts of 32 m2) against the real form of the 32-point DFT
+// matrix; its 32 x 32 FP32 result, times W1024^(m3 k2) and rounded to FP16,
+// is the B operand of stage B2 (over m3) as it lies in the registers: a lane
+// holds the result rows 8 g + 4 h + j (h = lane / 32), so B1 orders the K
+// slots of its operands the same way (cm_kslot) and the two stages exchange
+// nothing. The DFT matrix is symmetric, so one set of fragments serves as
+// B1's B operand and B2's A operand. The last stage (NT points over n1,
+// twiddles W_N^(n1 q)) and the split of the N-point transform into the N + 1
+// bins of the real one run in FP32 registers: the thread of row q also takes
+// row 1024 - q, which holds the bins N - k.
+// Phases per pass: the windowed points into the tiles' LDS operands (16-byte
 // chunks, XOR-swizzled); B1 and B2 for two tiles per wave, whose V (packed
 // FP16 pairs) goes where those two tiles lay; then the last stage and the
-// spectrum stores. 64 KiB of LDS and at most 128 VGPRs: two workgroups per CU.
-// The operands are FP16 (unit roundoff 2^-11) with FP32 accumulation: z,
-// the B1 results and V are rounded once each, which puts the transform's
-// error near that of the FP16 spectra themselves, within the mixed-precision
-// visibility bar.
+// spectrum stores. 64 KiB of operands, the window table and per-thread
+// constants: about 152 KiB of LDS at N_c = 16384 and up to 256 VGPRs, one
+// workgroup per CU.
+// The operands are FP16 (unit roundoff 2^-11) with FP32 accumulation: the
+// series (FP16 from the delay stage, or FP32 rounded on load), the windowed
+// points, the B1 results and V are rounded once each, which puts the
+// transform's error near that of the FP16 spectra themselves, within the
+// mixed-precision visibility bar.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -1412,13 +1417,13 @@ __device__ __forceinline__ void cm_phase0(uint4 *lds, const uint32_t (&xp)[32], 
   }
 }
 
-// A workgroup walks a run of consecutive segments of one stream. With a
-// window, the second half of a segment (points 512 m2 + t, m2 >= 16) is the
-// first half of the next one in the same thread, so a thread carries it in
-// registers and reads only the 16 new points; they are loaded before this
-// segment's spectrum stores (the vector memory counter is in order, so
-// waiting for them does not wait for the stores). Without a window a segment
-// uses only its first half and nothing carries. One workgroup per CU.
+// A workgroup walks a run of consecutive segments of one stream, a pass of
+// 16 / NT segments at a time. With a window the second half of a segment is
+// the first half of the next one, so a pass reads only the 16 new points of
+// each thread; they are loaded before this pass's spectrum stores (the
+// vector memory counter is in order, so waiting for them does not wait for
+// the stores). Without a window a segment uses only its first half and
+// nothing carries. One workgroup per CU.
 template <int SP, int NT, bool H>
 __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restrict__ streams, DevJob job,
                                                       int segs_per_wg) {
