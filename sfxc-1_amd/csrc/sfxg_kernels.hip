@@ -981,6 +981,7 @@ __device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[1
   sc(bc.fringe_arg, &sn, &cs);
   float2 r = make_float2(cs, sn);
   const float2 rstep = make_float2(bc.fringe16c, bc.fringe16s);
+  if constexpr (H) r = make_float2(amp * cs, amp * sn);  // the amplitude rides on the phasor
   if constexpr (!H) {
     auto *o = gmem(static_cast<float *>(out));
 #pragma unroll
@@ -998,9 +999,9 @@ __device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[1
     const int wb = (l >> 1) + (odd ? 8 : 0);  // word 16 j + wb
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const float y0 = amp * (v[2 * j].x * r.x + v[2 * j].y * r.y);
+      const float y0 = v[2 * j].x * r.x + v[2 * j].y * r.y;
       r = cmul(r, rstep);
-      const float y1 = amp * (v[2 * j + 1].x * r.x + v[2 * j + 1].y * r.y);
+      const float y1 = v[2 * j + 1].x * r.x + v[2 * j + 1].y * r.y;
       r = cmul(r, rstep);
       const uint32_t h0 = __builtin_bit_cast(uint16_t, (_Float16)y0), h1 = __builtin_bit_cast(uint16_t, (_Float16)y1);
       const uint32_t recv = (uint32_t)__builtin_amdgcn_mov_dpp((int)(odd ? h0 : h1), 0xB1, 0xF, 0xF, false);
@@ -1310,12 +1311,12 @@ __device__ __forceinline__ void cm_last_stage(float2 (&z)[NT], float2 w) {
 
 // Real-transform bins from Z[k] and Z[N - k]: X[k] = E + W_2N^k O and
 // X[N - k] = conj(E - W_2N^k O), E = (Z[k] + conj Z[N - k]) / 2,
-// O = -i (Z[k] - conj Z[N - k]) / 2.
-// hs = scale / 2 with the spectra's power-of-two scale: the bins come out
-// scaled (exactly as scaling them afterwards would).
-__device__ __forceinline__ void cm_split(float2 zk, float2 zn, float2 w, float hs, float2 &xk, float2 &xn) {
-  const float2 e = make_float2(hs * (zk.x + zn.x), hs * (zk.y - zn.y));
-  const float2 o = make_float2(hs * (zk.y + zn.y), -hs * (zk.x - zn.x));
+// O = -i (Z[k] - conj Z[N - k]) / 2. The factor 1/2 and the spectra's
+// power-of-two scale are already in Z (folded into the twiddles between the
+// MFMA stages: exact, as scaling afterwards would be).
+__device__ __forceinline__ void cm_split(float2 zk, float2 zn, float2 w, float2 &xk, float2 &xn) {
+  const float2 e = make_float2(zk.x + zn.x, zk.y - zn.y);
+  const float2 o = make_float2(zk.y + zn.y, zn.x - zk.x);
   const float2 wo = cmul(w, o);
   xk = cadd(e, wo);
   xn = make_float2(e.x - wo.x, -(e.y - wo.y));
@@ -1459,13 +1460,17 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
       fi[g][e] = (_Float16)w.y;
       nfi[g][e] = (_Float16)-w.y;
     }
-  // W1024^(m3 col) of the B1 result in register r (m3 = 8 (r / 4) + 4 h + r % 4)
+  // W1024^(m3 col) of the B1 result in register r (m3 = 8 (r / 4) + 4 h + r % 4),
+  // times hs = scale / 2: the split's factor 1/2 and the spectra's
+  // power-of-two scale (exact), which also keeps the FP16 operand of B2 and
+  // V near unit rms, far from the FP16 range's ends
   if (t < 64) {
+    const float hs = 0.5f * job.spec_scale;
 #pragma unroll
     for (int p = 0; p < 8; p++) {
       const float2 a = cm_w((8 * (p >> 1) + 4 * h + 2 * (p & 1)) * col, 1024);
       const float2 b = cm_w((8 * (p >> 1) + 4 * h + 2 * (p & 1) + 1) * col, 1024);
-      ctw[p][t] = make_float4(a.x, a.y, b.x, b.y);
+      ctw[p][t] = make_float4(hs * a.x, hs * a.y, hs * b.x, hs * b.y);
     }
   }
   // the last stage's rows: q0 = t and q1 = 1024 - t. Thread 0 takes row 0
@@ -1660,7 +1665,6 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
           for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i]));
         }
       }
-      const float hs = 0.5f * job.spec_scale;
       const float2 wn0 = cth[0][tq], wn1 = cth[1][tq], w0 = cth[2][tq];
 #pragma unroll
       for (int sig = 0; sig < SG; sig++) {
@@ -1674,7 +1678,7 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
 #pragma unroll
         for (int k1 = 0; k1 < NT; k1++) {
           float2 xk, xn;
-          cm_split(z0[k1], z1[NT - 1 - k1], cmul(w0, cm_w32((16 / NT) * k1)), hs, xk, xn);
+          cm_split(z0[k1], z1[NT - 1 - k1], cmul(w0, cm_w32((16 / NT) * k1)), xk, xn);
           cm_put<SP>(row, (uint32_t)(r0 + 1024 * k1), xk);
           cm_put<SP>(row, (uint32_t)(r1 + 1024 * (NT - 1 - k1)), xn);
         }
@@ -1686,7 +1690,7 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
 #pragma unroll
           for (int k1 = 0; k1 < (NT + 1) / 2; k1++) {
             float2 xk, xn;
-            cm_split(z1[k1], z1[NT - 1 - k1], cmul(w5, cm_w32((16 / NT) * k1)), hs, xk, xn);
+            cm_split(z1[k1], z1[NT - 1 - k1], cmul(w5, cm_w32((16 / NT) * k1)), xk, xn);
             cm_put<SP>(row, (uint32_t)(512 + 1024 * k1), xk);
             if (NT - 1 - k1 != k1) cm_put<SP>(row, (uint32_t)(512 + 1024 * (NT - 1 - k1)), xn);
           }

@@ -636,6 +636,54 @@ def test_mfma_channeliser(ctx, nchan, prec, window, sideband):
     assert l2 <= MIXED_BARS[prec][0] and worst <= MIXED_BARS[prec][1]
 
 
+@pytest.mark.parametrize("nchan", [16384, 1024])
+def test_mfma_channeliser_strong_tone(ctx, nchan):
+    """Every stream carries a tone of 3 sigma (before the 2-bit quantiser)
+    plus its own noise. A 1024-point partial transform of the MFMA
+    channeliser then sums the tone coherently to about 2.3e5 in the delay
+    stage's units (a numpy model of the series), beyond FP16's 65504 (an
+    unscaled FP16 operand would overflow to infinity); its stage operands
+    carry the spectra's
+    power-of-two scale before they are rounded to FP16, which keeps them near
+    unit rms. The FP16 spectra stay finite and the visibilities meet the
+    mixed-precision bars against the FP64 oracle."""
+    ro = {}
+    fs = 32_000_000
+    job, _ = configs.build_slices(seed=45, n_stations=2, pols=["R"], fs=fs, bw=16_000_000, nchan=nchan,
+                                  seconds=0.25, t_int_usec=250000, rho=0.05, bits=2, sky_freq=1_400_000_000,
+                                  offsets=[0.0, 1e-6], rates=[0.0, 0.0], raw_out=ro)[0]
+    from sfxg import synth
+    rng = np.random.default_rng(45)
+    nsamp = job.n_ffts_per_integration(0) * job.fft_size_delaycor
+    hdrs = []
+    for i, raw in enumerate(ro["raw"]):
+        n = len(raw) * 4
+        x = 3.0 * np.cos(2 * np.pi * 0.2345678 * np.arange(n) + i) + rng.standard_normal(n)
+        raw2 = synth.pack(synth.quantise(x, 2), 2)
+        hdrs.append(synth.header_stream(raw2, ro["raw_start"], job.delay_models[i], job.start_time, nsamp, fs, 2))
+    capi.tune_clear()
+    try:
+        out = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=1)
+        sp = ctx.stage(capi.STAGE_SPECTRA, 0)
+    finally:
+        ctx.set_options(lut=LUT_CONFIG, precision=0)
+    assert np.all(np.isfinite(sp.real)) and np.all(np.isfinite(sp.imag))
+    peak = np.abs(sp).max() / np.sqrt(np.mean(np.abs(sp) ** 2))
+    assert peak > 10.0  # the tone stands out of the noise
+    r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64")
+    heads_g, vis_g = vis_from_bytes(job, out)
+    heads_r, vis_r = vis_from_bytes(job, r64["output"])
+    assert heads_g == heads_r
+    l2, _ = mixed_errors(vis_g[0], vis_r[0])
+    # one channel holds most of each product's power, so the worst channel
+    # is measured against the product's peak rather than its rms
+    g, r = vis_g[0].astype(np.complex128), vis_r[0].astype(np.complex128)
+    worst = max(np.abs(g[k] - r[k]).max() / np.abs(r[k]).max() for k in range(r.shape[0]))
+    print(f"strong tone {nchan} channels: spectrum peak {peak:.0f} x rms; vs FP64 oracle relative L2 {l2:.3e}, "
+          f"worst channel {worst:.3e} of the peak")
+    assert l2 <= MIXED_BARS[1][0] and worst <= 1e-3
+
+
 @pytest.mark.parametrize("prec", [1, 2])
 def test_packed_unaligned_folds(ctx, prec):
     """Sub-integrations of 453 segments start and end inside the four-segment
