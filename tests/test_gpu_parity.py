@@ -637,6 +637,44 @@ def test_mfma_channeliser(ctx, nchan, prec, window, sideband):
 
 
 @pytest.mark.parametrize("nchan", [16384, 1024])
+def test_mfma_channeliser_fp32_series(ctx, nchan):
+    """With keep_time_series the MFMA channeliser reads the FP32 series (and
+    rounds each point to FP16 as it loads it) where it otherwise gets the
+    delay stage's FP16 series: the two agree up to that rounding of slightly
+    different FP32 values (the FP16 series' phasors come from the hardware
+    sine and cosine), the kept series matches the FP64 oracle's delay stage,
+    and both runs meet the mixed-precision bars against the FP64 oracle."""
+    job, hdrs = configs.build_slices(seed=46, n_stations=2, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
+                                     seconds=0.25, t_int_usec=250000, rho=0.05, bits=2, sky_freq=1_400_000_000,
+                                     offsets=[0.0, 1.5e-6], rates=[0.0, 1e-7])[0]
+    capi.tune_clear()
+    try:
+        out_h = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=1)
+        sp_h = ctx.stage(capi.STAGE_SPECTRA, 1)
+        out_f = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=1, keep_time_series=True)
+        sp_f = ctx.stage(capi.STAGE_SPECTRA, 1)
+        ts = ctx.stage(capi.STAGE_TIME_SERIES, 1)
+    finally:
+        ctx.set_options(lut=LUT_CONFIG, precision=0)
+    rms = np.sqrt(np.mean(np.abs(sp_f) ** 2))
+    d = np.abs(sp_h.astype(np.complex128) - sp_f.astype(np.complex128)) / rms
+    print(f"FP16 vs FP32 series at {nchan} channels: spectra differ by {np.sqrt(np.mean(d ** 2)):.2e} rms, "
+          f"{d.max():.2e} at most, of the spectrum rms")
+    assert np.sqrt(np.mean(d ** 2)) <= 4e-4 and d.max() <= 4e-3
+    r64 = O.run_slice(job, hdrs, lut=LUT_CONFIG, precision="f64", keep_intermediate=True)
+    t_ref = r64["dc"][1]["time_series"].astype(np.float64)
+    err_t = np.max(np.abs(ts - t_ref)) / np.sqrt(np.mean(t_ref ** 2))
+    print(f"kept FP32 series vs the FP64 oracle's: {err_t:.2e} of its rms at most")
+    assert err_t < 2e-5
+    heads_r, vis_r = vis_from_bytes(job, r64["output"])
+    for out in (out_h, out_f):
+        heads_g, vis_g = vis_from_bytes(job, out)
+        assert heads_g == heads_r
+        l2, worst = mixed_errors(vis_g[0], vis_r[0])
+        assert l2 <= MIXED_BARS[1][0] and worst <= MIXED_BARS[1][1]
+
+
+@pytest.mark.parametrize("nchan", [16384, 1024])
 def test_mfma_channeliser_strong_tone(ctx, nchan):
     """Every stream carries a tone of 3 sigma (before the 2-bit quantiser)
     plus its own noise. A 1024-point partial transform of the MFMA
