@@ -962,7 +962,9 @@ __device__ __forceinline__ void finish_block16(const BlockC16 &bc, float2 (&v)[1
   {
     float sn, cs;
     sc(bc.ramp_arg, &sn, &cs);
-    float2 e = make_float2(cs, -sn);
+    // H: the pair split's factor 1/2 rides on the ramp (exact)
+    constexpr float rs = H ? 0.5f : 1.0f;
+    float2 e = make_float2(rs * cs, -rs * sn);
     const float2 step = make_float2(bc.ramp16c, -bc.ramp16s);
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -1056,12 +1058,13 @@ __global__ __launch_bounds__(256, 3) void k_delay(const DevStream *__restrict__ 
     wave_lds_sync();
     const int ln = (16 - l) & 15;
     float2 va[16], vb[16];
+    constexpr float hf = H ? 1.0f : 0.5f;  // H: the 1/2 is applied with the ramp
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int rq = l ? 15 - q : (16 - q) & 15;
       const float2 zk = v[q], zn = buf[ln * G16_ROW + rq];
-      va[q] = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-      vb[q] = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+      va[q] = make_float2(hf * (zk.x + zn.x), hf * (zk.y - zn.y));
+      vb[q] = make_float2(hf * (zk.y + zn.y), -hf * (zk.x - zn.x));
     }
     constexpr int64_t bb = H ? 512 : 1024;  // bytes per block
     char *base = reinterpret_cast<char *>(s.time_series) + (ba - job.blk_base) * bb;
