@@ -783,7 +783,7 @@ def main():
         print(f"8-station setup done: {len(wl.inputs)} slices in {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
         c1 = {"workload": "8 stations x 2 pol, 8 subbands x 32 MHz at 64 Ms/s, 2-bit, 4096 channels, 1 s "
                           f"integrations, {args.c1_steps} s", "station_samples_per_step": wl.station_samples_per_step}
-        for name in ("fp32", "split_fp16"):
+        for name in ("fp32", "fp16", "split_fp16"):
             wall, kt = wl.run(torch, capi, PREC[name], 1, args.c1_contexts, dev, autotune=not args.no_autotune)
             r = summarise(wl, wall, kt, name)
             r.pop("_kb")
@@ -792,7 +792,8 @@ def main():
                 add_mfma_frac(r, peaks)
             c1[name] = r
             print(f"8-station {name} done: {r['value']:.2f}", file=sys.stderr, flush=True)
-        c1["mixed_vs_plain"] = round(c1["split_fp16"]["value"] / c1["fp32"]["value"], 3)
+        c1["mixed_vs_plain"] = round(c1["fp16"]["value"] / c1["fp32"]["value"], 3)
+        c1["split_fp16_vs_plain"] = round(c1["split_fp16"]["value"] / c1["fp32"]["value"], 3)
         line["config1"] = c1
         del wl
         torch.cuda.empty_cache()
