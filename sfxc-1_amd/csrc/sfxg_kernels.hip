@@ -883,47 +883,67 @@ __device__ __forceinline__ void fft256_g16(float2 (&v)[16], float2 *buf, const f
   dft16s<SIGN>(v);
 }
 
-// Samples 16 n1 + l (n1 < 16) of block blk (zeros for a block past the
-// end): a block inside one payload run takes them with 16 byte loads at a
-// fixed stride and decodes them from the table in registers; a block that
-// crosses a run boundary or invalid data goes sample by sample.
-// bs: the block's block_src entry (loaded ahead by the caller)
-__device__ __forceinline__ void load_block16(const DevStream &s, int64_t blk, int64_t n_blocks, int l,
-                                             float (&x)[16], int64_t bs) {
+// Payload words of a block that lies in one payload run (bs >= 0, its
+// first payload sample): the aligned 32-bit words that hold its 256 samples,
+// 17 at most for 2-bit data and 9 for 1-bit; lane l of the group loads word
+// l, and word 16 + l where the block reaches it (lane 0, 2-bit). Every word
+// loaded holds at least one byte of the block, so it lies in a page of the
+// payload's allocation. A block past the end or across a run boundary loads
+// nothing here.
+struct Words16 {
+  uint32_t w0, w1;
+};
+// field offset of the block's first sample in its first word (samples of
+// 2 or 1 bits, least significant first)
+__device__ __forceinline__ int words16_f0(const DevStream &s, int64_t bs, uintptr_t &a) {
+  const int lb = s.bits == 2 ? 2 : 3;  // log2 samples per byte
+  a = reinterpret_cast<uintptr_t>(s.payload) + (uintptr_t)(bs >> lb);
+  return (int)((a & 3) << lb) + (int)(bs & ((1 << lb) - 1));
+}
+__device__ __forceinline__ Words16 load_words16(const DevStream &s, int64_t bs, bool fast, int l) {
+  Words16 r{0u, 0u};
+  if (!fast) return r;
+  uintptr_t a;
+  const int f0 = words16_f0(s, bs, a);
+  const int last = (f0 + 255) >> (s.bits == 2 ? 4 : 5);
+  const auto *w = gmem(reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3));
+  if (l <= last) r.w0 = w[l];
+  if (l + 16 <= last) r.w1 = w[l + 16];
+  return r;
+}
+
+// Samples 16 n1 + l (n1 < 16) of block blk: an in-run block decodes them
+// from its payload words, staged in the group's LDS (wl), with the table in
+// registers; a block that crosses a run boundary or invalid data goes sample
+// by sample; a block past the end is zeros. bs: the block's block_src entry.
+__device__ __forceinline__ void block16(const DevStream &s, int64_t blk, bool valid, int64_t bs, const uint32_t *wl,
+                                        int l, float (&x)[16]) {
   constexpr int nd = 256;
-  if (blk >= n_blocks) {
+  if (!valid) {
 #pragma unroll
     for (int q = 0; q < 16; q++) x[q] = 0.0f;
     return;
   }
-  const int64_t o0 = blk * nd;
   if (bs >= 0) {
-    const int64_t p0 = bs;
-    const auto *pay = gmem(s.payload);
-    uint32_t c[16];
+    uintptr_t a;
+    const int f = words16_f0(s, bs, a) + l;
     if (s.bits == 2) {
-      const int u = (int)(p0 & 3) + l;
-      const auto *b = pay + (p0 >> 2) + (u >> 2);
-#pragma unroll
-      for (int q = 0; q < 16; q++) c[q] = b[4 * q];
-      const int sh = (u & 3) * 2;
+      const int dq = f >> 4, sh = 2 * (f & 15);
       const float l0 = s.lut[0], l1 = s.lut[1], l2 = s.lut[2], l3 = s.lut[3];
 #pragma unroll
-      for (int q = 0; q < 16; q++) x[q] = lut4((c[q] >> sh) & 3, l0, l1, l2, l3);
+      for (int q = 0; q < 16; q++) x[q] = lut4((wl[q + dq] >> sh) & 3, l0, l1, l2, l3);
     } else {
-      const int u = (int)(p0 & 7) + l;
-      const auto *b = pay + (p0 >> 3) + (u >> 3);
-#pragma unroll
-      for (int q = 0; q < 16; q++) c[q] = b[2 * q];
-      const int sh = u & 7;
       const float l0 = s.lut[0], l1 = s.lut[1];
 #pragma unroll
-      for (int q = 0; q < 16; q++) x[q] = ((c[q] >> sh) & 1) ? l1 : l0;
+      for (int q = 0; q < 16; q++) {
+        const int i = f + 16 * q;
+        x[q] = ((wl[i >> 5] >> (i & 31)) & 1) ? l1 : l0;
+      }
     }
   } else {
     int64_t r = ~bs;
 #pragma unroll
-    for (int q = 0; q < 16; q++) x[q] = sample_at(s, o0 + 16 * q + l, r);
+    for (int q = 0; q < 16; q++) x[q] = sample_at(s, blk * nd + 16 * q + l, r);
   }
 }
 
@@ -1032,20 +1052,44 @@ __global__ __launch_bounds__(256, 3) void k_delay(const DevStream *__restrict__ 
   float2 *buf = lds[wave][g];
   const DevStream &s = streams[blockIdx.y];
   const int64_t end = job.blk_end < job.n_blocks ? job.blk_end : job.n_blocks;
-  // block_src of this group's pair, loaded one iteration ahead, so that the
-  // payload loads at the top of an iteration depend on no load of their own
+  // block_src of this group's pair two iterations ahead and its payload
+  // words one iteration ahead, so that nothing at the top of an iteration
+  // waits on a load of its own
   const int64_t step = (int64_t)gridDim.x * 32;
   auto src_of = [&](int64_t b) { return b < end ? cmem(s.block_src)[b] : (int64_t)0; };
+  auto fast = [&](int64_t b, int64_t bs) { return b < end && bs >= 0; };
   int64_t blk = job.blk_base + ((int64_t)blockIdx.x * 4 + wave) * 8;
   int64_t bsa = src_of(blk + 2 * g), bsb = src_of(blk + 2 * g + 1);
+  int64_t bsa_n = src_of(blk + step + 2 * g), bsb_n = src_of(blk + step + 2 * g + 1);
+  Words16 wa = load_words16(s, bsa, fast(blk + 2 * g, bsa), l);
+  Words16 wb = load_words16(s, bsb, fast(blk + 2 * g + 1, bsb), l);
+  // complete before the loop as in it (before each pair's stores), so that
+  // the loop top waits for nothing
+  asm volatile("" ::"v"(wa.w0), "v"(wa.w1), "v"(wb.w0), "v"(wb.w1), "v"(bsa_n), "v"(bsb_n));
+  uint32_t *ws = reinterpret_cast<uint32_t *>(buf);  // staged words: block a at 0, block b at 32
   for (; blk < end; blk += step) {
     const int64_t ba = blk + 2 * g;  // this group's pair (ba, ba + 1)
-    float xa[16], xb[16];
-    load_block16(s, ba, ba < end ? job.n_blocks : 0, l, xa, bsa);
-    load_block16(s, ba + 1, ba + 1 < end ? job.n_blocks : 0, l, xb, bsb);
-    bsa = src_of(ba + step);
-    bsb = src_of(ba + step + 1);
     const BlockC16 ca = block_c16(s, ba < end ? ba : 0, l), cb = block_c16(s, ba + 1 < end ? ba + 1 : 0, l);
+    // this pair's words (loaded one iteration ago) through the group's LDS;
+    // the next pair's loads are issued only after them, so that waiting for
+    // these does not wait for those (the vector memory counter is in order)
+    wave_lds_sync();  // the group's last reads of buf (previous pair) are done
+    ws[l] = wa.w0;
+    ws[32 + l] = wb.w0;
+    if (l == 0) {
+      ws[16] = wa.w1;
+      ws[48] = wb.w1;
+    }
+    wave_lds_sync();
+    float xa[16], xb[16];
+    block16(s, ba, ba < end, bsa, ws, l, xa);
+    block16(s, ba + 1, ba + 1 < end, bsb, ws + 32, l, xb);
+    wa = load_words16(s, bsa_n, fast(ba + step, bsa_n), l);
+    wb = load_words16(s, bsb_n, fast(ba + step + 1, bsb_n), l);
+    bsa = bsa_n;
+    bsb = bsb_n;
+    bsa_n = src_of(ba + 2 * step);
+    bsb_n = src_of(ba + 2 * step + 1);
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) v[q] = make_float2(xa[q], xb[q]);
@@ -1066,6 +1110,9 @@ __global__ __launch_bounds__(256, 3) void k_delay(const DevStream *__restrict__ 
       va[q] = make_float2(hf * (zk.x + zn.x), hf * (zk.y - zn.y));
       vb[q] = make_float2(hf * (zk.y + zn.y), -hf * (zk.x - zn.x));
     }
+    // the next pair's words and block_src complete here, before this pair's
+    // series stores, so that no later wait for them also waits for stores
+    asm volatile("" ::"v"(wa.w0), "v"(wa.w1), "v"(wb.w0), "v"(wb.w1), "v"(bsa_n), "v"(bsb_n));
     constexpr int64_t bb = H ? 512 : 1024;  // bytes per block
     char *base = reinterpret_cast<char *>(s.time_series) + (ba - job.blk_base) * bb;
     finish_block16<H>(ca, va, buf, tw, l, base, ba < end);
