@@ -3395,14 +3395,20 @@ template <int PREC, bool HS>
 __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar_bin(
     const void *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT, int CH,
     int64_t seg_base, int64_t n_segments, float scale, const int16_t *__restrict__ bins,
-    const uint32_t *__restrict__ masks, int W, float *__restrict__ gram) {
+    const uint32_t *__restrict__ masks, int W, float *__restrict__ gram, int n_cb, int G) {
   extern __shared__ _Float16 ylds[];
   constexpr int planes = PREC == SFXG_PREC_SPLIT_FP16 ? 2 : 1;
   const int R = 32 * RT;
   const int nT = RT * (RT + 1) / 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int cb = blockIdx.x, n_cb = gridDim.x;
-  const int j = blockIdx.y;
+  // (channel block, bin) of this workgroup: G neighbouring channel blocks of
+  // one bin, whose rows share 128-byte lines and which walk nearly the same
+  // stages, run on one XCD together (its L2 fetches a line once); the block
+  // count is padded to whole groups, padding workgroups return at once
+  const int n_cbp = (n_cb + G - 1) / G * G;
+  const int L = xcd_grouped(blockIdx.x, gridDim.x, G);
+  const int cb = L % n_cbp, j = L / n_cbp;
+  if (cb >= n_cb) return;
   const int b = j < MFB_Z0 ? 0 : j - MFB_Z0 + 1;
   const int part = j < MFB_Z0 ? j : 0, parts = j < MFB_Z0 ? MFB_Z0 : 1;
   const int64_t n_stages = (n_segments + MF_KS - 1) / MF_KS;
@@ -4317,12 +4323,12 @@ template <int PREC, bool HS>
 static hipError_t launch_mfma_pulsar_bin_t(dim3 grid, size_t lds, hipStream_t s, const void *spectra,
                                            int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT, int CH,
                                            int64_t seg_base, int64_t n_segments, float scale, const int16_t *bins,
-                                           const uint32_t *masks, int W, float *gram) {
+                                           const uint32_t *masks, int W, float *gram, int n_cb, int G) {
   hipError_t e = hipFuncSetAttribute((const void *)k_xstep_mfma_pulsar_bin<PREC, HS>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_xstep_mfma_pulsar_bin<PREC, HS>), grid, dim3(MF_THREADS), lds, s, spectra, stream_stride,
-                     seg_stride, n_ch, M, RT, CH, seg_base, n_segments, scale, bins, masks, W, gram);
+                     seg_stride, n_ch, M, RT, CH, seg_base, n_segments, scale, bins, masks, W, gram, n_cb, G);
   return hipGetLastError();
 }
 
@@ -4357,11 +4363,15 @@ hipError_t launch_xstep_mfma_pulsar(int precision, const void *spectra, bool hal
                        n_segments, n_ch_run, CH, n_cb, n_stages, W, masks);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    dim3 grid((unsigned)n_cb, (unsigned)(nbins - 1 + MFB_Z0));
+    // groups of G = 8 channel blocks on one XCD (a group's rows span several
+    // 128-byte lines, so few lines are shared with the next group)
+    const int G = 8;
+    const int n_cbp = (n_cb + G - 1) / G * G;
+    dim3 grid((unsigned)(n_cbp * (nbins - 1 + MFB_Z0)));
     const void *sp = static_cast<const char *>(spectra) + (size_t)ch_base * (half_spectra ? 4 : 8);
 #define SFXG_MFB(P, H)                                                                                            \
   return launch_mfma_pulsar_bin_t<P, H>(grid, lds, s, sp, stream_stride, seg_stride, n_ch_run, M, RT, CH, seg_base, \
-                                        n_segments, scale, bins, masks, W, gram)
+                                        n_segments, scale, bins, masks, W, gram, n_cb, G)
     if (half_spectra) {
       if (precision == SFXG_PREC_FP16) SFXG_MFB(SFXG_PREC_FP16, true);
       if (precision == SFXG_PREC_BF16) SFXG_MFB(SFXG_PREC_BF16, true);
