@@ -1359,6 +1359,26 @@ __device__ __forceinline__ void cm_last_stage(float2 (&z)[NT], float2 w) {
   }
 }
 
+// The last stage of rows q and 1024 - q together: the second row's twiddles
+// are W_N^(n1 (1024 - q)) = W_NT^n1 conj(W_N^(n1 q)), so one running power
+// serves both (conjugated for the second row), and the factor W_NT^n1 shifts
+// the second row's NT-point result by one bin: on return z1[k1] holds
+// Z[(1024 - q) + 1024 ((k1 + NT - 1) mod NT)].
+template <int NT>
+__device__ __forceinline__ void cm_last_stage2(float2 (&z0)[NT], float2 (&z1)[NT], float2 w) {
+  if constexpr (NT > 1) {
+    float2 p = w;
+#pragma unroll
+    for (int i = 1; i < NT; i++) {
+      z0[i] = cmul(z0[i], p);
+      z1[i] = cmul(z1[i], make_float2(p.x, -p.y));
+      if (i < NT - 1) p = cmul(p, w);
+    }
+    dft<NT, -1>(z0);
+    dft<NT, -1>(z1);
+  }
+}
+
 // Real-transform bins from Z[k] and Z[N - k]: X[k] = E + W_2N^k O and
 // X[N - k] = conj(E - W_2N^k O), E = (Z[k] + conj Z[N - k]) / 2,
 // O = -i (Z[k] - conj Z[N - k]) / 2. The factor 1/2 and the spectra's
@@ -1485,8 +1505,8 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
   __shared__ uint4 lds[4096];  // 64 KiB: the 16 tiles' operands, then V
   __shared__ float2 wt[N / 2];  // the window as FP32 pairs (cm_window_table)
   // per-thread constants (kept in LDS, not in registers held across the
-  // segment loop): W_N^q0, W_N^q1, W_2N^q0 and row 512's constants
-  __shared__ float2 cth[4][CM_T];
+  // segment loop): W_N^q0, W_2N^q0 and row 512's constants
+  __shared__ float2 cth[3][CM_T];
   __shared__ float4 ctw[8][64];  // per lane: W1024^(m3 col) of registers 2 p, 2 p + 1
   const DevStream &s = streams[blockIdx.y];
   const int64_t sa = (int64_t)blockIdx.x * segs_per_wg;
@@ -1523,19 +1543,18 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
       ctw[p][t] = make_float4(hs * a.x, hs * a.y, hs * b.x, hs * b.y);
     }
   }
-  // the last stage's rows: q0 = t and q1 = 1024 - t. Thread 0 takes row 0
-  // twice, the second time as "row 1024": its twiddles W_N^(1024 n1) =
-  // W_NT^n1 shift the NT-point result by one bin, so that the pairing of
-  // bins k1 and NT - 1 - k1 below pairs row 0's bins k1 and NT - k1 and
-  // yields bin N. Row 512 (its bins k1 and NT - 1 - k1 pair with each other)
-  // goes to lane 0 of wave 0 besides.
+  // the last stage's rows: q0 = t and q1 = 1024 - t (twiddles from those of
+  // q0, cm_last_stage2). Thread 0 takes row 0 twice, the second time as "row
+  // 1024", whose twiddles W_N^(1024 n1) = W_NT^n1 shift the NT-point result
+  // by one bin, so that the pairing below pairs row 0's bins k1 and NT - k1
+  // and yields bin N. Row 512 (its bins k1 and NT - 1 - k1 pair with each
+  // other) goes to lane 0 of wave 0 besides.
   {
-    const int q0 = t, q1 = 1024 - t;
+    const int q0 = t;
     cth[0][t] = cm_w(q0, N);
-    cth[1][t] = cm_w(q1, N);
-    cth[2][t] = cm_w(q0, 2 * N);
+    cth[1][t] = cm_w(q0, 2 * N);
     // row 512's constants: W_N^512 (last stage) and W_2N^512 (split)
-    cth[3][t] = t == 0 ? cm_w(512, N) : t == 1 ? cm_w(512, 2 * N) : make_float2(0.0f, 0.0f);
+    cth[2][t] = t == 0 ? cm_w(512, N) : t == 1 ? cm_w(512, 2 * N) : make_float2(0.0f, 0.0f);
   }
   const uint32_t flipmask = s.flip ? 0x80000000u : 0u;  // sign of the imaginary half
   // the real series as pairs of reals: float2 (FP32) or one 32-bit word of
@@ -1715,28 +1734,29 @@ __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restri
           for (int i = 0; i < 32; i++) asm volatile("" ::"v"(xp[i]));
         }
       }
-      const float2 wn0 = cth[0][tq], wn1 = cth[1][tq], w0 = cth[2][tq];
+      const float2 wn0 = cth[0][tq], w0 = cth[1][tq];
 #pragma unroll
       for (int sig = 0; sig < SG; sig++) {
         if (seg + sig >= sb) break;  // uniform
         float2 z0[NT], z1[NT];
         cm_read_rows<NT>(vlds, sig, r0, z0);
         cm_read_rows<NT>(vlds, sig, r1 & 1023, z1);
-        cm_last_stage<NT>(z0, wn0);
-        cm_last_stage<NT>(z1, wn1);
+        cm_last_stage2<NT>(z0, z1, wn0);
         uint32_t *row = reinterpret_cast<uint32_t *>(s.spectra) + (seg + sig) * job.spectra_stride;
 #pragma unroll
         for (int k1 = 0; k1 < NT; k1++) {
           float2 xk, xn;
-          cm_split(z0[k1], z1[NT - 1 - k1], cmul(w0, cm_w32((16 / NT) * k1)), xk, xn);
+          // Z[N - k] of bin k = q0 + 1024 k1: row q1, bin NT - 1 - k1, held
+          // in z1[(NT - k1) mod NT]
+          cm_split(z0[k1], z1[(NT - k1) & (NT - 1)], cmul(w0, cm_w32((16 / NT) * k1)), xk, xn);
           cm_put<SP>(row, (uint32_t)(r0 + 1024 * k1), xk);
           cm_put<SP>(row, (uint32_t)(r1 + 1024 * (NT - 1 - k1)), xn);
         }
         if (tq == 0) {
           // row 512: bins 512 + 1024 k1 and 512 + 1024 (NT - 1 - k1) pair
           cm_read_rows<NT>(vlds, sig, 512, z1);
-          cm_last_stage<NT>(z1, cth[3][0]);
-          const float2 w5 = cth[3][1];
+          cm_last_stage<NT>(z1, cth[2][0]);
+          const float2 w5 = cth[2][1];
 #pragma unroll
           for (int k1 = 0; k1 < (NT + 1) / 2; k1++) {
             float2 xk, xn;
