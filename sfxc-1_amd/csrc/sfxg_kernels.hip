@@ -3412,7 +3412,7 @@ __global__ __launch_bounds__(256) void k_pulsar_stage_masks(const int16_t *__res
 }
 
 template <int PREC, bool HS>
-__global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar_bin(
+__global__ __launch_bounds__(MF_THREADS, 4) void k_xstep_mfma_pulsar_bin(
     const void *__restrict__ spectra, int64_t stream_stride, int64_t seg_stride, int n_ch, int M, int RT, int CH,
     int64_t seg_base, int64_t n_segments, float scale, const int16_t *__restrict__ bins,
     const uint32_t *__restrict__ masks, int W, float *__restrict__ gram, int n_cb, int G) {
@@ -3439,6 +3439,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar_bin(
   uint32_t *cmask = reinterpret_cast<uint32_t *>(ylds + planes * plane_elems);  // [64]: the bin's segments per channel
   int32_t *list = reinterpret_cast<int32_t *>(cmask + 64);                      // [MFB_WIN]
   int32_t *wcnt = list + MFB_WIN;                                               // [MF_WAVES]
+  int16_t *sbin = reinterpret_cast<int16_t *>(wcnt + MF_WAVES);                 // [channel][16]: the stage's bins
   for (int i = threadIdx.x; i < planes * plane_elems; i += MF_THREADS) ylds[i] = (_Float16)0.0f;
   f16v_t acc[MFP_MAXP];
 #pragma unroll
@@ -3447,6 +3448,42 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar_bin(
     for (int e = 0; e < 16; e++) acc[q][e] = 0.0f;
   const int n_pairs = CH * nT;
   const int ko = 8 * (lane >> 5);
+  // packed spectra whose stage items fit one per thread: each thread's item
+  // (channel ic, segment group ikg, stream ist: 8 words) and one bin of the
+  // stage (segment bsl of channel bcc) are loaded one listed stage ahead,
+  // while the current stage is multiplied; otherwise a stage is loaded when
+  // it is reached
+  const int n_items = CH * 2 * M;
+  const bool pf = HS && n_items <= MF_THREADS && 16 * CH <= MF_THREADS;  // uniform
+  const int ic = threadIdx.x % CH, ikg = (threadIdx.x / CH) & 1, ist = threadIdx.x / CH / 2;
+  const int bcc = threadIdx.x >> 4, bsl = threadIdx.x & 15;
+  const uint32_t *sp32 = static_cast<const uint32_t *>(spectra);
+  uint32_t pw[8];
+  int pb = -1;
+  auto fetch = [&](int64_t g_rel) {
+    const bool it = (int)threadIdx.x < n_items && ch0 + ic < n_ch;
+#pragma unroll
+    for (int kk = 0; kk < 8; kk++) {
+      const int64_t seg = g_rel * MF_KS + ikg * 8 + kk;
+      pw[kk] = it && seg < n_segments ? sp32[ist * stream_stride + seg * seg_stride + ch0 + ic] : 0u;
+    }
+    const int64_t sb = g_rel * MF_KS + bsl;
+    pb = (int)threadIdx.x < 16 * CH && sb < n_segments && ch0 + bcc < n_ch ? bins[sb * n_ch + ch0 + bcc] : -1;
+  };
+  auto put = [&]() {
+    if ((int)threadIdx.x < n_items) {
+      typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
+      u4_t r, i;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        r[e] = (pw[2 * e] & 0xffffu) | (pw[2 * e + 1] << 16);
+        i[e] = (pw[2 * e] >> 16) | (pw[2 * e + 1] & 0xffff0000u);
+      }
+      *reinterpret_cast<u4_t *>(ylds + ic * chan_stride + ist * MF_RS + ikg * 8) = r;
+      *reinterpret_cast<u4_t *>(ylds + ic * chan_stride + (M + ist) * MF_RS + ikg * 8) = i;
+    }
+    if ((int)threadIdx.x < 16 * CH) sbin[bcc * 16 + bsl] = (int16_t)pb;
+  };
   for (int64_t w0 = g_lo; w0 < g_hi; w0 += MFB_WIN) {
     // this window's stages holding the bin, in order
     const int64_t g = w0 + threadIdx.x;
@@ -3463,22 +3500,28 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar_bin(
     }
     if (has) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)(g - w0);
     __syncthreads();
+    if (pf && total > 0) fetch(w0 + list[0]);
     for (int e = 0; e < total; e++) {
       const int64_t g_rel = w0 + list[e];
-      mf_load_stage<PREC, MF_KS, HS>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0,
-                                     seg_base + g_rel * MF_KS, seg_base + n_segments, scale, chan_stride,
-                                     plane_elems, ylds);
-      if ((int)threadIdx.x < CH) {
-        uint32_t mk = 0u;
-        const int ch = ch0 + threadIdx.x;
-        if (ch < n_ch) {
+      if (pf) {
+        put();
+        if (e + 1 < total) fetch(w0 + list[e + 1]);
+      } else {
+        mf_load_stage<PREC, MF_KS, HS>(spectra, stream_stride, seg_stride, seg_base, n_ch, M, CH, ch0,
+                                       seg_base + g_rel * MF_KS, seg_base + n_segments, scale, chan_stride,
+                                       plane_elems, ylds);
+        if ((int)threadIdx.x < CH) {
+          uint32_t mk = 0u;
+          const int ch = ch0 + threadIdx.x;
+          if (ch < n_ch) {
 #pragma unroll
-          for (int k = 0; k < MF_KS; k++) {
-            const int64_t seg = g_rel * MF_KS + k;
-            if (seg < n_segments && bins[seg * n_ch + ch] == b) mk |= 1u << k;
+            for (int k = 0; k < MF_KS; k++) {
+              const int64_t seg = g_rel * MF_KS + k;
+              if (seg < n_segments && bins[seg * n_ch + ch] == b) mk |= 1u << k;
+            }
           }
+          cmask[threadIdx.x] = mk;
         }
-        cmask[threadIdx.x] = mk;
       }
       __syncthreads();
 #pragma unroll
@@ -3487,7 +3530,19 @@ __global__ __launch_bounds__(MF_THREADS) void k_xstep_mfma_pulsar_bin(
         if (p < n_pairs) {
           int c, t, ti, tj;
           mf_pair(p, nT, RT, c, t, ti, tj);
-          const uint32_t mk = cmask[c];
+          uint32_t mk = 0u;
+          if (pf) {  // the channel's 16 bins of the stage (wave-uniform reads)
+            typedef int16_t s8_t __attribute__((ext_vector_type(8)));
+            const s8_t lo8 = *reinterpret_cast<const s8_t *>(sbin + c * 16);
+            const s8_t hi8 = *reinterpret_cast<const s8_t *>(sbin + c * 16 + 8);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+              mk |= (lo8[k] == b ? 1u : 0u) << k;
+              mk |= (hi8[k] == b ? 1u : 0u) << (k + 8);
+            }
+          } else {
+            mk = cmask[c];
+          }
           if (mk != 0u) {
             const int oa = c * chan_stride + (ti * 32 + (lane & 31)) * MF_RS + ko;
             const int ob = c * chan_stride + (tj * 32 + (lane & 31)) * MF_RS + ko;
@@ -4329,7 +4384,7 @@ static hipError_t launch_mfma_pulsar_t(dim3 grid, size_t lds, hipStream_t s, con
 }
 
 // LDS beyond the stage planes of the by-bin kernel: channel masks, stage list, wave counts
-constexpr size_t MFB_EXTRA = 64 * sizeof(uint32_t) + (MFB_WIN + MF_WAVES) * sizeof(int32_t);
+constexpr size_t MFB_EXTRA = 64 * sizeof(uint32_t) + (MFB_WIN + MF_WAVES) * sizeof(int32_t) + 64 * 16 * sizeof(int16_t);
 
 size_t pulsar_bin_mask_words(int precision, int M, int n_ch_run, int nbins, int64_t n_segments) {
   size_t lds = 0;
