@@ -630,8 +630,8 @@ def summarise(wl, wall, kt, precision="fp32", steps=None):
     nfold = max(1, round(job.integration_time / job.sub_integration_time))
     fused = kt[2] == 0.0
     packed = precision in ("fp16", "bf16")
-    # packed spectra at 1024, 4096 and 16384 channels: MFMA channeliser fed an FP16 series
-    ts_bytes = 2 if packed and not fused and job.fft_size_correlation in (1024, 4096, 16384) else 4
+    # packed spectra at 1024 to 16384 channels: MFMA channeliser fed an FP16 series
+    ts_bytes = 2 if packed and not fused and job.fft_size_correlation in (1024, 2048, 4096, 8192, 16384) else 4
     kb = kernel_bytes(job, wl.M, nseg, wl.nsamp, nfold, B, fused, 4 if packed else 8, ts_bytes)
     value = steps * wl.station_samples_per_step / wall / 1e9
     xflops = 8.0 * B * (job.fft_size_correlation + 1) * nseg

@@ -51,10 +51,10 @@ extern "C" {
  * per-channel Hermitian rank-k update on MFMA with FP32 accumulation:
  * FP16 or BF16 operands, or FP16 operands split into hi + lo parts (three
  * products). Pulsar binning takes every precision (per-bin Gram tiles on MFMA).
- * With FP16 or BF16 at fft_size_correlation 1024, 4096 or 16384 the
+ * With FP16 or BF16 at fft_size_correlation 1024, 2048, 4096, 8192 or 16384 the
  * channeliser's transform also runs on MFMA with FP16 operands and FP32
  * accumulation, fed an FP16 real series (unless the tuning table's chan_fp32
- * selects the FP32 transform, fused with the delay stage at 1024 and 4096);
+ * selects the FP32 transform, fused with the delay stage up to 4096);
  * the visibility error stays within the FP16 bar of 1e-3 relative to the
  * FP64 pipeline. */
 #define SFXG_PREC_FP32 0
@@ -416,11 +416,10 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap);
  * the register tiles on 64- and 16-channel blocks, 4 and 8 the per-baseline
  * kernel; with pulsar binning in FP32, 2 forces the register tiles and 1, 4
  * or 8 the per-baseline kernel; chan_fp32 1 gives packed FP16 / BF16 spectra
- * at 1024, 4096 and 16384 channels the FP32 transform instead of the MFMA
- * one (the fused F-engine at 1024 and 4096, whose threads fe_threads sets;
- * at 16384 chan_wgs sets either channeliser's workgroups); *written = bytes
- * needed including the
- * terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear
+ * at 1024 to 16384 channels the FP32 transform instead of the MFMA
+ * one (the fused F-engine up to 4096, whose threads fe_threads sets;
+ * at 8192 and 16384 chan_wgs sets either channeliser's workgroups);
+ * *written = bytes needed including the terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear
  * empties the table. */
 int sfxg_tune_export(char *dst, size_t cap, size_t *written);
 int sfxg_tune_import(const char *text);

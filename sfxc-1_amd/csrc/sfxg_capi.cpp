@@ -2070,11 +2070,12 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
   char line[160];
   Tune best = ctx->tune;
   // F-engine: threads per workgroup (N_c <= 4096), or the channeliser's
-  // workgroups. Packed FP16 / BF16 spectra at N_c 1024, 4096 and 16384 take
+  // workgroups. Packed FP16 / BF16 spectra at N_c 1024 to 16384 take
   // the MFMA channeliser, fixed by the prepared plan with its FP16 series
   // (chan_fp32 = 1 in the table selects the FP32 transform, and at N_c <=
   // 4096 the fused F-engine, at the next preparation); with an FP32 series
-  // (keep_time_series) the FP32 transform at 16384 is among the candidates
+  // (keep_time_series) the FP32 transform at 8192 and 16384 is among the
+  // candidates
   const bool small = nc <= 4096;
   const bool mf = chan_mfma_supported(nc, ctx->opt.precision);
   const bool fe = lp.fused || (small && !mf);  // candidates are F-engine thread counts

@@ -1254,7 +1254,7 @@ __global__ __launch_bounds__(N / P) void k_channelise_run(const DevStream *__res
 
 // ---------------------------------------------------------------------------
 // k_chan_mfma: the channeliser of the mixed-precision X-steps (spectra
-// stored as packed FP16 or BF16) for N_c = 1024 NT (NT = 1, 4, 16), its
+// stored as packed FP16 or BF16) for N_c = 1024 NT (NT = 1, 2, 4, 8, 16), its
 // transform on MFMA. The 2 N_c windowed reals of a segment are N = N_c
 // complex points z[n],
 //   n = NT (32 m2 + m3) + n1,  q = k2 + 32 k3,  k = q + 1024 k1:
@@ -1499,7 +1499,7 @@ template <int SP, int NT, bool H>
 __global__ __launch_bounds__(CM_T, 2) void k_chan_mfma(const DevStream *__restrict__ streams, DevJob job,
                                                       int segs_per_wg) {
   static_assert(SP == SFXG_PREC_FP16 || SP == SFXG_PREC_BF16, "packed FP16 / BF16 spectra");
-  static_assert(NT == 1 || NT == 4 || NT == 16, "1024, 4096 or 16384 complex points");
+  static_assert(NT == 1 || NT == 2 || NT == 4 || NT == 8 || NT == 16, "1024 NT complex points, NT <= 16");
   constexpr int N = 1024 * NT;  // complex points of a segment (N_c)
   constexpr int SG = 16 / NT;   // segments per pass (16 tiles)
   __shared__ uint4 lds[4096];  // 64 KiB: the 16 tiles' operands, then V
@@ -3787,7 +3787,7 @@ static int device_cus() {
   return n;
 }
 
-// the MFMA channeliser (N_c = 1024, 4096, 16384, packed FP16 / BF16
+// the MFMA channeliser (N_c = 1024, 2048, 4096, 8192, 16384, packed FP16 / BF16
 // spectra): each workgroup a run of consecutive segments of one stream, the
 // same number of workgroups per stream, at most chan_wgs in all. One
 // workgroup fits a CU (its LDS), so by default chan_wgs is 2, 4 or 1 times
@@ -3824,14 +3824,17 @@ template <int SP>
 static hipError_t launch_chan_mfma(const DevStream *d_streams, const DevJob &job, const Tune &t, hipStream_t s) {
   switch (job.nc) {
     case 1024: return launch_chan_mfma_nt<SP, 1>(d_streams, job, t, s);
+    case 2048: return launch_chan_mfma_nt<SP, 2>(d_streams, job, t, s);
     case 4096: return launch_chan_mfma_nt<SP, 4>(d_streams, job, t, s);
+    case 8192: return launch_chan_mfma_nt<SP, 8>(d_streams, job, t, s);
     case 16384: return launch_chan_mfma_nt<SP, 16>(d_streams, job, t, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 bool chan_mfma_supported(int nc, int spec_prec) {
-  return (nc == 1024 || nc == 4096 || nc == 16384) && (spec_prec == SFXG_PREC_FP16 || spec_prec == SFXG_PREC_BF16);
+  return (nc == 1024 || nc == 2048 || nc == 4096 || nc == 8192 || nc == 16384) &&
+         (spec_prec == SFXG_PREC_FP16 || spec_prec == SFXG_PREC_BF16);
 }
 
 size_t channelise_scratch_elems(int nc) { return nc == LG_N ? (size_t)2 * 1024 * LG_N : 0; }

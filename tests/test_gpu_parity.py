@@ -547,10 +547,10 @@ def test_packed_spectra(ctx, nchan, prec, keep_ts):
     job, hdrs = configs.build_slices(seed=42, n_stations=3, pols=["R"], fs=32_000_000, bw=16_000_000, nchan=nchan,
                                      seconds=0.125, t_int_usec=125000, rho=0.05, bits=2, sky_freq=1_400_000_000,
                                      offsets=[0.0, 2e-6, -1e-6], rates=[0.0, 1e-7, -2e-7])[0]
-    # packed FP16 / BF16 at 1024, 4096 and 16384 channels take the FP32
-    # transform here (the MFMA one has its own test, test_mfma_channeliser)
+    # packed FP16 / BF16 at 1024 to 16384 channels take the FP32 transform
+    # here (the MFMA one has its own test, test_mfma_channeliser)
     capi.tune_clear()
-    if nchan in (1024, 4096, 16384) and prec in (1, 2):
+    if nchan in (1024, 2048, 4096, 8192, 16384) and prec in (1, 2):
         capi.tune_import(capi.tune_key(nchan, 3, prec, False, nchan <= 4096 and not keep_ts) + "|0|0|0|0|0|0|1\n")
     try:
         capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=0, keep_time_series=keep_ts)
@@ -578,11 +578,13 @@ def test_packed_spectra(ctx, nchan, prec, keep_ts):
 
 
 @pytest.mark.parametrize("nchan,prec,window,sideband", [(16384, 1, 4, "U"), (16384, 2, 4, "U"), (16384, 1, 0, "U"),
-                                                        (16384, 1, 3, "L"), (4096, 1, 4, "U"), (4096, 2, 2, "L"),
-                                                        (1024, 1, 4, "U"), (1024, 1, 1, "U"), (1024, 2, 0, "L")])
+                                                        (16384, 1, 3, "L"), (8192, 1, 4, "U"), (8192, 2, 1, "L"),
+                                                        (4096, 1, 4, "U"), (4096, 2, 2, "L"), (2048, 1, 3, "U"),
+                                                        (2048, 1, 0, "L"), (1024, 1, 4, "U"), (1024, 1, 1, "U"),
+                                                        (1024, 2, 0, "L")])
 def test_mfma_channeliser(ctx, nchan, prec, window, sideband):
-    """Packed FP16 / BF16 spectra at 1024, 4096 and 16384 channels come from
-    the MFMA channeliser (after the unfused delay stage at 1024 and 4096): its
+    """Packed FP16 / BF16 spectra at 1024 to 16384 channels come from the
+    MFMA channeliser (after the unfused delay stage up to 4096): its
     transform of N = 1024 NT points takes NT tiles of 32 x 32 on MFMA with
     FP16 operands and FP32 accumulation (the windowed points, the first
     stage's results and the second stage's results are each rounded to FP16
