@@ -52,7 +52,7 @@ MEASURED_TRAFFIC = {
 # independent VALU instructions per quad-cycle (SQ_ACTIVE_INST_VALU2 counts
 # those), so 256 CUs x 4 SIMDs x 2 / 4 cycles x 2.4 GHz.
 MEASURED_VALU_INSTS = {
-    ("channelise", 16384, 64, "fp16"): 1.740e9,
+    ("channelise", 16384, 64, "fp16"): 1.666e9,
 }
 VALU_ISSUE_PEAK = 256 * 4 * 2 / 4 * 2.4e9
 KERNELS = ["stats", "fengine", "channelise", "xstep", "normalise"]
