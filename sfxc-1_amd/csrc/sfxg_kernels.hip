@@ -3178,6 +3178,28 @@ __global__ __launch_bounds__(64 * NW) void k_xstep_pulsar_bin(
   sums(false);
   const int64_t n_stages = (n_segments + MF_KS - 1) / MF_KS;
   const int row_items = M * 8;
+  // stage item idx: segment sl, stream st, channels ch0 + 2 q, + 1 (16 bytes)
+  auto item = [&](int64_t seg0, int idx) {
+    const int sl = idx / row_items, r = idx - sl * row_items;
+    const int st = r >> 3, q = r & 7;
+    float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (seg0 + sl < n_segments)
+      x = *reinterpret_cast<const float4 *>(spectra + st * stream_stride + (seg0 + sl) * seg_stride + ch0 + 2 * q);
+    return x;
+  };
+  constexpr int PFN = 8;  // items per thread loaded one stage ahead
+  float4 pf[PFN];
+  int pbin = -1;
+  auto fetch = [&](int64_t seg0) {
+#pragma unroll
+    for (int k = 0; k < PFN; k++) {
+      const int idx = threadIdx.x + k * NT;
+      pf[k] = idx < MF_KS * row_items ? item(seg0, idx) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    const int sl = threadIdx.x >> 4, cc = threadIdx.x & 15;
+    pbin = (int)threadIdx.x < MF_KS * 16 && seg0 + sl < n_segments && ch0 + cc < n_ch
+               ? bins[(seg0 + sl) * n_ch + ch0 + cc] : -1;
+  };
   for (int64_t w0 = 0; w0 < n_stages; w0 += NT) {
     const int64_t g = w0 + threadIdx.x;
     const bool has = g < n_stages && ((masks[(g * n_cb + cb) * W + (b >> 5)] >> (b & 31)) & 1u);
@@ -3193,20 +3215,19 @@ __global__ __launch_bounds__(64 * NW) void k_xstep_pulsar_bin(
     }
     if (has) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)(g - w0);
     __syncthreads();
+    if (total > 0) fetch((w0 + list[0]) * MF_KS);
     for (int e = 0; e < total; e++) {
       const int64_t seg0 = (w0 + list[e]) * MF_KS;
-      for (int idx = threadIdx.x; idx < MF_KS * row_items; idx += NT) {
-        const int sl = idx / row_items, r = idx - sl * row_items;
-        const int st = r >> 3, q = r & 7;
-        float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (seg0 + sl < n_segments)
-          x = *reinterpret_cast<const float4 *>(spectra + st * stream_stride + (seg0 + sl) * seg_stride + ch0 + 2 * q);
-        xq[idx] = x;
+      // this stage's items (the first PFN per thread loaded one listed stage
+      // ahead, the rest now) and bins into LDS, then the next stage's loads
+#pragma unroll
+      for (int k = 0; k < PFN; k++) {
+        const int idx = threadIdx.x + k * NT;
+        if (idx < MF_KS * row_items) xq[idx] = pf[k];
       }
-      if ((int)threadIdx.x < MF_KS * 16) {
-        const int sl = threadIdx.x >> 4, cc = threadIdx.x & 15;
-        sb[threadIdx.x] = seg0 + sl < n_segments && ch0 + cc < n_ch ? bins[(seg0 + sl) * n_ch + ch0 + cc] : -1;
-      }
+      for (int idx = threadIdx.x + PFN * NT; idx < MF_KS * row_items; idx += NT) xq[idx] = item(seg0, idx);
+      if ((int)threadIdx.x < MF_KS * 16) sb[threadIdx.x] = pbin;
+      if (e + 1 < total) fetch((w0 + list[e + 1]) * MF_KS);
       __syncthreads();
 #pragma unroll 1
       for (int sl = 0; sl < MF_KS; sl++) {
