@@ -54,7 +54,7 @@ extern "C" {
  * With FP16 or BF16 at fft_size_correlation 1024, 2048, 4096, 8192 or 16384 the
  * channeliser's transform also runs on MFMA with FP16 operands and FP32
  * accumulation, fed an FP16 real series (unless the tuning table's chan_fp32
- * selects the FP32 transform, fused with the delay stage up to 4096);
+ * selects the FP32 transform);
  * the visibility error stays within the FP16 bar of 1e-3 relative to the
  * FP64 pipeline. */
 #define SFXG_PREC_FP32 0
@@ -158,13 +158,15 @@ typedef struct {
   int32_t precision;  /* SFXG_PREC_* */
   /* Nonzero: run delay correction and the channeliser as two kernels and
    * keep the FP32 real series in HBM for SFXG_STAGE_TIME_SERIES
-   * (verification). Zero (default): for N_c <= 4096 they run fused and the
-   * series stays in LDS, except with packed FP16 / BF16 spectra at N_c 1024
-   * and 4096, which take the MFMA channeliser; that channeliser (N_c 1024,
-   * 4096, 16384) gets the series as FP16 (it rounds its points to FP16
-   * anyway). FP32 and split FP16 give identical results either way; packed
-   * FP16 / BF16 agree within their rounding (the FP16 series' phasors come
-   * from the hardware sine and cosine). */
+   * (verification). Zero (default): the two kernels too, unless the tuning
+   * table picks the fused F-engine (fe_fused, N_c <= 4096, FP32 and split
+   * FP16), whose series stays in LDS; the MFMA channeliser of packed FP16 /
+   * BF16 spectra (N_c 1024 to 16384) gets the series as FP16 (it rounds its
+   * points to FP16 anyway). FP32 and split FP16 give identical results
+   * either way on the same route; the fused F-engine differs from the two
+   * kernels by FP32 rounding (another radix schedule); packed FP16 / BF16
+   * agree within their rounding (the FP16 series' phasors come from the
+   * hardware sine and cosine). */
   int32_t keep_time_series;
   /* Spectra budget in MiB of one chunk: a slice is processed in chunks of
    * whole sub-integrations whose spectra fit it (at least one
@@ -397,7 +399,8 @@ int sfxg_chunk_plan(const sfxg_ctx *ctx, int32_t *n_chunks, int64_t *max_segment
 
 /* Autotuner of launch shapes: on a prepared context (sfxg_prepare), times the
  * candidate FFT schedules of the F-engine (threads, i.e. points per thread and
- * radix stages, for N_c <= 4096; workgroup count of the large-N channeliser)
+ * radix stages, for N_c <= 4096, on the fused F-engine and on the delay stage
+ * plus channeliser; workgroup count of the large-N channeliser)
  * and the X-step tile shapes (FP32 X-step: register tiles on 16- or 64-channel
  * blocks, or the per-baseline kernel and its waves per workgroup; channels
  * per workgroup of the MFMA X-step, workgroups of the MFMA pulsar X-step) on
@@ -411,14 +414,17 @@ int sfxg_chunk_plan(const sfxg_ctx *ctx, int32_t *n_chunks, int64_t *max_segment
  * rounding of the FFT schedule. */
 int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap);
 /* The tuning table as text, one "shape|fe_threads|chan_wgs|xs_waves|mf_channels|
- * mfp_wgs|mf_ks|chan_fp32" line per shape (0: default; fields missing at the
+ * mfp_wgs|mf_ks|chan_fp32|fe_fused" line per shape (0: default; fields missing at the
  * end of a line (at least five given) are read as 0; xs_waves 1 and 2 force
  * the register tiles on 64- and 16-channel blocks, 4 and 8 the per-baseline
  * kernel; with pulsar binning in FP32, 2 forces the register tiles and 1, 4
  * or 8 the per-baseline kernel; chan_fp32 1 gives packed FP16 / BF16 spectra
  * at 1024 to 16384 channels the FP32 transform instead of the MFMA
- * one (the fused F-engine up to 4096, whose threads fe_threads sets;
- * at 8192 and 16384 chan_wgs sets either channeliser's workgroups);
+ * one (up to 4096 its threads fe_threads sets; at 8192 and 16384
+ * chan_wgs sets either channeliser's workgroups); fe_fused 1 runs the
+ * fused F-engine up to 4096 channels when the series is not kept (FP32
+ * spectra: FP32, split FP16, or chan_fp32 1), 0 the delay stage and the
+ * channeliser as two kernels;
  * *written = bytes needed including the terminating zero. sfxg_tune_import adds such lines; sfxg_tune_clear
  * empties the table. */
 int sfxg_tune_export(char *dst, size_t cap, size_t *written);

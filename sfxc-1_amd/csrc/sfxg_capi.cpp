@@ -94,6 +94,7 @@ struct LaunchPlan {
   int64_t chunk_segs = 0;   // spectra rows per stream (largest chunk)
   int64_t ts_stride = 0;    // real-series floats per stream (largest chunk)
   bool fused = false, mixed = false;
+  bool fused_ok = false;    // the fused F-engine can run this plan (FP32 series, not kept)
   bool key_fused = false;   // the shape's tuning key: N_c within the fused F-engine's range
   int k_group = 1, psr_run = 0;
   size_t vis_elems = 0;
@@ -1057,9 +1058,11 @@ static int prepare_slice(sfxg_ctx *ctx) {
   HIPCHK(ctx, ctx->block_consts.ensure((size_t)M * ctx->n_blocks * sizeof(BlockConst)));
   HIPCHK(ctx, ctx->model_buf.ensure((size_t)mod_tot * sizeof(double)));
   HIPCHK(ctx, ctx->dstreams.ensure((size_t)M * sizeof(DevStream)));
-  // the fused F-engine for N_c <= 4096, unless packed FP16 / BF16 spectra
-  // take the MFMA channeliser (delay stage and channeliser unfused; the
-  // tuning table's chan_fp32 = 1 keeps the fused one)
+  // N_c <= 4096 without keep_time_series may take the fused F-engine when
+  // the tuning table picks it (fe_fused = 1), unless packed FP16 / BF16
+  // spectra take the MFMA channeliser; by default the delay stage and the
+  // channeliser run as two kernels, which measured faster at 1024 and 4096
+  // channels (the table's chan_fp32 = 1 keeps the FP32 transform)
   const bool key_fused = !ctx->opt.keep_time_series && fengine_supported(nc);
   {
     const std::string key = tune_key(ctx, key_fused);
@@ -1070,7 +1073,8 @@ static int prepare_slice(sfxg_ctx *ctx) {
   const int packed = ctx->opt.precision == SFXG_PREC_FP16 || ctx->opt.precision == SFXG_PREC_BF16
                          ? ctx->opt.precision : 0;
   const bool mfma_chan = chan_mfma_supported(nc, packed) && ctx->tune.chan_fp32 == 0;
-  const bool fused = key_fused && !mfma_chan;
+  const bool fused_ok = key_fused && !mfma_chan;
+  const bool fused = fused_ok && ctx->tune.fe_fused == 1;
   // the MFMA channeliser rounds its input points to FP16 anyway: the delay
   // stage hands it an FP16 series (half the bytes each way), unless the
   // series is kept for reading back
@@ -1301,6 +1305,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
   }
   // launch parameters of the device work
   lp.fused = fused;
+  lp.fused_ok = fused_ok;
   lp.key_fused = key_fused;
   lp.mixed = mixed;
   lp.k_group = k_group;
@@ -1391,7 +1396,9 @@ static int enqueue_fengine(sfxg_ctx *ctx, size_t c, hipStream_t s, bool timing, 
   cj.blk_end = ch.blk1;
   const DevStream *dstr = ctx->dstreams.as<DevStream>();
   if (c == 0) HIPCHK(ctx, launch_block_consts(dstr, cj, s));
-  if (lp.fused) {
+  // a plan without the real series (lp.fused) runs the fused F-engine; one
+  // with it runs whichever route t names (the autotuner times both)
+  if (lp.fused || (lp.fused_ok && t.fe_fused == 1)) {
     HIPCHK(ctx, launch_fengine(dstr, cj, ctx->window.as<float>(), ctx->tw_nd.as<float2>(), ctx->tw_split.as<float2>(),
                                t, s));
     HIPCHK(ctx, timing ? hipEventRecord(ctx->cev[4 * c + 1], s) : hipSuccess);
@@ -2069,39 +2076,46 @@ int sfxg_autotune(sfxg_ctx *ctx, int32_t reps, char *report, size_t cap) {
   std::string rep;
   char line[160];
   Tune best = ctx->tune;
-  // F-engine: threads per workgroup (N_c <= 4096), or the channeliser's
-  // workgroups. Packed FP16 / BF16 spectra at N_c 1024 to 16384 take
-  // the MFMA channeliser, fixed by the prepared plan with its FP16 series
-  // (chan_fp32 = 1 in the table selects the FP32 transform, and at N_c <=
-  // 4096 the fused F-engine, at the next preparation); with an FP32 series
-  // (keep_time_series) the FP32 transform at 8192 and 16384 is among the
-  // candidates
+  // F-engine: threads per workgroup (N_c <= 4096) on the fused F-engine or
+  // the two-kernel route, or the channeliser's workgroups. Packed FP16 /
+  // BF16 spectra at N_c 1024 to 16384 take the MFMA channeliser, fixed by
+  // the prepared plan with its FP16 series (chan_fp32 = 1 in the table
+  // selects the FP32 transform at the next preparation); with an FP32
+  // series (keep_time_series) the FP32 transform at 8192 and 16384 is
+  // among the candidates. The two-kernel route needs the real series in
+  // HBM: not among the candidates of a plan prepared for the fused one.
   const bool small = nc <= 4096;
   const bool mf = chan_mfma_supported(nc, ctx->opt.precision);
-  const bool fe = lp.fused || (small && !mf);  // candidates are F-engine thread counts
-  std::vector<std::pair<int, int>> fc;          // (chan_fp32, threads or workgroups)
+  const bool fe = lp.fused_ok || (small && !mf);  // candidates are F-engine thread counts
+  struct Cand {
+    int fp32, fused, v;  // chan_fp32, fe_fused, threads or workgroups
+  };
+  std::vector<Cand> fc;
   if (fe) {
-    for (int v : fe_thread_candidates(nc)) fc.push_back({ctx->tune.chan_fp32, v});
+    for (int f = lp.fused ? 1 : 0; f <= (lp.fused_ok ? 1 : 0); f++)
+      for (int v : fe_thread_candidates(nc)) fc.push_back({ctx->tune.chan_fp32, f, v});
   } else {
     if (mf)
-      for (int v : chan_mfma_wgs_candidates()) fc.push_back({0, v});
+      for (int v : chan_mfma_wgs_candidates()) fc.push_back({0, 0, v});
     // (the FP32 transform needs an FP32 series: not among the candidates
     // when the MFMA channeliser's FP16 one was prepared)
     if (!small && !lp.job.ts_half)
-      for (int v : chan_wgs_candidates(nc)) fc.push_back({mf ? 1 : 0, v});
+      for (int v : chan_wgs_candidates(nc)) fc.push_back({mf ? 1 : 0, 0, v});
   }
   if (fc.size() > 1) {
     float best_ms = -1.0f;
     Tune pick = best;
     for (const auto &c : fc) {
       Tune t = best;
-      (fe ? t.fe_threads : t.chan_wgs) = c.second;
-      t.chan_fp32 = c.first;
+      (fe ? t.fe_threads : t.chan_wgs) = c.v;
+      t.chan_fp32 = c.fp32;
+      if (fe) t.fe_fused = c.fused;
       float ms = 0.0f;
       const int rc = time_stage(ctx, reps, [&] { return enqueue_fengine(ctx, 0, ctx->stream, false, t); }, &ms);
       if (rc != SFXG_OK) return rc;
-      snprintf(line, sizeof(line), "%s%s=%d: %.3f ms\n", fe ? "fe_threads" : "chan_wgs",
-               !fe && mf ? (c.first ? " (FP32 transform)" : " (MFMA)") : "", c.second, ms);
+      const char *route = !fe ? (mf ? (c.fp32 ? " (FP32 transform)" : " (MFMA)") : "")
+                              : lp.fused_ok ? (c.fused ? " (fused)" : " (delay + channeliser)") : "";
+      snprintf(line, sizeof(line), "%s%s=%d: %.3f ms\n", fe ? "fe_threads" : "chan_wgs", route, c.v, ms);
       rep += line;
       if (best_ms < 0 || ms < best_ms) {
         best_ms = ms;
@@ -2175,8 +2189,8 @@ int sfxg_tune_export(char *dst, size_t cap, size_t *written) {
     char line[256];
     for (const auto &kv : g_tune) {
       const Tune &t = kv.second;
-      snprintf(line, sizeof(line), "%s|%d|%d|%d|%d|%d|%d|%d\n", kv.first.c_str(), t.fe_threads, t.chan_wgs,
-               t.xs_waves, t.mf_channels, t.mfp_wgs, t.mf_ks, t.chan_fp32);
+      snprintf(line, sizeof(line), "%s|%d|%d|%d|%d|%d|%d|%d|%d\n", kv.first.c_str(), t.fe_threads, t.chan_wgs,
+               t.xs_waves, t.mf_channels, t.mfp_wgs, t.mf_ks, t.chan_fp32, t.fe_fused);
       out += line;
     }
   }
@@ -2198,8 +2212,8 @@ int sfxg_tune_import(const char *text) {
     const size_t bar = ln.find('|');
     if (bar == std::string::npos) return SFXG_ERR_ARG;
     Tune t;
-    const int nf = sscanf(ln.c_str() + bar + 1, "%d|%d|%d|%d|%d|%d|%d", &t.fe_threads, &t.chan_wgs, &t.xs_waves,
-                          &t.mf_channels, &t.mfp_wgs, &t.mf_ks, &t.chan_fp32);
+    const int nf = sscanf(ln.c_str() + bar + 1, "%d|%d|%d|%d|%d|%d|%d|%d", &t.fe_threads, &t.chan_wgs,
+                          &t.xs_waves, &t.mf_channels, &t.mfp_wgs, &t.mf_ks, &t.chan_fp32, &t.fe_fused);
     if (nf < 5) return SFXG_ERR_ARG;
     add[ln.substr(0, bar)] = t;
   }

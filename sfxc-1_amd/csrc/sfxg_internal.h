@@ -226,6 +226,9 @@ struct Tune {
   int mfp_wgs = 0;      // MFMA pulsar X-step: 0 by bin, else the time-sliced kernel's workgroups
   int chan_fp32 = 0;    // packed FP16 / BF16 spectra at N_c = 16384: 0 transform on MFMA (FP16
                         // operands), 1 the FP32 transform of the other precisions
+  int fe_fused = 0;     // N_c <= 4096 with an FP32 real series that is not kept: 1 the fused
+                        // F-engine (series in LDS), 0 the delay stage and the channeliser as
+                        // two kernels (series through HBM)
 };
 // Candidates of each knob for a shape (the default first).
 std::vector<int> fe_thread_candidates(int nc);

@@ -116,15 +116,28 @@ def test_c0_slice_stages(ctx, c0_short, lut):
 
 
 def test_fused_matches_unfused(ctx, c0_short):
-    """The fused F-engine (series in LDS) and the two-kernel path agree to
-    FP32 rounding: they share the channeliser's FFT schedule, and the
-    unfused delay correction factors its 256-point transforms as 16 x 16
-    where the fused one uses four radix-4 stages. Headers are identical."""
+    """The fused F-engine (series in LDS, picked by the tuning table's
+    fe_fused) and the two-kernel path agree to FP32 rounding: they share the
+    channeliser's FFT schedule, and the unfused delay correction factors its
+    256-point transforms as 16 x 16 where the fused one uses four radix-4
+    stages. Headers are identical. Without the table's pick the series is
+    not kept and the two kernels run: the same bytes as keep_time_series."""
     job, hdrs = c0_short[0]
+    nc = job.fft_size_correlation
+    M = len(job.station_streams)
     a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=True)
     sp_a = ctx.stage(capi.STAGE_SPECTRA, 1)
-    b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=False)
-    sp_b = ctx.stage(capi.STAGE_SPECTRA, 1)
+    capi.tune_clear()
+    try:
+        d = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=False)
+        capi.tune_import(capi.tune_key(nc, M, 0, False, True) + "|0|0|0|0|0|0|0|1\n")
+        b = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=False)
+        sp_b = ctx.stage(capi.STAGE_SPECTRA, 1)
+        with pytest.raises(capi.SfxgError):
+            ctx.stage(capi.STAGE_TIME_SERIES, 1)  # the fused F-engine kept the series in LDS
+    finally:
+        capi.tune_clear()
+    assert d == a
     err = np.max(np.abs(sp_a - sp_b)) / np.sqrt(np.mean(np.abs(sp_b.astype(np.complex128)) ** 2))
     print("fused vs unfused spectra", err)
     assert err < 1e-5  # each path is within 2e-5 of the oracle (test_c0_slice_stages); they differ by ~2e-6
@@ -916,7 +929,7 @@ def test_tuned_fused_matches_unfused(ctx):
                                      seconds=0.0625, t_int_usec=62500, rho=0.05, bits=2, sky_freq=5_000_000_000,
                                      offsets=[0.0, 1e-6], rates=[0.0, 1e-7])[0]
     capi.tune_clear()
-    capi.tune_import(capi.tune_key(1024, 2, 0, False, True) + "|128|0|0|0|0\n" +
+    capi.tune_import(capi.tune_key(1024, 2, 0, False, True) + "|128|0|0|0|0|0|0|1\n" +
                      capi.tune_key(1024, 2, 0, False, False) + "|128|0|0|0|0\n")
     try:
         a = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, keep_time_series=True)
@@ -945,7 +958,9 @@ def test_autotune_then_correlate(ctx):
         ctx.prepare()
         report = ctx.autotune(reps=2)
         print(report)
-        assert "fe_threads=512" in report and "fe_threads=256" in report and "xs_waves=4" in report
+        for route in ("(delay + channeliser)", "(fused)"):  # both F-engine routes, two thread counts each
+            assert f"fe_threads {route}=512" in report and f"fe_threads {route}=256" in report
+        assert "xs_waves=4" in report
         assert capi.tune_key(2048, 3, 0, False, True) in capi.tune_export()
         ctx.correlate()
         out = ctx.output()
