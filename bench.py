@@ -536,9 +536,12 @@ class MixedBatch:
                     b.add(c)
             return b
 
-        # warm-up round (captures, allocations), then the timed pass
-        prepare(0).launch()
-        batches[0].synchronize()
+        # a warm-up round on each context set (device buffers allocated at
+        # their sizes, graphs captured once), then the timed pass
+        for r in range(min(2, self.rounds)):
+            prepare(r).launch()
+        for b in batches:
+            b.synchronize()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for r in range(self.rounds):

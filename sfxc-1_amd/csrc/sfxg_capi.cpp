@@ -174,6 +174,7 @@ struct sfxg_ctx {
   std::vector<int32_t> h_bl, h_fold_start;
   int tables_nc = -1;
   int tables_window = -1;
+  double window_sumsq = 0.0;  // sum of the uploaded window's squares
   bool enqueued = false;
   bool have_results = false;  // the current slice ran (correlate or a batch launch)
   LaunchPlan lp;
@@ -990,6 +991,8 @@ static int prepare_slice(sfxg_ctx *ctx) {
     HIPCHK(ctx, hipMemcpy(ctx->window.p, win.data(), win.size() * sizeof(float), hipMemcpyHostToDevice));
     ctx->tables_nc = nc;
     ctx->tables_window = ctx->p.window;
+    ctx->window_sumsq = 0.0;
+    for (float w : win) ctx->window_sumsq += (double)w * w;
   }
 
   // lag re-channelisation tables (create_mask / create_window,
@@ -1329,13 +1332,7 @@ static int prepare_slice(sfxg_ctx *ctx) {
     // power-of-two scale that brings the spectra's rms near 16: the delay
     // correction's unnormalised FFT pair gains N_d / 2, the channeliser
     // sqrt(sum w^2); the decode table bounds the sample rms
-    double wsum = 0.0;
-    {
-      std::vector<float2> tnd, tn, t2n, tsp;
-      std::vector<float> win;
-      make_tables(ctx, tnd, tn, t2n, tsp, win);
-      for (float w : win) wsum += (double)w * w;
-    }
+    const double wsum = ctx->window_sumsq;
     double lmax = 0.0;
     for (int i = 0; i < M; i++) {
       const int lut = ctx->opt.lut;
