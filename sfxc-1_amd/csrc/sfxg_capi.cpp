@@ -1732,6 +1732,10 @@ int sfxg_fetch_stage(sfxg_ctx *ctx, int32_t stage, int32_t stream_idx, void *dst
     case SFXG_STAGE_TIME_SERIES:
       if (ctx->fused)
         return ctx->fail(SFXG_ERR_STATE, "the real series stayed in LDS: set keep_time_series in sfxg_options");
+      if (ctx->lp.job.ts_half)
+        return ctx->fail(SFXG_ERR_STATE,
+                         "the real series was held as FP16 for the MFMA channeliser: set keep_time_series in "
+                         "sfxg_options");
       if (ctx->lp.chunks.size() != 1)
         return ctx->fail(SFXG_ERR_STATE, "the slice ran in %d chunks: stages hold the last one only",
                          (int)ctx->lp.chunks.size());

@@ -666,6 +666,8 @@ def test_mfma_channeliser_fp32_series(ctx, nchan):
     try:
         out_h = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=1)
         sp_h = ctx.stage(capi.STAGE_SPECTRA, 1)
+        with pytest.raises(capi.SfxgError):
+            ctx.stage(capi.STAGE_TIME_SERIES, 1)  # the series was held as FP16: refused on the host
         out_f = capi.correlate_slice(ctx, job, hdrs, lut=LUT_CONFIG, precision=1, keep_time_series=True)
         sp_f = ctx.stage(capi.STAGE_SPECTRA, 1)
         ts = ctx.stage(capi.STAGE_TIME_SERIES, 1)
