@@ -64,7 +64,9 @@ extern "C" {
 
 /* Stages readable with sfxg_fetch_stage (verification of each stage). */
 #define SFXG_STAGE_SAMPLES 0      /* Bit2float_worker output, float[n_samples] */
-#define SFXG_STAGE_TIME_SERIES 1  /* fringe-stopped real series, float[n_samples] */
+#define SFXG_STAGE_TIME_SERIES 1  /* fringe-stopped real series, float[n_samples]; SFXG_ERR_STATE
+                                    unless keep_time_series held it as FP32 (or the two-kernel
+                                    FP32 route ran) */
 #define SFXG_STAGE_SPECTRA 2      /* float2[n_segments][N_c + 1]; with SFXG_PREC_FP16 / BF16 the
                                      spectra are kept in that type (scaled by a power of two) and
                                      are returned converted back to float2 */
